@@ -1,0 +1,106 @@
+/* CPU oracle for the AMIE FE-solve hot path: a plain C++ (FP64, OpenMP)
+ * restatement of the reference algorithm, used ONLY as the checker by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  Nothing on the
+ * product path links or calls it.
+ *
+ * Layout conventions (shared with the product C ABI, include/amie_gpu.h):
+ *  - element types: 1 = P1 triangle (3 nodes), 2 = P2 triangle (6 nodes,
+ *    [v0, m01, v1, m12, v2, m20]), 3 = linear tetrahedron (4 nodes);
+ *  - a node-block CSR in the reference layout: row_size[nrows],
+ *    column_index[nnzb] (block column = node id), array[nnzb*b*colLen] with
+ *    each b x b block column-major and columns padded to colLen = b + b%2
+ *    (sparse/sparse_matrix.cpp:59-66, sparse/sparse_vector.cpp:27-58);
+ *  - DOF = node*b + axis.
+ */
+#ifndef AMIE_ORACLE_H
+#define AMIE_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* utilities/tensor.cpp:1845-1931 (YOUNG_POISSON); dim 2 -> 3x3, 3 -> 6x6, row-major. */
+void or_cauchy_green(double E, double nu, int dim, int plane_strain, double *C);
+
+/* geometry/geometry_2D.cpp:476-497 and geometry/geometry_3D.cpp:254-268: local
+ * node order after the reference's orientation rule.  perm[k] = input slot. */
+void or_orient(int etype, const double *coords, int *perm);
+
+/* mesher/delaunay.cpp:2408-2535, mesher/delaunay_3d.cpp:2452-2530.
+ * coords in local order (nper x dim), C row-major (3x3 or 6x6).  Ke is dense
+ * (nper*b) x (nper*b), dof = local_node*b + axis. */
+void or_element_matrix(int etype, const double *coords, const double *C, double *Ke);
+
+/* solvers/assembly.cpp:445-537 (2D) / 829-922 (3D): returns nnzb.  With
+ * row_size/column_index NULL only counts.  nrows = nnodes. */
+int64_t or_pattern(int etype, int64_t nelem, const int32_t *conn, int64_t nnodes,
+                   uint32_t *row_size, uint32_t *column_index);
+
+/* solvers/assembly.cpp:657-735 / 1060-1206: serial Kahan scatter of every
+ * element matrix (scale 1) into array (zeroed first).  cmat: nphase C
+ * matrices; damage (nullable): C_e = C*(1-d_e). */
+void or_assemble(int etype, int64_t nelem, const int32_t *conn, const double *coords,
+                 const double *cmat, const int32_t *phase, const double *damage,
+                 int64_t nrows, const uint32_t *row_size, const uint32_t *column_index,
+                 double *array);
+
+/* solvers/assembly.cpp:125-322: symmetric elimination, unit diagonal,
+ * RHS lift, b[id] = g.  dofs need not be sorted. */
+void or_apply_dirichlet(int stride, int64_t nrows, const uint32_t *row_size,
+                        const uint32_t *column_index, double *array, double *rhs,
+                        int64_t nbc, const int64_t *dofs, const double *vals);
+
+/* sparse/sparse_matrix.h:203-333 + sparse/sparse_matrix.cpp:462-547:
+ * y = A x - b (b nullable -> y = A x), Kahan per row, OpenMP over rows. */
+void or_spmv(int stride, int64_t nrows, const uint32_t *row_size, const uint32_t *column_index,
+             const double *array, const double *x, const double *b, double *y);
+
+/* utilities/matrixops.h:246-262: per-thread Kahan + reduction. */
+double or_dot(const double *a, const double *b, int64_t n);
+
+/* sparse/sparse_matrix.cpp:216-231 */
+void or_inverse_diagonal(int stride, int64_t nrows, const uint32_t *row_size,
+                         const uint32_t *column_index, const double *array, double *dinv);
+/* solvers/inversediagonal.cpp:84-133; out: (n/2) row-major 2x2 inverses. */
+int or_inverse_2x2_diagonal(int stride, int64_t nrows, const uint32_t *row_size,
+                            const uint32_t *column_index, const double *array, double *blocks);
+
+/* solvers/conjugategradient.cpp:69-318 with the InverseDiagonal
+ * preconditioner.  Returns 1 converged, 0 not.  nit_out = inner iterations. */
+int or_cg_reference(int stride, int64_t nrows, const uint32_t *row_size,
+                    const uint32_t *column_index, const double *array, const double *b,
+                    const double *x0, double eps, int64_t maxit, int nssor,
+                    double *x, int64_t *nit_out);
+
+/* Textbook preconditioned CG stopped on ||b - A x|| / ||b|| <= rtol (true
+ * residual checked at exit).  precond 0 = point Jacobi, 1 = b x b block
+ * Jacobi.  Returns 1 converged.  Used as the FP64 parity solution and as the
+ * CPU baseline. */
+int or_pcg(int stride, int64_t nrows, const uint32_t *row_size, const uint32_t *column_index,
+           const double *array, const double *b, const double *x0, double rtol, int64_t maxit,
+           int precond, double *x, int64_t *nit_out, double *relres_out);
+
+/* elements/integrable_entity.cpp:977-1097: element strain (Voigt, engineering
+ * shear) from the nodal displacement vector u (dof = node*b + axis), at the
+ * element centroid with the reference's centroid Jinv.  out: nelem x 3|6. */
+void or_element_strain(int etype, int64_t nelem, const int32_t *conn, const double *coords,
+                       const double *u, double *strain);
+
+/* Damage law of the 3D incremental-loading case: Mazars equivalent strain
+ * (physics/fracturecriteria/mazars.cpp:124-139 tension branch) of the
+ * principal strains, kappa = max(kappa, eps_eq), d = max(d0,
+ * 1 - (k0/kappa) exp(-(kappa-k0)/kf)) for kappa > k0, capped at dmax.
+ * Updates kappa and d in place.  Returns number of elements whose d grew. */
+int64_t or_damage_update(int64_t nelem, int dim, const double *strain, const double *d0,
+                         double kappa0, double kappaf, double dmax, double *kappa, double *d);
+
+/* Volume average of C_e (strain) over elements (stress average), with
+ * element measure |det J|/2 (tri) or /6 (tet).  out: 3|6 components. */
+void or_average_stress(int etype, int64_t nelem, const int32_t *conn, const double *coords,
+                       const double *cmat, const int32_t *phase, const double *strain,
+                       double *out, double *measure_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

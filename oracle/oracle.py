@@ -1,0 +1,232 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+The product (xfem-amie_amd/) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+P1, P2, TET4 = 1, 2, 3
+NPER = {P1: 3, P2: 6, TET4: 4}
+DIM = {P1: 2, P2: 2, TET4: 3}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        i64, i32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        vp = ctypes.c_void_p
+        L.or_cauchy_green.argtypes = [dbl, dbl, i32, i32, vp]
+        L.or_orient.argtypes = [i32, vp, vp]
+        L.or_element_matrix.argtypes = [i32, vp, vp, vp]
+        L.or_pattern.argtypes = [i32, i64, vp, i64, vp, vp]
+        L.or_pattern.restype = i64
+        L.or_assemble.argtypes = [i32, i64, vp, vp, vp, vp, vp, i64, vp, vp, vp]
+        L.or_apply_dirichlet.argtypes = [i32, i64, vp, vp, vp, vp, i64, vp, vp]
+        L.or_spmv.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp]
+        L.or_dot.argtypes = [vp, vp, i64]
+        L.or_dot.restype = dbl
+        L.or_inverse_diagonal.argtypes = [i32, i64, vp, vp, vp, vp]
+        L.or_inverse_2x2_diagonal.argtypes = [i32, i64, vp, vp, vp, vp]
+        L.or_inverse_2x2_diagonal.restype = i32
+        L.or_cg_reference.argtypes = [i32, i64, vp, vp, vp, vp, vp, dbl, i64, i32, vp, vp]
+        L.or_cg_reference.restype = i32
+        L.or_pcg.argtypes = [i32, i64, vp, vp, vp, vp, vp, dbl, i64, i32, vp, vp, vp]
+        L.or_pcg.restype = i32
+        L.or_element_strain.argtypes = [i32, i64, vp, vp, vp, vp]
+        L.or_damage_update.argtypes = [i64, i32, vp, vp, dbl, dbl, dbl, vp, vp]
+        L.or_damage_update.restype = i64
+        L.or_average_stress.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+def cauchy_green(E, nu, dim=2, plane_strain=False):
+    s = 3 if dim == 2 else 6
+    C = np.zeros((s, s))
+    lib().or_cauchy_green(E, nu, dim, int(plane_strain), _p(C))
+    return C
+
+
+def orient(etype, coords):
+    coords = _c(coords, np.float64)
+    perm = np.zeros(NPER[etype], dtype=np.int32)
+    lib().or_orient(etype, _p(coords), _p(perm))
+    return perm
+
+
+def element_matrix(etype, coords, C):
+    coords = _c(coords, np.float64)
+    C = _c(C, np.float64)
+    b = DIM[etype]
+    n = NPER[etype] * b
+    K = np.zeros((n, n))
+    lib().or_element_matrix(etype, _p(coords), _p(C), _p(K))
+    return K
+
+
+class BlockCSR:
+    """Node-block CSR in the reference layout (padded column-major blocks)."""
+
+    def __init__(self, stride, row_size, column_index, array=None):
+        self.stride = stride
+        self.row_size = np.ascontiguousarray(row_size, dtype=np.uint32)
+        self.column_index = np.ascontiguousarray(column_index, dtype=np.uint32)
+        self.nrows = len(self.row_size)
+        self.col_len = stride + stride % 2
+        nnzb = len(self.column_index)
+        self.array = (np.zeros(nnzb * stride * self.col_len) if array is None
+                      else np.ascontiguousarray(array, dtype=np.float64))
+
+    @property
+    def n(self):
+        return self.nrows * self.stride
+
+    def args(self):
+        return (self.stride, self.nrows, _p(self.row_size), _p(self.column_index), _p(self.array))
+
+    def to_dense(self):
+        b, cl = self.stride, self.col_len
+        A = np.zeros((self.n, self.n))
+        start = np.concatenate([[0], np.cumsum(self.row_size, dtype=np.int64)])
+        blocks = self.array.reshape(-1, b, cl)  # [blk][col][row]
+        for r in range(self.nrows):
+            for q in range(start[r], start[r + 1]):
+                c = int(self.column_index[q])
+                A[r * b:(r + 1) * b, c * b:(c + 1) * b] = blocks[q, :, :b].T
+        return A
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        b, cl = self.stride, self.col_len
+        start = np.concatenate([[0], np.cumsum(self.row_size, dtype=np.int64)])
+        blocks = self.array.reshape(-1, b, cl)[:, :, :b].transpose(0, 2, 1)
+        return sp.bsr_matrix((np.ascontiguousarray(blocks), self.column_index.astype(np.int64), start),
+                             shape=(self.n, self.n)).tocsr()
+
+
+def pattern(etype, conn, nnodes):
+    conn = _c(conn, np.int32)
+    nelem = conn.shape[0]
+    nnzb = lib().or_pattern(etype, nelem, _p(conn), nnodes, None, None)
+    rs = np.zeros(nnodes, dtype=np.uint32)
+    ci = np.zeros(nnzb, dtype=np.uint32)
+    lib().or_pattern(etype, nelem, _p(conn), nnodes, _p(rs), _p(ci))
+    return rs, ci
+
+
+def assemble(etype, conn, coords, cmats, phase=None, damage=None):
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    cmats = _c(cmats, np.float64)
+    phase = _c(phase, np.int32)
+    damage = _c(damage, np.float64)
+    nnodes = coords.shape[0]
+    rs, ci = pattern(etype, conn, nnodes)
+    A = BlockCSR(DIM[etype], rs, ci)
+    lib().or_assemble(etype, conn.shape[0], _p(conn), _p(coords), _p(cmats), _p(phase), _p(damage),
+                      A.nrows, _p(A.row_size), _p(A.column_index), _p(A.array))
+    return A
+
+
+def apply_dirichlet(A, rhs, dofs, vals):
+    dofs = _c(dofs, np.int64)
+    vals = _c(vals, np.float64)
+    lib().or_apply_dirichlet(*A.args(), _p(rhs), len(dofs), _p(dofs), _p(vals))
+
+
+def spmv(A, x, b=None):
+    x = _c(x, np.float64)
+    b = _c(b, np.float64)
+    y = np.zeros(A.n)
+    lib().or_spmv(*A.args(), _p(x), _p(b), _p(y))
+    return y
+
+
+def dot(a, b):
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    return lib().or_dot(_p(a), _p(b), len(a))
+
+
+def inverse_diagonal(A):
+    d = np.zeros(A.n)
+    lib().or_inverse_diagonal(*A.args(), _p(d))
+    return d
+
+
+def inverse_2x2_diagonal(A):
+    blk = np.zeros((A.n // 2, 4))
+    if lib().or_inverse_2x2_diagonal(*A.args(), _p(blk)) != 0:
+        raise ValueError("odd number of dofs")
+    return blk
+
+
+def cg_reference(A, b, x0=None, eps=1e-10, maxit=-1, nssor=32):
+    b = _c(b, np.float64)
+    x0 = _c(x0, np.float64)
+    x = np.zeros(A.n)
+    nit = np.zeros(1, dtype=np.int64)
+    ok = lib().or_cg_reference(*A.args(), _p(b), _p(x0), eps, maxit, nssor, _p(x), _p(nit))
+    return x, int(nit[0]), ok
+
+
+def pcg(A, b, x0=None, rtol=1e-10, maxit=100000, precond=0):
+    b = _c(b, np.float64)
+    x0 = _c(x0, np.float64)
+    x = np.zeros(A.n)
+    nit = np.zeros(1, dtype=np.int64)
+    rr = np.zeros(1)
+    ok = lib().or_pcg(*A.args(), _p(b), _p(x0), rtol, maxit, precond, _p(x), _p(nit), _p(rr))
+    return x, int(nit[0]), float(rr[0]), ok
+
+
+def element_strain(etype, conn, coords, u):
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    u = _c(u, np.float64)
+    s = 3 if DIM[etype] == 2 else 6
+    out = np.zeros((conn.shape[0], s))
+    lib().or_element_strain(etype, conn.shape[0], _p(conn), _p(coords), _p(u), _p(out))
+    return out
+
+
+def damage_update(strain, kappa, d, d0=None, kappa0=1e-4, kappaf=1e-3, dmax=1 - 1e-6):
+    strain = _c(strain, np.float64)
+    d0 = _c(d0, np.float64)
+    dim = 2 if strain.shape[1] == 3 else 3
+    grown = lib().or_damage_update(strain.shape[0], dim, _p(strain), _p(d0), kappa0, kappaf, dmax,
+                                   _p(kappa), _p(d))
+    return int(grown)
+
+
+def average_stress(etype, conn, coords, cmats, phase, strain):
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    cmats = _c(cmats, np.float64)
+    phase = _c(phase, np.int32)
+    strain = _c(strain, np.float64)
+    s = strain.shape[1]
+    out = np.zeros(s)
+    vol = np.zeros(1)
+    lib().or_average_stress(etype, conn.shape[0], _p(conn), _p(coords), _p(cmats), _p(phase), _p(strain),
+                            _p(out), _p(vol))
+    return out, float(vol[0])

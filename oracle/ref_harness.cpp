@@ -1,0 +1,387 @@
+// Golden-vector harness for the compiled reference AMIE library (oracle/_ref).
+//
+// Test infrastructure only.  Built by oracle/Makefile.ref against the
+// reference sources where they lie and run in the development container; the
+// small .npy fixtures it writes under tests/golden/ref/ are committed and pin
+// the CPU restatement in oracle/amie_oracle.cpp.  Everything below drives the
+// reference's own public API:
+//   Tensor::cauchyGreen                    utilities/tensor.cpp:1845-1931
+//   DelaunayTriangle::getElementaryMatrix  mesher/delaunay.cpp:2408-2535
+//   DelaunayTetrahedron::getElementaryMatrix mesher/delaunay_3d.cpp:2452
+//   Assembly::add / setPointAlong / make_final / setBoundaryConditions
+//                                          solvers/assembly.cpp:106-123,425-1223,1590
+//   assign(r, A*x-b), assign(q, A*x)       sparse/sparse_matrix.cpp:462-547
+//   InverseDiagonal, Inverse2x2Diagonal    solvers/inversediagonal.cpp:48-133
+//   parallel_inner_product                 utilities/matrixops.h:246-262
+//   ConjugateGradient::solve               solvers/conjugategradient.cpp:69-318
+//
+// Usage: ref_harness <outdir>
+
+#include "elements/elements.h"
+#include "mesher/delaunay.h"
+#include "mesher/delaunay_3d.h"
+#include "physics/stiffness.h"
+#include "solvers/assembly.h"
+#include "solvers/conjugategradient.h"
+#include "solvers/inversediagonal.h"
+#include "sparse/sparse_matrix.h"
+#include "utilities/matrixops.h"
+#include "utilities/tensor.h"
+
+#include <omp.h>
+#include <cstdio>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+using namespace Amie;
+
+static std::string g_out;
+
+// ---- minimal .npy writer (format version 1.0, little endian) -------------
+static void write_npy(const std::string &name, const char *descr, size_t itemsize,
+                      const std::vector<size_t> &shape, const void *data)
+{
+    std::string shp = "(";
+    size_t count = 1;
+    for (size_t i = 0; i < shape.size(); i++) {
+        shp += std::to_string(shape[i]);
+        shp += (shape.size() == 1) ? "," : (i + 1 < shape.size() ? ", " : "");
+        count *= shape[i];
+    }
+    shp += ")";
+    std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + shp + ", }";
+    size_t total = 10 + hdr.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    hdr += std::string(pad, ' ');
+    hdr += "\n";
+    std::string path = g_out + "/" + name + ".npy";
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) { perror(path.c_str()); exit(1); }
+    const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    fwrite(magic, 1, 8, f);
+    uint16_t hl = (uint16_t)hdr.size();
+    fwrite(&hl, 2, 1, f);
+    fwrite(hdr.data(), 1, hdr.size(), f);
+    fwrite(data, itemsize, count, f);
+    fclose(f);
+}
+static void npy_f64(const std::string &n, const std::vector<double> &v, std::vector<size_t> shape = {})
+{
+    if (shape.empty()) shape = {v.size()};
+    write_npy(n, "<f8", 8, shape, v.data());
+}
+static void npy_u32(const std::string &n, const std::vector<uint32_t> &v)
+{
+    write_npy(n, "<u4", 4, {v.size()}, v.data());
+}
+static void npy_i64(const std::string &n, const std::vector<int64_t> &v, std::vector<size_t> shape = {})
+{
+    if (shape.empty()) shape = {v.size()};
+    write_npy(n, "<i8", 8, shape, v.data());
+}
+static std::vector<double> vec(const Vector &v) { return std::vector<double>(&v[0], &v[0] + v.size()); }
+template <class T> static std::vector<uint32_t> uvec(const std::valarray<T> &v)
+{
+    std::vector<uint32_t> r(v.size());
+    for (size_t i = 0; i < v.size(); i++) r[i] = (uint32_t)v[i];
+    return r;
+}
+static std::vector<double> mat(const Matrix &m)
+{
+    std::vector<double> r(m.numRows() * m.numCols());
+    for (size_t i = 0; i < m.numRows(); i++)
+        for (size_t j = 0; j < m.numCols(); j++) r[i * m.numCols() + j] = m[i][j];
+    return r;
+}
+
+// Dense element matrix from the reference's block cache: dof = local_node*ndof + axis.
+static std::vector<double> dense_elem(std::valarray<std::valarray<Matrix>> &K, size_t nn, size_t nd)
+{
+    std::vector<double> d(nn * nd * nn * nd);
+    for (size_t i = 0; i < nn; i++)
+        for (size_t j = 0; j < nn; j++)
+            for (size_t a = 0; a < nd; a++)
+                for (size_t b = 0; b < nd; b++) d[(i * nd + a) * (nn * nd) + j * nd + b] = K[i][j][a][b];
+    return d;
+}
+
+// ---------------------------------------------------------------- C matrices
+static void golden_cauchy_green()
+{
+    npy_f64("C_ps_E1e10_nu0.2", mat(Tensor::cauchyGreen(1e10, 0.2, SPACE_TWO_DIMENSIONAL, PLANE_STRESS, YOUNG_POISSON)), {3, 3});
+    npy_f64("C_ps_E12e9_nu0.3", mat(Tensor::cauchyGreen(12e9, 0.3, SPACE_TWO_DIMENSIONAL, PLANE_STRESS, YOUNG_POISSON)), {3, 3});
+    npy_f64("C_ps_E60e9_nu0.2", mat(Tensor::cauchyGreen(60e9, 0.2, SPACE_TWO_DIMENSIONAL, PLANE_STRESS, YOUNG_POISSON)), {3, 3});
+    npy_f64("C_3d_E30e9_nu0.2", mat(Tensor::cauchyGreen(30e9, 0.2, SPACE_THREE_DIMENSIONAL, PLANE_STRESS, YOUNG_POISSON)), {6, 6});
+}
+
+// ----------------------------------------------------------- element matrices
+static void golden_p1(const std::string &tag, double x0, double y0, double x1, double y1, double x2, double y2, double E, double nu)
+{
+    TriElement father(LINEAR);
+    Point *p[3] = {new Point(x0, y0), new Point(x1, y1), new Point(x2, y2)};
+    for (int i = 0; i < 3; i++) p[i]->getId() = i;
+    DelaunayTriangle *t = new DelaunayTriangle(nullptr, nullptr, p[0], p[1], p[2], nullptr);
+    t->refresh(&father);
+    t->setBehaviour(nullptr, new Stiffness(E, nu));
+    VirtualMachine vm;
+    auto &K = t->getElementaryMatrix(&vm);
+    std::vector<int64_t> order(3);
+    for (int i = 0; i < 3; i++) order[i] = t->getBoundingPoint(i).getId();
+    npy_f64("Ke_" + tag, dense_elem(K, 3, 2), {6, 6});
+    npy_i64("Ke_" + tag + "_order", order);
+    npy_f64("Ke_" + tag + "_xy", {x0, y0, x1, y1, x2, y2}, {3, 2});
+}
+
+static void golden_p2(const std::string &tag, double x0, double y0, double x1, double y1, double x2, double y2, double E, double nu)
+{
+    // P2 bounding points follow the shape-function layout of
+    // TriElement(QUADRATIC) (elements/elements.cpp:1002-1061):
+    // [v0, m(v0,v1), v1, m(v1,v2), v2, m(v2,v0)], with v0..v2 already CCW.
+    TriElement father(QUADRATIC);
+    Point *v[3] = {new Point(x0, y0), new Point(x1, y1), new Point(x2, y2)};
+    for (int i = 0; i < 3; i++) v[i]->getId() = 2 * i;
+    DelaunayTriangle *t = new DelaunayTriangle(nullptr, nullptr, v[0], v[1], v[2], nullptr);
+    Point *a = &t->getBoundingPoint(0), *b = &t->getBoundingPoint(1), *c = &t->getBoundingPoint(2);
+    Point *m01 = new Point(0.5 * (a->getX() + b->getX()), 0.5 * (a->getY() + b->getY()));
+    Point *m12 = new Point(0.5 * (b->getX() + c->getX()), 0.5 * (b->getY() + c->getY()));
+    Point *m20 = new Point(0.5 * (c->getX() + a->getX()), 0.5 * (c->getY() + a->getY()));
+    m01->getId() = 1; m12->getId() = 3; m20->getId() = 5;
+    std::valarray<Point *> pts(6);
+    pts[0] = a; pts[1] = m01; pts[2] = b; pts[3] = m12; pts[4] = c; pts[5] = m20;
+    t->setBoundingPoints(pts);
+    t->refresh(&father);
+    t->setBehaviour(nullptr, new Stiffness(E, nu));
+    VirtualMachine vm;
+    auto &K = t->getElementaryMatrix(&vm);
+    std::vector<double> xy;
+    for (int i = 0; i < 6; i++) { xy.push_back(t->getBoundingPoint(i).getX()); xy.push_back(t->getBoundingPoint(i).getY()); }
+    npy_f64("Ke_" + tag, dense_elem(K, 6, 2), {12, 12});
+    npy_f64("Ke_" + tag + "_xy", xy, {6, 2});
+}
+
+static void golden_tet(const std::string &tag, const double *c, double E, double nu)
+{
+    TetrahedralElement father(LINEAR);
+    Point *p[4];
+    for (int i = 0; i < 4; i++) { p[i] = new Point(c[3 * i], c[3 * i + 1], c[3 * i + 2]); p[i]->getId() = i; }
+    DelaunayTetrahedron *t = new DelaunayTetrahedron(nullptr, nullptr, p[0], p[1], p[2], p[3], nullptr);
+    t->refresh(&father);
+    t->setBehaviour(nullptr, new Stiffness(E, nu, SPACE_THREE_DIMENSIONAL));
+    VirtualMachine vm;
+    auto &K = t->getElementaryMatrix(&vm);
+    std::vector<int64_t> order(4);
+    for (int i = 0; i < 4; i++) order[i] = t->getBoundingPoint(i).getId();
+    npy_f64("Ke_" + tag, dense_elem(K, 4, 3), {12, 12});
+    npy_i64("Ke_" + tag + "_order", order);
+    npy_f64("Ke_" + tag + "_xyz", std::vector<double>(c, c + 12), {4, 3});
+}
+
+// ------------------------------------------------- structured meshes + systems
+// P1: node (i,j) -> id i*(ny+1)+j, x = i*lx/nx, y = j*ly/ny; two triangles per
+// cell as mesher/structuredmesh.cpp:26-31 (no jitter).  Phase of a triangle is
+// given by `phase_of` (0 = E0/nu0, 1 = E1/nu1).
+struct P1Case {
+    int nx, ny;
+    double lx, ly;
+    double E0, nu0, E1, nu1;
+};
+
+static void golden_system_p1(const std::string &tag, const P1Case &pc, int nssor, bool solve)
+{
+    TriElement father(LINEAR);
+    std::vector<Point *> pts;
+    for (int i = 0; i <= pc.nx; i++)
+        for (int j = 0; j <= pc.ny; j++) {
+            pts.push_back(new Point(pc.lx * i / pc.nx, pc.ly * j / pc.ny));
+            pts.back()->getId() = (int)pts.size() - 1;
+        }
+    auto P = [&](int i, int j) { return pts[i * (pc.ny + 1) + j]; };
+    Assembly K;
+    std::vector<DelaunayTriangle *> tris;
+    std::vector<int64_t> conn, phase;
+    for (int i = 0; i < pc.nx; i++)
+        for (int j = 0; j < pc.ny; j++) {
+            Point *c0[3] = {P(i, j), P(i, j + 1), P(i + 1, j + 1)};
+            Point *c1[3] = {P(i, j), P(i + 1, j), P(i + 1, j + 1)};
+            for (int h = 0; h < 2; h++) {
+                Point **c = h ? c1 : c0;
+                DelaunayTriangle *t = new DelaunayTriangle(nullptr, nullptr, c[0], c[1], c[2], nullptr);
+                t->refresh(&father);
+                int ph = ((i * 7 + j * 3 + h) % 5 == 0) ? 1 : 0;
+                if (pc.E1 == pc.E0) ph = 0;
+                t->setBehaviour(nullptr, ph ? new Stiffness(pc.E1, pc.nu1) : new Stiffness(pc.E0, pc.nu0));
+                tris.push_back(t);
+                for (int k = 0; k < 3; k++) conn.push_back(t->getBoundingPoint(k).getId());
+                phase.push_back(ph);
+                K.add(t, 1.);
+            }
+        }
+    K.setMaxDof(pts.size());
+    // BCs of the uniaxial configuration: bottom edge u_y = 0, corner (0,0)
+    // u_x = 0, top edge u_y = 1e-3.
+    std::vector<int64_t> bc_dof;
+    std::vector<double> bc_val;
+    for (int i = 0; i <= pc.nx; i++) {
+        K.setPointAlong(ETA, 0., P(i, 0)->getId());
+        bc_dof.push_back(2 * P(i, 0)->getId() + 1); bc_val.push_back(0.);
+        K.setPointAlong(ETA, 1e-3, P(i, pc.ny)->getId());
+        bc_dof.push_back(2 * P(i, pc.ny)->getId() + 1); bc_val.push_back(1e-3);
+    }
+    K.setPointAlong(XI, 0., P(0, 0)->getId());
+    bc_dof.push_back(0); bc_val.push_back(0.);
+
+    // Raw (pre-BC) matrix: a second assembly without multipliers.
+    Assembly Kraw;
+    for (auto t : tris) Kraw.add(t, 1.);
+    Kraw.setMaxDof(pts.size());
+    Kraw.make_final();
+    for (auto t : tris) t->behaviourUpdated = true;
+
+    K.make_final();
+    CoordinateIndexedSparseMatrix &A = K.getMatrix();
+    std::vector<double> xy;
+    for (auto p : pts) { xy.push_back(p->getX()); xy.push_back(p->getY()); }
+    npy_f64(tag + "_xy", xy, {pts.size(), 2});
+    npy_i64(tag + "_conn", conn, {tris.size(), 3});
+    npy_i64(tag + "_phase", phase);
+    npy_f64(tag + "_mat", {pc.E0, pc.nu0, pc.E1, pc.nu1});
+    npy_i64(tag + "_bc_dof", bc_dof);
+    npy_f64(tag + "_bc_val", bc_val);
+    npy_u32(tag + "_row_size", uvec(A.row_size));
+    npy_u32(tag + "_column_index", uvec(A.column_index));
+    npy_f64(tag + "_array", vec(A.array));
+    npy_f64(tag + "_raw_array", vec(Kraw.getMatrix().array));
+    npy_f64(tag + "_forces", vec(K.getForces()));
+
+    // SpMV A*x - b and A*x with a seeded x.
+    size_t n = K.getForces().size();
+    std::mt19937_64 rng(12345);
+    std::uniform_real_distribution<double> U(-1., 1.);
+    Vector x(n), r(n), q(n);
+    for (size_t i = 0; i < n; i++) x[i] = U(rng);
+    assign(r, A * x - K.getForces());
+    assign(q, A * x);
+    npy_f64(tag + "_spmv_x", vec(x));
+    npy_f64(tag + "_spmv_Axmb", vec(r));
+    npy_f64(tag + "_spmv_Ax", vec(q));
+    Vector t(n);
+    InverseDiagonal D(A);
+    D.precondition(x, t);
+    npy_f64(tag + "_invdiag_x", vec(t));
+    Inverse2x2Diagonal D2(A);
+    D2.precondition(x, t);
+    npy_f64(tag + "_inv2x2_x", vec(t));
+    npy_f64(tag + "_dot_x_Ax", {parallel_inner_product(&x[0], &q[0], (int)n)});
+
+    if (solve) {
+        ConjugateGradient cg(&K);
+        cg.nssor = nssor;
+        Vector x0(0., n);
+        bool ok = cg.solve(x0, nullptr, 1e-10, -1, false);
+        npy_f64(tag + "_cg_x", vec(cg.x));
+        npy_i64(tag + "_cg_info", {(int64_t)ok, (int64_t)cg.nit, (int64_t)nssor});
+    }
+}
+
+// Tet4 structured cube mesh, 6-tet split of filters/voxelfilter.cpp:352-405,
+// node id i*(n+1)^2 + j*(n+1) + k.
+static void golden_system_tet(const std::string &tag, int n, double E, double nu)
+{
+    TetrahedralElement father(LINEAR);
+    std::vector<Point *> pts;
+    int m = n + 1;
+    for (int i = 0; i <= n; i++)
+        for (int j = 0; j <= n; j++)
+            for (int k = 0; k <= n; k++) {
+                pts.push_back(new Point((double)i / n, (double)j / n, (double)k / n));
+                pts.back()->getId() = (int)pts.size() - 1;
+            }
+    auto P = [&](int i, int j, int k) { return pts[(size_t)i * m * m + j * m + k]; };
+    Assembly K;
+    std::vector<DelaunayTetrahedron *> tets;
+    std::vector<int64_t> conn;
+    static const int split[6][4] = {{0, 1, 2, 4}, {3, 5, 7, 6}, {2, 3, 4, 6}, {1, 2, 3, 4}, {1, 3, 4, 5}, {3, 4, 6, 5}};
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++) {
+                // corners c0..c7 of the voxel (filters/voxelfilter.cpp:352-359)
+                Point *c[8] = {P(i, j + 1, k), P(i + 1, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1),
+                               P(i, j, k), P(i + 1, j, k), P(i, j, k + 1), P(i + 1, j, k + 1)};
+                for (int s = 0; s < 6; s++) {
+                    DelaunayTetrahedron *t = new DelaunayTetrahedron(nullptr, nullptr, c[split[s][0]], c[split[s][1]], c[split[s][2]], c[split[s][3]], nullptr);
+                    t->refresh(&father);
+                    t->setBehaviour(nullptr, new Stiffness(E, nu, SPACE_THREE_DIMENSIONAL));
+                    tets.push_back(t);
+                    for (int q = 0; q < 4; q++) conn.push_back(t->getBoundingPoint(q).getId());
+                    K.add(t, 1.);
+                }
+            }
+    K.setMaxDof(pts.size());
+    std::vector<int64_t> bc_dof;
+    std::vector<double> bc_val;
+    for (int i = 0; i <= n; i++)
+        for (int j = 0; j <= n; j++) {
+            for (int a = 0; a < 3; a++) {
+                Variable v = a == 0 ? XI : (a == 1 ? ETA : ZETA);
+                K.setPointAlong(v, 0., P(i, j, 0)->getId());
+                bc_dof.push_back(3 * P(i, j, 0)->getId() + a); bc_val.push_back(0.);
+            }
+            K.setPointAlong(ZETA, 2e-5, P(i, j, n)->getId());
+            bc_dof.push_back(3 * P(i, j, n)->getId() + 2); bc_val.push_back(2e-5);
+        }
+    K.make_final();
+    CoordinateIndexedSparseMatrix &A = K.getMatrix();
+    std::vector<double> xyz;
+    for (auto p : pts) { xyz.push_back(p->getX()); xyz.push_back(p->getY()); xyz.push_back(p->getZ()); }
+    npy_f64(tag + "_xyz", xyz, {pts.size(), 3});
+    npy_i64(tag + "_conn", conn, {tets.size(), 4});
+    npy_f64(tag + "_mat", {E, nu});
+    npy_i64(tag + "_bc_dof", bc_dof);
+    npy_f64(tag + "_bc_val", bc_val);
+    npy_u32(tag + "_row_size", uvec(A.row_size));
+    npy_u32(tag + "_column_index", uvec(A.column_index));
+    npy_f64(tag + "_array", vec(A.array));
+    npy_f64(tag + "_forces", vec(K.getForces()));
+    size_t nd = K.getForces().size();
+    std::mt19937_64 rng(777);
+    std::uniform_real_distribution<double> U(-1., 1.);
+    Vector x(nd), r(nd);
+    for (size_t i = 0; i < nd; i++) x[i] = U(rng);
+    assign(r, A * x - K.getForces());
+    npy_f64(tag + "_spmv_x", vec(x));
+    npy_f64(tag + "_spmv_Axmb", vec(r));
+    ConjugateGradient cg(&K);
+    cg.nssor = 32;
+    Vector x0(0., nd);
+    bool ok = cg.solve(x0, nullptr, 1e-10, -1, false);
+    npy_f64(tag + "_cg_x", vec(cg.x));
+    npy_i64(tag + "_cg_info", {(int64_t)ok, (int64_t)cg.nit, 32});
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
+    g_out = argv[1];
+    omp_set_num_threads(1);
+    golden_cauchy_green();
+    golden_p1("p1_a", 0.1, 0.2, 0.9, 0.35, 0.3, 0.8, 1e10, 0.2);
+    golden_p1("p1_cw", 0.0, 0.0, 0.0, 0.5, 0.25, 0.0, 12e9, 0.3); // needs CCW reorientation
+    golden_p1("p1_unit", 0.0, 0.0, 0.0, 1.0 / 128, 1.0 / 128, 1.0 / 128, 1e10, 0.2);
+    golden_p2("p2_a", 0.1, 0.2, 0.9, 0.35, 0.3, 0.8, 60e9, 0.2);
+    golden_p2("p2_unit", 0.0, 1.0 / 400, 0.0, 0.0, 1.0 / 400, 0.0, 12e9, 0.3);
+    const double t1[12] = {0.0, 0.0, 0.0, 1.0, 0.1, 0.0, 0.2, 0.9, 0.1, 0.3, 0.2, 1.1};
+    const double t2[12] = {0.0, 0.0, 0.0, 0.2, 0.9, 0.1, 1.0, 0.1, 0.0, 0.3, 0.2, 1.1}; // other orientation
+    golden_tet("tet_a", t1, 30e9, 0.2);
+    golden_tet("tet_b", t2, 30e9, 0.2);
+    golden_system_p1("sys_p1_4x3", {4, 3, 1.0, 1.0, 1e10, 0.2, 60e9, 0.2}, 32, true);
+    golden_system_p1("sys_p1_8x8", {8, 8, 1.0, 1.0, 1e10, 0.2, 1e10, 0.2}, 32, true);
+    golden_system_p1("sys_p1_8x8_nos", {8, 8, 1.0, 1.0, 1e10, 0.2, 1e10, 0.2}, 0, true);
+    golden_system_p1("sys_p1_12x10_mix", {12, 10, 1.0, 1.0, 12e9, 0.3, 60e9, 0.2}, 32, true);
+    golden_system_tet("sys_tet_2", 2, 30e9, 0.2);
+    golden_system_tet("sys_tet_3", 3, 30e9, 0.2);
+    printf("golden vectors written to %s\n", g_out.c_str());
+    return 0;
+}

@@ -1,0 +1,144 @@
+"""Pins the CPU oracle against the compiled reference's golden vectors.
+
+The fixtures in tests/golden/ref were written by oracle/ref_harness.cpp linked
+against the reference AMIE sources (oracle/Makefile.ref), with one OpenMP
+thread; they hold the reference's own outputs for element matrices,
+sparsity pattern, Kahan scatter, Dirichlet elimination, SpMV, preconditioners,
+dot products and ConjugateGradient::solve.  The reference-held fixture
+examples/test/test_cauchy_green_base is restated in test_cauchy_green_fixture.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+REF = os.path.join(os.path.dirname(__file__), "golden", "ref")
+
+
+def g(name):
+    return np.load(os.path.join(REF, name + ".npy"))
+
+
+def test_cauchy_green_golden():
+    np.testing.assert_array_equal(orc.cauchy_green(1e10, 0.2), g("C_ps_E1e10_nu0.2"))
+    np.testing.assert_array_equal(orc.cauchy_green(12e9, 0.3), g("C_ps_E12e9_nu0.3"))
+    np.testing.assert_array_equal(orc.cauchy_green(60e9, 0.2), g("C_ps_E60e9_nu0.2"))
+    np.testing.assert_array_equal(orc.cauchy_green(30e9, 0.2, dim=3), g("C_3d_E30e9_nu0.2"))
+
+
+def test_cauchy_green_fixture():
+    # examples/test/test_cauchy_green_base (6 significant digits): plane stress
+    # and plane strain rows for E = 1e10, nu = 0.2.
+    ps = orc.cauchy_green(1e10, 0.2)
+    assert ps[0, 0] == pytest.approx(1.04167e10, rel=5e-6)
+    assert ps[0, 1] == pytest.approx(2.08333e9, rel=5e-6)
+    assert ps[2, 2] == pytest.approx(8.33333e9, rel=5e-6)
+    pe = orc.cauchy_green(1e10, 0.2, plane_strain=True)
+    assert pe[0, 0] == pytest.approx(1.11111e10, rel=5e-6)
+    assert pe[0, 1] == pytest.approx(2.77778e9, rel=5e-6)
+    assert pe[2, 2] == pytest.approx(8.33333e9, rel=5e-6)
+
+
+@pytest.mark.parametrize("tag,etype,E,nu", [
+    ("p1_a", orc.P1, 1e10, 0.2), ("p1_cw", orc.P1, 12e9, 0.3), ("p1_unit", orc.P1, 1e10, 0.2),
+    ("tet_a", orc.TET4, 30e9, 0.2), ("tet_b", orc.TET4, 30e9, 0.2)])
+def test_element_matrix_golden(tag, etype, E, nu):
+    xy = g("Ke_" + tag + ("_xy" if etype != orc.TET4 else "_xyz"))
+    perm = orc.orient(etype, xy)
+    np.testing.assert_array_equal(perm, g("Ke_" + tag + "_order"))
+    C = orc.cauchy_green(E, nu, dim=orc.DIM[etype])
+    K = orc.element_matrix(etype, xy[perm], C)
+    Kref = g("Ke_" + tag)
+    scale = np.abs(Kref).max()
+    assert np.abs(K - Kref).max() <= 4e-16 * scale
+
+
+@pytest.mark.parametrize("tag,E,nu", [("p2_a", 60e9, 0.2), ("p2_unit", 12e9, 0.3)])
+def test_p2_element_matrix_golden(tag, E, nu):
+    xy = g("Ke_" + tag + "_xy")  # already in [v0, m01, v1, m12, v2, m20] order
+    np.testing.assert_array_equal(orc.orient(orc.P2, xy), np.arange(6))
+    K = orc.element_matrix(orc.P2, xy, orc.cauchy_green(E, nu))
+    Kref = g("Ke_" + tag)
+    assert np.abs(K - Kref).max() <= 4e-16 * np.abs(Kref).max()
+
+
+P1_SYSTEMS = ["sys_p1_4x3", "sys_p1_8x8", "sys_p1_8x8_nos", "sys_p1_12x10_mix"]
+TET_SYSTEMS = ["sys_tet_2", "sys_tet_3"]
+
+
+def load_system(tag):
+    if tag.startswith("sys_tet"):
+        etype = orc.TET4
+        coords = g(tag + "_xyz")
+        E, nu = g(tag + "_mat")
+        cm = orc.cauchy_green(E, nu, dim=3)[None]
+        phase = np.zeros(len(g(tag + "_conn")), dtype=np.int32)
+    else:
+        etype = orc.P1
+        coords = g(tag + "_xy")
+        E0, nu0, E1, nu1 = g(tag + "_mat")
+        cm = np.stack([orc.cauchy_green(E0, nu0), orc.cauchy_green(E1, nu1)])
+        phase = g(tag + "_phase").astype(np.int32)
+    return etype, g(tag + "_conn").astype(np.int32), coords, cm, phase
+
+
+@pytest.mark.parametrize("tag", P1_SYSTEMS + TET_SYSTEMS)
+def test_pattern_bit_exact(tag):
+    etype, conn, coords, cm, phase = load_system(tag)
+    rs, ci = orc.pattern(etype, conn, coords.shape[0])
+    np.testing.assert_array_equal(rs, g(tag + "_row_size"))
+    np.testing.assert_array_equal(ci, g(tag + "_column_index"))
+
+
+@pytest.mark.parametrize("tag", P1_SYSTEMS)
+def test_raw_assembly_golden(tag):
+    etype, conn, coords, cm, phase = load_system(tag)
+    A = orc.assemble(etype, conn, coords, cm, phase)
+    ref = g(tag + "_raw_array")
+    assert np.abs(A.array - ref).max() <= 1e-15 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("tag", P1_SYSTEMS + TET_SYSTEMS)
+def test_assembly_with_bcs_golden(tag):
+    etype, conn, coords, cm, phase = load_system(tag)
+    A = orc.assemble(etype, conn, coords, cm, phase)
+    rhs = np.zeros(A.n)
+    orc.apply_dirichlet(A, rhs, g(tag + "_bc_dof"), g(tag + "_bc_val"))
+    ref = g(tag + "_array")
+    assert np.abs(A.array - ref).max() <= 1e-15 * np.abs(ref).max()
+    fref = g(tag + "_forces")
+    assert np.abs(rhs - fref).max() <= 1e-14 * np.abs(fref).max()
+
+
+@pytest.mark.parametrize("tag", P1_SYSTEMS + TET_SYSTEMS)
+def test_spmv_golden(tag):
+    A = orc.BlockCSR(2 if "p1" in tag else 3, g(tag + "_row_size"), g(tag + "_column_index"), g(tag + "_array"))
+    b = g(tag + "_forces")
+    x = g(tag + "_spmv_x")
+    y = orc.spmv(A, x, b)
+    np.testing.assert_array_equal(y, g(tag + "_spmv_Axmb"))
+    if "p1" in tag:
+        np.testing.assert_array_equal(orc.spmv(A, x), g(tag + "_spmv_Ax"))
+        assert orc.dot(x, g(tag + "_spmv_Ax")) == g(tag + "_dot_x_Ax")[0]
+        np.testing.assert_array_equal(orc.inverse_diagonal(A) * x, g(tag + "_invdiag_x"))
+        blk = orc.inverse_2x2_diagonal(A)
+        xx = x.reshape(-1, 2)
+        t = np.stack([blk[:, 0] * xx[:, 0] + blk[:, 1] * xx[:, 1], blk[:, 2] * xx[:, 0] + blk[:, 3] * xx[:, 1]], 1)
+        np.testing.assert_allclose(t.ravel(), g(tag + "_inv2x2_x"), rtol=0, atol=1e-25)
+
+
+@pytest.mark.parametrize("tag", P1_SYSTEMS + TET_SYSTEMS)
+def test_cg_reference_golden(tag):
+    A = orc.BlockCSR(2 if "p1" in tag else 3, g(tag + "_row_size"), g(tag + "_column_index"), g(tag + "_array"))
+    b = g(tag + "_forces")
+    ok_ref, nit_ref, nssor = g(tag + "_cg_info")
+    x, nit, ok = orc.cg_reference(A, b, np.zeros(A.n), eps=1e-10, nssor=int(nssor))
+    assert ok == ok_ref
+    assert nit == nit_ref
+    # identical iteration count; the solution agrees to a few ulps of its
+    # largest entry (the first q = A p of each restart goes through another
+    # expression path in the reference)
+    xref = g(tag + "_cg_x")
+    assert np.abs(x - xref).max() <= 1e-12 * np.abs(xref).max()
