@@ -1,0 +1,237 @@
+/* amie_gpu.h -- C ABI of the MI355X (gfx950) FE-solve hot path for AMIE.
+ *
+ * The boundary sits where the reference hands a finished linear system to a
+ * LinearSolver and where it integrates / assembles element matrices:
+ *
+ *   LinearSolver::solve(x0, precond, eps, maxit, verbose)   solvers/solver.h:46-52
+ *   Assembly::getMatrix() / getForces()                      solvers/assembly.h:337-346
+ *   Assembly::mgprepare() + mgsolve(LinearSolver*, ...)      solvers/assembly.cpp:1225-1228,1955-1969
+ *   Assembly::add(element, scale) + make_final()             solvers/assembly.cpp:106-123,425-1223
+ *   Assembly::setPointAlong(variable, value, node)           solvers/assembly.cpp:1590-1634
+ *   DelaunayTriangle/Tetrahedron::getElementaryMatrix        mesher/delaunay.cpp:2408, delaunay_3d.cpp:2452
+ *   FeatureTree::stepElements (strain gather + damage)       features/features.cpp:4875-5100
+ *
+ * All arguments are plain pointers and sizes; host arrays are caller-owned
+ * and copied.  Every call returns an int status (AMIE_OK = 0, 1 = solver did
+ * not converge, < 0 = error) instead of exiting the process as the reference
+ * does (solvers/conjugategradient.cpp:160-165); amie_gpu_last_error() gives
+ * the message of the calling thread's last failure.  Handles are not
+ * thread-safe: use one handle per host thread or lock around it.
+ *
+ * Layout conventions:
+ *  - element types: AMIE_P1 (3-node triangle), AMIE_P2 (6-node triangle, nodes
+ *    [v0, m01, v1, m12, v2, m20]), AMIE_TET4 (4-node tetrahedron), each in
+ *    the local order the reference produces after its orientation rule
+ *    (amie_mesh_orient applies it);
+ *  - DOF = node*b + axis with b = 2 (2D) or 3 (3D);
+ *  - the reference block CSR: row_size[nrows], column_index[nnzb],
+ *    array[nnzb*b*colLen], blocks column-major, colLen = b + b%2
+ *    (sparse/sparse_matrix.h:129-137, sparse/sparse_vector.cpp:27-58).
+ */
+#ifndef AMIE_GPU_H
+#define AMIE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AMIE_OK = 0,
+    AMIE_NOT_CONVERGED = 1,
+    AMIE_ERR_ARG = -1,
+    AMIE_ERR_HIP = -2,
+    AMIE_ERR_NOMEM = -3,
+    AMIE_ERR_NAN = -4,
+    AMIE_ERR_STATE = -5,
+    AMIE_ERR_BUSY = -6
+};
+
+enum { AMIE_P1 = 1, AMIE_P2 = 2, AMIE_TET4 = 3 };
+
+enum { AMIE_FP64 = 0, AMIE_MIXED = 1 };               /* solve precision */
+enum { AMIE_JACOBI = 0, AMIE_BLOCK_JACOBI = 1 };      /* preconditioner */
+enum { AMIE_REFINE_REPLACE = 0, AMIE_REFINE_RESTART = 1 };
+
+typedef struct amie_gpu_ctx amie_gpu_ctx;
+typedef struct amie_gpu_batch amie_gpu_batch;
+typedef struct amie_gpu_server amie_gpu_server;
+
+/* One finite-element problem: mesh, materials, damage, boundary conditions
+ * and right-hand sides. */
+typedef struct {
+    int32_t elem_type;       /* AMIE_P1 | AMIE_P2 | AMIE_TET4 */
+    int64_t nnodes;          /* geometric nodes */
+    int64_t nelem;
+    const double *coords;    /* nnodes*dim */
+    const int32_t *conn;     /* nelem*nper geometric node ids, oriented */
+    const int32_t *dofmap;   /* nullable: geometric node -> DOF node (periodic
+                                condensation); NULL = identity */
+    int64_t ndofnodes;       /* DOF nodes (block rows); = nnodes when dofmap is NULL */
+    int32_t nphase;
+    const double *cmat;      /* nphase C matrices, 3x3 (2D) or 6x6 (3D), row-major */
+    const int32_t *phase;    /* nullable: element phase, default 0 */
+    const double *damage;    /* nullable: element damage d, C_e = C*(1-d) */
+    /* Dirichlet conditions (Assembly::setPointAlong): DOF ids and values;
+       a later entry for the same DOF replaces an earlier one. */
+    int64_t nbc;
+    const int64_t *bc_dofs;
+    const double *bc_vals;
+    /* right-hand sides: nrhs columns of external forces (nullable, ndof each,
+       column-major), plus optional macro strains (nrhs x 3|6, Voigt with
+       engineering shear) imposed as u = E.x + periodic fluctuation */
+    int32_t nrhs;
+    const double *forces;
+    const double *macro_strain;
+} amie_problem_desc;
+
+typedef struct {
+    int32_t precision;       /* AMIE_FP64 | AMIE_MIXED */
+    int32_t precond;         /* AMIE_JACOBI | AMIE_BLOCK_JACOBI */
+    int32_t refine;          /* AMIE_REFINE_REPLACE | AMIE_REFINE_RESTART */
+    double rtol;             /* stop on ||b - K x|| / ||b|| <= rtol (FP64 residual) */
+    double refine_drop;      /* FP32 residual drop that triggers an FP64 refinement (mixed) */
+    int64_t maxit;           /* per right-hand side, <= 0: ndof */
+    int32_t use_x0;          /* 1: start from the current device solution */
+    int32_t use_graph;       /* 1: replay captured hipGraphs of iteration chunks */
+    int32_t chunk;           /* iterations per chunk between host status checks */
+    int32_t time_kernels;    /* 1: time every SpMV-kernel launch with HIP events */
+} amie_solve_opts;
+
+typedef struct {
+    int32_t status;          /* AMIE_OK | AMIE_NOT_CONVERGED | AMIE_ERR_NAN */
+    int32_t nproblems_converged;
+    int64_t iterations_max;  /* max over problems and right-hand sides */
+    int64_t iterations_sum;
+    int64_t refinements;     /* FP64 residual evaluations, summed */
+    double relres_max;       /* max final ||b - K x|| / ||b|| (FP64) */
+    double solve_ms;         /* wall time of the solve on the stream */
+    double spmv_kernel_ms;   /* summed HIP-event time of the fused SpMV kernel (time_kernels) */
+    int64_t spmv_launches;
+    double spmv_bytes;       /* algorithmic bytes per SpMV launch summed over launches */
+} amie_solve_stats;
+
+/* Damage law of the incremental-loading case (Mazars tension branch with
+ * B_t = 1/kappa_f, physics/fracturecriteria/mazars.cpp:113-139). */
+typedef struct {
+    double kappa0;
+    double kappaf;
+    double dmax;             /* cap of d (residual stiffness 1 - dmax) */
+} amie_damage_params;
+
+/* ------------------------------------------------------------ context */
+int amie_gpu_create(amie_gpu_ctx **ctx, int device);
+void amie_gpu_destroy(amie_gpu_ctx *ctx);
+const char *amie_gpu_last_error(void);
+/* Device properties useful to callers (admission control). */
+int amie_gpu_device_info(amie_gpu_ctx *ctx, size_t *free_bytes, size_t *total_bytes, int *num_cu);
+void amie_gpu_default_opts(amie_solve_opts *opts);
+
+/* ------------------------------------------------------ host mesh helpers
+ * Synthetic inputs of the benchmark configurations (no GPU needed). */
+/* Reference orientation rule (geometry/geometry_2D.cpp:476-497,
+ * geometry/geometry_3D.cpp:254-268) applied in place to conn. */
+int amie_mesh_orient(int32_t elem_type, int64_t nelem, int32_t *conn, const double *coords);
+/* Structured unit-square / unit-cube meshes; sizes via the *_size call. */
+int amie_mesh_structured_size(int32_t elem_type, int32_t n, int64_t *nnodes, int64_t *nelem);
+int amie_mesh_structured(int32_t elem_type, int32_t n, double *coords, int32_t *conn);
+/* Periodic DOF map of the structured P1 square: node (i,j) -> (i mod n, j mod n). */
+int amie_mesh_periodic_dofmap(int32_t n, int32_t *dofmap);
+/* Seeded rejection sampling of circular inclusions in the unit square
+ * (fully inside, pairwise gap >= gap) until area fraction >= target or the
+ * try cap; returns the number placed in *ncircles (xyr: 3 per circle). */
+int amie_inclusions_2d(uint64_t seed, double target_fraction, double rmin, double rmax, double gap,
+                       int64_t max_circles, double *xyr, int64_t *ncircles, double *fraction);
+/* Phase of every element by centroid: 1 inside any circle, else 0. */
+int amie_phase_by_centroid(int32_t elem_type, int64_t nelem, const int32_t *conn, const double *coords,
+                           int64_t ncircles, const double *xyr, int32_t *phase);
+/* Seeded Gaussian random field on an n^3 voxel grid (correlation length
+ * ell, separable Gaussian filter of white noise), mapped affinely to
+ * [lo, hi]; one value per tetrahedron of the 6-tet split. */
+int amie_damage_field_3d(uint64_t seed, int32_t n, double ell, double lo, double hi, double *d0);
+/* Isotropic C matrix (utilities/tensor.cpp:1845-1931, plane stress in 2D). */
+void amie_cauchy_green(double E, double nu, int32_t dim, double *C);
+
+/* ------------------------------------------------------------- batches
+ * A batch holds one or more problems resident in HBM: pattern, element->
+ * block maps, FP64 and FP32 matrices, damage state and Krylov vectors. */
+int amie_gpu_batch_create(amie_gpu_ctx *ctx, int32_t nprob, const amie_problem_desc *probs,
+                          amie_gpu_batch **out);
+/* Drop-in for a matrix the reference assembled itself (Assembly::getMatrix()
+ * and getForces()): one problem, one right-hand side. */
+int amie_gpu_batch_from_bcsr(amie_gpu_ctx *ctx, int32_t stride, int64_t nrows, const uint32_t *row_size,
+                             const uint32_t *column_index, const double *array, const double *rhs,
+                             amie_gpu_batch **out);
+void amie_gpu_batch_destroy(amie_gpu_batch *b);
+/* HBM bytes held by the batch. */
+size_t amie_gpu_batch_bytes(const amie_gpu_batch *b);
+/* Element integration + coloured scatter + Dirichlet elimination + scaling,
+ * all on the device (re-run after damage changes). */
+int amie_gpu_batch_assemble(amie_gpu_batch *b);
+int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_solve_stats *stats);
+/* Solution of problem p, original DOF order, nrhs columns (ndof each). */
+int amie_gpu_batch_get_solution(amie_gpu_batch *b, int32_t p, double *x);
+int amie_gpu_batch_set_solution(amie_gpu_batch *b, int32_t p, const double *x);
+/* Pattern and assembled matrix of problem p in the reference layout (after
+ * Dirichlet elimination) and its right-hand sides. */
+int amie_gpu_batch_get_nnzb(amie_gpu_batch *b, int32_t p, int64_t *nrows, int64_t *nnzb);
+int amie_gpu_batch_get_matrix(amie_gpu_batch *b, int32_t p, uint32_t *row_size, uint32_t *column_index,
+                              double *array, double *rhs);
+/* Element -> block slot map of problem p: nelem*nper*nper indices into the
+ * reference block order (column_index positions), pair (i,j) row-major. */
+int amie_gpu_batch_get_element_blocks(amie_gpu_batch *b, int32_t p, int64_t *slots);
+/* y = K x for problem p (FP64 matrix, or the scaled FP32 matrix un-scaled
+ * back when precision = AMIE_MIXED); ndof vectors, original order. */
+int amie_gpu_batch_spmv(amie_gpu_batch *b, int32_t p, int32_t precision, const double *x, double *y);
+/* Element strains (nelem x 3|6, Voigt, engineering shear) of right-hand side
+ * rhs, including the macro strain. */
+int amie_gpu_batch_element_strain(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain);
+/* Volume-averaged stress per right-hand side: nprob x nrhs x 3|6.  For unit
+ * macro strains this is the effective stiffness. */
+int amie_gpu_batch_average_stress(amie_gpu_batch *b, double *out);
+/* One damage step on the device for every problem: strain of the current
+ * solution (right-hand side 0), kappa history, d update, then re-assembly.
+ * d0 (nullable, per problem element count) seeds the damage on first call.
+ * n_grown (nullable) returns the number of elements whose damage grew. */
+int amie_gpu_batch_damage_step(amie_gpu_batch *b, const amie_damage_params *prm, int64_t *n_grown);
+int amie_gpu_batch_get_damage(amie_gpu_batch *b, int32_t p, double *d, double *kappa);
+int amie_gpu_batch_set_damage(amie_gpu_batch *b, int32_t p, const double *d);
+/* Scale the Dirichlet values of every problem (incremental loading). */
+int amie_gpu_batch_scale_bc(amie_gpu_batch *b, double factor);
+
+/* -------------------------------------------------------------- serving
+ * Continuous batching of solve requests: each request is admitted when its
+ * HBM footprint fits the budget and a slot is free; admitted requests share
+ * every kernel launch of the solve loop, and new ones join between
+ * iteration chunks while others are in flight. */
+typedef struct {
+    uint64_t ticket;
+    int32_t status;
+    int32_t iterations;
+    double relres;
+    double submit_ms;        /* host clock at submission */
+    double start_ms;         /* admission */
+    double done_ms;          /* completion */
+} amie_request_result;
+
+int amie_gpu_server_create(amie_gpu_ctx *ctx, size_t hbm_budget, int32_t max_slots, const amie_solve_opts *opts,
+                           amie_gpu_server **out);
+void amie_gpu_server_destroy(amie_gpu_server *s);
+/* Queue a request (copied).  persistent_key != 0 keeps the problem's
+ * matrix and damage state resident between requests carrying that key
+ * (damage steps); the request then runs one damage step + solve. */
+int amie_gpu_server_submit(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t persistent_key,
+                           double submit_ms, uint64_t *ticket);
+/* Advance the solve loop by one chunk (admits, iterates, retires). */
+int amie_gpu_server_step(amie_gpu_server *s, double now_ms, int32_t *n_active, int32_t *n_queued);
+/* Completed requests since the last poll. */
+int amie_gpu_server_poll(amie_gpu_server *s, amie_request_result *out, int32_t max, int32_t *n);
+/* Solution of a completed request (until the next poll). */
+int amie_gpu_server_get_solution(amie_gpu_server *s, uint64_t ticket, double *x, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -460,6 +460,13 @@ void or_assemble(int etype, int64_t nelem, const int32_t *conn, const double *co
                  const int32_t *phase, const double *damage, int64_t nrows, const uint32_t *row_size,
                  const uint32_t *column_index, double *array)
 {
+    or_assemble_dofmap(etype, nelem, conn, nullptr, coords, cmat, phase, damage, nrows, row_size, column_index, array);
+}
+
+void or_assemble_dofmap(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap, const double *coords,
+                        const double *cmat, const int32_t *phase, const double *damage, int64_t nrows,
+                        const uint32_t *row_size, const uint32_t *column_index, double *array)
+{
     Shape sh = shape_of(etype);
     int b = sh.b, cl = col_len(b), s = (b == 2) ? 3 : 6, n = sh.nper * b;
     std::vector<uint32_t> start = row_starts(nrows, row_size);
@@ -468,9 +475,11 @@ void or_assemble(int etype, int64_t nelem, const int32_t *conn, const double *co
     std::vector<double> comp(nnzb * b * cl, 0.);
     std::vector<double> Ke(n * n), xy(sh.nper * sh.dim), C(s * s);
     for (int64_t e = 0; e < nelem; e++) {
-        const int32_t *ids = conn + e * sh.nper;
+        const int32_t *geo = conn + e * sh.nper;
+        int32_t ids[6];
+        for (int i = 0; i < sh.nper; i++) ids[i] = dofmap ? dofmap[geo[i]] : geo[i];
         for (int i = 0; i < sh.nper; i++)
-            for (int d = 0; d < sh.dim; d++) xy[i * sh.dim + d] = coords[(size_t)ids[i] * sh.dim + d];
+            for (int d = 0; d < sh.dim; d++) xy[i * sh.dim + d] = coords[(size_t)geo[i] * sh.dim + d];
         const double *Cp = cmat + (size_t)(phase ? phase[e] : 0) * s * s;
         // physics/damagemodels/isotropiclineardamage.cpp:39-46: C * (1 - d)
         for (int q = 0; q < s * s; q++) C[q] = damage ? Cp[q] * (1. - damage[e]) : Cp[q];
@@ -496,6 +505,43 @@ void or_assemble(int etype, int64_t nelem, const int32_t *conn, const double *co
                         kadd(a0 + m * cl + nn, c0 + m * cl + nn, Ke[(j * b + nn) * n + k * b + m]);
                         kadd(a1 + m * cl + nn, c1 + m * cl + nn, Ke[(k * b + nn) * n + j * b + m]);
                     }
+            }
+        }
+    }
+}
+
+void or_macro_rhs(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap, const double *coords,
+                  const double *cmat, const int32_t *phase, const double *damage, const double *macro,
+                  int64_t nrows, double *f)
+{
+    Shape sh = shape_of(etype);
+    int b = sh.b, s = (b == 2) ? 3 : 6, n = sh.nper * b;
+    std::fill(f, f + nrows * b, 0.);
+    std::vector<double> Ke(n * n), xy(sh.nper * sh.dim), C(s * s), u(n);
+    for (int64_t e = 0; e < nelem; e++) {
+        const int32_t *geo = conn + e * sh.nper;
+        for (int i = 0; i < sh.nper; i++)
+            for (int d = 0; d < sh.dim; d++) xy[i * sh.dim + d] = coords[(size_t)geo[i] * sh.dim + d];
+        const double *Cp = cmat + (size_t)(phase ? phase[e] : 0) * s * s;
+        for (int q = 0; q < s * s; q++) C[q] = damage ? Cp[q] * (1. - damage[e]) : Cp[q];
+        or_element_matrix(etype, xy.data(), C.data(), Ke.data());
+        for (int i = 0; i < sh.nper; i++) {
+            const double *X = &xy[i * sh.dim];
+            if (sh.dim == 2) {
+                u[i * 2] = macro[0] * X[0] + 0.5 * macro[2] * X[1];
+                u[i * 2 + 1] = 0.5 * macro[2] * X[0] + macro[1] * X[1];
+            } else {
+                u[i * 3] = macro[0] * X[0] + 0.5 * macro[5] * X[1] + 0.5 * macro[4] * X[2];
+                u[i * 3 + 1] = 0.5 * macro[5] * X[0] + macro[1] * X[1] + 0.5 * macro[3] * X[2];
+                u[i * 3 + 2] = 0.5 * macro[4] * X[0] + 0.5 * macro[3] * X[1] + macro[2] * X[2];
+            }
+        }
+        for (int i = 0; i < sh.nper; i++) {
+            const int64_t row = dofmap ? dofmap[geo[i]] : geo[i];
+            for (int r = 0; r < b; r++) {
+                double t = 0.;
+                for (int c = 0; c < n; c++) t += Ke[(i * b + r) * n + c] * u[c];
+                f[row * b + r] -= t;
             }
         }
     }
@@ -809,6 +855,12 @@ int or_pcg(int stride, int64_t nrows, const uint32_t *row_size, const uint32_t *
 void or_element_strain(int etype, int64_t nelem, const int32_t *conn, const double *coords, const double *u,
                        double *strain)
 {
+    or_element_strain_dofmap(etype, nelem, conn, nullptr, coords, u, nullptr, strain);
+}
+
+void or_element_strain_dofmap(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap,
+                              const double *coords, const double *u, const double *macro, double *strain)
+{
     Shape sh = shape_of(etype);
     int s = sh.dim == 2 ? 3 : 6;
 #pragma omp parallel for schedule(static)
@@ -824,18 +876,20 @@ void or_element_strain(int etype, int64_t nelem, const int32_t *conn, const doub
         if (sh.dim == 2) {
             double x_xi = 0, x_eta = 0, y_xi = 0, y_eta = 0;
             for (int j = 0; j < sh.nper; j++) {
-                double ux = u[(size_t)ids[j] * 2], uy = u[(size_t)ids[j] * 2 + 1];
+                const size_t dn = dofmap ? dofmap[ids[j]] : ids[j];
+                double ux = u[dn * 2], uy = u[dn * 2 + 1];
                 x_xi += dxi[j] * ux; x_eta += deta[j] * ux;
                 y_xi += dxi[j] * uy; y_eta += deta[j] * uy;
             }
             out[0] = x_xi * Jinv[0] + x_eta * Jinv[1];
             out[1] = y_xi * Jinv[2] + y_eta * Jinv[3];
             out[2] = x_xi * Jinv[2] + x_eta * Jinv[3] + y_xi * Jinv[0] + y_eta * Jinv[1];
+            if (macro) for (int q = 0; q < 3; q++) out[q] += macro[q];
         } else {
             double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}; // g[comp][ref dir]
             for (int j = 0; j < sh.nper; j++)
                 for (int c = 0; c < 3; c++) {
-                    double v = u[(size_t)ids[j] * 3 + c];
+                    double v = u[(size_t)(dofmap ? dofmap[ids[j]] : ids[j]) * 3 + c];
                     g[c][0] += dxi[j] * v; g[c][1] += deta[j] * v; g[c][2] += dzeta[j] * v;
                 }
             auto d = [&](int c, int row) { return g[c][0] * Jinv[row * 3] + g[c][1] * Jinv[row * 3 + 1] + g[c][2] * Jinv[row * 3 + 2]; };
@@ -845,6 +899,7 @@ void or_element_strain(int etype, int64_t nelem, const int32_t *conn, const doub
             out[3] = d(1, 2) + d(2, 1);
             out[4] = d(0, 2) + d(2, 0);
             out[5] = d(1, 0) + d(0, 1);
+            if (macro) for (int q = 0; q < 6; q++) out[q] += macro[q];
         }
     }
 }

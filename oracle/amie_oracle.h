@@ -44,6 +44,24 @@ void or_assemble(int etype, int64_t nelem, const int32_t *conn, const double *co
                  int64_t nrows, const uint32_t *row_size, const uint32_t *column_index,
                  double *array);
 
+/* or_assemble with a DOF map: element node i of geometric node conn[e][i]
+ * contributes to block row dofmap[conn[e][i]] (periodic condensation; the
+ * reference realises it with points that share an id).  dofmap NULL =
+ * identity. */
+void or_assemble_dofmap(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap, const double *coords,
+                        const double *cmat, const int32_t *phase, const double *damage, int64_t nrows,
+                        const uint32_t *row_size, const uint32_t *column_index, double *array);
+
+/* Load of an imposed macro strain E (Voigt, engineering shear): u_E = E.x at
+ * every node, f -= sum_e K_e u_E,e scattered through the DOF map. */
+void or_macro_rhs(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap, const double *coords,
+                  const double *cmat, const int32_t *phase, const double *damage, const double *macro,
+                  int64_t nrows, double *f);
+
+/* or_element_strain through a DOF map, plus the macro strain (nullable). */
+void or_element_strain_dofmap(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap,
+                              const double *coords, const double *u, const double *macro, double *strain);
+
 /* solvers/assembly.cpp:125-322: symmetric elimination, unit diagonal,
  * RHS lift, b[id] = g.  dofs need not be sorted. */
 void or_apply_dirichlet(int stride, int64_t nrows, const uint32_t *row_size,

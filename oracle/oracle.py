@@ -44,6 +44,9 @@ def lib():
         L.or_pcg.argtypes = [i32, i64, vp, vp, vp, vp, vp, dbl, i64, i32, vp, vp, vp]
         L.or_pcg.restype = i32
         L.or_element_strain.argtypes = [i32, i64, vp, vp, vp, vp]
+        L.or_assemble_dofmap.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp]
+        L.or_macro_rhs.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp, i64, vp]
+        L.or_element_strain_dofmap.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp]
         L.or_damage_update.argtypes = [i64, i32, vp, vp, dbl, dbl, dbl, vp, vp]
         L.or_damage_update.restype = i64
         L.or_average_stress.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp]
@@ -145,6 +148,49 @@ def assemble(etype, conn, coords, cmats, phase=None, damage=None):
     lib().or_assemble(etype, conn.shape[0], _p(conn), _p(coords), _p(cmats), _p(phase), _p(damage),
                       A.nrows, _p(A.row_size), _p(A.column_index), _p(A.array))
     return A
+
+
+def assemble_periodic(etype, conn, dofmap, coords, cmats, phase=None, damage=None):
+    """Stiffness through a DOF map (periodic condensation)."""
+    conn = _c(conn, np.int32)
+    dofmap = _c(dofmap, np.int32)
+    coords = _c(coords, np.float64)
+    cmats = _c(cmats, np.float64)
+    phase = _c(phase, np.int32)
+    damage = _c(damage, np.float64)
+    nrows = int(dofmap.max()) + 1
+    rs, ci = pattern(etype, dofmap[conn], nrows)
+    A = BlockCSR(DIM[etype], rs, ci)
+    lib().or_assemble_dofmap(etype, conn.shape[0], _p(conn), _p(dofmap), _p(coords), _p(cmats), _p(phase),
+                             _p(damage), A.nrows, _p(A.row_size), _p(A.column_index), _p(A.array))
+    return A
+
+
+def macro_rhs(etype, conn, dofmap, coords, cmats, phase, macro, nrows, damage=None):
+    conn = _c(conn, np.int32)
+    dofmap = _c(dofmap, np.int32)
+    coords = _c(coords, np.float64)
+    cmats = _c(cmats, np.float64)
+    phase = _c(phase, np.int32)
+    damage = _c(damage, np.float64)
+    macro = _c(macro, np.float64)
+    f = np.zeros(nrows * DIM[etype])
+    lib().or_macro_rhs(etype, conn.shape[0], _p(conn), _p(dofmap), _p(coords), _p(cmats), _p(phase), _p(damage),
+                       _p(macro), nrows, _p(f))
+    return f
+
+
+def element_strain_dofmap(etype, conn, dofmap, coords, u, macro=None):
+    conn = _c(conn, np.int32)
+    dofmap = _c(dofmap, np.int32)
+    coords = _c(coords, np.float64)
+    u = _c(u, np.float64)
+    macro = _c(macro, np.float64)
+    s = 3 if DIM[etype] == 2 else 6
+    out = np.zeros((conn.shape[0], s))
+    lib().or_element_strain_dofmap(etype, conn.shape[0], _p(conn), _p(dofmap), _p(coords), _p(u), _p(macro),
+                                   _p(out))
+    return out
 
 
 def apply_dirichlet(A, rhs, dofs, vals):

@@ -1,0 +1,309 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle (which is
+itself pinned to the compiled reference, tests/test_oracle_golden.py) and
+against the reference's golden vectors directly.
+
+Tolerances: pattern, element->block maps and DOF maps are compared bit for
+bit; FP64 matrices and right-hand sides to 1e-14 of their largest entry
+(summation order differs from the reference's serial Kahan scatter); the
+FP64 SpMV to 1e-13; the scaled FP32 SpMV to 2e-6; solutions through the
+relative residual they reach plus the displacement error against the FP64
+oracle solution.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import amie_gpu as ag
+import configs
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+REF = os.path.join(os.path.dirname(__file__), "golden", "ref")
+
+
+def g(name):
+    return np.load(os.path.join(REF, name + ".npy"))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ag.Context(0)
+    yield c
+    c.close()
+
+
+def golden_problem(tag):
+    if tag.startswith("sys_tet"):
+        coords = g(tag + "_xyz")
+        E, nu = g(tag + "_mat")
+        cm = ag.cauchy_green(E, nu, dim=3)[None]
+        return ag.Problem(ag.TET4, coords, g(tag + "_conn"), cm, bc_dofs=g(tag + "_bc_dof"),
+                          bc_vals=g(tag + "_bc_val"))
+    E0, nu0, E1, nu1 = g(tag + "_mat")
+    cm = np.stack([ag.cauchy_green(E0, nu0), ag.cauchy_green(E1, nu1)])
+    return ag.Problem(ag.P1, g(tag + "_xy"), g(tag + "_conn"), cm, phase=g(tag + "_phase"),
+                      bc_dofs=g(tag + "_bc_dof"), bc_vals=g(tag + "_bc_val"))
+
+
+SYSTEMS = ["sys_p1_4x3", "sys_p1_8x8", "sys_p1_12x10_mix", "sys_tet_2", "sys_tet_3"]
+
+
+@pytest.mark.parametrize("tag", SYSTEMS)
+def test_assembly_matches_reference_golden(ctx, tag):
+    print("case", tag, flush=True)
+    B = ag.Batch(ctx, [golden_problem(tag)])
+    rs, ci, ar, rhs = B.matrix()
+    np.testing.assert_array_equal(rs, g(tag + "_row_size"))
+    np.testing.assert_array_equal(ci, g(tag + "_column_index"))
+    ref = g(tag + "_array")
+    assert np.abs(ar - ref).max() <= 1e-14 * np.abs(ref).max()
+    fref = g(tag + "_forces")
+    assert np.abs(rhs - fref).max() <= 1e-13 * np.abs(fref).max()
+
+
+def element_blocks_oracle(etype, conn, dofmap=None):
+    dconn = conn if dofmap is None else dofmap[conn]
+    nrows = int(dconn.max()) + 1
+    rs, ci = orc.pattern(etype, dconn, nrows)
+    start = np.concatenate([[0], np.cumsum(rs.astype(np.int64))])
+    n = conn.shape[1]
+    out = np.zeros((conn.shape[0], n, n), dtype=np.int64)
+    for i in range(n):
+        for j in range(n):
+            r = dconn[:, i]
+            c = dconn[:, j]
+            pos = np.array([start[a] + np.searchsorted(ci[start[a]:start[a + 1]], b) for a, b in zip(r, c)])
+            out[:, i, j] = pos
+    return out
+
+
+@pytest.mark.parametrize("etype,n", [(ag.P1, 9), (ag.P2, 5), (ag.TET4, 3)])
+def test_element_block_map_bit_exact(ctx, etype, n):
+    print("case", etype, n, flush=True)
+    coords, conn = ag.structured_mesh(etype, n)
+    C = ag.cauchy_green(1e10, 0.2, dim=ag.DIM[etype])[None]
+    B = ag.Batch(ctx, [ag.Problem(etype, coords, conn, C)])
+    np.testing.assert_array_equal(B.element_blocks(), element_blocks_oracle(etype, conn))
+    rs, ci, _, _ = B.matrix()
+    ors, oci = orc.pattern(etype, conn, coords.shape[0])
+    np.testing.assert_array_equal(rs, ors)
+    np.testing.assert_array_equal(ci, oci)
+
+
+def test_periodic_block_map_bit_exact(ctx):
+    p = configs.rve(10, 7, 0.3)
+    B = ag.Batch(ctx, [p])
+    np.testing.assert_array_equal(B.element_blocks(), element_blocks_oracle(ag.P1, p.conn, p.dofmap))
+
+
+@pytest.mark.parametrize("make", ["p1", "p2", "tet"])
+def test_assembly_vs_oracle(ctx, make):
+    print("case", make, flush=True)
+    p = {"p1": lambda: configs.config1(16), "p2": lambda: configs.config2(10),
+         "tet": lambda: configs.config3(5)}[make]()
+    B = ag.Batch(ctx, [p])
+    rs, ci, ar, rhs = B.matrix()
+    A = orc.assemble(p.etype, p.conn, p.coords, p.cmats, p.phase, p.damage)
+    b = np.zeros(A.n)
+    orc.apply_dirichlet(A, b, p.bc_dofs, p.bc_vals)
+    np.testing.assert_array_equal(rs, A.row_size)
+    np.testing.assert_array_equal(ci, A.column_index)
+    assert np.abs(ar - A.array).max() <= 1e-14 * np.abs(A.array).max()
+    assert np.abs(rhs - b).max() <= 1e-13 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("make", ["p1", "p2", "tet"])
+def test_spmv_fp64_fp32(ctx, make):
+    print("case", make, flush=True)
+    p = {"p1": lambda: configs.config1(20), "p2": lambda: configs.config2(12),
+         "tet": lambda: configs.config3(6)}[make]()
+    B = ag.Batch(ctx, [p])
+    rs, ci, ar, _ = B.matrix()
+    A = orc.BlockCSR(ag.DIM[p.etype], rs, ci, ar)
+    x = np.random.default_rng(1).uniform(-1, 1, A.n)
+    yref = orc.spmv(A, x)
+    scale = np.abs(yref).max()
+    y64 = B.spmv(x, precision=ag.FP64)
+    assert np.abs(y64 - yref).max() <= 1e-13 * scale
+    y32 = B.spmv(x, precision=ag.MIXED)
+    assert np.abs(y32 - yref).max() <= 2e-6 * scale
+
+
+def test_fp64_cg_patch_test(ctx):
+    # uniaxial patch test: exact P1 solution u_y = 1e-3 y, u_x = -nu 1e-3 x
+    p = configs.config1(32)
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=ag.FP64, rtol=1e-10)
+    assert st["status"] == 0 and st["relres_max"] <= 1e-10
+    u = B.solution().reshape(-1, 2)
+    x, y = p.coords[:, 0], p.coords[:, 1]
+    np.testing.assert_allclose(u[:, 1], 1e-3 * y, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(u[:, 0], -0.2 * 1e-3 * x, rtol=0, atol=1e-10)
+
+
+def oracle_solution(p, rtol=1e-13, precond=0):
+    A = orc.assemble(p.etype, p.conn, p.coords, p.cmats, p.phase, p.damage)
+    b = np.zeros(A.n)
+    orc.apply_dirichlet(A, b, p.bc_dofs, p.bc_vals)
+    x, it, rr, ok = orc.pcg(A, b, rtol=rtol, maxit=200000, precond=precond)
+    assert ok
+    return A, b, x
+
+
+@pytest.mark.parametrize("precision,precond", [(ag.FP64, ag.JACOBI), (ag.MIXED, ag.JACOBI),
+                                               (ag.MIXED, ag.BLOCK_JACOBI), (ag.FP64, ag.BLOCK_JACOBI)])
+def test_cg_composite_p2_vs_oracle(ctx, precision, precond):
+    print("case", precision, precond, flush=True)
+    p = configs.config2(24)
+    A, b, xref = oracle_solution(p)
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=precision, precond=precond, rtol=1e-8)
+    assert st["status"] == 0, st
+    x = B.solution()
+    rel = np.linalg.norm(orc.spmv(A, x, b)) / np.linalg.norm(b)
+    assert rel <= 1.01e-8
+    assert abs(rel - st["relres_max"]) <= 1e-3 * rel + 1e-15
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
+
+
+@pytest.mark.parametrize("precision,precond", [(ag.MIXED, ag.BLOCK_JACOBI), (ag.FP64, ag.JACOBI)])
+def test_cg_damaged_tet_vs_oracle(ctx, precision, precond):
+    p = configs.config3(8)
+    A, b, xref = oracle_solution(p)
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=precision, precond=precond, rtol=1e-8)
+    assert st["status"] == 0, st
+    x = B.solution()
+    assert np.linalg.norm(orc.spmv(A, x, b)) / np.linalg.norm(b) <= 1.01e-8
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
+
+
+@pytest.mark.parametrize("tag", ["sys_p1_8x8", "sys_p1_12x10_mix", "sys_tet_3"])
+def test_bcsr_dropin_vs_reference_cg(ctx, tag):
+    """Drop-in path: the reference-assembled matrix and forces in, the
+    reference ConjugateGradient's converged solution out."""
+    stride = 3 if "tet" in tag else 2
+    B = ag.Batch(ctx, bcsr=(stride, g(tag + "_row_size"), g(tag + "_column_index"), g(tag + "_array"),
+                            g(tag + "_forces")))
+    st = B.solve(precision=ag.FP64, rtol=1e-12)
+    assert st["status"] == 0
+    x = B.solution()
+    xref = g(tag + "_cg_x")
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-8
+    st = B.solve(precision=ag.MIXED, rtol=1e-10)
+    assert st["status"] == 0
+    assert np.linalg.norm(B.solution() - xref) / np.linalg.norm(xref) <= 1e-7
+
+
+def test_rve_batch_effective_stiffness_vs_oracle(ctx):
+    probs = [configs.rve(n, 100 + n, f) for n, f in [(8, 0.25), (12, 0.3), (16, 0.4)]]
+    B = ag.Batch(ctx, probs)
+    st = B.solve(precision=ag.MIXED, rtol=1e-10)
+    assert st["status"] == 0 and st["relres_max"] <= 1e-10
+    ceff = B.average_stress()  # [prob][load][stress]
+    for k, p in enumerate(probs):
+        A = orc.assemble_periodic(ag.P1, p.conn, p.dofmap, p.coords, p.cmats, p.phase)
+        loads = [orc.macro_rhs(ag.P1, p.conn, p.dofmap, p.coords, p.cmats, p.phase, p.macro[lc], A.nrows)
+                 for lc in range(3)]
+        orc.apply_dirichlet(A, np.zeros(A.n), p.bc_dofs, p.bc_vals)  # pinned node: unit rows, value 0
+        ref = np.zeros((3, 3))
+        for lc in range(3):
+            f = loads[lc]
+            f[p.bc_dofs] = p.bc_vals
+            x, it, rr, ok = orc.pcg(A, f, rtol=1e-13, maxit=100000)
+            assert ok
+            xg = B.solution(k)[lc]
+            assert np.linalg.norm(xg - x) <= 1e-7 * np.linalg.norm(x)
+            eps = orc.element_strain_dofmap(ag.P1, p.conn, p.dofmap, p.coords, x, p.macro[lc])
+            s, vol = orc.average_stress(ag.P1, p.conn, p.coords, p.cmats, p.phase, eps)
+            ref[lc] = s
+        np.testing.assert_allclose(ceff[k], ref, rtol=1e-7, atol=1e-7 * np.abs(ref).max())
+        C = ceff[k] / 1e-3
+        assert np.abs(C - C.T).max() <= 1e-5 * np.abs(C).max()
+
+
+def test_batch_result_independent_of_mix(ctx):
+    probs = [configs.rve(n, 300 + n, 0.3) for n in (10, 14, 20)]
+    alone = ag.Batch(ctx, [configs.rve(14, 314, 0.3)])
+    alone.solve(precision=ag.MIXED, rtol=1e-8)
+    mixed = ag.Batch(ctx, probs)
+    mixed.solve(precision=ag.MIXED, rtol=1e-8)
+    np.testing.assert_array_equal(alone.solution(0), mixed.solution(1))
+
+
+def test_graph_replay_matches_launches(ctx):
+    p = configs.config2(16)
+    B = ag.Batch(ctx, [p])
+    B.solve(precision=ag.MIXED, rtol=1e-8, use_graph=1)
+    x1 = B.solution().copy()
+    B.solve(precision=ag.MIXED, rtol=1e-8, use_graph=0)
+    x2 = B.solution().copy()
+    B.solve(precision=ag.MIXED, rtol=1e-8, use_graph=1)
+    x3 = B.solution().copy()
+    np.testing.assert_array_equal(x1, x2)
+    np.testing.assert_array_equal(x1, x3)
+
+
+def test_damage_step_vs_oracle(ctx):
+    p = configs.config3(6, step=10)
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=ag.FP64, rtol=1e-12)
+    assert st["status"] == 0
+    x = B.solution()
+    d0 = p.damage.copy()
+    grown = B.damage_step(kappa0=1e-5, kappaf=1e-3)
+    d, kap = B.damage()
+    eps = orc.element_strain(ag.TET4, p.conn, p.coords, x)
+    kref = np.zeros(len(d0))
+    dref = d0.copy()
+    gref = orc.damage_update(eps, kref, dref, d0=d0, kappa0=1e-5, kappaf=1e-3)
+    assert grown == gref
+    np.testing.assert_allclose(kap, kref, rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(d, dref, rtol=1e-12, atol=1e-15)
+    # re-assembled with C (1 - d)
+    rs, ci, ar, _ = B.matrix()
+    A = orc.assemble(ag.TET4, p.conn, p.coords, p.cmats, None, d)
+    bb = np.zeros(A.n)
+    orc.apply_dirichlet(A, bb, p.bc_dofs, p.bc_vals)
+    assert np.abs(ar - A.array).max() <= 1e-13 * np.abs(A.array).max()
+
+
+def test_homogeneous_rhs_returns_zero(ctx):
+    coords, conn = ag.structured_mesh(ag.P1, 6)
+    p = ag.Problem(ag.P1, coords, conn, ag.cauchy_green(1e10, 0.2)[None],
+                   bc_dofs=np.array([0, 1, 3], dtype=np.int64), bc_vals=np.zeros(3))
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=ag.MIXED)
+    assert st["status"] == 0
+    assert np.all(B.solution() == 0.)
+
+
+def test_bad_input_is_an_error(ctx):
+    coords, conn = ag.structured_mesh(ag.P1, 4)
+    bad = conn.copy()
+    bad[0, 0] = 10 ** 6
+    with pytest.raises(ag.AmieError):
+        ag.Batch(ctx, [ag.Problem(ag.P1, coords, bad, ag.cauchy_green(1e10, 0.2)[None])])
+
+
+def test_server_result_same_alone_and_in_mix(ctx):
+    probs = [configs.rve(n, 500 + n, 0.3) for n in (8, 12, 16, 10)]
+    S = ag.Server(ctx, 8 << 30, 8, precision=ag.MIXED, rtol=1e-8)
+    t = S.submit(probs[1], 0.0)
+    res = []
+    while not res:
+        S.step(0.0)
+        res = S.poll()
+    x_alone = S.solution(t, probs[1].ndof * 3)
+    tickets = [S.submit(p, 1.0) for p in probs]
+    got = {}
+    for _ in range(100000):
+        S.step(1.0)
+        for r in S.poll():
+            got[r["ticket"]] = S.solution(r["ticket"], probs[tickets.index(r["ticket"])].ndof * 3)
+        if len(got) == len(probs):
+            break
+    assert len(got) == len(probs)
+    np.testing.assert_array_equal(got[tickets[1]], x_alone)
+    S.close()

@@ -1,0 +1,417 @@
+"""ctypes binding of libamie_gpu.so (include/amie_gpu.h).
+
+The binding is thin: numpy arrays in, numpy arrays out, every call checked.
+There is no fallback: when the HIP library is missing or the GPU is not a
+gfx950, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamie_gpu.so")
+
+P1, P2, TET4 = 1, 2, 3
+FP64, MIXED = 0, 1
+JACOBI, BLOCK_JACOBI = 0, 1
+REFINE_REPLACE, REFINE_RESTART = 0, 1
+NPER = {P1: 3, P2: 6, TET4: 4}
+DIM = {P1: 2, P2: 2, TET4: 3}
+VOIGT = {P1: 3, P2: 3, TET4: 6}
+
+c_i32, c_i64, c_dbl, c_u64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64
+vp = ctypes.c_void_p
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [("elem_type", c_i32), ("nnodes", c_i64), ("nelem", c_i64), ("coords", vp), ("conn", vp),
+                ("dofmap", vp), ("ndofnodes", c_i64), ("nphase", c_i32), ("cmat", vp), ("phase", vp),
+                ("damage", vp), ("nbc", c_i64), ("bc_dofs", vp), ("bc_vals", vp), ("nrhs", c_i32),
+                ("forces", vp), ("macro_strain", vp)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("precision", c_i32), ("precond", c_i32), ("refine", c_i32), ("rtol", c_dbl),
+                ("refine_drop", c_dbl), ("maxit", c_i64), ("use_x0", c_i32), ("use_graph", c_i32),
+                ("chunk", c_i32), ("time_kernels", c_i32)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("status", c_i32), ("nproblems_converged", c_i32), ("iterations_max", c_i64),
+                ("iterations_sum", c_i64), ("refinements", c_i64), ("relres_max", c_dbl), ("solve_ms", c_dbl),
+                ("spmv_kernel_ms", c_dbl), ("spmv_launches", c_i64), ("spmv_bytes", c_dbl)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DamageParams(ctypes.Structure):
+    _fields_ = [("kappa0", c_dbl), ("kappaf", c_dbl), ("dmax", c_dbl)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("ticket", c_u64), ("status", c_i32), ("iterations", c_i32), ("relres", c_dbl),
+                ("submit_ms", c_dbl), ("start_ms", c_dbl), ("done_ms", c_dbl)]
+
+
+# exported symbols and their signatures (restype, argtypes)
+SIGNATURES = {
+    "amie_gpu_create": (c_i32, [vp, c_i32]),
+    "amie_gpu_destroy": (None, [vp]),
+    "amie_gpu_last_error": (ctypes.c_char_p, []),
+    "amie_gpu_device_info": (c_i32, [vp, vp, vp, vp]),
+    "amie_gpu_default_opts": (None, [vp]),
+    "amie_mesh_orient": (c_i32, [c_i32, c_i64, vp, vp]),
+    "amie_mesh_structured_size": (c_i32, [c_i32, c_i32, vp, vp]),
+    "amie_mesh_structured": (c_i32, [c_i32, c_i32, vp, vp]),
+    "amie_mesh_periodic_dofmap": (c_i32, [c_i32, vp]),
+    "amie_inclusions_2d": (c_i32, [c_u64, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, vp, vp, vp]),
+    "amie_phase_by_centroid": (c_i32, [c_i32, c_i64, vp, vp, c_i64, vp, vp]),
+    "amie_damage_field_3d": (c_i32, [c_u64, c_i32, c_dbl, c_dbl, c_dbl, vp]),
+    "amie_cauchy_green": (None, [c_dbl, c_dbl, c_i32, vp]),
+    "amie_gpu_batch_create": (c_i32, [vp, c_i32, vp, vp]),
+    "amie_gpu_batch_from_bcsr": (c_i32, [vp, c_i32, c_i64, vp, vp, vp, vp, vp]),
+    "amie_gpu_batch_destroy": (None, [vp]),
+    "amie_gpu_batch_bytes": (ctypes.c_size_t, [vp]),
+    "amie_gpu_batch_assemble": (c_i32, [vp]),
+    "amie_gpu_batch_solve": (c_i32, [vp, vp, vp]),
+    "amie_gpu_batch_get_solution": (c_i32, [vp, c_i32, vp]),
+    "amie_gpu_batch_set_solution": (c_i32, [vp, c_i32, vp]),
+    "amie_gpu_batch_get_nnzb": (c_i32, [vp, c_i32, vp, vp]),
+    "amie_gpu_batch_get_matrix": (c_i32, [vp, c_i32, vp, vp, vp, vp]),
+    "amie_gpu_batch_get_element_blocks": (c_i32, [vp, c_i32, vp]),
+    "amie_gpu_batch_spmv": (c_i32, [vp, c_i32, c_i32, vp, vp]),
+    "amie_gpu_batch_element_strain": (c_i32, [vp, c_i32, c_i32, vp]),
+    "amie_gpu_batch_average_stress": (c_i32, [vp, vp]),
+    "amie_gpu_batch_damage_step": (c_i32, [vp, vp, vp]),
+    "amie_gpu_batch_get_damage": (c_i32, [vp, c_i32, vp, vp]),
+    "amie_gpu_batch_set_damage": (c_i32, [vp, c_i32, vp]),
+    "amie_gpu_batch_scale_bc": (c_i32, [vp, c_dbl]),
+    "amie_gpu_server_create": (c_i32, [vp, ctypes.c_size_t, c_i32, vp, vp]),
+    "amie_gpu_server_destroy": (None, [vp]),
+    "amie_gpu_server_submit": (c_i32, [vp, vp, c_u64, c_dbl, vp]),
+    "amie_gpu_server_step": (c_i32, [vp, c_dbl, vp, vp]),
+    "amie_gpu_server_poll": (c_i32, [vp, vp, c_i32, vp]),
+    "amie_gpu_server_get_solution": (c_i32, [vp, c_u64, vp, c_i64]),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libamie_gpu.so is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class AmieError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"amie error {code}: {msg}")
+        self.code = code
+
+
+def check(rc, allow=(0,)):
+    if rc not in allow:
+        raise AmieError(rc, lib().amie_gpu_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+# ------------------------------------------------------------ mesh helpers
+def cauchy_green(E, nu, dim=2):
+    s = 3 if dim == 2 else 6
+    C = np.zeros((s, s))
+    lib().amie_cauchy_green(E, nu, dim, _p(C))
+    return C
+
+
+def structured_mesh(etype, n):
+    nn, ne = c_i64(), c_i64()
+    check(lib().amie_mesh_structured_size(etype, n, ctypes.byref(nn), ctypes.byref(ne)))
+    coords = np.zeros((nn.value, DIM[etype]))
+    conn = np.zeros((ne.value, NPER[etype]), dtype=np.int32)
+    check(lib().amie_mesh_structured(etype, n, _p(coords), _p(conn)))
+    return coords, conn
+
+
+def orient(etype, conn, coords):
+    conn = np.array(conn, dtype=np.int32, order="C")
+    coords = _c(coords, np.float64)
+    check(lib().amie_mesh_orient(etype, conn.shape[0], _p(conn), _p(coords)))
+    return conn
+
+
+def periodic_dofmap(n):
+    m = np.zeros((n + 1) * (n + 1), dtype=np.int32)
+    check(lib().amie_mesh_periodic_dofmap(n, _p(m)))
+    return m
+
+
+def inclusions_2d(seed, target, rmin, rmax, gap, max_circles=200000):
+    xyr = np.zeros((max_circles, 3))
+    nc, frac = c_i64(), c_dbl()
+    check(lib().amie_inclusions_2d(seed, target, rmin, rmax, gap, max_circles, _p(xyr), ctypes.byref(nc),
+                                   ctypes.byref(frac)))
+    return xyr[:nc.value].copy(), frac.value
+
+
+def phase_by_centroid(etype, conn, coords, xyr):
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    xyr = _c(xyr, np.float64)
+    ph = np.zeros(conn.shape[0], dtype=np.int32)
+    check(lib().amie_phase_by_centroid(etype, conn.shape[0], _p(conn), _p(coords), xyr.shape[0], _p(xyr), _p(ph)))
+    return ph
+
+
+def damage_field_3d(seed, n, ell, lo, hi):
+    d = np.zeros(6 * n ** 3)
+    check(lib().amie_damage_field_3d(seed, n, ell, lo, hi, _p(d)))
+    return d
+
+
+# --------------------------------------------------------------- problems
+class Problem:
+    """Owns the numpy arrays behind one amie_problem_desc."""
+
+    def __init__(self, etype, coords, conn, cmats, phase=None, damage=None, bc_dofs=None, bc_vals=None,
+                 forces=None, macro=None, dofmap=None, ndofnodes=None, nrhs=1):
+        self.etype = etype
+        self.coords = _c(coords, np.float64)
+        self.conn = _c(conn, np.int32)
+        self.cmats = _c(cmats, np.float64)
+        self.phase = _c(phase, np.int32)
+        self.damage = _c(damage, np.float64)
+        self.bc_dofs = _c(bc_dofs if bc_dofs is not None else np.zeros(0), np.int64)
+        self.bc_vals = _c(bc_vals if bc_vals is not None else np.zeros(0), np.float64)
+        self.forces = _c(forces, np.float64)
+        self.macro = _c(macro, np.float64)
+        self.dofmap = _c(dofmap, np.int32)
+        self.nrhs = nrhs
+        self.ndofnodes = ndofnodes if ndofnodes is not None else (
+            int(self.dofmap.max()) + 1 if self.dofmap is not None else self.coords.shape[0])
+        d = ProblemDesc()
+        d.elem_type = etype
+        d.nnodes = self.coords.shape[0]
+        d.nelem = self.conn.shape[0]
+        d.coords = _p(self.coords)
+        d.conn = _p(self.conn)
+        d.dofmap = _p(self.dofmap)
+        d.ndofnodes = self.ndofnodes
+        d.nphase = self.cmats.shape[0] if self.cmats.ndim == 3 else 1
+        d.cmat = _p(self.cmats)
+        d.phase = _p(self.phase)
+        d.damage = _p(self.damage)
+        d.nbc = len(self.bc_dofs)
+        d.bc_dofs = _p(self.bc_dofs)
+        d.bc_vals = _p(self.bc_vals)
+        d.nrhs = nrhs
+        d.forces = _p(self.forces)
+        d.macro_strain = _p(self.macro)
+        self.desc = d
+
+    @property
+    def ndof(self):
+        return self.ndofnodes * DIM[self.etype]
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = vp()
+        check(lib().amie_gpu_create(ctypes.byref(self.h), device))
+
+    def info(self):
+        f, t, cu = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+        check(lib().amie_gpu_device_info(self.h, ctypes.byref(f), ctypes.byref(t), ctypes.byref(cu)))
+        return f.value, t.value, cu.value
+
+    def close(self):
+        if self.h:
+            lib().amie_gpu_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_opts(**kw):
+    o = SolveOpts()
+    lib().amie_gpu_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Batch:
+    def __init__(self, ctx, problems=None, bcsr=None):
+        self.ctx = ctx
+        self.h = vp()
+        if problems is not None:
+            self.problems = list(problems)
+            arr = (ProblemDesc * len(self.problems))(*[p.desc for p in self.problems])
+            check(lib().amie_gpu_batch_create(ctx.h, len(self.problems), arr, ctypes.byref(self.h)))
+            self.etype = self.problems[0].etype
+            self.nrhs = self.problems[0].nrhs
+            self.b = DIM[self.etype]
+        else:
+            stride, row_size, column_index, array, rhs = bcsr
+            self.problems = None
+            rs = _c(row_size, np.uint32)
+            ci = _c(column_index, np.uint32)
+            ar = _c(array, np.float64)
+            b = _c(rhs, np.float64)
+            check(lib().amie_gpu_batch_from_bcsr(ctx.h, stride, len(rs), _p(rs), _p(ci), _p(ar), _p(b),
+                                                 ctypes.byref(self.h)))
+            self.etype = P1 if stride == 2 else TET4
+            self.nrhs = 1
+            self.b = stride
+
+    def close(self):
+        if self.h:
+            lib().amie_gpu_batch_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nbytes(self):
+        return lib().amie_gpu_batch_bytes(self.h)
+
+    def shape(self, p=0):
+        nr, nz = c_i64(), c_i64()
+        check(lib().amie_gpu_batch_get_nnzb(self.h, p, ctypes.byref(nr), ctypes.byref(nz)))
+        return nr.value, nz.value
+
+    def assemble(self):
+        check(lib().amie_gpu_batch_assemble(self.h))
+
+    def solve(self, **kw):
+        o = default_opts(**kw)
+        st = SolveStats()
+        rc = lib().amie_gpu_batch_solve(self.h, ctypes.byref(o), ctypes.byref(st))
+        check(rc, allow=(0, 1))
+        return st.as_dict()
+
+    def solution(self, p=0):
+        nr, _ = self.shape(p)
+        x = np.zeros(self.nrhs * nr * self.b)
+        check(lib().amie_gpu_batch_get_solution(self.h, p, _p(x)))
+        return x.reshape(self.nrhs, -1) if self.nrhs > 1 else x
+
+    def set_solution(self, x, p=0):
+        x = _c(x, np.float64)
+        check(lib().amie_gpu_batch_set_solution(self.h, p, _p(x)))
+
+    def matrix(self, p=0):
+        nr, nz = self.shape(p)
+        rs = np.zeros(nr, dtype=np.uint32)
+        ci = np.zeros(nz, dtype=np.uint32)
+        cl = self.b + self.b % 2
+        ar = np.zeros(nz * self.b * cl)
+        rhs = np.zeros(self.nrhs * nr * self.b)
+        check(lib().amie_gpu_batch_get_matrix(self.h, p, _p(rs), _p(ci), _p(ar), _p(rhs)))
+        return rs, ci, ar, (rhs.reshape(self.nrhs, -1) if self.nrhs > 1 else rhs)
+
+    def element_blocks(self, p=0):
+        pr = self.problems[p]
+        n = NPER[self.etype]
+        s = np.zeros(pr.conn.shape[0] * n * n, dtype=np.int64)
+        check(lib().amie_gpu_batch_get_element_blocks(self.h, p, _p(s)))
+        return s.reshape(-1, n, n)
+
+    def spmv(self, x, p=0, precision=FP64):
+        x = _c(x, np.float64)
+        y = np.zeros_like(x)
+        check(lib().amie_gpu_batch_spmv(self.h, p, precision, _p(x), _p(y)))
+        return y
+
+    def element_strain(self, p=0, rhs=0):
+        pr = self.problems[p]
+        out = np.zeros((pr.conn.shape[0], VOIGT[self.etype]))
+        check(lib().amie_gpu_batch_element_strain(self.h, p, rhs, _p(out)))
+        return out
+
+    def average_stress(self):
+        out = np.zeros((len(self.problems), self.nrhs, VOIGT[self.etype]))
+        check(lib().amie_gpu_batch_average_stress(self.h, _p(out)))
+        return out
+
+    def damage_step(self, kappa0=1e-4, kappaf=1e-3, dmax=1 - 1e-6):
+        prm = DamageParams(kappa0, kappaf, dmax)
+        g = c_i64()
+        check(lib().amie_gpu_batch_damage_step(self.h, ctypes.byref(prm), ctypes.byref(g)))
+        return g.value
+
+    def damage(self, p=0):
+        ne = self.problems[p].conn.shape[0]
+        d, k = np.zeros(ne), np.zeros(ne)
+        check(lib().amie_gpu_batch_get_damage(self.h, p, _p(d), _p(k)))
+        return d, k
+
+    def scale_bc(self, f):
+        check(lib().amie_gpu_batch_scale_bc(self.h, f))
+
+
+class Server:
+    def __init__(self, ctx, hbm_budget, max_slots, **opts):
+        self.ctx = ctx
+        self.h = vp()
+        self.opts = default_opts(**opts)
+        check(lib().amie_gpu_server_create(ctx.h, hbm_budget, max_slots, ctypes.byref(self.opts),
+                                           ctypes.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, problem, t_ms, key=0):
+        t = c_u64()
+        check(lib().amie_gpu_server_submit(self.h, ctypes.byref(problem.desc), key, t_ms, ctypes.byref(t)))
+        return t.value
+
+    def step(self, now_ms):
+        na, nq = c_i32(), c_i32()
+        check(lib().amie_gpu_server_step(self.h, now_ms, ctypes.byref(na), ctypes.byref(nq)))
+        return na.value, nq.value
+
+    def poll(self, maxn=4096):
+        out = (RequestResult * maxn)()
+        n = c_i32()
+        check(lib().amie_gpu_server_poll(self.h, out, maxn, ctypes.byref(n)))
+        return [{k: getattr(out[i], k) for k, _ in RequestResult._fields_} for i in range(n.value)]
+
+    def solution(self, ticket, n):
+        x = np.zeros(n)
+        check(lib().amie_gpu_server_get_solution(self.h, ticket, _p(x), n))
+        return x
+
+    def close(self):
+        if self.h:
+            lib().amie_gpu_server_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

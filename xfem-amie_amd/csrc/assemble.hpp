@@ -1,0 +1,553 @@
+// Element integration, coloured scatter, Dirichlet elimination, scaling,
+// strain / damage kernels (device side).
+//
+// Element matrices follow the reference quadrature exactly in structure:
+//   Jinv once at the element centroid (mesher/delaunay.cpp:2436-2447,
+//   mesher/delaunay_3d.cpp:2478), Gauss rules of elements/elements.cpp:676-689
+//   and :2399-2402 with weights divided by det(Jinv) (:939-945), B rows of
+//   polynomial/vm_base.cpp:2188-2256, K_ij = sum_g w_g B_i C B_j^T with the
+//   first Gauss point as a plain product scaled by w and later points added
+//   with Kahan compensation (polynomial/vm_base.cpp:2974-2989,
+//   utilities/matrixops.h:462-507), K_ji = K_ij^T (mesher/delaunay.cpp:2457-2468).
+// One thread integrates one element and adds its blocks straight into the
+// FP64 matrix; elements of one colour share no DOF node, so the adds need no
+// atomics and their order per block is fixed by the colour order.
+#pragma once
+#include "device.hpp"
+
+namespace amie {
+
+struct AsmArgs {
+    const double *coords;     // geometry coordinates, dim per node
+    const int32_t *econn;     // element -> geometry node (global)
+    const int32_t *erow;      // element node -> internal block row (global)
+    const uint32_t *eslot;    // element (i,j) -> SELL slot (global)
+    const int32_t *ephase;
+    const double *edamage;    // nullable
+    const int32_t *eprob;
+    const int32_t *cmat_off;  // per problem: first C matrix
+    const double *cmat;
+    const double *macro;      // per problem nrhs x s (nullable)
+    double *v64;
+    int64_t nslots;
+    double *b64;
+};
+
+template <int ET> struct Elem;
+template <> struct Elem<1> { static constexpr int NPER = 3, DIM = 2, B = 2, S = 3, NGP = 1; };
+template <> struct Elem<2> { static constexpr int NPER = 6, DIM = 2, B = 2, S = 3, NGP = 4; };
+template <> struct Elem<3> { static constexpr int NPER = 4, DIM = 3, B = 3, S = 6, NGP = 1; };
+
+// shape-function derivatives (see oracle: elements/elements.cpp:976-1061,2491-2530)
+template <int ET> __device__ inline void dshape(double x, double y, double *dxi, double *deta, double *dzeta)
+{
+    if constexpr (ET == 1) {
+        dxi[0] = 0.; deta[0] = 1.;
+        dxi[1] = -1.; deta[1] = -1.;
+        dxi[2] = 1.; deta[2] = 0.;
+    } else if constexpr (ET == 2) {
+        dxi[0] = 0.;                     deta[0] = 4. * y - 1.;
+        dxi[1] = y * -4.;                deta[1] = (4. - x * 4.) - y * 8.;
+        dxi[2] = (y * 4. + x * 4.) - 3.; deta[2] = (x * 4. + y * 4.) - 3.;
+        dxi[3] = (4. - 8. * x) - y * 4.; deta[3] = x * -4.;
+        dxi[4] = 4. * x - 1.;            deta[4] = 0.;
+        dxi[5] = 4. * y;                 deta[5] = 4. * x;
+    } else {
+        dxi[0] = 0.; deta[0] = 0.; dzeta[0] = 1.;
+        dxi[1] = -1.; deta[1] = -1.; dzeta[1] = -1.;
+        dxi[2] = 1.; deta[2] = 0.; dzeta[2] = 0.;
+        dxi[3] = 0.; deta[3] = 1.; dzeta[3] = 0.;
+    }
+}
+
+// Jinv at the centroid (utilities/matrixops.cpp:536-561, :681-702) and its
+// determinant (:826-847).
+template <int ET> __device__ inline double centroid_jinv(const double (*X)[3], double *m)
+{
+    using E = Elem<ET>;
+    double dxi[6], deta[6], dzeta[6];
+    if constexpr (E::DIM == 2) {
+        dshape<ET>(1. / 3., 1. / 3., dxi, deta, dzeta);
+        double J[4] = {0., 0., 0., 0.};
+        for (int i = 0; i < E::NPER; i++) {
+            J[0] += dxi[i] * X[i][0]; J[1] += dxi[i] * X[i][1];
+            J[2] += deta[i] * X[i][0]; J[3] += deta[i] * X[i][1];
+        }
+        if (fabs(J[0]) < 1e-24) {
+            double dt = 1. / (J[0] * J[3] - J[1] * J[2]);
+            m[0] = J[3] * dt; m[1] = -J[1] * dt; m[2] = -J[2] * dt; m[3] = J[0] * dt;
+        } else {
+            double r1 = 1. / J[0], r3 = r1 * J[1], r6 = 1. / (J[2] * r3 - J[3]);
+            m[1] = r3 * r6;
+            m[2] = r6 * J[2] * r1;
+            m[0] = r1 - r3 * m[2];
+            m[3] = -r6;
+        }
+        return fma(m[0], m[3], -m[1] * m[2]);
+    } else {
+        dshape<ET>(.25, .25, dxi, deta, dzeta);
+        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < E::NPER; i++)
+            for (int d = 0; d < 3; d++) {
+                J[d] += dxi[i] * X[i][d]; J[3 + d] += deta[i] * X[i][d]; J[6 + d] += dzeta[i] * X[i][d];
+            }
+        double r11 = fma(J[4], J[8], -J[5] * J[7]);
+        double r21 = fma(J[5], J[6], -J[3] * J[8]);
+        double r31 = fma(J[3], J[7], -J[4] * J[6]);
+        double det = 1. / fma(J[0], r11, fma(J[1], r21, J[2] * r31));
+        double v[9] = {r11, fma(J[2], J[7], -J[1] * J[8]), fma(J[1], J[5], -J[2] * J[4]),
+                       r21, fma(J[0], J[8], -J[2] * J[6]), fma(J[2], J[3], -J[0] * J[5]),
+                       r31, fma(J[1], J[6], -J[0] * J[7]), fma(J[4], J[0], -J[1] * J[3])};
+        for (int i = 0; i < 9; i++) m[i] = v[i] * det;
+        return fma(m[0], m[4] * m[8],
+                   fma(m[5], m[6] * m[1], fma(m[2], m[7] * m[3], fma(-m[0], m[5] * m[7], fma(-m[1], m[3] * m[8], -m[6] * m[4] * m[2])))));
+    }
+}
+
+template <int ET> __device__ inline void gauss(int g, double &x, double &y, double &w)
+{
+    if constexpr (ET == 1) { x = 0.333333333333333; y = x; w = 0.5; }
+    else if constexpr (ET == 2) {
+        const double P[4][2] = {{0.2, 0.2}, {0.6, 0.2}, {0.2, 0.6}, {0.33333333333333, 0.33333333333333}};
+        x = P[g][0]; y = P[g][1];
+        w = g < 3 ? 0.260416666666667 : -0.28125;
+    } else { x = .25; y = .25; w = 0.1666666666666667; }
+}
+
+// Physical gradients G[g][node][dim] of the shape functions at every Gauss
+// point (with the centroid Jinv) and the Gauss weights w[g].
+template <int ET> __device__ inline void element_grads(const double (*X)[3], double (*G)[6][3], double *w)
+{
+    using E = Elem<ET>;
+    double m[9];
+    double detJinv = centroid_jinv<ET>(X, m);
+    for (int g = 0; g < E::NGP; g++) {
+        double gx, gy, gw, dxi[6], deta[6], dzeta[6];
+        gauss<ET>(g, gx, gy, gw);
+        dshape<ET>(gx, gy, dxi, deta, dzeta);
+        w[g] = gw / detJinv;
+        for (int i = 0; i < E::NPER; i++) {
+            if constexpr (E::DIM == 2) {
+                G[g][i][0] = dxi[i] * m[0] + deta[i] * m[1];
+                G[g][i][1] = dxi[i] * m[2] + deta[i] * m[3];
+            } else {
+                G[g][i][0] = dxi[i] * m[0] + deta[i] * m[1] + dzeta[i] * m[2];
+                G[g][i][1] = dxi[i] * m[3] + deta[i] * m[4] + dzeta[i] * m[5];
+                G[g][i][2] = dxi[i] * m[6] + deta[i] * m[7] + dzeta[i] * m[8];
+            }
+        }
+    }
+}
+
+// B_i C B_j^T for one Gauss point; out row-major b x b.
+template <int ET> __device__ inline void btcb(const double *gi, const double *gj, const double *C, double *out)
+{
+    using E = Elem<ET>;
+    constexpr int S = E::S, B = E::B;
+    double Bi[B][S], Bj[S][B];
+    for (int r = 0; r < B; r++) for (int c = 0; c < S; c++) Bi[r][c] = 0.;
+    for (int r = 0; r < S; r++) for (int c = 0; c < B; c++) Bj[r][c] = 0.;
+    if constexpr (B == 2) {
+        Bi[0][0] = gi[0]; Bi[0][2] = gi[1]; Bi[1][1] = gi[1]; Bi[1][2] = gi[0];
+        Bj[0][0] = gj[0]; Bj[1][1] = gj[1]; Bj[2][0] = gj[1]; Bj[2][1] = gj[0];
+    } else {
+        Bi[0][0] = gi[0]; Bi[1][1] = gi[1]; Bi[2][2] = gi[2];
+        Bi[1][3] = gi[2]; Bi[2][3] = gi[1]; Bi[0][4] = gi[2]; Bi[2][4] = gi[0]; Bi[0][5] = gi[1]; Bi[1][5] = gi[0];
+        Bj[0][0] = gj[0]; Bj[1][1] = gj[1]; Bj[2][2] = gj[2];
+        Bj[3][1] = gj[2]; Bj[3][2] = gj[1]; Bj[4][0] = gj[2]; Bj[4][2] = gj[0]; Bj[5][0] = gj[1]; Bj[5][1] = gj[0];
+    }
+    for (int r = 0; r < B; r++)
+        for (int k = 0; k < B; k++) out[r * B + k] = 0.;
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < S; c++) {
+            double rij = 0.;
+            for (int l = 0; l < S; l++) rij = rij + Bi[r][l] * C[l * S + c];
+            for (int k = 0; k < B; k++) out[r * B + k] += rij * Bj[c][k];
+        }
+}
+
+template <int B> __device__ inline void add_block64(double *v64, int64_t ns, int64_t slot, const double *blk /*row-major*/)
+{
+    // planes of double2 (entry (r,c) at c*B + r), remainder plane of double
+    double a[B * B];
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < B; c++) a[c * B + r] = blk[r * B + c];
+    double2 *p = reinterpret_cast<double2 *>(v64);
+    constexpr int NP = (B * B) / 2;
+    for (int k = 0; k < NP; k++) {
+        double2 v = p[(int64_t)k * ns + slot];
+        v.x += a[2 * k]; v.y += a[2 * k + 1];
+        p[(int64_t)k * ns + slot] = v;
+    }
+    if constexpr ((B * B) % 2) {
+        double *rem = reinterpret_cast<double *>(p + (int64_t)NP * ns);
+        rem[slot] += a[B * B - 1];
+    }
+}
+
+template <int ET, int NR>
+__global__ __launch_bounds__(256) void k_assemble(AsmArgs a, const int32_t *elems, int n, int S_voigt)
+{
+    using E = Elem<ET>;
+    constexpr int NP = E::NPER, B = E::B, S = E::S, D = E::DIM;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int e = elems[gid];
+    const int p = a.eprob[e];
+    double X[NP][3];
+    for (int i = 0; i < NP; i++) {
+        const int nd = a.econn[(int64_t)e * NP + i];
+        for (int d = 0; d < D; d++) X[i][d] = a.coords[(int64_t)nd * D + d];
+        if (D == 2) X[i][2] = 0.;
+    }
+    double C[S * S];
+    const double *Cp = a.cmat + (int64_t)(a.cmat_off[p] + a.ephase[e]) * S * S;
+    const double f = a.edamage ? 1. - a.edamage[e] : 1.;
+    for (int k = 0; k < S * S; k++) C[k] = a.edamage ? Cp[k] * f : Cp[k];
+    double G[E::NGP][6][3], w[E::NGP];
+    element_grads<ET>(X, G, w);
+    // macro-strain displacement of every node, per right-hand side
+    double U[NR][NP][B];
+    const bool macro = a.macro != nullptr;
+    if (macro) {
+        for (int k = 0; k < NR; k++) {
+            const double *m = a.macro + ((int64_t)p * NR + k) * S;
+            for (int i = 0; i < NP; i++) {
+                if constexpr (D == 2) {
+                    U[k][i][0] = m[0] * X[i][0] + 0.5 * m[2] * X[i][1];
+                    U[k][i][1] = 0.5 * m[2] * X[i][0] + m[1] * X[i][1];
+                } else {
+                    // Voigt [xx, yy, zz, yz, xz, xy]
+                    U[k][i][0] = m[0] * X[i][0] + 0.5 * m[5] * X[i][1] + 0.5 * m[4] * X[i][2];
+                    U[k][i][1] = 0.5 * m[5] * X[i][0] + m[1] * X[i][1] + 0.5 * m[3] * X[i][2];
+                    U[k][i][2] = 0.5 * m[4] * X[i][0] + 0.5 * m[3] * X[i][1] + m[2] * X[i][2];
+                }
+            }
+        }
+    }
+    double F[NR][NP][B];
+    for (int k = 0; k < NR; k++)
+        for (int i = 0; i < NP; i++)
+            for (int c = 0; c < B; c++) F[k][i][c] = 0.;
+    const uint32_t *slots = a.eslot + (int64_t)e * NP * NP;
+    for (int i = 0; i < NP; i++) {
+        for (int j = 0; j <= i; j++) {
+            double K[B * B], t[B * B], comp[B * B];
+            btcb<ET>(G[0][i], G[0][j], C, K);
+            for (int q = 0; q < B * B; q++) { K[q] *= w[0]; comp[q] = 0.; }
+            for (int g = 1; g < E::NGP; g++) {
+                btcb<ET>(G[g][i], G[g][j], C, t);
+                for (int q = 0; q < B * B; q++) {
+                    double yk = t[q] * w[g] - comp[q];
+                    double tt = K[q] + yk;
+                    comp[q] = (tt - K[q]) - yk;
+                    K[q] = tt;
+                }
+            }
+            add_block64<B>(a.v64, a.nslots, slots[i * NP + j], K);
+            double KT[B * B];
+            for (int r = 0; r < B; r++)
+                for (int c = 0; c < B; c++) KT[r * B + c] = K[c * B + r];
+            if (j != i) add_block64<B>(a.v64, a.nslots, slots[j * NP + i], KT);
+            if (macro) {
+                for (int k = 0; k < NR; k++)
+                    for (int r = 0; r < B; r++)
+                        for (int c = 0; c < B; c++) {
+                            F[k][i][r] -= K[r * B + c] * U[k][j][c];
+                            if (j != i) F[k][j][r] -= KT[r * B + c] * U[k][i][c];
+                        }
+            }
+        }
+    }
+    if (macro) {
+        for (int i = 0; i < NP; i++) {
+            const int64_t row = a.erow[(int64_t)e * NP + i];
+            for (int k = 0; k < NR; k++)
+                for (int c = 0; c < B; c++) a.b64[(row * NR + k) * B + c] += F[k][i][c];
+        }
+    }
+}
+
+// ------------------------------------------------- Dirichlet elimination
+// solvers/assembly.cpp:125-253 restated per block row: for a free row i,
+// b_i -= g_j A(i,j) and A(i,j) = 0 for every constrained column j (blocks in
+// column order); a constrained row becomes a unit row with b_i = g_i.
+template <int B, int NR>
+__global__ __launch_bounds__(WG) void k_apply_bc(SolveArgs a, double *v64, double *b64, const uint8_t *bcmask,
+                                                 const double *bcval, int ntiles)
+{
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (wave >= a.tile_nslices[t]) continue;
+        const int s = a.tile_slice0[t] + wave;
+        if (lane >= a.slice_rows[s]) continue;
+        const int64_t row = (int64_t)s * WAVE + lane;
+        bool rfix[B];
+        bool any_r = false;
+        for (int i = 0; i < B; i++) { rfix[i] = bcmask[row * B + i]; any_r |= rfix[i]; }
+        const int w = a.slice_width[s];
+        const int64_t sp = a.slice_ptr[s] + lane;
+        for (int k = 0; k < w; k++) {
+            const int64_t slot = sp + (int64_t)k * WAVE;
+            const uint32_t c = a.col[slot];
+            double blk[B * B];
+            Blk<double, B>::load(v64, a.nslots, slot, blk);
+            bool mod = false;
+            for (int j = 0; j < B; j++) {
+                const int64_t cd = (int64_t)c * B + j;
+                if (!bcmask[cd]) continue;
+                const double g = bcval[cd];
+                for (int i = 0; i < B; i++) {
+                    if (rfix[i]) continue;
+                    for (int r = 0; r < NR; r++) b64[(row * NR + r) * B + i] -= g * blk[j * B + i];
+                    blk[j * B + i] = 0.;
+                    mod = true;
+                }
+            }
+            if (any_r) {
+                for (int i = 0; i < B; i++) {
+                    if (!rfix[i]) continue;
+                    for (int j = 0; j < B; j++) blk[j * B + i] = ((int64_t)c * B + j == row * B + i) ? 1. : 0.;
+                }
+                mod = true;
+            }
+            if (mod) Blk<double, B>::store(v64, a.nslots, slot, blk);
+        }
+        for (int i = 0; i < B; i++)
+            if (rfix[i])
+                for (int r = 0; r < NR; r++) b64[(row * NR + r) * B + i] = bcval[row * B + i];
+    }
+}
+
+// S = diag(K)^-1/2 per dof (1 where the diagonal is not positive).
+template <int B>
+__global__ __launch_bounds__(WG) void k_diag_scale(SolveArgs a, const int64_t *dslot, double *S, int ntiles)
+{
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (wave >= a.tile_nslices[t]) continue;
+        const int s = a.tile_slice0[t] + wave;
+        const int64_t row = (int64_t)s * WAVE + lane;
+        if (lane >= a.slice_rows[s]) {
+            for (int i = 0; i < B; i++) S[row * B + i] = 1.;
+            continue;
+        }
+        double blk[B * B];
+        Blk<double, B>::load(a.v64, a.nslots, dslot[row], blk);
+        for (int i = 0; i < B; i++) {
+            double d = blk[i * B + i];
+            S[row * B + i] = d > 0. ? 1. / sqrt(d) : 1.;
+        }
+    }
+}
+
+// Scaled working matrix Ks = S K S in W, and the inverse of each scaled
+// diagonal block (block Jacobi).
+template <typename W, int B>
+__global__ __launch_bounds__(WG) void k_scale_cast(SolveArgs a, void *vw, const int64_t *dslot, W *minv, int ntiles)
+{
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (wave >= a.tile_nslices[t]) continue;
+        const int s = a.tile_slice0[t] + wave;
+        const int64_t row = (int64_t)s * WAVE + lane;
+        const bool valid = lane < a.slice_rows[s];
+        double sr[B];
+        for (int i = 0; i < B; i++) sr[i] = a.S[row * B + i];
+        const int w = a.slice_width[s];
+        const int64_t sp = a.slice_ptr[s] + lane;
+        for (int k = 0; k < w; k++) {
+            const int64_t slot = sp + (int64_t)k * WAVE;
+            const uint32_t c = a.col[slot];
+            double blk[B * B];
+            Blk<double, B>::load(a.v64, a.nslots, slot, blk);
+            W o[B * B];
+            for (int j = 0; j < B; j++) {
+                const double sc = a.S[(int64_t)c * B + j];
+                for (int i = 0; i < B; i++) o[j * B + i] = valid ? (W)(sr[i] * blk[j * B + i] * sc) : (W)0;
+            }
+            if (vw) Blk<W, B>::store(vw, a.nslots, slot, o);
+        }
+        if (!valid || !minv) continue;
+        double d[B * B];
+        Blk<double, B>::load(a.v64, a.nslots, dslot[row], d);
+        double m[B * B]; // row-major scaled diagonal block
+        for (int i = 0; i < B; i++)
+            for (int j = 0; j < B; j++) m[i * B + j] = sr[i] * d[j * B + i] * sr[j];
+        double inv[B * B];
+        if constexpr (B == 2) {
+            double dt = 1. / (m[0] * m[3] - m[1] * m[2]);
+            inv[0] = m[3] * dt; inv[1] = -m[1] * dt; inv[2] = -m[2] * dt; inv[3] = m[0] * dt;
+        } else {
+            double c0 = m[4] * m[8] - m[5] * m[7], c1 = m[2] * m[7] - m[1] * m[8], c2 = m[1] * m[5] - m[2] * m[4];
+            double c3 = m[5] * m[6] - m[3] * m[8], c4 = m[0] * m[8] - m[2] * m[6], c5 = m[2] * m[3] - m[0] * m[5];
+            double c6 = m[3] * m[7] - m[4] * m[6], c7 = m[1] * m[6] - m[0] * m[7], c8 = m[0] * m[4] - m[1] * m[3];
+            double dt = 1. / (m[0] * c0 + m[1] * c3 + m[2] * c6);
+            inv[0] = c0 * dt; inv[1] = c1 * dt; inv[2] = c2 * dt; inv[3] = c3 * dt; inv[4] = c4 * dt;
+            inv[5] = c5 * dt; inv[6] = c6 * dt; inv[7] = c7 * dt; inv[8] = c8 * dt;
+        }
+        for (int q = 0; q < B * B; q++) minv[row * B * B + q] = (W)inv[q];
+    }
+}
+
+// ------------------------------------------------------ strain / damage
+struct StrainArgs {
+    const double *coords;
+    const int32_t *econn;
+    const int32_t *erow;
+    const int32_t *eprob;
+    const double *macro;      // nullable
+    const double *x64;
+    int nrhs;
+};
+
+// Centroid strain, elements/integrable_entity.cpp:977-1097 (Voigt,
+// engineering shear), displacement = macro part + solved part.
+template <int ET>
+__device__ inline void element_strain(const StrainArgs &a, int e, int k, double *eps)
+{
+    using E = Elem<ET>;
+    constexpr int NP = E::NPER, D = E::DIM, B = E::B;
+    double X[NP][3];
+    for (int i = 0; i < NP; i++) {
+        const int nd = a.econn[(int64_t)e * NP + i];
+        for (int d = 0; d < D; d++) X[i][d] = a.coords[(int64_t)nd * D + d];
+        if (D == 2) X[i][2] = 0.;
+    }
+    double m[9];
+    centroid_jinv<ET>(X, m);
+    double dxi[6], deta[6], dzeta[6];
+    if constexpr (D == 2) dshape<ET>(1. / 3., 1. / 3., dxi, deta, dzeta);
+    else dshape<ET>(.25, .25, dxi, deta, dzeta);
+    double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}; // [component][reference direction]
+    for (int j = 0; j < NP; j++) {
+        const int64_t row = a.erow[(int64_t)e * NP + j];
+        for (int c = 0; c < B; c++) {
+            double u = a.x64[(row * a.nrhs + k) * B + c];
+            g[c][0] += dxi[j] * u; g[c][1] += deta[j] * u;
+            if (D == 3) g[c][2] += dzeta[j] * u;
+        }
+    }
+    const double *mac = a.macro ? a.macro + ((int64_t)a.eprob[e] * a.nrhs + k) * E::S : nullptr;
+    if constexpr (D == 2) {
+        eps[0] = g[0][0] * m[0] + g[0][1] * m[1];
+        eps[1] = g[1][0] * m[2] + g[1][1] * m[3];
+        eps[2] = g[0][0] * m[2] + g[0][1] * m[3] + g[1][0] * m[0] + g[1][1] * m[1];
+        if (mac) for (int q = 0; q < 3; q++) eps[q] += mac[q];
+    } else {
+        auto d = [&](int c, int r) { return g[c][0] * m[r * 3] + g[c][1] * m[r * 3 + 1] + g[c][2] * m[r * 3 + 2]; };
+        eps[0] = d(0, 0); eps[1] = d(1, 1); eps[2] = d(2, 2);
+        eps[3] = d(1, 2) + d(2, 1); eps[4] = d(0, 2) + d(2, 0); eps[5] = d(1, 0) + d(0, 1);
+        if (mac) for (int q = 0; q < 6; q++) eps[q] += mac[q];
+    }
+}
+
+template <int ET>
+__global__ __launch_bounds__(256) void k_strain(StrainArgs a, int e0, int n, int k, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    element_strain<ET>(a, e0 + i, k, out + (int64_t)i * Elem<ET>::S);
+}
+
+// Principal strains, Mazars equivalent strain (tension branch,
+// physics/fracturecriteria/mazars.cpp:124), kappa history and the damage law
+// d = 1 - (k0/kappa) exp(-(kappa - k0)/kf) for kappa > k0, combined with the
+// initial damage as max(d0, d_law), never decreasing, capped at dmax.
+__device__ inline double mazars_equivalent(int dim, const double *s)
+{
+    double ev[3];
+    if (dim == 2) {
+        double a = s[0], b = s[1], c = 0.5 * s[2];
+        double mm = 0.5 * (a + b), r = sqrt(0.25 * (a - b) * (a - b) + c * c);
+        ev[0] = mm + r; ev[1] = mm - r; ev[2] = 0.;
+    } else {
+        double a11 = s[0], a22 = s[1], a33 = s[2], a23 = 0.5 * s[3], a13 = 0.5 * s[4], a12 = 0.5 * s[5];
+        double p1 = a12 * a12 + a13 * a13 + a23 * a23;
+        double q = (a11 + a22 + a33) / 3.;
+        if (p1 <= 1e-300) { ev[0] = a11; ev[1] = a22; ev[2] = a33; }
+        else {
+            double p2 = (a11 - q) * (a11 - q) + (a22 - q) * (a22 - q) + (a33 - q) * (a33 - q) + 2. * p1;
+            double p = sqrt(p2 / 6.);
+            double b11 = (a11 - q) / p, b22 = (a22 - q) / p, b33 = (a33 - q) / p;
+            double b12 = a12 / p, b13 = a13 / p, b23 = a23 / p;
+            double detB = b11 * (b22 * b33 - b23 * b23) - b12 * (b12 * b33 - b23 * b13) + b13 * (b12 * b23 - b22 * b13);
+            double rr = fmin(1., fmax(-1., detB / 2.));
+            double phi = acos(rr) / 3.;
+            ev[0] = q + 2. * p * cos(phi);
+            ev[2] = q + 2. * p * cos(phi + 2. * 3.14159265358979323846 / 3.);
+            ev[1] = 3. * q - ev[0] - ev[2];
+        }
+    }
+    double eq = 0.;
+    for (int i = 0; i < 3; i++) { double t = fmax(0., ev[i]); eq += t * t; }
+    return sqrt(eq);
+}
+
+template <int ET>
+__global__ __launch_bounds__(256) void k_damage(StrainArgs a, int n, const double *d0, double *kappa, double *d,
+                                               double k0, double kf, double dmax, unsigned long long *grown)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double eps[6];
+    element_strain<ET>(a, e, 0, eps);
+    double eq = mazars_equivalent(Elem<ET>::DIM, eps);
+    double kap = fmax(kappa[e], eq);
+    kappa[e] = kap;
+    double dl = kap > k0 ? 1. - (k0 / kap) * exp(-(kap - k0) / kf) : 0.;
+    double dn = fmin(dmax, fmax(d0 ? d0[e] : 0., fmax(d[e], dl)));
+    if (dn > d[e]) atomicAdd(grown, 1ull);
+    d[e] = dn;
+}
+
+// Volume average of C_e (1 - d_e) eps_e per problem and right-hand side:
+// one workgroup per problem, fixed summation order.
+template <int ET>
+__global__ __launch_bounds__(256) void k_average_stress(StrainArgs a, const int32_t *prob_e0, const int32_t *prob_ne,
+                                                        const int32_t *ephase, const double *edamage,
+                                                        const int32_t *cmat_off, const double *cmat, double *out)
+{
+    using E = Elem<ET>;
+    constexpr int S = E::S, NP = E::NPER, D = E::DIM;
+    __shared__ double lds[4][MAXQ];
+    const int p = blockIdx.x;
+    const int e0 = prob_e0[p], ne = prob_ne[p];
+    for (int k = 0; k < a.nrhs; k++) {
+        double acc[S + 1];
+        for (int q = 0; q <= S; q++) acc[q] = 0.;
+        for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+            const int e = e0 + i;
+            double eps[6];
+            element_strain<ET>(a, e, k, eps);
+            double X[NP][3];
+            for (int j = 0; j < NP; j++) {
+                const int nd = a.econn[(int64_t)e * NP + j];
+                for (int d = 0; d < D; d++) X[j][d] = a.coords[(int64_t)nd * D + d];
+            }
+            double meas;
+            if constexpr (D == 2) {
+                const int v1 = NP == 6 ? 2 : 1, v2 = NP == 6 ? 4 : 2;
+                meas = 0.5 * fabs((X[v1][0] - X[0][0]) * (X[v2][1] - X[0][1]) - (X[v1][1] - X[0][1]) * (X[v2][0] - X[0][0]));
+            } else {
+                double u[3], v[3], w[3];
+                for (int d = 0; d < 3; d++) { u[d] = X[1][d] - X[0][d]; v[d] = X[2][d] - X[0][d]; w[d] = X[3][d] - X[0][d]; }
+                meas = fabs(u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) + u[2] * (v[0] * w[1] - v[1] * w[0])) / 6.;
+            }
+            const double *C = cmat + (int64_t)(cmat_off[p] + ephase[e]) * S * S;
+            const double f = edamage ? 1. - edamage[e] : 1.;
+            for (int r = 0; r < S; r++) {
+                double t = 0.;
+                for (int c = 0; c < S; c++) t += C[r * S + c] * eps[c];
+                acc[r] += meas * f * t;
+            }
+            acc[S] += meas;
+        }
+        wg_sum<S + 1>(acc, lds);
+        if (threadIdx.x == 0)
+            for (int r = 0; r < S; r++) out[((int64_t)p * a.nrhs + k) * S + r] = acc[r] / acc[S];
+        __syncthreads();
+    }
+}
+
+} // namespace amie

@@ -1,0 +1,1201 @@
+// Batch construction, device assembly and the solve loop (host side), plus
+// the C ABI entry points that operate on batches.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "assemble.hpp"
+#include "layout.hpp"
+
+namespace amie {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+bool hip_ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    set_error("%s failed: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+const char *last_error() { return g_err.c_str(); }
+
+bool DevBuf::alloc(size_t bytes)
+{
+    release();
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        p = nullptr;
+        set_error("hipMalloc of %zu bytes failed", bytes);
+        return false;
+    }
+    n = bytes;
+    return true;
+}
+
+EType etype_info(int et)
+{
+    if (et == AMIE_P1) return {1, 3, 2, 2, 3};
+    if (et == AMIE_P2) return {2, 6, 2, 2, 3};
+    if (et == AMIE_TET4) return {3, 4, 3, 3, 6};
+    return {0, 0, 0, 0, 0};
+}
+
+static size_t slot_bytes(bool fp64, int b)
+{
+    if (b == 2) return fp64 ? Blk<double, 2>::bytes_per_slot : Blk<float, 2>::bytes_per_slot;
+    return fp64 ? Blk<double, 3>::bytes_per_slot : Blk<float, 3>::bytes_per_slot;
+}
+
+Batch::~Batch()
+{
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (h_prob_pinned) (void)hipHostFree(h_prob_pinned);
+    if (h_nactive_pinned) (void)hipHostFree(h_nactive_pinned);
+}
+
+size_t Batch::bytes() const
+{
+    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &v64, &v32, &tile_slice0,
+                           &tile_nslices, &tile_prob, &active, &n_active, &prob, &ticket, &partial, &x64, &b64,
+                           &fext, &S, &y, &r, &z, &p0, &p1, &q, &minv32, &minv64, &bcmask, &bcval, &coords,
+                           &econn, &erow, &eslot, &ephase, &edamage, &ekappa, &ed0, &eprob, &cmat, &cmat_off,
+                           &macro, &colour_elems, &prob_e0, &prob_ne, &grown};
+    size_t s = 0;
+    for (auto b : all) s += b->n;
+    return s;
+}
+
+SolveArgs Batch::args(bool mixed) const
+{
+    SolveArgs a{};
+    a.slice_ptr = slice_ptr.as<int64_t>();
+    a.slice_width = slice_width.as<int32_t>();
+    a.slice_rows = slice_rows.as<int32_t>();
+    a.col = col.as<uint32_t>();
+    a.nslots = nslots;
+    a.v64 = v64.as<double>();
+    a.vw = mixed ? v32.p : v64.p;
+    a.tile_slice0 = tile_slice0.as<int32_t>();
+    a.tile_nslices = tile_nslices.as<int32_t>();
+    a.tile_prob = tile_prob.as<int32_t>();
+    a.active = active.as<int32_t>();
+    a.n_active = n_active.as<int32_t>();
+    a.prob = prob.as<ProbDev>();
+    a.ticket = ticket.as<unsigned>();
+    a.partial = partial.as<double>();
+    a.x64 = x64.as<double>();
+    a.b64 = b64.as<double>();
+    a.S = S.as<double>();
+    a.y = y.p;
+    a.r = r.p;
+    a.z = z.p;
+    a.p0 = p0.p;
+    a.p1 = p1.p;
+    a.q = q.p;
+    a.minv = mixed ? minv32.p : minv64.p;
+    return a;
+}
+
+int Batch::grid() const
+{
+    // a multiple of the 8 XCDs, at most 8 workgroups per CU
+    int g = std::max(1, ctx->num_cu) * 8;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
+}
+
+int Batch::set_active_all()
+{
+    h_active.resize(ntiles);
+    std::iota(h_active.begin(), h_active.end(), 0);
+    *h_nactive_pinned = (int32_t)ntiles;
+    AMIE_HIP(hipMemcpyAsync(active.p, h_active.data(), ntiles * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    AMIE_HIP(hipStreamSynchronize(stream));
+    return AMIE_OK;
+}
+
+// ------------------------------------------------------------ batch build
+struct Tmp {
+    std::vector<int32_t> edof, perm, width, colour;
+    std::vector<int64_t> row_start;
+    int ncol = 0;
+};
+
+static int validate(const amie_problem_desc &d, const EType &et)
+{
+    if (!et.et) { set_error("unknown element type %d", d.elem_type); return AMIE_ERR_ARG; }
+    if (d.nnodes <= 0 || d.nelem < 0) { set_error("empty mesh"); return AMIE_ERR_ARG; }
+    if (!d.coords || (!d.conn && d.nelem)) { set_error("coords/conn missing"); return AMIE_ERR_ARG; }
+    if (d.nphase <= 0 || !d.cmat) { set_error("material table missing"); return AMIE_ERR_ARG; }
+    if (d.nrhs != 1 && d.nrhs != 3) { set_error("nrhs must be 1 or 3 (got %d)", d.nrhs); return AMIE_ERR_ARG; }
+    const int64_t nd = d.dofmap ? d.ndofnodes : d.nnodes;
+    if (nd <= 0 || nd > (int64_t)INT32_MAX) { set_error("bad DOF node count"); return AMIE_ERR_ARG; }
+    for (int64_t e = 0; e < d.nelem * et.nper; e++) {
+        const int32_t c = d.conn[e];
+        if (c < 0 || c >= d.nnodes) { set_error("connectivity entry %lld out of range", (long long)e); return AMIE_ERR_ARG; }
+        if (d.dofmap && (d.dofmap[c] < 0 || d.dofmap[c] >= nd)) { set_error("dofmap out of range"); return AMIE_ERR_ARG; }
+    }
+    if (d.phase)
+        for (int64_t e = 0; e < d.nelem; e++)
+            if (d.phase[e] < 0 || d.phase[e] >= d.nphase) { set_error("phase out of range"); return AMIE_ERR_ARG; }
+    for (int64_t i = 0; i < d.nbc; i++)
+        if (d.bc_dofs[i] < 0 || d.bc_dofs[i] >= nd * et.b) { set_error("bc dof out of range"); return AMIE_ERR_ARG; }
+    return AMIE_OK;
+}
+
+static int alloc_and_upload(DevBuf &buf, const void *src, size_t bytes)
+{
+    if (!buf.alloc(bytes)) return AMIE_ERR_NOMEM;
+    if (src && bytes) AMIE_HIP(hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice));
+    return AMIE_OK;
+}
+
+static int alloc_vectors(Batch &B)
+{
+    const size_t nv = (size_t)B.nrows_pad * B.et.b * B.nrhs;
+    DevBuf *vecs[] = {&B.x64, &B.b64, &B.fext, &B.y, &B.r, &B.z, &B.p0, &B.p1, &B.q};
+    for (auto v : vecs) {
+        if (!v->alloc(nv * 8)) return AMIE_ERR_NOMEM;
+        AMIE_HIP(hipMemset(v->p, 0, v->n));
+    }
+    if (!B.S.alloc((size_t)B.nrows_pad * B.et.b * 8)) return AMIE_ERR_NOMEM;
+    if (!B.minv32.alloc((size_t)B.nrows_pad * B.et.b * B.et.b * 4)) return AMIE_ERR_NOMEM;
+    if (!B.minv64.alloc((size_t)B.nrows_pad * B.et.b * B.et.b * 8)) return AMIE_ERR_NOMEM;
+    if (!B.v64.alloc(B.nslots * slot_bytes(true, B.et.b))) return AMIE_ERR_NOMEM;
+    if (!B.v32.alloc(B.nslots * slot_bytes(false, B.et.b))) return AMIE_ERR_NOMEM;
+    AMIE_HIP(hipMemset(B.v64.p, 0, B.v64.n));
+    AMIE_HIP(hipMemset(B.v32.p, 0, B.v32.n));
+    // solver bookkeeping
+    if (!B.prob.alloc(B.nprob * sizeof(ProbDev))) return AMIE_ERR_NOMEM;
+    if (!B.ticket.alloc(B.nprob * sizeof(unsigned))) return AMIE_ERR_NOMEM;
+    AMIE_HIP(hipMemset(B.ticket.p, 0, B.ticket.n));
+    if (!B.partial.alloc((size_t)B.ntiles * MAXQ * 8)) return AMIE_ERR_NOMEM;
+    if (!B.active.alloc(B.ntiles * 4)) return AMIE_ERR_NOMEM;
+    if (!B.n_active.alloc(4)) return AMIE_ERR_NOMEM;
+    AMIE_HIP(hipHostMalloc((void **)&B.h_prob_pinned, B.nprob * sizeof(ProbDev), hipHostMallocDefault));
+    AMIE_HIP(hipHostMalloc((void **)&B.h_nactive_pinned, sizeof(int32_t), hipHostMallocDefault));
+    B.h_prob.resize(B.nprob);
+    for (int p = 0; p < B.nprob; p++) {
+        ProbDev &P = B.h_prob[p];
+        std::memset(&P, 0, sizeof(P));
+        P.tile0 = B.ph[p].tile0;
+        P.ntiles = B.ph[p].ntiles;
+        P.slice0 = B.ph[p].slice0;
+        P.nslices = B.ph[p].nslices;
+        P.row0 = B.ph[p].row0;
+        P.nrows = (int32_t)B.ph[p].nrows;
+    }
+    AMIE_HIP(hipMemcpy(B.prob.p, B.h_prob.data(), B.nprob * sizeof(ProbDev), hipMemcpyHostToDevice));
+    return B.set_active_all();
+}
+
+// SELL structure of every problem from its reference-order pattern; fills
+// the global slice / tile / column arrays.
+static int build_sell(Batch &B, std::vector<Tmp> &tmp)
+{
+    // offsets
+    int64_t slices = 0, tiles = 0, slots = 0;
+    std::vector<int64_t> slot0(B.nprob);
+    for (int p = 0; p < B.nprob; p++) {
+        ProbHost &P = B.ph[p];
+        P.slice0 = (int32_t)slices;
+        P.nslices = (int32_t)((P.nrows + WAVE - 1) / WAVE);
+        P.row0 = slices * WAVE;
+        P.tile0 = (int32_t)tiles;
+        P.ntiles = (P.nslices + TILE_SLICES - 1) / TILE_SLICES;
+        slot0[p] = slots;
+        for (int s = 0; s < P.nslices; s++) slots += (int64_t)tmp[p].width[s] * WAVE;
+        slices += P.nslices;
+        tiles += P.ntiles;
+    }
+    if (slots >= (int64_t)UINT32_MAX || slices * WAVE >= (int64_t)INT32_MAX) {
+        set_error("batch too large for 32-bit slot / row indices");
+        return AMIE_ERR_ARG;
+    }
+    B.nslices = slices;
+    B.ntiles = tiles;
+    B.nslots = slots;
+    B.nrows_pad = slices * WAVE;
+    std::vector<int64_t> h_slice_ptr(slices + 1);
+    std::vector<int32_t> h_width(slices), h_rows(slices), h_ts0(tiles), h_tns(tiles), h_tprob(tiles);
+    std::vector<uint32_t> h_col(slots);
+    std::vector<int64_t> h_dslot(B.nrows_pad);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < B.nprob; p++) {
+        ProbHost &P = B.ph[p];
+        Tmp &T = tmp[p];
+        int64_t sl = slot0[p];
+        P.ref_slot.assign(P.column_index.size(), -1);
+        for (int s = 0; s < P.nslices; s++) {
+            const int64_t gs = P.slice0 + s;
+            const int w = T.width[s];
+            h_slice_ptr[gs] = sl;
+            h_width[gs] = w;
+            h_rows[gs] = (int32_t)std::min<int64_t>(WAVE, P.nrows - (int64_t)s * WAVE);
+            for (int l = 0; l < WAVE; l++) {
+                const int64_t ir = (int64_t)s * WAVE + l;
+                const int64_t grow = P.row0 + ir;
+                h_dslot[grow] = sl + l;
+                int k = 0;
+                if (ir < P.nrows) {
+                    const int32_t orow = T.perm[ir];
+                    for (int64_t qb = T.row_start[orow]; qb < T.row_start[orow + 1]; qb++, k++) {
+                        const int64_t slot = sl + (int64_t)k * WAVE + l;
+                        const uint32_t c = P.column_index[qb];
+                        h_col[slot] = (uint32_t)(P.row0 + P.iperm[c]);
+                        P.ref_slot[qb] = slot;
+                        if ((int32_t)c == orow) h_dslot[grow] = slot;
+                    }
+                }
+                for (; k < w; k++) h_col[sl + (int64_t)k * WAVE + l] = (uint32_t)grow;
+            }
+            sl += (int64_t)w * WAVE;
+        }
+        for (int t = 0; t < P.ntiles; t++) {
+            h_ts0[P.tile0 + t] = P.slice0 + t * TILE_SLICES;
+            h_tns[P.tile0 + t] = std::min(TILE_SLICES, P.nslices - t * TILE_SLICES);
+            h_tprob[P.tile0 + t] = p;
+        }
+    }
+    h_slice_ptr[slices] = slots;
+    int rc;
+    if ((rc = alloc_and_upload(B.slice_ptr, h_slice_ptr.data(), h_slice_ptr.size() * 8))) return rc;
+    if ((rc = alloc_and_upload(B.slice_width, h_width.data(), h_width.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.slice_rows, h_rows.data(), h_rows.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.col, h_col.data(), h_col.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.dslot, h_dslot.data(), h_dslot.size() * 8))) return rc;
+    if ((rc = alloc_and_upload(B.tile_slice0, h_ts0.data(), h_ts0.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.tile_nslices, h_tns.data(), h_tns.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.tile_prob, h_tprob.data(), h_tprob.size() * 4))) return rc;
+    return AMIE_OK;
+}
+
+static void finish_tmp(ProbHost &P, Tmp &T)
+{
+    T.row_start.assign(P.nrows + 1, 0);
+    for (int64_t r = 0; r < P.nrows; r++) T.row_start[r + 1] = T.row_start[r] + P.row_size[r];
+    choose_permutation(P.row_size, T.perm);
+    P.iperm.assign(P.nrows, 0);
+    for (int64_t i = 0; i < P.nrows; i++) P.iperm[T.perm[i]] = (int32_t)i;
+    const int64_t ns = (P.nrows + WAVE - 1) / WAVE;
+    T.width.assign(ns, 1);
+    for (int64_t i = 0; i < P.nrows; i++)
+        T.width[i / WAVE] = std::max<int32_t>(T.width[i / WAVE], (int32_t)P.row_size[T.perm[i]]);
+}
+
+int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, Batch **out)
+{
+    if (!ctx || nprob <= 0 || !probs || !out) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    std::unique_ptr<Batch> Bp(new Batch());
+    Batch &B = *Bp;
+    B.ctx = ctx;
+    B.stream = ctx->stream;
+    B.et = etype_info(probs[0].elem_type);
+    B.nrhs = probs[0].nrhs;
+    B.nprob = nprob;
+    B.has_elements = true;
+    B.has_macro = probs[0].macro_strain != nullptr;
+    for (int p = 0; p < nprob; p++) {
+        int rc = validate(probs[p], B.et);
+        if (rc) return rc;
+        if (probs[p].elem_type != probs[0].elem_type || probs[p].nrhs != B.nrhs ||
+            (probs[p].macro_strain != nullptr) != B.has_macro) {
+            set_error("all problems of a batch need the same element type, nrhs and load kind");
+            return AMIE_ERR_ARG;
+        }
+    }
+    const EType et = B.et;
+    B.ph.resize(nprob);
+    std::vector<Tmp> tmp(nprob);
+    int bad_colour = -1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < nprob; p++) {
+        const amie_problem_desc &d = probs[p];
+        ProbHost &P = B.ph[p];
+        Tmp &T = tmp[p];
+        P.nelem = d.nelem;
+        P.nnodes = d.nnodes;
+        P.nrows = d.dofmap ? d.ndofnodes : d.nnodes;
+        P.has_elements = true;
+        T.edof.resize((size_t)d.nelem * et.nper);
+        for (int64_t q = 0; q < d.nelem * et.nper; q++) T.edof[q] = d.dofmap ? d.dofmap[d.conn[q]] : d.conn[q];
+        build_pattern(et.nper, d.nelem, T.edof.data(), P.nrows, P.row_size, P.column_index);
+        finish_tmp(P, T);
+        colour_elements(et.nper, d.nelem, T.edof.data(), P.nrows, T.colour, T.ncol);
+        if (T.ncol < 0) {
+#pragma omp critical
+            bad_colour = p;
+        }
+    }
+    if (bad_colour >= 0) { set_error("problem %d needs more than 128 element colours", bad_colour); return AMIE_ERR_ARG; }
+    int rc = build_sell(B, tmp);
+    if (rc) return rc;
+    // element and node offsets
+    int64_t ne = 0, nn = 0;
+    int32_t nc = 0;
+    int maxcol = 0;
+    for (int p = 0; p < nprob; p++) {
+        B.ph[p].elem0 = ne;
+        B.ph[p].node0 = nn;
+        B.ph[p].cmat0 = nc;
+        ne += probs[p].nelem;
+        nn += probs[p].nnodes;
+        nc += probs[p].nphase;
+        maxcol = std::max(maxcol, tmp[p].ncol);
+    }
+    if (ne >= INT32_MAX || nn >= INT32_MAX) { set_error("too many elements/nodes in one batch"); return AMIE_ERR_ARG; }
+    B.nelem = ne;
+    B.nnodes_geo = nn;
+    const int S = et.s, nper = et.nper, b = et.b, NR = B.nrhs;
+    std::vector<double> h_coords((size_t)nn * et.dim), h_cmat((size_t)nc * S * S), h_macro;
+    std::vector<int32_t> h_econn((size_t)ne * nper), h_erow((size_t)ne * nper), h_phase(ne), h_eprob(ne), h_cmat_off(nprob);
+    std::vector<int32_t> h_pe0(nprob), h_pne(nprob);
+    std::vector<uint32_t> h_eslot((size_t)ne * nper * nper);
+    std::vector<double> h_fext((size_t)B.nrows_pad * b * NR, 0.), h_bcval((size_t)B.nrows_pad * b, 0.);
+    std::vector<uint8_t> h_bcmask((size_t)B.nrows_pad * b, 0);
+    std::vector<double> h_damage;
+    bool any_damage = false;
+    for (int p = 0; p < nprob; p++) any_damage |= probs[p].damage != nullptr;
+    if (any_damage) h_damage.assign(ne, 0.);
+    if (B.has_macro) h_macro.resize((size_t)nprob * NR * S);
+    std::vector<std::vector<int32_t>> col_elems(maxcol);
+    int slot_fail = -1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < nprob; p++) {
+        const amie_problem_desc &d = probs[p];
+        ProbHost &P = B.ph[p];
+        Tmp &T = tmp[p];
+        h_cmat_off[p] = P.cmat0;
+        h_pe0[p] = (int32_t)P.elem0;
+        h_pne[p] = (int32_t)d.nelem;
+        std::memcpy(&h_coords[(size_t)P.node0 * et.dim], d.coords, (size_t)d.nnodes * et.dim * 8);
+        std::memcpy(&h_cmat[(size_t)P.cmat0 * S * S], d.cmat, (size_t)d.nphase * S * S * 8);
+        if (B.has_macro) std::memcpy(&h_macro[(size_t)p * NR * S], d.macro_strain, (size_t)NR * S * 8);
+        for (int64_t e = 0; e < d.nelem; e++) {
+            const int64_t ge = P.elem0 + e;
+            h_phase[ge] = d.phase ? d.phase[e] : 0;
+            h_eprob[ge] = p;
+            if (d.damage) h_damage[ge] = d.damage[e];
+            for (int i = 0; i < nper; i++) {
+                h_econn[ge * nper + i] = (int32_t)(P.node0 + d.conn[e * nper + i]);
+                h_erow[ge * nper + i] = (int32_t)(P.row0 + P.iperm[T.edof[e * nper + i]]);
+                for (int j = 0; j < nper; j++) {
+                    int64_t qb = find_block(P.column_index, T.row_start, T.edof[e * nper + i], (uint32_t)T.edof[e * nper + j]);
+                    if (qb < 0) { slot_fail = p; continue; }
+                    h_eslot[(ge * nper + i) * nper + j] = (uint32_t)P.ref_slot[qb];
+                }
+            }
+        }
+        const int64_t ndof = P.nrows * b;
+        if (d.forces)
+            for (int k = 0; k < NR; k++)
+                for (int64_t dof = 0; dof < ndof; dof++)
+                    h_fext[((P.row0 + P.iperm[dof / b]) * NR + k) * b + dof % b] = d.forces[(size_t)k * ndof + dof];
+        for (int64_t i = 0; i < d.nbc; i++) {
+            const int64_t dof = d.bc_dofs[i];
+            const int64_t g = (P.row0 + P.iperm[dof / b]) * b + dof % b;
+            h_bcmask[g] = 1;
+            h_bcval[g] = d.bc_vals[i];
+        }
+    }
+    if (slot_fail >= 0) { set_error("internal: element block missing from pattern (problem %d)", slot_fail); return AMIE_ERR_STATE; }
+    for (int p = 0; p < nprob; p++)
+        for (int64_t e = 0; e < probs[p].nelem; e++) col_elems[tmp[p].colour[e]].push_back((int32_t)(B.ph[p].elem0 + e));
+    std::vector<int32_t> h_colour_elems;
+    h_colour_elems.reserve(ne);
+    B.colour_ptr.assign(1, 0);
+    for (int c = 0; c < maxcol; c++) {
+        h_colour_elems.insert(h_colour_elems.end(), col_elems[c].begin(), col_elems[c].end());
+        B.colour_ptr.push_back((int32_t)h_colour_elems.size());
+    }
+    if ((rc = alloc_and_upload(B.coords, h_coords.data(), h_coords.size() * 8))) return rc;
+    if ((rc = alloc_and_upload(B.econn, h_econn.data(), h_econn.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.erow, h_erow.data(), h_erow.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.eslot, h_eslot.data(), h_eslot.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.ephase, h_phase.data(), h_phase.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.eprob, h_eprob.data(), h_eprob.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.cmat, h_cmat.data(), h_cmat.size() * 8))) return rc;
+    if ((rc = alloc_and_upload(B.cmat_off, h_cmat_off.data(), h_cmat_off.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.prob_e0, h_pe0.data(), h_pe0.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.prob_ne, h_pne.data(), h_pne.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.colour_elems, h_colour_elems.data(), h_colour_elems.size() * 4))) return rc;
+    if (B.has_macro && (rc = alloc_and_upload(B.macro, h_macro.data(), h_macro.size() * 8))) return rc;
+    // damage state: d (C_e = C (1 - d)), kappa history, initial damage d0
+    if ((rc = alloc_and_upload(B.edamage, any_damage ? h_damage.data() : nullptr, (size_t)ne * 8))) return rc;
+    if (!any_damage) AMIE_HIP(hipMemset(B.edamage.p, 0, B.edamage.n));
+    if ((rc = alloc_and_upload(B.ed0, any_damage ? h_damage.data() : nullptr, (size_t)ne * 8))) return rc;
+    if (!any_damage) AMIE_HIP(hipMemset(B.ed0.p, 0, B.ed0.n));
+    if ((rc = alloc_and_upload(B.ekappa, nullptr, (size_t)ne * 8))) return rc;
+    AMIE_HIP(hipMemset(B.ekappa.p, 0, B.ekappa.n));
+    B.has_damage_state = any_damage;
+    if ((rc = alloc_and_upload(B.grown, nullptr, 8))) return rc;
+    if ((rc = alloc_and_upload(B.bcmask, h_bcmask.data(), h_bcmask.size()))) return rc;
+    if ((rc = alloc_and_upload(B.bcval, h_bcval.data(), h_bcval.size() * 8))) return rc;
+    if ((rc = alloc_vectors(B))) return rc;
+    AMIE_HIP(hipMemcpy(B.fext.p, h_fext.data(), h_fext.size() * 8, hipMemcpyHostToDevice));
+    if ((rc = B.assemble())) return rc;
+    *out = Bp.release();
+    return AMIE_OK;
+}
+
+int batch_from_bcsr(amie_gpu_ctx *ctx, int stride, int64_t nrows, const uint32_t *row_size,
+                    const uint32_t *column_index, const double *array, const double *rhs, Batch **out)
+{
+    if (!ctx || (stride != 2 && stride != 3) || nrows <= 0 || !row_size || !column_index || !array || !rhs || !out) {
+        set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    std::unique_ptr<Batch> Bp(new Batch());
+    Batch &B = *Bp;
+    B.ctx = ctx;
+    B.stream = ctx->stream;
+    B.et = etype_info(stride == 2 ? AMIE_P1 : AMIE_TET4);
+    B.nrhs = 1;
+    B.nprob = 1;
+    B.ph.resize(1);
+    ProbHost &P = B.ph[0];
+    P.nrows = nrows;
+    P.row_size.assign(row_size, row_size + nrows);
+    int64_t nnzb = 0;
+    for (int64_t r = 0; r < nrows; r++) nnzb += row_size[r];
+    P.column_index.assign(column_index, column_index + nnzb);
+    std::vector<Tmp> tmp(1);
+    finish_tmp(P, tmp[0]);
+    // every row needs its diagonal block and sorted, in-range columns
+    for (int64_t r = 0; r < nrows; r++) {
+        bool diag = false;
+        for (int64_t q = tmp[0].row_start[r]; q < tmp[0].row_start[r + 1]; q++) {
+            if (P.column_index[q] >= nrows || (q > tmp[0].row_start[r] && P.column_index[q] <= P.column_index[q - 1])) {
+                set_error("row %lld: column indices must be sorted and < nrows", (long long)r);
+                return AMIE_ERR_ARG;
+            }
+            diag |= P.column_index[q] == (uint32_t)r;
+        }
+        if (!diag) { set_error("row %lld has no diagonal block", (long long)r); return AMIE_ERR_ARG; }
+    }
+    int rc = build_sell(B, tmp);
+    if (rc) return rc;
+    if ((rc = alloc_vectors(B))) return rc;
+    // reference padded column-major blocks -> FP64 planes
+    const int b = stride, cl = stride + stride % 2;
+    std::vector<double> planes(B.nslots * b * b, 0.);
+    const int np = (b * b) / 2;
+    for (int64_t q = 0; q < nnzb; q++) {
+        const int64_t s = P.ref_slot[q];
+        double a[9];
+        for (int c = 0; c < b; c++)
+            for (int r = 0; r < b; r++) a[c * b + r] = array[q * b * cl + c * cl + r];
+        for (int k = 0; k < np; k++) {
+            planes[(k * B.nslots + s) * 2] = a[2 * k];
+            planes[(k * B.nslots + s) * 2 + 1] = a[2 * k + 1];
+        }
+        if ((b * b) % 2) planes[np * B.nslots * 2 + s] = a[b * b - 1];
+    }
+    AMIE_HIP(hipMemcpy(B.v64.p, planes.data(), B.nslots * slot_bytes(true, b), hipMemcpyHostToDevice));
+    std::vector<double> h_b((size_t)B.nrows_pad * b, 0.);
+    for (int64_t dof = 0; dof < nrows * b; dof++) h_b[(size_t)P.iperm[dof / b] * b + dof % b] = rhs[dof];
+    AMIE_HIP(hipMemcpy(B.b64.p, h_b.data(), h_b.size() * 8, hipMemcpyHostToDevice));
+    AMIE_HIP(hipMemcpy(B.fext.p, h_b.data(), h_b.size() * 8, hipMemcpyHostToDevice));
+    if ((rc = B.assemble())) return rc;
+    *out = Bp.release();
+    return AMIE_OK;
+}
+
+// ---------------------------------------------------------- assembly
+template <int ET, int NR> static void launch_assemble(Batch &B, const AsmArgs &a)
+{
+    for (size_t c = 0; c + 1 < B.colour_ptr.size(); c++) {
+        const int n = B.colour_ptr[c + 1] - B.colour_ptr[c];
+        if (!n) continue;
+        hipLaunchKernelGGL((k_assemble<ET, NR>), dim3((n + 255) / 256), dim3(256), 0, B.stream, a,
+                           B.colour_elems.as<int32_t>() + B.colour_ptr[c], n, Elem<ET>::S);
+    }
+}
+
+template <int Bk> static void launch_post(Batch &B)
+{
+    SolveArgs a = B.args(true);
+    const int g = B.grid();
+    hipStream_t st = B.stream;
+    if (B.has_elements) {
+        if (B.nrhs == 1)
+            hipLaunchKernelGGL((k_apply_bc<Bk, 1>), dim3(g), dim3(WG), 0, st, a, B.v64.as<double>(), B.b64.as<double>(),
+                               B.bcmask.as<uint8_t>(), B.bcval.as<double>(), (int)B.ntiles);
+        else
+            hipLaunchKernelGGL((k_apply_bc<Bk, 3>), dim3(g), dim3(WG), 0, st, a, B.v64.as<double>(), B.b64.as<double>(),
+                               B.bcmask.as<uint8_t>(), B.bcval.as<double>(), (int)B.ntiles);
+    }
+    hipLaunchKernelGGL((k_diag_scale<Bk>), dim3(g), dim3(WG), 0, st, a, B.dslot.as<int64_t>(), B.S.as<double>(), (int)B.ntiles);
+    hipLaunchKernelGGL((k_scale_cast<float, Bk>), dim3(g), dim3(WG), 0, st, a, B.v32.p, B.dslot.as<int64_t>(),
+                       B.minv32.as<float>(), (int)B.ntiles);
+    hipLaunchKernelGGL((k_scale_cast<double, Bk>), dim3(g), dim3(WG), 0, st, a, (void *)nullptr, B.dslot.as<int64_t>(),
+                       B.minv64.as<double>(), (int)B.ntiles);
+}
+
+int Batch::assemble()
+{
+    hipStream_t st = stream;
+    if (has_elements) {
+        AMIE_HIP(hipMemsetAsync(v64.p, 0, v64.n, st));
+        AMIE_HIP(hipMemcpyAsync(b64.p, fext.p, b64.n, hipMemcpyDeviceToDevice, st));
+        AsmArgs a{};
+        a.coords = coords.as<double>();
+        a.econn = econn.as<int32_t>();
+        a.erow = erow.as<int32_t>();
+        a.eslot = eslot.as<uint32_t>();
+        a.ephase = ephase.as<int32_t>();
+        a.edamage = edamage.as<double>();
+        a.eprob = eprob.as<int32_t>();
+        a.cmat_off = cmat_off.as<int32_t>();
+        a.cmat = cmat.as<double>();
+        a.macro = has_macro ? macro.as<double>() : nullptr;
+        a.v64 = v64.as<double>();
+        a.nslots = nslots;
+        a.b64 = b64.as<double>();
+        if (et.et == 1) { if (nrhs == 1) launch_assemble<1, 1>(*this, a); else launch_assemble<1, 3>(*this, a); }
+        else if (et.et == 2) { if (nrhs == 1) launch_assemble<2, 1>(*this, a); else launch_assemble<2, 3>(*this, a); }
+        else { if (nrhs == 1) launch_assemble<3, 1>(*this, a); else launch_assemble<3, 3>(*this, a); }
+    }
+    if (et.b == 2) launch_post<2>(*this);
+    else launch_post<3>(*this);
+    AMIE_HIP(hipGetLastError());
+    AMIE_HIP(hipStreamSynchronize(st));
+    return AMIE_OK;
+}
+
+// -------------------------------------------------------------- solve
+template <typename W, int Bk, int NR, bool BJ> static Kern kern_for()
+{
+    return {k_spmv_p<W, Bk, NR, BJ>, k_update<W, Bk, NR, BJ>, k_ckpt_x<W, Bk, NR>, k_ckpt_r<W, Bk, NR, BJ>,
+            k_rhs_norms<Bk, NR>};
+}
+
+template <typename W, int Bk, int NR> static Kern kern_bj(bool bj)
+{
+    return bj ? kern_for<W, Bk, NR, true>() : kern_for<W, Bk, NR, false>();
+}
+
+static Kern select_kern(bool mixed, int b, int nr, bool bj)
+{
+    if (mixed) {
+        if (b == 2) return nr == 1 ? kern_bj<float, 2, 1>(bj) : kern_bj<float, 2, 3>(bj);
+        return nr == 1 ? kern_bj<float, 3, 1>(bj) : kern_bj<float, 3, 3>(bj);
+    }
+    if (b == 2) return nr == 1 ? kern_bj<double, 2, 1>(bj) : kern_bj<double, 2, 3>(bj);
+    return nr == 1 ? kern_bj<double, 3, 1>(bj) : kern_bj<double, 3, 3>(bj);
+}
+
+static void launch1(void (*k)(SolveArgs, int), int g, hipStream_t st, const SolveArgs &a, int par)
+{
+    hipLaunchKernelGGL(k, dim3(g), dim3(WG), 0, st, a, par);
+}
+static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs &a)
+{
+    hipLaunchKernelGGL(k, dim3(g), dim3(WG), 0, st, a);
+}
+
+// algorithmic bytes of one fused SpMV launch over problem p (values + block
+// column indices + row pointer + 4 vector streams per right-hand side)
+static double spmv_bytes(const Batch &B, const ProbHost &P, bool mixed)
+{
+    const double s = mixed ? 4. : 8.;
+    const double b = B.et.b;
+    const double nnzb = (double)P.column_index.size();
+    const double n = (double)P.nrows * b;
+    return nnzb * (b * b * s + 4.) + 4. * (P.nrows + 1) + 4. * n * s * B.nrhs;
+}
+
+int Batch::solve_begin(const amie_solve_opts &o)
+{
+    SolveRun &R = run;
+    R = SolveRun();
+    R.o = o;
+    R.mixed = o.precision == AMIE_MIXED;
+    R.bj = o.precond == AMIE_BLOCK_JACOBI;
+    R.per_check = std::max(1, (o.chunk > 0 ? o.chunk : 32) / R.ce);
+    hipStream_t s = stream;
+    R.K = select_kern(R.mixed, et.b, nrhs, R.bj);
+    R.a = args(R.mixed);
+    R.g = grid();
+    for (int p = 0; p < nprob; p++) {
+        ProbDev &P = h_prob[p];
+        P.refine = o.refine;
+        P.rtol = o.rtol > 0 ? o.rtol : 1e-8;
+        P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
+        P.maxit = o.maxit > 0 ? o.maxit : ph[p].nrows * et.b;
+        P.status = ST_RUNNING;
+        P.ckpt_pending = 1;
+        P.iters = 0;
+        P.nckpt = 0;
+        for (int k = 0; k < MAXRHS; k++) {
+            P.rho[k] = P.rho_prev[k] = P.alpha[k] = P.beta[k] = 0.;
+            P.rr[k] = 0.;
+            P.rr_ckpt[k] = DBL_MAX;
+            P.bb[k] = P.bb64[k] = 0.;
+            P.relres[k] = 0.;
+            P.done[k] = k < nrhs ? 0 : 1;
+        }
+    }
+    AMIE_HIP(hipMemcpyAsync(prob.p, h_prob.data(), nprob * sizeof(ProbDev), hipMemcpyHostToDevice, s));
+    if (!o.use_x0) AMIE_HIP(hipMemsetAsync(x64.p, 0, x64.n, s));
+    AMIE_HIP(hipMemsetAsync(y.p, 0, y.n, s));
+    AMIE_HIP(hipMemsetAsync(p0.p, 0, p0.n, s));
+    AMIE_HIP(hipMemsetAsync(p1.p, 0, p1.n, s));
+    int rc = set_active_all();
+    if (rc) return rc;
+    AMIE_HIP(hipEventCreate(&R.e0));
+    AMIE_HIP(hipEventCreate(&R.e1));
+    AMIE_HIP(hipEventCreateWithFlags(&R.ev_status, hipEventDisableTiming));
+    AMIE_HIP(hipEventRecord(R.e0, s));
+    launch0(R.K.norms, R.g, s, R.a);
+    launch0(R.K.cx, R.g, s, R.a);
+    launch0(R.K.cr, R.g, s, R.a);
+    if (o.use_graph && !o.time_kernels) {
+        const int key = (R.mixed ? 1 : 0) | (R.bj ? 2 : 0) | (R.ce << 4);
+        for (auto &ge : graphs)
+            if (ge.key == key) R.exec = ge.exec;
+        if (!R.exec) {
+            hipGraph_t graph;
+            AMIE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < R.ce; i++) {
+                launch1(R.K.k1, R.g, s, R.a, i & 1);
+                launch1(R.K.k2, R.g, s, R.a, i & 1);
+            }
+            launch0(R.K.cx, R.g, s, R.a);
+            launch0(R.K.cr, R.g, s, R.a);
+            AMIE_HIP(hipStreamEndCapture(s, &graph));
+            AMIE_HIP(hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0));
+            AMIE_HIP(hipGraphDestroy(graph));
+            graphs.push_back({key, R.exec});
+        }
+    }
+    R.pbytes.resize(nprob);
+    R.active_bytes = 0.;
+    for (int p = 0; p < nprob; p++) {
+        R.pbytes[p] = spmv_bytes(*this, ph[p], R.mixed);
+        R.active_bytes += R.pbytes[p];
+    }
+    R.was_running.assign(nprob, 1);
+    int64_t maxit_all = 0;
+    for (int p = 0; p < nprob; p++) maxit_all = std::max<int64_t>(maxit_all, h_prob[p].maxit);
+    R.max_periods = maxit_all / R.ce + 4;
+    R.periods = 0;
+    R.in_flight = false;
+    R.finished = false;
+    R.started = true;
+    return AMIE_OK;
+}
+
+int Batch::solve_advance()
+{
+    SolveRun &R = run;
+    if (!R.started || R.finished || R.in_flight) return AMIE_OK;
+    hipStream_t s = stream;
+    R.nt = 0;
+    for (int c = 0; c < R.per_check; c++, R.periods++) {
+        if (R.exec) {
+            AMIE_HIP(hipGraphLaunch(R.exec, s));
+        } else {
+            for (int i = 0; i < R.ce; i++) {
+                if (R.o.time_kernels) {
+                    if ((size_t)(2 * R.nt + 1) >= R.tev.size()) {
+                        hipEvent_t ev;
+                        AMIE_HIP(hipEventCreate(&ev));
+                        R.tev.push_back(ev);
+                        AMIE_HIP(hipEventCreate(&ev));
+                        R.tev.push_back(ev);
+                    }
+                    AMIE_HIP(hipEventRecord(R.tev[2 * R.nt], s));
+                }
+                launch1(R.K.k1, R.g, s, R.a, i & 1);
+                if (R.o.time_kernels) {
+                    AMIE_HIP(hipEventRecord(R.tev[2 * R.nt + 1], s));
+                    R.nt++;
+                }
+                launch1(R.K.k2, R.g, s, R.a, i & 1);
+            }
+            launch0(R.K.cx, R.g, s, R.a);
+            launch0(R.K.cr, R.g, s, R.a);
+        }
+    }
+    AMIE_HIP(hipGetLastError());
+    AMIE_HIP(hipMemcpyAsync(h_prob_pinned, prob.p, nprob * sizeof(ProbDev), hipMemcpyDeviceToHost, s));
+    AMIE_HIP(hipEventRecord(R.ev_status, s));
+    R.in_flight = true;
+    return AMIE_OK;
+}
+
+int Batch::solve_poll(bool wait, bool *finished)
+{
+    SolveRun &R = run;
+    *finished = R.finished;
+    if (!R.started || R.finished || !R.in_flight) return AMIE_OK;
+    if (wait) {
+        AMIE_HIP(hipEventSynchronize(R.ev_status));
+    } else {
+        hipError_t q = hipEventQuery(R.ev_status);
+        if (q == hipErrorNotReady) return AMIE_OK;
+        if (!hip_ok(q, "hipEventQuery")) return AMIE_ERR_HIP;
+    }
+    R.in_flight = false;
+    hipStream_t s = stream;
+    for (int i = 0; i < R.nt; i++) {
+        float ms = 0.f;
+        AMIE_HIP(hipEventElapsedTime(&ms, R.tev[2 * i], R.tev[2 * i + 1]));
+        R.stats.spmv_kernel_ms += ms;
+        R.stats.spmv_launches += 1;
+        R.stats.spmv_bytes += R.active_bytes;
+    }
+    bool any_running = false, changed = false;
+    for (int p = 0; p < nprob; p++) {
+        const bool runp = h_prob_pinned[p].status == ST_RUNNING;
+        any_running |= runp;
+        if (R.was_running[p] && !runp) {
+            changed = true;
+            R.was_running[p] = 0;
+            R.active_bytes -= R.pbytes[p];
+        }
+    }
+    if (any_running && changed) {
+        h_active.clear();
+        for (int p = 0; p < nprob; p++)
+            if (R.was_running[p])
+                for (int t = 0; t < ph[p].ntiles; t++) h_active.push_back(ph[p].tile0 + t);
+        *h_nactive_pinned = (int32_t)h_active.size();
+        AMIE_HIP(hipMemcpyAsync(active.p, h_active.data(), h_active.size() * 4, hipMemcpyHostToDevice, s));
+        AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, 4, hipMemcpyHostToDevice, s));
+    }
+    if (!any_running || R.periods >= R.max_periods) {
+        AMIE_HIP(hipEventRecord(R.e1, s));
+        AMIE_HIP(hipEventSynchronize(R.e1));
+        float ms = 0.f;
+        AMIE_HIP(hipEventElapsedTime(&ms, R.e0, R.e1));
+        R.stats.solve_ms = ms;
+        std::memcpy(h_prob.data(), h_prob_pinned, nprob * sizeof(ProbDev));
+        R.finished = true;
+        *finished = true;
+    }
+    return AMIE_OK;
+}
+
+int Batch::solve_end(amie_solve_stats *st)
+{
+    SolveRun &R = run;
+    if (!R.finished) { set_error("solve not finished"); return AMIE_ERR_STATE; }
+    (void)hipEventDestroy(R.e0);
+    (void)hipEventDestroy(R.e1);
+    (void)hipEventDestroy(R.ev_status);
+    for (auto ev : R.tev) (void)hipEventDestroy(ev);
+    R.tev.clear();
+    R.started = false;
+    amie_solve_stats &stats = R.stats;
+    bool nan = false, all = true;
+    for (int p = 0; p < nprob; p++) {
+        const ProbDev &P = h_prob[p];
+        stats.iterations_max = std::max<int64_t>(stats.iterations_max, P.iters);
+        stats.iterations_sum += P.iters;
+        stats.refinements += P.nckpt;
+        for (int k = 0; k < nrhs; k++) stats.relres_max = std::max(stats.relres_max, P.relres[k]);
+        if (P.status == ST_CONVERGED) stats.nproblems_converged++;
+        else all = false;
+        if (P.status == ST_NAN) nan = true;
+    }
+    stats.status = nan ? AMIE_ERR_NAN : (all ? AMIE_OK : AMIE_NOT_CONVERGED);
+    if (st) *st = stats;
+    // full active list again for later kernels (SpMV hook, strains)
+    int rc = set_active_all();
+    if (rc) return rc;
+    return stats.status;
+}
+
+int Batch::solve(const amie_solve_opts &o, amie_solve_stats *st)
+{
+    int rc = solve_begin(o);
+    if (rc) return rc;
+    bool fin = false;
+    while (!fin) {
+        if ((rc = solve_advance())) return rc;
+        if ((rc = solve_poll(true, &fin))) return rc;
+    }
+    return solve_end(st);
+}
+
+// ------------------------------------------------------------ helpers
+static int check_p(Batch *B, int p)
+{
+    if (!B || p < 0 || p >= B->nprob) { set_error("bad problem index %d", p); return AMIE_ERR_ARG; }
+    return AMIE_OK;
+}
+
+// internal vector (entries of 8 bytes, [row][rhs][axis]) -> original order, column-major per rhs
+static int download_vec(Batch &B, int p, const DevBuf &buf, double *out)
+{
+    const ProbHost &P = B.ph[p];
+    const int b = B.et.b, NR = B.nrhs;
+    std::vector<double> h((size_t)P.nslices * WAVE * b * NR);
+    AMIE_HIP(hipMemcpy(h.data(), buf.as<double>() + (size_t)P.row0 * b * NR, h.size() * 8, hipMemcpyDeviceToHost));
+    const int64_t ndof = P.nrows * b;
+    for (int k = 0; k < NR; k++)
+        for (int64_t d = 0; d < ndof; d++) out[(size_t)k * ndof + d] = h[((size_t)P.iperm[d / b] * NR + k) * b + d % b];
+    return AMIE_OK;
+}
+
+static int upload_vec(Batch &B, int p, DevBuf &buf, const double *in)
+{
+    const ProbHost &P = B.ph[p];
+    const int b = B.et.b, NR = B.nrhs;
+    std::vector<double> h((size_t)P.nslices * WAVE * b * NR, 0.);
+    const int64_t ndof = P.nrows * b;
+    for (int k = 0; k < NR; k++)
+        for (int64_t d = 0; d < ndof; d++) h[((size_t)P.iperm[d / b] * NR + k) * b + d % b] = in[(size_t)k * ndof + d];
+    AMIE_HIP(hipMemcpy(buf.as<double>() + (size_t)P.row0 * b * NR, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    return AMIE_OK;
+}
+
+} // namespace amie
+
+using namespace amie;
+
+// =========================================================== C ABI: batches
+extern "C" {
+
+const char *amie_gpu_last_error(void) { return amie::last_error(); }
+
+int amie_gpu_create(amie_gpu_ctx **ctx, int device)
+{
+    if (!ctx) { set_error("ctx is NULL"); return AMIE_ERR_ARG; }
+    int n = 0;
+    AMIE_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) { set_error("device %d not present (%d devices)", device, n); return AMIE_ERR_ARG; }
+    AMIE_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    AMIE_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("device %d is %s; this library is built for gfx950 (MI355X)", device, prop.gcnArchName);
+        return AMIE_ERR_ARG;
+    }
+    std::unique_ptr<amie_gpu_ctx> c(new amie_gpu_ctx());
+    c->device = device;
+    c->num_cu = prop.multiProcessorCount;
+    c->total_mem = prop.totalGlobalMem;
+    AMIE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *ctx = c.release();
+    return AMIE_OK;
+}
+
+void amie_gpu_destroy(amie_gpu_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int amie_gpu_device_info(amie_gpu_ctx *ctx, size_t *free_bytes, size_t *total_bytes, int *num_cu)
+{
+    if (!ctx) { set_error("ctx is NULL"); return AMIE_ERR_ARG; }
+    size_t f = 0, t = 0;
+    AMIE_HIP(hipSetDevice(ctx->device));
+    AMIE_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    if (num_cu) *num_cu = ctx->num_cu;
+    return AMIE_OK;
+}
+
+void amie_gpu_default_opts(amie_solve_opts *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->precision = AMIE_MIXED;
+    o->precond = AMIE_JACOBI;
+    o->refine = AMIE_REFINE_REPLACE;
+    o->rtol = 1e-8;
+    o->refine_drop = 1e-2;
+    o->maxit = 0;
+    o->use_x0 = 0;
+    o->use_graph = 1;
+    o->chunk = 32;
+    o->time_kernels = 0;
+}
+
+int amie_gpu_batch_create(amie_gpu_ctx *ctx, int32_t nprob, const amie_problem_desc *probs, amie_gpu_batch **out)
+{
+    if (!out) { set_error("out is NULL"); return AMIE_ERR_ARG; }
+    if (ctx) AMIE_HIP(hipSetDevice(ctx->device));
+    Batch *B = nullptr;
+    int rc = batch_create(ctx, nprob, probs, &B);
+    if (rc) return rc;
+    *out = new amie_gpu_batch{B};
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_from_bcsr(amie_gpu_ctx *ctx, int32_t stride, int64_t nrows, const uint32_t *row_size,
+                             const uint32_t *column_index, const double *array, const double *rhs, amie_gpu_batch **out)
+{
+    if (!out) { set_error("out is NULL"); return AMIE_ERR_ARG; }
+    if (ctx) AMIE_HIP(hipSetDevice(ctx->device));
+    Batch *B = nullptr;
+    int rc = batch_from_bcsr(ctx, stride, nrows, row_size, column_index, array, rhs, &B);
+    if (rc) return rc;
+    *out = new amie_gpu_batch{B};
+    return AMIE_OK;
+}
+
+void amie_gpu_batch_destroy(amie_gpu_batch *b)
+{
+    if (!b) return;
+    if (b->b) (void)hipStreamSynchronize(b->b->stream);
+    delete b->b;
+    delete b;
+}
+
+size_t amie_gpu_batch_bytes(const amie_gpu_batch *b) { return b && b->b ? b->b->bytes() : 0; }
+
+int amie_gpu_batch_assemble(amie_gpu_batch *b)
+{
+    if (!b || !b->b) { set_error("batch is NULL"); return AMIE_ERR_ARG; }
+    return b->b->assemble();
+}
+
+int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_solve_stats *stats)
+{
+    if (!b || !b->b) { set_error("batch is NULL"); return AMIE_ERR_ARG; }
+    amie_solve_opts o;
+    amie_gpu_default_opts(&o);
+    if (opts) o = *opts;
+    if (o.precision != AMIE_FP64 && o.precision != AMIE_MIXED) { set_error("bad precision"); return AMIE_ERR_ARG; }
+    if (o.precond != AMIE_JACOBI && o.precond != AMIE_BLOCK_JACOBI) { set_error("bad preconditioner"); return AMIE_ERR_ARG; }
+    return b->b->solve(o, stats);
+}
+
+int amie_gpu_batch_get_solution(amie_gpu_batch *b, int32_t p, double *x)
+{
+    if (!b || check_p(b->b, p) || !x) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    return download_vec(*b->b, p, b->b->x64, x);
+}
+
+int amie_gpu_batch_set_solution(amie_gpu_batch *b, int32_t p, const double *x)
+{
+    if (!b || check_p(b->b, p) || !x) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    return upload_vec(*b->b, p, b->b->x64, x);
+}
+
+int amie_gpu_batch_get_nnzb(amie_gpu_batch *b, int32_t p, int64_t *nrows, int64_t *nnzb)
+{
+    if (!b || check_p(b->b, p)) return AMIE_ERR_ARG;
+    if (nrows) *nrows = b->b->ph[p].nrows;
+    if (nnzb) *nnzb = (int64_t)b->b->ph[p].column_index.size();
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_get_matrix(amie_gpu_batch *bb, int32_t p, uint32_t *row_size, uint32_t *column_index, double *array,
+                              double *rhs)
+{
+    if (!bb || check_p(bb->b, p)) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    const ProbHost &P = B.ph[p];
+    if (row_size) std::memcpy(row_size, P.row_size.data(), P.row_size.size() * 4);
+    if (column_index) std::memcpy(column_index, P.column_index.data(), P.column_index.size() * 4);
+    if (array) {
+        const int b = B.et.b, cl = b + b % 2;
+        std::vector<double> planes(B.nslots * b * b);
+        AMIE_HIP(hipMemcpy(planes.data(), B.v64.p, B.nslots * b * b * 8, hipMemcpyDeviceToHost));
+        const int np = (b * b) / 2;
+        for (size_t q = 0; q < P.column_index.size(); q++) {
+            const int64_t s = P.ref_slot[q];
+            double a[9];
+            for (int k = 0; k < np; k++) {
+                a[2 * k] = planes[(k * B.nslots + s) * 2];
+                a[2 * k + 1] = planes[(k * B.nslots + s) * 2 + 1];
+            }
+            if ((b * b) % 2) a[b * b - 1] = planes[np * B.nslots * 2 + s];
+            double *dst = array + q * b * cl;
+            for (int c = 0; c < b; c++) {
+                for (int r = 0; r < b; r++) dst[c * cl + r] = a[c * b + r];
+                for (int r = b; r < cl; r++) dst[c * cl + r] = 0.;
+            }
+        }
+    }
+    if (rhs) return download_vec(B, p, B.b64, rhs);
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_get_element_blocks(amie_gpu_batch *bb, int32_t p, int64_t *slots)
+{
+    if (!bb || check_p(bb->b, p) || !slots) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
+    const ProbHost &P = B.ph[p];
+    const int nper = B.et.nper;
+    std::vector<uint32_t> h((size_t)P.nelem * nper * nper);
+    AMIE_HIP(hipMemcpy(h.data(), B.eslot.as<uint32_t>() + (size_t)P.elem0 * nper * nper, h.size() * 4, hipMemcpyDeviceToHost));
+    // global SELL slot -> reference block index
+    std::vector<std::pair<int64_t, int64_t>> inv(P.ref_slot.size());
+    for (size_t q = 0; q < P.ref_slot.size(); q++) inv[q] = {P.ref_slot[q], (int64_t)q};
+    std::sort(inv.begin(), inv.end());
+    for (size_t i = 0; i < h.size(); i++) {
+        auto it = std::lower_bound(inv.begin(), inv.end(), std::make_pair((int64_t)h[i], (int64_t)-1));
+        slots[i] = (it != inv.end() && it->first == (int64_t)h[i]) ? it->second : -1;
+    }
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const double *x, double *y)
+{
+    if (!bb || check_p(bb->b, p) || !x || !y) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    int rc = upload_vec(B, p, B.q, x); // q as scratch input (8-byte entries)
+    if (rc) return rc;
+    SolveArgs a = B.args(precision == AMIE_MIXED);
+    const int g = B.grid();
+    const int mode = precision == AMIE_MIXED ? 1 : 0;
+    double *xin = B.q.as<double>(), *yo = B.z.as<double>();
+    hipStream_t st = B.stream;
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(g), dim3(WG), 0, st, a, mode, xin, yo, (int)B.ntiles); };
+    if (B.et.b == 2) {
+        if (B.nrhs == 1) go(k_spmv_plain<float, 2, 1>);
+        else go(k_spmv_plain<float, 2, 3>);
+    } else {
+        if (B.nrhs == 1) go(k_spmv_plain<float, 3, 1>);
+        else go(k_spmv_plain<float, 3, 3>);
+    }
+    AMIE_HIP(hipGetLastError());
+    AMIE_HIP(hipStreamSynchronize(st));
+    return download_vec(B, p, B.z, y);
+}
+
+static StrainArgs strain_args(Batch &B)
+{
+    StrainArgs s{};
+    s.coords = B.coords.as<double>();
+    s.econn = B.econn.as<int32_t>();
+    s.erow = B.erow.as<int32_t>();
+    s.eprob = B.eprob.as<int32_t>();
+    s.macro = B.has_macro ? B.macro.as<double>() : nullptr;
+    s.x64 = B.x64.as<double>();
+    s.nrhs = B.nrhs;
+    return s;
+}
+
+int amie_gpu_batch_element_strain(amie_gpu_batch *bb, int32_t p, int32_t rhs, double *strain)
+{
+    if (!bb || check_p(bb->b, p) || !strain) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    if (!B.has_elements || rhs < 0 || rhs >= B.nrhs) { set_error("no elements or bad rhs"); return AMIE_ERR_ARG; }
+    const ProbHost &P = B.ph[p];
+    DevBuf out;
+    if (!out.alloc((size_t)P.nelem * B.et.s * 8)) return AMIE_ERR_NOMEM;
+    StrainArgs s = strain_args(B);
+    const int n = (int)P.nelem;
+    dim3 gr((n + 255) / 256), bl(256);
+    hipStream_t st = B.stream;
+    if (B.et.et == 1) hipLaunchKernelGGL(k_strain<1>, gr, bl, 0, st, s, (int)P.elem0, n, rhs, out.as<double>());
+    else if (B.et.et == 2) hipLaunchKernelGGL(k_strain<2>, gr, bl, 0, st, s, (int)P.elem0, n, rhs, out.as<double>());
+    else hipLaunchKernelGGL(k_strain<3>, gr, bl, 0, st, s, (int)P.elem0, n, rhs, out.as<double>());
+    AMIE_HIP(hipGetLastError());
+    AMIE_HIP(hipStreamSynchronize(st));
+    AMIE_HIP(hipMemcpy(strain, out.p, (size_t)n * B.et.s * 8, hipMemcpyDeviceToHost));
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_average_stress(amie_gpu_batch *bb, double *outp)
+{
+    if (!bb || !bb->b || !outp) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
+    DevBuf out;
+    const size_t n = (size_t)B.nprob * B.nrhs * B.et.s;
+    if (!out.alloc(n * 8)) return AMIE_ERR_NOMEM;
+    StrainArgs s = strain_args(B);
+    hipStream_t st = B.stream;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(B.nprob), dim3(256), 0, st, s, B.prob_e0.as<int32_t>(), B.prob_ne.as<int32_t>(),
+                           B.ephase.as<int32_t>(), B.edamage.as<double>(), B.cmat_off.as<int32_t>(), B.cmat.as<double>(),
+                           out.as<double>());
+    };
+    if (B.et.et == 1) go(k_average_stress<1>);
+    else if (B.et.et == 2) go(k_average_stress<2>);
+    else go(k_average_stress<3>);
+    AMIE_HIP(hipGetLastError());
+    AMIE_HIP(hipStreamSynchronize(st));
+    AMIE_HIP(hipMemcpy(outp, out.p, n * 8, hipMemcpyDeviceToHost));
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_damage_step(amie_gpu_batch *bb, const amie_damage_params *prm, int64_t *n_grown)
+{
+    if (!bb || !bb->b || !prm) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
+    hipStream_t st = B.stream;
+    AMIE_HIP(hipMemsetAsync(B.grown.p, 0, 8, st));
+    StrainArgs s = strain_args(B);
+    const int n = (int)B.nelem;
+    dim3 gr((n + 255) / 256), bl(256);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, gr, bl, 0, st, s, n, B.ed0.as<double>(), B.ekappa.as<double>(), B.edamage.as<double>(),
+                           prm->kappa0, prm->kappaf, prm->dmax, B.grown.as<unsigned long long>());
+    };
+    if (B.et.et == 1) go(k_damage<1>);
+    else if (B.et.et == 2) go(k_damage<2>);
+    else go(k_damage<3>);
+    AMIE_HIP(hipGetLastError());
+    unsigned long long g = 0;
+    AMIE_HIP(hipMemcpyAsync(&g, B.grown.p, 8, hipMemcpyDeviceToHost, st));
+    AMIE_HIP(hipStreamSynchronize(st));
+    if (n_grown) *n_grown = (int64_t)g;
+    B.has_damage_state = true;
+    return B.assemble();
+}
+
+int amie_gpu_batch_get_damage(amie_gpu_batch *bb, int32_t p, double *d, double *kappa)
+{
+    if (!bb || check_p(bb->b, p)) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    const ProbHost &P = B.ph[p];
+    if (d) AMIE_HIP(hipMemcpy(d, B.edamage.as<double>() + P.elem0, P.nelem * 8, hipMemcpyDeviceToHost));
+    if (kappa) AMIE_HIP(hipMemcpy(kappa, B.ekappa.as<double>() + P.elem0, P.nelem * 8, hipMemcpyDeviceToHost));
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_set_damage(amie_gpu_batch *bb, int32_t p, const double *d)
+{
+    if (!bb || check_p(bb->b, p) || !d) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    const ProbHost &P = B.ph[p];
+    AMIE_HIP(hipMemcpy(B.edamage.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
+    AMIE_HIP(hipMemcpy(B.ed0.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_scale_bc(amie_gpu_batch *bb, double factor)
+{
+    if (!bb || !bb->b) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    std::vector<double> h(B.bcval.n / 8);
+    AMIE_HIP(hipMemcpy(h.data(), B.bcval.p, B.bcval.n, hipMemcpyDeviceToHost));
+    for (auto &v : h) v *= factor;
+    AMIE_HIP(hipMemcpy(B.bcval.p, h.data(), B.bcval.n, hipMemcpyHostToDevice));
+    return AMIE_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,150 @@
+// Host-side batch object: problems resident in HBM and the solve loop.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "amie_gpu.h"
+#include "device.hpp"
+
+struct amie_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    size_t total_mem = 0;
+};
+
+namespace amie {
+
+void set_error(const char *fmt, ...);
+bool hip_ok(hipError_t e, const char *what);
+#define AMIE_HIP(call)                                      \
+    do {                                                    \
+        if (!::amie::hip_ok((call), #call)) return AMIE_ERR_HIP; \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    bool alloc(size_t bytes);
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// Element types: 1 P1, 2 P2, 3 TET4
+struct EType {
+    int et, nper, dim, b, s;
+};
+EType etype_info(int et);
+
+struct Kern {
+    void (*k1)(SolveArgs, int);
+    void (*k2)(SolveArgs, int);
+    void (*cx)(SolveArgs);
+    void (*cr)(SolveArgs);
+    void (*norms)(SolveArgs);
+};
+
+// State of one asynchronous solve (begin / advance / poll / end).
+struct SolveRun {
+    amie_solve_opts o{};
+    bool mixed = false, bj = false;
+    int ce = 4;              // iterations between checkpoint opportunities (even: p buffers alternate)
+    int per_check = 8;       // checkpoint periods per host status check
+    Kern K{};
+    SolveArgs a{};
+    int g = 1;
+    hipGraphExec_t exec = nullptr;
+    std::vector<double> pbytes;
+    double active_bytes = 0.;
+    std::vector<hipEvent_t> tev;
+    int nt = 0;
+    std::vector<char> was_running;
+    int64_t periods = 0, max_periods = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr, ev_status = nullptr;
+    bool started = false, in_flight = false, finished = false;
+    amie_solve_stats stats{};
+};
+
+struct ProbHost {
+    int64_t nrows = 0;                // DOF nodes (block rows)
+    int64_t nelem = 0, nnodes = 0;
+    int32_t slice0 = 0, nslices = 0, tile0 = 0, ntiles = 0;
+    int64_t row0 = 0;                 // first internal global row
+    int64_t elem0 = 0, node0 = 0;     // offsets of this problem's elements / geometry nodes
+    int32_t cmat0 = 0;
+    std::vector<uint32_t> row_size, column_index;   // reference layout, original numbering
+    std::vector<int64_t> ref_slot;                  // reference block -> global SELL slot
+    std::vector<int32_t> iperm;                     // original row -> local internal row
+    std::vector<int64_t> elem_ref;                  // element (i,j) -> reference block index
+    bool has_elements = false;
+};
+
+struct Batch {
+    amie_gpu_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;    // every launch of this batch goes here
+    SolveRun run;
+    EType et{};
+    int nrhs = 1;
+    int nprob = 0;
+    std::vector<ProbHost> ph;
+    int64_t nslices = 0, ntiles = 0, nrows_pad = 0, nslots = 0, nelem = 0, nnodes_geo = 0;
+    bool has_elements = false, has_macro = false, has_damage_state = false;
+    std::vector<int32_t> colour_ptr;  // colour c elements: colour_elems[colour_ptr[c]..colour_ptr[c+1])
+
+    // SELL structure
+    DevBuf slice_ptr, slice_width, slice_rows, col, dslot;
+    DevBuf v64, v32;
+    DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active;
+    DevBuf prob, ticket, partial;
+    // vectors (8 bytes per entry, reinterpreted as float in mixed mode)
+    DevBuf x64, b64, fext, S, y, r, z, p0, p1, q, minv32, minv64;
+    DevBuf bcmask, bcval;
+    // elements
+    DevBuf coords, econn, erow, eslot, ephase, edamage, ekappa, ed0, eprob, cmat, cmat_off, macro, colour_elems,
+        prob_e0, prob_ne, grown;
+    std::vector<int32_t> h_active;
+    std::vector<ProbDev> h_prob;
+    ProbDev *h_prob_pinned = nullptr;
+    int32_t *h_nactive_pinned = nullptr;
+
+    // captured iteration chunks, keyed by solve variant
+    struct GraphEntry {
+        int key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+
+    ~Batch();
+    size_t bytes() const;
+    SolveArgs args(bool mixed) const;
+    int grid() const;
+    int assemble();
+    int solve(const amie_solve_opts &o, amie_solve_stats *st);
+    int solve_begin(const amie_solve_opts &o);
+    int solve_advance();
+    int solve_poll(bool wait, bool *finished);
+    int solve_end(amie_solve_stats *st);
+    int set_active_all();
+};
+
+int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, Batch **out);
+int batch_from_bcsr(amie_gpu_ctx *ctx, int stride, int64_t nrows, const uint32_t *row_size,
+                    const uint32_t *column_index, const double *array, const double *rhs, Batch **out);
+
+} // namespace amie
+
+struct amie_gpu_batch {
+    amie::Batch *b;
+};

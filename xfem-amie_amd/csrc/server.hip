@@ -1,0 +1,348 @@
+// Serving loop: continuous batching of solve requests with admission by HBM
+// footprint and device-resident problem state.
+//
+// Requests queue on the host.  Each step() (1) retires in-flight groups whose
+// solve finished (non-blocking event polls), (2) admits queued requests of
+// one class (element type, right-hand-side count, load kind) as a new group
+// while solve slots and the HBM budget allow, builds that group's batch
+// (pattern, layout, assembly on the device) and starts its solve on its own
+// stream, and (3) enqueues one more chunk of iterations for every in-flight
+// group.  Groups admitted at different times therefore run concurrently on
+// the device, each problem with its own scalars and reductions: a problem's
+// result does not depend on which other problems share its group.
+// Requests with a persistent key keep their batch (matrix, damage state and
+// solution) resident between requests: such a request runs one damage step
+// on the resident state and a solve.
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "batch.hpp"
+
+namespace amie {
+
+// deep copy of a problem description
+struct OwnedProblem {
+    amie_problem_desc d{};
+    std::vector<double> coords, cmat, damage, bc_vals, forces, macro;
+    std::vector<int32_t> conn, dofmap, phase;
+    std::vector<int64_t> bc_dofs;
+    int64_t footprint = 0;
+
+    void copy(const amie_problem_desc &s)
+    {
+        const EType et = etype_info(s.elem_type);
+        d = s;
+        const int S = et.s;
+        coords.assign(s.coords, s.coords + s.nnodes * et.dim);
+        conn.assign(s.conn, s.conn + s.nelem * et.nper);
+        cmat.assign(s.cmat, s.cmat + (size_t)s.nphase * S * S);
+        if (s.dofmap) dofmap.assign(s.dofmap, s.dofmap + s.nnodes);
+        if (s.phase) phase.assign(s.phase, s.phase + s.nelem);
+        if (s.damage) damage.assign(s.damage, s.damage + s.nelem);
+        if (s.nbc) {
+            bc_dofs.assign(s.bc_dofs, s.bc_dofs + s.nbc);
+            bc_vals.assign(s.bc_vals, s.bc_vals + s.nbc);
+        }
+        const int64_t nd = (s.dofmap ? s.ndofnodes : s.nnodes) * et.b;
+        if (s.forces) forces.assign(s.forces, s.forces + nd * s.nrhs);
+        if (s.macro_strain) macro.assign(s.macro_strain, s.macro_strain + (size_t)s.nrhs * S);
+        d.coords = coords.data();
+        d.conn = conn.data();
+        d.cmat = cmat.data();
+        d.dofmap = s.dofmap ? dofmap.data() : nullptr;
+        d.phase = s.phase ? phase.data() : nullptr;
+        d.damage = s.damage ? damage.data() : nullptr;
+        d.bc_dofs = s.nbc ? bc_dofs.data() : nullptr;
+        d.bc_vals = s.nbc ? bc_vals.data() : nullptr;
+        d.forces = s.forces ? forces.data() : nullptr;
+        d.macro_strain = s.macro_strain ? macro.data() : nullptr;
+        // HBM estimate: blocks per row of the element type, FP64 + FP32
+        // values, column index, 11 vectors, element data
+        const int64_t rows = s.dofmap ? s.ndofnodes : s.nnodes;
+        const int bpr = et.et == 1 ? 7 : (et.et == 2 ? 12 : 15);
+        const int64_t slots = rows * bpr * 5 / 4;
+        footprint = slots * (int64_t)(8 * et.b * et.b + 4 * et.b * et.b + 4) + rows * et.b * s.nrhs * 8 * 11 +
+                    s.nelem * (int64_t)(et.nper * et.nper * 4 + et.nper * 8 + 40) + s.nnodes * et.dim * 8 + (1 << 20);
+    }
+    int cls() const { return d.elem_type * 64 + d.nrhs * 2 + (d.macro_strain ? 1 : 0); }
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    uint64_t key = 0;
+    OwnedProblem prob;
+    double submit_ms = 0., start_ms = 0.;
+};
+
+struct Group {
+    std::unique_ptr<Batch> batch;
+    std::vector<Request> reqs;
+    hipStream_t stream = nullptr;
+    bool persistent = false;
+    size_t bytes = 0;
+};
+
+struct Server {
+    amie_gpu_ctx *ctx = nullptr;
+    size_t budget = 0, in_use = 0;
+    int max_slots = 1, used_slots = 0;
+    amie_solve_opts opts{};
+    uint64_t next_ticket = 1;
+    std::deque<Request> queue;
+    std::vector<std::unique_ptr<Group>> inflight;
+    std::map<uint64_t, std::unique_ptr<Group>> resident;   // persistent key -> idle resident batch
+    std::vector<amie_request_result> done;
+    std::map<uint64_t, std::vector<double>> solutions;
+    std::vector<hipStream_t> streams;
+    size_t next_stream = 0;
+
+    ~Server()
+    {
+        for (auto &g : inflight)
+            if (g->batch) (void)hipStreamSynchronize(g->batch->stream);
+        inflight.clear();
+        resident.clear();
+        for (auto s : streams) (void)hipStreamDestroy(s);
+    }
+
+    hipStream_t pick_stream()
+    {
+        hipStream_t s = streams[next_stream % streams.size()];
+        next_stream++;
+        return s;
+    }
+
+    int start_group(std::unique_ptr<Group> g, double now)
+    {
+        for (auto &r : g->reqs) r.start_ms = now;
+        int rc;
+        amie_solve_opts o = opts;
+        if (g->persistent) o.use_x0 = 1;
+        if ((rc = g->batch->solve_begin(o))) return rc;
+        if ((rc = g->batch->solve_advance())) return rc;
+        inflight.push_back(std::move(g));
+        return AMIE_OK;
+    }
+
+    int retire(std::unique_ptr<Group> &g, double now)
+    {
+        amie_solve_stats st{};
+        int rc = g->batch->solve_end(&st);
+        if (rc < 0 && rc != AMIE_ERR_NAN) return rc;
+        Batch &B = *g->batch;
+        for (size_t i = 0; i < g->reqs.size(); i++) {
+            const ProbDev &P = B.h_prob[i];
+            amie_request_result r{};
+            r.ticket = g->reqs[i].ticket;
+            r.status = P.status == ST_CONVERGED ? AMIE_OK : (P.status == ST_NAN ? AMIE_ERR_NAN : AMIE_NOT_CONVERGED);
+            r.iterations = (int32_t)P.iters;
+            double rel = 0.;
+            for (int k = 0; k < B.nrhs; k++) rel = std::max(rel, P.relres[k]);
+            r.relres = rel;
+            r.submit_ms = g->reqs[i].submit_ms;
+            r.start_ms = g->reqs[i].start_ms;
+            r.done_ms = now;
+            done.push_back(r);
+            // solution of the request (original order, nrhs columns)
+            const ProbHost &H = B.ph[i];
+            const int b = B.et.b, NR = B.nrhs;
+            const int64_t ndof = H.nrows * b;
+            std::vector<double> h((size_t)H.nslices * WAVE * b * NR), x((size_t)ndof * NR);
+            if (hipMemcpy(h.data(), B.x64.as<double>() + (size_t)H.row0 * b * NR, h.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+                return AMIE_ERR_HIP;
+            for (int k = 0; k < NR; k++)
+                for (int64_t d = 0; d < ndof; d++) x[(size_t)k * ndof + d] = h[((size_t)H.iperm[d / b] * NR + k) * b + d % b];
+            solutions[r.ticket] = std::move(x);
+        }
+        used_slots -= (int)g->reqs.size();
+        if (g->persistent) {
+            const uint64_t key = g->reqs[0].key;
+            g->reqs.clear();
+            resident[key] = std::move(g);   // stays in HBM, still counted in in_use
+        } else {
+            in_use -= g->bytes;
+            g.reset();
+        }
+        return AMIE_OK;
+    }
+
+    int step(double now, int *n_active, int *n_queued)
+    {
+        int rc;
+        // (1) retire finished groups
+        for (size_t i = 0; i < inflight.size();) {
+            bool fin = false;
+            if ((rc = inflight[i]->batch->solve_poll(false, &fin))) return rc;
+            if (fin) {
+                if ((rc = retire(inflight[i], now))) return rc;
+                inflight.erase(inflight.begin() + i);
+            } else {
+                i++;
+            }
+        }
+        // (2) admit: persistent requests whose key is resident and idle, then
+        // one group of the class at the head of the queue
+        for (size_t i = 0; i < queue.size();) {
+            Request &r = queue[i];
+            auto it = r.key ? resident.find(r.key) : resident.end();
+            if (it != resident.end() && used_slots < max_slots) {
+                std::unique_ptr<Group> g = std::move(it->second);
+                resident.erase(it);
+                g->reqs.push_back(std::move(r));
+                queue.erase(queue.begin() + i);
+                amie_damage_params prm{1e-4, 1e-3, 1. - 1e-6};
+                amie_gpu_batch wrap{g->batch.get()};
+                int64_t grown = 0;
+                if ((rc = amie_gpu_batch_damage_step(&wrap, &prm, &grown))) return rc;
+                used_slots += 1;
+                if ((rc = start_group(std::move(g), now))) return rc;
+            } else {
+                i++;
+            }
+        }
+        if (!queue.empty() && used_slots < max_slots) {
+            const int cls = queue.front().prob.cls();
+            const bool pers = queue.front().key != 0;
+            std::vector<Request> pick;
+            size_t bytes = 0;
+            for (size_t i = 0; i < queue.size() && used_slots + (int)pick.size() < max_slots;) {
+                Request &r = queue[i];
+                const bool resident_key = r.key && (resident.count(r.key) ||
+                                                    std::any_of(inflight.begin(), inflight.end(), [&](auto &g) {
+                                                        return g->persistent && !g->reqs.empty() && g->reqs[0].key == r.key;
+                                                    }));
+                if (r.prob.cls() != cls || (r.key != 0) != pers || resident_key) { i++; continue; }
+                if (in_use + bytes + (size_t)r.prob.footprint > budget) break;
+                bytes += (size_t)r.prob.footprint;
+                pick.push_back(std::move(r));
+                queue.erase(queue.begin() + i);
+                if (pers) break;   // a persistent problem keeps a batch of its own
+            }
+            if (!pick.empty()) {
+                std::unique_ptr<Group> g(new Group());
+                g->persistent = pers;
+                std::vector<amie_problem_desc> descs;
+                for (auto &r : pick) descs.push_back(r.prob.d);
+                Batch *B = nullptr;
+                if ((rc = batch_create(ctx, (int)descs.size(), descs.data(), &B))) return rc;
+                g->batch.reset(B);
+                B->stream = pick_stream();
+                g->bytes = B->bytes();
+                in_use += g->bytes;
+                used_slots += (int)pick.size();
+                g->reqs = std::move(pick);
+                if ((rc = start_group(std::move(g), now))) return rc;
+            }
+        }
+        // (3) one more chunk for every in-flight group
+        for (auto &g : inflight)
+            if ((rc = g->batch->solve_advance())) return rc;
+        if (n_active) *n_active = used_slots;
+        if (n_queued) *n_queued = (int)queue.size();
+        return AMIE_OK;
+    }
+};
+
+} // namespace amie
+
+using namespace amie;
+
+struct amie_gpu_server {
+    amie::Server *s;
+};
+
+extern "C" {
+
+int amie_gpu_server_create(amie_gpu_ctx *ctx, size_t hbm_budget, int32_t max_slots, const amie_solve_opts *opts,
+                           amie_gpu_server **out)
+{
+    if (!ctx || !out || max_slots <= 0 || hbm_budget == 0) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    AMIE_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<Server> S(new Server());
+    S->ctx = ctx;
+    S->budget = hbm_budget;
+    S->max_slots = max_slots;
+    amie_gpu_default_opts(&S->opts);
+    if (opts) S->opts = *opts;
+    S->opts.time_kernels = 0;
+    // default-priority streams, one per hardware queue
+    for (int i = 0; i < 4; i++) {
+        hipStream_t st;
+        AMIE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        S->streams.push_back(st);
+    }
+    *out = new amie_gpu_server{S.release()};
+    return AMIE_OK;
+}
+
+void amie_gpu_server_destroy(amie_gpu_server *s)
+{
+    if (!s) return;
+    delete s->s;
+    delete s;
+}
+
+int amie_gpu_server_submit(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t key, double submit_ms,
+                           uint64_t *ticket)
+{
+    if (!s || !desc) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    const EType et = etype_info(desc->elem_type);
+    if (!et.et || !desc->coords || !desc->conn || !desc->cmat) { set_error("bad problem"); return AMIE_ERR_ARG; }
+    Request r;
+    r.ticket = s->s->next_ticket++;
+    r.key = key;
+    r.submit_ms = submit_ms;
+    r.prob.copy(*desc);
+    if ((size_t)r.prob.footprint > s->s->budget) { set_error("request larger than the HBM budget"); return AMIE_ERR_NOMEM; }
+    if (ticket) *ticket = r.ticket;
+    s->s->queue.push_back(std::move(r));
+    return AMIE_OK;
+}
+
+int amie_gpu_server_step(amie_gpu_server *s, double now_ms, int32_t *n_active, int32_t *n_queued)
+{
+    if (!s) { set_error("server is NULL"); return AMIE_ERR_ARG; }
+    int a = 0, q = 0;
+    int rc = s->s->step(now_ms, &a, &q);
+    if (n_active) *n_active = a;
+    if (n_queued) *n_queued = q;
+    return rc;
+}
+
+int amie_gpu_server_poll(amie_gpu_server *s, amie_request_result *out, int32_t max, int32_t *n)
+{
+    if (!s || !n || (max > 0 && !out)) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    Server &S = *s->s;
+    const int k = std::min<int>(max, (int)S.done.size());
+    for (int i = 0; i < k; i++) out[i] = S.done[i];
+    // solutions of results handed out stay until the next poll
+    std::map<uint64_t, std::vector<double>> keep;
+    for (int i = 0; i < k; i++) {
+        auto it = S.solutions.find(out[i].ticket);
+        if (it != S.solutions.end()) keep[out[i].ticket] = std::move(it->second);
+    }
+    for (int i = k; i < (int)S.done.size(); i++) {
+        auto it = S.solutions.find(S.done[i].ticket);
+        if (it != S.solutions.end()) keep[S.done[i].ticket] = std::move(it->second);
+    }
+    S.solutions.swap(keep);
+    S.done.erase(S.done.begin(), S.done.begin() + k);
+    *n = k;
+    return AMIE_OK;
+}
+
+int amie_gpu_server_get_solution(amie_gpu_server *s, uint64_t ticket, double *x, int64_t n)
+{
+    if (!s || !x) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    auto it = s->s->solutions.find(ticket);
+    if (it == s->s->solutions.end()) { set_error("no solution for ticket %llu", (unsigned long long)ticket); return AMIE_ERR_ARG; }
+    if ((int64_t)it->second.size() != n) { set_error("solution has %zu entries", it->second.size()); return AMIE_ERR_ARG; }
+    std::memcpy(x, it->second.data(), n * 8);
+    return AMIE_OK;
+}
+
+} // extern "C"
