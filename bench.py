@@ -1,0 +1,162 @@
+"""Benchmark of the FE-solve hot path on one MI355X.
+
+Workload (one "step"): 512 periodic P1 RVEs (n ~ U{32..256}, seed 4, 2k-130k
+DOFs each, 20-45 % circular inclusions, 12/60 GPa), three macro-strain load
+cases each (1536 right-hand sides), resident in HBM.  A step runs the whole
+hot path on the device: element integration + coloured scatter + Dirichlet
+pin + scaling (assembly), the FP32 CG with FP64 refinement of all 1536
+right-hand sides to relative residual 1e-8 in one batch, and the
+volume-averaged stress (effective stiffness) of every RVE.
+
+value = RVE load-case solves per second (1536 per step / step time).
+roofline = the fused SpMV kernel (k_spmv_p), timed with HIP events on the
+library stream during the timed steps.
+cpu_baseline = the oracle's threaded FP64 Jacobi PCG (rtol 1e-8) on a bounded
+sample of the same RVEs.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "xfem-amie_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(problems, budget_s=15.0, rtol=1e-8):
+    """Oracle FP64 Jacobi PCG (OpenMP) on RVEs drawn in a seeded order until
+    the time budget is spent.  Returns solves/s and the sample description."""
+    import oracle as orc
+    order = np.random.default_rng(11).permutation(len(problems))
+    solved, t_solve, ndof = 0, 0.0, 0
+    used = []
+    for k in order:
+        p = problems[k]
+        A = orc.assemble_periodic(1, p.conn, p.dofmap, p.coords, p.cmats, p.phase)
+        loads = [orc.macro_rhs(1, p.conn, p.dofmap, p.coords, p.cmats, p.phase, p.macro[lc], A.nrows)
+                 for lc in range(p.nrhs)]
+        orc.apply_dirichlet(A, np.zeros(A.n), p.bc_dofs, p.bc_vals)
+        t0 = time.perf_counter()
+        for f in loads:
+            f[p.bc_dofs] = p.bc_vals
+            x, it, rr, ok = orc.pcg(A, f, rtol=rtol, maxit=200000)
+        t_solve += time.perf_counter() - t0
+        solved += len(loads)
+        ndof += A.n
+        used.append(int(np.sqrt(p.ndofnodes)))
+        log(f"cpu baseline: rve n={used[-1]} {len(loads)} solves, total {t_solve:.1f}s")
+        if t_solve >= budget_s:
+            break
+    threads = orc.get_threads()
+    return {"value": solved / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{len(used)} of the {len(problems)} RVEs (n = {sorted(used)}), {solved} load-case solves, "
+                      f"oracle FP64 Jacobi PCG to rel. res {rtol:g}, {threads} OpenMP threads, assembly excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nrve", type=int, default=512)
+    ap.add_argument("--nmax", type=int, default=256)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import amie_gpu as ag
+    import configs
+
+    t0 = time.perf_counter()
+    problems = configs.config4(args.nrve, seed=4, nmax=args.nmax)
+    ndof = sum(p.ndof for p in problems)
+    log(f"built {len(problems)} RVE meshes ({ndof} DOFs) in {time.perf_counter() - t0:.1f}s")
+    ctx = ag.Context(0)
+    t0 = time.perf_counter()
+    B = ag.Batch(ctx, problems)
+    log(f"batch resident in HBM: {B.nbytes / 2**30:.2f} GiB, setup {time.perf_counter() - t0:.1f}s")
+    nnzb = sum(B.shape(k)[1] for k in range(len(problems)))
+
+    def step(precision, time_kernels=1):
+        B.assemble()
+        st = B.solve(precision=precision, rtol=args.rtol, time_kernels=time_kernels, use_graph=0)
+        ceff = B.average_stress()
+        return st, ceff
+
+    for w in range(args.warmup):
+        st, _ = step(ag.MIXED)
+        log(f"warmup {w}: {st['solve_ms']:.1f} ms solve, iters max {st['iterations_max']}, status {st['status']}")
+    sts = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st, ceff = step(ag.MIXED)
+        sts.append(st)
+        log(f"step {k}: solve {st['solve_ms']:.1f} ms, iters max {st['iterations_max']}, relres max "
+            f"{st['relres_max']:.2e}, converged {st['nproblems_converged']}/{len(problems)}")
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    solves = len(problems) * problems[0].nrhs
+    value = solves * args.steps / elapsed
+    spmv_ms = sum(s["spmv_kernel_ms"] for s in sts)
+    spmv_bytes = sum(s["spmv_bytes"] for s in sts)
+    launches = sum(s["spmv_launches"] for s in sts)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
+    ok = all(s["status"] == 0 for s in sts)
+
+    fp64 = None
+    if not args.no_fp64:
+        t1 = time.perf_counter()
+        st64, _ = step(ag.FP64, time_kernels=0)
+        dt = time.perf_counter() - t1
+        fp64 = {"solves_per_s": solves / dt, "ms_per_step": 1e3 * dt, "iterations_max": st64["iterations_max"],
+                "status": st64["status"]}
+        log(f"fp64 batched step: {1e3 * dt:.1f} ms, iters max {st64['iterations_max']}")
+
+    cpu = None if args.no_cpu else cpu_baseline(problems, args.cpu_budget, args.rtol)
+    out = {
+        "metric": "RVE solves/s + p50/p99 latency @rel.res 1e-8; mixed vs FP64 CG; SpMV GB/s",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(problems)} periodic P1 RVEs n~U{{32..{args.nmax}}} (seed 4), 3 macro-strain "
+                               f"load cases each, FP32 CG + FP64 refinement to rel.res {args.rtol:g}, "
+                               f"assembly + solve + effective stiffness per step",
+                   "rves": len(problems), "load_cases": problems[0].nrhs, "dofs": ndof, "blocks": nnzb,
+                   "hbm_bytes": B.nbytes, "converged": ok},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_spmv_p (fused p-update + block SpMV + p.q)", "launches": launches,
+                     "avg_launch_us": 1e3 * spmv_ms / max(1, launches),
+                     "avg_bytes_per_launch": spmv_bytes / max(1, launches)},
+        "cpu_baseline": cpu,
+        "fp64_batched": fp64,
+        "mixed_vs_fp64": (value / fp64["solves_per_s"]) if fp64 else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -151,8 +151,20 @@ int amie_phase_by_centroid(int32_t elem_type, int64_t nelem, const int32_t *conn
  * ell, separable Gaussian filter of white noise), mapped affinely to
  * [lo, hi]; one value per tetrahedron of the 6-tet split. */
 int amie_damage_field_3d(uint64_t seed, int32_t n, double ell, double lo, double hi, double *d0);
+/* Element stiffness of one element on the host, with exactly the
+ * formulation the assembly kernel uses (Jinv at the centroid, reference Gauss
+ * rules, K_ij = sum_g w_g B_i C B_j^T): DelaunayTriangle /
+ * DelaunayTetrahedron::getElementaryMatrix, mesher/delaunay.cpp:2408-2535.
+ * coords: nper x dim in local order; Ke: (nper*b)^2, dof = node*b + axis. */
+int amie_element_matrix(int32_t elem_type, const double *coords, const double *C, double *Ke);
 /* Isotropic C matrix (utilities/tensor.cpp:1845-1931, plane stress in 2D). */
 void amie_cauchy_green(double E, double nu, int32_t dim, double *C);
+
+/* Batched element integration on the device (no assembly): element e has
+ * coordinates coords[e][nper][dim] and material cmat[e] (3x3 | 6x6);
+ * Ke[e] is (nper*b)^2 as amie_element_matrix. */
+int amie_gpu_element_matrices(amie_gpu_ctx *ctx, int32_t elem_type, int64_t nelem, const double *coords,
+                              const double *cmat, double *Ke);
 
 /* ------------------------------------------------------------- batches
  * A batch holds one or more problems resident in HBM: pattern, element->

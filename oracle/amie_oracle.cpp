@@ -345,6 +345,24 @@ double kahan_dot(const double *v0, const double *v1, int64_t n)
 
 extern "C" {
 
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int or_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 void or_cauchy_green(double p1, double p2, int dim, int plane_strain, double *C)
 {
     // utilities/tensor.cpp:1845-1931, YOUNG_POISSON branch

@@ -19,6 +19,12 @@
 extern "C" {
 #endif
 
+/* OpenMP threads of the oracle (the reference's Kahan dot products are
+ * reduced per thread, so bit-level agreement with golden vectors made with
+ * one thread needs one thread here). */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 /* utilities/tensor.cpp:1845-1931 (YOUNG_POISSON); dim 2 -> 3x3, 3 -> 6x6, row-major. */
 void or_cauchy_green(double E, double nu, int dim, int plane_strain, double *C);
 

@@ -26,6 +26,8 @@ def lib():
         L = ctypes.CDLL(path)
         i64, i32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
         vp = ctypes.c_void_p
+        L.or_set_threads.argtypes = [i32]
+        L.or_get_threads.restype = i32
         L.or_cauchy_green.argtypes = [dbl, dbl, i32, i32, vp]
         L.or_orient.argtypes = [i32, vp, vp]
         L.or_element_matrix.argtypes = [i32, vp, vp, vp]
@@ -52,6 +54,25 @@ def lib():
         L.or_average_stress.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp]
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    lib().or_set_threads(n)
+
+
+def get_threads():
+    return lib().or_get_threads()
+
+
+class one_thread:
+    """Context manager: run the oracle with one OpenMP thread."""
+
+    def __enter__(self):
+        self.n = get_threads()
+        set_threads(1)
+
+    def __exit__(self, *a):
+        set_threads(self.n)
 
 
 def _p(a):

@@ -127,3 +127,21 @@ def test_context_without_gpu_is_an_error():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
                        env={**os.environ, "HIP_VISIBLE_DEVICES": "-1"})
     assert "ERR" in r.stdout
+
+
+GOLD = os.path.join(ROOT, "tests", "golden", "ref")
+
+
+@pytest.mark.parametrize("tag,etype,E,nu", [("p1_a", ag.P1, 1e10, 0.2), ("p1_unit", ag.P1, 1e10, 0.2),
+                                            ("p2_a", ag.P2, 60e9, 0.2), ("p2_unit", ag.P2, 12e9, 0.3),
+                                            ("tet_a", ag.TET4, 30e9, 0.2), ("tet_b", ag.TET4, 30e9, 0.2)])
+def test_device_element_formulation_matches_reference(tag, etype, E, nu):
+    """The element routine the assembly kernel runs (host instantiation of the
+    same __host__ __device__ code) against the compiled reference's element
+    matrices (tests/golden/ref)."""
+    xy = np.load(os.path.join(GOLD, "Ke_%s_%s.npy" % (tag, "xyz" if etype == ag.TET4 else "xy")))
+    if etype != ag.P2:
+        xy = xy[np.load(os.path.join(GOLD, "Ke_%s_order.npy" % tag))]
+    K = ag.element_matrix(etype, xy, ag.cauchy_green(E, nu, dim=ag.DIM[etype]))
+    Kref = np.load(os.path.join(GOLD, "Ke_%s.npy" % tag))
+    assert np.abs(K - Kref).max() <= 1e-15 * np.abs(Kref).max()

@@ -109,7 +109,14 @@ def test_assembly_vs_oracle(ctx, make):
     orc.apply_dirichlet(A, b, p.bc_dofs, p.bc_vals)
     np.testing.assert_array_equal(rs, A.row_size)
     np.testing.assert_array_equal(ci, A.column_index)
-    assert np.abs(ar - A.array).max() <= 1e-14 * np.abs(A.array).max()
+    err = np.abs(ar - A.array)
+    if err.max() > 1e-14 * np.abs(A.array).max():
+        b2 = p.coords.shape[1] * (p.coords.shape[1] + p.coords.shape[1] % 2)
+        q = int(np.argmax(err)) // b2
+        row = int(np.searchsorted(np.cumsum(A.row_size.astype(np.int64)), q, side="right"))
+        print("worst block", q, "row", row, "col", int(A.column_index[q]), "gpu", ar[q * b2:(q + 1) * b2],
+              "oracle", A.array[q * b2:(q + 1) * b2], "nbad blocks", int((err.reshape(-1, b2).max(1) > 1).sum()))
+    assert err.max() <= 1e-14 * np.abs(A.array).max()
     assert np.abs(rhs - b).max() <= 1e-13 * np.abs(b).max()
 
 
@@ -307,3 +314,20 @@ def test_server_result_same_alone_and_in_mix(ctx):
     assert len(got) == len(probs)
     np.testing.assert_array_equal(got[tickets[1]], x_alone)
     S.close()
+
+
+@pytest.mark.parametrize("etype,n", [(ag.P1, 6), (ag.P2, 6), (ag.TET4, 3)])
+def test_device_element_matrices_match_host_and_oracle(ctx, etype, n):
+    print("case", etype, n, flush=True)
+    coords, conn = ag.structured_mesh(etype, n)
+    dim = ag.DIM[etype]
+    X = coords[conn]
+    rng = np.random.default_rng(5)
+    X = X + rng.uniform(-0.1, 0.1, X.shape) / n if etype != ag.P2 else X
+    C = np.stack([ag.cauchy_green(1e10 * (1 + e % 3), 0.2, dim=dim) for e in range(conn.shape[0])])
+    K = ag.element_matrices_gpu(ctx, etype, X, C)
+    for e in range(conn.shape[0]):
+        Kh = ag.element_matrix(etype, X[e], C[e])
+        Ko = orc.element_matrix(etype, X[e], C[e])
+        assert np.abs(K[e] - Kh).max() <= 1e-14 * np.abs(Kh).max(), e
+        assert np.abs(K[e] - Ko).max() <= 1e-14 * np.abs(Ko).max(), e

@@ -17,6 +17,13 @@ import oracle as orc
 REF = os.path.join(os.path.dirname(__file__), "golden", "ref")
 
 
+@pytest.fixture(autouse=True)
+def _one_thread():
+    # the golden vectors were written with one OpenMP thread
+    with orc.one_thread():
+        yield
+
+
 def g(name):
     return np.load(os.path.join(REF, name + ".npy"))
 

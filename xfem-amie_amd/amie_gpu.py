@@ -70,6 +70,8 @@ SIGNATURES = {
     "amie_phase_by_centroid": (c_i32, [c_i32, c_i64, vp, vp, c_i64, vp, vp]),
     "amie_damage_field_3d": (c_i32, [c_u64, c_i32, c_dbl, c_dbl, c_dbl, vp]),
     "amie_cauchy_green": (None, [c_dbl, c_dbl, c_i32, vp]),
+    "amie_element_matrix": (c_i32, [c_i32, vp, vp, vp]),
+    "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
     "amie_gpu_batch_create": (c_i32, [vp, c_i32, vp, vp]),
     "amie_gpu_batch_from_bcsr": (c_i32, [vp, c_i32, c_i64, vp, vp, vp, vp, vp]),
     "amie_gpu_batch_destroy": (None, [vp]),
@@ -139,6 +141,15 @@ def cauchy_green(E, nu, dim=2):
     C = np.zeros((s, s))
     lib().amie_cauchy_green(E, nu, dim, _p(C))
     return C
+
+
+def element_matrix(etype, coords, C):
+    coords = _c(coords, np.float64)
+    C = _c(C, np.float64)
+    n = NPER[etype] * DIM[etype]
+    K = np.zeros((n, n))
+    check(lib().amie_element_matrix(etype, _p(coords), _p(C), _p(K)))
+    return K
 
 
 def structured_mesh(etype, n):
@@ -251,6 +262,17 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def element_matrices_gpu(ctx, etype, coords, cmats):
+    """coords: nelem x nper x dim, cmats: nelem x s x s -> nelem x N x N."""
+    coords = _c(coords, np.float64)
+    cmats = _c(cmats, np.float64)
+    ne = coords.shape[0]
+    n = NPER[etype] * DIM[etype]
+    K = np.zeros((ne, n, n))
+    check(lib().amie_gpu_element_matrices(ctx.h, etype, ne, _p(coords), _p(cmats), _p(K)))
+    return K
 
 
 def default_opts(**kw):

@@ -17,6 +17,13 @@
 
 namespace amie {
 
+#define AMIE_HD __host__ __device__
+// The element formulas are evaluated without fused multiply-add contraction,
+// in the reference's operation order: its 2x2 inverse
+// (utilities/matrixops.cpp:536-561) branches on J[0][0] being exactly zero,
+// which a contracted P2 Jacobian sum would turn into a ~1e-17 residue.
+#define AMIE_NO_CONTRACT _Pragma("clang fp contract(off)")
+
 struct AsmArgs {
     const double *coords;     // geometry coordinates, dim per node
     const int32_t *econn;     // element -> geometry node (global)
@@ -39,8 +46,9 @@ template <> struct Elem<2> { static constexpr int NPER = 6, DIM = 2, B = 2, S = 
 template <> struct Elem<3> { static constexpr int NPER = 4, DIM = 3, B = 3, S = 6, NGP = 1; };
 
 // shape-function derivatives (see oracle: elements/elements.cpp:976-1061,2491-2530)
-template <int ET> __device__ inline void dshape(double x, double y, double *dxi, double *deta, double *dzeta)
+template <int ET> AMIE_HD inline void dshape(double x, double y, double *dxi, double *deta, double *dzeta)
 {
+    AMIE_NO_CONTRACT
     if constexpr (ET == 1) {
         dxi[0] = 0.; deta[0] = 1.;
         dxi[1] = -1.; deta[1] = -1.;
@@ -62,8 +70,9 @@ template <int ET> __device__ inline void dshape(double x, double y, double *dxi,
 
 // Jinv at the centroid (utilities/matrixops.cpp:536-561, :681-702) and its
 // determinant (:826-847).
-template <int ET> __device__ inline double centroid_jinv(const double (*X)[3], double *m)
+template <int ET> AMIE_HD inline double centroid_jinv(const double (*X)[3], double *m)
 {
+    AMIE_NO_CONTRACT
     using E = Elem<ET>;
     double dxi[6], deta[6], dzeta[6];
     if constexpr (E::DIM == 2) {
@@ -104,7 +113,7 @@ template <int ET> __device__ inline double centroid_jinv(const double (*X)[3], d
     }
 }
 
-template <int ET> __device__ inline void gauss(int g, double &x, double &y, double &w)
+template <int ET> AMIE_HD inline void gauss(int g, double &x, double &y, double &w)
 {
     if constexpr (ET == 1) { x = 0.333333333333333; y = x; w = 0.5; }
     else if constexpr (ET == 2) {
@@ -116,8 +125,9 @@ template <int ET> __device__ inline void gauss(int g, double &x, double &y, doub
 
 // Physical gradients G[g][node][dim] of the shape functions at every Gauss
 // point (with the centroid Jinv) and the Gauss weights w[g].
-template <int ET> __device__ inline void element_grads(const double (*X)[3], double (*G)[6][3], double *w)
+template <int ET> AMIE_HD inline void element_grads(const double (*X)[3], double (*G)[6][3], double *w)
 {
+    AMIE_NO_CONTRACT
     using E = Elem<ET>;
     double m[9];
     double detJinv = centroid_jinv<ET>(X, m);
@@ -140,8 +150,9 @@ template <int ET> __device__ inline void element_grads(const double (*X)[3], dou
 }
 
 // B_i C B_j^T for one Gauss point; out row-major b x b.
-template <int ET> __device__ inline void btcb(const double *gi, const double *gj, const double *C, double *out)
+template <int ET> AMIE_HD inline void btcb(const double *gi, const double *gj, const double *C, double *out)
 {
+    AMIE_NO_CONTRACT
     using E = Elem<ET>;
     constexpr int S = E::S, B = E::B;
     double Bi[B][S], Bj[S][B];
@@ -164,6 +175,64 @@ template <int ET> __device__ inline void btcb(const double *gi, const double *gj
             for (int l = 0; l < S; l++) rij = rij + Bi[r][l] * C[l * S + c];
             for (int k = 0; k < B; k++) out[r * B + k] += rij * Bj[c][k];
         }
+}
+
+// Dense element matrix (nper*b)^2, dof = local_node*b + axis: the exact
+// formulation the assembly kernel adds into the global matrix.
+template <int ET> AMIE_HD inline void element_block(const double (*G)[6][3], const double *w, const double *C, int i,
+                                                    int j, double *K)
+{
+    AMIE_NO_CONTRACT
+    using E = Elem<ET>;
+    constexpr int B = E::B;
+    double t[B * B], comp[B * B];
+    btcb<ET>(G[0][i], G[0][j], C, K);
+    for (int q = 0; q < B * B; q++) { K[q] *= w[0]; comp[q] = 0.; }
+    for (int g = 1; g < E::NGP; g++) {
+        btcb<ET>(G[g][i], G[g][j], C, t);
+        for (int q = 0; q < B * B; q++) {
+            double yk = t[q] * w[g] - comp[q];
+            double tt = K[q] + yk;
+            comp[q] = (tt - K[q]) - yk;
+            K[q] = tt;
+        }
+    }
+}
+
+template <int ET> AMIE_HD inline void element_matrix(const double (*X)[3], const double *C, double *Ke)
+{
+    using E = Elem<ET>;
+    constexpr int B = E::B, NP = E::NPER, N = NP * B;
+    double G[E::NGP][6][3], w[E::NGP];
+    element_grads<ET>(X, G, w);
+    for (int i = 0; i < NP; i++)
+        for (int j = 0; j <= i; j++) {
+            double K[B * B];
+            element_block<ET>(G, w, C, i, j, K);
+            for (int r = 0; r < B; r++)
+                for (int c = 0; c < B; c++) {
+                    Ke[(i * B + r) * N + j * B + c] = K[r * B + c];
+                    Ke[(j * B + c) * N + i * B + r] = K[r * B + c];
+                }
+        }
+}
+
+// Element matrices of a list of elements (no assembly): nelem x (nper*b)^2.
+template <int ET>
+__global__ __launch_bounds__(256) void k_element_matrices(int n, const double *coords, const double *cmat, double *Ke)
+{
+    using E = Elem<ET>;
+    constexpr int NP = E::NPER, D = E::DIM, S = E::S, N = NP * E::B;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double X[NP][3];
+    for (int i = 0; i < NP; i++) {
+        for (int d = 0; d < D; d++) X[i][d] = coords[((int64_t)e * NP + i) * D + d];
+        if (D == 2) X[i][2] = 0.;
+    }
+    double C[S * S];
+    for (int q = 0; q < S * S; q++) C[q] = cmat[(int64_t)e * S * S + q];
+    element_matrix<ET>(X, C, Ke + (int64_t)e * N * N);
 }
 
 template <int B> __device__ inline void add_block64(double *v64, int64_t ns, int64_t slot, const double *blk /*row-major*/)
@@ -232,18 +301,8 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a, const int32_t *elem
     const uint32_t *slots = a.eslot + (int64_t)e * NP * NP;
     for (int i = 0; i < NP; i++) {
         for (int j = 0; j <= i; j++) {
-            double K[B * B], t[B * B], comp[B * B];
-            btcb<ET>(G[0][i], G[0][j], C, K);
-            for (int q = 0; q < B * B; q++) { K[q] *= w[0]; comp[q] = 0.; }
-            for (int g = 1; g < E::NGP; g++) {
-                btcb<ET>(G[g][i], G[g][j], C, t);
-                for (int q = 0; q < B * B; q++) {
-                    double yk = t[q] * w[g] - comp[q];
-                    double tt = K[q] + yk;
-                    comp[q] = (tt - K[q]) - yk;
-                    K[q] = tt;
-                }
-            }
+            double K[B * B];
+            element_block<ET>(G, w, C, i, j, K);
             add_block64<B>(a.v64, a.nslots, slots[i * NP + j], K);
             double KT[B * B];
             for (int r = 0; r < B; r++)
@@ -274,7 +333,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a, const int32_t *elem
 // column order); a constrained row becomes a unit row with b_i = g_i.
 template <int B, int NR>
 __global__ __launch_bounds__(WG) void k_apply_bc(SolveArgs a, double *v64, double *b64, const uint8_t *bcmask,
-                                                 const double *bcval, int ntiles)
+                                                 const double *bcval, const int32_t *row_len, int ntiles)
 {
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -285,7 +344,9 @@ __global__ __launch_bounds__(WG) void k_apply_bc(SolveArgs a, double *v64, doubl
         bool rfix[B];
         bool any_r = false;
         for (int i = 0; i < B; i++) { rfix[i] = bcmask[row * B + i]; any_r |= rfix[i]; }
-        const int w = a.slice_width[s];
+        // the row's own blocks only: the zero padding blocks of a SELL slice
+        // (column = the row itself) must stay zero
+        const int w = row_len[row];
         const int64_t sp = a.slice_ptr[s] + lane;
         for (int k = 0; k < w; k++) {
             const int64_t slot = sp + (int64_t)k * WAVE;
@@ -406,6 +467,7 @@ struct StrainArgs {
 template <int ET>
 __device__ inline void element_strain(const StrainArgs &a, int e, int k, double *eps)
 {
+    AMIE_NO_CONTRACT
     using E = Elem<ET>;
     constexpr int NP = E::NPER, D = E::DIM, B = E::B;
     double X[NP][3];

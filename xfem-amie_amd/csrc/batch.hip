@@ -74,7 +74,7 @@ Batch::~Batch()
 
 size_t Batch::bytes() const
 {
-    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &v64, &v32, &tile_slice0,
+    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &row_len, &v64, &v32, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &prob, &ticket, &partial, &x64, &b64,
                            &fext, &S, &y, &r, &z, &p0, &p1, &q, &minv32, &minv64, &bcmask, &bcval, &coords,
                            &econn, &erow, &eslot, &ephase, &edamage, &ekappa, &ed0, &eprob, &cmat, &cmat_off,
@@ -239,6 +239,7 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
     std::vector<int32_t> h_width(slices), h_rows(slices), h_ts0(tiles), h_tns(tiles), h_tprob(tiles);
     std::vector<uint32_t> h_col(slots);
     std::vector<int64_t> h_dslot(B.nrows_pad);
+    std::vector<int32_t> h_rowlen(B.nrows_pad, 0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int p = 0; p < B.nprob; p++) {
         ProbHost &P = B.ph[p];
@@ -258,6 +259,7 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
                 int k = 0;
                 if (ir < P.nrows) {
                     const int32_t orow = T.perm[ir];
+                    h_rowlen[grow] = (int32_t)P.row_size[orow];
                     for (int64_t qb = T.row_start[orow]; qb < T.row_start[orow + 1]; qb++, k++) {
                         const int64_t slot = sl + (int64_t)k * WAVE + l;
                         const uint32_t c = P.column_index[qb];
@@ -283,6 +285,7 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
     if ((rc = alloc_and_upload(B.slice_rows, h_rows.data(), h_rows.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.col, h_col.data(), h_col.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.dslot, h_dslot.data(), h_dslot.size() * 8))) return rc;
+    if ((rc = alloc_and_upload(B.row_len, h_rowlen.data(), h_rowlen.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.tile_slice0, h_ts0.data(), h_ts0.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.tile_nslices, h_tns.data(), h_tns.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.tile_prob, h_tprob.data(), h_tprob.size() * 4))) return rc;
@@ -539,10 +542,10 @@ template <int Bk> static void launch_post(Batch &B)
     if (B.has_elements) {
         if (B.nrhs == 1)
             hipLaunchKernelGGL((k_apply_bc<Bk, 1>), dim3(g), dim3(WG), 0, st, a, B.v64.as<double>(), B.b64.as<double>(),
-                               B.bcmask.as<uint8_t>(), B.bcval.as<double>(), (int)B.ntiles);
+                               B.bcmask.as<uint8_t>(), B.bcval.as<double>(), B.row_len.as<int32_t>(), (int)B.ntiles);
         else
             hipLaunchKernelGGL((k_apply_bc<Bk, 3>), dim3(g), dim3(WG), 0, st, a, B.v64.as<double>(), B.b64.as<double>(),
-                               B.bcmask.as<uint8_t>(), B.bcval.as<double>(), (int)B.ntiles);
+                               B.bcmask.as<uint8_t>(), B.bcval.as<double>(), B.row_len.as<int32_t>(), (int)B.ntiles);
     }
     hipLaunchKernelGGL((k_diag_scale<Bk>), dim3(g), dim3(WG), 0, st, a, B.dslot.as<int64_t>(), B.S.as<double>(), (int)B.ntiles);
     hipLaunchKernelGGL((k_scale_cast<float, Bk>), dim3(g), dim3(WG), 0, st, a, B.v32.p, B.dslot.as<int64_t>(),
@@ -652,6 +655,7 @@ int Batch::solve_begin(const amie_solve_opts &o)
             P.rr_ckpt[k] = DBL_MAX;
             P.bb[k] = P.bb64[k] = 0.;
             P.relres[k] = 0.;
+            P.rel_prev[k] = DBL_MAX;
             P.done[k] = k < nrhs ? 0 : 1;
         }
     }
@@ -880,6 +884,20 @@ extern "C" {
 
 const char *amie_gpu_last_error(void) { return amie::last_error(); }
 
+int amie_element_matrix(int32_t etype, const double *coords, const double *C, double *Ke)
+{
+    if (!coords || !C || !Ke) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    const EType et = etype_info(etype);
+    if (!et.et) { set_error("bad element type"); return AMIE_ERR_ARG; }
+    double X[6][3] = {};
+    for (int i = 0; i < et.nper; i++)
+        for (int d = 0; d < et.dim; d++) X[i][d] = coords[i * et.dim + d];
+    if (etype == AMIE_P1) element_matrix<1>(X, C, Ke);
+    else if (etype == AMIE_P2) element_matrix<2>(X, C, Ke);
+    else element_matrix<3>(X, C, Ke);
+    return AMIE_OK;
+}
+
 int amie_gpu_create(amie_gpu_ctx **ctx, int device)
 {
     if (!ctx) { set_error("ctx is NULL"); return AMIE_ERR_ARG; }
@@ -935,6 +953,30 @@ void amie_gpu_default_opts(amie_solve_opts *o)
     o->use_graph = 1;
     o->chunk = 32;
     o->time_kernels = 0;
+}
+
+int amie_gpu_element_matrices(amie_gpu_ctx *ctx, int32_t etype, int64_t nelem, const double *coords, const double *cmat,
+                              double *Ke)
+{
+    if (!ctx || !coords || !cmat || !Ke || nelem <= 0 || nelem > INT32_MAX) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    const EType et = etype_info(etype);
+    if (!et.et) { set_error("bad element type"); return AMIE_ERR_ARG; }
+    AMIE_HIP(hipSetDevice(ctx->device));
+    const size_t nc = (size_t)nelem * et.nper * et.dim, ncm = (size_t)nelem * et.s * et.s;
+    const size_t nk = (size_t)nelem * et.nper * et.b * et.nper * et.b;
+    DevBuf dc, dm, dk;
+    if (!dc.alloc(nc * 8) || !dm.alloc(ncm * 8) || !dk.alloc(nk * 8)) return AMIE_ERR_NOMEM;
+    AMIE_HIP(hipMemcpy(dc.p, coords, nc * 8, hipMemcpyHostToDevice));
+    AMIE_HIP(hipMemcpy(dm.p, cmat, ncm * 8, hipMemcpyHostToDevice));
+    const int n = (int)nelem;
+    dim3 gr((n + 255) / 256), bl(256);
+    if (etype == AMIE_P1) hipLaunchKernelGGL(k_element_matrices<1>, gr, bl, 0, ctx->stream, n, dc.as<double>(), dm.as<double>(), dk.as<double>());
+    else if (etype == AMIE_P2) hipLaunchKernelGGL(k_element_matrices<2>, gr, bl, 0, ctx->stream, n, dc.as<double>(), dm.as<double>(), dk.as<double>());
+    else hipLaunchKernelGGL(k_element_matrices<3>, gr, bl, 0, ctx->stream, n, dc.as<double>(), dm.as<double>(), dk.as<double>());
+    AMIE_HIP(hipGetLastError());
+    AMIE_HIP(hipStreamSynchronize(ctx->stream));
+    AMIE_HIP(hipMemcpy(Ke, dk.p, nk * 8, hipMemcpyDeviceToHost));
+    return AMIE_OK;
 }
 
 int amie_gpu_batch_create(amie_gpu_ctx *ctx, int32_t nprob, const amie_problem_desc *probs, amie_gpu_batch **out)

@@ -104,7 +104,7 @@ struct Batch {
     std::vector<int32_t> colour_ptr;  // colour c elements: colour_elems[colour_ptr[c]..colour_ptr[c+1])
 
     // SELL structure
-    DevBuf slice_ptr, slice_width, slice_rows, col, dslot;
+    DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
     DevBuf v64, v32;
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active;
     DevBuf prob, ticket, partial;

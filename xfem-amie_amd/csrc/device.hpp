@@ -48,7 +48,7 @@ struct ProbDev {
     int32_t status, ckpt_pending;
     int64_t iters, nckpt;
     double rho[MAXRHS], rho_prev[MAXRHS], alpha[MAXRHS], beta[MAXRHS];
-    double rr[MAXRHS], rr_ckpt[MAXRHS], bb[MAXRHS], bb64[MAXRHS], relres[MAXRHS];
+    double rr[MAXRHS], rr_ckpt[MAXRHS], bb[MAXRHS], bb64[MAXRHS], relres[MAXRHS], rel_prev[MAXRHS];
     int32_t done[MAXRHS];
 };
 
@@ -507,7 +507,12 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
                     continue;
                 }
                 all = false;
-                bool restart = Pw.refine == 1 || Pw.nckpt == 0;
+                // restart the recurrences (beta = 0) when the FP64 residual did
+                // not halve since the previous checkpoint: the search directions
+                // kept across a residual replacement have lost conjugacy
+                const bool stalled = Pw.nckpt > 0 && rel > 0.5 * Pw.rel_prev[k];
+                Pw.rel_prev[k] = rel;
+                bool restart = Pw.refine == 1 || Pw.nckpt == 0 || stalled;
                 Pw.beta[k] = (restart || Pw.rho_prev[k] == 0.) ? 0. : rho / Pw.rho_prev[k];
                 Pw.rho[k] = rho;
                 Pw.rr[k] = rr;
@@ -566,8 +571,8 @@ __global__ __launch_bounds__(WG) void k_rhs_norms(SolveArgs a)
 }
 
 // ------------------------------------------------- plain SpMV (test hook)
-// y = K x with the FP64 planes (mode 0) or S^-1 (Ks (S x)) with the scaled
-// working planes (mode 1).  One lane per row, every tile.
+// y = K x with the FP64 planes (mode 0) or S^-1 (Ks (S^-1 x)) with the
+// scaled working planes Ks = S K S (mode 1).  One lane per row, every tile.
 template <typename W, int B, int NR>
 __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const double *x, double *yout, int ntiles)
 {
@@ -598,7 +603,7 @@ __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const 
                 for (int i = 0; i < B; i++) acc[k][i] = 0;
             auto src = [&](uint32_t c, int k, W *xv) {
 #pragma unroll
-                for (int j = 0; j < B; j++) xv[j] = (W)(a.S[(size_t)c * B + j] * x[((size_t)c * NR + k) * B + j]);
+                for (int j = 0; j < B; j++) xv[j] = (W)(x[((size_t)c * NR + k) * B + j] / a.S[(size_t)c * B + j]);
             };
             sell_row<W, B, NR>(a, a.vw, s, lane, acc, src);
             if (lane < a.slice_rows[s])
