@@ -51,7 +51,11 @@ enum {
 
 enum { AMIE_P1 = 1, AMIE_P2 = 2, AMIE_TET4 = 3 };
 
-enum { AMIE_FP64 = 0, AMIE_MIXED = 1 };               /* solve precision */
+/* solve precision: AMIE_FP64 = FP64 matrix and vectors; AMIE_MIXED = FP32
+ * matrix, vectors and SpMV inside, FP64 residual / update / scalars outside
+ * (problems whose FP32 iteration stagnates continue in AMIE_FP64 when
+ * opts.escalate is set); AMIE_MIXED64 = FP32 matrix with FP64 vectors. */
+enum { AMIE_FP64 = 0, AMIE_MIXED = 1, AMIE_MIXED64 = 2 };
 enum { AMIE_JACOBI = 0, AMIE_BLOCK_JACOBI = 1 };      /* preconditioner */
 enum { AMIE_REFINE_REPLACE = 0, AMIE_REFINE_RESTART = 1 };
 
@@ -93,11 +97,14 @@ typedef struct {
     int32_t refine;          /* AMIE_REFINE_REPLACE | AMIE_REFINE_RESTART */
     double rtol;             /* stop on ||b - K x|| / ||b|| <= rtol (FP64 residual) */
     double refine_drop;      /* FP32 residual drop that triggers an FP64 refinement (mixed) */
+    int64_t ckpt_interval;   /* mixed: FP64 refinement at least every this many iterations (<= 0: 128) */
     int64_t maxit;           /* per right-hand side, <= 0: ndof */
     int32_t use_x0;          /* 1: start from the current device solution */
     int32_t use_graph;       /* 1: replay captured hipGraphs of iteration chunks */
     int32_t chunk;           /* iterations per chunk between host status checks */
     int32_t time_kernels;    /* 1: time every SpMV-kernel launch with HIP events */
+    int32_t escalate;        /* AMIE_MIXED: continue stagnating problems in AMIE_FP64 (default 1) */
+    int32_t max_grid;        /* > 0: cap on workgroups per launch (0: 16 per CU) */
 } amie_solve_opts;
 
 typedef struct {
@@ -111,6 +118,7 @@ typedef struct {
     double spmv_kernel_ms;   /* summed HIP-event time of the fused SpMV kernel (time_kernels) */
     int64_t spmv_launches;
     double spmv_bytes;       /* algorithmic bytes per SpMV launch summed over launches */
+    int32_t nproblems_escalated; /* problems continued in AMIE_FP64 */
 } amie_solve_stats;
 
 /* Damage law of the incremental-loading case (Mazars tension branch with
@@ -183,6 +191,11 @@ size_t amie_gpu_batch_bytes(const amie_gpu_batch *b);
  * all on the device (re-run after damage changes). */
 int amie_gpu_batch_assemble(amie_gpu_batch *b);
 int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_solve_stats *stats);
+/* Outcome of problem p's last solve: status (AMIE_OK, AMIE_NOT_CONVERGED,
+ * AMIE_ERR_NAN), iterations, refinements and the largest final relative
+ * residual over its right-hand sides. */
+int amie_gpu_batch_problem_info(amie_gpu_batch *b, int32_t p, int32_t *status, int64_t *iterations,
+                                int64_t *refinements, double *relres);
 /* Solution of problem p, original DOF order, nrhs columns (ndof each). */
 int amie_gpu_batch_get_solution(amie_gpu_batch *b, int32_t p, double *x);
 int amie_gpu_batch_set_solution(amie_gpu_batch *b, int32_t p, const double *x);

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libamie_gpu.so")
 
 P1, P2, TET4 = 1, 2, 3
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED64 = 0, 1, 2
 JACOBI, BLOCK_JACOBI = 0, 1
 REFINE_REPLACE, REFINE_RESTART = 0, 1
 NPER = {P1: 3, P2: 6, TET4: 4}
@@ -33,14 +33,16 @@ class ProblemDesc(ctypes.Structure):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", c_i32), ("precond", c_i32), ("refine", c_i32), ("rtol", c_dbl),
-                ("refine_drop", c_dbl), ("maxit", c_i64), ("use_x0", c_i32), ("use_graph", c_i32),
-                ("chunk", c_i32), ("time_kernels", c_i32)]
+                ("refine_drop", c_dbl), ("ckpt_interval", c_i64), ("maxit", c_i64), ("use_x0", c_i32), ("use_graph", c_i32),
+                ("chunk", c_i32), ("time_kernels", c_i32), ("escalate", c_i32),
+                ("max_grid", c_i32)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("status", c_i32), ("nproblems_converged", c_i32), ("iterations_max", c_i64),
                 ("iterations_sum", c_i64), ("refinements", c_i64), ("relres_max", c_dbl), ("solve_ms", c_dbl),
-                ("spmv_kernel_ms", c_dbl), ("spmv_launches", c_i64), ("spmv_bytes", c_dbl)]
+                ("spmv_kernel_ms", c_dbl), ("spmv_launches", c_i64), ("spmv_bytes", c_dbl),
+                ("nproblems_escalated", c_i32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -78,6 +80,7 @@ SIGNATURES = {
     "amie_gpu_batch_bytes": (ctypes.c_size_t, [vp]),
     "amie_gpu_batch_assemble": (c_i32, [vp]),
     "amie_gpu_batch_solve": (c_i32, [vp, vp, vp]),
+    "amie_gpu_batch_problem_info": (c_i32, [vp, c_i32, vp, vp, vp, vp]),
     "amie_gpu_batch_get_solution": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_set_solution": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_get_nnzb": (c_i32, [vp, c_i32, vp, vp]),
@@ -334,8 +337,15 @@ class Batch:
         o = default_opts(**kw)
         st = SolveStats()
         rc = lib().amie_gpu_batch_solve(self.h, ctypes.byref(o), ctypes.byref(st))
-        check(rc, allow=(0, 1))
+        # a NaN or a non-converged problem is reported in the stats (status)
+        check(rc, allow=(0, 1, -4))
         return st.as_dict()
+
+    def problem_info(self, p=0):
+        st, it, ck, rr = c_i32(), c_i64(), c_i64(), c_dbl()
+        check(lib().amie_gpu_batch_problem_info(self.h, p, ctypes.byref(st), ctypes.byref(it), ctypes.byref(ck),
+                                                ctypes.byref(rr)))
+        return {"status": st.value, "iterations": it.value, "refinements": ck.value, "relres": rr.value}
 
     def solution(self, p=0):
         nr, _ = self.shape(p)

@@ -605,7 +605,7 @@ __global__ __launch_bounds__(256) void k_average_stress(StrainArgs a, const int3
             }
             acc[S] += meas;
         }
-        wg_sum<S + 1>(acc, lds);
+        wg_sum_all<S + 1>(acc, lds);
         if (threadIdx.x == 0)
             for (int r = 0; r < S; r++) out[((int64_t)p * a.nrhs + k) * S + r] = acc[r] / acc[S];
         __syncthreads();

@@ -75,7 +75,7 @@ Batch::~Batch()
 size_t Batch::bytes() const
 {
     const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &row_len, &v64, &v32, &tile_slice0,
-                           &tile_nslices, &tile_prob, &active, &n_active, &prob, &ticket, &partial, &x64, &b64,
+                           &tile_nslices, &tile_prob, &active, &n_active, &prob, &part1, &part2, &partc, &partn, &x64, &b64,
                            &fext, &S, &y, &r, &z, &p0, &p1, &q, &minv32, &minv64, &bcmask, &bcval, &coords,
                            &econn, &erow, &eslot, &ephase, &edamage, &ekappa, &ed0, &eprob, &cmat, &cmat_off,
                            &macro, &colour_elems, &prob_e0, &prob_ne, &grown};
@@ -84,8 +84,10 @@ size_t Batch::bytes() const
     return s;
 }
 
-SolveArgs Batch::args(bool mixed) const
+SolveArgs Batch::args(int mode) const
 {
+    const bool m32 = mode == AMIE_MIXED || mode == AMIE_MIXED64;   // FP32 matrix planes
+    const bool w32 = mode == AMIE_MIXED;                           // FP32 vectors
     SolveArgs a{};
     a.slice_ptr = slice_ptr.as<int64_t>();
     a.slice_width = slice_width.as<int32_t>();
@@ -93,15 +95,17 @@ SolveArgs Batch::args(bool mixed) const
     a.col = col.as<uint32_t>();
     a.nslots = nslots;
     a.v64 = v64.as<double>();
-    a.vw = mixed ? v32.p : v64.p;
+    a.vw = m32 ? v32.p : v64.p;
     a.tile_slice0 = tile_slice0.as<int32_t>();
     a.tile_nslices = tile_nslices.as<int32_t>();
     a.tile_prob = tile_prob.as<int32_t>();
     a.active = active.as<int32_t>();
     a.n_active = n_active.as<int32_t>();
     a.prob = prob.as<ProbDev>();
-    a.ticket = ticket.as<unsigned>();
-    a.partial = partial.as<double>();
+    a.part1 = part1.as<double>();
+    a.part2 = part2.as<double>();
+    a.partc = partc.as<double>();
+    a.partn = partn.as<double>();
     a.x64 = x64.as<double>();
     a.b64 = b64.as<double>();
     a.S = S.as<double>();
@@ -111,15 +115,17 @@ SolveArgs Batch::args(bool mixed) const
     a.p0 = p0.p;
     a.p1 = p1.p;
     a.q = q.p;
-    a.minv = mixed ? minv32.p : minv64.p;
+    a.minv = w32 ? minv32.p : minv64.p;
     return a;
 }
 
-int Batch::grid() const
+int Batch::grid(int64_t tiles) const
 {
-    // a multiple of the 8 XCDs, at most 8 workgroups per CU
-    int g = std::max(1, ctx->num_cu) * 8;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
+    // one tile per workgroup up to 16 workgroups per CU; beyond that the
+    // workgroups loop over their XCD's tile range
+    if (tiles < 0) tiles = ntiles;
+    const int64_t cap = run.o.max_grid > 0 ? run.o.max_grid : (int64_t)std::max(1, ctx->num_cu) * 16;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(cap, tiles));
 }
 
 int Batch::set_active_all()
@@ -186,9 +192,10 @@ static int alloc_vectors(Batch &B)
     AMIE_HIP(hipMemset(B.v32.p, 0, B.v32.n));
     // solver bookkeeping
     if (!B.prob.alloc(B.nprob * sizeof(ProbDev))) return AMIE_ERR_NOMEM;
-    if (!B.ticket.alloc(B.nprob * sizeof(unsigned))) return AMIE_ERR_NOMEM;
-    AMIE_HIP(hipMemset(B.ticket.p, 0, B.ticket.n));
-    if (!B.partial.alloc((size_t)B.ntiles * MAXQ * 8)) return AMIE_ERR_NOMEM;
+    for (DevBuf *pb : {&B.part1, &B.part2, &B.partc, &B.partn}) {
+        if (!pb->alloc((size_t)B.ntiles * MAXQ * 8)) return AMIE_ERR_NOMEM;
+        AMIE_HIP(hipMemset(pb->p, 0, pb->n));
+    }
     if (!B.active.alloc(B.ntiles * 4)) return AMIE_ERR_NOMEM;
     if (!B.n_active.alloc(4)) return AMIE_ERR_NOMEM;
     AMIE_HIP(hipHostMalloc((void **)&B.h_prob_pinned, B.nprob * sizeof(ProbDev), hipHostMallocDefault));
@@ -536,7 +543,7 @@ template <int ET, int NR> static void launch_assemble(Batch &B, const AsmArgs &a
 
 template <int Bk> static void launch_post(Batch &B)
 {
-    SolveArgs a = B.args(true);
+    SolveArgs a = B.args(AMIE_MIXED);
     const int g = B.grid();
     hipStream_t st = B.stream;
     if (B.has_elements) {
@@ -586,25 +593,23 @@ int Batch::assemble()
 }
 
 // -------------------------------------------------------------- solve
-template <typename W, int Bk, int NR, bool BJ> static Kern kern_for()
+template <typename W, typename M, int Bk, int NR, bool BJ> static Kern kern_for()
 {
-    return {k_spmv_p<W, Bk, NR, BJ>, k_update<W, Bk, NR, BJ>, k_ckpt_x<W, Bk, NR>, k_ckpt_r<W, Bk, NR, BJ>,
-            k_rhs_norms<Bk, NR>};
+    return {k_spmv_p<W, M, Bk, NR, BJ>, k_update<W, Bk, NR, BJ>, k_ckpt_x<W, Bk, NR>, k_ckpt_r<W, Bk, NR, BJ>,
+            k_rhs_norms<Bk, NR>, k_norms_final<NR>};
 }
 
-template <typename W, int Bk, int NR> static Kern kern_bj(bool bj)
+template <typename W, typename M, int Bk> static Kern kern_nr(int nr, bool bj)
 {
-    return bj ? kern_for<W, Bk, NR, true>() : kern_for<W, Bk, NR, false>();
+    if (nr == 1) return bj ? kern_for<W, M, Bk, 1, true>() : kern_for<W, M, Bk, 1, false>();
+    return bj ? kern_for<W, M, Bk, 3, true>() : kern_for<W, M, Bk, 3, false>();
 }
 
-static Kern select_kern(bool mixed, int b, int nr, bool bj)
+static Kern select_kern(int mode, int b, int nr, bool bj)
 {
-    if (mixed) {
-        if (b == 2) return nr == 1 ? kern_bj<float, 2, 1>(bj) : kern_bj<float, 2, 3>(bj);
-        return nr == 1 ? kern_bj<float, 3, 1>(bj) : kern_bj<float, 3, 3>(bj);
-    }
-    if (b == 2) return nr == 1 ? kern_bj<double, 2, 1>(bj) : kern_bj<double, 2, 3>(bj);
-    return nr == 1 ? kern_bj<double, 3, 1>(bj) : kern_bj<double, 3, 3>(bj);
+    if (mode == AMIE_MIXED) return b == 2 ? kern_nr<float, float, 2>(nr, bj) : kern_nr<float, float, 3>(nr, bj);
+    if (mode == AMIE_MIXED64) return b == 2 ? kern_nr<double, float, 2>(nr, bj) : kern_nr<double, float, 3>(nr, bj);
+    return b == 2 ? kern_nr<double, double, 2>(nr, bj) : kern_nr<double, double, 3>(nr, bj);
 }
 
 static void launch1(void (*k)(SolveArgs, int), int g, hipStream_t st, const SolveArgs &a, int par)
@@ -618,13 +623,14 @@ static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs
 
 // algorithmic bytes of one fused SpMV launch over problem p (values + block
 // column indices + row pointer + 4 vector streams per right-hand side)
-static double spmv_bytes(const Batch &B, const ProbHost &P, bool mixed)
+static double spmv_bytes(const Batch &B, const ProbHost &P, int mode)
 {
-    const double s = mixed ? 4. : 8.;
+    const double sm = mode == AMIE_FP64 ? 8. : 4.;     // matrix values
+    const double sv = mode == AMIE_MIXED ? 4. : 8.;    // vectors
     const double b = B.et.b;
     const double nnzb = (double)P.column_index.size();
     const double n = (double)P.nrows * b;
-    return nnzb * (b * b * s + 4.) + 4. * (P.nrows + 1) + 4. * n * s * B.nrhs;
+    return nnzb * (b * b * sm + 4.) + 4. * (P.nrows + 1) + 4. * n * sv * B.nrhs;
 }
 
 int Batch::solve_begin(const amie_solve_opts &o)
@@ -632,49 +638,79 @@ int Batch::solve_begin(const amie_solve_opts &o)
     SolveRun &R = run;
     R = SolveRun();
     R.o = o;
-    R.mixed = o.precision == AMIE_MIXED;
+    R.iters_done.assign(nprob, 0);
+    R.started = true;
+    hipStream_t s = stream;
+    if (!o.use_x0) AMIE_HIP(hipMemsetAsync(x64.p, 0, x64.n, s));
+    AMIE_HIP(hipEventCreate(&R.e0));
+    AMIE_HIP(hipEventCreate(&R.e1));
+    AMIE_HIP(hipEventCreateWithFlags(&R.ev_status, hipEventDisableTiming));
+    AMIE_HIP(hipEventRecord(R.e0, s));
+    return start_pass(o.precision, std::vector<char>(nprob, 1));
+}
+
+// Starts (or continues with another precision) the solve of the problems in
+// `subset` from their current x64; the other problems keep their state.
+int Batch::start_pass(int mode, const std::vector<char> &subset)
+{
+    SolveRun &R = run;
+    const amie_solve_opts &o = R.o;
+    R.mode = mode;
+    R.mixed = mode != AMIE_FP64;
     R.bj = o.precond == AMIE_BLOCK_JACOBI;
     R.per_check = std::max(1, (o.chunk > 0 ? o.chunk : 32) / R.ce);
     hipStream_t s = stream;
-    R.K = select_kern(R.mixed, et.b, nrhs, R.bj);
-    R.a = args(R.mixed);
+    R.K = select_kern(mode, et.b, nrhs, R.bj);
+    R.a = args(mode);
     R.g = grid();
+    const bool fp32vec = mode == AMIE_MIXED;
     for (int p = 0; p < nprob; p++) {
+        if (!subset[p]) continue;
         ProbDev &P = h_prob[p];
         P.refine = o.refine;
         P.rtol = o.rtol > 0 ? o.rtol : 1e-8;
         P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
         P.maxit = o.maxit > 0 ? o.maxit : ph[p].nrows * et.b;
+        P.ckpt_interval = R.mixed ? (o.ckpt_interval > 0 ? o.ckpt_interval : 128) : ((int64_t)1 << 62);
+        P.last_ckpt = 0;
+        P.stall_limit = (fp32vec && o.escalate) ? 8 : 0;
         P.status = ST_RUNNING;
         P.ckpt_pending = 1;
+        P.src[0] = P.src[1] = 0;
+        P.first[0] = P.first[1] = 1;
         P.iters = 0;
         P.nckpt = 0;
+        P.last_ckpt = 0;
         for (int k = 0; k < MAXRHS; k++) {
-            P.rho[k] = P.rho_prev[k] = P.alpha[k] = P.beta[k] = 0.;
+            P.rho_hist[0][k] = P.rho_hist[1][k] = 0.;
             P.rr[k] = 0.;
             P.rr_ckpt[k] = DBL_MAX;
             P.bb[k] = P.bb64[k] = 0.;
-            P.relres[k] = 0.;
-            P.rel_prev[k] = DBL_MAX;
-            P.done[k] = k < nrhs ? 0 : 1;
+            P.relres[0][k] = P.relres[1][k] = P.relres_out[k] = 0.;
+            P.rel_best[k] = DBL_MAX;
+            P.stall[k] = 0;
+            P.done[0][k] = P.done[1][k] = P.done_out[k] = k < nrhs ? 0 : 1;
         }
     }
     AMIE_HIP(hipMemcpyAsync(prob.p, h_prob.data(), nprob * sizeof(ProbDev), hipMemcpyHostToDevice, s));
-    if (!o.use_x0) AMIE_HIP(hipMemsetAsync(x64.p, 0, x64.n, s));
     AMIE_HIP(hipMemsetAsync(y.p, 0, y.n, s));
     AMIE_HIP(hipMemsetAsync(p0.p, 0, p0.n, s));
     AMIE_HIP(hipMemsetAsync(p1.p, 0, p1.n, s));
-    int rc = set_active_all();
-    if (rc) return rc;
-    AMIE_HIP(hipEventCreate(&R.e0));
-    AMIE_HIP(hipEventCreate(&R.e1));
-    AMIE_HIP(hipEventCreateWithFlags(&R.ev_status, hipEventDisableTiming));
-    AMIE_HIP(hipEventRecord(R.e0, s));
-    launch0(R.K.norms, R.g, s, R.a);
-    launch0(R.K.cx, R.g, s, R.a);
-    launch0(R.K.cr, R.g, s, R.a);
+    h_active.clear();
+    for (int p = 0; p < nprob; p++)
+        if (subset[p])
+            for (int t = 0; t < ph[p].ntiles; t++) h_active.push_back(ph[p].tile0 + t);
+    *h_nactive_pinned = (int32_t)h_active.size();
+    AMIE_HIP(hipMemcpyAsync(active.p, h_active.data(), h_active.size() * 4, hipMemcpyHostToDevice, s));
+    AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, 4, hipMemcpyHostToDevice, s));
+    const int g0 = grid((int64_t)h_active.size());
+    launch0(R.K.norms, g0, s, R.a);
+    hipLaunchKernelGGL(R.K.norms_final, dim3(std::min(nprob, 4096)), dim3(WG), 0, s, R.a, nprob);
+    launch0(R.K.cx, g0, s, R.a);
+    launch0(R.K.cr, g0, s, R.a);
+    R.exec = nullptr;
     if (o.use_graph && !o.time_kernels) {
-        const int key = (R.mixed ? 1 : 0) | (R.bj ? 2 : 0) | (R.ce << 4);
+        const int key = mode | (R.bj ? 4 : 0) | (R.ce << 4);
         for (auto &ge : graphs)
             if (ge.key == key) R.exec = ge.exec;
         if (!R.exec) {
@@ -692,20 +728,21 @@ int Batch::solve_begin(const amie_solve_opts &o)
             graphs.push_back({key, R.exec});
         }
     }
-    R.pbytes.resize(nprob);
+    R.pbytes.assign(nprob, 0.);
     R.active_bytes = 0.;
-    for (int p = 0; p < nprob; p++) {
-        R.pbytes[p] = spmv_bytes(*this, ph[p], R.mixed);
-        R.active_bytes += R.pbytes[p];
-    }
-    R.was_running.assign(nprob, 1);
+    R.was_running.assign(nprob, 0);
     int64_t maxit_all = 0;
-    for (int p = 0; p < nprob; p++) maxit_all = std::max<int64_t>(maxit_all, h_prob[p].maxit);
+    for (int p = 0; p < nprob; p++) {
+        if (!subset[p]) continue;
+        R.pbytes[p] = spmv_bytes(*this, ph[p], mode);
+        R.active_bytes += R.pbytes[p];
+        R.was_running[p] = 1;
+        maxit_all = std::max<int64_t>(maxit_all, h_prob[p].maxit);
+    }
     R.max_periods = maxit_all / R.ce + 4;
     R.periods = 0;
     R.in_flight = false;
     R.finished = false;
-    R.started = true;
     return AMIE_OK;
 }
 
@@ -715,6 +752,7 @@ int Batch::solve_advance()
     if (!R.started || R.finished || R.in_flight) return AMIE_OK;
     hipStream_t s = stream;
     R.nt = 0;
+    R.g = grid((int64_t)h_active.size());
     for (int c = 0; c < R.per_check; c++, R.periods++) {
         if (R.exec) {
             AMIE_HIP(hipGraphLaunch(R.exec, s));
@@ -789,6 +827,22 @@ int Batch::solve_poll(bool wait, bool *finished)
         AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, 4, hipMemcpyHostToDevice, s));
     }
     if (!any_running || R.periods >= R.max_periods) {
+        // problems whose FP32 iteration stagnated continue from their current
+        // iterate in FP64, in a pass of their own (an FP32 copy of a badly
+        // conditioned K can lose the accuracy CG needs, so the matrix goes to
+        // FP64 too)
+        std::vector<char> stalled(nprob, 0);
+        bool any_stalled = false;
+        for (int p = 0; p < nprob; p++)
+            if (h_prob_pinned[p].status == ST_STALLED) stalled[p] = 1, any_stalled = true;
+        if (any_stalled && R.mode == AMIE_MIXED && R.o.escalate) {
+            std::memcpy(h_prob.data(), h_prob_pinned, nprob * sizeof(ProbDev));
+            for (int p = 0; p < nprob; p++) {
+                R.iters_done[p] += h_prob[p].iters;
+                if (stalled[p]) R.stats.nproblems_escalated++;
+            }
+            return start_pass(AMIE_FP64, stalled);
+        }
         AMIE_HIP(hipEventRecord(R.e1, s));
         AMIE_HIP(hipEventSynchronize(R.e1));
         float ms = 0.f;
@@ -814,11 +868,12 @@ int Batch::solve_end(amie_solve_stats *st)
     amie_solve_stats &stats = R.stats;
     bool nan = false, all = true;
     for (int p = 0; p < nprob; p++) {
-        const ProbDev &P = h_prob[p];
+        ProbDev &P = h_prob[p];
+        P.iters += R.iters_done[p];   // iterations of every pass
         stats.iterations_max = std::max<int64_t>(stats.iterations_max, P.iters);
         stats.iterations_sum += P.iters;
         stats.refinements += P.nckpt;
-        for (int k = 0; k < nrhs; k++) stats.relres_max = std::max(stats.relres_max, P.relres[k]);
+        for (int k = 0; k < nrhs; k++) stats.relres_max = std::max(stats.relres_max, P.relres_out[k]);
         if (P.status == ST_CONVERGED) stats.nproblems_converged++;
         else all = false;
         if (P.status == ST_NAN) nan = true;
@@ -879,6 +934,23 @@ static int upload_vec(Batch &B, int p, DevBuf &buf, const double *in)
 
 using namespace amie;
 
+namespace amie {
+void ctx_retain(amie_gpu_ctx *ctx)
+{
+    if (ctx) ctx->children++;
+}
+static void ctx_free(amie_gpu_ctx *ctx)
+{
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+void ctx_release(amie_gpu_ctx *ctx)
+{
+    if (!ctx) return;
+    if (--ctx->children == 0 && ctx->closing) ctx_free(ctx);
+}
+} // namespace amie
+
 // =========================================================== C ABI: batches
 extern "C" {
 
@@ -920,11 +992,16 @@ int amie_gpu_create(amie_gpu_ctx **ctx, int device)
     return AMIE_OK;
 }
 
+
+// A context lives until its last batch or server is destroyed.
 void amie_gpu_destroy(amie_gpu_ctx *ctx)
 {
     if (!ctx) return;
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    if (ctx->children > 0) {
+        ctx->closing = true;
+        return;
+    }
+    amie::ctx_free(ctx);
 }
 
 int amie_gpu_device_info(amie_gpu_ctx *ctx, size_t *free_bytes, size_t *total_bytes, int *num_cu)
@@ -948,11 +1025,13 @@ void amie_gpu_default_opts(amie_solve_opts *o)
     o->refine = AMIE_REFINE_REPLACE;
     o->rtol = 1e-8;
     o->refine_drop = 1e-2;
+    o->ckpt_interval = 128;
     o->maxit = 0;
     o->use_x0 = 0;
     o->use_graph = 1;
     o->chunk = 32;
     o->time_kernels = 0;
+    o->escalate = 1;
 }
 
 int amie_gpu_element_matrices(amie_gpu_ctx *ctx, int32_t etype, int64_t nelem, const double *coords, const double *cmat,
@@ -986,6 +1065,7 @@ int amie_gpu_batch_create(amie_gpu_ctx *ctx, int32_t nprob, const amie_problem_d
     Batch *B = nullptr;
     int rc = batch_create(ctx, nprob, probs, &B);
     if (rc) return rc;
+    ctx_retain(ctx);
     *out = new amie_gpu_batch{B};
     return AMIE_OK;
 }
@@ -998,6 +1078,7 @@ int amie_gpu_batch_from_bcsr(amie_gpu_ctx *ctx, int32_t stride, int64_t nrows, c
     Batch *B = nullptr;
     int rc = batch_from_bcsr(ctx, stride, nrows, row_size, column_index, array, rhs, &B);
     if (rc) return rc;
+    ctx_retain(ctx);
     *out = new amie_gpu_batch{B};
     return AMIE_OK;
 }
@@ -1005,9 +1086,11 @@ int amie_gpu_batch_from_bcsr(amie_gpu_ctx *ctx, int32_t stride, int64_t nrows, c
 void amie_gpu_batch_destroy(amie_gpu_batch *b)
 {
     if (!b) return;
+    amie_gpu_ctx *ctx = b->b ? b->b->ctx : nullptr;
     if (b->b) (void)hipStreamSynchronize(b->b->stream);
     delete b->b;
     delete b;
+    ctx_release(ctx);
 }
 
 size_t amie_gpu_batch_bytes(const amie_gpu_batch *b) { return b && b->b ? b->b->bytes() : 0; }
@@ -1024,9 +1107,29 @@ int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_so
     amie_solve_opts o;
     amie_gpu_default_opts(&o);
     if (opts) o = *opts;
-    if (o.precision != AMIE_FP64 && o.precision != AMIE_MIXED) { set_error("bad precision"); return AMIE_ERR_ARG; }
+    if (o.precision != AMIE_FP64 && o.precision != AMIE_MIXED && o.precision != AMIE_MIXED64) {
+        set_error("bad precision");
+        return AMIE_ERR_ARG;
+    }
     if (o.precond != AMIE_JACOBI && o.precond != AMIE_BLOCK_JACOBI) { set_error("bad preconditioner"); return AMIE_ERR_ARG; }
     return b->b->solve(o, stats);
+}
+
+int amie_gpu_batch_problem_info(amie_gpu_batch *b, int32_t p, int32_t *status, int64_t *iterations,
+                                int64_t *refinements, double *relres)
+{
+    if (!b || check_p(b->b, p)) return AMIE_ERR_ARG;
+    const ProbDev &P = b->b->h_prob[p];
+    if (status)
+        *status = P.status == ST_CONVERGED ? AMIE_OK : (P.status == ST_NAN ? AMIE_ERR_NAN : AMIE_NOT_CONVERGED);
+    if (iterations) *iterations = P.iters;
+    if (refinements) *refinements = P.nckpt;
+    if (relres) {
+        double r = 0.;
+        for (int k = 0; k < b->b->nrhs; k++) r = std::max(r, P.relres_out[k]);
+        *relres = r;
+    }
+    return AMIE_OK;
 }
 
 int amie_gpu_batch_get_solution(amie_gpu_batch *b, int32_t p, double *x)
@@ -1107,7 +1210,7 @@ int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const 
     Batch &B = *bb->b;
     int rc = upload_vec(B, p, B.q, x); // q as scratch input (8-byte entries)
     if (rc) return rc;
-    SolveArgs a = B.args(precision == AMIE_MIXED);
+    SolveArgs a = B.args(precision == AMIE_MIXED ? AMIE_MIXED : AMIE_FP64);
     const int g = B.grid();
     const int mode = precision == AMIE_MIXED ? 1 : 0;
     double *xin = B.q.as<double>(), *yo = B.z.as<double>();

@@ -14,6 +14,8 @@ struct amie_gpu_ctx {
     hipStream_t stream = nullptr;
     int num_cu = 0;
     size_t total_mem = 0;
+    int children = 0;        // live batches and servers
+    bool closing = false;    // amie_gpu_destroy called while children live
 };
 
 namespace amie {
@@ -54,12 +56,15 @@ struct Kern {
     void (*cx)(SolveArgs);
     void (*cr)(SolveArgs);
     void (*norms)(SolveArgs);
+    void (*norms_final)(SolveArgs, int);
 };
 
 // State of one asynchronous solve (begin / advance / poll / end).
 struct SolveRun {
     amie_solve_opts o{};
+    int mode = 0;            // precision of the current pass
     bool mixed = false, bj = false;
+    std::vector<int64_t> iters_done;   // iterations of earlier passes, per problem
     int ce = 4;              // iterations between checkpoint opportunities (even: p buffers alternate)
     int per_check = 8;       // checkpoint periods per host status check
     Kern K{};
@@ -107,7 +112,7 @@ struct Batch {
     DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
     DevBuf v64, v32;
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active;
-    DevBuf prob, ticket, partial;
+    DevBuf prob, part1, part2, partc, partn;
     // vectors (8 bytes per entry, reinterpreted as float in mixed mode)
     DevBuf x64, b64, fext, S, y, r, z, p0, p1, q, minv32, minv64;
     DevBuf bcmask, bcval;
@@ -128,11 +133,12 @@ struct Batch {
 
     ~Batch();
     size_t bytes() const;
-    SolveArgs args(bool mixed) const;
-    int grid() const;
+    SolveArgs args(int mode) const;
+    int grid(int64_t tiles = -1) const;
     int assemble();
     int solve(const amie_solve_opts &o, amie_solve_stats *st);
     int solve_begin(const amie_solve_opts &o);
+    int start_pass(int mode, const std::vector<char> &subset);
     int solve_advance();
     int solve_poll(bool wait, bool *finished);
     int solve_end(amie_solve_stats *st);
@@ -140,6 +146,8 @@ struct Batch {
 };
 
 int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, Batch **out);
+void ctx_retain(amie_gpu_ctx *ctx);
+void ctx_release(amie_gpu_ctx *ctx);
 int batch_from_bcsr(amie_gpu_ctx *ctx, int stride, int64_t nrows, const uint32_t *row_size,
                     const uint32_t *column_index, const double *array, const double *rhs, Batch **out);
 

@@ -17,11 +17,17 @@
 // Vectors are stored per internal block row as [row][rhs][axis].
 //
 // Work is distributed in tiles of up to 4 slices (256 rows, one 256-thread
-// workgroup) that never straddle two problems of a batch; kernels loop over
-// the active-tile list with a grid stride.  Every reduction is deterministic:
-// each tile writes its partial sums, and the last tile of a problem to finish
-// (ticket counter) sums that problem's partials in tile order and updates
-// the problem's scalars.
+// workgroup) that never straddle two problems of a batch.
+//
+// Reductions never cross workgroups inside a kernel: on CDNA the L2 of each
+// XCD is not coherent with the others, so making one workgroup's writes
+// visible to another inside a kernel needs a device-scope release (a write
+// back of the whole L2).  Instead every kernel writes per-tile partial sums
+// and the workgroups of the NEXT kernel sum their problem's partials (in tile
+// order, so every workgroup gets the same bits) and derive the CG scalars
+// from them.  Per-problem state that later kernels need is written by one
+// designated workgroup (the problem's first tile) into fields that no other
+// workgroup of the same kernel reads; kernel boundaries publish it.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -34,22 +40,38 @@ constexpr int WG = WAVE * TILE_SLICES;
 constexpr int MAXRHS = 4;
 constexpr int MAXQ = 3 * MAXRHS;
 
-enum Status : int32_t { ST_IDLE = 0, ST_RUNNING = 1, ST_CONVERGED = 2, ST_MAXIT = 3, ST_NAN = 4 };
+enum Status : int32_t { ST_IDLE = 0, ST_RUNNING = 1, ST_CONVERGED = 2, ST_MAXIT = 3, ST_NAN = 4, ST_STALLED = 5 };
 
-// Per-problem constants and Krylov scalars (FP64), resident in HBM.
+// Per-problem constants (host) and CG state (device), FP64, resident in HBM.
 struct ProbDev {
     int32_t tile0, ntiles;
     int32_t slice0, nslices;
     int64_t row0;
     int32_t nrows;
-    int32_t refine;      // 0 replace residual, keep p; 1 restart
+    int32_t refine;          // 0 replace residual, keep p; 1 restart at each refinement
     double rtol, drop;
     int64_t maxit;
-    int32_t status, ckpt_pending;
-    int64_t iters, nckpt;
-    double rho[MAXRHS], rho_prev[MAXRHS], alpha[MAXRHS], beta[MAXRHS];
-    double rr[MAXRHS], rr_ckpt[MAXRHS], bb[MAXRHS], bb64[MAXRHS], relres[MAXRHS], rel_prev[MAXRHS];
-    int32_t done[MAXRHS];
+    int64_t ckpt_interval;   // FP64 refinement at least every this many iterations
+    int32_t stall_limit;     // > 0: ST_STALLED after that many refinements without halving
+    // State written by the designated workgroup and read by later kernels.
+    // Fields the p-update kernel reads AND its designated workgroup writes
+    // are kept per iteration parity: the kernel of parity par reads slot
+    // par ^ 1 and writes slot par, so no workgroup sees a value of its own
+    // kernel; the refinement kernels write slot 1, which the next p update
+    // (parity 0) reads.
+    int32_t status;          // a finished status only ever stops work
+    int32_t ckpt_pending;    // a refinement is due at the next checkpoint slot
+    int32_t src[2];          // 1: the latest rho/rr partials are a refinement's
+    int32_t first[2];        // 1: the next p update starts a new recurrence (beta = 0)
+    int32_t done[2][MAXRHS];
+    double relres[2][MAXRHS];
+    double rho_hist[2][MAXRHS];   // rho of the p update
+    int64_t iters, nckpt, last_ckpt;
+    double rr[MAXRHS], rr_ckpt[MAXRHS], bb[MAXRHS], bb64[MAXRHS], rel_best[MAXRHS];
+    int32_t stall[MAXRHS];
+    // results for the host (never read by kernels)
+    double relres_out[MAXRHS];
+    int32_t done_out[MAXRHS];
 };
 
 // Arguments shared by the solve kernels.
@@ -60,13 +82,15 @@ struct SolveArgs {
     const uint32_t *col;
     int64_t nslots;
     const double *v64;       // FP64 planes (unscaled K)
-    const void *vw;          // working planes (scaled FP32 K in mixed mode; == v64 in FP64 mode)
+    const void *vw;          // planes used by the p update's SpMV (scaled FP32 S K S, or v64)
     const int32_t *tile_slice0, *tile_nslices, *tile_prob;
     const int32_t *active;
     const int32_t *n_active;
     ProbDev *prob;
-    unsigned *ticket;
-    double *partial;         // [tile][MAXQ]
+    double *part1;           // [tile][MAXQ] p.q
+    double *part2;           // [tile][MAXQ] r.z, r.r after the update
+    double *partc;           // [tile][MAXQ] r.z, r.r, |b - K x|^2 after a refinement
+    double *partn;           // [tile][MAXQ] |S b|^2, |b|^2
     double *x64;
     const double *b64;
     const double *S;         // symmetric Jacobi scaling, per dof
@@ -165,80 +189,115 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Sums v[0..NQ) over the workgroup; result valid in thread 0.
-template <int NQ> __device__ inline void wg_sum(double *v, double (*lds)[MAXQ])
+// Sums v[0..NQ) over the workgroup; every thread gets the totals.
+template <int NQ> __device__ inline void wg_sum_all(double *v, double (*lds)[MAXQ])
 {
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         double s = wave_sum(v[q]);
         if (lane == 0) lds[wave][q] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) v[q] = lds[0][q] + lds[1][q] + lds[2][q] + lds[3][q];
-    }
-    __syncthreads();
+    for (int q = 0; q < NQ; q++) v[q] = (lds[0][q] + lds[1][q]) + (lds[2][q] + lds[3][q]);
 }
 
-__device__ inline double load_coherent(const double *p)
+// Writes this tile's partial sums (from thread 0 after a workgroup sum).
+template <int NQ> __device__ inline void write_partials(double *v, double *part, int tile, double (*lds)[MAXQ])
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wg_sum_all<NQ>(v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) part[(size_t)tile * MAXQ + q] = v[q];
 }
 
-// Writes this tile's partials; returns true (in every thread) when this
-// workgroup holds the last tile of problem `p`, in which case tot[] (thread 0)
-// holds the problem totals summed in tile order.
-template <int NQ>
-__device__ inline bool tile_reduce(double *v, int tile, int p, const ProbDev &P, unsigned *ticket, double *partial,
-                                   double (*lds)[MAXQ], int *flag)
+// Status of a problem as ONE value for the whole workgroup.  The p-update
+// kernel's designated workgroup may finish a problem while other workgroups
+// of that kernel start tiles of it; without the broadcast, waves of one
+// workgroup could read different values and disagree about entering the
+// workgroup barriers below.
+__device__ inline int32_t wg_status(const ProbDev &P, int32_t *sh)
 {
-    wg_sum<NQ>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            __hip_atomic_store(partial + (size_t)tile * MAXQ + q, v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned prev = atomicAdd(ticket + p, 1u);
-        *flag = (prev == (unsigned)(P.ntiles - 1));
-    }
     __syncthreads();
-    bool last = *flag;
-    if (last) {
-        __threadfence();
-        double acc[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; q++) acc[q] = 0.;
-        for (int t = threadIdx.x; t < P.ntiles; t += WG) {
-#pragma unroll
-            for (int q = 0; q < NQ; q++) acc[q] += load_coherent(partial + (size_t)(P.tile0 + t) * MAXQ + q);
-        }
-        wg_sum<NQ>(acc, lds);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int q = 0; q < NQ; q++) v[q] = acc[q];
-            ticket[p] = 0;
-        }
-    }
+    if (threadIdx.x == 0) *sh = __hip_atomic_load(&P.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    return last;
+    return *sh;
+}
+
+// Sums the partials a previous kernel wrote for problem P, in a fixed order:
+// every workgroup of the problem obtains the same totals.
+template <int NQ> __device__ inline void problem_sums(const double *part, const ProbDev &P, double *out,
+                                                      double (*lds)[MAXQ])
+{
+#pragma unroll
+    for (int q = 0; q < NQ; q++) out[q] = 0.;
+    for (int t = threadIdx.x; t < P.ntiles; t += WG)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) out[q] += part[(size_t)(P.tile0 + t) * MAXQ + q];
+    wg_sum_all<NQ>(out, lds);
+}
+
+// ------------------------------------------------------- tile scheduling
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8); each
+// XCD gets a contiguous range of the active tiles, so rows whose vector
+// entries neighbouring rows gather are processed under the same L2.
+constexpr int NXCD = 8;
+template <typename F> __device__ inline void for_tiles(int na, F body)
+{
+    if (gridDim.x >= NXCD) {
+        const int xcd = blockIdx.x % NXCD, lw = blockIdx.x / NXCD;
+        const int nw = (gridDim.x - xcd + NXCD - 1) / NXCD;
+        const int per = (na + NXCD - 1) / NXCD;
+        for (int j = lw; j < per; j += nw) {
+            const int it = xcd * per + j;
+            if (it >= na) break;
+            body(it);
+        }
+    } else {
+        for (int it = blockIdx.x; it < na; it += gridDim.x) body(it);
+    }
 }
 
 // ------------------------------------------------------------ SpMV core
 // Accumulates acc[k][i] += sum_blocks A(i,j) * src(c, k, j) for one lane
 // (row) of slice s, reading the block planes `vals` of precision W.
 // src(c, k, j) is supplied by a functor (gathers of neighbouring rows).
-template <typename W, int B, int NR, typename Src>
+template <typename W, typename M, int B, int NR, typename Src>
 __device__ inline void sell_row(const SolveArgs &a, const void *vals, int s, int lane, W (*acc)[B], Src src)
 {
+    // U blocks per step: their column indices and values are all requested
+    // before the dependent gathers, so each wavefront keeps 2U independent
+    // loads in flight instead of one load -> gather chain per block.
+    constexpr int U = 4;
     const int w = a.slice_width[s];
     const int64_t sp = a.slice_ptr[s] + lane;
-    for (int k = 0; k < w; k++) {
+    int k = 0;
+    for (; k + U <= w; k += U) {
+        uint32_t c[U];
+        M blk[U][B * B];
+#pragma unroll
+        for (int u = 0; u < U; u++) c[u] = __builtin_nontemporal_load(a.col + sp + (int64_t)(k + u) * WAVE);
+#pragma unroll
+        for (int u = 0; u < U; u++) Blk<M, B>::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                W xv[B];
+                src(c[u], r, xv);
+#pragma unroll
+                for (int j = 0; j < B; j++)
+#pragma unroll
+                    for (int i = 0; i < B; i++) acc[r][i] += (W)blk[u][j * B + i] * xv[j];
+            }
+    }
+    for (; k < w; k++) {
         const int64_t slot = sp + (int64_t)k * WAVE;
         const uint32_t c = __builtin_nontemporal_load(a.col + slot);
-        W blk[B * B];
-        Blk<W, B>::load(vals, a.nslots, slot, blk);
+        M blk[B * B];
+        Blk<M, B>::load(vals, a.nslots, slot, blk);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             W xv[B];
@@ -246,38 +305,90 @@ __device__ inline void sell_row(const SolveArgs &a, const void *vals, int s, int
 #pragma unroll
             for (int j = 0; j < B; j++)
 #pragma unroll
-                for (int i = 0; i < B; i++) acc[r][i] += blk[j * B + i] * xv[j];
+                for (int i = 0; i < B; i++) acc[r][i] += (W)blk[j * B + i] * xv[j];
         }
     }
 }
 
+
 template <typename W> __device__ inline W *vw(void *p) { return reinterpret_cast<W *>(p); }
 template <typename W> __device__ inline const W *cvw(const void *p) { return reinterpret_cast<const W *>(p); }
 
+// The B components of one right-hand side of node c, as one vector load for B = 2.
+template <typename W, int B> __device__ inline void load_node(const W *v, size_t idx, W *o)
+{
+    if constexpr (B == 2) {
+        typedef W v2 __attribute__((ext_vector_type(2)));
+        const v2 t = *reinterpret_cast<const v2 *>(v + idx);
+        o[0] = t.x;
+        o[1] = t.y;
+    } else {
+#pragma unroll
+        for (int j = 0; j < B; j++) o[j] = v[idx + j];
+    }
+}
+
 // ---------------------------------------------------- K1: p update + SpMV
-// p_new = z + beta p_old (own rows stored; neighbours recomputed on the fly),
-// q = A p_new, and pq = p_new . q per right-hand side.  In FP64 mode the
-// unscaled K is used with the scaling applied on the fly: q = S K (S p).
-template <typename W, int B, int NR, bool BJ>
+// beta from the partials of the previous update (or refinement), then
+// p_new = z + beta p_old (own rows stored, neighbours recomputed on the fly),
+// q = A p_new and partial p_new . q.  With the FP64 matrix the unscaled K is
+// used with the scaling applied on the fly: q = S K (S p).
+// The designated workgroup records rho, consumes a refinement's residual
+// (convergence, stagnation) and decides when the next refinement is due.
+template <typename W, typename M, int B, int NR, bool BJ>
 __global__ __launch_bounds__(WG) void k_spmv_p(SolveArgs a, int parity)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    __shared__ int flag;
-    constexpr bool ONFLY = sizeof(W) == 8;
+    __shared__ int32_t sh_status;
+    constexpr bool ONFLY = sizeof(M) == 8;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const W *zs = BJ ? cvw<W>(a.z) : cvw<W>(a.r);
     const W *pold = cvw<W>(parity ? a.p1 : a.p0);
     W *pnew = vw<W>(parity ? a.p0 : a.p1);
     W *q = vw<W>(a.q);
     const int na = *a.n_active;
-    for (int it = blockIdx.x; it < na; it += gridDim.x) {
+    for_tiles(na, [&](int it) {
         const int t = a.active[it];
         const int p = a.tile_prob[t];
         const ProbDev &P = a.prob[p];
-        if (P.status != ST_RUNNING) continue;
+        if (wg_status(P, &sh_status) != ST_RUNNING) return;
+        const int rd = parity ^ 1;   // slot written before this kernel
+        const bool ck = P.src[rd] != 0;
+        double sums[3 * NR];
+        if (ck) problem_sums<3 * NR>(a.partc, P, sums, lds);
+        else problem_sums<2 * NR>(a.part2, P, sums, lds);
+        bool done[NR], all = true, bad = false;
+        double rel[NR];
         W beta[NR];
 #pragma unroll
-        for (int r = 0; r < NR; r++) beta[r] = (W)P.beta[r];
+        for (int k = 0; k < NR; k++) {
+            bool dk = P.done[rd][k] != 0;
+            rel[k] = P.relres[rd][k];
+            if (ck && !dk) {
+                const double rr64 = sums[2 * NR + k];
+                if (!(rr64 == rr64)) bad = true;
+                rel[k] = P.bb64[k] > 0. ? sqrt(rr64 / P.bb64[k]) : sqrt(rr64);
+                dk = rel[k] <= P.rtol;
+            }
+            if (!(sums[k] == sums[k])) bad = true;
+            done[k] = dk;
+            all = all && dk;
+            const double rho_old = P.rho_hist[rd][k];
+            const bool restart = P.first[rd] || (ck && P.refine == 1) || rho_old == 0.;
+            beta[k] = (dk || restart) ? (W)0 : (W)(sums[k] / rho_old);
+        }
+        const bool designated = t == P.tile0 && threadIdx.x == 0;
+        if (all || bad) {
+            if (designated) {
+                ProbDev &Pw = a.prob[p];
+                for (int k = 0; k < NR; k++) {
+                    Pw.done[parity][k] = Pw.done_out[k] = 1;
+                    Pw.relres[parity][k] = Pw.relres_out[k] = rel[k];
+                }
+                Pw.status = bad ? ST_NAN : ST_CONVERGED;
+            }
+            return;
+        }
         double part[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) part[r] = 0.;
@@ -290,15 +401,18 @@ __global__ __launch_bounds__(WG) void k_spmv_p(SolveArgs a, int parity)
 #pragma unroll
                 for (int i = 0; i < B; i++) acc[r][i] = 0;
             auto src = [&](uint32_t c, int r, W *xv) {
+                W zv[B], pv[B];
+                const size_t idx = ((size_t)c * NR + r) * B;
+                load_node<W, B>(zs, idx, zv);
+                load_node<W, B>(pold, idx, pv);
 #pragma unroll
                 for (int j = 0; j < B; j++) {
-                    const size_t idx = ((size_t)c * NR + r) * B + j;
-                    W v = zs[idx] + beta[r] * pold[idx];
+                    W v = zv[j] + beta[r] * pv[j];
                     if (ONFLY) v *= (W)a.S[(size_t)c * B + j];
                     xv[j] = v;
                 }
             };
-            sell_row<W, B, NR>(a, a.vw, s, lane, acc, src);
+            sell_row<W, M, B, NR>(a, a.vw, s, lane, acc, src);
             if (lane < a.slice_rows[s]) {
 #pragma unroll
                 for (int r = 0; r < NR; r++)
@@ -314,14 +428,47 @@ __global__ __launch_bounds__(WG) void k_spmv_p(SolveArgs a, int parity)
                     }
             }
         }
-        if (tile_reduce<NR>(part, t, p, P, a.ticket, a.partial, lds, &flag) && threadIdx.x == 0) {
+        write_partials<NR>(part, a.part1, t, lds);
+        if (designated) {
             ProbDev &Pw = a.prob[p];
-            for (int r = 0; r < NR; r++) {
-                double pq = part[r];
-                Pw.alpha[r] = (Pw.done[r] || pq == 0.) ? 0. : Pw.rho[r] / pq;
+            bool stalled = false;
+            for (int k = 0; k < NR; k++) Pw.rho_hist[parity][k] = sums[k];
+            for (int k = 0; k < NR; k++) {
+                Pw.relres[parity][k] = Pw.relres_out[k] = rel[k];
+                Pw.done[parity][k] = Pw.done_out[k] = done[k];
             }
+            if (ck) {
+                // a refinement's FP64 residual: record, watch for stagnation
+                for (int k = 0; k < NR; k++) {
+                    Pw.rr_ckpt[k] = sums[NR + k];
+                    if (done[k]) continue;
+                    if (rel[k] < 0.5 * Pw.rel_best[k]) {
+                        Pw.rel_best[k] = rel[k];
+                        Pw.stall[k] = 0;
+                    } else if (++Pw.stall[k] >= Pw.stall_limit && Pw.stall_limit > 0) {
+                        stalled = true;
+                    }
+                }
+                Pw.nckpt += 1;
+                Pw.last_ckpt = Pw.iters;
+                Pw.ckpt_pending = 0;
+            }
+            Pw.src[parity] = 0;
+            Pw.first[parity] = 0;
+            if (ck && Pw.iters >= Pw.maxit) { Pw.status = ST_MAXIT; return; }
+            if (stalled) { Pw.status = ST_STALLED; return; }
+            Pw.iters += 1;
+            bool need = Pw.iters >= Pw.maxit || Pw.iters - Pw.last_ckpt >= Pw.ckpt_interval;
+            for (int k = 0; k < NR; k++) {
+                if (done[k]) continue;
+                const double rr = sums[NR + k];
+                Pw.rr[k] = rr;
+                if (rr < Pw.drop * Pw.drop * Pw.rr_ckpt[k]) need = true;
+                if (rr <= Pw.rtol * Pw.rtol * Pw.bb[k]) need = true;
+            }
+            if (need) Pw.ckpt_pending = 1;
         }
-    }
+    });
 }
 
 // ------------------------------------ K2: y, r update + preconditioner + dots
@@ -329,21 +476,24 @@ template <typename W, int B, int NR, bool BJ>
 __global__ __launch_bounds__(WG) void k_update(SolveArgs a, int parity)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    __shared__ int flag;
+    __shared__ int32_t sh_status;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const W *pnew = cvw<W>(parity ? a.p0 : a.p1);
     const W *q = cvw<W>(a.q);
     W *y = vw<W>(a.y), *r = vw<W>(a.r), *z = vw<W>(a.z);
     const W *minv = cvw<W>(a.minv);
     const int na = *a.n_active;
-    for (int it = blockIdx.x; it < na; it += gridDim.x) {
+    for_tiles(na, [&](int it) {
         const int t = a.active[it];
         const int p = a.tile_prob[t];
         const ProbDev &P = a.prob[p];
-        if (P.status != ST_RUNNING) continue;
+        if (wg_status(P, &sh_status) != ST_RUNNING) return;
+        double pq[NR];
+        problem_sums<NR>(a.part1, P, pq, lds);
         W alpha[NR];
 #pragma unroll
-        for (int k = 0; k < NR; k++) alpha[k] = (W)P.alpha[k];
+        for (int k = 0; k < NR; k++)
+            alpha[k] = (P.done[parity][k] || pq[k] == 0.) ? (W)0 : (W)(P.rho_hist[parity][k] / pq[k]);
         double part[2 * NR];
 #pragma unroll
         for (int k = 0; k < 2 * NR; k++) part[k] = 0.;
@@ -381,42 +531,25 @@ __global__ __launch_bounds__(WG) void k_update(SolveArgs a, int parity)
                 }
             }
         }
-        if (tile_reduce<2 * NR>(part, t, p, P, a.ticket, a.partial, lds, &flag) && threadIdx.x == 0) {
-            ProbDev &Pw = a.prob[p];
-            Pw.iters += 1;
-            bool need = false, bad = false;
-            for (int k = 0; k < NR; k++) {
-                if (Pw.done[k]) { Pw.alpha[k] = 0.; Pw.beta[k] = 0.; continue; }
-                double rho = part[k], rr = part[NR + k];
-                if (!(rho == rho) || !(rr == rr)) bad = true;
-                Pw.beta[k] = (Pw.rho[k] != 0.) ? rho / Pw.rho[k] : 0.;
-                Pw.rho_prev[k] = Pw.rho[k];
-                Pw.rho[k] = rho;
-                Pw.rr[k] = rr;
-                if (rr < Pw.drop * Pw.drop * Pw.rr_ckpt[k]) need = true;
-                if (rr <= Pw.rtol * Pw.rtol * Pw.bb[k]) need = true;
-            }
-            if (Pw.iters >= Pw.maxit) need = true;
-            if (bad) Pw.status = ST_NAN;
-            if (need) Pw.ckpt_pending = 1;
-        }
-    }
+        write_partials<2 * NR>(part, a.part2, t, lds);
+    });
 }
 
-// ------------------------------------------- checkpoint 1: x64 += S y, y = 0
+// ------------------------------------------- refinement 1: x64 += S y, y = 0
 template <typename W, int B, int NR>
 __global__ __launch_bounds__(WG) void k_ckpt_x(SolveArgs a)
 {
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     W *y = vw<W>(a.y);
     const int na = *a.n_active;
-    for (int it = blockIdx.x; it < na; it += gridDim.x) {
+    for_tiles(na, [&](int it) {
         const int t = a.active[it];
         const ProbDev &P = a.prob[a.tile_prob[t]];
-        if (P.status != ST_RUNNING || !P.ckpt_pending) continue;
-        if (wave >= a.tile_nslices[t]) continue;
+        if (P.status != ST_RUNNING || !P.ckpt_pending) return;
+        if (t == P.tile0 && threadIdx.x == 0) a.prob[a.tile_prob[t]].src[1] = 1;
+        if (wave >= a.tile_nslices[t]) return;
         const int s = a.tile_slice0[t] + wave;
-        if (lane >= a.slice_rows[s]) continue;
+        if (lane >= a.slice_rows[s]) return;
         const int64_t row = (int64_t)s * WAVE + lane;
 #pragma unroll
         for (int k = 0; k < NR; k++)
@@ -426,24 +559,24 @@ __global__ __launch_bounds__(WG) void k_ckpt_x(SolveArgs a)
                 a.x64[idx] += a.S[(size_t)row * B + i] * (double)y[idx];
                 y[idx] = 0;
             }
-    }
+    });
 }
 
-// ----------------------- checkpoint 2: r64 = b - K x64 (FP64), r = S r64 (W)
+// -------------------- refinement 2: r64 = b - K x64 (FP64), r = S r64 (W), z
 template <typename W, int B, int NR, bool BJ>
 __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    __shared__ int flag;
+    __shared__ int32_t sh_status;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     W *r = vw<W>(a.r), *z = vw<W>(a.z);
     const W *minv = cvw<W>(a.minv);
     const int na = *a.n_active;
-    for (int it = blockIdx.x; it < na; it += gridDim.x) {
+    for_tiles(na, [&](int it) {
         const int t = a.active[it];
         const int p = a.tile_prob[t];
         const ProbDev &P = a.prob[p];
-        if (P.status != ST_RUNNING || !P.ckpt_pending) continue;
+        if (P.status != ST_RUNNING || !P.ckpt_pending) return;
         double part[3 * NR];
 #pragma unroll
         for (int k = 0; k < 3 * NR; k++) part[k] = 0.;
@@ -455,11 +588,8 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
             for (int k = 0; k < NR; k++)
 #pragma unroll
                 for (int i = 0; i < B; i++) acc[k][i] = 0.;
-            auto src = [&](uint32_t c, int k, double *xv) {
-#pragma unroll
-                for (int j = 0; j < B; j++) xv[j] = a.x64[((size_t)c * NR + k) * B + j];
-            };
-            sell_row<double, B, NR>(a, a.v64, s, lane, acc, src);
+            auto src = [&](uint32_t c, int k, double *xv) { load_node<double, B>(a.x64, ((size_t)c * NR + k) * B, xv); };
+            sell_row<double, double, B, NR>(a, a.v64, s, lane, acc, src);
             if (lane < a.slice_rows[s]) {
                 W mi[B * B];
                 if (BJ) {
@@ -492,39 +622,8 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
                 }
             }
         }
-        if (tile_reduce<3 * NR>(part, t, p, P, a.ticket, a.partial, lds, &flag) && threadIdx.x == 0) {
-            ProbDev &Pw = a.prob[p];
-            bool all = true, bad = false;
-            for (int k = 0; k < NR; k++) {
-                double rho = part[k], rr = part[NR + k], rr64 = part[2 * NR + k];
-                if (!(rr64 == rr64)) bad = true;
-                double rel = Pw.bb64[k] > 0. ? sqrt(rr64 / Pw.bb64[k]) : sqrt(rr64);
-                Pw.relres[k] = rel;
-                if (Pw.done[k] || rel <= Pw.rtol) {
-                    Pw.done[k] = 1;
-                    Pw.alpha[k] = 0.;
-                    Pw.beta[k] = 0.;
-                    continue;
-                }
-                all = false;
-                // restart the recurrences (beta = 0) when the FP64 residual did
-                // not halve since the previous checkpoint: the search directions
-                // kept across a residual replacement have lost conjugacy
-                const bool stalled = Pw.nckpt > 0 && rel > 0.5 * Pw.rel_prev[k];
-                Pw.rel_prev[k] = rel;
-                bool restart = Pw.refine == 1 || Pw.nckpt == 0 || stalled;
-                Pw.beta[k] = (restart || Pw.rho_prev[k] == 0.) ? 0. : rho / Pw.rho_prev[k];
-                Pw.rho[k] = rho;
-                Pw.rr[k] = rr;
-                Pw.rr_ckpt[k] = rr;
-            }
-            Pw.nckpt += 1;
-            Pw.ckpt_pending = 0;
-            if (bad) Pw.status = ST_NAN;
-            else if (all) Pw.status = ST_CONVERGED;
-            else if (Pw.iters >= Pw.maxit) Pw.status = ST_MAXIT;
-        }
-    }
+        write_partials<3 * NR>(part, a.partc, t, lds);
+    });
 }
 
 // ------------------------------------------ norms of the right-hand sides
@@ -532,13 +631,10 @@ template <int B, int NR>
 __global__ __launch_bounds__(WG) void k_rhs_norms(SolveArgs a)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    __shared__ int flag;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int na = *a.n_active;
-    for (int it = blockIdx.x; it < na; it += gridDim.x) {
+    for_tiles(na, [&](int it) {
         const int t = a.active[it];
-        const int p = a.tile_prob[t];
-        const ProbDev &P = a.prob[p];
         double part[2 * NR];
 #pragma unroll
         for (int k = 0; k < 2 * NR; k++) part[k] = 0.;
@@ -557,14 +653,31 @@ __global__ __launch_bounds__(WG) void k_rhs_norms(SolveArgs a)
                     }
             }
         }
-        if (tile_reduce<2 * NR>(part, t, p, P, a.ticket, a.partial, lds, &flag) && threadIdx.x == 0) {
+        write_partials<2 * NR>(part, a.partn, t, lds);
+    });
+}
+
+// One workgroup per problem: |S b|^2 and |b|^2 from the norm partials.
+template <int NR>
+__global__ __launch_bounds__(WG) void k_norms_final(SolveArgs a, int nprob)
+{
+    __shared__ double lds[TILE_SLICES][MAXQ];
+    for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
+        const ProbDev &P = a.prob[p];
+        if (P.status != ST_RUNNING) continue;
+        double sums[2 * NR];
+        problem_sums<2 * NR>(a.partn, P, sums, lds);
+        if (threadIdx.x == 0) {
             ProbDev &Pw = a.prob[p];
             for (int k = 0; k < NR; k++) {
-                Pw.bb[k] = part[k];
-                Pw.bb64[k] = part[NR + k];
+                Pw.bb[k] = sums[k];
+                Pw.bb64[k] = sums[NR + k];
                 // a homogeneous right-hand side is solved by x = 0
                 // (solvers/conjugategradient.cpp:74-78)
-                if (part[NR + k] == 0.) Pw.done[k] = 1;
+                if (sums[NR + k] == 0.) {
+                    Pw.done[0][k] = Pw.done[1][k] = Pw.done_out[k] = 1;
+                    Pw.relres[0][k] = Pw.relres[1][k] = Pw.relres_out[k] = 0.;
+                }
             }
         }
     }
@@ -591,7 +704,7 @@ __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const 
 #pragma unroll
                 for (int j = 0; j < B; j++) xv[j] = x[((size_t)c * NR + k) * B + j];
             };
-            sell_row<double, B, NR>(a, a.v64, s, lane, acc, src);
+            sell_row<double, double, B, NR>(a, a.v64, s, lane, acc, src);
             if (lane < a.slice_rows[s])
                 for (int k = 0; k < NR; k++)
                     for (int i = 0; i < B; i++) yout[((size_t)row * NR + k) * B + i] = acc[k][i];
@@ -605,7 +718,7 @@ __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const 
 #pragma unroll
                 for (int j = 0; j < B; j++) xv[j] = (W)(x[((size_t)c * NR + k) * B + j] / a.S[(size_t)c * B + j]);
             };
-            sell_row<W, B, NR>(a, a.vw, s, lane, acc, src);
+            sell_row<W, W, B, NR>(a, a.vw, s, lane, acc, src);
             if (lane < a.slice_rows[s])
                 for (int k = 0; k < NR; k++)
                     for (int i = 0; i < B; i++)

@@ -141,7 +141,7 @@ struct Server {
             r.status = P.status == ST_CONVERGED ? AMIE_OK : (P.status == ST_NAN ? AMIE_ERR_NAN : AMIE_NOT_CONVERGED);
             r.iterations = (int32_t)P.iters;
             double rel = 0.;
-            for (int k = 0; k < B.nrhs; k++) rel = std::max(rel, P.relres[k]);
+            for (int k = 0; k < B.nrhs; k++) rel = std::max(rel, P.relres_out[k]);
             r.relres = rel;
             r.submit_ms = g->reqs[i].submit_ms;
             r.start_ms = g->reqs[i].start_ms;
@@ -275,6 +275,7 @@ int amie_gpu_server_create(amie_gpu_ctx *ctx, size_t hbm_budget, int32_t max_slo
         AMIE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         S->streams.push_back(st);
     }
+    ctx_retain(ctx);
     *out = new amie_gpu_server{S.release()};
     return AMIE_OK;
 }
@@ -282,8 +283,10 @@ int amie_gpu_server_create(amie_gpu_ctx *ctx, size_t hbm_budget, int32_t max_slo
 void amie_gpu_server_destroy(amie_gpu_server *s)
 {
     if (!s) return;
+    amie_gpu_ctx *ctx = s->s->ctx;
     delete s->s;
     delete s;
+    ctx_release(ctx);
 }
 
 int amie_gpu_server_submit(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t key, double submit_ms,
