@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--maxit", type=int, default=0, help="cap on CG iterations (profiling runs only)")
+    ap.add_argument("--copy", action="store_true", help="also time the plain stream-copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -94,7 +96,8 @@ def main():
 
     def step(precision, time_kernels=1):
         B.assemble()
-        st = B.solve(precision=precision, rtol=args.rtol, time_kernels=time_kernels, use_graph=0)
+        st = B.solve(precision=precision, rtol=args.rtol, time_kernels=time_kernels, use_graph=0,
+                     maxit=args.maxit)
         ceff = B.average_stress()
         return st, ceff
 
@@ -127,6 +130,8 @@ def main():
                 "status": st64["status"]}
         log(f"fp64 batched step: {1e3 * dt:.1f} ms, iters max {st64['iterations_max']}")
 
+    copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
+    log(f"stream copy: {copy_gbs:.0f} GB/s")
     cpu = None if args.no_cpu else cpu_baseline(problems, args.cpu_budget, args.rtol)
     out = {
         "metric": "RVE solves/s + p50/p99 latency @rel.res 1e-8; mixed vs FP64 CG; SpMV GB/s",
@@ -147,7 +152,7 @@ def main():
                    "rves": len(problems), "load_cases": problems[0].nrhs, "dofs": ndof, "blocks": nnzb,
                    "hbm_bytes": B.nbytes, "converged": ok},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "stream_copy_gbs": copy_gbs,
                      "kernel": "k_spmv_p (fused p-update + block SpMV + p.q)", "launches": launches,
                      "avg_launch_us": 1e3 * spmv_ms / max(1, launches),
                      "avg_bytes_per_launch": spmv_bytes / max(1, launches)},

@@ -137,6 +137,12 @@ const char *amie_gpu_last_error(void);
 int amie_gpu_device_info(amie_gpu_ctx *ctx, size_t *free_bytes, size_t *total_bytes, int *num_cu);
 void amie_gpu_default_opts(amie_solve_opts *opts);
 
+/* Achievable HBM bandwidth: a plain float4 stream copy of `bytes` bytes,
+ * repeated `iters` times; returns the mean time of one copy (read + write =
+ * 2*bytes of traffic).  Also the reference kernel for scaling hardware
+ * byte counters. */
+int amie_gpu_stream_copy(amie_gpu_ctx *ctx, size_t bytes, int32_t iters, double *ms_per_copy);
+
 /* ------------------------------------------------------ host mesh helpers
  * Synthetic inputs of the benchmark configurations (no GPU needed). */
 /* Reference orientation rule (geometry/geometry_2D.cpp:476-497,

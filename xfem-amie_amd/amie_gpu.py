@@ -74,6 +74,7 @@ SIGNATURES = {
     "amie_cauchy_green": (None, [c_dbl, c_dbl, c_i32, vp]),
     "amie_element_matrix": (c_i32, [c_i32, vp, vp, vp]),
     "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
+    "amie_gpu_stream_copy": (c_i32, [vp, ctypes.c_size_t, c_i32, vp]),
     "amie_gpu_batch_create": (c_i32, [vp, c_i32, vp, vp]),
     "amie_gpu_batch_from_bcsr": (c_i32, [vp, c_i32, c_i64, vp, vp, vp, vp, vp]),
     "amie_gpu_batch_destroy": (None, [vp]),
@@ -254,6 +255,12 @@ class Context:
         f, t, cu = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
         check(lib().amie_gpu_device_info(self.h, ctypes.byref(f), ctypes.byref(t), ctypes.byref(cu)))
         return f.value, t.value, cu.value
+
+    def stream_copy_gbs(self, nbytes=1 << 30, iters=20):
+        """Achievable HBM bandwidth of a plain copy (read + write bytes / s)."""
+        ms = c_dbl()
+        check(lib().amie_gpu_stream_copy(self.h, nbytes, iters, ctypes.byref(ms)))
+        return 2 * nbytes / (ms.value * 1e-3) / 1e9
 
     def close(self):
         if self.h:

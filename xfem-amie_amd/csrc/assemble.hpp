@@ -37,7 +37,8 @@ struct AsmArgs {
     const double *macro;      // per problem nrhs x s (nullable)
     double *v64;
     int64_t nslots;
-    double *b64;
+    double *b64;              // right-hand-side planes (vplane entries each)
+    int64_t vplane;
 };
 
 template <int ET> struct Elem;
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a, const int32_t *elem
         for (int i = 0; i < NP; i++) {
             const int64_t row = a.erow[(int64_t)e * NP + i];
             for (int k = 0; k < NR; k++)
-                for (int c = 0; c < B; c++) a.b64[(row * NR + k) * B + c] += F[k][i][c];
+                for (int c = 0; c < B; c++) a.b64[k * a.vplane + row * B + c] += F[k][i][c];
         }
     }
 }
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(WG) void k_apply_bc(SolveArgs a, double *v64, doubl
                 const double g = bcval[cd];
                 for (int i = 0; i < B; i++) {
                     if (rfix[i]) continue;
-                    for (int r = 0; r < NR; r++) b64[(row * NR + r) * B + i] -= g * blk[j * B + i];
+                    for (int r = 0; r < NR; r++) b64[r * a.vplane + row * B + i] -= g * blk[j * B + i];
                     blk[j * B + i] = 0.;
                     mod = true;
                 }
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(WG) void k_apply_bc(SolveArgs a, double *v64, doubl
         }
         for (int i = 0; i < B; i++)
             if (rfix[i])
-                for (int r = 0; r < NR; r++) b64[(row * NR + r) * B + i] = bcval[row * B + i];
+                for (int r = 0; r < NR; r++) b64[r * a.vplane + row * B + i] = bcval[row * B + i];
     }
 }
 
@@ -458,7 +459,8 @@ struct StrainArgs {
     const int32_t *erow;
     const int32_t *eprob;
     const double *macro;      // nullable
-    const double *x64;
+    const double *x64;        // solution planes (vplane entries each)
+    int64_t vplane;
     int nrhs;
 };
 
@@ -485,7 +487,7 @@ __device__ inline void element_strain(const StrainArgs &a, int e, int k, double 
     for (int j = 0; j < NP; j++) {
         const int64_t row = a.erow[(int64_t)e * NP + j];
         for (int c = 0; c < B; c++) {
-            double u = a.x64[(row * a.nrhs + k) * B + c];
+            double u = a.x64[k * a.vplane + row * B + c];
             g[c][0] += dxi[j] * u; g[c][1] += deta[j] * u;
             if (D == 3) g[c][2] += dzeta[j] * u;
         }

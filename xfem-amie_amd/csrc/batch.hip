@@ -75,8 +75,9 @@ Batch::~Batch()
 size_t Batch::bytes() const
 {
     const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &row_len, &v64, &v32, &tile_slice0,
-                           &tile_nslices, &tile_prob, &active, &n_active, &prob, &part1, &part2, &partc, &partn, &x64, &b64,
-                           &fext, &S, &y, &r, &z, &p0, &p1, &q, &minv32, &minv64, &bcmask, &bcval, &coords,
+                           &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
+                           &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
+                           &bcval, &coords,
                            &econn, &erow, &eslot, &ephase, &edamage, &ekappa, &ed0, &eprob, &cmat, &cmat_off,
                            &macro, &colour_elems, &prob_e0, &prob_ne, &grown};
     size_t s = 0;
@@ -101,19 +102,20 @@ SolveArgs Batch::args(int mode) const
     a.tile_prob = tile_prob.as<int32_t>();
     a.active = active.as<int32_t>();
     a.n_active = n_active.as<int32_t>();
+    a.active_prob = active_prob.as<int32_t>();
+    a.n_active_prob = n_active_prob.as<int32_t>();
     a.prob = prob.as<ProbDev>();
     a.part1 = part1.as<double>();
     a.part2 = part2.as<double>();
     a.partc = partc.as<double>();
     a.partn = partn.as<double>();
+    a.vplane = vplane();
     a.x64 = x64.as<double>();
     a.b64 = b64.as<double>();
     a.S = S.as<double>();
     a.y = y.p;
     a.r = r.p;
-    a.z = z.p;
-    a.p0 = p0.p;
-    a.p1 = p1.p;
+    a.p = p.p;
     a.q = q.p;
     a.minv = w32 ? minv32.p : minv64.p;
     return a;
@@ -128,16 +130,31 @@ int Batch::grid(int64_t tiles) const
     return (int)std::max<int64_t>(1, std::min<int64_t>(cap, tiles));
 }
 
-int Batch::set_active_all()
+// Active tile and problem lists of the problems flagged in `probs` (queued
+// on the batch stream; the host lists stay valid until the next call).
+int Batch::set_active(const std::vector<char> &probs)
 {
-    h_active.resize(ntiles);
-    std::iota(h_active.begin(), h_active.end(), 0);
-    *h_nactive_pinned = (int32_t)ntiles;
-    AMIE_HIP(hipMemcpyAsync(active.p, h_active.data(), ntiles * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    h_active.clear();
+    h_active_prob.clear();
+    for (int p = 0; p < nprob; p++) {
+        if (!probs[p]) continue;
+        h_active_prob.push_back(p);
+        for (int t = 0; t < ph[p].ntiles; t++) h_active.push_back(ph[p].tile0 + t);
+    }
+    h_nactive_pinned[0] = (int32_t)h_active.size();
+    h_nactive_pinned[1] = (int32_t)h_active_prob.size();
+    if (!h_active.empty())
+        AMIE_HIP(hipMemcpyAsync(active.p, h_active.data(), h_active.size() * 4, hipMemcpyHostToDevice, stream));
+    if (!h_active_prob.empty())
+        AMIE_HIP(hipMemcpyAsync(active_prob.p, h_active_prob.data(), h_active_prob.size() * 4, hipMemcpyHostToDevice, stream));
+    AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, 4, hipMemcpyHostToDevice, stream));
+    AMIE_HIP(hipMemcpyAsync(n_active_prob.p, h_nactive_pinned + 1, 4, hipMemcpyHostToDevice, stream));
+    // the host lists are the sources of queued copies: wait before they change
     AMIE_HIP(hipStreamSynchronize(stream));
     return AMIE_OK;
 }
+
+int Batch::set_active_all() { return set_active(std::vector<char>(nprob, 1)); }
 
 // ------------------------------------------------------------ batch build
 struct Tmp {
@@ -178,7 +195,7 @@ static int alloc_and_upload(DevBuf &buf, const void *src, size_t bytes)
 static int alloc_vectors(Batch &B)
 {
     const size_t nv = (size_t)B.nrows_pad * B.et.b * B.nrhs;
-    DevBuf *vecs[] = {&B.x64, &B.b64, &B.fext, &B.y, &B.r, &B.z, &B.p0, &B.p1, &B.q};
+    DevBuf *vecs[] = {&B.x64, &B.b64, &B.fext, &B.y, &B.r, &B.p, &B.q};
     for (auto v : vecs) {
         if (!v->alloc(nv * 8)) return AMIE_ERR_NOMEM;
         AMIE_HIP(hipMemset(v->p, 0, v->n));
@@ -198,8 +215,10 @@ static int alloc_vectors(Batch &B)
     }
     if (!B.active.alloc(B.ntiles * 4)) return AMIE_ERR_NOMEM;
     if (!B.n_active.alloc(4)) return AMIE_ERR_NOMEM;
+    if (!B.active_prob.alloc(B.nprob * 4)) return AMIE_ERR_NOMEM;
+    if (!B.n_active_prob.alloc(4)) return AMIE_ERR_NOMEM;
     AMIE_HIP(hipHostMalloc((void **)&B.h_prob_pinned, B.nprob * sizeof(ProbDev), hipHostMallocDefault));
-    AMIE_HIP(hipHostMalloc((void **)&B.h_nactive_pinned, sizeof(int32_t), hipHostMallocDefault));
+    AMIE_HIP(hipHostMalloc((void **)&B.h_nactive_pinned, 2 * sizeof(int32_t), hipHostMallocDefault));
     B.h_prob.resize(B.nprob);
     for (int p = 0; p < B.nprob; p++) {
         ProbDev &P = B.h_prob[p];
@@ -419,7 +438,8 @@ int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, B
         if (d.forces)
             for (int k = 0; k < NR; k++)
                 for (int64_t dof = 0; dof < ndof; dof++)
-                    h_fext[((P.row0 + P.iperm[dof / b]) * NR + k) * b + dof % b] = d.forces[(size_t)k * ndof + dof];
+                    h_fext[(size_t)k * B.nrows_pad * b + (P.row0 + P.iperm[dof / b]) * b + dof % b] =
+                        d.forces[(size_t)k * ndof + dof];
         for (int64_t i = 0; i < d.nbc; i++) {
             const int64_t dof = d.bc_dofs[i];
             const int64_t g = (P.row0 + P.iperm[dof / b]) * b + dof % b;
@@ -581,6 +601,7 @@ int Batch::assemble()
         a.v64 = v64.as<double>();
         a.nslots = nslots;
         a.b64 = b64.as<double>();
+        a.vplane = vplane();
         if (et.et == 1) { if (nrhs == 1) launch_assemble<1, 1>(*this, a); else launch_assemble<1, 3>(*this, a); }
         else if (et.et == 2) { if (nrhs == 1) launch_assemble<2, 1>(*this, a); else launch_assemble<2, 3>(*this, a); }
         else { if (nrhs == 1) launch_assemble<3, 1>(*this, a); else launch_assemble<3, 3>(*this, a); }
@@ -595,8 +616,8 @@ int Batch::assemble()
 // -------------------------------------------------------------- solve
 template <typename W, typename M, int Bk, int NR, bool BJ> static Kern kern_for()
 {
-    return {k_spmv_p<W, M, Bk, NR, BJ>, k_update<W, Bk, NR, BJ>, k_ckpt_x<W, Bk, NR>, k_ckpt_r<W, Bk, NR, BJ>,
-            k_rhs_norms<Bk, NR>, k_norms_final<NR>};
+    return {k_spmv<W, M, Bk, NR, BJ>, k_reduce<NR>, k_update<W, Bk, NR, BJ>, k_ckpt_x<W, Bk, NR>,
+            k_ckpt_r<W, Bk, NR, BJ>, k_rhs_norms<Bk, NR>, k_norms_final<NR>};
 }
 
 template <typename W, typename M, int Bk> static Kern kern_nr(int nr, bool bj)
@@ -612,25 +633,23 @@ static Kern select_kern(int mode, int b, int nr, bool bj)
     return b == 2 ? kern_nr<double, double, 2>(nr, bj) : kern_nr<double, double, 3>(nr, bj);
 }
 
-static void launch1(void (*k)(SolveArgs, int), int g, hipStream_t st, const SolveArgs &a, int par)
-{
-    hipLaunchKernelGGL(k, dim3(g), dim3(WG), 0, st, a, par);
-}
 static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs &a)
 {
-    hipLaunchKernelGGL(k, dim3(g), dim3(WG), 0, st, a);
+    hipLaunchKernelGGL(k, dim3(std::max(1, g)), dim3(WG), 0, st, a);
 }
 
-// algorithmic bytes of one fused SpMV launch over problem p (values + block
-// column indices + row pointer + 4 vector streams per right-hand side)
-static double spmv_bytes(const Batch &B, const ProbHost &P, int mode)
+// Algorithmic bytes of one SpMV launch (K1) over problem p: matrix values
+// and block column indices, row pointer, the gathered vector p read once,
+// r read for the dots, q written, per right-hand side (+ the block-Jacobi
+// inverses).  Padding blocks are not counted.
+static double spmv_bytes(const Batch &B, const ProbHost &P, int mode, bool bj)
 {
     const double sm = mode == AMIE_FP64 ? 8. : 4.;     // matrix values
     const double sv = mode == AMIE_MIXED ? 4. : 8.;    // vectors
     const double b = B.et.b;
     const double nnzb = (double)P.column_index.size();
     const double n = (double)P.nrows * b;
-    return nnzb * (b * b * sm + 4.) + 4. * (P.nrows + 1) + 4. * n * sv * B.nrhs;
+    return nnzb * (b * b * sm + 4.) + 4. * (P.nrows + 1) + 3. * n * sv * B.nrhs + (bj ? n * b * sv : 0.);
 }
 
 int Batch::solve_begin(const amie_solve_opts &o)
@@ -649,6 +668,32 @@ int Batch::solve_begin(const amie_solve_opts &o)
     return start_pass(o.precision, std::vector<char>(nprob, 1));
 }
 
+// Queues one checkpoint period: ce iterations, then the refinement slot.
+static void queue_period(Batch &B, SolveRun &R, hipStream_t s)
+{
+    for (int i = 0; i < R.ce; i++) {
+        const bool timed = R.o.time_kernels && !R.exec;
+        if (timed) {
+            if ((size_t)(2 * R.nt + 1) >= R.tev.size()) {
+                hipEvent_t ev;
+                if (hipEventCreate(&ev) == hipSuccess) R.tev.push_back(ev);
+                if (hipEventCreate(&ev) == hipSuccess) R.tev.push_back(ev);
+            }
+            (void)hipEventRecord(R.tev[2 * R.nt], s);
+        }
+        launch0(R.K.k1, R.gt, s, R.a);
+        if (timed) {
+            (void)hipEventRecord(R.tev[2 * R.nt + 1], s);
+            R.nt++;
+        }
+        launch0(R.K.red, R.gp, s, R.a);
+        hipLaunchKernelGGL(R.K.k2, dim3(std::max(1, R.gt)), dim3(WG), 0, s, R.a, i == R.ce - 1 ? 1 : 0);
+    }
+    launch0(R.K.cx, R.gt, s, R.a);
+    launch0(R.K.cr, R.gt, s, R.a);
+    (void)B;
+}
+
 // Starts (or continues with another precision) the solve of the problems in
 // `subset` from their current x64; the other problems keep their state.
 int Batch::start_pass(int mode, const std::vector<char> &subset)
@@ -662,7 +707,6 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
     hipStream_t s = stream;
     R.K = select_kern(mode, et.b, nrhs, R.bj);
     R.a = args(mode);
-    R.g = grid();
     const bool fp32vec = mode == AMIE_MIXED;
     for (int p = 0; p < nprob; p++) {
         if (!subset[p]) continue;
@@ -672,61 +716,55 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
         P.maxit = o.maxit > 0 ? o.maxit : ph[p].nrows * et.b;
         P.ckpt_interval = R.mixed ? (o.ckpt_interval > 0 ? o.ckpt_interval : 128) : ((int64_t)1 << 62);
-        P.last_ckpt = 0;
         P.stall_limit = (fp32vec && o.escalate) ? 8 : 0;
         P.status = ST_RUNNING;
-        P.ckpt_pending = 1;
-        P.src[0] = P.src[1] = 0;
-        P.first[0] = P.first[1] = 1;
+        P.ckpt_pending = 1;   // the pass opens with a refinement: r = S(b - K x), p = M r
+        P.src_ckpt = 0;
+        P.first = 1;
         P.iters = 0;
         P.nckpt = 0;
         P.last_ckpt = 0;
         for (int k = 0; k < MAXRHS; k++) {
-            P.rho_hist[0][k] = P.rho_hist[1][k] = 0.;
+            P.alpha[k] = P.beta[k] = 0.;
             P.rr[k] = 0.;
             P.rr_ckpt[k] = DBL_MAX;
+            P.rr_target[k] = 0.;
             P.bb[k] = P.bb64[k] = 0.;
-            P.relres[0][k] = P.relres[1][k] = P.relres_out[k] = 0.;
+            P.relres[k] = P.relres_out[k] = 0.;
             P.rel_best[k] = DBL_MAX;
             P.stall[k] = 0;
-            P.done[0][k] = P.done[1][k] = P.done_out[k] = k < nrhs ? 0 : 1;
+            P.done[k] = P.done_out[k] = k < nrhs ? 0 : 1;
         }
     }
     AMIE_HIP(hipMemcpyAsync(prob.p, h_prob.data(), nprob * sizeof(ProbDev), hipMemcpyHostToDevice, s));
     AMIE_HIP(hipMemsetAsync(y.p, 0, y.n, s));
-    AMIE_HIP(hipMemsetAsync(p0.p, 0, p0.n, s));
-    AMIE_HIP(hipMemsetAsync(p1.p, 0, p1.n, s));
-    h_active.clear();
-    for (int p = 0; p < nprob; p++)
-        if (subset[p])
-            for (int t = 0; t < ph[p].ntiles; t++) h_active.push_back(ph[p].tile0 + t);
-    *h_nactive_pinned = (int32_t)h_active.size();
-    AMIE_HIP(hipMemcpyAsync(active.p, h_active.data(), h_active.size() * 4, hipMemcpyHostToDevice, s));
-    AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, 4, hipMemcpyHostToDevice, s));
-    const int g0 = grid((int64_t)h_active.size());
-    launch0(R.K.norms, g0, s, R.a);
-    hipLaunchKernelGGL(R.K.norms_final, dim3(std::min(nprob, 4096)), dim3(WG), 0, s, R.a, nprob);
-    launch0(R.K.cx, g0, s, R.a);
-    launch0(R.K.cr, g0, s, R.a);
+    AMIE_HIP(hipMemsetAsync(p.p, 0, p.n, s));
+    int rc = set_active(subset);
+    if (rc) return rc;
+    R.gt = (int)h_active.size();
+    R.gp = (int)h_active_prob.size();
+    launch0(R.K.norms, R.gt, s, R.a);
+    launch0(R.K.norms_final, R.gp, s, R.a);
+    launch0(R.K.cx, R.gt, s, R.a);
+    launch0(R.K.cr, R.gt, s, R.a);
     R.exec = nullptr;
     if (o.use_graph && !o.time_kernels) {
+        // the captured grids cover this pass's problems; problems that finish
+        // leave the device lists and their workgroups return at once
         const int key = mode | (R.bj ? 4 : 0) | (R.ce << 4);
-        for (auto &ge : graphs)
-            if (ge.key == key) R.exec = ge.exec;
-        if (!R.exec) {
-            hipGraph_t graph;
-            AMIE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < R.ce; i++) {
-                launch1(R.K.k1, R.g, s, R.a, i & 1);
-                launch1(R.K.k2, R.g, s, R.a, i & 1);
+        for (auto it = graphs.begin(); it != graphs.end(); ++it)
+            if (it->key == key) {
+                (void)hipGraphExecDestroy(it->exec);
+                graphs.erase(it);
+                break;
             }
-            launch0(R.K.cx, R.g, s, R.a);
-            launch0(R.K.cr, R.g, s, R.a);
-            AMIE_HIP(hipStreamEndCapture(s, &graph));
-            AMIE_HIP(hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0));
-            AMIE_HIP(hipGraphDestroy(graph));
-            graphs.push_back({key, R.exec});
-        }
+        hipGraph_t graph;
+        AMIE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        queue_period(*this, R, s);
+        AMIE_HIP(hipStreamEndCapture(s, &graph));
+        AMIE_HIP(hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0));
+        AMIE_HIP(hipGraphDestroy(graph));
+        graphs.push_back({key, R.exec});
     }
     R.pbytes.assign(nprob, 0.);
     R.active_bytes = 0.;
@@ -734,7 +772,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
     int64_t maxit_all = 0;
     for (int p = 0; p < nprob; p++) {
         if (!subset[p]) continue;
-        R.pbytes[p] = spmv_bytes(*this, ph[p], mode);
+        R.pbytes[p] = spmv_bytes(*this, ph[p], mode, R.bj);
         R.active_bytes += R.pbytes[p];
         R.was_running[p] = 1;
         maxit_all = std::max<int64_t>(maxit_all, h_prob[p].maxit);
@@ -752,32 +790,9 @@ int Batch::solve_advance()
     if (!R.started || R.finished || R.in_flight) return AMIE_OK;
     hipStream_t s = stream;
     R.nt = 0;
-    R.g = grid((int64_t)h_active.size());
     for (int c = 0; c < R.per_check; c++, R.periods++) {
-        if (R.exec) {
-            AMIE_HIP(hipGraphLaunch(R.exec, s));
-        } else {
-            for (int i = 0; i < R.ce; i++) {
-                if (R.o.time_kernels) {
-                    if ((size_t)(2 * R.nt + 1) >= R.tev.size()) {
-                        hipEvent_t ev;
-                        AMIE_HIP(hipEventCreate(&ev));
-                        R.tev.push_back(ev);
-                        AMIE_HIP(hipEventCreate(&ev));
-                        R.tev.push_back(ev);
-                    }
-                    AMIE_HIP(hipEventRecord(R.tev[2 * R.nt], s));
-                }
-                launch1(R.K.k1, R.g, s, R.a, i & 1);
-                if (R.o.time_kernels) {
-                    AMIE_HIP(hipEventRecord(R.tev[2 * R.nt + 1], s));
-                    R.nt++;
-                }
-                launch1(R.K.k2, R.g, s, R.a, i & 1);
-            }
-            launch0(R.K.cx, R.g, s, R.a);
-            launch0(R.K.cr, R.g, s, R.a);
-        }
+        if (R.exec) AMIE_HIP(hipGraphLaunch(R.exec, s));
+        else queue_period(*this, R, s);
     }
     AMIE_HIP(hipGetLastError());
     AMIE_HIP(hipMemcpyAsync(h_prob_pinned, prob.p, nprob * sizeof(ProbDev), hipMemcpyDeviceToHost, s));
@@ -799,7 +814,6 @@ int Batch::solve_poll(bool wait, bool *finished)
         if (!hip_ok(q, "hipEventQuery")) return AMIE_ERR_HIP;
     }
     R.in_flight = false;
-    hipStream_t s = stream;
     for (int i = 0; i < R.nt; i++) {
         float ms = 0.f;
         AMIE_HIP(hipEventElapsedTime(&ms, R.tev[2 * i], R.tev[2 * i + 1]));
@@ -818,13 +832,12 @@ int Batch::solve_poll(bool wait, bool *finished)
         }
     }
     if (any_running && changed) {
-        h_active.clear();
-        for (int p = 0; p < nprob; p++)
-            if (R.was_running[p])
-                for (int t = 0; t < ph[p].ntiles; t++) h_active.push_back(ph[p].tile0 + t);
-        *h_nactive_pinned = (int32_t)h_active.size();
-        AMIE_HIP(hipMemcpyAsync(active.p, h_active.data(), h_active.size() * 4, hipMemcpyHostToDevice, s));
-        AMIE_HIP(hipMemcpyAsync(n_active.p, h_nactive_pinned, 4, hipMemcpyHostToDevice, s));
+        int rc = set_active(R.was_running);
+        if (rc) return rc;
+        if (!R.exec) {
+            R.gt = (int)h_active.size();
+            R.gp = (int)h_active_prob.size();
+        }
     }
     if (!any_running || R.periods >= R.max_periods) {
         // problems whose FP32 iteration stagnated continue from their current
@@ -843,7 +856,7 @@ int Batch::solve_poll(bool wait, bool *finished)
             }
             return start_pass(AMIE_FP64, stalled);
         }
-        AMIE_HIP(hipEventRecord(R.e1, s));
+        AMIE_HIP(hipEventRecord(R.e1, stream));
         AMIE_HIP(hipEventSynchronize(R.e1));
         float ms = 0.f;
         AMIE_HIP(hipEventElapsedTime(&ms, R.e0, R.e1));
@@ -880,7 +893,7 @@ int Batch::solve_end(amie_solve_stats *st)
     }
     stats.status = nan ? AMIE_ERR_NAN : (all ? AMIE_OK : AMIE_NOT_CONVERGED);
     if (st) *st = stats;
-    // full active list again for later kernels (SpMV hook, strains)
+    // full active lists again for later kernels (SpMV hook, strains)
     int rc = set_active_all();
     if (rc) return rc;
     return stats.status;
@@ -905,28 +918,35 @@ static int check_p(Batch *B, int p)
     return AMIE_OK;
 }
 
-// internal vector (entries of 8 bytes, [row][rhs][axis]) -> original order, column-major per rhs
-static int download_vec(Batch &B, int p, const DevBuf &buf, double *out)
+// FP64 plane vector (entries of 8 bytes) -> original order, column-major per rhs
+int download_vec(Batch &B, int p, const DevBuf &buf, double *out)
 {
     const ProbHost &P = B.ph[p];
     const int b = B.et.b, NR = B.nrhs;
-    std::vector<double> h((size_t)P.nslices * WAVE * b * NR);
-    AMIE_HIP(hipMemcpy(h.data(), buf.as<double>() + (size_t)P.row0 * b * NR, h.size() * 8, hipMemcpyDeviceToHost));
+    const size_t nloc = (size_t)P.nslices * WAVE * b;
+    std::vector<double> h(nloc);
     const int64_t ndof = P.nrows * b;
-    for (int k = 0; k < NR; k++)
-        for (int64_t d = 0; d < ndof; d++) out[(size_t)k * ndof + d] = h[((size_t)P.iperm[d / b] * NR + k) * b + d % b];
+    for (int k = 0; k < NR; k++) {
+        AMIE_HIP(hipMemcpy(h.data(), buf.as<double>() + (size_t)k * B.vplane() + (size_t)P.row0 * b, nloc * 8,
+                           hipMemcpyDeviceToHost));
+        for (int64_t d = 0; d < ndof; d++) out[(size_t)k * ndof + d] = h[(size_t)P.iperm[d / b] * b + d % b];
+    }
     return AMIE_OK;
 }
 
-static int upload_vec(Batch &B, int p, DevBuf &buf, const double *in)
+int upload_vec(Batch &B, int p, DevBuf &buf, const double *in)
 {
     const ProbHost &P = B.ph[p];
     const int b = B.et.b, NR = B.nrhs;
-    std::vector<double> h((size_t)P.nslices * WAVE * b * NR, 0.);
+    const size_t nloc = (size_t)P.nslices * WAVE * b;
+    std::vector<double> h(nloc);
     const int64_t ndof = P.nrows * b;
-    for (int k = 0; k < NR; k++)
-        for (int64_t d = 0; d < ndof; d++) h[((size_t)P.iperm[d / b] * NR + k) * b + d % b] = in[(size_t)k * ndof + d];
-    AMIE_HIP(hipMemcpy(buf.as<double>() + (size_t)P.row0 * b * NR, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    for (int k = 0; k < NR; k++) {
+        std::fill(h.begin(), h.end(), 0.);
+        for (int64_t d = 0; d < ndof; d++) h[(size_t)P.iperm[d / b] * b + d % b] = in[(size_t)k * ndof + d];
+        AMIE_HIP(hipMemcpy(buf.as<double>() + (size_t)k * B.vplane() + (size_t)P.row0 * b, h.data(), nloc * 8,
+                           hipMemcpyHostToDevice));
+    }
     return AMIE_OK;
 }
 
@@ -948,6 +968,13 @@ void ctx_release(amie_gpu_ctx *ctx)
 {
     if (!ctx) return;
     if (--ctx->children == 0 && ctx->closing) ctx_free(ctx);
+}
+
+// Plain grid-stride copy of 16-byte words: the achievable-bandwidth yardstick.
+__global__ __launch_bounds__(256) void k_stream_copy(const f4v *src, f4v *dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = __builtin_nontemporal_load(src + i);
 }
 } // namespace amie
 
@@ -1032,6 +1059,32 @@ void amie_gpu_default_opts(amie_solve_opts *o)
     o->chunk = 32;
     o->time_kernels = 0;
     o->escalate = 1;
+}
+
+int amie_gpu_stream_copy(amie_gpu_ctx *ctx, size_t bytes, int32_t iters, double *ms_per_copy)
+{
+    if (!ctx || bytes < 16 || iters <= 0 || !ms_per_copy) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    AMIE_HIP(hipSetDevice(ctx->device));
+    const size_t n4 = bytes / 16;
+    DevBuf a, b;
+    if (!a.alloc(n4 * 16) || !b.alloc(n4 * 16)) return AMIE_ERR_NOMEM;
+    AMIE_HIP(hipMemset(a.p, 0, a.n));
+    const int grid = std::max(1, ctx->num_cu) * 8;
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, a.as<f4v>(), b.as<f4v>(), n4);
+    hipEvent_t e0, e1;
+    AMIE_HIP(hipEventCreate(&e0));
+    AMIE_HIP(hipEventCreate(&e1));
+    AMIE_HIP(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iters; i++)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, a.as<f4v>(), b.as<f4v>(), n4);
+    AMIE_HIP(hipEventRecord(e1, ctx->stream));
+    AMIE_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    AMIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_copy = ms / iters;
+    return AMIE_OK;
 }
 
 int amie_gpu_element_matrices(amie_gpu_ctx *ctx, int32_t etype, int64_t nelem, const double *coords, const double *cmat,
@@ -1213,7 +1266,7 @@ int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const 
     SolveArgs a = B.args(precision == AMIE_MIXED ? AMIE_MIXED : AMIE_FP64);
     const int g = B.grid();
     const int mode = precision == AMIE_MIXED ? 1 : 0;
-    double *xin = B.q.as<double>(), *yo = B.z.as<double>();
+    double *xin = B.q.as<double>(), *yo = B.r.as<double>();
     hipStream_t st = B.stream;
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(g), dim3(WG), 0, st, a, mode, xin, yo, (int)B.ntiles); };
     if (B.et.b == 2) {
@@ -1225,7 +1278,7 @@ int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const 
     }
     AMIE_HIP(hipGetLastError());
     AMIE_HIP(hipStreamSynchronize(st));
-    return download_vec(B, p, B.z, y);
+    return download_vec(B, p, B.r, y);
 }
 
 static StrainArgs strain_args(Batch &B)
@@ -1237,6 +1290,7 @@ static StrainArgs strain_args(Batch &B)
     s.eprob = B.eprob.as<int32_t>();
     s.macro = B.has_macro ? B.macro.as<double>() : nullptr;
     s.x64 = B.x64.as<double>();
+    s.vplane = B.vplane();
     s.nrhs = B.nrhs;
     return s;
 }

@@ -51,12 +51,13 @@ struct EType {
 EType etype_info(int et);
 
 struct Kern {
-    void (*k1)(SolveArgs, int);
-    void (*k2)(SolveArgs, int);
-    void (*cx)(SolveArgs);
-    void (*cr)(SolveArgs);
+    void (*k1)(SolveArgs);          // SpMV + dots
+    void (*red)(SolveArgs);         // per-problem scalars and decisions
+    void (*k2)(SolveArgs, int);     // vector update (arg: checkpoint slot follows)
+    void (*cx)(SolveArgs);          // refinement: x64 += S y
+    void (*cr)(SolveArgs);          // refinement: FP64 residual, p
     void (*norms)(SolveArgs);
-    void (*norms_final)(SolveArgs, int);
+    void (*norms_final)(SolveArgs);
 };
 
 // State of one asynchronous solve (begin / advance / poll / end).
@@ -65,11 +66,11 @@ struct SolveRun {
     int mode = 0;            // precision of the current pass
     bool mixed = false, bj = false;
     std::vector<int64_t> iters_done;   // iterations of earlier passes, per problem
-    int ce = 4;              // iterations between checkpoint opportunities (even: p buffers alternate)
+    int ce = 4;              // iterations between checkpoint opportunities
     int per_check = 8;       // checkpoint periods per host status check
     Kern K{};
     SolveArgs a{};
-    int g = 1;
+    int gt = 1, gp = 1;      // grids: active tiles, active problems
     hipGraphExec_t exec = nullptr;
     std::vector<double> pbytes;
     double active_bytes = 0.;
@@ -111,18 +112,19 @@ struct Batch {
     // SELL structure
     DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
     DevBuf v64, v32;
-    DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active;
+    DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active, active_prob, n_active_prob;
     DevBuf prob, part1, part2, partc, partn;
-    // vectors (8 bytes per entry, reinterpreted as float in mixed mode)
-    DevBuf x64, b64, fext, S, y, r, z, p0, p1, q, minv32, minv64;
+    // vectors: one plane of nrows_pad * b entries per right-hand side; 8 bytes
+    // per entry (the iteration vectors y, r, p, q hold floats in mixed mode)
+    DevBuf x64, b64, fext, S, y, r, p, q, minv32, minv64;
     DevBuf bcmask, bcval;
     // elements
     DevBuf coords, econn, erow, eslot, ephase, edamage, ekappa, ed0, eprob, cmat, cmat_off, macro, colour_elems,
         prob_e0, prob_ne, grown;
-    std::vector<int32_t> h_active;
+    std::vector<int32_t> h_active, h_active_prob;
     std::vector<ProbDev> h_prob;
     ProbDev *h_prob_pinned = nullptr;
-    int32_t *h_nactive_pinned = nullptr;
+    int32_t *h_nactive_pinned = nullptr;   // [0] tiles, [1] problems
 
     // captured iteration chunks, keyed by solve variant
     struct GraphEntry {
@@ -143,7 +145,14 @@ struct Batch {
     int solve_poll(bool wait, bool *finished);
     int solve_end(amie_solve_stats *st);
     int set_active_all();
+    int set_active(const std::vector<char> &probs);
+    int64_t vplane() const { return nrows_pad * et.b; }
 };
+
+// FP64 plane vector of problem p <-> original DOF order, one column of
+// nrows*b entries per right-hand side.
+int download_vec(Batch &B, int p, const DevBuf &buf, double *out);
+int upload_vec(Batch &B, int p, DevBuf &buf, const double *in);
 
 int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, Batch **out);
 void ctx_retain(amie_gpu_ctx *ctx);

@@ -14,20 +14,36 @@
 //   FP32 2x2: 1 x float4            FP32 3x3: 2 x float4 + 1 float
 //   FP64 2x2: 2 x double2           FP64 3x3: 4 x double2 + 1 double
 //
-// Vectors are stored per internal block row as [row][rhs][axis].
+// Vectors hold one plane per right-hand side: entry (k, row, i) sits at
+// (k * nrows_pad + row) * b + i.  A wavefront's access to one right-hand
+// side of its 64 rows, and its gather of neighbour rows (whose columns mostly
+// run along the rows), is then one contiguous run of 64 b-vectors.
 //
 // Work is distributed in tiles of up to 4 slices (256 rows, one 256-thread
-// workgroup) that never straddle two problems of a batch.
+// workgroup) that never straddle two problems of a batch; the iteration
+// kernels run one workgroup per active tile.
+//
+// One CG iteration on the scaled system Ks = S K S (S = diag(K)^-1/2) with
+// preconditioner M (identity, or the inverses of the scaled diagonal blocks):
+//   K1 (k_spmv)   q = Ks p; partial sums of p.q, q.z and q.Mq (z = M r)
+//   R  (k_reduce) per problem: alpha = rho / p.q, the next
+//                 rho' = (r - alpha q).M(r - alpha q) = rho - 2 alpha q.z +
+//                 alpha^2 q.Mq, beta = rho' / rho; convergence, refinement
+//                 and stagnation decisions
+//   K2 (k_update) y += alpha p; r -= alpha q; z = M r; p = z + beta p;
+//                 partial sums of the direct rho = r.z and r.r for the next R
+// so K1 gathers a single vector and K2 is a pure stream.  A refinement (the
+// mixed-precision outer step, and the FP64 confirmation of convergence) is
+// C1 (k_ckpt_x) x64 += S y, y = 0 and C2 (k_ckpt_r) r64 = b - K x64 in FP64,
+// r = S r64, p = M r + beta p.
 //
 // Reductions never cross workgroups inside a kernel: on CDNA the L2 of each
 // XCD is not coherent with the others, so making one workgroup's writes
 // visible to another inside a kernel needs a device-scope release (a write
-// back of the whole L2).  Instead every kernel writes per-tile partial sums
-// and the workgroups of the NEXT kernel sum their problem's partials (in tile
-// order, so every workgroup gets the same bits) and derive the CG scalars
-// from them.  Per-problem state that later kernels need is written by one
-// designated workgroup (the problem's first tile) into fields that no other
-// workgroup of the same kernel reads; kernel boundaries publish it.
+// back of the whole L2).  Every kernel writes per-tile partial sums instead,
+// and R sums a problem's tiles in tile order.  All per-problem solver state
+// is written by R (thread 0 of the problem's workgroup), except the flag
+// C1 raises for R; kernel boundaries publish it.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,6 +55,7 @@ constexpr int TILE_SLICES = 4;
 constexpr int WG = WAVE * TILE_SLICES;
 constexpr int MAXRHS = 4;
 constexpr int MAXQ = 3 * MAXRHS;
+constexpr int NXCD = 8;
 
 enum Status : int32_t { ST_IDLE = 0, ST_RUNNING = 1, ST_CONVERGED = 2, ST_MAXIT = 3, ST_NAN = 4, ST_STALLED = 5 };
 
@@ -48,28 +65,23 @@ struct ProbDev {
     int32_t slice0, nslices;
     int64_t row0;
     int32_t nrows;
-    int32_t refine;          // 0 replace residual, keep p; 1 restart at each refinement
+    int32_t refine;          // 0 replace the residual, keep p; 1 restart at each refinement
     double rtol, drop;
     int64_t maxit;
-    int64_t ckpt_interval;   // FP64 refinement at least every this many iterations
+    int64_t ckpt_interval;   // refinement at least every this many iterations
     int32_t stall_limit;     // > 0: ST_STALLED after that many refinements without halving
-    // State written by the designated workgroup and read by later kernels.
-    // Fields the p-update kernel reads AND its designated workgroup writes
-    // are kept per iteration parity: the kernel of parity par reads slot
-    // par ^ 1 and writes slot par, so no workgroup sees a value of its own
-    // kernel; the refinement kernels write slot 1, which the next p update
-    // (parity 0) reads.
+    // state, written by k_reduce (src_ckpt: raised by k_ckpt_x)
     int32_t status;          // a finished status only ever stops work
     int32_t ckpt_pending;    // a refinement is due at the next checkpoint slot
-    int32_t src[2];          // 1: the latest rho/rr partials are a refinement's
-    int32_t first[2];        // 1: the next p update starts a new recurrence (beta = 0)
-    int32_t done[2][MAXRHS];
-    double relres[2][MAXRHS];
-    double rho_hist[2][MAXRHS];   // rho of the p update
+    int32_t src_ckpt;        // the latest residual partials are a refinement's
+    int32_t first;           // the next refinement starts the recurrence (p = M r)
+    int32_t done[MAXRHS];
+    double alpha[MAXRHS], beta[MAXRHS];
+    double relres[MAXRHS];
     int64_t iters, nckpt, last_ckpt;
-    double rr[MAXRHS], rr_ckpt[MAXRHS], bb[MAXRHS], bb64[MAXRHS], rel_best[MAXRHS];
+    double rr[MAXRHS], rr_ckpt[MAXRHS], rr_target[MAXRHS], bb[MAXRHS], bb64[MAXRHS], rel_best[MAXRHS];
     int32_t stall[MAXRHS];
-    // results for the host (never read by kernels)
+    // results for the host
     double relres_out[MAXRHS];
     int32_t done_out[MAXRHS];
 };
@@ -82,20 +94,23 @@ struct SolveArgs {
     const uint32_t *col;
     int64_t nslots;
     const double *v64;       // FP64 planes (unscaled K)
-    const void *vw;          // planes used by the p update's SpMV (scaled FP32 S K S, or v64)
+    const void *vw;          // planes of the iteration's SpMV (scaled FP32 S K S, or v64)
     const int32_t *tile_slice0, *tile_nslices, *tile_prob;
-    const int32_t *active;
+    const int32_t *active;   // active tiles
     const int32_t *n_active;
+    const int32_t *active_prob;
+    const int32_t *n_active_prob;
     ProbDev *prob;
-    double *part1;           // [tile][MAXQ] p.q
+    double *part1;           // [tile][MAXQ] p.q, q.z, q.Mq
     double *part2;           // [tile][MAXQ] r.z, r.r after the update
     double *partc;           // [tile][MAXQ] r.z, r.r, |b - K x|^2 after a refinement
     double *partn;           // [tile][MAXQ] |S b|^2, |b|^2
+    int64_t vplane;          // entries per right-hand-side plane (nrows_pad * b)
     double *x64;
     const double *b64;
     const double *S;         // symmetric Jacobi scaling, per dof
-    void *y, *r, *z, *p0, *p1, *q;
-    const void *minv;        // block-Jacobi inverses of the scaled diagonal blocks, per row (b*b)
+    void *y, *r, *p, *q;     // iteration vectors (float or double)
+    const void *minv;        // block-Jacobi inverses of the scaled diagonal blocks, per row (b*b, row-major)
 };
 
 // ------------------------------------------------------------ block planes
@@ -181,6 +196,7 @@ template <> struct Blk<double, 3> {
     static constexpr size_t bytes_per_slot = 72;
 };
 
+
 // ------------------------------------------------------------ reductions
 __device__ inline double wave_sum(double v)
 {
@@ -213,21 +229,8 @@ template <int NQ> __device__ inline void write_partials(double *v, double *part,
         for (int q = 0; q < NQ; q++) part[(size_t)tile * MAXQ + q] = v[q];
 }
 
-// Status of a problem as ONE value for the whole workgroup.  The p-update
-// kernel's designated workgroup may finish a problem while other workgroups
-// of that kernel start tiles of it; without the broadcast, waves of one
-// workgroup could read different values and disagree about entering the
-// workgroup barriers below.
-__device__ inline int32_t wg_status(const ProbDev &P, int32_t *sh)
-{
-    __syncthreads();
-    if (threadIdx.x == 0) *sh = __hip_atomic_load(&P.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    return *sh;
-}
-
-// Sums the partials a previous kernel wrote for problem P, in a fixed order:
-// every workgroup of the problem obtains the same totals.
+// Sums the tile partials of problem P in tile order: the same bits on every
+// run, whatever the launch shape.
 template <int NQ> __device__ inline void problem_sums(const double *part, const ProbDev &P, double *out,
                                                       double (*lds)[MAXQ])
 {
@@ -240,390 +243,420 @@ template <int NQ> __device__ inline void problem_sums(const double *part, const 
 }
 
 // ------------------------------------------------------- tile scheduling
-// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8); each
-// XCD gets a contiguous range of the active tiles, so rows whose vector
-// entries neighbouring rows gather are processed under the same L2.
-constexpr int NXCD = 8;
-template <typename F> __device__ inline void for_tiles(int na, F body)
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).  In a
+// one-workgroup-per-tile launch each XCD takes a contiguous range of the
+// active tiles, so the neighbour rows a tile gathers were mostly brought
+// into the same L2 by nearby tiles.  A bijection of [0, gridDim.x).
+__device__ inline int xcd_index()
 {
-    if (gridDim.x >= NXCD) {
-        const int xcd = blockIdx.x % NXCD, lw = blockIdx.x / NXCD;
-        const int nw = (gridDim.x - xcd + NXCD - 1) / NXCD;
-        const int per = (na + NXCD - 1) / NXCD;
-        for (int j = lw; j < per; j += nw) {
-            const int it = xcd * per + j;
-            if (it >= na) break;
-            body(it);
-        }
-    } else {
-        for (int it = blockIdx.x; it < na; it += gridDim.x) body(it);
-    }
+    const int G = gridDim.x, b = blockIdx.x, x = b % NXCD, per = G / NXCD, rem = G % NXCD;
+    return x * per + (x < rem ? x : rem) + b / NXCD;
 }
 
-// ------------------------------------------------------------ SpMV core
-// Accumulates acc[k][i] += sum_blocks A(i,j) * src(c, k, j) for one lane
-// (row) of slice s, reading the block planes `vals` of precision W.
-// src(c, k, j) is supplied by a functor (gathers of neighbouring rows).
-template <typename W, typename M, int B, int NR, typename Src>
-__device__ inline void sell_row(const SolveArgs &a, const void *vals, int s, int lane, W (*acc)[B], Src src)
-{
-    // U blocks per step: their column indices and values are all requested
-    // before the dependent gathers, so each wavefront keeps 2U independent
-    // loads in flight instead of one load -> gather chain per block.
-    constexpr int U = 4;
-    const int w = a.slice_width[s];
-    const int64_t sp = a.slice_ptr[s] + lane;
-    int k = 0;
-    for (; k + U <= w; k += U) {
-        uint32_t c[U];
-        M blk[U][B * B];
+// The wavefront index as a scalar (uniform) value.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
+
+// ------------------------------------------------------------ vector nodes
+// The b components of one right-hand side of one row: a float2 / double2
+// (b = 2) or three scalars (b = 3).  NT: streaming (non-temporal) access.
+template <typename W, int B, bool NT = false> struct Node {
+    __device__ static inline void load(const W *v, W *o)
+    {
+        if constexpr (B == 2) {
+            typedef W v2 __attribute__((ext_vector_type(2)));
+            const v2 t = NT ? __builtin_nontemporal_load(reinterpret_cast<const v2 *>(v)) : *reinterpret_cast<const v2 *>(v);
+            o[0] = t.x;
+            o[1] = t.y;
+        } else {
 #pragma unroll
-        for (int u = 0; u < U; u++) c[u] = __builtin_nontemporal_load(a.col + sp + (int64_t)(k + u) * WAVE);
+            for (int j = 0; j < B; j++) o[j] = NT ? __builtin_nontemporal_load(v + j) : v[j];
+        }
+    }
+    __device__ static inline void store(W *v, const W *o)
+    {
+        if constexpr (B == 2) {
+            typedef W v2 __attribute__((ext_vector_type(2)));
+            v2 t;
+            t.x = o[0];
+            t.y = o[1];
+            if (NT) __builtin_nontemporal_store(t, reinterpret_cast<v2 *>(v));
+            else *reinterpret_cast<v2 *>(v) = t;
+        } else {
 #pragma unroll
-        for (int u = 0; u < U; u++) Blk<M, B>::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int r = 0; r < NR; r++) {
-                W xv[B];
-                src(c[u], r, xv);
-#pragma unroll
-                for (int j = 0; j < B; j++)
-#pragma unroll
-                    for (int i = 0; i < B; i++) acc[r][i] += (W)blk[u][j * B + i] * xv[j];
+            for (int j = 0; j < B; j++) {
+                if (NT) __builtin_nontemporal_store(o[j], v + j);
+                else v[j] = o[j];
             }
-    }
-    for (; k < w; k++) {
-        const int64_t slot = sp + (int64_t)k * WAVE;
-        const uint32_t c = __builtin_nontemporal_load(a.col + slot);
-        M blk[B * B];
-        Blk<M, B>::load(vals, a.nslots, slot, blk);
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            W xv[B];
-            src(c, r, xv);
-#pragma unroll
-            for (int j = 0; j < B; j++)
-#pragma unroll
-                for (int i = 0; i < B; i++) acc[r][i] += (W)blk[j * B + i] * xv[j];
         }
     }
-}
-
+};
 
 template <typename W> __device__ inline W *vw(void *p) { return reinterpret_cast<W *>(p); }
 template <typename W> __device__ inline const W *cvw(const void *p) { return reinterpret_cast<const W *>(p); }
 
-// The B components of one right-hand side of node c, as one vector load for B = 2.
-template <typename W, int B> __device__ inline void load_node(const W *v, size_t idx, W *o)
+// z = M v with M the row's inverse diagonal block (BJ) or the identity.
+template <typename W, int B, bool BJ> __device__ inline void precond(const W *mi, const W *v, W *z)
 {
-    if constexpr (B == 2) {
-        typedef W v2 __attribute__((ext_vector_type(2)));
-        const v2 t = *reinterpret_cast<const v2 *>(v + idx);
-        o[0] = t.x;
-        o[1] = t.y;
-    } else {
 #pragma unroll
-        for (int j = 0; j < B; j++) o[j] = v[idx + j];
+    for (int i = 0; i < B; i++) {
+        if (BJ) {
+            W s = 0;
+#pragma unroll
+            for (int j = 0; j < B; j++) s += mi[i * B + j] * v[j];
+            z[i] = s;
+        } else {
+            z[i] = v[i];
+        }
     }
 }
 
-// ---------------------------------------------------- K1: p update + SpMV
-// beta from the partials of the previous update (or refinement), then
-// p_new = z + beta p_old (own rows stored, neighbours recomputed on the fly),
-// q = A p_new and partial p_new . q.  With the FP64 matrix the unscaled K is
-// used with the scaling applied on the fly: q = S K (S p).
-// The designated workgroup records rho, consumes a refinement's residual
-// (convergence, stagnation) and decides when the next refinement is due.
-template <typename W, typename M, int B, int NR, bool BJ>
-__global__ __launch_bounds__(WG) void k_spmv_p(SolveArgs a, int parity)
+template <typename W, int B, bool BJ> __device__ inline void load_minv(const SolveArgs &a, int64_t row, W *mi)
 {
-    __shared__ double lds[TILE_SLICES][MAXQ];
-    __shared__ int32_t sh_status;
-    constexpr bool ONFLY = sizeof(M) == 8;
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const W *zs = BJ ? cvw<W>(a.z) : cvw<W>(a.r);
-    const W *pold = cvw<W>(parity ? a.p1 : a.p0);
-    W *pnew = vw<W>(parity ? a.p0 : a.p1);
-    W *q = vw<W>(a.q);
-    const int na = *a.n_active;
-    for_tiles(na, [&](int it) {
-        const int t = a.active[it];
-        const int p = a.tile_prob[t];
-        const ProbDev &P = a.prob[p];
-        if (wg_status(P, &sh_status) != ST_RUNNING) return;
-        const int rd = parity ^ 1;   // slot written before this kernel
-        const bool ck = P.src[rd] != 0;
-        double sums[3 * NR];
-        if (ck) problem_sums<3 * NR>(a.partc, P, sums, lds);
-        else problem_sums<2 * NR>(a.part2, P, sums, lds);
-        bool done[NR], all = true, bad = false;
-        double rel[NR];
-        W beta[NR];
+    if (BJ) {
+        const W *m = cvw<W>(a.minv) + (size_t)row * B * B;
 #pragma unroll
-        for (int k = 0; k < NR; k++) {
-            bool dk = P.done[rd][k] != 0;
-            rel[k] = P.relres[rd][k];
-            if (ck && !dk) {
-                const double rr64 = sums[2 * NR + k];
-                if (!(rr64 == rr64)) bad = true;
-                rel[k] = P.bb64[k] > 0. ? sqrt(rr64 / P.bb64[k]) : sqrt(rr64);
-                dk = rel[k] <= P.rtol;
-            }
-            if (!(sums[k] == sums[k])) bad = true;
-            done[k] = dk;
-            all = all && dk;
-            const double rho_old = P.rho_hist[rd][k];
-            const bool restart = P.first[rd] || (ck && P.refine == 1) || rho_old == 0.;
-            beta[k] = (dk || restart) ? (W)0 : (W)(sums[k] / rho_old);
-        }
-        const bool designated = t == P.tile0 && threadIdx.x == 0;
-        if (all || bad) {
-            if (designated) {
-                ProbDev &Pw = a.prob[p];
-                for (int k = 0; k < NR; k++) {
-                    Pw.done[parity][k] = Pw.done_out[k] = 1;
-                    Pw.relres[parity][k] = Pw.relres_out[k] = rel[k];
-                }
-                Pw.status = bad ? ST_NAN : ST_CONVERGED;
-            }
-            return;
-        }
-        double part[NR];
+        for (int k = 0; k < B * B; k++) mi[k] = __builtin_nontemporal_load(m + k);
+    }
+}
+
+// ------------------------------------------------------------ SpMV core
+// Blocks requested per step: their column indices and values are all loaded
+// before the dependent gathers, so a wavefront keeps many independent loads
+// in flight (a P1 row of 7 blocks is one step with 2x2 FP32 blocks).
+template <typename M, int B> struct Unroll {
+    static constexpr int U = B * B * sizeof(M) <= 16 ? 8 : (B * B * sizeof(M) <= 36 ? 4 : 2);
+};
+
+// acc[k][i] += sum over the row's blocks of A(row, c)(i, j) * x_k(c)_j for
+// lane `lane` of slice s; gather(c, xv) supplies x_k(c) for every
+// right-hand side k.  The slice width is uniform over the wavefront, so the
+// step tails are scalar branches.
+template <typename A, typename M, int B, int NR, typename G>
+__device__ inline void spmv_row(const SolveArgs &a, const void *vals, int s, int lane, A (*acc)[B], G gather)
+{
+    constexpr int U = Unroll<M, B>::U;
+    const int w = a.slice_width[s];
+    const int64_t sp = a.slice_ptr[s] + lane;
+    for (int k = 0; k < w; k += U) {
+        uint32_t c[U];
+        M blk[U][B * B];
 #pragma unroll
-        for (int r = 0; r < NR; r++) part[r] = 0.;
-        if (wave < a.tile_nslices[t]) {
-            const int s = a.tile_slice0[t] + wave;
-            const int64_t row = (int64_t)s * WAVE + lane;
-            W acc[NR][B];
+        for (int u = 0; u < U; u++)
+            if (k + u < w) c[u] = __builtin_nontemporal_load(a.col + sp + (int64_t)(k + u) * WAVE);
 #pragma unroll
-            for (int r = 0; r < NR; r++)
+        for (int u = 0; u < U; u++)
+            if (k + u < w) Blk<M, B>::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
 #pragma unroll
-                for (int i = 0; i < B; i++) acc[r][i] = 0;
-            auto src = [&](uint32_t c, int r, W *xv) {
-                W zv[B], pv[B];
-                const size_t idx = ((size_t)c * NR + r) * B;
-                load_node<W, B>(zs, idx, zv);
-                load_node<W, B>(pold, idx, pv);
-#pragma unroll
-                for (int j = 0; j < B; j++) {
-                    W v = zv[j] + beta[r] * pv[j];
-                    if (ONFLY) v *= (W)a.S[(size_t)c * B + j];
-                    xv[j] = v;
-                }
-            };
-            sell_row<W, M, B, NR>(a, a.vw, s, lane, acc, src);
-            if (lane < a.slice_rows[s]) {
+        for (int u = 0; u < U; u++)
+            if (k + u < w) {
+                A xv[NR][B];
+                gather(c[u], xv);
 #pragma unroll
                 for (int r = 0; r < NR; r++)
 #pragma unroll
-                    for (int i = 0; i < B; i++) {
-                        const size_t idx = ((size_t)row * NR + r) * B + i;
-                        W pn = zs[idx] + beta[r] * pold[idx];
-                        W qv = acc[r][i];
-                        if (ONFLY) qv *= (W)a.S[(size_t)row * B + i];
-                        pnew[idx] = pn;
-                        q[idx] = qv;
-                        part[r] += (double)pn * (double)qv;
-                    }
+                    for (int j = 0; j < B; j++)
+#pragma unroll
+                        for (int i = 0; i < B; i++) acc[r][i] += (A)blk[u][j * B + i] * xv[r][j];
             }
-        }
-        write_partials<NR>(part, a.part1, t, lds);
-        if (designated) {
-            ProbDev &Pw = a.prob[p];
-            bool stalled = false;
-            for (int k = 0; k < NR; k++) Pw.rho_hist[parity][k] = sums[k];
-            for (int k = 0; k < NR; k++) {
-                Pw.relres[parity][k] = Pw.relres_out[k] = rel[k];
-                Pw.done[parity][k] = Pw.done_out[k] = done[k];
-            }
-            if (ck) {
-                // a refinement's FP64 residual: record, watch for stagnation
-                for (int k = 0; k < NR; k++) {
-                    Pw.rr_ckpt[k] = sums[NR + k];
-                    if (done[k]) continue;
-                    if (rel[k] < 0.5 * Pw.rel_best[k]) {
-                        Pw.rel_best[k] = rel[k];
-                        Pw.stall[k] = 0;
-                    } else if (++Pw.stall[k] >= Pw.stall_limit && Pw.stall_limit > 0) {
-                        stalled = true;
-                    }
-                }
-                Pw.nckpt += 1;
-                Pw.last_ckpt = Pw.iters;
-                Pw.ckpt_pending = 0;
-            }
-            Pw.src[parity] = 0;
-            Pw.first[parity] = 0;
-            if (ck && Pw.iters >= Pw.maxit) { Pw.status = ST_MAXIT; return; }
-            if (stalled) { Pw.status = ST_STALLED; return; }
-            Pw.iters += 1;
-            bool need = Pw.iters >= Pw.maxit || Pw.iters - Pw.last_ckpt >= Pw.ckpt_interval;
-            for (int k = 0; k < NR; k++) {
-                if (done[k]) continue;
-                const double rr = sums[NR + k];
-                Pw.rr[k] = rr;
-                if (rr < Pw.drop * Pw.drop * Pw.rr_ckpt[k]) need = true;
-                if (rr <= Pw.rtol * Pw.rtol * Pw.bb[k]) need = true;
-            }
-            if (need) Pw.ckpt_pending = 1;
-        }
-    });
+    }
 }
 
-// ------------------------------------ K2: y, r update + preconditioner + dots
-template <typename W, int B, int NR, bool BJ>
-__global__ __launch_bounds__(WG) void k_update(SolveArgs a, int parity)
+// The tile of a one-workgroup-per-tile launch, or -1.
+__device__ inline int my_tile(const SolveArgs &a)
+{
+    const int it = xcd_index();
+    return it < *a.n_active ? a.active[it] : -1;
+}
+
+// ------------------------------------------------------------ K1: SpMV
+// q = Ks p with partial sums of p.q, q.z and q.Mq (z = M r).  With the FP64
+// matrix the unscaled K is used and the scaling applied on the fly:
+// q = S K (S p).
+template <typename W, typename M, int B, int NR, bool BJ>
+__global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    __shared__ int32_t sh_status;
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const W *pnew = cvw<W>(parity ? a.p0 : a.p1);
-    const W *q = cvw<W>(a.q);
-    W *y = vw<W>(a.y), *r = vw<W>(a.r), *z = vw<W>(a.z);
-    const W *minv = cvw<W>(a.minv);
-    const int na = *a.n_active;
-    for_tiles(na, [&](int it) {
-        const int t = a.active[it];
-        const int p = a.tile_prob[t];
-        const ProbDev &P = a.prob[p];
-        if (wg_status(P, &sh_status) != ST_RUNNING) return;
-        double pq[NR];
-        problem_sums<NR>(a.part1, P, pq, lds);
-        W alpha[NR];
+    constexpr bool ONFLY = sizeof(M) == 8;
+    const int t = my_tile(a);
+    if (t < 0) return;
+    const ProbDev &P = a.prob[a.tile_prob[t]];
+    if (P.status != ST_RUNNING) return;
+    const int wave = wave_id(), lane = threadIdx.x % WAVE;
+    const W *p = cvw<W>(a.p), *r = cvw<W>(a.r);
+    W *q = vw<W>(a.q);
+    double part[3 * NR];
+#pragma unroll
+    for (int k = 0; k < 3 * NR; k++) part[k] = 0.;
+    if (wave < a.tile_nslices[t]) {
+        const int s = a.tile_slice0[t] + wave;
+        const int64_t row = (int64_t)s * WAVE + lane;
+        W acc[NR][B];
 #pragma unroll
         for (int k = 0; k < NR; k++)
-            alpha[k] = (P.done[parity][k] || pq[k] == 0.) ? (W)0 : (W)(P.rho_hist[parity][k] / pq[k]);
-        double part[2 * NR];
 #pragma unroll
-        for (int k = 0; k < 2 * NR; k++) part[k] = 0.;
-        if (wave < a.tile_nslices[t]) {
-            const int s = a.tile_slice0[t] + wave;
-            const int64_t row = (int64_t)s * WAVE + lane;
-            if (lane < a.slice_rows[s]) {
-                W mi[B * B];
-                if (BJ) {
+            for (int i = 0; i < B; i++) acc[k][i] = 0;
+        auto gather = [&](uint32_t c, W (*xv)[B]) {
+            double sc[B];
+            if (ONFLY) Node<double, B>::load(a.S + (size_t)c * B, sc);
 #pragma unroll
-                    for (int k = 0; k < B * B; k++) mi[k] = minv[(size_t)row * B * B + k];
-                }
+            for (int k = 0; k < NR; k++) {
+                Node<W, B>::load(p + k * a.vplane + (size_t)c * B, xv[k]);
+                if (ONFLY)
 #pragma unroll
-                for (int k = 0; k < NR; k++) {
-                    W rv[B];
+                    for (int j = 0; j < B; j++) xv[k][j] *= (W)sc[j];
+            }
+        };
+        spmv_row<W, M, B, NR>(a, a.vw, s, lane, acc, gather);
+        if (lane < a.slice_rows[s]) {
+            W mi[B * B];
+            load_minv<W, B, BJ>(a, row, mi);
+            double sr[B];
+            if (ONFLY) Node<double, B>::load(a.S + (size_t)row * B, sr);
 #pragma unroll
-                    for (int i = 0; i < B; i++) {
-                        const size_t idx = ((size_t)row * NR + k) * B + i;
-                        y[idx] = y[idx] + alpha[k] * pnew[idx];
-                        rv[i] = r[idx] - alpha[k] * q[idx];
-                        r[idx] = rv[i];
-                    }
+            for (int k = 0; k < NR; k++) {
+                const size_t idx = k * a.vplane + (size_t)row * B;
+                W pv[B], rv[B], qv[B], zv[B], mq[B];
+                Node<W, B>::load(p + idx, pv);
+                Node<W, B, true>::load(r + idx, rv);
 #pragma unroll
-                    for (int i = 0; i < B; i++) {
-                        W zv = rv[i];
-                        if (BJ) {
-                            zv = 0;
+                for (int i = 0; i < B; i++) qv[i] = ONFLY ? (W)(acc[k][i] * (W)sr[i]) : acc[k][i];
+                Node<W, B, true>::store(q + idx, qv);
+                precond<W, B, BJ>(mi, rv, zv);
+                precond<W, B, BJ>(mi, qv, mq);
 #pragma unroll
-                            for (int j = 0; j < B; j++) zv += mi[i * B + j] * rv[j];
-                            z[((size_t)row * NR + k) * B + i] = zv;
-                        }
-                        part[k] += (double)rv[i] * (double)zv;
-                        part[NR + k] += (double)rv[i] * (double)rv[i];
-                    }
+                for (int i = 0; i < B; i++) {
+                    part[k] += (double)pv[i] * (double)qv[i];
+                    part[NR + k] += (double)qv[i] * (double)zv[i];
+                    part[2 * NR + k] += (double)qv[i] * (double)mq[i];
                 }
             }
         }
-        write_partials<2 * NR>(part, a.part2, t, lds);
-    });
+    }
+    write_partials<3 * NR>(part, a.part1, t, lds);
+}
+
+// ------------------------------------------- R: per-problem CG scalars
+// One workgroup per active problem.  Consumes the residual partials of the
+// previous update (or refinement) and this iteration's SpMV partials.
+template <int NR>
+__global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
+{
+    __shared__ double lds[TILE_SLICES][MAXQ];
+    if ((int)blockIdx.x >= *a.n_active_prob) return;
+    ProbDev &P = a.prob[a.active_prob[blockIdx.x]];
+    if (P.status != ST_RUNNING) return;   // read by every thread before thread 0 writes (barriers below)
+    const bool ck = P.src_ckpt != 0;
+    double s1[3 * NR], s0[3 * NR];
+    problem_sums<3 * NR>(a.part1, P, s1, lds);
+    if (ck) problem_sums<3 * NR>(a.partc, P, s0, lds);
+    else problem_sums<2 * NR>(a.part2, P, s0, lds);
+    if (threadIdx.x != 0) return;
+    bool all = true, bad = false, stalled = false, need = false;
+    for (int k = 0; k < NR; k++) {
+        if (P.done[k]) {
+            P.alpha[k] = P.beta[k] = 0.;
+            continue;
+        }
+        const double rho = s0[k], rr = s0[NR + k];
+        if (ck) {
+            // the refinement's FP64 residual decides convergence
+            const double rr64 = s0[2 * NR + k];
+            if (!(rr64 == rr64)) bad = true;
+            const double rel = P.bb64[k] > 0. ? sqrt(rr64 / P.bb64[k]) : sqrt(rr64);
+            P.relres[k] = P.relres_out[k] = rel;
+            P.rr_ckpt[k] = rr;
+            if (rel <= P.rtol) {
+                P.done[k] = P.done_out[k] = 1;
+                P.alpha[k] = P.beta[k] = 0.;
+                continue;
+            }
+            // expected working-precision r.r at convergence, from this ratio
+            P.rr_target[k] = rr * (P.rtol / rel) * (P.rtol / rel);
+            if (rel < 0.5 * P.rel_best[k]) {
+                P.rel_best[k] = rel;
+                P.stall[k] = 0;
+            } else if (P.stall_limit > 0 && ++P.stall[k] >= P.stall_limit) {
+                stalled = true;
+            }
+        }
+        all = false;
+        const double pq = s1[k], qz = s1[NR + k], qmq = s1[2 * NR + k];
+        if (!(pq == pq) || !(rho == rho) || !(qz == qz) || !(qmq == qmq)) bad = true;
+        const double alpha = pq != 0. ? rho / pq : 0.;
+        const double rho1 = rho - 2. * alpha * qz + alpha * alpha * qmq;
+        P.alpha[k] = alpha;
+        P.beta[k] = rho != 0. ? rho1 / rho : 0.;
+        P.rr[k] = rr;
+        if (rr < P.drop * P.drop * P.rr_ckpt[k]) need = true;   // FP32 residual dropped: refine
+        if (rr <= P.rr_target[k]) need = true;                   // may have converged: confirm in FP64
+    }
+    if (ck) {
+        P.nckpt += 1;
+        P.last_ckpt = P.iters;
+        P.ckpt_pending = 0;
+        P.src_ckpt = 0;
+        P.first = 0;
+    }
+    if (bad) { P.status = ST_NAN; return; }
+    if (all) { P.status = ST_CONVERGED; return; }
+    if (ck && P.iters >= P.maxit) { P.status = ST_MAXIT; return; }
+    if (stalled) { P.status = ST_STALLED; return; }
+    P.iters += 1;
+    if (P.iters >= P.maxit || P.iters - P.last_ckpt >= P.ckpt_interval) need = true;
+    if (need) P.ckpt_pending = 1;
+}
+
+// ------------------------------------ K2: y, r, p update + dots (streaming)
+// slot != 0: a checkpoint follows this iteration; a problem with a pending
+// refinement then only completes its y update (C2 replaces r and forms p).
+template <typename W, int B, int NR, bool BJ>
+__global__ __launch_bounds__(WG) void k_update(SolveArgs a, int slot)
+{
+    __shared__ double lds[TILE_SLICES][MAXQ];
+    const int t = my_tile(a);
+    if (t < 0) return;
+    const ProbDev &P = a.prob[a.tile_prob[t]];
+    if (P.status != ST_RUNNING) return;
+    const bool ck = slot && P.ckpt_pending;
+    const int wave = wave_id(), lane = threadIdx.x % WAVE;
+    W *p = vw<W>(a.p), *r = vw<W>(a.r), *y = vw<W>(a.y);
+    const W *q = cvw<W>(a.q);
+    double part[2 * NR];
+#pragma unroll
+    for (int k = 0; k < 2 * NR; k++) part[k] = 0.;
+    if (wave < a.tile_nslices[t]) {
+        const int s = a.tile_slice0[t] + wave;
+        const int64_t row = (int64_t)s * WAVE + lane;
+        if (lane < a.slice_rows[s]) {
+            W mi[B * B];
+            if (!ck) load_minv<W, B, BJ>(a, row, mi);
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                if (P.done[k]) continue;
+                const W al = (W)P.alpha[k], be = (W)P.beta[k];
+                const size_t idx = k * a.vplane + (size_t)row * B;
+                W pv[B], yv[B];
+                Node<W, B, true>::load(p + idx, pv);
+                Node<W, B, true>::load(y + idx, yv);
+#pragma unroll
+                for (int i = 0; i < B; i++) yv[i] += al * pv[i];
+                Node<W, B, true>::store(y + idx, yv);
+                if (ck) continue;
+                W qv[B], rv[B], zv[B];
+                Node<W, B, true>::load(q + idx, qv);
+                Node<W, B, true>::load(r + idx, rv);
+#pragma unroll
+                for (int i = 0; i < B; i++) rv[i] -= al * qv[i];
+                Node<W, B, true>::store(r + idx, rv);
+                precond<W, B, BJ>(mi, rv, zv);
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    pv[i] = zv[i] + be * pv[i];
+                    part[k] += (double)rv[i] * (double)zv[i];
+                    part[NR + k] += (double)rv[i] * (double)rv[i];
+                }
+                Node<W, B, true>::store(p + idx, pv);
+            }
+        }
+    }
+    if (!ck) write_partials<2 * NR>(part, a.part2, t, lds);
 }
 
 // ------------------------------------------- refinement 1: x64 += S y, y = 0
 template <typename W, int B, int NR>
 __global__ __launch_bounds__(WG) void k_ckpt_x(SolveArgs a)
 {
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int t = my_tile(a);
+    if (t < 0) return;
+    const int pi = a.tile_prob[t];
+    const ProbDev &P = a.prob[pi];
+    if (P.status != ST_RUNNING || !P.ckpt_pending) return;
+    if (t == P.tile0 && threadIdx.x == 0) a.prob[pi].src_ckpt = 1;
+    const int wave = wave_id(), lane = threadIdx.x % WAVE;
+    if (wave >= a.tile_nslices[t]) return;
+    const int s = a.tile_slice0[t] + wave;
+    if (lane >= a.slice_rows[s]) return;
+    const int64_t row = (int64_t)s * WAVE + lane;
     W *y = vw<W>(a.y);
-    const int na = *a.n_active;
-    for_tiles(na, [&](int it) {
-        const int t = a.active[it];
-        const ProbDev &P = a.prob[a.tile_prob[t]];
-        if (P.status != ST_RUNNING || !P.ckpt_pending) return;
-        if (t == P.tile0 && threadIdx.x == 0) a.prob[a.tile_prob[t]].src[1] = 1;
-        if (wave >= a.tile_nslices[t]) return;
-        const int s = a.tile_slice0[t] + wave;
-        if (lane >= a.slice_rows[s]) return;
-        const int64_t row = (int64_t)s * WAVE + lane;
+    double sr[B];
+    Node<double, B>::load(a.S + (size_t)row * B, sr);
 #pragma unroll
-        for (int k = 0; k < NR; k++)
+    for (int k = 0; k < NR; k++) {
+        const size_t idx = k * a.vplane + (size_t)row * B;
+        W yv[B];
+        double xv[B];
+        Node<W, B>::load(y + idx, yv);
+        Node<double, B>::load(a.x64 + idx, xv);
 #pragma unroll
-            for (int i = 0; i < B; i++) {
-                const size_t idx = ((size_t)row * NR + k) * B + i;
-                a.x64[idx] += a.S[(size_t)row * B + i] * (double)y[idx];
-                y[idx] = 0;
-            }
-    });
+        for (int i = 0; i < B; i++) {
+            xv[i] += sr[i] * (double)yv[i];
+            yv[i] = 0;
+        }
+        Node<double, B>::store(a.x64 + idx, xv);
+        Node<W, B>::store(y + idx, yv);
+    }
 }
 
-// -------------------- refinement 2: r64 = b - K x64 (FP64), r = S r64 (W), z
+// ------------- refinement 2: r64 = b - K x64 (FP64), r = S r64, p = M r + beta p
 template <typename W, int B, int NR, bool BJ>
 __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    __shared__ int32_t sh_status;
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    W *r = vw<W>(a.r), *z = vw<W>(a.z);
-    const W *minv = cvw<W>(a.minv);
-    const int na = *a.n_active;
-    for_tiles(na, [&](int it) {
-        const int t = a.active[it];
-        const int p = a.tile_prob[t];
-        const ProbDev &P = a.prob[p];
-        if (P.status != ST_RUNNING || !P.ckpt_pending) return;
-        double part[3 * NR];
+    const int t = my_tile(a);
+    if (t < 0) return;
+    const ProbDev &P = a.prob[a.tile_prob[t]];
+    if (P.status != ST_RUNNING || !P.ckpt_pending) return;
+    const int wave = wave_id(), lane = threadIdx.x % WAVE;
+    W *r = vw<W>(a.r), *p = vw<W>(a.p);
+    double part[3 * NR];
 #pragma unroll
-        for (int k = 0; k < 3 * NR; k++) part[k] = 0.;
-        if (wave < a.tile_nslices[t]) {
-            const int s = a.tile_slice0[t] + wave;
-            const int64_t row = (int64_t)s * WAVE + lane;
-            double acc[NR][B];
+    for (int k = 0; k < 3 * NR; k++) part[k] = 0.;
+    if (wave < a.tile_nslices[t]) {
+        const int s = a.tile_slice0[t] + wave;
+        const int64_t row = (int64_t)s * WAVE + lane;
+        double acc[NR][B];
 #pragma unroll
-            for (int k = 0; k < NR; k++)
+        for (int k = 0; k < NR; k++)
 #pragma unroll
-                for (int i = 0; i < B; i++) acc[k][i] = 0.;
-            auto src = [&](uint32_t c, int k, double *xv) { load_node<double, B>(a.x64, ((size_t)c * NR + k) * B, xv); };
-            sell_row<double, double, B, NR>(a, a.v64, s, lane, acc, src);
-            if (lane < a.slice_rows[s]) {
-                W mi[B * B];
-                if (BJ) {
+            for (int i = 0; i < B; i++) acc[k][i] = 0.;
+        auto gather = [&](uint32_t c, double (*xv)[B]) {
 #pragma unroll
-                    for (int k = 0; k < B * B; k++) mi[k] = minv[(size_t)row * B * B + k];
+            for (int k = 0; k < NR; k++) Node<double, B>::load(a.x64 + k * a.vplane + (size_t)c * B, xv[k]);
+        };
+        spmv_row<double, double, B, NR>(a, a.v64, s, lane, acc, gather);
+        if (lane < a.slice_rows[s]) {
+            W mi[B * B];
+            load_minv<W, B, BJ>(a, row, mi);
+            double sr[B];
+            Node<double, B>::load(a.S + (size_t)row * B, sr);
+            const bool restart = P.first || P.refine == 1;
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                const size_t idx = k * a.vplane + (size_t)row * B;
+                const W be = restart ? (W)0 : (W)P.beta[k];
+                double bv[B];
+                Node<double, B>::load(a.b64 + idx, bv);
+                W rv[B], zv[B], pv[B];
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    const double r64 = bv[i] - acc[k][i];
+                    part[2 * NR + k] += r64 * r64;
+                    rv[i] = (W)(sr[i] * r64);
                 }
+                Node<W, B>::store(r + idx, rv);
+                precond<W, B, BJ>(mi, rv, zv);
+                Node<W, B>::load(p + idx, pv);
 #pragma unroll
-                for (int k = 0; k < NR; k++) {
-                    W rv[B];
-#pragma unroll
-                    for (int i = 0; i < B; i++) {
-                        const size_t idx = ((size_t)row * NR + k) * B + i;
-                        double r64 = a.b64[idx] - acc[k][i];
-                        part[2 * NR + k] += r64 * r64;
-                        rv[i] = (W)(a.S[(size_t)row * B + i] * r64);
-                        r[idx] = rv[i];
-                    }
-#pragma unroll
-                    for (int i = 0; i < B; i++) {
-                        W zv = rv[i];
-                        if (BJ) {
-                            zv = 0;
-#pragma unroll
-                            for (int j = 0; j < B; j++) zv += mi[i * B + j] * rv[j];
-                            z[((size_t)row * NR + k) * B + i] = zv;
-                        }
-                        part[k] += (double)rv[i] * (double)zv;
-                        part[NR + k] += (double)rv[i] * (double)rv[i];
-                    }
+                for (int i = 0; i < B; i++) {
+                    pv[i] = zv[i] + be * pv[i];
+                    part[k] += (double)rv[i] * (double)zv[i];
+                    part[NR + k] += (double)rv[i] * (double)rv[i];
                 }
+                Node<W, B>::store(p + idx, pv);
             }
         }
-        write_partials<3 * NR>(part, a.partc, t, lds);
-    });
+    }
+    write_partials<3 * NR>(part, a.partc, t, lds);
 }
 
 // ------------------------------------------ norms of the right-hand sides
@@ -631,65 +664,61 @@ template <int B, int NR>
 __global__ __launch_bounds__(WG) void k_rhs_norms(SolveArgs a)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int na = *a.n_active;
-    for_tiles(na, [&](int it) {
-        const int t = a.active[it];
-        double part[2 * NR];
+    const int t = my_tile(a);
+    if (t < 0) return;
+    const int wave = wave_id(), lane = threadIdx.x % WAVE;
+    double part[2 * NR];
 #pragma unroll
-        for (int k = 0; k < 2 * NR; k++) part[k] = 0.;
-        if (wave < a.tile_nslices[t]) {
-            const int s = a.tile_slice0[t] + wave;
-            const int64_t row = (int64_t)s * WAVE + lane;
-            if (lane < a.slice_rows[s]) {
+    for (int k = 0; k < 2 * NR; k++) part[k] = 0.;
+    if (wave < a.tile_nslices[t]) {
+        const int s = a.tile_slice0[t] + wave;
+        const int64_t row = (int64_t)s * WAVE + lane;
+        if (lane < a.slice_rows[s]) {
 #pragma unroll
-                for (int k = 0; k < NR; k++)
+            for (int k = 0; k < NR; k++)
 #pragma unroll
-                    for (int i = 0; i < B; i++) {
-                        double bv = a.b64[((size_t)row * NR + k) * B + i];
-                        double sb = a.S[(size_t)row * B + i] * bv;
-                        part[k] += sb * sb;
-                        part[NR + k] += bv * bv;
-                    }
-            }
+                for (int i = 0; i < B; i++) {
+                    const double bv = a.b64[k * a.vplane + (size_t)row * B + i];
+                    const double sb = a.S[(size_t)row * B + i] * bv;
+                    part[k] += sb * sb;
+                    part[NR + k] += bv * bv;
+                }
         }
-        write_partials<2 * NR>(part, a.partn, t, lds);
-    });
+    }
+    write_partials<2 * NR>(part, a.partn, t, lds);
 }
 
-// One workgroup per problem: |S b|^2 and |b|^2 from the norm partials.
+// One workgroup per active problem: |S b|^2 and |b|^2 from the norm partials.
 template <int NR>
-__global__ __launch_bounds__(WG) void k_norms_final(SolveArgs a, int nprob)
+__global__ __launch_bounds__(WG) void k_norms_final(SolveArgs a)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
-        const ProbDev &P = a.prob[p];
-        if (P.status != ST_RUNNING) continue;
-        double sums[2 * NR];
-        problem_sums<2 * NR>(a.partn, P, sums, lds);
-        if (threadIdx.x == 0) {
-            ProbDev &Pw = a.prob[p];
-            for (int k = 0; k < NR; k++) {
-                Pw.bb[k] = sums[k];
-                Pw.bb64[k] = sums[NR + k];
-                // a homogeneous right-hand side is solved by x = 0
-                // (solvers/conjugategradient.cpp:74-78)
-                if (sums[NR + k] == 0.) {
-                    Pw.done[0][k] = Pw.done[1][k] = Pw.done_out[k] = 1;
-                    Pw.relres[0][k] = Pw.relres[1][k] = Pw.relres_out[k] = 0.;
-                }
-            }
+    if ((int)blockIdx.x >= *a.n_active_prob) return;
+    ProbDev &P = a.prob[a.active_prob[blockIdx.x]];
+    if (P.status != ST_RUNNING) return;
+    double sums[2 * NR];
+    problem_sums<2 * NR>(a.partn, P, sums, lds);
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < NR; k++) {
+        P.bb[k] = sums[k];
+        P.bb64[k] = sums[NR + k];
+        P.rr_target[k] = P.rtol * P.rtol * sums[k];
+        // a homogeneous right-hand side is solved by x = 0
+        // (solvers/conjugategradient.cpp:74-78)
+        if (sums[NR + k] == 0.) {
+            P.done[k] = P.done_out[k] = 1;
+            P.relres[k] = P.relres_out[k] = 0.;
         }
     }
 }
 
 // ------------------------------------------------- plain SpMV (test hook)
 // y = K x with the FP64 planes (mode 0) or S^-1 (Ks (S^-1 x)) with the
-// scaled working planes Ks = S K S (mode 1).  One lane per row, every tile.
+// scaled working planes Ks = S K S (mode 1).  x, y: FP64 planes.
 template <typename W, int B, int NR>
 __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const double *x, double *yout, int ntiles)
 {
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int wave = wave_id(), lane = threadIdx.x % WAVE;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         if (wave >= a.tile_nslices[t]) continue;
         const int s = a.tile_slice0[t] + wave;
@@ -700,29 +729,32 @@ __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const 
             for (int k = 0; k < NR; k++)
 #pragma unroll
                 for (int i = 0; i < B; i++) acc[k][i] = 0.;
-            auto src = [&](uint32_t c, int k, double *xv) {
+            auto gather = [&](uint32_t c, double (*xv)[B]) {
 #pragma unroll
-                for (int j = 0; j < B; j++) xv[j] = x[((size_t)c * NR + k) * B + j];
+                for (int k = 0; k < NR; k++) Node<double, B>::load(x + k * a.vplane + (size_t)c * B, xv[k]);
             };
-            sell_row<double, double, B, NR>(a, a.v64, s, lane, acc, src);
+            spmv_row<double, double, B, NR>(a, a.v64, s, lane, acc, gather);
             if (lane < a.slice_rows[s])
                 for (int k = 0; k < NR; k++)
-                    for (int i = 0; i < B; i++) yout[((size_t)row * NR + k) * B + i] = acc[k][i];
+                    for (int i = 0; i < B; i++) yout[k * a.vplane + (size_t)row * B + i] = acc[k][i];
         } else {
             W acc[NR][B];
 #pragma unroll
             for (int k = 0; k < NR; k++)
 #pragma unroll
                 for (int i = 0; i < B; i++) acc[k][i] = 0;
-            auto src = [&](uint32_t c, int k, W *xv) {
+            auto gather = [&](uint32_t c, W (*xv)[B]) {
 #pragma unroll
-                for (int j = 0; j < B; j++) xv[j] = (W)(x[((size_t)c * NR + k) * B + j] / a.S[(size_t)c * B + j]);
+                for (int k = 0; k < NR; k++)
+#pragma unroll
+                    for (int j = 0; j < B; j++)
+                        xv[k][j] = (W)(x[k * a.vplane + (size_t)c * B + j] / a.S[(size_t)c * B + j]);
             };
-            sell_row<W, W, B, NR>(a, a.vw, s, lane, acc, src);
+            spmv_row<W, W, B, NR>(a, a.vw, s, lane, acc, gather);
             if (lane < a.slice_rows[s])
                 for (int k = 0; k < NR; k++)
                     for (int i = 0; i < B; i++)
-                        yout[((size_t)row * NR + k) * B + i] = (double)acc[k][i] / a.S[(size_t)row * B + i];
+                        yout[k * a.vplane + (size_t)row * B + i] = (double)acc[k][i] / a.S[(size_t)row * B + i];
         }
     }
 }

@@ -149,13 +149,8 @@ struct Server {
             done.push_back(r);
             // solution of the request (original order, nrhs columns)
             const ProbHost &H = B.ph[i];
-            const int b = B.et.b, NR = B.nrhs;
-            const int64_t ndof = H.nrows * b;
-            std::vector<double> h((size_t)H.nslices * WAVE * b * NR), x((size_t)ndof * NR);
-            if (hipMemcpy(h.data(), B.x64.as<double>() + (size_t)H.row0 * b * NR, h.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
-                return AMIE_ERR_HIP;
-            for (int k = 0; k < NR; k++)
-                for (int64_t d = 0; d < ndof; d++) x[(size_t)k * ndof + d] = h[((size_t)H.iperm[d / b] * NR + k) * b + d % b];
+            std::vector<double> x((size_t)H.nrows * B.et.b * B.nrhs);
+            if (int rc = download_vec(B, (int)i, B.x64, x.data())) return rc;
             solutions[r.ticket] = std::move(x);
         }
         used_slots -= (int)g->reqs.size();
