@@ -4,13 +4,16 @@ Workload (one "step"): 512 periodic P1 RVEs (n ~ U{32..256}, seed 4, 2k-130k
 DOFs each, 20-45 % circular inclusions, 12/60 GPa), three macro-strain load
 cases each (1536 right-hand sides), resident in HBM.  A step runs the whole
 hot path on the device: element integration + coloured scatter + Dirichlet
-pin + scaling (assembly), the FP32 CG with FP64 refinement of all 1536
+pin + scaling (assembly), the mixed-precision CG (FP32 vectors, FP32+bf16
+split matrix with FP64 accumulation, FP64 refinement) of all 1536
 right-hand sides to relative residual 1e-8 in one batch, and the
 volume-averaged stress (effective stiffness) of every RVE.
 
 value = RVE load-case solves per second (1536 per step / step time).
-roofline = the fused SpMV kernel (k_spmv_p), timed with HIP events on the
-library stream during the timed steps.
+roofline = the SpMV kernel (k_spmv), timed with HIP events on the library
+stream during the timed steps.
+Beside it: the same step with the all-FP32 inner iteration, and with the
+plain FP64 CG (same kernels, FP64 matrix and vectors).
 cpu_baseline = the oracle's threaded FP64 Jacobi PCG (rtol 1e-8) on a bounded
 sample of the same RVEs.
 """
@@ -101,13 +104,14 @@ def main():
         ceff = B.average_stress()
         return st, ceff
 
+    headline = ag.MIXED_SPLIT
     for w in range(args.warmup):
-        st, _ = step(ag.MIXED)
+        st, _ = step(headline)
         log(f"warmup {w}: {st['solve_ms']:.1f} ms solve, iters max {st['iterations_max']}, status {st['status']}")
     sts = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        st, ceff = step(ag.MIXED)
+        st, ceff = step(headline)
         sts.append(st)
         log(f"step {k}: solve {st['solve_ms']:.1f} ms, iters max {st['iterations_max']}, relres max "
             f"{st['relres_max']:.2e}, converged {st['nproblems_converged']}/{len(problems)}")
@@ -120,15 +124,25 @@ def main():
     launches = sum(s["spmv_launches"] for s in sts)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     ok = all(s["status"] == 0 for s in sts)
+    sample = list(range(0, len(problems), max(1, len(problems) // 8)))[:8]
+    x_head = {k: B.solution(k).copy() for k in sample}
 
-    fp64 = None
-    if not args.no_fp64:
+    def other(precision, name):
         t1 = time.perf_counter()
-        st64, _ = step(ag.FP64, time_kernels=0)
+        st2, ceff2 = step(precision, time_kernels=0)
         dt = time.perf_counter() - t1
-        fp64 = {"solves_per_s": solves / dt, "ms_per_step": 1e3 * dt, "iterations_max": st64["iterations_max"],
-                "status": st64["status"]}
-        log(f"fp64 batched step: {1e3 * dt:.1f} ms, iters max {st64['iterations_max']}")
+        log(f"{name} step: {1e3 * dt:.1f} ms, iters max {st2['iterations_max']}, status {st2['status']}")
+        return {"solves_per_s": solves / dt, "ms_per_step": 1e3 * dt, "iterations_max": st2["iterations_max"],
+                "iterations_sum": st2["iterations_sum"], "status": st2["status"]}, ceff2
+
+    fp32, fp64, accuracy = None, None, None
+    if not args.no_fp64:
+        fp32, _ = other(ag.MIXED, "all-FP32 inner iteration")
+        fp64, ceff64 = other(ag.FP64, "plain FP64 CG")
+        errs = [float(np.linalg.norm(x_head[k] - B.solution(k)) / np.linalg.norm(B.solution(k))) for k in sample]
+        accuracy = {"displacement_rel_err_vs_fp64_max": max(errs), "sample_rves": len(sample),
+                    "ceff_rel_diff_vs_fp64_max": float(np.abs(ceff - ceff64).max() / np.abs(ceff64).max())}
+        log(f"accuracy vs FP64 CG: {accuracy}")
 
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
@@ -147,18 +161,22 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{len(problems)} periodic P1 RVEs n~U{{32..{args.nmax}}} (seed 4), 3 macro-strain "
-                               f"load cases each, FP32 CG + FP64 refinement to rel.res {args.rtol:g}, "
+                               f"load cases each, mixed-precision CG (FP32 vectors, FP32+bf16 matrix, FP64 "
+                               f"accumulation and refinement) to rel.res {args.rtol:g}, "
                                f"assembly + solve + effective stiffness per step",
                    "rves": len(problems), "load_cases": problems[0].nrhs, "dofs": ndof, "blocks": nnzb,
                    "hbm_bytes": B.nbytes, "converged": ok},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "stream_copy_gbs": copy_gbs,
-                     "kernel": "k_spmv_p (fused p-update + block SpMV + p.q)", "launches": launches,
+                     "kernel": "k_spmv (block SpMV q = Ks p, split FP32+bf16 values, fused p.q, q.z, q.Mq)",
+                     "launches": launches,
                      "avg_launch_us": 1e3 * spmv_ms / max(1, launches),
                      "avg_bytes_per_launch": spmv_bytes / max(1, launches)},
         "cpu_baseline": cpu,
         "fp64_batched": fp64,
+        "fp32_inner_batched": fp32,
         "mixed_vs_fp64": (value / fp64["solves_per_s"]) if fp64 else None,
+        "accuracy": accuracy,
     }
     print(json.dumps(out), flush=True)
 

@@ -51,11 +51,20 @@ enum {
 
 enum { AMIE_P1 = 1, AMIE_P2 = 2, AMIE_TET4 = 3 };
 
-/* solve precision: AMIE_FP64 = FP64 matrix and vectors; AMIE_MIXED = FP32
- * matrix, vectors and SpMV inside, FP64 residual / update / scalars outside
- * (problems whose FP32 iteration stagnates continue in AMIE_FP64 when
- * opts.escalate is set); AMIE_MIXED64 = FP32 matrix with FP64 vectors. */
-enum { AMIE_FP64 = 0, AMIE_MIXED = 1, AMIE_MIXED64 = 2 };
+/* solve precision.  Every mixed mode keeps the residual r = b - K x, the
+ * solution and the CG scalars in FP64 (refinement) and iterates on FP32 or
+ * FP64 vectors inside:
+ *   AMIE_FP64         FP64 matrix and vectors
+ *   AMIE_MIXED        FP32 matrix, vectors and SpMV accumulation
+ *   AMIE_MIXED64      FP32 matrix, FP64 vectors
+ *   AMIE_MIXED_K64    FP64 matrix, FP32 vectors, FP64 accumulation
+ *   AMIE_MIXED_SPLIT  matrix as FP32 + bf16 remainder (6 bytes a value,
+ *                     relative error < 2^-32), FP32 vectors, FP64
+ *                     accumulation: the FP64 iteration count at nearly the
+ *                     FP32 traffic
+ * Problems whose FP32-vector iteration stagnates continue in AMIE_FP64 when
+ * opts.escalate is set. */
+enum { AMIE_FP64 = 0, AMIE_MIXED = 1, AMIE_MIXED64 = 2, AMIE_MIXED_K64 = 3, AMIE_MIXED_SPLIT = 4 };
 enum { AMIE_JACOBI = 0, AMIE_BLOCK_JACOBI = 1 };      /* preconditioner */
 enum { AMIE_REFINE_REPLACE = 0, AMIE_REFINE_RESTART = 1 };
 

@@ -137,6 +137,11 @@ def test_spmv_fp64_fp32(ctx, make):
     assert np.abs(y64 - yref).max() <= 1e-13 * scale
     y32 = B.spmv(x, precision=ag.MIXED)
     assert np.abs(y32 - yref).max() <= 2e-6 * scale
+    # FP32 + bf16 split values (relative error < 2^-32), FP64 accumulation
+    ysp = B.spmv(x, precision=ag.MIXED_SPLIT)
+    err = np.abs(ysp - yref).max()
+    assert err <= 1e-9 * scale
+    assert err < 1e-2 * np.abs(y32 - yref).max()
 
 
 def test_fp64_cg_patch_test(ctx):
@@ -162,7 +167,8 @@ def oracle_solution(p, rtol=1e-13, precond=0):
 
 @pytest.mark.parametrize("precision,precond", [(ag.FP64, ag.JACOBI), (ag.MIXED, ag.JACOBI),
                                                (ag.MIXED, ag.BLOCK_JACOBI), (ag.FP64, ag.BLOCK_JACOBI),
-                                               (ag.MIXED64, ag.JACOBI)])
+                                               (ag.MIXED64, ag.JACOBI), (ag.MIXED_K64, ag.JACOBI),
+                                               (ag.MIXED_SPLIT, ag.JACOBI), (ag.MIXED_SPLIT, ag.BLOCK_JACOBI)])
 def test_cg_composite_p2_vs_oracle(ctx, precision, precond):
     """Relative residual 1e-8 measured with the oracle's FP64 SpMV on the
     oracle's own matrix; the displacement error against the oracle solution
@@ -181,7 +187,7 @@ def test_cg_composite_p2_vs_oracle(ctx, precision, precond):
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-4
 
 
-@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED])
+@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED, ag.MIXED_SPLIT])
 def test_tight_solve_error(ctx, precision):
     """Solved to relative residual 1e-12, FP64 and mixed precision (FP64
     refinement) reproduce the FP64 oracle displacement to 1e-7."""
@@ -195,7 +201,8 @@ def test_tight_solve_error(ctx, precision):
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-7
 
 
-@pytest.mark.parametrize("precision,precond", [(ag.MIXED, ag.BLOCK_JACOBI), (ag.FP64, ag.JACOBI)])
+@pytest.mark.parametrize("precision,precond", [(ag.MIXED, ag.BLOCK_JACOBI), (ag.FP64, ag.JACOBI),
+                                               (ag.MIXED_SPLIT, ag.BLOCK_JACOBI)])
 def test_cg_damaged_tet_vs_oracle(ctx, precision, precond):
     p = configs.config3(8)
     A, b, xref = oracle_solution(p)
@@ -219,15 +226,17 @@ def test_bcsr_dropin_vs_reference_cg(ctx, tag):
     x = B.solution()
     xref = g(tag + "_cg_x")
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-8
-    st = B.solve(precision=ag.MIXED, rtol=1e-10)
-    assert st["status"] == 0
-    assert np.linalg.norm(B.solution() - xref) / np.linalg.norm(xref) <= 1e-7
+    for prec in (ag.MIXED, ag.MIXED_SPLIT):
+        st = B.solve(precision=prec, rtol=1e-10)
+        assert st["status"] == 0
+        assert np.linalg.norm(B.solution() - xref) / np.linalg.norm(xref) <= 1e-7
 
 
-def test_rve_batch_effective_stiffness_vs_oracle(ctx):
+@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT])
+def test_rve_batch_effective_stiffness_vs_oracle(ctx, precision):
     probs = [configs.rve(n, 100 + n, f) for n, f in [(8, 0.25), (12, 0.3), (16, 0.4)]]
     B = ag.Batch(ctx, probs)
-    st = B.solve(precision=ag.MIXED, rtol=1e-10)
+    st = B.solve(precision=precision, rtol=1e-10)
     assert st["status"] == 0 and st["relres_max"] <= 1e-10
     ceff = B.average_stress()  # [prob][load][stress]
     for k, p in enumerate(probs):
@@ -251,23 +260,25 @@ def test_rve_batch_effective_stiffness_vs_oracle(ctx):
         assert np.abs(C - C.T).max() <= 1e-5 * np.abs(C).max()
 
 
-def test_batch_result_independent_of_mix(ctx):
+@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT])
+def test_batch_result_independent_of_mix(ctx, precision):
     probs = [configs.rve(n, 300 + n, 0.3) for n in (10, 14, 20)]
     alone = ag.Batch(ctx, [configs.rve(14, 314, 0.3)])
-    alone.solve(precision=ag.MIXED, rtol=1e-8)
+    alone.solve(precision=precision, rtol=1e-8)
     mixed = ag.Batch(ctx, probs)
-    mixed.solve(precision=ag.MIXED, rtol=1e-8)
+    mixed.solve(precision=precision, rtol=1e-8)
     np.testing.assert_array_equal(alone.solution(0), mixed.solution(1))
 
 
-def test_graph_replay_matches_launches(ctx):
+@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT])
+def test_graph_replay_matches_launches(ctx, precision):
     p = configs.config2(16)
     B = ag.Batch(ctx, [p])
-    B.solve(precision=ag.MIXED, rtol=1e-8, use_graph=1)
+    B.solve(precision=precision, rtol=1e-8, use_graph=1)
     x1 = B.solution().copy()
-    B.solve(precision=ag.MIXED, rtol=1e-8, use_graph=0)
+    B.solve(precision=precision, rtol=1e-8, use_graph=0)
     x2 = B.solution().copy()
-    B.solve(precision=ag.MIXED, rtol=1e-8, use_graph=1)
+    B.solve(precision=precision, rtol=1e-8, use_graph=1)
     x3 = B.solution().copy()
     np.testing.assert_array_equal(x1, x2)
     np.testing.assert_array_equal(x1, x3)

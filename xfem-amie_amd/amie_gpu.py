@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libamie_gpu.so")
 
 P1, P2, TET4 = 1, 2, 3
-FP64, MIXED, MIXED64 = 0, 1, 2
+FP64, MIXED, MIXED64, MIXED_K64, MIXED_SPLIT = 0, 1, 2, 3, 4
 JACOBI, BLOCK_JACOBI = 0, 1
 REFINE_REPLACE, REFINE_RESTART = 0, 1
 NPER = {P1: 3, P2: 6, TET4: 4}

@@ -405,7 +405,7 @@ __global__ __launch_bounds__(WG) void k_diag_scale(SolveArgs a, const int64_t *d
 
 // Scaled working matrix Ks = S K S in W, and the inverse of each scaled
 // diagonal block (block Jacobi).
-template <typename W, int B>
+template <typename M, typename W, int B>
 __global__ __launch_bounds__(WG) void k_scale_cast(SolveArgs a, void *vw, const int64_t *dslot, W *minv, int ntiles)
 {
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -423,12 +423,13 @@ __global__ __launch_bounds__(WG) void k_scale_cast(SolveArgs a, void *vw, const 
             const uint32_t c = a.col[slot];
             double blk[B * B];
             Blk<double, B>::load(a.v64, a.nslots, slot, blk);
-            W o[B * B];
+            typedef typename Blk<M, B>::T T;
+            T o[B * B];
             for (int j = 0; j < B; j++) {
                 const double sc = a.S[(int64_t)c * B + j];
-                for (int i = 0; i < B; i++) o[j * B + i] = valid ? (W)(sr[i] * blk[j * B + i] * sc) : (W)0;
+                for (int i = 0; i < B; i++) o[j * B + i] = valid ? (T)(sr[i] * blk[j * B + i] * sc) : (T)0;
             }
-            if (vw) Blk<W, B>::store(vw, a.nslots, slot, o);
+            if (vw) Blk<M, B>::store(vw, a.nslots, slot, o);
         }
         if (!valid || !minv) continue;
         double d[B * B];

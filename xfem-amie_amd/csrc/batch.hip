@@ -59,6 +59,8 @@ EType etype_info(int et)
     return {0, 0, 0, 0, 0};
 }
 
+static size_t split_slot_bytes(int b) { return b == 2 ? Blk<Split, 2>::bytes_per_slot : Blk<Split, 3>::bytes_per_slot; }
+
 static size_t slot_bytes(bool fp64, int b)
 {
     if (b == 2) return fp64 ? Blk<double, 2>::bytes_per_slot : Blk<float, 2>::bytes_per_slot;
@@ -74,7 +76,7 @@ Batch::~Batch()
 
 size_t Batch::bytes() const
 {
-    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &row_len, &v64, &v32, &tile_slice0,
+    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &row_len, &v64, &v32, &vsp, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
                            &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
                            &bcval, &coords,
@@ -87,8 +89,8 @@ size_t Batch::bytes() const
 
 SolveArgs Batch::args(int mode) const
 {
-    const bool m32 = mode == AMIE_MIXED || mode == AMIE_MIXED64;   // FP32 matrix planes
-    const bool w32 = mode == AMIE_MIXED;                           // FP32 vectors
+    const bool m32 = mode == AMIE_MIXED || mode == AMIE_MIXED64;      // FP32 matrix planes
+    const bool w32 = mode != AMIE_FP64 && mode != AMIE_MIXED64;       // FP32 vectors
     SolveArgs a{};
     a.slice_ptr = slice_ptr.as<int64_t>();
     a.slice_width = slice_width.as<int32_t>();
@@ -96,7 +98,7 @@ SolveArgs Batch::args(int mode) const
     a.col = col.as<uint32_t>();
     a.nslots = nslots;
     a.v64 = v64.as<double>();
-    a.vw = m32 ? v32.p : v64.p;
+    a.vw = m32 ? v32.p : (mode == AMIE_MIXED_SPLIT ? vsp.p : v64.p);
     a.tile_slice0 = tile_slice0.as<int32_t>();
     a.tile_nslices = tile_nslices.as<int32_t>();
     a.tile_prob = tile_prob.as<int32_t>();
@@ -205,8 +207,10 @@ static int alloc_vectors(Batch &B)
     if (!B.minv64.alloc((size_t)B.nrows_pad * B.et.b * B.et.b * 8)) return AMIE_ERR_NOMEM;
     if (!B.v64.alloc(B.nslots * slot_bytes(true, B.et.b))) return AMIE_ERR_NOMEM;
     if (!B.v32.alloc(B.nslots * slot_bytes(false, B.et.b))) return AMIE_ERR_NOMEM;
+    if (!B.vsp.alloc(B.nslots * split_slot_bytes(B.et.b))) return AMIE_ERR_NOMEM;
     AMIE_HIP(hipMemset(B.v64.p, 0, B.v64.n));
     AMIE_HIP(hipMemset(B.v32.p, 0, B.v32.n));
+    AMIE_HIP(hipMemset(B.vsp.p, 0, B.vsp.n));
     // solver bookkeeping
     if (!B.prob.alloc(B.nprob * sizeof(ProbDev))) return AMIE_ERR_NOMEM;
     for (DevBuf *pb : {&B.part1, &B.part2, &B.partc, &B.partn}) {
@@ -575,9 +579,11 @@ template <int Bk> static void launch_post(Batch &B)
                                B.bcmask.as<uint8_t>(), B.bcval.as<double>(), B.row_len.as<int32_t>(), (int)B.ntiles);
     }
     hipLaunchKernelGGL((k_diag_scale<Bk>), dim3(g), dim3(WG), 0, st, a, B.dslot.as<int64_t>(), B.S.as<double>(), (int)B.ntiles);
-    hipLaunchKernelGGL((k_scale_cast<float, Bk>), dim3(g), dim3(WG), 0, st, a, B.v32.p, B.dslot.as<int64_t>(),
+    hipLaunchKernelGGL((k_scale_cast<Split, float, Bk>), dim3(g), dim3(WG), 0, st, a, B.vsp.p, B.dslot.as<int64_t>(),
+                       (float *)nullptr, (int)B.ntiles);
+    hipLaunchKernelGGL((k_scale_cast<float, float, Bk>), dim3(g), dim3(WG), 0, st, a, B.v32.p, B.dslot.as<int64_t>(),
                        B.minv32.as<float>(), (int)B.ntiles);
-    hipLaunchKernelGGL((k_scale_cast<double, Bk>), dim3(g), dim3(WG), 0, st, a, (void *)nullptr, B.dslot.as<int64_t>(),
+    hipLaunchKernelGGL((k_scale_cast<double, double, Bk>), dim3(g), dim3(WG), 0, st, a, (void *)nullptr, B.dslot.as<int64_t>(),
                        B.minv64.as<double>(), (int)B.ntiles);
 }
 
@@ -630,6 +636,8 @@ static Kern select_kern(int mode, int b, int nr, bool bj)
 {
     if (mode == AMIE_MIXED) return b == 2 ? kern_nr<float, float, 2>(nr, bj) : kern_nr<float, float, 3>(nr, bj);
     if (mode == AMIE_MIXED64) return b == 2 ? kern_nr<double, float, 2>(nr, bj) : kern_nr<double, float, 3>(nr, bj);
+    if (mode == AMIE_MIXED_K64) return b == 2 ? kern_nr<float, double, 2>(nr, bj) : kern_nr<float, double, 3>(nr, bj);
+    if (mode == AMIE_MIXED_SPLIT) return b == 2 ? kern_nr<float, Split, 2>(nr, bj) : kern_nr<float, Split, 3>(nr, bj);
     return b == 2 ? kern_nr<double, double, 2>(nr, bj) : kern_nr<double, double, 3>(nr, bj);
 }
 
@@ -644,8 +652,8 @@ static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs
 // inverses).  Padding blocks are not counted.
 static double spmv_bytes(const Batch &B, const ProbHost &P, int mode, bool bj)
 {
-    const double sm = mode == AMIE_FP64 ? 8. : 4.;     // matrix values
-    const double sv = mode == AMIE_MIXED ? 4. : 8.;    // vectors
+    const double sm = (mode == AMIE_FP64 || mode == AMIE_MIXED_K64) ? 8. : (mode == AMIE_MIXED_SPLIT ? 6. : 4.);
+    const double sv = (mode == AMIE_FP64 || mode == AMIE_MIXED64) ? 8. : 4.;   // vectors
     const double b = B.et.b;
     const double nnzb = (double)P.column_index.size();
     const double n = (double)P.nrows * b;
@@ -707,7 +715,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
     hipStream_t s = stream;
     R.K = select_kern(mode, et.b, nrhs, R.bj);
     R.a = args(mode);
-    const bool fp32vec = mode == AMIE_MIXED;
+    const bool fp32vec = mode != AMIE_FP64 && mode != AMIE_MIXED64;
     for (int p = 0; p < nprob; p++) {
         if (!subset[p]) continue;
         ProbDev &P = h_prob[p];
@@ -716,7 +724,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
         P.maxit = o.maxit > 0 ? o.maxit : ph[p].nrows * et.b;
         P.ckpt_interval = R.mixed ? (o.ckpt_interval > 0 ? o.ckpt_interval : 128) : ((int64_t)1 << 62);
-        P.stall_limit = (fp32vec && o.escalate) ? 8 : 0;
+        P.stall_iters = (fp32vec && o.escalate) ? 1024 : 0;
         P.status = ST_RUNNING;
         P.ckpt_pending = 1;   // the pass opens with a refinement: r = S(b - K x), p = M r
         P.src_ckpt = 0;
@@ -732,7 +740,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
             P.bb[k] = P.bb64[k] = 0.;
             P.relres[k] = P.relres_out[k] = 0.;
             P.rel_best[k] = DBL_MAX;
-            P.stall[k] = 0;
+            P.best_it[k] = 0;
             P.done[k] = P.done_out[k] = k < nrhs ? 0 : 1;
         }
     }
@@ -848,7 +856,7 @@ int Batch::solve_poll(bool wait, bool *finished)
         bool any_stalled = false;
         for (int p = 0; p < nprob; p++)
             if (h_prob_pinned[p].status == ST_STALLED) stalled[p] = 1, any_stalled = true;
-        if (any_stalled && R.mode == AMIE_MIXED && R.o.escalate) {
+        if (any_stalled && R.mode != AMIE_FP64 && R.o.escalate) {
             std::memcpy(h_prob.data(), h_prob_pinned, nprob * sizeof(ProbDev));
             for (int p = 0; p < nprob; p++) {
                 R.iters_done[p] += h_prob[p].iters;
@@ -970,11 +978,22 @@ void ctx_release(amie_gpu_ctx *ctx)
     if (--ctx->children == 0 && ctx->closing) ctx_free(ctx);
 }
 
-// Plain grid-stride copy of 16-byte words: the achievable-bandwidth yardstick.
+// Plain copy of 16-byte words, four per thread, one pass over the grid: the
+// achievable-bandwidth yardstick.
 __global__ __launch_bounds__(256) void k_stream_copy(const f4v *src, f4v *dst, size_t n)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = __builtin_nontemporal_load(src + i);
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    f4v v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n) v[u] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n) __builtin_nontemporal_store(v[u], dst + i);
+    }
 }
 } // namespace amie
 
@@ -1069,7 +1088,7 @@ int amie_gpu_stream_copy(amie_gpu_ctx *ctx, size_t bytes, int32_t iters, double 
     DevBuf a, b;
     if (!a.alloc(n4 * 16) || !b.alloc(n4 * 16)) return AMIE_ERR_NOMEM;
     AMIE_HIP(hipMemset(a.p, 0, a.n));
-    const int grid = std::max(1, ctx->num_cu) * 8;
+    const int grid = (int)((n4 + 1023) / 1024);
     hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, a.as<f4v>(), b.as<f4v>(), n4);
     hipEvent_t e0, e1;
     AMIE_HIP(hipEventCreate(&e0));
@@ -1160,7 +1179,7 @@ int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_so
     amie_solve_opts o;
     amie_gpu_default_opts(&o);
     if (opts) o = *opts;
-    if (o.precision != AMIE_FP64 && o.precision != AMIE_MIXED && o.precision != AMIE_MIXED64) {
+    if (o.precision < AMIE_FP64 || o.precision > AMIE_MIXED_SPLIT) {
         set_error("bad precision");
         return AMIE_ERR_ARG;
     }
@@ -1263,18 +1282,18 @@ int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const 
     Batch &B = *bb->b;
     int rc = upload_vec(B, p, B.q, x); // q as scratch input (8-byte entries)
     if (rc) return rc;
-    SolveArgs a = B.args(precision == AMIE_MIXED ? AMIE_MIXED : AMIE_FP64);
+    const int mode = precision == AMIE_MIXED ? 1 : (precision == AMIE_MIXED_SPLIT ? 2 : 0);
+    SolveArgs a = B.args(mode == 1 ? AMIE_MIXED : (mode == 2 ? AMIE_MIXED_SPLIT : AMIE_FP64));
     const int g = B.grid();
-    const int mode = precision == AMIE_MIXED ? 1 : 0;
     double *xin = B.q.as<double>(), *yo = B.r.as<double>();
     hipStream_t st = B.stream;
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(g), dim3(WG), 0, st, a, mode, xin, yo, (int)B.ntiles); };
     if (B.et.b == 2) {
-        if (B.nrhs == 1) go(k_spmv_plain<float, 2, 1>);
-        else go(k_spmv_plain<float, 2, 3>);
+        if (B.nrhs == 1) go(k_spmv_plain<2, 1>);
+        else go(k_spmv_plain<2, 3>);
     } else {
-        if (B.nrhs == 1) go(k_spmv_plain<float, 3, 1>);
-        else go(k_spmv_plain<float, 3, 3>);
+        if (B.nrhs == 1) go(k_spmv_plain<3, 1>);
+        else go(k_spmv_plain<3, 3>);
     }
     AMIE_HIP(hipGetLastError());
     AMIE_HIP(hipStreamSynchronize(st));

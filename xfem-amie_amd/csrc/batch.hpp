@@ -111,7 +111,7 @@ struct Batch {
 
     // SELL structure
     DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
-    DevBuf v64, v32;
+    DevBuf v64, v32, vsp;   // FP64 unscaled K; FP32 and split FP32+bf16 scaled Ks
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active, active_prob, n_active_prob;
     DevBuf prob, part1, part2, partc, partn;
     // vectors: one plane of nrows_pad * b entries per right-hand side; 8 bytes

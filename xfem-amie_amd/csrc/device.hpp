@@ -48,6 +48,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace amie {
 
 constexpr int WAVE = 64;
@@ -69,7 +71,7 @@ struct ProbDev {
     double rtol, drop;
     int64_t maxit;
     int64_t ckpt_interval;   // refinement at least every this many iterations
-    int32_t stall_limit;     // > 0: ST_STALLED after that many refinements without halving
+    int64_t stall_iters;     // > 0: ST_STALLED when the FP64 residual has not halved in that many iterations
     // state, written by k_reduce (src_ckpt: raised by k_ckpt_x)
     int32_t status;          // a finished status only ever stops work
     int32_t ckpt_pending;    // a refinement is due at the next checkpoint slot
@@ -80,7 +82,7 @@ struct ProbDev {
     double relres[MAXRHS];
     int64_t iters, nckpt, last_ckpt;
     double rr[MAXRHS], rr_ckpt[MAXRHS], rr_target[MAXRHS], bb[MAXRHS], bb64[MAXRHS], rel_best[MAXRHS];
-    int32_t stall[MAXRHS];
+    int64_t best_it[MAXRHS]; // iteration of the last halving of rel_best
     // results for the host
     double relres_out[MAXRHS];
     int32_t done_out[MAXRHS];
@@ -119,7 +121,19 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 
 template <typename W, int B> struct Blk;
 
+// Blocks are held in registers in their stored form (Raw) and expanded to
+// values only where they are used.
+template <typename M, int B> struct BlkRaw {
+    typedef typename Blk<M, B>::T T;
+    struct Raw {
+        T v[B * B];
+    };
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, Raw &r) { Blk<M, B>::load(base, ns, s, r.v); }
+    __device__ static inline T at(const Raw &r, int k) { return r.v[k]; }
+};
+
 template <> struct Blk<float, 2> {
+    typedef float T;
     static constexpr int N = 4;
     __device__ static inline void load(const void *base, int64_t ns, int64_t s, float *a)
     {
@@ -135,6 +149,7 @@ template <> struct Blk<float, 2> {
     static constexpr size_t bytes_per_slot = 16;
 };
 template <> struct Blk<float, 3> {
+    typedef float T;
     static constexpr int N = 9;
     __device__ static inline void load(const void *base, int64_t ns, int64_t s, float *a)
     {
@@ -155,6 +170,7 @@ template <> struct Blk<float, 3> {
     static constexpr size_t bytes_per_slot = 36;
 };
 template <> struct Blk<double, 2> {
+    typedef double T;
     static constexpr int N = 4;
     __device__ static inline void load(const void *base, int64_t ns, int64_t s, double *a)
     {
@@ -172,6 +188,7 @@ template <> struct Blk<double, 2> {
     static constexpr size_t bytes_per_slot = 32;
 };
 template <> struct Blk<double, 3> {
+    typedef double T;
     static constexpr int N = 9;
     __device__ static inline void load(const void *base, int64_t ns, int64_t s, double *a)
     {
@@ -196,6 +213,160 @@ template <> struct Blk<double, 3> {
     static constexpr size_t bytes_per_slot = 72;
 };
 
+
+// Split FP32 + bf16 values: hi = fl32(v), lo = bf16(v - hi), so hi + lo
+// carries v to a relative error below 2^-32 in 6 bytes.  The hi planes come
+// first (laid out as the FP32 planes), then the lo planes:
+//   2x2: 4 bf16 (8 bytes) per slot          3x3: 8 bf16 (16 bytes) + 1 bf16
+struct Split {};
+
+__host__ __device__ inline uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_value(uint32_t h)
+{
+    const uint32_t u = h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+template <> struct Blk<Split, 2> {
+    typedef double T;
+    static constexpr int N = 4;
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, double *a)
+    {
+        const f4v h = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(base) + s);
+        const u2v l = __builtin_nontemporal_load(reinterpret_cast<const u2v *>(reinterpret_cast<const char *>(base) + ns * 16) + s);
+        a[0] = (double)h.x + (double)bf16_value(l.x & 0xFFFFu);
+        a[1] = (double)h.y + (double)bf16_value(l.x >> 16);
+        a[2] = (double)h.z + (double)bf16_value(l.y & 0xFFFFu);
+        a[3] = (double)h.w + (double)bf16_value(l.y >> 16);
+    }
+    __device__ static inline void store(void *base, int64_t ns, int64_t s, const double *a)
+    {
+        float h[4];
+        uint32_t lo[4];
+        for (int i = 0; i < 4; i++) {
+            h[i] = (float)a[i];
+            lo[i] = bf16_rne((float)(a[i] - (double)h[i]));
+        }
+        reinterpret_cast<float4 *>(base)[s] = make_float4(h[0], h[1], h[2], h[3]);
+        reinterpret_cast<uint2 *>(reinterpret_cast<char *>(base) + ns * 16)[s] = make_uint2(lo[0] | lo[1] << 16, lo[2] | lo[3] << 16);
+    }
+    static constexpr size_t bytes_per_slot = 24;
+};
+template <> struct Blk<Split, 3> {
+    typedef double T;
+    static constexpr int N = 9;
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, double *a)
+    {
+        const f4v *p = reinterpret_cast<const f4v *>(base);
+        const f4v h0 = __builtin_nontemporal_load(p + s);
+        const f4v h1 = __builtin_nontemporal_load(p + ns + s);
+        const float h2 = __builtin_nontemporal_load(reinterpret_cast<const float *>(p + 2 * ns) + s);
+        const char *lb = reinterpret_cast<const char *>(base) + ns * 36;
+        const u4v l0 = __builtin_nontemporal_load(reinterpret_cast<const u4v *>(lb) + s);
+        const uint16_t l1 = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(lb + ns * 16) + s);
+        const float hv[9] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w, h2};
+        const uint32_t lw[4] = {l0.x, l0.y, l0.z, l0.w};
+#pragma unroll
+        for (int i = 0; i < 8; i++) a[i] = (double)hv[i] + (double)bf16_value((lw[i / 2] >> (16 * (i % 2))) & 0xFFFFu);
+        a[8] = (double)hv[8] + (double)bf16_value(l1);
+    }
+    __device__ static inline void store(void *base, int64_t ns, int64_t s, const double *a)
+    {
+        float h[9];
+        uint32_t lo[9];
+        for (int i = 0; i < 9; i++) {
+            h[i] = (float)a[i];
+            lo[i] = bf16_rne((float)(a[i] - (double)h[i]));
+        }
+        float4 *p = reinterpret_cast<float4 *>(base);
+        p[s] = make_float4(h[0], h[1], h[2], h[3]);
+        p[ns + s] = make_float4(h[4], h[5], h[6], h[7]);
+        reinterpret_cast<float *>(p + 2 * ns)[s] = h[8];
+        char *lb = reinterpret_cast<char *>(base) + ns * 36;
+        reinterpret_cast<uint4 *>(lb)[s] = make_uint4(lo[0] | lo[1] << 16, lo[2] | lo[3] << 16, lo[4] | lo[5] << 16, lo[6] | lo[7] << 16);
+        reinterpret_cast<uint16_t *>(lb + ns * 16)[s] = (uint16_t)lo[8];
+    }
+    static constexpr size_t bytes_per_slot = 54;
+};
+
+template <> struct BlkRaw<Split, 2> {
+    typedef double T;
+    struct Raw {
+        f4v h;
+        u2v l;
+    };
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, Raw &r)
+    {
+        r.h = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(base) + s);
+        r.l = __builtin_nontemporal_load(reinterpret_cast<const u2v *>(reinterpret_cast<const char *>(base) + ns * 16) + s);
+    }
+    __device__ static inline double at(const Raw &r, int k)
+    {
+        const float h = k == 0 ? r.h.x : k == 1 ? r.h.y : k == 2 ? r.h.z : r.h.w;
+        const uint32_t w = k < 2 ? r.l.x : r.l.y;
+        return (double)h + (double)bf16_value((w >> (16 * (k & 1))) & 0xFFFFu);
+    }
+};
+template <> struct BlkRaw<Split, 3> {
+    typedef double T;
+    struct Raw {
+        f4v h0, h1;
+        float h2;
+        u4v l0;
+        uint32_t l1;
+    };
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, Raw &r)
+    {
+        const f4v *p = reinterpret_cast<const f4v *>(base);
+        r.h0 = __builtin_nontemporal_load(p + s);
+        r.h1 = __builtin_nontemporal_load(p + ns + s);
+        r.h2 = __builtin_nontemporal_load(reinterpret_cast<const float *>(p + 2 * ns) + s);
+        const char *lb = reinterpret_cast<const char *>(base) + ns * 36;
+        r.l0 = __builtin_nontemporal_load(reinterpret_cast<const u4v *>(lb) + s);
+        r.l1 = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(lb + ns * 16) + s);
+    }
+    __device__ static inline double at(const Raw &r, int k)
+    {
+        float h;
+        uint32_t w;
+        switch (k) {
+        case 0: h = r.h0.x; w = r.l0.x; break;
+        case 1: h = r.h0.y; w = r.l0.x >> 16; break;
+        case 2: h = r.h0.z; w = r.l0.y; break;
+        case 3: h = r.h0.w; w = r.l0.y >> 16; break;
+        case 4: h = r.h1.x; w = r.l0.z; break;
+        case 5: h = r.h1.y; w = r.l0.z >> 16; break;
+        case 6: h = r.h1.z; w = r.l0.w; break;
+        case 7: h = r.h1.w; w = r.l0.w >> 16; break;
+        default: h = r.h2; w = r.l1; break;
+        }
+        return (double)h + (double)bf16_value(w & 0xFFFFu);
+    }
+};
+
+// Matrix value kinds: FP64 planes hold the unscaled K (the scaling is applied
+// on the fly); FP32 and split planes hold the scaled Ks.  FP64 and split
+// products accumulate in FP64.
+template <typename M> struct MatKind {
+    static constexpr bool onfly = false, acc64 = false;
+};
+template <> struct MatKind<double> {
+    static constexpr bool onfly = true, acc64 = true;
+};
+template <> struct MatKind<Split> {
+    static constexpr bool onfly = false, acc64 = true;
+};
 
 // ------------------------------------------------------------ reductions
 __device__ inline double wave_sum(double v)
@@ -324,7 +495,8 @@ template <typename W, int B, bool BJ> __device__ inline void load_minv(const Sol
 // before the dependent gathers, so a wavefront keeps many independent loads
 // in flight (a P1 row of 7 blocks is one step with 2x2 FP32 blocks).
 template <typename M, int B> struct Unroll {
-    static constexpr int U = B * B * sizeof(M) <= 16 ? 8 : (B * B * sizeof(M) <= 36 ? 4 : 2);
+    static constexpr size_t bytes = Blk<M, B>::bytes_per_slot;
+    static constexpr int U = bytes <= 16 ? 8 : (bytes <= 36 ? 4 : 2);
 };
 
 // acc[k][i] += sum over the row's blocks of A(row, c)(i, j) * x_k(c)_j for
@@ -338,25 +510,28 @@ __device__ inline void spmv_row(const SolveArgs &a, const void *vals, int s, int
     const int w = a.slice_width[s];
     const int64_t sp = a.slice_ptr[s] + lane;
     for (int k = 0; k < w; k += U) {
+        typedef BlkRaw<M, B> BR;
         uint32_t c[U];
-        M blk[U][B * B];
+        typename BR::Raw blk[U];
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (k + u < w) c[u] = __builtin_nontemporal_load(a.col + sp + (int64_t)(k + u) * WAVE);
 #pragma unroll
         for (int u = 0; u < U; u++)
-            if (k + u < w) Blk<M, B>::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
+            if (k + u < w) BR::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (k + u < w) {
                 A xv[NR][B];
                 gather(c[u], xv);
 #pragma unroll
-                for (int r = 0; r < NR; r++)
+                for (int j = 0; j < B; j++)
 #pragma unroll
-                    for (int j = 0; j < B; j++)
+                    for (int i = 0; i < B; i++) {
+                        const A m = (A)BR::at(blk[u], j * B + i);
 #pragma unroll
-                        for (int i = 0; i < B; i++) acc[r][i] += (A)blk[u][j * B + i] * xv[r][j];
+                        for (int r = 0; r < NR; r++) acc[r][i] += m * xv[r][j];
+                    }
             }
     }
 }
@@ -376,7 +551,9 @@ template <typename W, typename M, int B, int NR, bool BJ>
 __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    constexpr bool ONFLY = sizeof(M) == 8;
+    constexpr bool ONFLY = MatKind<M>::onfly;
+    // products accumulate in FP64 with the FP64 and split matrices, else in W
+    typedef typename std::conditional<MatKind<M>::acc64, double, W>::type A;
     const int t = my_tile(a);
     if (t < 0) return;
     const ProbDev &P = a.prob[a.tile_prob[t]];
@@ -390,23 +567,23 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
     if (wave < a.tile_nslices[t]) {
         const int s = a.tile_slice0[t] + wave;
         const int64_t row = (int64_t)s * WAVE + lane;
-        W acc[NR][B];
+        A acc[NR][B];
 #pragma unroll
         for (int k = 0; k < NR; k++)
 #pragma unroll
             for (int i = 0; i < B; i++) acc[k][i] = 0;
-        auto gather = [&](uint32_t c, W (*xv)[B]) {
+        auto gather = [&](uint32_t c, A (*xv)[B]) {
             double sc[B];
             if (ONFLY) Node<double, B>::load(a.S + (size_t)c * B, sc);
 #pragma unroll
             for (int k = 0; k < NR; k++) {
-                Node<W, B>::load(p + k * a.vplane + (size_t)c * B, xv[k]);
-                if (ONFLY)
+                W t[B];
+                Node<W, B>::load(p + k * a.vplane + (size_t)c * B, t);
 #pragma unroll
-                    for (int j = 0; j < B; j++) xv[k][j] *= (W)sc[j];
+                for (int j = 0; j < B; j++) xv[k][j] = ONFLY ? (A)t[j] * (A)sc[j] : (A)t[j];
             }
         };
-        spmv_row<W, M, B, NR>(a, a.vw, s, lane, acc, gather);
+        spmv_row<A, M, B, NR>(a, a.vw, s, lane, acc, gather);
         if (lane < a.slice_rows[s]) {
             W mi[B * B];
             load_minv<W, B, BJ>(a, row, mi);
@@ -419,7 +596,7 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
                 Node<W, B>::load(p + idx, pv);
                 Node<W, B, true>::load(r + idx, rv);
 #pragma unroll
-                for (int i = 0; i < B; i++) qv[i] = ONFLY ? (W)(acc[k][i] * (W)sr[i]) : acc[k][i];
+                for (int i = 0; i < B; i++) qv[i] = ONFLY ? (W)(acc[k][i] * (A)sr[i]) : (W)acc[k][i];
                 Node<W, B, true>::store(q + idx, qv);
                 precond<W, B, BJ>(mi, rv, zv);
                 precond<W, B, BJ>(mi, qv, mq);
@@ -474,8 +651,8 @@ __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
             P.rr_target[k] = rr * (P.rtol / rel) * (P.rtol / rel);
             if (rel < 0.5 * P.rel_best[k]) {
                 P.rel_best[k] = rel;
-                P.stall[k] = 0;
-            } else if (P.stall_limit > 0 && ++P.stall[k] >= P.stall_limit) {
+                P.best_it[k] = P.iters;
+            } else if (P.stall_iters > 0 && P.iters - P.best_it[k] >= P.stall_iters) {
                 stalled = true;
             }
         }
@@ -713,49 +890,45 @@ __global__ __launch_bounds__(WG) void k_norms_final(SolveArgs a)
 }
 
 // ------------------------------------------------- plain SpMV (test hook)
-// y = K x with the FP64 planes (mode 0) or S^-1 (Ks (S^-1 x)) with the
-// scaled working planes Ks = S K S (mode 1).  x, y: FP64 planes.
-template <typename W, int B, int NR>
+// y = K x with the FP64 planes (mode 0), or S^-1 (Ks (S^-1 x)) with the
+// scaled FP32 planes (mode 1, FP32 accumulation) or the split planes (mode 2,
+// FP64 accumulation), a.vw pointing at them.  x, y: FP64 planes.
+template <typename A, typename M, int B, int NR>
+__device__ inline void spmv_plain_row(const SolveArgs &a, const void *vals, bool scaled, const double *x, double *yout,
+                                      int s, int lane)
+{
+    const int64_t row = (int64_t)s * WAVE + lane;
+    A acc[NR][B];
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+#pragma unroll
+        for (int i = 0; i < B; i++) acc[k][i] = 0;
+    auto gather = [&](uint32_t c, A (*xv)[B]) {
+#pragma unroll
+        for (int k = 0; k < NR; k++)
+#pragma unroll
+            for (int j = 0; j < B; j++) {
+                const double v = x[k * a.vplane + (size_t)c * B + j];
+                xv[k][j] = (A)(scaled ? v / a.S[(size_t)c * B + j] : v);
+            }
+    };
+    spmv_row<A, M, B, NR>(a, vals, s, lane, acc, gather);
+    if (lane < a.slice_rows[s])
+        for (int k = 0; k < NR; k++)
+            for (int i = 0; i < B; i++)
+                yout[k * a.vplane + (size_t)row * B + i] = scaled ? (double)acc[k][i] / a.S[(size_t)row * B + i] : (double)acc[k][i];
+}
+
+template <int B, int NR>
 __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const double *x, double *yout, int ntiles)
 {
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         if (wave >= a.tile_nslices[t]) continue;
         const int s = a.tile_slice0[t] + wave;
-        const int64_t row = (int64_t)s * WAVE + lane;
-        if (mode == 0) {
-            double acc[NR][B];
-#pragma unroll
-            for (int k = 0; k < NR; k++)
-#pragma unroll
-                for (int i = 0; i < B; i++) acc[k][i] = 0.;
-            auto gather = [&](uint32_t c, double (*xv)[B]) {
-#pragma unroll
-                for (int k = 0; k < NR; k++) Node<double, B>::load(x + k * a.vplane + (size_t)c * B, xv[k]);
-            };
-            spmv_row<double, double, B, NR>(a, a.v64, s, lane, acc, gather);
-            if (lane < a.slice_rows[s])
-                for (int k = 0; k < NR; k++)
-                    for (int i = 0; i < B; i++) yout[k * a.vplane + (size_t)row * B + i] = acc[k][i];
-        } else {
-            W acc[NR][B];
-#pragma unroll
-            for (int k = 0; k < NR; k++)
-#pragma unroll
-                for (int i = 0; i < B; i++) acc[k][i] = 0;
-            auto gather = [&](uint32_t c, W (*xv)[B]) {
-#pragma unroll
-                for (int k = 0; k < NR; k++)
-#pragma unroll
-                    for (int j = 0; j < B; j++)
-                        xv[k][j] = (W)(x[k * a.vplane + (size_t)c * B + j] / a.S[(size_t)c * B + j]);
-            };
-            spmv_row<W, W, B, NR>(a, a.vw, s, lane, acc, gather);
-            if (lane < a.slice_rows[s])
-                for (int k = 0; k < NR; k++)
-                    for (int i = 0; i < B; i++)
-                        yout[k * a.vplane + (size_t)row * B + i] = (double)acc[k][i] / a.S[(size_t)row * B + i];
-        }
+        if (mode == 0) spmv_plain_row<double, double, B, NR>(a, a.v64, false, x, yout, s, lane);
+        else if (mode == 1) spmv_plain_row<float, float, B, NR>(a, a.vw, true, x, yout, s, lane);
+        else spmv_plain_row<double, Split, B, NR>(a, a.vw, true, x, yout, s, lane);
     }
 }
 
