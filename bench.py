@@ -30,6 +30,21 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_spmv_rve512_split.md")
+
+
+def measured_traffic(kernel="k_spmv"):
+    """HBM bytes / algorithmic bytes of a kernel, from the FETCH_SIZE and
+    WRITE_SIZE counter runs summarised in the committed profile (scaled by
+    the same counters' reading for a plain copy of known size)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("<!-- traffic:"):
+                    return json.loads(line[len("<!-- traffic:"):].rsplit("-->", 1)[0])[kernel]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
 
 
 def log(*a):
@@ -144,6 +159,9 @@ def main():
                     "ceff_rel_diff_vs_fp64_max": float(np.abs(ceff - ceff64).max() / np.abs(ceff64).max())}
         log(f"accuracy vs FP64 CG: {accuracy}")
 
+    tr = measured_traffic()
+    avg_bytes = spmv_bytes / max(1, launches)
+    traffic = tr["ratio"] * avg_bytes if tr else None
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
     cpu = None if args.no_cpu else cpu_baseline(problems, args.cpu_budget, args.rtol)
@@ -167,11 +185,15 @@ def main():
                    "rves": len(problems), "load_cases": problems[0].nrhs, "dofs": ndof, "blocks": nnzb,
                    "hbm_bytes": B.nbytes, "converged": ok},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "stream_copy_gbs": copy_gbs,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": (f"FETCH_SIZE+WRITE_SIZE / algorithmic = {tr['ratio']} at full occupancy "
+                                        f"({os.path.relpath(TRAFFIC_PROFILE, ROOT)}), times avg_bytes_per_launch"
+                                        if tr else None),
+                     "stream_copy_gbs": copy_gbs,
                      "kernel": "k_spmv (block SpMV q = Ks p, split FP32+bf16 values, fused p.q, q.z, q.Mq)",
                      "launches": launches,
                      "avg_launch_us": 1e3 * spmv_ms / max(1, launches),
-                     "avg_bytes_per_launch": spmv_bytes / max(1, launches)},
+                     "avg_bytes_per_launch": avg_bytes},
         "cpu_baseline": cpu,
         "fp64_batched": fp64,
         "fp32_inner_batched": fp32,

@@ -101,7 +101,7 @@ typedef struct {
 } amie_problem_desc;
 
 typedef struct {
-    int32_t precision;       /* AMIE_FP64 | AMIE_MIXED */
+    int32_t precision;       /* AMIE_FP64 | AMIE_MIXED | ... (see above) */
     int32_t precond;         /* AMIE_JACOBI | AMIE_BLOCK_JACOBI */
     int32_t refine;          /* AMIE_REFINE_REPLACE | AMIE_REFINE_RESTART */
     double rtol;             /* stop on ||b - K x|| / ||b|| <= rtol (FP64 residual) */
@@ -113,7 +113,10 @@ typedef struct {
     int32_t chunk;           /* iterations per chunk between host status checks */
     int32_t time_kernels;    /* 1: time every SpMV-kernel launch with HIP events */
     int32_t escalate;        /* AMIE_MIXED: continue stagnating problems in AMIE_FP64 (default 1) */
-    int32_t max_grid;        /* > 0: cap on workgroups per launch (0: 16 per CU) */
+    int32_t max_grid;        /* > 0: cap on workgroups of the set-up kernels (0: 16 per CU) */
+    double ref_eps;          /* > 0: the reference's absolute stop instead of rtol:
+                                sqrt(r . D^-1 r) <= max(1e-12, ref_eps) with D = diag(K)
+                                (solvers/conjugategradient.cpp:101,218,305) */
 } amie_solve_opts;
 
 typedef struct {

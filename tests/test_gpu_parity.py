@@ -232,6 +232,28 @@ def test_bcsr_dropin_vs_reference_cg(ctx, tag):
         assert np.linalg.norm(B.solution() - xref) / np.linalg.norm(xref) <= 1e-7
 
 
+@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED_SPLIT])
+def test_reference_absolute_stop(ctx, precision):
+    """ref_eps: the reference's stop sqrt(r . D^-1 r) <= eps
+    (solvers/conjugategradient.cpp:101,218,305), measured here with the
+    oracle's FP64 SpMV and inverse diagonal on the reference matrix."""
+    tag = "sys_p1_12x10_mix"
+    A = orc.BlockCSR(2, g(tag + "_row_size"), g(tag + "_column_index"), g(tag + "_array"))
+    b = g(tag + "_forces")
+    B = ag.Batch(ctx, bcsr=(2, A.row_size, A.column_index, A.array, b))
+    eps = 1e-10
+    st = B.solve(precision=precision, ref_eps=eps, rtol=1e-30)
+    assert st["status"] == 0, st
+    x = B.solution()
+    r = orc.spmv(A, x, b)
+    meas = np.sqrt(np.dot(r, orc.inverse_diagonal(A) * r))
+    assert meas <= eps
+    # and not far past it: the stop is the first refinement below eps
+    assert meas >= 1e-4 * eps
+    xref = g(tag + "_cg_x")
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
+
+
 @pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT])
 def test_rve_batch_effective_stiffness_vs_oracle(ctx, precision):
     probs = [configs.rve(n, 100 + n, f) for n, f in [(8, 0.25), (12, 0.3), (16, 0.4)]]

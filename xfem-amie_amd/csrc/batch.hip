@@ -721,6 +721,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         ProbDev &P = h_prob[p];
         P.refine = o.refine;
         P.rtol = o.rtol > 0 ? o.rtol : 1e-8;
+        P.ref_eps = o.ref_eps > 0 ? o.ref_eps : 0.;
         P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
         P.maxit = o.maxit > 0 ? o.maxit : ph[p].nrows * et.b;
         P.ckpt_interval = R.mixed ? (o.ckpt_interval > 0 ? o.ckpt_interval : 128) : ((int64_t)1 << 62);
@@ -775,13 +776,11 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         graphs.push_back({key, R.exec});
     }
     R.pbytes.assign(nprob, 0.);
-    R.active_bytes = 0.;
     R.was_running.assign(nprob, 0);
     int64_t maxit_all = 0;
     for (int p = 0; p < nprob; p++) {
         if (!subset[p]) continue;
         R.pbytes[p] = spmv_bytes(*this, ph[p], mode, R.bj);
-        R.active_bytes += R.pbytes[p];
         R.was_running[p] = 1;
         maxit_all = std::max<int64_t>(maxit_all, h_prob[p].maxit);
     }
@@ -827,7 +826,6 @@ int Batch::solve_poll(bool wait, bool *finished)
         AMIE_HIP(hipEventElapsedTime(&ms, R.tev[2 * i], R.tev[2 * i + 1]));
         R.stats.spmv_kernel_ms += ms;
         R.stats.spmv_launches += 1;
-        R.stats.spmv_bytes += R.active_bytes;
     }
     bool any_running = false, changed = false;
     for (int p = 0; p < nprob; p++) {
@@ -836,7 +834,6 @@ int Batch::solve_poll(bool wait, bool *finished)
         if (R.was_running[p] && !runp) {
             changed = true;
             R.was_running[p] = 0;
-            R.active_bytes -= R.pbytes[p];
         }
     }
     if (any_running && changed) {
@@ -848,6 +845,13 @@ int Batch::solve_poll(bool wait, bool *finished)
         }
     }
     if (!any_running || R.periods >= R.max_periods) {
+        // algorithmic bytes of the pass: every SpMV launch that worked on a
+        // problem is one the reduce kernel then counted (iters) or finished
+        if (R.o.time_kernels)
+            for (int p = 0; p < nprob; p++)
+                if (R.pbytes[p] > 0.)
+                    R.stats.spmv_bytes += R.pbytes[p] * (double)(h_prob_pinned[p].iters +
+                                                                 (h_prob_pinned[p].status != ST_RUNNING ? 1 : 0));
         // problems whose FP32 iteration stagnated continue from their current
         // iterate in FP64, in a pass of their own (an FP32 copy of a badly
         // conditioned K can lose the accuracy CG needs, so the matrix goes to
@@ -1066,7 +1070,7 @@ void amie_gpu_default_opts(amie_solve_opts *o)
 {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
-    o->precision = AMIE_MIXED;
+    o->precision = AMIE_MIXED_SPLIT;
     o->precond = AMIE_JACOBI;
     o->refine = AMIE_REFINE_REPLACE;
     o->rtol = 1e-8;

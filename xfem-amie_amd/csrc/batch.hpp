@@ -72,8 +72,7 @@ struct SolveRun {
     SolveArgs a{};
     int gt = 1, gp = 1;      // grids: active tiles, active problems
     hipGraphExec_t exec = nullptr;
-    std::vector<double> pbytes;
-    double active_bytes = 0.;
+    std::vector<double> pbytes;       // algorithmic bytes of one SpMV launch, per problem of the pass
     std::vector<hipEvent_t> tev;
     int nt = 0;
     std::vector<char> was_running;

@@ -69,6 +69,7 @@ struct ProbDev {
     int32_t nrows;
     int32_t refine;          // 0 replace the residual, keep p; 1 restart at each refinement
     double rtol, drop;
+    double ref_eps;          // > 0: converged when ||S r64|| <= ref_eps (absolute, scaled)
     int64_t maxit;
     int64_t ckpt_interval;   // refinement at least every this many iterations
     int64_t stall_iters;     // > 0: ST_STALLED when the FP64 residual has not halved in that many iterations
@@ -105,7 +106,7 @@ struct SolveArgs {
     ProbDev *prob;
     double *part1;           // [tile][MAXQ] p.q, q.z, q.Mq
     double *part2;           // [tile][MAXQ] r.z, r.r after the update
-    double *partc;           // [tile][MAXQ] r.z, r.r, |b - K x|^2 after a refinement
+    double *partc;           // [tile][MAXQ] r.z, r.r, |b - K x|^2, |S (b - K x)|^2 after a refinement
     double *partn;           // [tile][MAXQ] |S b|^2, |b|^2
     int64_t vplane;          // entries per right-hand-side plane (nrows_pad * b)
     double *x64;
@@ -494,9 +495,15 @@ template <typename W, int B, bool BJ> __device__ inline void load_minv(const Sol
 // Blocks requested per step: their column indices and values are all loaded
 // before the dependent gathers, so a wavefront keeps many independent loads
 // in flight (a P1 row of 7 blocks is one step with 2x2 FP32 blocks).
+#ifndef AMIE_UNROLL_16
+#define AMIE_UNROLL_16 8      // blocks per step for slots of <= 16 bytes
+#endif
+#ifndef AMIE_UNROLL_36
+#define AMIE_UNROLL_36 4      // ... of <= 36 bytes (split 2x2, FP32 3x3, FP64 2x2)
+#endif
 template <typename M, int B> struct Unroll {
     static constexpr size_t bytes = Blk<M, B>::bytes_per_slot;
-    static constexpr int U = bytes <= 16 ? 8 : (bytes <= 36 ? 4 : 2);
+    static constexpr int U = bytes <= 16 ? AMIE_UNROLL_16 : (bytes <= 36 ? AMIE_UNROLL_36 : 2);
 };
 
 // acc[k][i] += sum over the row's blocks of A(row, c)(i, j) * x_k(c)_j for
@@ -623,9 +630,9 @@ __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
     ProbDev &P = a.prob[a.active_prob[blockIdx.x]];
     if (P.status != ST_RUNNING) return;   // read by every thread before thread 0 writes (barriers below)
     const bool ck = P.src_ckpt != 0;
-    double s1[3 * NR], s0[3 * NR];
+    double s1[3 * NR], s0[4 * NR];
     problem_sums<3 * NR>(a.part1, P, s1, lds);
-    if (ck) problem_sums<3 * NR>(a.partc, P, s0, lds);
+    if (ck) problem_sums<4 * NR>(a.partc, P, s0, lds);
     else problem_sums<2 * NR>(a.part2, P, s0, lds);
     if (threadIdx.x != 0) return;
     bool all = true, bad = false, stalled = false, need = false;
@@ -642,13 +649,17 @@ __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
             const double rel = P.bb64[k] > 0. ? sqrt(rr64 / P.bb64[k]) : sqrt(rr64);
             P.relres[k] = P.relres_out[k] = rel;
             P.rr_ckpt[k] = rr;
-            if (rel <= P.rtol) {
+            // the stop measure and its bound: relative FP64 residual, or the
+            // reference's absolute sqrt(r . D^-1 r) = ||S r64||
+            const double meas = P.ref_eps > 0. ? sqrt(s0[3 * NR + k]) : rel;
+            const double bound = P.ref_eps > 0. ? fmax(1e-12, P.ref_eps) : P.rtol;
+            if (meas <= bound) {
                 P.done[k] = P.done_out[k] = 1;
                 P.alpha[k] = P.beta[k] = 0.;
                 continue;
             }
             // expected working-precision r.r at convergence, from this ratio
-            P.rr_target[k] = rr * (P.rtol / rel) * (P.rtol / rel);
+            P.rr_target[k] = rr * (bound / meas) * (bound / meas);
             if (rel < 0.5 * P.rel_best[k]) {
                 P.rel_best[k] = rel;
                 P.best_it[k] = P.iters;
@@ -785,9 +796,9 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
     if (P.status != ST_RUNNING || !P.ckpt_pending) return;
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
     W *r = vw<W>(a.r), *p = vw<W>(a.p);
-    double part[3 * NR];
+    double part[4 * NR];
 #pragma unroll
-    for (int k = 0; k < 3 * NR; k++) part[k] = 0.;
+    for (int k = 0; k < 4 * NR; k++) part[k] = 0.;
     if (wave < a.tile_nslices[t]) {
         const int s = a.tile_slice0[t] + wave;
         const int64_t row = (int64_t)s * WAVE + lane;
@@ -817,8 +828,10 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
 #pragma unroll
                 for (int i = 0; i < B; i++) {
                     const double r64 = bv[i] - acc[k][i];
+                    const double rs = sr[i] * r64;
                     part[2 * NR + k] += r64 * r64;
-                    rv[i] = (W)(sr[i] * r64);
+                    part[3 * NR + k] += rs * rs;
+                    rv[i] = (W)rs;
                 }
                 Node<W, B>::store(r + idx, rv);
                 precond<W, B, BJ>(mi, rv, zv);
@@ -833,7 +846,7 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
             }
         }
     }
-    write_partials<3 * NR>(part, a.partc, t, lds);
+    write_partials<4 * NR>(part, a.partc, t, lds);
 }
 
 // ------------------------------------------ norms of the right-hand sides
@@ -879,7 +892,7 @@ __global__ __launch_bounds__(WG) void k_norms_final(SolveArgs a)
     for (int k = 0; k < NR; k++) {
         P.bb[k] = sums[k];
         P.bb64[k] = sums[NR + k];
-        P.rr_target[k] = P.rtol * P.rtol * sums[k];
+        P.rr_target[k] = P.ref_eps > 0. ? fmax(1e-24, P.ref_eps * P.ref_eps) : P.rtol * P.rtol * sums[k];
         // a homogeneous right-hand side is solved by x = 0
         // (solvers/conjugategradient.cpp:74-78)
         if (sums[NR + k] == 0.) {
