@@ -4,6 +4,7 @@ orientation, DOF maps, inclusions, random field, C matrices) agree with the
 oracle and with the configurations' definitions.  No GPU compute calls."""
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -145,3 +146,21 @@ def test_device_element_formulation_matches_reference(tag, etype, E, nu):
     K = ag.element_matrix(etype, xy, ag.cauchy_green(E, nu, dim=ag.DIM[etype]))
     Kref = np.load(os.path.join(GOLD, "Ke_%s.npy" % tag))
     assert np.abs(K - Kref).max() <= 1e-15 * np.abs(Kref).max()
+
+
+DROPIN = os.path.join(os.path.dirname(__file__), "..", "xfem-amie_amd", "host", "dropin_check")
+
+
+def test_cpp_dropin_built_and_fails_loudly_without_gpu(tmp_path):
+    """The C++ LinearSolver drop-in links against the library and, with no GPU,
+    reports the device error and exits 2 (no CPU fallback)."""
+    assert os.path.exists(DROPIN), "run __graft_entry__.build()"
+    g = lambda n: os.path.join(os.path.dirname(__file__), "golden", "ref", n + ".npy")
+    files = []
+    for name, dt in (("row_size", np.uint32), ("column_index", np.uint32), ("array", np.float64), ("forces", np.float64)):
+        f = tmp_path / (name + ".bin")
+        np.load(g("sys_p1_8x8_" + name)).astype(dt).tofile(f)
+        files.append(str(f))
+    r = subprocess.run([DROPIN, "2", *files, "1e-10", "4", str(tmp_path / "x.bin")], capture_output=True, text=True)
+    assert r.returncode == 2
+    assert "GPU CG" in r.stderr

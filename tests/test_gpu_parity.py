@@ -10,6 +10,7 @@ relative residual they reach plus the displacement error against the FP64
 oracle solution.
 """
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -399,3 +400,32 @@ def test_result_independent_of_grid(ctx):
         assert st["status"] == 0, (g, st)
         for k in range(len(probs)):
             np.testing.assert_array_equal(B.solution(k), ref[k])
+
+
+DROPIN = os.path.join(os.path.dirname(__file__), "..", "xfem-amie_amd", "host", "dropin_check")
+
+
+@pytest.mark.parametrize("tag,precision", [("sys_p1_12x10_mix", ag.MIXED_SPLIT), ("sys_tet_3", ag.MIXED_SPLIT),
+                                           ("sys_p1_8x8", ag.FP64)])
+def test_cpp_linear_solver_dropin(tmp_path, tag, precision):
+    """The C++ LinearSolver drop-in (host/gpu_conjugate_gradient.hpp), driven as
+    Assembly::mgsolve drives a solver, on reference-assembled systems: the
+    reference's stop rule holds and the solution is the reference CG's."""
+    print("case", tag, precision, flush=True)
+    stride = 3 if "tet" in tag else 2
+    files = []
+    for name, dt in (("row_size", np.uint32), ("column_index", np.uint32), ("array", np.float64),
+                     ("forces", np.float64)):
+        f = tmp_path / (name + ".bin")
+        g(tag + "_" + name).astype(dt).tofile(f)
+        files.append(str(f))
+    out = tmp_path / "x.bin"
+    r = subprocess.run([DROPIN, str(stride), *files, "1e-10", str(precision), str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    x = np.fromfile(out, dtype=np.float64)
+    A = orc.BlockCSR(stride, g(tag + "_row_size"), g(tag + "_column_index"), g(tag + "_array"))
+    res = orc.spmv(A, x, g(tag + "_forces"))
+    assert np.sqrt(np.dot(res, orc.inverse_diagonal(A) * res)) <= 1e-10
+    xref = g(tag + "_cg_x")
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
