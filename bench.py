@@ -81,6 +81,36 @@ def cpu_baseline(problems, budget_s=15.0, rtol=1e-8):
                       f"oracle FP64 Jacobi PCG to rel. res {rtol:g}, {threads} OpenMP threads, assembly excluded"}
 
 
+def serving(ag, ctx, reqs, precision, rtol, limit_s=90.0):
+    """Replays a request trace against the server in wall-clock time and
+    returns p50/p99 request latency (arrival to result) and solves/s."""
+    S = ag.Server(ctx, 200 << 30, 64, precision=precision, rtol=rtol)
+    t0 = time.perf_counter()
+    lat, solves, nsub, bad = [], 0, 0, 0
+    nrhs = {}
+    while len(lat) < len(reqs):
+        now = time.perf_counter() - t0
+        if now > limit_s:
+            break
+        while nsub < len(reqs) and reqs[nsub][0] <= now:
+            t_arr, kind, prob, key = reqs[nsub]
+            tk = S.submit(prob, 1e3 * t_arr, key)
+            nrhs[tk] = prob.nrhs
+            nsub += 1
+        S.step(1e3 * (time.perf_counter() - t0))
+        for r in S.poll():
+            lat.append(r["done_ms"] - r["submit_ms"])
+            solves += nrhs[r["ticket"]]
+            bad += r["status"] != 0
+    elapsed = time.perf_counter() - t0
+    S.close()
+    lat = np.array(lat)
+    return {"requests": len(reqs), "completed": int(len(lat)), "not_converged": int(bad),
+            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+            "solves_per_s": solves / elapsed, "wall_s": elapsed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +123,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--maxit", type=int, default=0, help="cap on CG iterations (profiling runs only)")
-    ap.add_argument("--copy", action="store_true", help="also time the plain stream-copy kernel")
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="request rate of the serving leg (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the serving leg")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -111,6 +143,7 @@ def main():
     B = ag.Batch(ctx, problems)
     log(f"batch resident in HBM: {B.nbytes / 2**30:.2f} GiB, setup {time.perf_counter() - t0:.1f}s")
     nnzb = sum(B.shape(k)[1] for k in range(len(problems)))
+    hbm_bytes = B.nbytes
 
     def step(precision, time_kernels=1):
         B.assemble()
@@ -164,6 +197,16 @@ def main():
     traffic = tr["ratio"] * avg_bytes if tr else None
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
+    serve = None
+    if not args.no_serve:
+        B.close()
+        reqs = configs.config5(args.serve_rate, args.serve_seconds)
+        kinds = {k: sum(1 for r in reqs if r[1] == k) for k in ("rve2d", "rve3d", "damage")}
+        serve = {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s (seed 5): {kinds}, "
+                             f"rel.res {args.rtol:g}"}
+        for name, prec in (("mixed", ag.MIXED_SPLIT), ("fp64", ag.FP64)):
+            serve[name] = serving(ag, ctx, reqs, prec, args.rtol)
+            log(f"serving {name}: {serve[name]}")
     cpu = None if args.no_cpu else cpu_baseline(problems, args.cpu_budget, args.rtol)
     out = {
         "metric": "RVE solves/s + p50/p99 latency @rel.res 1e-8; mixed vs FP64 CG; SpMV GB/s",
@@ -183,7 +226,7 @@ def main():
                                f"accumulation and refinement) to rel.res {args.rtol:g}, "
                                f"assembly + solve + effective stiffness per step",
                    "rves": len(problems), "load_cases": problems[0].nrhs, "dofs": ndof, "blocks": nnzb,
-                   "hbm_bytes": B.nbytes, "converged": ok},
+                   "hbm_bytes": hbm_bytes, "converged": ok},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": (f"FETCH_SIZE+WRITE_SIZE / algorithmic = {tr['ratio']} at full occupancy "
@@ -199,6 +242,7 @@ def main():
         "fp32_inner_batched": fp32,
         "mixed_vs_fp64": (value / fp64["solves_per_s"]) if fp64 else None,
         "accuracy": accuracy,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

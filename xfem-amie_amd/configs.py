@@ -91,3 +91,31 @@ def rve3d(n, seed, fraction=0.3):
     macro = np.array([[1e-3, 0, 0, 0, 0, 0]], dtype=np.float64)
     return ag.Problem(ag.TET4, coords, conn, C, phase=phase, dofmap=dofmap, ndofnodes=n ** 3,
                       bc_dofs=np.array([0, 1, 2], dtype=np.int64), bc_vals=np.zeros(3), macro=macro, nrhs=1)
+
+
+def config5(rate=20.0, duration=10.0, seed=5, n2d=(64, 512), n3d=(16, 48), ndamage=32, damage_keys=4):
+    """Served solve stream: Poisson arrivals at `rate` req/s over `duration`
+    seconds, one third each of 2D P1 RVEs (n ~ U{64..512}, 30 % inclusions,
+    3 macro-strain load cases), 3D Tet4 RVEs (n ~ U{16..48}, 1 load case) and
+    damage steps of the 32^3 damage case on `damage_keys` persistent states.
+    Returns [(t_seconds, kind, problem, key)], key != 0 for damage steps."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    damage = {}
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            n = int(rng.integers(n2d[0], n2d[1] + 1))
+            out.append((t, "rve2d", rve(n, int(rng.integers(1, 2 ** 62)), 0.30), 0))
+        elif kind == 1:
+            n = int(rng.integers(n3d[0], n3d[1] + 1))
+            out.append((t, "rve3d", rve3d(n, int(rng.integers(1, 2 ** 62))), 0))
+        else:
+            key = int(rng.integers(1, damage_keys + 1))
+            if key not in damage:
+                damage[key] = config3(ndamage, seed=300 + key)
+            out.append((t, "damage", damage[key], key))
+    return out
