@@ -4,8 +4,8 @@ Workload (one "step"): 512 periodic P1 RVEs (n ~ U{32..256}, seed 4, 2k-130k
 DOFs each, 20-45 % circular inclusions, 12/60 GPa), three macro-strain load
 cases each (1536 right-hand sides), resident in HBM.  A step runs the whole
 hot path on the device: element integration + coloured scatter + Dirichlet
-pin + scaling (assembly), the mixed-precision CG (FP32 vectors, FP32+bf16
-split matrix with FP64 accumulation, FP64 refinement) of all 1536
+pin + scaling (assembly), the mixed-precision CG (FP32 vectors, the scaled
+matrix in 48-bit fixed point with FP64 accumulation, FP64 refinement) of all 1536
 right-hand sides to relative residual 1e-8 in one batch, and the
 volume-averaged stress (effective stiffness) of every RVE.
 
@@ -111,6 +111,64 @@ def serving(ag, ctx, reqs, precision, rtol, limit_s=90.0):
             "solves_per_s": solves / elapsed, "wall_s": elapsed}
 
 
+def single_problem_legs(ag, configs, ctx, rtol):
+    """The single-problem configurations beside the batch: P2 400^2 composite
+    (1.28 M DOFs) solved once, and the Tet4 64^3 damage case (824 k DOFs)
+    through 10 load steps (solve + damage update each), FIX48 mixed
+    precision against the plain FP64 CG."""
+    out = {}
+
+    def timed_solve(B, precision, precond, **kw):
+        st = B.solve(precision=precision, precond=precond, rtol=rtol, use_graph=0, time_kernels=1, **kw)
+        return st
+
+    p2 = configs.config2(400)
+    B = ag.Batch(ctx, [p2])
+    leg = {"workload": f"P2 400^2 composite, 35 % inclusions (seed 2), uniaxial, Jacobi, rel.res {rtol:g}",
+           "dofs": int(p2.ndof)}
+    for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
+        B.assemble()
+        t = time.perf_counter()
+        st = timed_solve(B, prec, ag.JACOBI)
+        dt = time.perf_counter() - t
+        leg[name] = {"solve_ms": 1e3 * dt, "iterations": st["iterations_max"], "status": st["status"],
+                     "spmv_gbs": st["spmv_bytes"] / (st["spmv_kernel_ms"] * 1e-3) / 1e9,
+                     "spmv_avg_us": 1e3 * st["spmv_kernel_ms"] / max(1, st["spmv_launches"])}
+        us, by = B.time_spmv(prec, ag.JACOBI, 50)
+        leg[name]["spmv_gbs_alone"] = by / (us * 1e-6) / 1e9
+    leg["mixed_vs_fp64"] = leg["fp64"]["solve_ms"] / leg["mixed"]["solve_ms"]
+    out["p2_400"] = leg
+    log(f"P2 400^2: {leg}")
+    B.close()
+
+    p3 = configs.config3(64)
+    leg = {"workload": f"Tet4 64^3 damage case (d0 field seed 3), 10 load steps of 2e-5, Jacobi, rel.res {rtol:g}",
+           "dofs": int(p3.ndof)}
+    for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
+        B = ag.Batch(ctx, [p3])
+        t = time.perf_counter()
+        its, ms, by, n = 0, 0.0, 0.0, 0
+        for s in range(1, 11):
+            if s > 1:
+                B.scale_bc(s / (s - 1))
+            B.assemble()
+            st = timed_solve(B, prec, ag.JACOBI, use_x0=1 if s > 1 else 0)
+            its += st["iterations_max"]
+            ms += st["spmv_kernel_ms"]
+            by += st["spmv_bytes"]
+            n += st["spmv_launches"]
+            B.damage_step()
+        dt = time.perf_counter() - t
+        d, _ = B.damage()
+        leg[name] = {"ten_steps_ms": 1e3 * dt, "iterations": its, "spmv_gbs": by / (ms * 1e-3) / 1e9,
+                     "spmv_avg_us": 1e3 * ms / max(1, n), "damage_mean": float(d.mean())}
+        B.close()
+    leg["mixed_vs_fp64"] = leg["fp64"]["ten_steps_ms"] / leg["mixed"]["ten_steps_ms"]
+    out["tet_64_damage"] = leg
+    log(f"Tet4 64^3: {leg}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +184,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=10.0, help="request rate of the serving leg (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the serving leg")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the P2 400^2 and Tet4 64^3 legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -152,7 +211,7 @@ def main():
         ceff = B.average_stress()
         return st, ceff
 
-    headline = ag.MIXED_SPLIT
+    headline = ag.MIXED_FIX48
     for w in range(args.warmup):
         st, _ = step(headline)
         log(f"warmup {w}: {st['solve_ms']:.1f} ms solve, iters max {st['iterations_max']}, status {st['status']}")
@@ -197,14 +256,15 @@ def main():
     traffic = tr["ratio"] * avg_bytes if tr else None
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
+    B.close()
+    single = None if args.no_single else single_problem_legs(ag, configs, ctx, args.rtol)
     serve = None
     if not args.no_serve:
-        B.close()
         reqs = configs.config5(args.serve_rate, args.serve_seconds)
         kinds = {k: sum(1 for r in reqs if r[1] == k) for k in ("rve2d", "rve3d", "damage")}
         serve = {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s (seed 5): {kinds}, "
                              f"rel.res {args.rtol:g}"}
-        for name, prec in (("mixed", ag.MIXED_SPLIT), ("fp64", ag.FP64)):
+        for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
             serve[name] = serving(ag, ctx, reqs, prec, args.rtol)
             log(f"serving {name}: {serve[name]}")
     cpu = None if args.no_cpu else cpu_baseline(problems, args.cpu_budget, args.rtol)
@@ -222,8 +282,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{len(problems)} periodic P1 RVEs n~U{{32..{args.nmax}}} (seed 4), 3 macro-strain "
-                               f"load cases each, mixed-precision CG (FP32 vectors, FP32+bf16 matrix, FP64 "
-                               f"accumulation and refinement) to rel.res {args.rtol:g}, "
+                               f"load cases each, mixed-precision CG (FP32 vectors, 48-bit fixed-point scaled "
+                               f"matrix, FP64 accumulation and refinement) to rel.res {args.rtol:g}, "
                                f"assembly + solve + effective stiffness per step",
                    "rves": len(problems), "load_cases": problems[0].nrhs, "dofs": ndof, "blocks": nnzb,
                    "hbm_bytes": hbm_bytes, "converged": ok},
@@ -233,7 +293,7 @@ def main():
                                         f"({os.path.relpath(TRAFFIC_PROFILE, ROOT)}), times avg_bytes_per_launch"
                                         if tr else None),
                      "stream_copy_gbs": copy_gbs,
-                     "kernel": "k_spmv (block SpMV q = Ks p, split FP32+bf16 values, fused p.q, q.z, q.Mq)",
+                     "kernel": "k_spmv (block SpMV q = Ks p, 48-bit fixed-point values, fused p.q, q.z, q.Mq)",
                      "launches": launches,
                      "avg_launch_us": 1e3 * spmv_ms / max(1, launches),
                      "avg_bytes_per_launch": avg_bytes},
@@ -242,6 +302,7 @@ def main():
         "fp32_inner_batched": fp32,
         "mixed_vs_fp64": (value / fp64["solves_per_s"]) if fp64 else None,
         "accuracy": accuracy,
+        "single_problems": single,
         "serving": serve,
     }
     print(json.dumps(out), flush=True)

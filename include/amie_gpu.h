@@ -60,11 +60,15 @@ enum { AMIE_P1 = 1, AMIE_P2 = 2, AMIE_TET4 = 3 };
  *   AMIE_MIXED_K64    FP64 matrix, FP32 vectors, FP64 accumulation
  *   AMIE_MIXED_SPLIT  matrix as FP32 + bf16 remainder (6 bytes a value,
  *                     relative error < 2^-32), FP32 vectors, FP64
- *                     accumulation: the FP64 iteration count at nearly the
- *                     FP32 traffic
+ *                     accumulation; hands a problem over to the FP64
+ *                     matrix when its accuracy floor is reached
+ *   AMIE_MIXED_FIX48  scaled matrix as 48-bit fixed point (6 bytes a value,
+ *                     absolute error < 2^-47: the scaled SPD matrix has
+ *                     |Ks_ij| <= 1), FP32 vectors, FP64 accumulation: the
+ *                     FP64 iteration count at the split's traffic
  * Problems whose FP32-vector iteration stagnates continue in AMIE_FP64 when
  * opts.escalate is set. */
-enum { AMIE_FP64 = 0, AMIE_MIXED = 1, AMIE_MIXED64 = 2, AMIE_MIXED_K64 = 3, AMIE_MIXED_SPLIT = 4 };
+enum { AMIE_FP64 = 0, AMIE_MIXED = 1, AMIE_MIXED64 = 2, AMIE_MIXED_K64 = 3, AMIE_MIXED_SPLIT = 4, AMIE_MIXED_FIX48 = 5 };
 enum { AMIE_JACOBI = 0, AMIE_BLOCK_JACOBI = 1 };      /* preconditioner */
 enum { AMIE_REFINE_REPLACE = 0, AMIE_REFINE_RESTART = 1 };
 
@@ -131,6 +135,9 @@ typedef struct {
     int64_t spmv_launches;
     double spmv_bytes;       /* algorithmic bytes per SpMV launch summed over launches */
     int32_t nproblems_escalated; /* problems continued in AMIE_FP64 */
+    int32_t nproblems_handover;  /* AMIE_MIXED_SPLIT: problems whose SpMV moved to the FP64
+                                    matrix at the split matrix's accuracy floor */
+    int64_t iterations_handover; /* iterations run after such a hand-over, summed */
 } amie_solve_stats;
 
 /* Damage law of the incremental-loading case (Mazars tension branch with
@@ -228,6 +235,13 @@ int amie_gpu_batch_get_element_blocks(amie_gpu_batch *b, int32_t p, int64_t *slo
 /* y = K x for problem p (FP64 matrix, or the scaled FP32 matrix un-scaled
  * back when precision = AMIE_MIXED); ndof vectors, original order. */
 int amie_gpu_batch_spmv(amie_gpu_batch *b, int32_t p, int32_t precision, const double *x, double *y);
+/* Times the iteration's SpMV kernel (q = Ks p with its dot products) over
+ * every problem of the batch, `iters` launches after one warm-up: the mean
+ * launch time and the algorithmic bytes of one launch (values + column
+ * indices + row pointers + p, r read and q written once; padding not
+ * counted).  The solver state is left as it was. */
+int amie_gpu_batch_time_spmv(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t iters,
+                             double *us_per_launch, double *bytes_per_launch);
 /* Element strains (nelem x 3|6, Voigt, engineering shear) of right-hand side
  * rhs, including the macro strain. */
 int amie_gpu_batch_element_strain(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain);

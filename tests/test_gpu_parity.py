@@ -143,6 +143,10 @@ def test_spmv_fp64_fp32(ctx, make):
     err = np.abs(ysp - yref).max()
     assert err <= 1e-9 * scale
     assert err < 1e-2 * np.abs(y32 - yref).max()
+    # 48-bit fixed point of the scaled matrix (absolute error < 2^-47 per
+    # scaled entry), FP64 accumulation
+    yfx = B.spmv(x, precision=ag.MIXED_FIX48)
+    assert np.abs(yfx - yref).max() <= 1e-12 * scale
 
 
 def test_fp64_cg_patch_test(ctx):
@@ -169,7 +173,8 @@ def oracle_solution(p, rtol=1e-13, precond=0):
 @pytest.mark.parametrize("precision,precond", [(ag.FP64, ag.JACOBI), (ag.MIXED, ag.JACOBI),
                                                (ag.MIXED, ag.BLOCK_JACOBI), (ag.FP64, ag.BLOCK_JACOBI),
                                                (ag.MIXED64, ag.JACOBI), (ag.MIXED_K64, ag.JACOBI),
-                                               (ag.MIXED_SPLIT, ag.JACOBI), (ag.MIXED_SPLIT, ag.BLOCK_JACOBI)])
+                                               (ag.MIXED_SPLIT, ag.JACOBI), (ag.MIXED_SPLIT, ag.BLOCK_JACOBI),
+                                               (ag.MIXED_FIX48, ag.JACOBI), (ag.MIXED_FIX48, ag.BLOCK_JACOBI)])
 def test_cg_composite_p2_vs_oracle(ctx, precision, precond):
     """Relative residual 1e-8 measured with the oracle's FP64 SpMV on the
     oracle's own matrix; the displacement error against the oracle solution
@@ -188,7 +193,7 @@ def test_cg_composite_p2_vs_oracle(ctx, precision, precond):
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-4
 
 
-@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED, ag.MIXED_SPLIT])
+@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED, ag.MIXED_SPLIT, ag.MIXED_FIX48])
 def test_tight_solve_error(ctx, precision):
     """Solved to relative residual 1e-12, FP64 and mixed precision (FP64
     refinement) reproduce the FP64 oracle displacement to 1e-7."""
@@ -203,7 +208,7 @@ def test_tight_solve_error(ctx, precision):
 
 
 @pytest.mark.parametrize("precision,precond", [(ag.MIXED, ag.BLOCK_JACOBI), (ag.FP64, ag.JACOBI),
-                                               (ag.MIXED_SPLIT, ag.BLOCK_JACOBI)])
+                                               (ag.MIXED_SPLIT, ag.BLOCK_JACOBI), (ag.MIXED_FIX48, ag.BLOCK_JACOBI)])
 def test_cg_damaged_tet_vs_oracle(ctx, precision, precond):
     p = configs.config3(8)
     A, b, xref = oracle_solution(p)
@@ -227,13 +232,13 @@ def test_bcsr_dropin_vs_reference_cg(ctx, tag):
     x = B.solution()
     xref = g(tag + "_cg_x")
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-8
-    for prec in (ag.MIXED, ag.MIXED_SPLIT):
+    for prec in (ag.MIXED, ag.MIXED_SPLIT, ag.MIXED_FIX48):
         st = B.solve(precision=prec, rtol=1e-10)
         assert st["status"] == 0
         assert np.linalg.norm(B.solution() - xref) / np.linalg.norm(xref) <= 1e-7
 
 
-@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED_SPLIT])
+@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED_SPLIT, ag.MIXED_FIX48])
 def test_reference_absolute_stop(ctx, precision):
     """ref_eps: the reference's stop sqrt(r . D^-1 r) <= eps
     (solvers/conjugategradient.cpp:101,218,305), measured here with the
@@ -255,7 +260,7 @@ def test_reference_absolute_stop(ctx, precision):
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
 
 
-@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT])
+@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT, ag.MIXED_FIX48])
 def test_rve_batch_effective_stiffness_vs_oracle(ctx, precision):
     probs = [configs.rve(n, 100 + n, f) for n, f in [(8, 0.25), (12, 0.3), (16, 0.4)]]
     B = ag.Batch(ctx, probs)
@@ -283,7 +288,7 @@ def test_rve_batch_effective_stiffness_vs_oracle(ctx, precision):
         assert np.abs(C - C.T).max() <= 1e-5 * np.abs(C).max()
 
 
-@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT])
+@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT, ag.MIXED_FIX48])
 def test_batch_result_independent_of_mix(ctx, precision):
     probs = [configs.rve(n, 300 + n, 0.3) for n in (10, 14, 20)]
     alone = ag.Batch(ctx, [configs.rve(14, 314, 0.3)])
@@ -293,7 +298,7 @@ def test_batch_result_independent_of_mix(ctx, precision):
     np.testing.assert_array_equal(alone.solution(0), mixed.solution(1))
 
 
-@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT])
+@pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT, ag.MIXED_FIX48])
 def test_graph_replay_matches_launches(ctx, precision):
     p = configs.config2(16)
     B = ag.Batch(ctx, [p])
@@ -405,8 +410,8 @@ def test_result_independent_of_grid(ctx):
 DROPIN = os.path.join(os.path.dirname(__file__), "..", "xfem-amie_amd", "host", "dropin_check")
 
 
-@pytest.mark.parametrize("tag,precision", [("sys_p1_12x10_mix", ag.MIXED_SPLIT), ("sys_tet_3", ag.MIXED_SPLIT),
-                                           ("sys_p1_8x8", ag.FP64)])
+@pytest.mark.parametrize("tag,precision", [("sys_p1_12x10_mix", ag.MIXED_FIX48), ("sys_tet_3", ag.MIXED_FIX48),
+                                           ("sys_p1_8x8", ag.FP64), ("sys_p1_8x8", ag.MIXED_SPLIT)])
 def test_cpp_linear_solver_dropin(tmp_path, tag, precision):
     """The C++ LinearSolver drop-in (host/gpu_conjugate_gradient.hpp), driven as
     Assembly::mgsolve drives a solver, on reference-assembled systems: the

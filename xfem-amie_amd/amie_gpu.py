@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libamie_gpu.so")
 
 P1, P2, TET4 = 1, 2, 3
-FP64, MIXED, MIXED64, MIXED_K64, MIXED_SPLIT = 0, 1, 2, 3, 4
+FP64, MIXED, MIXED64, MIXED_K64, MIXED_SPLIT, MIXED_FIX48 = 0, 1, 2, 3, 4, 5
 JACOBI, BLOCK_JACOBI = 0, 1
 REFINE_REPLACE, REFINE_RESTART = 0, 1
 NPER = {P1: 3, P2: 6, TET4: 4}
@@ -42,7 +42,7 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("status", c_i32), ("nproblems_converged", c_i32), ("iterations_max", c_i64),
                 ("iterations_sum", c_i64), ("refinements", c_i64), ("relres_max", c_dbl), ("solve_ms", c_dbl),
                 ("spmv_kernel_ms", c_dbl), ("spmv_launches", c_i64), ("spmv_bytes", c_dbl),
-                ("nproblems_escalated", c_i32)]
+                ("nproblems_escalated", c_i32), ("nproblems_handover", c_i32), ("iterations_handover", c_i64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -88,6 +88,7 @@ SIGNATURES = {
     "amie_gpu_batch_get_matrix": (c_i32, [vp, c_i32, vp, vp, vp, vp]),
     "amie_gpu_batch_get_element_blocks": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_spmv": (c_i32, [vp, c_i32, c_i32, vp, vp]),
+    "amie_gpu_batch_time_spmv": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp]),
     "amie_gpu_batch_element_strain": (c_i32, [vp, c_i32, c_i32, vp]),
     "amie_gpu_batch_average_stress": (c_i32, [vp, vp]),
     "amie_gpu_batch_damage_step": (c_i32, [vp, vp, vp]),
@@ -386,6 +387,13 @@ class Batch:
         y = np.zeros_like(x)
         check(lib().amie_gpu_batch_spmv(self.h, p, precision, _p(x), _p(y)))
         return y
+
+    def time_spmv(self, precision=MIXED_SPLIT, precond=JACOBI, iters=20):
+        """Mean SpMV-kernel launch time (us) and algorithmic bytes per launch
+        with every problem of the batch active."""
+        us, by = c_dbl(), c_dbl()
+        check(lib().amie_gpu_batch_time_spmv(self.h, precision, precond, iters, ctypes.byref(us), ctypes.byref(by)))
+        return us.value, by.value
 
     def element_strain(self, p=0, rhs=0):
         pr = self.problems[p]

@@ -429,6 +429,14 @@ __global__ __launch_bounds__(WG) void k_scale_cast(SolveArgs a, void *vw, const 
                 const double sc = a.S[(int64_t)c * B + j];
                 for (int i = 0; i < B; i++) o[j * B + i] = valid ? (T)(sr[i] * blk[j * B + i] * sc) : (T)0;
             }
+            if constexpr (std::is_same<M, Fix48>::value) {
+                bool bad = false;
+                for (int q = 0; q < B * B; q++) bad |= !(fabs((double)o[q]) < FIX48_LIMIT);
+                if (bad) {
+                    *a.fix48_bad = 1;   // every writer stores the same value
+                    for (int q = 0; q < B * B; q++) o[q] = 0.;
+                }
+            }
             if (vw) Blk<M, B>::store(vw, a.nslots, slot, o);
         }
         if (!valid || !minv) continue;

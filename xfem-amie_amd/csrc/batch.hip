@@ -76,7 +76,8 @@ Batch::~Batch()
 
 size_t Batch::bytes() const
 {
-    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &col, &dslot, &row_len, &v64, &v32, &vsp, &tile_slice0,
+    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &col, &dslot, &row_len, &v64,
+                           &v32, &vsp, &vfx, &fix48_bad, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
                            &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
                            &bcval, &coords,
@@ -98,8 +99,12 @@ SolveArgs Batch::args(int mode) const
     a.col = col.as<uint32_t>();
     a.nslots = nslots;
     a.v64 = v64.as<double>();
-    a.vw = m32 ? v32.p : (mode == AMIE_MIXED_SPLIT ? vsp.p : v64.p);
+    a.vw = m32 ? v32.p : (mode == AMIE_MIXED_SPLIT ? vsp.p : (mode == AMIE_MIXED_FIX48 ? vfx.p : v64.p));
+    a.fix48_bad = fix48_bad.as<int32_t>();
     a.tile_slice0 = tile_slice0.as<int32_t>();
+    a.tile_desc = tile_desc.as<int4>();
+    a.slice_desc = slice_desc.as<int4>();
+    a.ntiles_all = (int32_t)ntiles;
     a.tile_nslices = tile_nslices.as<int32_t>();
     a.tile_prob = tile_prob.as<int32_t>();
     a.active = active.as<int32_t>();
@@ -208,6 +213,10 @@ static int alloc_vectors(Batch &B)
     if (!B.v64.alloc(B.nslots * slot_bytes(true, B.et.b))) return AMIE_ERR_NOMEM;
     if (!B.v32.alloc(B.nslots * slot_bytes(false, B.et.b))) return AMIE_ERR_NOMEM;
     if (!B.vsp.alloc(B.nslots * split_slot_bytes(B.et.b))) return AMIE_ERR_NOMEM;
+    if (!B.vfx.alloc(B.nslots * split_slot_bytes(B.et.b))) return AMIE_ERR_NOMEM;   // same slot size
+    if (!B.fix48_bad.alloc(4)) return AMIE_ERR_NOMEM;
+    AMIE_HIP(hipMemset(B.vfx.p, 0, B.vfx.n));
+    AMIE_HIP(hipMemset(B.fix48_bad.p, 0, 4));
     AMIE_HIP(hipMemset(B.v64.p, 0, B.v64.n));
     AMIE_HIP(hipMemset(B.v32.p, 0, B.v32.n));
     AMIE_HIP(hipMemset(B.vsp.p, 0, B.vsp.n));
@@ -319,6 +328,13 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
     if ((rc = alloc_and_upload(B.tile_slice0, h_ts0.data(), h_ts0.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.tile_nslices, h_tns.data(), h_tns.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.tile_prob, h_tprob.data(), h_tprob.size() * 4))) return rc;
+    // packed descriptors for the iteration kernels: one 16-byte load each
+    std::vector<int4> h_td(tiles), h_sd(slices);
+    for (int64_t t = 0; t < tiles; t++) h_td[t] = make_int4(h_ts0[t], h_tns[t], h_tprob[t], 0);
+    for (int64_t sl = 0; sl < slices; sl++)
+        h_sd[sl] = make_int4((int32_t)(h_slice_ptr[sl] & 0xFFFFFFFF), (int32_t)(h_slice_ptr[sl] >> 32), h_width[sl], h_rows[sl]);
+    if ((rc = alloc_and_upload(B.tile_desc, h_td.data(), h_td.size() * 16))) return rc;
+    if ((rc = alloc_and_upload(B.slice_desc, h_sd.data(), h_sd.size() * 16))) return rc;
     return AMIE_OK;
 }
 
@@ -581,6 +597,8 @@ template <int Bk> static void launch_post(Batch &B)
     hipLaunchKernelGGL((k_diag_scale<Bk>), dim3(g), dim3(WG), 0, st, a, B.dslot.as<int64_t>(), B.S.as<double>(), (int)B.ntiles);
     hipLaunchKernelGGL((k_scale_cast<Split, float, Bk>), dim3(g), dim3(WG), 0, st, a, B.vsp.p, B.dslot.as<int64_t>(),
                        (float *)nullptr, (int)B.ntiles);
+    hipLaunchKernelGGL((k_scale_cast<Fix48, float, Bk>), dim3(g), dim3(WG), 0, st, a, B.vfx.p, B.dslot.as<int64_t>(),
+                       (float *)nullptr, (int)B.ntiles);
     hipLaunchKernelGGL((k_scale_cast<float, float, Bk>), dim3(g), dim3(WG), 0, st, a, B.v32.p, B.dslot.as<int64_t>(),
                        B.minv32.as<float>(), (int)B.ntiles);
     hipLaunchKernelGGL((k_scale_cast<double, double, Bk>), dim3(g), dim3(WG), 0, st, a, (void *)nullptr, B.dslot.as<int64_t>(),
@@ -612,10 +630,14 @@ int Batch::assemble()
         else if (et.et == 2) { if (nrhs == 1) launch_assemble<2, 1>(*this, a); else launch_assemble<2, 3>(*this, a); }
         else { if (nrhs == 1) launch_assemble<3, 1>(*this, a); else launch_assemble<3, 3>(*this, a); }
     }
+    AMIE_HIP(hipMemsetAsync(fix48_bad.p, 0, 4, st));
     if (et.b == 2) launch_post<2>(*this);
     else launch_post<3>(*this);
     AMIE_HIP(hipGetLastError());
+    int32_t bad = 0;
+    AMIE_HIP(hipMemcpyAsync(&bad, fix48_bad.p, 4, hipMemcpyDeviceToHost, st));
     AMIE_HIP(hipStreamSynchronize(st));
+    fix48_ok = bad == 0;
     return AMIE_OK;
 }
 
@@ -638,6 +660,7 @@ static Kern select_kern(int mode, int b, int nr, bool bj)
     if (mode == AMIE_MIXED64) return b == 2 ? kern_nr<double, float, 2>(nr, bj) : kern_nr<double, float, 3>(nr, bj);
     if (mode == AMIE_MIXED_K64) return b == 2 ? kern_nr<float, double, 2>(nr, bj) : kern_nr<float, double, 3>(nr, bj);
     if (mode == AMIE_MIXED_SPLIT) return b == 2 ? kern_nr<float, Split, 2>(nr, bj) : kern_nr<float, Split, 3>(nr, bj);
+    if (mode == AMIE_MIXED_FIX48) return b == 2 ? kern_nr<float, Fix48, 2>(nr, bj) : kern_nr<float, Fix48, 3>(nr, bj);
     return b == 2 ? kern_nr<double, double, 2>(nr, bj) : kern_nr<double, double, 3>(nr, bj);
 }
 
@@ -652,7 +675,8 @@ static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs
 // inverses).  Padding blocks are not counted.
 static double spmv_bytes(const Batch &B, const ProbHost &P, int mode, bool bj)
 {
-    const double sm = (mode == AMIE_FP64 || mode == AMIE_MIXED_K64) ? 8. : (mode == AMIE_MIXED_SPLIT ? 6. : 4.);
+    const double sm = (mode == AMIE_FP64 || mode == AMIE_MIXED_K64) ? 8.
+                      : ((mode == AMIE_MIXED_SPLIT || mode == AMIE_MIXED_FIX48) ? 6. : 4.);
     const double sv = (mode == AMIE_FP64 || mode == AMIE_MIXED64) ? 8. : 4.;   // vectors
     const double b = B.et.b;
     const double nnzb = (double)P.column_index.size();
@@ -673,7 +697,9 @@ int Batch::solve_begin(const amie_solve_opts &o)
     AMIE_HIP(hipEventCreate(&R.e1));
     AMIE_HIP(hipEventCreateWithFlags(&R.ev_status, hipEventDisableTiming));
     AMIE_HIP(hipEventRecord(R.e0, s));
-    return start_pass(o.precision, std::vector<char>(nprob, 1));
+    const int rc = start_pass(o.precision, std::vector<char>(nprob, 1));
+    if (rc) R.started = false;
+    return rc;
 }
 
 // Queues one checkpoint period: ce iterations, then the refinement slot.
@@ -713,6 +739,10 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
     R.bj = o.precond == AMIE_BLOCK_JACOBI;
     R.per_check = std::max(1, (o.chunk > 0 ? o.chunk : 32) / R.ce);
     hipStream_t s = stream;
+    if (mode == AMIE_MIXED_FIX48 && !fix48_ok) {
+        set_error("AMIE_MIXED_FIX48: a scaled matrix entry is outside (-2, 2) (K not symmetric positive definite?)");
+        return AMIE_ERR_ARG;
+    }
     R.K = select_kern(mode, et.b, nrhs, R.bj);
     R.a = args(mode);
     const bool fp32vec = mode != AMIE_FP64 && mode != AMIE_MIXED64;
@@ -725,7 +755,10 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
         P.maxit = o.maxit > 0 ? o.maxit : ph[p].nrows * et.b;
         P.ckpt_interval = R.mixed ? (o.ckpt_interval > 0 ? o.ckpt_interval : 128) : ((int64_t)1 << 62);
-        P.stall_iters = (fp32vec && o.escalate) ? 1024 : 0;
+        P.stall_iters = (fp32vec && o.escalate) ? 8192 : 0;
+        P.ladder = mode == AMIE_MIXED_SPLIT ? 1 : 0;
+        P.kmode = 0;
+        P.iters_k64 = 0;
         P.status = ST_RUNNING;
         P.ckpt_pending = 1;   // the pass opens with a refinement: r = S(b - K x), p = M r
         P.src_ckpt = 0;
@@ -735,6 +768,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         P.last_ckpt = 0;
         for (int k = 0; k < MAXRHS; k++) {
             P.alpha[k] = P.beta[k] = 0.;
+            P.rho_pred[k] = 0.;
             P.rr[k] = 0.;
             P.rr_ckpt[k] = DBL_MAX;
             P.rr_target[k] = 0.;
@@ -776,11 +810,13 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         graphs.push_back({key, R.exec});
     }
     R.pbytes.assign(nprob, 0.);
+    R.pbytes_k64.assign(nprob, 0.);
     R.was_running.assign(nprob, 0);
     int64_t maxit_all = 0;
     for (int p = 0; p < nprob; p++) {
         if (!subset[p]) continue;
         R.pbytes[p] = spmv_bytes(*this, ph[p], mode, R.bj);
+        R.pbytes_k64[p] = spmv_bytes(*this, ph[p], AMIE_MIXED_K64, R.bj);
         R.was_running[p] = 1;
         maxit_all = std::max<int64_t>(maxit_all, h_prob[p].maxit);
     }
@@ -849,9 +885,12 @@ int Batch::solve_poll(bool wait, bool *finished)
         // problem is one the reduce kernel then counted (iters) or finished
         if (R.o.time_kernels)
             for (int p = 0; p < nprob; p++)
-                if (R.pbytes[p] > 0.)
-                    R.stats.spmv_bytes += R.pbytes[p] * (double)(h_prob_pinned[p].iters +
-                                                                 (h_prob_pinned[p].status != ST_RUNNING ? 1 : 0));
+                if (R.pbytes[p] > 0.) {
+                    const ProbDev &P = h_prob_pinned[p];
+                    const double n = (double)(P.iters + (P.status != ST_RUNNING ? 1 : 0));
+                    const double nk = std::min(n, (double)P.iters_k64);
+                    R.stats.spmv_bytes += R.pbytes[p] * (n - nk) + R.pbytes_k64[p] * nk;
+                }
         // problems whose FP32 iteration stagnated continue from their current
         // iterate in FP64, in a pass of their own (an FP32 copy of a badly
         // conditioned K can lose the accuracy CG needs, so the matrix goes to
@@ -898,6 +937,10 @@ int Batch::solve_end(amie_solve_stats *st)
         stats.iterations_max = std::max<int64_t>(stats.iterations_max, P.iters);
         stats.iterations_sum += P.iters;
         stats.refinements += P.nckpt;
+        if (P.kmode) {
+            stats.nproblems_handover++;
+            stats.iterations_handover += P.iters_k64;
+        }
         for (int k = 0; k < nrhs; k++) stats.relres_max = std::max(stats.relres_max, P.relres_out[k]);
         if (P.status == ST_CONVERGED) stats.nproblems_converged++;
         else all = false;
@@ -1070,7 +1113,7 @@ void amie_gpu_default_opts(amie_solve_opts *o)
 {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
-    o->precision = AMIE_MIXED_SPLIT;
+    o->precision = AMIE_MIXED_FIX48;
     o->precond = AMIE_JACOBI;
     o->refine = AMIE_REFINE_REPLACE;
     o->rtol = 1e-8;
@@ -1183,7 +1226,7 @@ int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_so
     amie_solve_opts o;
     amie_gpu_default_opts(&o);
     if (opts) o = *opts;
-    if (o.precision < AMIE_FP64 || o.precision > AMIE_MIXED_SPLIT) {
+    if (o.precision < AMIE_FP64 || o.precision > AMIE_MIXED_FIX48) {
         set_error("bad precision");
         return AMIE_ERR_ARG;
     }
@@ -1286,8 +1329,9 @@ int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const 
     Batch &B = *bb->b;
     int rc = upload_vec(B, p, B.q, x); // q as scratch input (8-byte entries)
     if (rc) return rc;
-    const int mode = precision == AMIE_MIXED ? 1 : (precision == AMIE_MIXED_SPLIT ? 2 : 0);
-    SolveArgs a = B.args(mode == 1 ? AMIE_MIXED : (mode == 2 ? AMIE_MIXED_SPLIT : AMIE_FP64));
+    const int mode = precision == AMIE_MIXED ? 1 : (precision == AMIE_MIXED_SPLIT ? 2 : (precision == AMIE_MIXED_FIX48 ? 3 : 0));
+    SolveArgs a = B.args(precision == AMIE_MIXED || precision == AMIE_MIXED_SPLIT || precision == AMIE_MIXED_FIX48 ? precision
+                                                                                                               : AMIE_FP64);
     const int g = B.grid();
     double *xin = B.q.as<double>(), *yo = B.r.as<double>();
     hipStream_t st = B.stream;
@@ -1302,6 +1346,49 @@ int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const 
     AMIE_HIP(hipGetLastError());
     AMIE_HIP(hipStreamSynchronize(st));
     return download_vec(B, p, B.r, y);
+}
+
+int amie_gpu_batch_time_spmv(amie_gpu_batch *bb, int32_t precision, int32_t precond, int32_t iters,
+                             double *us_per_launch, double *bytes_per_launch)
+{
+    if (!bb || !bb->b || iters <= 0 || precision < AMIE_FP64 || precision > AMIE_MIXED_FIX48 || !us_per_launch) {
+        set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    Batch &B = *bb->b;
+    if (B.run.started) { set_error("a solve is in progress"); return AMIE_ERR_BUSY; }
+    const bool bj = precond == AMIE_BLOCK_JACOBI;
+    const Kern K = select_kern(precision, B.et.b, B.nrhs, bj);
+    const SolveArgs a = B.args(precision);
+    hipStream_t st = B.stream;
+    // every problem running for the measurement, the recorded state after
+    std::vector<ProbDev> run(B.h_prob);
+    for (auto &P : run) P.status = ST_RUNNING;
+    int rc = B.set_active_all();
+    if (rc) return rc;
+    AMIE_HIP(hipMemcpyAsync(B.prob.p, run.data(), B.nprob * sizeof(ProbDev), hipMemcpyHostToDevice, st));
+    const int g = (int)B.ntiles;
+    hipEvent_t e0, e1;
+    AMIE_HIP(hipEventCreate(&e0));
+    AMIE_HIP(hipEventCreate(&e1));
+    launch0(K.k1, g, st, a);
+    AMIE_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < iters; i++) launch0(K.k1, g, st, a);
+    AMIE_HIP(hipEventRecord(e1, st));
+    AMIE_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    AMIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    AMIE_HIP(hipMemcpyAsync(B.prob.p, B.h_prob.data(), B.nprob * sizeof(ProbDev), hipMemcpyHostToDevice, st));
+    AMIE_HIP(hipStreamSynchronize(st));
+    *us_per_launch = 1e3 * ms / iters;
+    if (bytes_per_launch) {
+        double s = 0.;
+        for (int p = 0; p < B.nprob; p++) s += spmv_bytes(B, B.ph[p], precision, bj);
+        *bytes_per_launch = s;
+    }
+    return AMIE_OK;
 }
 
 static StrainArgs strain_args(Batch &B)

@@ -73,6 +73,7 @@ struct SolveRun {
     int gt = 1, gp = 1;      // grids: active tiles, active problems
     hipGraphExec_t exec = nullptr;
     std::vector<double> pbytes;       // algorithmic bytes of one SpMV launch, per problem of the pass
+    std::vector<double> pbytes_k64;   // the same with the FP64 matrix (after a precision hand-over)
     std::vector<hipEvent_t> tev;
     int nt = 0;
     std::vector<char> was_running;
@@ -106,11 +107,14 @@ struct Batch {
     std::vector<ProbHost> ph;
     int64_t nslices = 0, ntiles = 0, nrows_pad = 0, nslots = 0, nelem = 0, nnodes_geo = 0;
     bool has_elements = false, has_macro = false, has_damage_state = false;
+    bool fix48_ok = false;            // the last assembly's scaled values fit the fixed-point range
     std::vector<int32_t> colour_ptr;  // colour c elements: colour_elems[colour_ptr[c]..colour_ptr[c+1])
 
     // SELL structure
     DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
-    DevBuf v64, v32, vsp;   // FP64 unscaled K; FP32 and split FP32+bf16 scaled Ks
+    DevBuf tile_desc, slice_desc;     // packed {slice0, nslices, prob} / {slot ptr lo, hi, width, rows}
+    DevBuf v64, v32, vsp, vfx;   // FP64 unscaled K; FP32, split FP32+bf16 and 48-bit fixed-point scaled Ks
+    DevBuf fix48_bad;            // int flag: a scaled value outside the fixed-point range
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active, active_prob, n_active_prob;
     DevBuf prob, part1, part2, partc, partn;
     // vectors: one plane of nrows_pad * b entries per right-hand side; 8 bytes

@@ -72,7 +72,7 @@ struct ProbDev {
     double ref_eps;          // > 0: converged when ||S r64|| <= ref_eps (absolute, scaled)
     int64_t maxit;
     int64_t ckpt_interval;   // refinement at least every this many iterations
-    int64_t stall_iters;     // > 0: ST_STALLED when the FP64 residual has not halved in that many iterations
+    int64_t stall_iters;     // > 0: ST_STALLED at the accuracy floor, or when the FP64 residual has not halved in that many iterations
     // state, written by k_reduce (src_ckpt: raised by k_ckpt_x)
     int32_t status;          // a finished status only ever stops work
     int32_t ckpt_pending;    // a refinement is due at the next checkpoint slot
@@ -80,8 +80,12 @@ struct ProbDev {
     int32_t first;           // the next refinement starts the recurrence (p = M r)
     int32_t done[MAXRHS];
     double alpha[MAXRHS], beta[MAXRHS];
+    double rho_pred[MAXRHS]; // the recurrence's rho for the next residual
     double relres[MAXRHS];
     int64_t iters, nckpt, last_ckpt;
+    int32_t ladder;          // the split matrix may hand over to the FP64 matrix
+    int32_t kmode;           // 1: the iteration's SpMV uses the FP64 matrix (S K S on the fly)
+    int64_t iters_k64;       // iterations run with kmode = 1
     double rr[MAXRHS], rr_ckpt[MAXRHS], rr_target[MAXRHS], bb[MAXRHS], bb64[MAXRHS], rel_best[MAXRHS];
     int64_t best_it[MAXRHS]; // iteration of the last halving of rel_best
     // results for the host
@@ -99,6 +103,9 @@ struct SolveArgs {
     const double *v64;       // FP64 planes (unscaled K)
     const void *vw;          // planes of the iteration's SpMV (scaled FP32 S K S, or v64)
     const int32_t *tile_slice0, *tile_nslices, *tile_prob;
+    const int4 *tile_desc;   // per tile {first slice, slices, problem, 0}
+    const int4 *slice_desc;  // per slice {slot pointer lo, hi, width, rows}
+    int32_t ntiles_all;
     const int32_t *active;   // active tiles
     const int32_t *n_active;
     const int32_t *active_prob;
@@ -114,6 +121,7 @@ struct SolveArgs {
     const double *S;         // symmetric Jacobi scaling, per dof
     void *y, *r, *p, *q;     // iteration vectors (float or double)
     const void *minv;        // block-Jacobi inverses of the scaled diagonal blocks, per row (b*b, row-major)
+    int32_t *fix48_bad;      // set when a scaled value does not fit the fixed-point range
 };
 
 // ------------------------------------------------------------ block planes
@@ -356,6 +364,136 @@ template <> struct BlkRaw<Split, 3> {
     }
 };
 
+// 48-bit fixed point: every entry of the scaled matrix Ks = S K S of an SPD
+// K lies in (-1, 1] (unit diagonal, |Ks_ij| <= sqrt(Ks_ii Ks_jj)), so one
+// global scale 2^-46 holds each value with an absolute error below 2^-47
+// (7e-15) in 6 bytes: the low 32 bits in the first planes (laid out as the
+// FP32 planes), the high 16 bits (signed) after them:
+//   2x2: 4 x uint32 + 4 x int16          3x3: 8 + 1 x uint32, 8 + 1 x int16
+struct Fix48 {};
+constexpr double FIX48_SCALE = 70368744177664.0;        // 2^46
+constexpr double FIX48_ULP = 1.0 / 70368744177664.0;    // 2^-46
+constexpr double FIX48_LIMIT = 2.0;                     // |value| < 2 fits 48 bits
+
+__host__ __device__ inline double fix48_value(uint32_t lo, uint32_t hi16)
+{
+    const double h = (double)(int32_t)(int16_t)(uint16_t)hi16;
+    return (h * 4294967296.0 + (double)lo) * FIX48_ULP;
+}
+__host__ __device__ inline void fix48_encode(double v, uint32_t &lo, uint32_t &hi16)
+{
+    const int64_t q = (int64_t)llrint(v * FIX48_SCALE);
+    lo = (uint32_t)(uint64_t)q;
+    hi16 = (uint32_t)(uint16_t)(int16_t)(q >> 32);
+}
+
+template <> struct Blk<Fix48, 2> {
+    typedef double T;
+    static constexpr int N = 4;
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, double *a)
+    {
+        const u4v l = __builtin_nontemporal_load(reinterpret_cast<const u4v *>(base) + s);
+        const u2v h = __builtin_nontemporal_load(reinterpret_cast<const u2v *>(reinterpret_cast<const char *>(base) + ns * 16) + s);
+        a[0] = fix48_value(l.x, h.x & 0xFFFFu);
+        a[1] = fix48_value(l.y, h.x >> 16);
+        a[2] = fix48_value(l.z, h.y & 0xFFFFu);
+        a[3] = fix48_value(l.w, h.y >> 16);
+    }
+    __device__ static inline void store(void *base, int64_t ns, int64_t s, const double *a)
+    {
+        uint32_t lo[4], hi[4];
+        for (int i = 0; i < 4; i++) fix48_encode(a[i], lo[i], hi[i]);
+        reinterpret_cast<uint4 *>(base)[s] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+        reinterpret_cast<uint2 *>(reinterpret_cast<char *>(base) + ns * 16)[s] = make_uint2(hi[0] | hi[1] << 16, hi[2] | hi[3] << 16);
+    }
+    static constexpr size_t bytes_per_slot = 24;
+};
+template <> struct Blk<Fix48, 3> {
+    typedef double T;
+    static constexpr int N = 9;
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, double *a)
+    {
+        const u4v *p = reinterpret_cast<const u4v *>(base);
+        const u4v l0 = __builtin_nontemporal_load(p + s);
+        const u4v l1 = __builtin_nontemporal_load(p + ns + s);
+        const uint32_t l2 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p + 2 * ns) + s);
+        const char *hb = reinterpret_cast<const char *>(base) + ns * 36;
+        const u4v h0 = __builtin_nontemporal_load(reinterpret_cast<const u4v *>(hb) + s);
+        const uint16_t h1 = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(hb + ns * 16) + s);
+        const uint32_t lv[9] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w, l2};
+        const uint32_t hw[4] = {h0.x, h0.y, h0.z, h0.w};
+#pragma unroll
+        for (int i = 0; i < 8; i++) a[i] = fix48_value(lv[i], (hw[i / 2] >> (16 * (i % 2))) & 0xFFFFu);
+        a[8] = fix48_value(lv[8], h1);
+    }
+    __device__ static inline void store(void *base, int64_t ns, int64_t s, const double *a)
+    {
+        uint32_t lo[9], hi[9];
+        for (int i = 0; i < 9; i++) fix48_encode(a[i], lo[i], hi[i]);
+        uint4 *p = reinterpret_cast<uint4 *>(base);
+        p[s] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+        p[ns + s] = make_uint4(lo[4], lo[5], lo[6], lo[7]);
+        reinterpret_cast<uint32_t *>(p + 2 * ns)[s] = lo[8];
+        char *hb = reinterpret_cast<char *>(base) + ns * 36;
+        reinterpret_cast<uint4 *>(hb)[s] = make_uint4(hi[0] | hi[1] << 16, hi[2] | hi[3] << 16, hi[4] | hi[5] << 16, hi[6] | hi[7] << 16);
+        reinterpret_cast<uint16_t *>(hb + ns * 16)[s] = (uint16_t)hi[8];
+    }
+    static constexpr size_t bytes_per_slot = 54;
+};
+template <> struct BlkRaw<Fix48, 2> {
+    typedef double T;
+    struct Raw {
+        u4v l;
+        u2v h;
+    };
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, Raw &r)
+    {
+        r.l = __builtin_nontemporal_load(reinterpret_cast<const u4v *>(base) + s);
+        r.h = __builtin_nontemporal_load(reinterpret_cast<const u2v *>(reinterpret_cast<const char *>(base) + ns * 16) + s);
+    }
+    __device__ static inline double at(const Raw &r, int k)
+    {
+        const uint32_t lo = k == 0 ? r.l.x : k == 1 ? r.l.y : k == 2 ? r.l.z : r.l.w;
+        const uint32_t w = k < 2 ? r.h.x : r.h.y;
+        return fix48_value(lo, (w >> (16 * (k & 1))) & 0xFFFFu);
+    }
+};
+template <> struct BlkRaw<Fix48, 3> {
+    typedef double T;
+    struct Raw {
+        u4v l0, l1;
+        uint32_t l2;
+        u4v h0;
+        uint32_t h1;
+    };
+    __device__ static inline void load(const void *base, int64_t ns, int64_t s, Raw &r)
+    {
+        const u4v *p = reinterpret_cast<const u4v *>(base);
+        r.l0 = __builtin_nontemporal_load(p + s);
+        r.l1 = __builtin_nontemporal_load(p + ns + s);
+        r.l2 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p + 2 * ns) + s);
+        const char *hb = reinterpret_cast<const char *>(base) + ns * 36;
+        r.h0 = __builtin_nontemporal_load(reinterpret_cast<const u4v *>(hb) + s);
+        r.h1 = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(hb + ns * 16) + s);
+    }
+    __device__ static inline double at(const Raw &r, int k)
+    {
+        uint32_t lo, w;
+        switch (k) {
+        case 0: lo = r.l0.x; w = r.h0.x; break;
+        case 1: lo = r.l0.y; w = r.h0.x >> 16; break;
+        case 2: lo = r.l0.z; w = r.h0.y; break;
+        case 3: lo = r.l0.w; w = r.h0.y >> 16; break;
+        case 4: lo = r.l1.x; w = r.h0.z; break;
+        case 5: lo = r.l1.y; w = r.h0.z >> 16; break;
+        case 6: lo = r.l1.z; w = r.h0.w; break;
+        case 7: lo = r.l1.w; w = r.h0.w >> 16; break;
+        default: lo = r.l2; w = r.h1; break;
+        }
+        return fix48_value(lo, w & 0xFFFFu);
+    }
+};
+
 // Matrix value kinds: FP64 planes hold the unscaled K (the scaling is applied
 // on the fly); FP32 and split planes hold the scaled Ks.  FP64 and split
 // products accumulate in FP64.
@@ -366,6 +504,9 @@ template <> struct MatKind<double> {
     static constexpr bool onfly = true, acc64 = true;
 };
 template <> struct MatKind<Split> {
+    static constexpr bool onfly = false, acc64 = true;
+};
+template <> struct MatKind<Fix48> {
     static constexpr bool onfly = false, acc64 = true;
 };
 
@@ -543,37 +684,89 @@ __device__ inline void spmv_row(const SolveArgs &a, const void *vals, int s, int
     }
 }
 
-// The tile of a one-workgroup-per-tile launch, or -1.
+// The tile of a one-workgroup-per-tile launch, or -1.  With every tile
+// active the list lookup is skipped.
 __device__ inline int my_tile(const SolveArgs &a)
 {
     const int it = xcd_index();
-    return it < *a.n_active ? a.active[it] : -1;
+    const int na = *a.n_active;
+    if (it >= na) return -1;
+    return na == a.ntiles_all ? it : a.active[it];
+}
+
+// Row loop of one slice from its packed descriptor (same arithmetic as
+// spmv_row).
+template <typename A, typename M, int B, int NR, typename G>
+__device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t sp0, int w, int lane, A (*acc)[B],
+                                  G gather)
+{
+    constexpr int U = Unroll<M, B>::U;
+    const int64_t sp = sp0 + lane;
+    for (int k = 0; k < w; k += U) {
+        typedef BlkRaw<M, B> BR;
+        uint32_t c[U];
+        typename BR::Raw blk[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) c[u] = __builtin_nontemporal_load(a.col + sp + (int64_t)(k + u) * WAVE);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) BR::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) {
+                A xv[NR][B];
+                gather(c[u], xv);
+#pragma unroll
+                for (int j = 0; j < B; j++)
+#pragma unroll
+                    for (int i = 0; i < B; i++) {
+                        const A m = (A)BR::at(blk[u], j * B + i);
+#pragma unroll
+                        for (int r = 0; r < NR; r++) acc[r][i] += m * xv[r][j];
+                    }
+            }
+    }
 }
 
 // ------------------------------------------------------------ K1: SpMV
 // q = Ks p with partial sums of p.q, q.z and q.Mq (z = M r).  With the FP64
 // matrix the unscaled K is used and the scaling applied on the fly:
-// q = S K (S p).
+// q = S K (S p).  With AMIE_EARLY_OWN the wave's own p and r are requested
+// with its first matrix loads (the row loop's latency covers them, at the
+// cost of registers); otherwise after the row loop.
+#ifndef AMIE_EARLY_OWN
+#define AMIE_EARLY_OWN 0
+#endif
+// Body of K1 for one tile with matrix kind M (values at `vals`).
 template <typename W, typename M, int B, int NR, bool BJ>
-__global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
+__device__ inline void spmv_tile(const SolveArgs &a, const void *vals, int t, const int4 &td, int wave, int lane)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
     constexpr bool ONFLY = MatKind<M>::onfly;
     // products accumulate in FP64 with the FP64 and split matrices, else in W
     typedef typename std::conditional<MatKind<M>::acc64, double, W>::type A;
-    const int t = my_tile(a);
-    if (t < 0) return;
-    const ProbDev &P = a.prob[a.tile_prob[t]];
-    if (P.status != ST_RUNNING) return;
-    const int wave = wave_id(), lane = threadIdx.x % WAVE;
+    const bool has = wave < td.y;
+    const int4 sd = a.slice_desc[has ? td.x + wave : td.x];
     const W *p = cvw<W>(a.p), *r = cvw<W>(a.r);
     W *q = vw<W>(a.q);
     double part[3 * NR];
 #pragma unroll
     for (int k = 0; k < 3 * NR; k++) part[k] = 0.;
-    if (wave < a.tile_nslices[t]) {
-        const int s = a.tile_slice0[t] + wave;
+    if (has) {
+        const int s = td.x + wave;
         const int64_t row = (int64_t)s * WAVE + lane;
+        const bool own = lane < sd.w;
+        W pv[NR][B], rv[NR][B];
+        auto load_own = [&]() {
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                const size_t idx = k * a.vplane + (size_t)row * B;
+                Node<W, B>::load(p + idx, pv[k]);
+                Node<W, B, true>::load(r + idx, rv[k]);
+            }
+        };
+        if (AMIE_EARLY_OWN && own) load_own();
         A acc[NR][B];
 #pragma unroll
         for (int k = 0; k < NR; k++)
@@ -584,14 +777,16 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
             if (ONFLY) Node<double, B>::load(a.S + (size_t)c * B, sc);
 #pragma unroll
             for (int k = 0; k < NR; k++) {
-                W t[B];
-                Node<W, B>::load(p + k * a.vplane + (size_t)c * B, t);
+                W tv[B];
+                Node<W, B>::load(p + k * a.vplane + (size_t)c * B, tv);
 #pragma unroll
-                for (int j = 0; j < B; j++) xv[k][j] = ONFLY ? (A)t[j] * (A)sc[j] : (A)t[j];
+                for (int j = 0; j < B; j++) xv[k][j] = ONFLY ? (A)tv[j] * (A)sc[j] : (A)tv[j];
             }
         };
-        spmv_row<A, M, B, NR>(a, a.vw, s, lane, acc, gather);
-        if (lane < a.slice_rows[s]) {
+        const int64_t sp0 = (int64_t)(uint32_t)sd.x | ((int64_t)sd.y << 32);
+        spmv_slice<A, M, B, NR>(a, vals, sp0, sd.z, lane, acc, gather);
+        if (own) {
+            if (!AMIE_EARLY_OWN) load_own();
             W mi[B * B];
             load_minv<W, B, BJ>(a, row, mi);
             double sr[B];
@@ -599,17 +794,15 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
 #pragma unroll
             for (int k = 0; k < NR; k++) {
                 const size_t idx = k * a.vplane + (size_t)row * B;
-                W pv[B], rv[B], qv[B], zv[B], mq[B];
-                Node<W, B>::load(p + idx, pv);
-                Node<W, B, true>::load(r + idx, rv);
+                W qv[B], zv[B], mq[B];
 #pragma unroll
                 for (int i = 0; i < B; i++) qv[i] = ONFLY ? (W)(acc[k][i] * (A)sr[i]) : (W)acc[k][i];
                 Node<W, B, true>::store(q + idx, qv);
-                precond<W, B, BJ>(mi, rv, zv);
+                precond<W, B, BJ>(mi, rv[k], zv);
                 precond<W, B, BJ>(mi, qv, mq);
 #pragma unroll
                 for (int i = 0; i < B; i++) {
-                    part[k] += (double)pv[i] * (double)qv[i];
+                    part[k] += (double)pv[k][i] * (double)qv[i];
                     part[NR + k] += (double)qv[i] * (double)zv[i];
                     part[2 * NR + k] += (double)qv[i] * (double)mq[i];
                 }
@@ -617,6 +810,24 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
         }
     }
     write_partials<3 * NR>(part, a.part1, t, lds);
+}
+
+// ------------------------------------------------------------ K1: SpMV
+// q = Ks p with partial sums of p.q, q.z and q.Mq (z = M r).  With the FP64
+// matrix the unscaled K is used and the scaling applied on the fly:
+// q = S K (S p).  A problem on the split matrix whose reduce kernel found its
+// accuracy floor (kmode) runs the same product from the FP64 matrix.
+template <typename W, typename M, int B, int NR, bool BJ>
+__global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
+{
+    const int t = my_tile(a);
+    if (t < 0) return;
+    const int4 td = a.tile_desc[t];
+    const ProbDev &P = a.prob[td.z];
+    if (P.status != ST_RUNNING) return;
+    const int wave = wave_id(), lane = threadIdx.x % WAVE;
+    if (std::is_same<M, Split>::value && P.kmode) spmv_tile<W, double, B, NR, BJ>(a, a.v64, t, td, wave, lane);
+    else spmv_tile<W, M, B, NR, BJ>(a, a.vw, t, td, wave, lane);
 }
 
 // ------------------------------------------- R: per-problem CG scalars
@@ -635,7 +846,7 @@ __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
     if (ck) problem_sums<4 * NR>(a.partc, P, s0, lds);
     else problem_sums<2 * NR>(a.part2, P, s0, lds);
     if (threadIdx.x != 0) return;
-    bool all = true, bad = false, stalled = false, need = false;
+    bool all = true, bad = false, stalled = false, need = false, ladder_up = false;
     for (int k = 0; k < NR; k++) {
         if (P.done[k]) {
             P.alpha[k] = P.beta[k] = 0.;
@@ -643,6 +854,15 @@ __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
         }
         const double rho = s0[k], rr = s0[NR + k];
         if (ck) {
+            // accuracy floor: the true (replaced) rho well above the
+            // recurrence's prediction for the same residual means the working
+            // precision can no longer follow the iteration.  The split matrix
+            // hands over to the FP64 matrix; otherwise the problem is stalled
+            // (and continues in FP64 when escalation is on).
+            if (P.rho_pred[k] > 0. && s0[k] > 4. * P.rho_pred[k]) {
+                if (P.ladder && !P.kmode) ladder_up = true;
+                else if (P.stall_iters > 0) stalled = true;
+            }
             // the refinement's FP64 residual decides convergence
             const double rr64 = s0[2 * NR + k];
             if (!(rr64 == rr64)) bad = true;
@@ -674,10 +894,13 @@ __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
         const double rho1 = rho - 2. * alpha * qz + alpha * alpha * qmq;
         P.alpha[k] = alpha;
         P.beta[k] = rho != 0. ? rho1 / rho : 0.;
+        P.rho_pred[k] = rho1;
         P.rr[k] = rr;
         if (rr < P.drop * P.drop * P.rr_ckpt[k]) need = true;   // FP32 residual dropped: refine
         if (rr <= P.rr_target[k]) need = true;                   // may have converged: confirm in FP64
     }
+    if (ladder_up) P.kmode = 1;
+    if (P.kmode) P.iters_k64 += 1;
     if (ck) {
         P.nckpt += 1;
         P.last_ckpt = P.iters;
@@ -941,7 +1164,8 @@ __global__ __launch_bounds__(WG) void k_spmv_plain(SolveArgs a, int mode, const 
         const int s = a.tile_slice0[t] + wave;
         if (mode == 0) spmv_plain_row<double, double, B, NR>(a, a.v64, false, x, yout, s, lane);
         else if (mode == 1) spmv_plain_row<float, float, B, NR>(a, a.vw, true, x, yout, s, lane);
-        else spmv_plain_row<double, Split, B, NR>(a, a.vw, true, x, yout, s, lane);
+        else if (mode == 2) spmv_plain_row<double, Split, B, NR>(a, a.vw, true, x, yout, s, lane);
+        else spmv_plain_row<double, Fix48, B, NR>(a, a.vw, true, x, yout, s, lane);
     }
 }
 

@@ -56,7 +56,7 @@ public:
 
     bool ready() const { return ctx_ != nullptr; }
 
-    int precision = AMIE_MIXED_SPLIT;   // AMIE_FP64 for the plain FP64 CG
+    int precision = AMIE_MIXED_FIX48;   // AMIE_FP64 for the plain FP64 CG
     int precond = AMIE_JACOBI;          // AMIE_BLOCK_JACOBI for Inverse2x2Diagonal
     int64_t iterations = 0;             // of the last solve
     double relres = 0.;                 // ||b - K x|| / ||b|| of the last solve (FP64)
