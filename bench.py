@@ -30,7 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_spmv_rve512_split.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_spmv_rve512.md")
 
 
 def measured_traffic(kernel="k_spmv"):
