@@ -346,7 +346,7 @@ __global__ __launch_bounds__(WG) void k_apply_bc(SolveArgs a, double *v64, doubl
         bool any_r = false;
         for (int i = 0; i < B; i++) { rfix[i] = bcmask[row * B + i]; any_r |= rfix[i]; }
         // the row's own blocks only: the zero padding blocks of a SELL slice
-        // (column = the row itself) must stay zero
+        // must stay zero
         const int w = row_len[row];
         const int64_t sp = a.slice_ptr[s] + lane;
         for (int k = 0; k < w; k++) {

@@ -76,7 +76,7 @@ Batch::~Batch()
 
 size_t Batch::bytes() const
 {
-    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &col, &dslot, &row_len, &v64,
+    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &slot_off, &col, &dslot, &row_len, &v64,
                            &v32, &vsp, &vfx, &fix48_bad, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
                            &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
@@ -104,6 +104,7 @@ SolveArgs Batch::args(int mode) const
     a.tile_slice0 = tile_slice0.as<int32_t>();
     a.tile_desc = tile_desc.as<int4>();
     a.slice_desc = slice_desc.as<int4>();
+    a.slot_off = slot_off.as<int32_t>();
     a.ntiles_all = (int32_t)ntiles;
     a.tile_nslices = tile_nslices.as<int32_t>();
     a.tile_prob = tile_prob.as<int32_t>();
@@ -277,6 +278,7 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
     std::vector<int64_t> h_slice_ptr(slices + 1);
     std::vector<int32_t> h_width(slices), h_rows(slices), h_ts0(tiles), h_tns(tiles), h_tprob(tiles);
     std::vector<uint32_t> h_col(slots);
+    std::vector<int32_t> h_off(slots / WAVE, 0);
     std::vector<int64_t> h_dslot(B.nrows_pad);
     std::vector<int32_t> h_rowlen(B.nrows_pad, 0);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -285,6 +287,7 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
         Tmp &T = tmp[p];
         int64_t sl = slot0[p];
         P.ref_slot.assign(P.column_index.size(), -1);
+        P.offset_blocks = P.positions = 0;
         for (int s = 0; s < P.nslices; s++) {
             const int64_t gs = P.slice0 + s;
             const int w = T.width[s];
@@ -307,7 +310,41 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
                         if ((int32_t)c == orow) h_dslot[grow] = slot;
                     }
                 }
-                for (; k < w; k++) h_col[sl + (int64_t)k * WAVE + l] = (uint32_t)grow;
+            }
+            // Per block position k: the most common column - row over the
+            // slice's real blocks (ties: the smallest).  Padding blocks (zero)
+            // take that column too where it lies inside the problem.  When
+            // every lane then has it, the position's offset is stored and the
+            // SpMV gathers without loading its columns.
+            const int64_t prow0 = P.row0, prow1 = P.row0 + (int64_t)P.nslices * WAVE;
+            for (int k = 0; k < w; k++) {
+                int64_t d[WAVE];
+                int nd = 0;
+                for (int l = 0; l < WAVE; l++) {
+                    const int64_t ir = (int64_t)s * WAVE + l;
+                    if (ir < P.nrows && k < (int)P.row_size[T.perm[ir]])
+                        d[nd++] = (int64_t)h_col[sl + (int64_t)k * WAVE + l] - (P.row0 + ir);
+                }
+                std::sort(d, d + nd);
+                int64_t best = 0;
+                int bestn = 0;
+                for (int i = 0; i < nd;) {
+                    int j = i;
+                    while (j < nd && d[j] == d[i]) j++;
+                    if (j - i > bestn) { bestn = j - i; best = d[i]; }
+                    i = j;
+                }
+                bool all = true;
+                for (int l = 0; l < WAVE; l++) {
+                    const int64_t ir = (int64_t)s * WAVE + l, grow = P.row0 + ir, g = grow + best;
+                    uint32_t &c = h_col[sl + (int64_t)k * WAVE + l];
+                    if (!(ir < P.nrows && k < (int)P.row_size[T.perm[ir]]))
+                        c = (uint32_t)(g >= prow0 && g < prow1 ? g : grow);
+                    all &= (int64_t)c == g;
+                }
+                h_off[(sl >> 6) + k] = all ? (int32_t)best : NO_OFFSET;
+                if (all) P.offset_blocks += nd;
+                P.positions++;
             }
             sl += (int64_t)w * WAVE;
         }
@@ -323,6 +360,7 @@ static int build_sell(Batch &B, std::vector<Tmp> &tmp)
     if ((rc = alloc_and_upload(B.slice_width, h_width.data(), h_width.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.slice_rows, h_rows.data(), h_rows.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.col, h_col.data(), h_col.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.slot_off, h_off.data(), h_off.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.dslot, h_dslot.data(), h_dslot.size() * 8))) return rc;
     if ((rc = alloc_and_upload(B.row_len, h_rowlen.data(), h_rowlen.size() * 4))) return rc;
     if ((rc = alloc_and_upload(B.tile_slice0, h_ts0.data(), h_ts0.size() * 4))) return rc;
@@ -681,7 +719,10 @@ static double spmv_bytes(const Batch &B, const ProbHost &P, int mode, bool bj)
     const double b = B.et.b;
     const double nnzb = (double)P.column_index.size();
     const double n = (double)P.nrows * b;
-    return nnzb * (b * b * sm + 4.) + 4. * (P.nrows + 1) + 3. * n * sv * B.nrhs + (bj ? n * b * sv : 0.);
+    // K1 loads no column index for blocks at a position with a shared
+    // offset, but one 4-byte offset per position
+    const double cols = nnzb - (double)P.offset_blocks + (double)P.positions;
+    return nnzb * b * b * sm + 4. * cols + 4. * (P.nrows + 1) + 3. * n * sv * B.nrhs + (bj ? n * b * sv : 0.);
 }
 
 int Batch::solve_begin(const amie_solve_opts &o)

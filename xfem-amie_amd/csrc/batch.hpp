@@ -94,6 +94,8 @@ struct ProbHost {
     std::vector<int64_t> ref_slot;                  // reference block -> global SELL slot
     std::vector<int32_t> iperm;                     // original row -> local internal row
     std::vector<int64_t> elem_ref;                  // element (i,j) -> reference block index
+    int64_t positions = 0;                          // block positions of the slices (one per 64 slots)
+    int64_t offset_blocks = 0;                      // real blocks at positions with a shared column offset
     bool has_elements = false;
 };
 
@@ -113,6 +115,7 @@ struct Batch {
     // SELL structure
     DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
     DevBuf tile_desc, slice_desc;     // packed {slice0, nslices, prob} / {slot ptr lo, hi, width, rows}
+    DevBuf slot_off;                  // per 64 slots: the most common column - row (the SpMV's gather guess)
     DevBuf v64, v32, vsp, vfx;   // FP64 unscaled K; FP32, split FP32+bf16 and 48-bit fixed-point scaled Ks
     DevBuf fix48_bad;            // int flag: a scaled value outside the fixed-point range
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active, active_prob, n_active_prob;

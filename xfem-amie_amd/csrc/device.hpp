@@ -7,7 +7,9 @@
 // stored contiguously over the 64 lanes, so each load instruction of a
 // wavefront reads one contiguous run (a 2x2 FP32 block is one 16-byte
 // float4 per lane: 1 KiB per wavefront).  Rows shorter than the slice width
-// are padded with zero blocks whose column is the row itself.  Block entries
+// are padded with zero blocks, whose column is the row plus the column offset
+// most common at that block position (the row itself where that falls
+// outside the problem).  Block entries
 // keep the reference's column-major order (entry (r,c) at c*b + r), without
 // the reference's 4x3 padding of 3x3 blocks.  Blocks are split into 16-byte
 // "planes" so every plane load is a dwordx4 (or dword/dwordx2 remainder):
@@ -105,6 +107,7 @@ struct SolveArgs {
     const int32_t *tile_slice0, *tile_nslices, *tile_prob;
     const int4 *tile_desc;   // per tile {first slice, slices, problem, 0}
     const int4 *slice_desc;  // per slice {slot pointer lo, hi, width, rows}
+    const int32_t *slot_off; // per 64 slots (one block position of a slice): column - row shared by all lanes, or NO_OFFSET
     int32_t ntiles_all;
     const int32_t *active;   // active tiles
     const int32_t *n_active;
@@ -695,20 +698,39 @@ __device__ inline int my_tile(const SolveArgs &a)
 }
 
 // Row loop of one slice from its packed descriptor (same arithmetic as
-// spmv_row).
+// spmv_row).  slot_off holds, per block position of the slice, the column
+// offset that every one of its 64 lanes has (column = row + offset), or
+// NO_OFFSET.  A group of positions that all have one gathers at once,
+// without loading the columns; any other group loads its columns first.  On
+// structured meshes most positions have an offset, which saves the column
+// bytes and takes the column load off the gather's critical path.
+constexpr int32_t NO_OFFSET = INT32_MIN;
 template <typename A, typename M, int B, int NR, typename G>
-__device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t sp0, int w, int lane, A (*acc)[B],
-                                  G gather)
+__device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t sp0, int w, int lane, int64_t row,
+                                  A (*acc)[B], G gather)
 {
     constexpr int U = Unroll<M, B>::U;
+    typedef BlkRaw<M, B> BR;
     const int64_t sp = sp0 + lane;
+    const int32_t *offs = a.slot_off + (sp0 >> 6);
     for (int k = 0; k < w; k += U) {
-        typedef BlkRaw<M, B> BR;
+        int32_t o[U];
+        bool direct = true;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            o[u] = k + u < w ? offs[k + u] : 0;
+            direct &= o[u] != NO_OFFSET;
+        }
         uint32_t c[U];
         typename BR::Raw blk[U];
+        if (direct) {
 #pragma unroll
-        for (int u = 0; u < U; u++)
-            if (k + u < w) c[u] = __builtin_nontemporal_load(a.col + sp + (int64_t)(k + u) * WAVE);
+            for (int u = 0; u < U; u++) c[u] = (uint32_t)(row + o[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (k + u < w) c[u] = __builtin_nontemporal_load(a.col + sp + (int64_t)(k + u) * WAVE);
+        }
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (k + u < w) BR::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
@@ -784,7 +806,7 @@ __device__ inline void spmv_tile(const SolveArgs &a, const void *vals, int t, co
             }
         };
         const int64_t sp0 = (int64_t)(uint32_t)sd.x | ((int64_t)sd.y << 32);
-        spmv_slice<A, M, B, NR>(a, vals, sp0, sd.z, lane, acc, gather);
+        spmv_slice<A, M, B, NR>(a, vals, sp0, sd.z, lane, row, acc, gather);
         if (own) {
             if (!AMIE_EARLY_OWN) load_own();
             W mi[B * B];
