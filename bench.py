@@ -128,12 +128,14 @@ def single_problem_legs(ag, configs, ctx, rtol):
            "dofs": int(p2.ndof)}
     for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
         B.assemble()
+        tuned = B.autotune(prec, ag.JACOBI)
         t = time.perf_counter()
         st = timed_solve(B, prec, ag.JACOBI)
         dt = time.perf_counter() - t
         leg[name] = {"solve_ms": 1e3 * dt, "iterations": st["iterations_max"], "status": st["status"],
                      "spmv_gbs": st["spmv_bytes"] / (st["spmv_kernel_ms"] * 1e-3) / 1e9,
-                     "spmv_avg_us": 1e3 * st["spmv_kernel_ms"] / max(1, st["spmv_launches"])}
+                     "spmv_avg_us": 1e3 * st["spmv_kernel_ms"] / max(1, st["spmv_launches"]),
+                     "spmv_kernel": tuned}
         us, by = B.time_spmv(prec, ag.JACOBI, 50)
         leg[name]["spmv_gbs_alone"] = by / (us * 1e-6) / 1e9
     leg["mixed_vs_fp64"] = leg["fp64"]["solve_ms"] / leg["mixed"]["solve_ms"]
@@ -146,6 +148,7 @@ def single_problem_legs(ag, configs, ctx, rtol):
            "dofs": int(p3.ndof)}
     for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
         B = ag.Batch(ctx, [p3])
+        tuned = B.autotune(prec, ag.JACOBI)
         t = time.perf_counter()
         its, ms, by, n = 0, 0.0, 0.0, 0
         for s in range(1, 11):
@@ -161,7 +164,7 @@ def single_problem_legs(ag, configs, ctx, rtol):
         dt = time.perf_counter() - t
         d, _ = B.damage()
         leg[name] = {"ten_steps_ms": 1e3 * dt, "iterations": its, "spmv_gbs": by / (ms * 1e-3) / 1e9,
-                     "spmv_avg_us": 1e3 * ms / max(1, n), "damage_mean": float(d.mean())}
+                     "spmv_avg_us": 1e3 * ms / max(1, n), "damage_mean": float(d.mean()), "spmv_kernel": tuned}
         B.close()
     leg["mixed_vs_fp64"] = leg["fp64"]["ten_steps_ms"] / leg["mixed"]["ten_steps_ms"]
     out["tet_64_damage"] = leg
@@ -185,6 +188,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the serving leg")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the P2 400^2 and Tet4 64^3 legs")
+    ap.add_argument("--spmv-kernel", default="", help="STEPS,ORDER: set the batch's FIX48 SpMV candidate instead "
+                                                      "of autotuning (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -203,6 +208,15 @@ def main():
     log(f"batch resident in HBM: {B.nbytes / 2**30:.2f} GiB, setup {time.perf_counter() - t0:.1f}s")
     nnzb = sum(B.shape(k)[1] for k in range(len(problems)))
     hbm_bytes = B.nbytes
+    # set-up: the autotuner picks the SpMV kernel of each precision once
+    if args.spmv_kernel:
+        steps_, order_ = (int(v) for v in args.spmv_kernel.split(","))
+        B.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, steps_, order_)
+        tuned = {"fix48": {"steps": steps_, "tile_order": order_, "us": None}}
+    else:
+        tuned = {name: B.autotune(prec, ag.JACOBI) for name, prec in
+                 (("fix48", ag.MIXED_FIX48), ("fp32", ag.MIXED), ("fp64", ag.FP64))}
+    log(f"autotuned SpMV kernels: {tuned}")
 
     def step(precision, time_kernels=1):
         B.assemble()
@@ -256,6 +270,8 @@ def main():
     traffic = tr["ratio"] * avg_bytes if tr else None
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
+    us_full, by_full = B.time_spmv(headline, ag.JACOBI, 20)
+    log(f"k_spmv with every tile active: {us_full:.1f} us, {by_full / 1e9:.3f} GB, {by_full / us_full / 1e3:.0f} GB/s")
     B.close()
     single = None if args.no_single else single_problem_legs(ag, configs, ctx, args.rtol)
     serve = None
@@ -294,9 +310,11 @@ def main():
                                         if tr else None),
                      "stream_copy_gbs": copy_gbs,
                      "kernel": "k_spmv (block SpMV q = Ks p, 48-bit fixed-point values, fused p.q, q.z, q.Mq)",
+                     "autotuned": tuned,
                      "launches": launches,
                      "avg_launch_us": 1e3 * spmv_ms / max(1, launches),
-                     "avg_bytes_per_launch": avg_bytes},
+                     "avg_bytes_per_launch": avg_bytes,
+                     "full_occupancy": {"us": us_full, "bytes": by_full, "gbs": by_full / us_full / 1e3}},
         "cpu_baseline": cpu,
         "fp64_batched": fp64,
         "fp32_inner_batched": fp32,

@@ -121,6 +121,8 @@ typedef struct {
     double ref_eps;          /* > 0: the reference's absolute stop instead of rtol:
                                 sqrt(r . D^-1 r) <= max(1e-12, ref_eps) with D = diag(K)
                                 (solvers/conjugategradient.cpp:101,218,305) */
+    int32_t autotune;        /* 1: the first solve of a precision / preconditioner on this batch
+                                runs amie_gpu_batch_autotune first (default 0) */
 } amie_solve_opts;
 
 typedef struct {
@@ -237,11 +239,30 @@ int amie_gpu_batch_get_element_blocks(amie_gpu_batch *b, int32_t p, int64_t *slo
 int amie_gpu_batch_spmv(amie_gpu_batch *b, int32_t p, int32_t precision, const double *x, double *y);
 /* Times the iteration's SpMV kernel (q = Ks p with its dot products) over
  * every problem of the batch, `iters` launches after one warm-up: the mean
- * launch time and the algorithmic bytes of one launch (values + column
- * indices + row pointers + p, r read and q written once; padding not
- * counted).  The solver state is left as it was. */
+ * launch time and the algorithmic bytes of one launch (values + the column
+ * indices the kernel loads + row pointers + p, r read and q written once;
+ * padding not counted).  Uses the autotuned kernel when there is one.  The
+ * solver state is left as it was. */
 int amie_gpu_batch_time_spmv(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t iters,
                              double *us_per_launch, double *bytes_per_launch);
+/* Autotuner of the iteration's SpMV kernel (no reference counterpart: the
+ * reference's SpMV, sparse/sparse_matrix.h:203-678, has one fixed form).
+ * Times each candidate, `iters` launches after one warm-up with every
+ * problem of the batch active, and keeps the fastest for later solves of
+ * this batch with the same precision and preconditioner.  Candidates: half,
+ * equal and twice the default number of blocks per step of the row loop,
+ * each with XCD-contiguous or launch-order tile scheduling.  Results do not
+ * depend on the choice (the sums run in the same order).  Precisions with
+ * candidates: AMIE_FP64, AMIE_MIXED and AMIE_MIXED_FIX48.  Outputs (each
+ * nullable): the chosen blocks per step and tile order (0 XCD-contiguous,
+ * 1 launch order) and its time per launch. */
+int amie_gpu_batch_autotune(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t iters,
+                            int32_t *steps, int32_t *tile_order, double *us_per_launch);
+/* Sets the SpMV candidate directly (a choice kept from an earlier
+ * amie_gpu_batch_autotune, for instance); steps must be one of the
+ * candidates' (AMIE_ERR_ARG otherwise).  No reference counterpart. */
+int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t steps,
+                                   int32_t tile_order);
 /* Element strains (nelem x 3|6, Voigt, engineering shear) of right-hand side
  * rhs, including the macro strain. */
 int amie_gpu_batch_element_strain(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain);

@@ -407,6 +407,43 @@ def test_result_independent_of_grid(ctx):
             np.testing.assert_array_equal(B.solution(k), ref[k])
 
 
+@pytest.mark.parametrize("kind", ["rve2d", "rve3d"])
+@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED, ag.MIXED_FIX48])
+def test_spmv_candidates_give_same_bits(ctx, kind, precision):
+    """Every SpMV candidate of the autotuner (blocks per step x tile order)
+    gives the same solution bits: the row sums run in the same order and the
+    partial sums are per tile.  autotune keeps one of the candidates, and the
+    autotune solve option runs it."""
+    if kind == "rve2d":
+        probs = [configs.rve(n, 900 + n, 0.3) for n in (12, 20, 31)]
+    else:
+        probs = [configs.rve3d(n, 950 + n) for n in (4, 6)]
+    B = ag.Batch(ctx, probs)
+    assert B.solve(precision=precision, rtol=1e-8)["status"] == 0
+    ref = [B.solution(k).copy() for k in range(len(probs))]
+    tuned = B.autotune(precision, ag.JACOBI, iters=3)
+    assert tuned["tile_order"] in (0, 1) and tuned["us"] > 0
+    found = set()
+    for steps in (1, 2, 4, 8, 16):
+        for order in (0, 1):
+            try:
+                B.set_spmv_kernel(precision, ag.JACOBI, steps, order)
+            except ag.AmieError:
+                continue
+            found.add(steps)
+            st = B.solve(precision=precision, rtol=1e-8)
+            assert st["status"] == 0, (steps, order, st)
+            for k in range(len(probs)):
+                np.testing.assert_array_equal(B.solution(k), ref[k])
+    assert len(found) == 3 and tuned["steps"] in found
+    with pytest.raises(ag.AmieError):
+        B.set_spmv_kernel(precision, ag.JACOBI, 3, 0)
+    B2 = ag.Batch(ctx, probs)
+    assert B2.solve(precision=precision, rtol=1e-8, autotune=1)["status"] == 0
+    for k in range(len(probs)):
+        np.testing.assert_array_equal(B2.solution(k), ref[k])
+
+
 DROPIN = os.path.join(os.path.dirname(__file__), "..", "xfem-amie_amd", "host", "dropin_check")
 
 

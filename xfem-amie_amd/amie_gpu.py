@@ -35,7 +35,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", c_i32), ("precond", c_i32), ("refine", c_i32), ("rtol", c_dbl),
                 ("refine_drop", c_dbl), ("ckpt_interval", c_i64), ("maxit", c_i64), ("use_x0", c_i32), ("use_graph", c_i32),
                 ("chunk", c_i32), ("time_kernels", c_i32), ("escalate", c_i32),
-                ("max_grid", c_i32), ("ref_eps", c_dbl)]
+                ("max_grid", c_i32), ("ref_eps", c_dbl), ("autotune", c_i32)]
 
 
 class SolveStats(ctypes.Structure):
@@ -89,6 +89,8 @@ SIGNATURES = {
     "amie_gpu_batch_get_element_blocks": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_spmv": (c_i32, [vp, c_i32, c_i32, vp, vp]),
     "amie_gpu_batch_time_spmv": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp]),
+    "amie_gpu_batch_autotune": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp, vp]),
+    "amie_gpu_batch_set_spmv_kernel": (c_i32, [vp, c_i32, c_i32, c_i32, c_i32]),
     "amie_gpu_batch_element_strain": (c_i32, [vp, c_i32, c_i32, vp]),
     "amie_gpu_batch_average_stress": (c_i32, [vp, vp]),
     "amie_gpu_batch_damage_step": (c_i32, [vp, vp, vp]),
@@ -388,12 +390,26 @@ class Batch:
         check(lib().amie_gpu_batch_spmv(self.h, p, precision, _p(x), _p(y)))
         return y
 
-    def time_spmv(self, precision=MIXED_SPLIT, precond=JACOBI, iters=20):
+    def time_spmv(self, precision=MIXED_FIX48, precond=JACOBI, iters=20):
         """Mean SpMV-kernel launch time (us) and algorithmic bytes per launch
-        with every problem of the batch active."""
+        with every problem of the batch active (the autotuned kernel when
+        there is one)."""
         us, by = c_dbl(), c_dbl()
         check(lib().amie_gpu_batch_time_spmv(self.h, precision, precond, iters, ctypes.byref(us), ctypes.byref(by)))
         return us.value, by.value
+
+    def autotune(self, precision=MIXED_FIX48, precond=JACOBI, iters=5):
+        """Picks the fastest SpMV candidate for this batch: returns
+        {"steps", "tile_order", "us"}; later solves of the same precision and
+        preconditioner use it."""
+        st, order, us = c_i32(), c_i32(), c_dbl()
+        check(lib().amie_gpu_batch_autotune(self.h, precision, precond, iters, ctypes.byref(st), ctypes.byref(order),
+                                            ctypes.byref(us)))
+        return {"steps": st.value, "tile_order": order.value, "us": us.value}
+
+    def set_spmv_kernel(self, precision, precond, steps, tile_order):
+        """Selects an SpMV candidate directly (blocks per step, tile order)."""
+        check(lib().amie_gpu_batch_set_spmv_kernel(self.h, precision, precond, steps, tile_order))
 
     def element_strain(self, p=0, rhs=0):
         pr = self.problems[p]

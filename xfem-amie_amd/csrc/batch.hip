@@ -702,6 +702,50 @@ static Kern select_kern(int mode, int b, int nr, bool bj)
     return b == 2 ? kern_nr<double, double, 2>(nr, bj) : kern_nr<double, double, 3>(nr, bj);
 }
 
+// ------------------------------------------------------------ autotuner
+// Candidate SpMV kernels: which = 0, 1, 2 -> half, equal and twice the
+// default blocks per step.
+typedef void (*KFn)(SolveArgs);
+struct K1Var {
+    KFn k1;
+    int steps;
+};
+
+template <typename W, typename M, int Bk, int NR, bool BJ> static K1Var k1_steps(int which)
+{
+    constexpr int D = Unroll<M, Bk>::U;
+    constexpr int H = D > 1 ? D / 2 : 1;
+    if (which == 0) return {k_spmv<W, M, Bk, NR, BJ, H>, H};
+    if (which == 1) return {k_spmv<W, M, Bk, NR, BJ, D>, D};
+    return {k_spmv<W, M, Bk, NR, BJ, 2 * D>, 2 * D};
+}
+
+template <typename W, typename M, int Bk> static K1Var k1_nr(int nr, bool bj, int which)
+{
+    if (nr == 1) return bj ? k1_steps<W, M, Bk, 1, true>(which) : k1_steps<W, M, Bk, 1, false>(which);
+    return bj ? k1_steps<W, M, Bk, 3, true>(which) : k1_steps<W, M, Bk, 3, false>(which);
+}
+
+// the precisions with candidates: FP64, all-FP32 and the fixed-point matrix
+static K1Var k1_variant(int mode, int b, int nr, bool bj, int which)
+{
+    if (mode == AMIE_MIXED) return b == 2 ? k1_nr<float, float, 2>(nr, bj, which) : k1_nr<float, float, 3>(nr, bj, which);
+    if (mode == AMIE_MIXED_FIX48)
+        return b == 2 ? k1_nr<float, Fix48, 2>(nr, bj, which) : k1_nr<float, Fix48, 3>(nr, bj, which);
+    if (mode == AMIE_FP64) return b == 2 ? k1_nr<double, double, 2>(nr, bj, which) : k1_nr<double, double, 3>(nr, bj, which);
+    return {nullptr, 0};
+}
+
+static int tune_key(int mode, bool bj) { return mode | (bj ? 8 : 0); }
+
+void Batch::apply_tuned(int mode, bool bj, Kern &K, SolveArgs &a) const
+{
+    auto it = k1_tuned.find(tune_key(mode, bj));
+    if (it == k1_tuned.end()) return;
+    K.k1 = it->second.k1;
+    a.tile_order = it->second.tile_order;
+}
+
 static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs &a)
 {
     hipLaunchKernelGGL(k, dim3(std::max(1, g)), dim3(WG), 0, st, a);
@@ -784,8 +828,13 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         set_error("AMIE_MIXED_FIX48: a scaled matrix entry is outside (-2, 2) (K not symmetric positive definite?)");
         return AMIE_ERR_ARG;
     }
+    if (o.autotune && !k1_tuned.count(tune_key(mode, R.bj)) && k1_variant(mode, et.b, nrhs, R.bj, 1).k1) {
+        const int rc = autotune(mode, R.bj, 5);
+        if (rc) return rc;
+    }
     R.K = select_kern(mode, et.b, nrhs, R.bj);
     R.a = args(mode);
+    apply_tuned(mode, R.bj, R.K, R.a);
     const bool fp32vec = mode != AMIE_FP64 && mode != AMIE_MIXED64;
     for (int p = 0; p < nprob; p++) {
         if (!subset[p]) continue;
@@ -1163,6 +1212,7 @@ void amie_gpu_default_opts(amie_solve_opts *o)
     o->maxit = 0;
     o->use_x0 = 0;
     o->use_graph = 1;
+    o->autotune = 0;
     o->chunk = 32;
     o->time_kernels = 0;
     o->escalate = 1;
@@ -1389,6 +1439,61 @@ int amie_gpu_batch_spmv(amie_gpu_batch *bb, int32_t p, int32_t precision, const 
     return download_vec(B, p, B.r, y);
 }
 
+// Mean time of `iters` launches of SpMV kernel k (after one warm-up) with
+// every problem of the batch running; the recorded problem state is put back.
+static int time_k1(Batch &B, KFn k, const SolveArgs &a, int iters, double *us)
+{
+    hipStream_t st = B.stream;
+    std::vector<ProbDev> run(B.h_prob);
+    for (auto &P : run) P.status = ST_RUNNING;
+    int rc = B.set_active_all();
+    if (rc) return rc;
+    AMIE_HIP(hipMemcpyAsync(B.prob.p, run.data(), B.nprob * sizeof(ProbDev), hipMemcpyHostToDevice, st));
+    const int g = (int)B.ntiles;
+    hipEvent_t e0, e1;
+    AMIE_HIP(hipEventCreate(&e0));
+    AMIE_HIP(hipEventCreate(&e1));
+    launch0(k, g, st, a);
+    AMIE_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < iters; i++) launch0(k, g, st, a);
+    AMIE_HIP(hipEventRecord(e1, st));
+    AMIE_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    AMIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    AMIE_HIP(hipMemcpyAsync(B.prob.p, B.h_prob.data(), B.nprob * sizeof(ProbDev), hipMemcpyHostToDevice, st));
+    AMIE_HIP(hipStreamSynchronize(st));
+    *us = 1e3 * ms / iters;
+    return AMIE_OK;
+}
+
+int Batch::autotune(int mode, bool bj, int iters)
+{
+    if (!k1_variant(mode, et.b, nrhs, bj, 1).k1) {
+        set_error("no SpMV candidates for precision %d", mode);
+        return AMIE_ERR_ARG;
+    }
+    if (mode == AMIE_MIXED_FIX48 && !fix48_ok) {
+        set_error("AMIE_MIXED_FIX48: a scaled matrix entry is outside (-2, 2) (K not symmetric positive definite?)");
+        return AMIE_ERR_ARG;
+    }
+    K1Choice best;
+    best.us = DBL_MAX;
+    for (int which = 0; which < 3; which++)
+        for (int order = 0; order < 2; order++) {
+            const K1Var v = k1_variant(mode, et.b, nrhs, bj, which);
+            SolveArgs a = args(mode);
+            a.tile_order = order;
+            double us = 0.;
+            const int rc = time_k1(*this, v.k1, a, iters, &us);
+            if (rc) return rc;
+            if (us < best.us) best = {v.k1, v.steps, order, us};
+        }
+    k1_tuned[tune_key(mode, bj)] = best;
+    return AMIE_OK;
+}
+
 int amie_gpu_batch_time_spmv(amie_gpu_batch *bb, int32_t precision, int32_t precond, int32_t iters,
                              double *us_per_launch, double *bytes_per_launch)
 {
@@ -1399,37 +1504,57 @@ int amie_gpu_batch_time_spmv(amie_gpu_batch *bb, int32_t precision, int32_t prec
     Batch &B = *bb->b;
     if (B.run.started) { set_error("a solve is in progress"); return AMIE_ERR_BUSY; }
     const bool bj = precond == AMIE_BLOCK_JACOBI;
-    const Kern K = select_kern(precision, B.et.b, B.nrhs, bj);
-    const SolveArgs a = B.args(precision);
-    hipStream_t st = B.stream;
-    // every problem running for the measurement, the recorded state after
-    std::vector<ProbDev> run(B.h_prob);
-    for (auto &P : run) P.status = ST_RUNNING;
-    int rc = B.set_active_all();
+    Kern K = select_kern(precision, B.et.b, B.nrhs, bj);
+    SolveArgs a = B.args(precision);
+    B.apply_tuned(precision, bj, K, a);
+    const int rc = time_k1(B, K.k1, a, iters, us_per_launch);
     if (rc) return rc;
-    AMIE_HIP(hipMemcpyAsync(B.prob.p, run.data(), B.nprob * sizeof(ProbDev), hipMemcpyHostToDevice, st));
-    const int g = (int)B.ntiles;
-    hipEvent_t e0, e1;
-    AMIE_HIP(hipEventCreate(&e0));
-    AMIE_HIP(hipEventCreate(&e1));
-    launch0(K.k1, g, st, a);
-    AMIE_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < iters; i++) launch0(K.k1, g, st, a);
-    AMIE_HIP(hipEventRecord(e1, st));
-    AMIE_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    AMIE_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    AMIE_HIP(hipMemcpyAsync(B.prob.p, B.h_prob.data(), B.nprob * sizeof(ProbDev), hipMemcpyHostToDevice, st));
-    AMIE_HIP(hipStreamSynchronize(st));
-    *us_per_launch = 1e3 * ms / iters;
     if (bytes_per_launch) {
         double s = 0.;
         for (int p = 0; p < B.nprob; p++) s += spmv_bytes(B, B.ph[p], precision, bj);
         *bytes_per_launch = s;
     }
     return AMIE_OK;
+}
+
+int amie_gpu_batch_autotune(amie_gpu_batch *bb, int32_t precision, int32_t precond, int32_t iters, int32_t *steps,
+                            int32_t *tile_order, double *us_per_launch)
+{
+    if (!bb || !bb->b || iters <= 0) {
+        set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    Batch &B = *bb->b;
+    if (B.run.started) { set_error("a solve is in progress"); return AMIE_ERR_BUSY; }
+    const bool bj = precond == AMIE_BLOCK_JACOBI;
+    const int rc = B.autotune(precision, bj, iters);
+    if (rc) return rc;
+    const K1Choice &c = B.k1_tuned[tune_key(precision, bj)];
+    if (steps) *steps = c.steps;
+    if (tile_order) *tile_order = c.tile_order;
+    if (us_per_launch) *us_per_launch = c.us;
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *bb, int32_t precision, int32_t precond, int32_t steps,
+                                   int32_t tile_order)
+{
+    if (!bb || !bb->b || (tile_order != 0 && tile_order != 1)) {
+        set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    Batch &B = *bb->b;
+    if (B.run.started) { set_error("a solve is in progress"); return AMIE_ERR_BUSY; }
+    const bool bj = precond == AMIE_BLOCK_JACOBI;
+    for (int which = 0; which < 3; which++) {
+        const K1Var v = k1_variant(precision, B.et.b, B.nrhs, bj, which);
+        if (v.k1 && v.steps == steps) {
+            B.k1_tuned[tune_key(precision, bj)] = {v.k1, v.steps, tile_order, 0.};
+            return AMIE_OK;
+        }
+    }
+    set_error("no SpMV candidate with %d blocks per step for precision %d", steps, precision);
+    return AMIE_ERR_ARG;
 }
 
 static StrainArgs strain_args(Batch &B)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,14 @@ struct Kern {
 };
 
 // State of one asynchronous solve (begin / advance / poll / end).
+// The SpMV kernel the autotuner picked for one precision / preconditioner.
+struct K1Choice {
+    void (*k1)(SolveArgs) = nullptr;
+    int steps = 0;           // blocks per step of the row loop
+    int tile_order = 0;      // SolveArgs::tile_order
+    double us = 0.;          // its time per launch with every tile active
+};
+
 struct SolveRun {
     amie_solve_opts o{};
     int mode = 0;            // precision of the current pass
@@ -111,11 +120,12 @@ struct Batch {
     bool has_elements = false, has_macro = false, has_damage_state = false;
     bool fix48_ok = false;            // the last assembly's scaled values fit the fixed-point range
     std::vector<int32_t> colour_ptr;  // colour c elements: colour_elems[colour_ptr[c]..colour_ptr[c+1])
+    std::map<int, K1Choice> k1_tuned; // by precision | (block Jacobi ? 8 : 0)
 
     // SELL structure
     DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
     DevBuf tile_desc, slice_desc;     // packed {slice0, nslices, prob} / {slot ptr lo, hi, width, rows}
-    DevBuf slot_off;                  // per 64 slots: the most common column - row (the SpMV's gather guess)
+    DevBuf slot_off;                  // per 64 slots: column - row shared by the position's 64 lanes, or NO_OFFSET
     DevBuf v64, v32, vsp, vfx;   // FP64 unscaled K; FP32, split FP32+bf16 and 48-bit fixed-point scaled Ks
     DevBuf fix48_bad;            // int flag: a scaled value outside the fixed-point range
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active, active_prob, n_active_prob;
@@ -152,6 +162,8 @@ struct Batch {
     int solve_end(amie_solve_stats *st);
     int set_active_all();
     int set_active(const std::vector<char> &probs);
+    int autotune(int mode, bool bj, int iters);
+    void apply_tuned(int mode, bool bj, Kern &K, SolveArgs &a) const;
     int64_t vplane() const { return nrows_pad * et.b; }
 };
 

@@ -109,6 +109,7 @@ struct SolveArgs {
     const int4 *slice_desc;  // per slice {slot pointer lo, hi, width, rows}
     const int32_t *slot_off; // per 64 slots (one block position of a slice): column - row shared by all lanes, or NO_OFFSET
     int32_t ntiles_all;
+    int32_t tile_order;      // 0: XCD-contiguous tile ranges (xcd_index), 1: launch order
     const int32_t *active;   // active tiles
     const int32_t *n_active;
     const int32_t *active_prob;
@@ -688,10 +689,11 @@ __device__ inline void spmv_row(const SolveArgs &a, const void *vals, int s, int
 }
 
 // The tile of a one-workgroup-per-tile launch, or -1.  With every tile
-// active the list lookup is skipped.
+// active the list lookup is skipped.  tile_order (picked by the autotuner)
+// switches between the XCD-contiguous order and plain launch order.
 __device__ inline int my_tile(const SolveArgs &a)
 {
-    const int it = xcd_index();
+    const int it = a.tile_order ? (int)blockIdx.x : xcd_index();
     const int na = *a.n_active;
     if (it >= na) return -1;
     return na == a.ntiles_all ? it : a.active[it];
@@ -705,11 +707,10 @@ __device__ inline int my_tile(const SolveArgs &a)
 // structured meshes most positions have an offset, which saves the column
 // bytes and takes the column load off the gather's critical path.
 constexpr int32_t NO_OFFSET = INT32_MIN;
-template <typename A, typename M, int B, int NR, typename G>
+template <typename A, typename M, int B, int NR, int U, typename G>
 __device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t sp0, int w, int lane, int64_t row,
                                   A (*acc)[B], G gather)
 {
-    constexpr int U = Unroll<M, B>::U;
     typedef BlkRaw<M, B> BR;
     const int64_t sp = sp0 + lane;
     const int32_t *offs = a.slot_off + (sp0 >> 6);
@@ -760,8 +761,9 @@ __device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t 
 #ifndef AMIE_EARLY_OWN
 #define AMIE_EARLY_OWN 0
 #endif
-// Body of K1 for one tile with matrix kind M (values at `vals`).
-template <typename W, typename M, int B, int NR, bool BJ>
+// Body of K1 for one tile with matrix kind M (values at `vals`), U blocks
+// per step.
+template <typename W, typename M, int B, int NR, bool BJ, int U>
 __device__ inline void spmv_tile(const SolveArgs &a, const void *vals, int t, const int4 &td, int wave, int lane)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
@@ -806,7 +808,7 @@ __device__ inline void spmv_tile(const SolveArgs &a, const void *vals, int t, co
             }
         };
         const int64_t sp0 = (int64_t)(uint32_t)sd.x | ((int64_t)sd.y << 32);
-        spmv_slice<A, M, B, NR>(a, vals, sp0, sd.z, lane, row, acc, gather);
+        spmv_slice<A, M, B, NR, U>(a, vals, sp0, sd.z, lane, row, acc, gather);
         if (own) {
             if (!AMIE_EARLY_OWN) load_own();
             W mi[B * B];
@@ -839,7 +841,7 @@ __device__ inline void spmv_tile(const SolveArgs &a, const void *vals, int t, co
 // matrix the unscaled K is used and the scaling applied on the fly:
 // q = S K (S p).  A problem on the split matrix whose reduce kernel found its
 // accuracy floor (kmode) runs the same product from the FP64 matrix.
-template <typename W, typename M, int B, int NR, bool BJ>
+template <typename W, typename M, int B, int NR, bool BJ, int U = Unroll<M, B>::U>
 __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
 {
     const int t = my_tile(a);
@@ -848,8 +850,9 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
     const ProbDev &P = a.prob[td.z];
     if (P.status != ST_RUNNING) return;
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
-    if (std::is_same<M, Split>::value && P.kmode) spmv_tile<W, double, B, NR, BJ>(a, a.v64, t, td, wave, lane);
-    else spmv_tile<W, M, B, NR, BJ>(a, a.vw, t, td, wave, lane);
+    if (std::is_same<M, Split>::value && P.kmode)
+        spmv_tile<W, double, B, NR, BJ, Unroll<double, B>::U>(a, a.v64, t, td, wave, lane);
+    else spmv_tile<W, M, B, NR, BJ, U>(a, a.vw, t, td, wave, lane);
 }
 
 // ------------------------------------------- R: per-problem CG scalars
