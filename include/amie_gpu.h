@@ -251,16 +251,18 @@ int amie_gpu_batch_time_spmv(amie_gpu_batch *b, int32_t precision, int32_t preco
  * problem of the batch active, and keeps the fastest for later solves of
  * this batch with the same precision and preconditioner.  Candidates: half,
  * equal and twice the default number of blocks per step of the row loop,
- * each with XCD-contiguous or launch-order tile scheduling.  Results do not
- * depend on the choice (the sums run in the same order).  Precisions with
- * candidates: AMIE_FP64, AMIE_MIXED and AMIE_MIXED_FIX48.  Outputs (each
- * nullable): the chosen blocks per step and tile order (0 XCD-contiguous,
- * 1 launch order) and its time per launch. */
+ * each with four tile orders: XCD-contiguous ranges (0), or chunks of C
+ * tiles dealt round-robin over the 8 XCDs (C = 1: launch order, 8, 64).
+ * Results do not depend on the choice (the sums run in the same order).
+ * Precisions with candidates: AMIE_FP64, AMIE_MIXED and AMIE_MIXED_FIX48.
+ * Outputs (each nullable): the chosen blocks per step and tile order and its
+ * time per launch. */
 int amie_gpu_batch_autotune(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t iters,
                             int32_t *steps, int32_t *tile_order, double *us_per_launch);
 /* Sets the SpMV candidate directly (a choice kept from an earlier
  * amie_gpu_batch_autotune, for instance); steps must be one of the
- * candidates' (AMIE_ERR_ARG otherwise).  No reference counterpart. */
+ * candidates', tile_order 0 or a chunk of 1..4096 tiles (AMIE_ERR_ARG
+ * otherwise).  No reference counterpart. */
 int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t steps,
                                    int32_t tile_order);
 /* Element strains (nelem x 3|6, Voigt, engineering shear) of right-hand side

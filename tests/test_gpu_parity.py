@@ -422,10 +422,10 @@ def test_spmv_candidates_give_same_bits(ctx, kind, precision):
     assert B.solve(precision=precision, rtol=1e-8)["status"] == 0
     ref = [B.solution(k).copy() for k in range(len(probs))]
     tuned = B.autotune(precision, ag.JACOBI, iters=3)
-    assert tuned["tile_order"] in (0, 1) and tuned["us"] > 0
+    assert tuned["tile_order"] in (0, 1, 8, 64) and tuned["us"] > 0
     found = set()
     for steps in (1, 2, 4, 8, 16):
-        for order in (0, 1):
+        for order in (0, 1, 8, 64):
             try:
                 B.set_spmv_kernel(precision, ag.JACOBI, steps, order)
             except ag.AmieError:
@@ -438,6 +438,8 @@ def test_spmv_candidates_give_same_bits(ctx, kind, precision):
     assert len(found) == 3 and tuned["steps"] in found
     with pytest.raises(ag.AmieError):
         B.set_spmv_kernel(precision, ag.JACOBI, 3, 0)
+    with pytest.raises(ag.AmieError):
+        B.set_spmv_kernel(precision, ag.JACOBI, tuned["steps"], -1)
     B2 = ag.Batch(ctx, probs)
     assert B2.solve(precision=precision, rtol=1e-8, autotune=1)["status"] == 0
     for k in range(len(probs)):

@@ -1480,8 +1480,9 @@ int Batch::autotune(int mode, bool bj, int iters)
     }
     K1Choice best;
     best.us = DBL_MAX;
+    static const int orders[] = {0, 1, 8, 64};
     for (int which = 0; which < 3; which++)
-        for (int order = 0; order < 2; order++) {
+        for (int order : orders) {
             const K1Var v = k1_variant(mode, et.b, nrhs, bj, which);
             SolveArgs a = args(mode);
             a.tile_order = order;
@@ -1539,7 +1540,7 @@ int amie_gpu_batch_autotune(amie_gpu_batch *bb, int32_t precision, int32_t preco
 int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *bb, int32_t precision, int32_t precond, int32_t steps,
                                    int32_t tile_order)
 {
-    if (!bb || !bb->b || (tile_order != 0 && tile_order != 1)) {
+    if (!bb || !bb->b || tile_order < 0 || tile_order > 4096) {
         set_error("bad arguments");
         return AMIE_ERR_ARG;
     }

@@ -109,7 +109,8 @@ struct SolveArgs {
     const int4 *slice_desc;  // per slice {slot pointer lo, hi, width, rows}
     const int32_t *slot_off; // per 64 slots (one block position of a slice): column - row shared by all lanes, or NO_OFFSET
     int32_t ntiles_all;
-    int32_t tile_order;      // 0: XCD-contiguous tile ranges (xcd_index), 1: launch order
+    int32_t tile_order;      // 0: XCD-contiguous tile ranges (xcd_index); C > 0: chunks of C tiles dealt
+                             // round-robin over the XCDs (1: launch order)
     const int32_t *active;   // active tiles
     const int32_t *n_active;
     const int32_t *active_prob;
@@ -688,12 +689,24 @@ __device__ inline void spmv_row(const SolveArgs &a, const void *vals, int s, int
     }
 }
 
+// Chunked order: the XCD of workgroup b (b mod 8) takes chunks of C
+// consecutive tiles, chunk after chunk round-robin over the XCDs; workgroups
+// past the last full round keep launch order.  A bijection of
+// [0, gridDim.x); C = 1 is launch order.
+__device__ inline int chunk_index(int C)
+{
+    const int G = gridDim.x, b = blockIdx.x, span = NXCD * C, M = G / span * span;
+    if (C == 1 || b >= M) return b;
+    const int x = b % NXCD, j = b / NXCD;
+    return (j / C * NXCD + x) * C + j % C;
+}
+
 // The tile of a one-workgroup-per-tile launch, or -1.  With every tile
 // active the list lookup is skipped.  tile_order (picked by the autotuner)
-// switches between the XCD-contiguous order and plain launch order.
+// chooses the XCD-contiguous or a chunked order.
 __device__ inline int my_tile(const SolveArgs &a)
 {
-    const int it = a.tile_order ? (int)blockIdx.x : xcd_index();
+    const int it = a.tile_order ? chunk_index(a.tile_order) : xcd_index();
     const int na = *a.n_active;
     if (it >= na) return -1;
     return na == a.ntiles_all ? it : a.active[it];
