@@ -184,8 +184,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--maxit", type=int, default=0, help="cap on CG iterations (profiling runs only)")
-    ap.add_argument("--serve-rate", type=float, default=10.0, help="request rate of the serving leg (req/s)")
-    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the serving leg")
+    ap.add_argument("--serve-rates", default="10:8,50:8,200:2",
+                    help="serving legs as RATE:SECONDS (Poisson arrivals in req/s over an arrival window)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the P2 400^2 and Tet4 64^3 legs")
     ap.add_argument("--spmv-kernel", default="", help="STEPS,ORDER: set the batch's FIX48 SpMV candidate instead "
@@ -276,13 +276,18 @@ def main():
     single = None if args.no_single else single_problem_legs(ag, configs, ctx, args.rtol)
     serve = None
     if not args.no_serve:
-        reqs = configs.config5(args.serve_rate, args.serve_seconds)
-        kinds = {k: sum(1 for r in reqs if r[1] == k) for k in ("rve2d", "rve3d", "damage")}
-        serve = {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s (seed 5): {kinds}, "
-                             f"rel.res {args.rtol:g}"}
-        for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
-            serve[name] = serving(ag, ctx, reqs, prec, args.rtol)
-            log(f"serving {name}: {serve[name]}")
+        serve = {"workload": f"served mix (seed 5): Poisson arrivals, 1/3 each 2D P1 RVE n~U{{64..512}} (3 load "
+                             f"cases), 3D Tet4 RVE n~U{{16..48}}, damage step of Tet4 32^3 on 4 persistent states; "
+                             f"rel.res {args.rtol:g}; latency = arrival to result", "rates": []}
+        for leg in args.serve_rates.split(","):
+            rate, secs = (float(v) for v in leg.split(":"))
+            reqs = configs.config5(rate, secs)
+            kinds = {k: sum(1 for r in reqs if r[1] == k) for k in ("rve2d", "rve3d", "damage")}
+            row = {"rate": rate, "seconds": secs, "requests": kinds}
+            for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
+                row[name] = serving(ag, ctx, reqs, prec, args.rtol)
+                log(f"serving {rate:g} req/s {name}: {row[name]}")
+            serve["rates"].append(row)
     cpu = None if args.no_cpu else cpu_baseline(problems, args.cpu_budget, args.rtol)
     out = {
         "metric": "RVE solves/s + p50/p99 latency @rel.res 1e-8; mixed vs FP64 CG; SpMV GB/s",

@@ -813,6 +813,41 @@ static void queue_period(Batch &B, SolveRun &R, hipStream_t s)
     (void)B;
 }
 
+// The graph of one checkpoint period (the launches of queue_period, untimed),
+// built node by node rather than by stream capture: capture is a state of the
+// stream that device-wide operations on other host threads (a hipFree's
+// synchronisation, say) can break.
+static int build_period_graph(SolveRun &R, hipGraph_t graph)
+{
+    hipGraphNode_t prev = nullptr;
+    int slot = 0;
+    void *args1[] = {&R.a};
+    void *args2[] = {&R.a, &slot};
+    auto add = [&](const void *fn, int grid, void **args) -> bool {
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void *>(fn);
+        kp.gridDim = dim3(std::max(1, grid));
+        kp.blockDim = dim3(WG);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t node;
+        if (!hip_ok(hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp),
+                    "hipGraphAddKernelNode"))
+            return false;
+        prev = node;
+        return true;
+    };
+    for (int i = 0; i < R.ce; i++) {
+        slot = i == R.ce - 1 ? 1 : 0;   // the parameter value is copied when the node is added
+        if (!add((const void *)R.K.k1, R.gt, args1) || !add((const void *)R.K.red, R.gp, args1) ||
+            !add((const void *)R.K.k2, R.gt, args2))
+            return AMIE_ERR_HIP;
+    }
+    if (!add((const void *)R.K.cx, R.gt, args1) || !add((const void *)R.K.cr, R.gt, args1)) return AMIE_ERR_HIP;
+    return AMIE_OK;
+}
+
 // Starts (or continues with another precision) the solve of the problems in
 // `subset` from their current x64; the other problems keep their state.
 int Batch::start_pass(int mode, const std::vector<char> &subset)
@@ -892,11 +927,11 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
                 break;
             }
         hipGraph_t graph;
-        AMIE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        queue_period(*this, R, s);
-        AMIE_HIP(hipStreamEndCapture(s, &graph));
-        AMIE_HIP(hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0));
-        AMIE_HIP(hipGraphDestroy(graph));
+        AMIE_HIP(hipGraphCreate(&graph, 0));
+        const int rc = build_period_graph(R, graph);
+        if (rc == AMIE_OK) AMIE_HIP(hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        if (rc) return rc;
         graphs.push_back({key, R.exec});
     }
     R.pbytes.assign(nprob, 0.);

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -15,7 +16,7 @@ struct amie_gpu_ctx {
     hipStream_t stream = nullptr;
     int num_cu = 0;
     size_t total_mem = 0;
-    int children = 0;        // live batches and servers
+    std::atomic<int> children{0};   // live batches and servers (created and destroyed on any host thread)
     bool closing = false;    // amie_gpu_destroy called while children live
 };
 

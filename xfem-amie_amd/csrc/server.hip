@@ -2,22 +2,29 @@
 // footprint and device-resident problem state.
 //
 // Requests queue on the host.  Each step() (1) retires in-flight groups whose
-// solve finished (non-blocking event polls), (2) admits queued requests of
-// one class (element type, right-hand-side count, load kind) as a new group
-// while solve slots and the HBM budget allow, builds that group's batch
-// (pattern, layout, assembly on the device) and starts its solve on its own
-// stream, and (3) enqueues one more chunk of iterations for every in-flight
-// group.  Groups admitted at different times therefore run concurrently on
-// the device, each problem with its own scalars and reductions: a problem's
-// result does not depend on which other problems share its group.
+// solve finished (non-blocking event polls), (2) starts the solve of every
+// group whose batch has been built, on its own stream, (3) admits queued
+// requests of one class (element type, right-hand-side count, load kind) as
+// a new group while solve slots and the HBM budget allow, and hands the
+// build of its batch (pattern, layout, assembly on the device) to a host
+// worker thread, and (4) enqueues one more chunk of iterations for every
+// in-flight group.  Builds and the release of finished batches (hipFree
+// synchronises the device) run off the serving thread, so a step never
+// waits for them and in-flight groups keep their chunks coming.  Groups
+// admitted at different times run concurrently on the device, each problem
+// with its own scalars and reductions: a problem's result does not depend
+// on which other problems share its group.
 // Requests with a persistent key keep their batch (matrix, damage state and
 // solution) resident between requests: such a request runs one damage step
 // on the resident state and a solve.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <deque>
+#include <future>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "batch.hpp"
@@ -86,6 +93,23 @@ struct Group {
     size_t bytes = 0;
 };
 
+// A batch built on a worker thread.
+struct Built {
+    Batch *batch = nullptr;
+    int rc = AMIE_OK;
+    std::string err;
+};
+
+struct Building {
+    std::unique_ptr<Group> g;
+    std::future<Built> fut;
+};
+
+template <typename T> static bool ready(std::future<T> &f)
+{
+    return f.wait_for(std::chrono::seconds(0)) == std::future_status::ready;
+}
+
 struct Server {
     amie_gpu_ctx *ctx = nullptr;
     size_t budget = 0, in_use = 0;
@@ -94,6 +118,8 @@ struct Server {
     uint64_t next_ticket = 1;
     std::deque<Request> queue;
     std::vector<std::unique_ptr<Group>> inflight;
+    std::vector<Building> building;              // batches being built on worker threads
+    std::vector<std::future<void>> releasing;    // finished batches being freed on worker threads
     std::map<uint64_t, std::unique_ptr<Group>> resident;   // persistent key -> idle resident batch
     std::vector<amie_request_result> done;
     std::map<uint64_t, std::vector<double>> solutions;
@@ -102,6 +128,9 @@ struct Server {
 
     ~Server()
     {
+        for (auto &b : building) delete b.fut.get().batch;
+        building.clear();
+        releasing.clear();   // each future waits for its release
         for (auto &g : inflight)
             if (g->batch) (void)hipStreamSynchronize(g->batch->stream);
         inflight.clear();
@@ -160,9 +189,55 @@ struct Server {
             resident[key] = std::move(g);   // stays in HBM, still counted in in_use
         } else {
             in_use -= g->bytes;
-            g.reset();
+            const int dev = ctx->device;
+            Group *gp = g.release();
+            releasing.push_back(std::async(std::launch::async, [gp, dev]() {
+                (void)hipSetDevice(dev);
+                delete gp;
+            }));
         }
         return AMIE_OK;
+    }
+
+    // (2) groups whose batch is built start their solve
+    int start_built(double now)
+    {
+        for (size_t i = 0; i < building.size();) {
+            if (!ready(building[i].fut)) {
+                i++;
+                continue;
+            }
+            Built b = building[i].fut.get();
+            std::unique_ptr<Group> g = std::move(building[i].g);
+            building.erase(building.begin() + i);
+            if (b.rc) {
+                in_use -= g->bytes;
+                used_slots -= (int)g->reqs.size();
+                set_error("%s", b.err.c_str());
+                return b.rc;
+            }
+            g->batch.reset(b.batch);
+            b.batch->stream = pick_stream();
+            in_use -= g->bytes;   // the estimate made at admission
+            g->bytes = b.batch->bytes();
+            in_use += g->bytes;
+            if (int rc = start_group(std::move(g), now)) return rc;
+        }
+        for (size_t i = 0; i < releasing.size();) {
+            if (ready(releasing[i])) releasing.erase(releasing.begin() + i);
+            else i++;
+        }
+        return AMIE_OK;
+    }
+
+    bool key_busy(uint64_t key) const
+    {
+        if (resident.count(key)) return true;
+        for (auto &g : inflight)
+            if (g->persistent && !g->reqs.empty() && g->reqs[0].key == key) return true;
+        for (auto &b : building)
+            if (b.g->persistent && !b.g->reqs.empty() && b.g->reqs[0].key == key) return true;
+        return false;
     }
 
     int step(double now, int *n_active, int *n_queued)
@@ -179,7 +254,8 @@ struct Server {
                 i++;
             }
         }
-        // (2) admit: persistent requests whose key is resident and idle, then
+        if ((rc = start_built(now))) return rc;
+        // (3) admit: persistent requests whose key is resident and idle, then
         // one group of the class at the head of the queue
         for (size_t i = 0; i < queue.size();) {
             Request &r = queue[i];
@@ -206,10 +282,7 @@ struct Server {
             size_t bytes = 0;
             for (size_t i = 0; i < queue.size() && used_slots + (int)pick.size() < max_slots;) {
                 Request &r = queue[i];
-                const bool resident_key = r.key && (resident.count(r.key) ||
-                                                    std::any_of(inflight.begin(), inflight.end(), [&](auto &g) {
-                                                        return g->persistent && !g->reqs.empty() && g->reqs[0].key == r.key;
-                                                    }));
+                const bool resident_key = r.key && key_busy(r.key);
                 if (r.prob.cls() != cls || (r.key != 0) != pers || resident_key) { i++; continue; }
                 if (in_use + bytes + (size_t)r.prob.footprint > budget) break;
                 bytes += (size_t)r.prob.footprint;
@@ -220,20 +293,32 @@ struct Server {
             if (!pick.empty()) {
                 std::unique_ptr<Group> g(new Group());
                 g->persistent = pers;
-                std::vector<amie_problem_desc> descs;
-                for (auto &r : pick) descs.push_back(r.prob.d);
-                Batch *B = nullptr;
-                if ((rc = batch_create(ctx, (int)descs.size(), descs.data(), &B))) return rc;
-                g->batch.reset(B);
-                B->stream = pick_stream();
-                g->bytes = B->bytes();
-                in_use += g->bytes;
+                g->bytes = bytes;
+                in_use += bytes;
                 used_slots += (int)pick.size();
                 g->reqs = std::move(pick);
-                if ((rc = start_group(std::move(g), now))) return rc;
+                // the descriptions point into the requests' own arrays, which
+                // stay in place while the group waits for its batch
+                std::vector<amie_problem_desc> descs;
+                for (auto &r : g->reqs) descs.push_back(r.prob.d);
+                amie_gpu_ctx *cx = ctx;
+                Building b;
+                b.fut = std::async(std::launch::async, [cx, descs]() {
+                    Built o;
+                    if (hipSetDevice(cx->device) != hipSuccess) {
+                        o.rc = AMIE_ERR_HIP;
+                        o.err = "hipSetDevice failed on the build thread";
+                        return o;
+                    }
+                    o.rc = batch_create(cx, (int)descs.size(), descs.data(), &o.batch);
+                    if (o.rc) o.err = amie_gpu_last_error();
+                    return o;
+                });
+                b.g = std::move(g);
+                building.push_back(std::move(b));
             }
         }
-        // (3) one more chunk for every in-flight group
+        // (4) one more chunk for every in-flight group
         for (auto &g : inflight)
             if ((rc = g->batch->solve_advance())) return rc;
         if (n_active) *n_active = used_slots;
