@@ -380,7 +380,7 @@ static void finish_tmp(ProbHost &P, Tmp &T)
 {
     T.row_start.assign(P.nrows + 1, 0);
     for (int64_t r = 0; r < P.nrows; r++) T.row_start[r + 1] = T.row_start[r] + P.row_size[r];
-    choose_permutation(P.row_size, T.perm);
+    choose_permutation(P.row_size, T.row_start, P.column_index, T.perm);
     P.iperm.assign(P.nrows, 0);
     for (int64_t i = 0; i < P.nrows; i++) P.iperm[T.perm[i]] = (int32_t)i;
     const int64_t ns = (P.nrows + WAVE - 1) / WAVE;

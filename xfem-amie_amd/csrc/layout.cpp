@@ -52,11 +52,18 @@ void build_pattern(int nper, int64_t nelem, const int32_t *edof, int64_t nrows, 
     }
 }
 
-void choose_permutation(const std::vector<uint32_t> &row_size, std::vector<int32_t> &perm)
+void choose_permutation(const std::vector<uint32_t> &row_size, const std::vector<int64_t> &row_start,
+                        const std::vector<uint32_t> &column_index, std::vector<int32_t> &perm)
 {
     // Rows keep their order unless sorting by length inside windows of 16
     // slices saves more than 5 % of the padded slots (P2 meshes, whose vertex
-    // and mid-edge rows alternate, are the case in point).
+    // and mid-edge rows alternate, are the case in point).  Within a length,
+    // rows with the same stencil (the same column - row offsets) are grouped,
+    // in order of first appearance, when a window holds at most 32 stencils
+    // (structured meshes): consecutive rows of a group then have neighbours
+    // at one fixed offset, which the SpMV's column offsets turn into gathers
+    // without column loads.  Otherwise (unstructured meshes) rows of a
+    // length keep their order.
     const int64_t n = (int64_t)row_size.size();
     perm.resize(n);
     std::iota(perm.begin(), perm.end(), 0);
@@ -70,11 +77,36 @@ void choose_permutation(const std::vector<uint32_t> &row_size, std::vector<int32
         return tot;
     };
     int64_t base = padded(perm);
+    // stencil signature: FNV-1a over the row's column - row offsets
+    std::vector<uint64_t> sig(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < n; r++) {
+        uint64_t h = 1469598103934665603ull;
+        for (int64_t q = row_start[r]; q < row_start[r + 1]; q++) {
+            const uint64_t d = (uint64_t)((int64_t)column_index[q] - r);
+            for (int byte = 0; byte < 8; byte++) h = (h ^ ((d >> (8 * byte)) & 0xFF)) * 1099511628211ull;
+        }
+        sig[r] = h;
+    }
     std::vector<int32_t> sorted(perm);
     const int64_t win = 64 * 16;
     for (int64_t s = 0; s < n; s += win) {
-        auto b = sorted.begin() + s, e = sorted.begin() + std::min(n, s + win);
-        std::stable_sort(b, e, [&](int32_t x, int32_t y) { return row_size[x] > row_size[y]; });
+        const int64_t e = std::min(n, s + win);
+        std::vector<uint64_t> seen;
+        std::vector<int32_t> group(e - s);
+        for (int64_t r = s; r < e; r++) {
+            auto it = std::find(seen.begin(), seen.end(), sig[r]);
+            if (it == seen.end() && seen.size() <= 32) {
+                seen.push_back(sig[r]);
+                it = seen.end() - 1;
+            }
+            group[r - s] = it == seen.end() ? 0 : (int32_t)(it - seen.begin());
+        }
+        const bool by_stencil = seen.size() <= 32;
+        std::stable_sort(sorted.begin() + s, sorted.begin() + e, [&](int32_t x, int32_t y) {
+            if (row_size[x] != row_size[y]) return row_size[x] > row_size[y];
+            return by_stencil && group[x - s] < group[y - s];
+        });
     }
     if (padded(sorted) < base * 0.95) perm.swap(sorted);
 }
