@@ -111,7 +111,7 @@ def serving(ag, ctx, reqs, precision, rtol, limit_s=90.0):
             "solves_per_s": solves / elapsed, "wall_s": elapsed}
 
 
-def single_problem_legs(ag, configs, ctx, rtol):
+def single_problem_legs(ag, configs, ctx, rtol, maxit=0):
     """The single-problem configurations beside the batch: P2 400^2 composite
     (1.28 M DOFs) solved once, and the Tet4 64^3 damage case (824 k DOFs)
     through 10 load steps (solve + damage update each), FIX48 mixed
@@ -119,7 +119,7 @@ def single_problem_legs(ag, configs, ctx, rtol):
     out = {}
 
     def timed_solve(B, precision, precond, **kw):
-        st = B.solve(precision=precision, precond=precond, rtol=rtol, use_graph=0, time_kernels=1, **kw)
+        st = B.solve(precision=precision, precond=precond, rtol=rtol, use_graph=0, time_kernels=1, maxit=maxit, **kw)
         return st
 
     p2 = configs.config2(400)
@@ -138,6 +138,7 @@ def single_problem_legs(ag, configs, ctx, rtol):
                      "spmv_kernel": tuned}
         us, by = B.time_spmv(prec, ag.JACOBI, 50)
         leg[name]["spmv_gbs_alone"] = by / (us * 1e-6) / 1e9
+        leg[name]["spmv_bytes_per_launch"] = by
     leg["mixed_vs_fp64"] = leg["fp64"]["solve_ms"] / leg["mixed"]["solve_ms"]
     out["p2_400"] = leg
     log(f"P2 400^2: {leg}")
@@ -149,6 +150,7 @@ def single_problem_legs(ag, configs, ctx, rtol):
     for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
         B = ag.Batch(ctx, [p3])
         tuned = B.autotune(prec, ag.JACOBI)
+        us_alone, by_alone = B.time_spmv(prec, ag.JACOBI, 50)
         t = time.perf_counter()
         its, ms, by, n = 0, 0.0, 0.0, 0
         for s in range(1, 11):
@@ -164,7 +166,8 @@ def single_problem_legs(ag, configs, ctx, rtol):
         dt = time.perf_counter() - t
         d, _ = B.damage()
         leg[name] = {"ten_steps_ms": 1e3 * dt, "iterations": its, "spmv_gbs": by / (ms * 1e-3) / 1e9,
-                     "spmv_avg_us": 1e3 * ms / max(1, n), "damage_mean": float(d.mean()), "spmv_kernel": tuned}
+                     "spmv_avg_us": 1e3 * ms / max(1, n), "damage_mean": float(d.mean()), "spmv_kernel": tuned,
+                     "spmv_gbs_alone": by_alone / us_alone / 1e3, "spmv_bytes_per_launch": by_alone}
         B.close()
     leg["mixed_vs_fp64"] = leg["fp64"]["ten_steps_ms"] / leg["mixed"]["ten_steps_ms"]
     out["tet_64_damage"] = leg
@@ -273,7 +276,7 @@ def main():
     us_full, by_full = B.time_spmv(headline, ag.JACOBI, 20)
     log(f"k_spmv with every tile active: {us_full:.1f} us, {by_full / 1e9:.3f} GB, {by_full / us_full / 1e3:.0f} GB/s")
     B.close()
-    single = None if args.no_single else single_problem_legs(ag, configs, ctx, args.rtol)
+    single = None if args.no_single else single_problem_legs(ag, configs, ctx, args.rtol, args.maxit)
     serve = None
     if not args.no_serve:
         serve = {"workload": f"served mix (seed 5): Poisson arrivals, 1/3 each 2D P1 RVE n~U{{64..512}} (3 load "
