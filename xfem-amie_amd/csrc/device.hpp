@@ -560,6 +560,58 @@ template <int NQ> __device__ inline void problem_sums(const double *part, const 
     wg_sum_all<NQ>(out, lds);
 }
 
+// problem_sums for two partial arrays at once (NA values from pa, NB from
+// pb), with the loads of four tiles in flight per thread: the same additions
+// in the same order as problem_sums, one round of barriers instead of two.
+template <int NA, int NB>
+__device__ inline void problem_sums2(const double *pa, const double *pb, const ProbDev &P, double *oa, double *ob,
+                                     double (*lds)[2 * MAXQ])
+{
+    constexpr int NQ = NA + NB;
+    double out[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) out[q] = 0.;
+    const int nt = P.ntiles;
+    const size_t t0 = (size_t)P.tile0 * MAXQ;
+    int t = threadIdx.x;
+    for (; t + 3 * WG < nt; t += 4 * WG) {
+        double v[4][NQ];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const size_t o = t0 + (size_t)(t + u * WG) * MAXQ;
+#pragma unroll
+            for (int q = 0; q < NA; q++) v[u][q] = pa[o + q];
+#pragma unroll
+            for (int q = 0; q < NB; q++) v[u][NA + q] = pb[o + q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) out[q] += v[u][q];
+    }
+    for (; t < nt; t += WG) {
+        const size_t o = t0 + (size_t)t * MAXQ;
+#pragma unroll
+        for (int q = 0; q < NA; q++) out[q] += pa[o + q];
+#pragma unroll
+        for (int q = 0; q < NB; q++) out[NA + q] += pb[o + q];
+    }
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        double sw = wave_sum(out[q]);
+        if (lane == 0) lds[wave][q] = sw;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) out[q] = (lds[0][q] + lds[1][q]) + (lds[2][q] + lds[3][q]);
+#pragma unroll
+    for (int q = 0; q < NA; q++) oa[q] = out[q];
+#pragma unroll
+    for (int q = 0; q < NB; q++) ob[q] = out[NA + q];
+}
+
 // ------------------------------------------------------- tile scheduling
 // Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).  In a
 // one-workgroup-per-tile launch each XCD takes a contiguous range of the
@@ -874,15 +926,14 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
 template <int NR>
 __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
 {
-    __shared__ double lds[TILE_SLICES][MAXQ];
+    __shared__ double lds2[TILE_SLICES][2 * MAXQ];
     if ((int)blockIdx.x >= *a.n_active_prob) return;
     ProbDev &P = a.prob[a.active_prob[blockIdx.x]];
     if (P.status != ST_RUNNING) return;   // read by every thread before thread 0 writes (barriers below)
     const bool ck = P.src_ckpt != 0;
     double s1[3 * NR], s0[4 * NR];
-    problem_sums<3 * NR>(a.part1, P, s1, lds);
-    if (ck) problem_sums<4 * NR>(a.partc, P, s0, lds);
-    else problem_sums<2 * NR>(a.part2, P, s0, lds);
+    if (ck) problem_sums2<3 * NR, 4 * NR>(a.part1, a.partc, P, s1, s0, lds2);
+    else problem_sums2<3 * NR, 2 * NR>(a.part1, a.part2, P, s1, s0, lds2);
     if (threadIdx.x != 0) return;
     bool all = true, bad = false, stalled = false, need = false, ladder_up = false;
     for (int k = 0; k < NR; k++) {
