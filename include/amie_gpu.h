@@ -111,7 +111,8 @@ typedef struct {
     double rtol;             /* stop on ||b - K x|| / ||b|| <= rtol (FP64 residual) */
     double refine_drop;      /* FP32 residual drop that triggers an FP64 refinement (mixed) */
     int64_t ckpt_interval;   /* mixed: FP64 refinement at least every this many iterations (<= 0: 128) */
-    int64_t maxit;           /* per right-hand side, <= 0: ndof */
+    int64_t maxit;           /* per right-hand side, <= 0: max(2 ndof, 1000) (a mixed-precision solve of a
+                                tiny system needs a few refinement cycles beyond ndof iterations) */
     int32_t use_x0;          /* 1: start from the current device solution */
     int32_t use_graph;       /* 1: replay captured hipGraphs of iteration chunks */
     int32_t chunk;           /* iterations per chunk between host status checks */

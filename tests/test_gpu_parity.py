@@ -313,6 +313,29 @@ def test_graph_replay_matches_launches(ctx, precision):
     np.testing.assert_array_equal(x1, x3)
 
 
+@pytest.mark.parametrize("n", [1, 3, 16])
+@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED_FIX48])
+def test_stiffness_behaviour_fixture(ctx, n, precision):
+    """The reference's elastic behaviour test (examples/test/test_stiffness.ini,
+    check_behaviour_test_stiffness_base, examples/main_check_behaviour.cpp:60-118)
+    through the device path: assembly, Dirichlet elimination, CG with the
+    reference's stop rule and default eps, element strains, volume-averaged
+    stress.  n = 1 is the reference's 2-element sample."""
+    import stiffness_fixture as sf
+    E, nu, smax, expected = sf.load()
+    coords, conn = ag.structured_mesh(ag.P1, n)
+    xy, dofs, vals, f = sf.sample(coords, traction=smax)
+    p = ag.Problem(ag.P1, xy, conn, ag.cauchy_green(E, nu)[None], bc_dofs=dofs, bc_vals=vals, forces=f)
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=precision, ref_eps=1e-10)
+    assert st["status"] == 0, st
+    eps = B.element_strain(0, 0)
+    sig = B.average_stress()[0, 0]
+    assert eps[:, 1].mean() * 1e3 == pytest.approx(expected[1], rel=5e-6)
+    assert sig[1] / 1e6 == pytest.approx(expected[2], rel=5e-6)
+    np.testing.assert_allclose(eps, np.tile([-nu * smax / E, smax / E, 0.], (len(eps), 1)), atol=1e-6 * smax / E)
+
+
 def test_damage_step_vs_oracle(ctx):
     p = configs.config3(6, step=10)
     B = ag.Batch(ctx, [p])

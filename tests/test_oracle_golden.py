@@ -5,7 +5,10 @@ against the reference AMIE sources (oracle/Makefile.ref), with one OpenMP
 thread; they hold the reference's own outputs for element matrices,
 sparsity pattern, Kahan scatter, Dirichlet elimination, SpMV, preconditioners,
 dot products and ConjugateGradient::solve.  The reference-held fixture
-examples/test/test_cauchy_green_base is restated in test_cauchy_green_fixture.
+examples/test/test_cauchy_green_base is restated in test_cauchy_green_fixture;
+the elastic behaviour fixture (examples/test/test_stiffness.ini and
+check_behaviour_test_stiffness_base) is read from golden/ref_fixtures by
+test_stiffness_behaviour_fixture.
 """
 import os
 
@@ -46,6 +49,33 @@ def test_cauchy_green_fixture():
     assert pe[0, 0] == pytest.approx(1.11111e10, rel=5e-6)
     assert pe[0, 1] == pytest.approx(2.77778e9, rel=5e-6)
     assert pe[2, 2] == pytest.approx(8.33333e9, rel=5e-6)
+
+
+@pytest.mark.parametrize("n", [1, 3])
+def test_stiffness_behaviour_fixture(n):
+    """End to end through the oracle (assembly, Dirichlet elimination, the
+    reference CG with its default eps, strains, volume-averaged stress) on
+    the reference's behaviour test; n = 1 is its 2-element sample."""
+    import amie_gpu as ag   # host-side structured mesh (oriented connectivity)
+    import stiffness_fixture as sf
+    E, nu, smax, expected = sf.load()
+    coords, conn = ag.structured_mesh(ag.P1, n)
+    xy, dofs, vals, f = sf.sample(coords, traction=smax)
+    C = orc.cauchy_green(E, nu)[None]
+    A = orc.assemble(orc.P1, conn, xy, C)
+    orc.apply_dirichlet(A, f, dofs, vals)
+    x, nit, ok = orc.cg_reference(A, f, eps=1e-10)
+    assert ok
+    eps = orc.element_strain(orc.P1, conn, xy, x)
+    sig, vol = orc.average_stress(orc.P1, conn, xy, C, None, eps)
+    assert vol == pytest.approx(1e-4, rel=1e-12)
+    # the base file's columns (6 significant digits): eps_yy * 1e3, sigma_yy / 1e6, damage * 100
+    assert eps[:, 1].mean() * 1e3 == pytest.approx(expected[1], rel=5e-6)
+    assert sig[1] / 1e6 == pytest.approx(expected[2], rel=5e-6)
+    assert expected[3] == 0.   # Stiffness has no damage
+    # closed form of the uniform uniaxial state beside it
+    np.testing.assert_allclose(eps, np.tile([-nu * smax / E, smax / E, 0.], (len(eps), 1)), atol=1e-9 * smax / E)
+    np.testing.assert_allclose(sig, [0., smax, 0.], atol=1e-8 * smax)
 
 
 @pytest.mark.parametrize("tag,etype,E,nu", [

@@ -878,7 +878,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         P.rtol = o.rtol > 0 ? o.rtol : 1e-8;
         P.ref_eps = o.ref_eps > 0 ? o.ref_eps : 0.;
         P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
-        P.maxit = o.maxit > 0 ? o.maxit : ph[p].nrows * et.b;
+        P.maxit = o.maxit > 0 ? o.maxit : std::max<int64_t>(2 * ph[p].nrows * et.b, 1000);
         P.ckpt_interval = R.mixed ? (o.ckpt_interval > 0 ? o.ckpt_interval : 128) : ((int64_t)1 << 62);
         P.stall_iters = (fp32vec && o.escalate) ? 8192 : 0;
         P.ladder = mode == AMIE_MIXED_SPLIT ? 1 : 0;
