@@ -253,19 +253,24 @@ int amie_gpu_batch_time_spmv(amie_gpu_batch *b, int32_t precision, int32_t preco
  * this batch with the same precision and preconditioner.  Candidates: half,
  * equal and twice the default number of blocks per step of the row loop,
  * each with four tile orders: XCD-contiguous ranges (0), or chunks of C
- * tiles dealt round-robin over the 8 XCDs (C = 1: launch order, 8, 64).
- * Results do not depend on the choice (the sums run in the same order).
- * Precisions with candidates: AMIE_FP64, AMIE_MIXED and AMIE_MIXED_FIX48.
- * Outputs (each nullable): the chosen blocks per step and tile order and its
- * time per launch. */
+ * tiles dealt round-robin over the 8 XCDs (C = 1: launch order, 8, 64);
+ * with AMIE_MIXED_FIX48 also the value-indexed matrix (format 1: a 16-bit
+ * index per block into a table of the distinct FIX48 blocks, built on the
+ * device when there are at most 65536 of them, then rebuilt at every
+ * assembly).  Results do not depend on the choice (the same values, the
+ * sums in the same order).  Precisions with candidates: AMIE_FP64,
+ * AMIE_MIXED and AMIE_MIXED_FIX48.  Outputs (each nullable): the chosen
+ * blocks per step, tile order and format (0 value planes, 1 value-indexed)
+ * and its time per launch. */
 int amie_gpu_batch_autotune(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t iters,
-                            int32_t *steps, int32_t *tile_order, double *us_per_launch);
+                            int32_t *steps, int32_t *tile_order, int32_t *format, double *us_per_launch);
 /* Sets the SpMV candidate directly (a choice kept from an earlier
  * amie_gpu_batch_autotune, for instance); steps must be one of the
- * candidates', tile_order 0 or a chunk of 1..4096 tiles (AMIE_ERR_ARG
- * otherwise).  No reference counterpart. */
+ * candidates', tile_order 0 or a chunk of 1..4096 tiles, format 1 only with
+ * AMIE_MIXED_FIX48 and a matrix of at most 65536 distinct blocks
+ * (AMIE_ERR_ARG otherwise).  No reference counterpart. */
 int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *b, int32_t precision, int32_t precond, int32_t steps,
-                                   int32_t tile_order);
+                                   int32_t tile_order, int32_t format);
 /* Element strains (nelem x 3|6, Voigt, engineering shear) of right-hand side
  * rhs, including the macro strain. */
 int amie_gpu_batch_element_strain(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain);

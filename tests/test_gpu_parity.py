@@ -436,10 +436,11 @@ def test_result_independent_of_grid(ctx):
 @pytest.mark.parametrize("kind", ["rve2d", "rve3d"])
 @pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED, ag.MIXED_FIX48])
 def test_spmv_candidates_give_same_bits(ctx, kind, precision):
-    """Every SpMV candidate of the autotuner (blocks per step x tile order)
-    gives the same solution bits: the row sums run in the same order and the
-    partial sums are per tile.  autotune keeps one of the candidates, and the
-    autotune solve option runs it."""
+    """Every SpMV candidate of the autotuner (blocks per step x tile order,
+    and with FIX48 the value-indexed matrix) gives the same solution bits:
+    the same values, the row sums in the same order and the partial sums per
+    tile.  autotune keeps one of the candidates, and the autotune solve
+    option runs it."""
     if kind == "rve2d":
         probs = [configs.rve(n, 900 + n, 0.3) for n in (12, 20, 31)]
     else:
@@ -449,19 +450,23 @@ def test_spmv_candidates_give_same_bits(ctx, kind, precision):
     ref = [B.solution(k).copy() for k in range(len(probs))]
     tuned = B.autotune(precision, ag.JACOBI, iters=3)
     assert tuned["tile_order"] in (0, 1, 8, 64) and tuned["us"] > 0
+    assert tuned["format"] in ((0, 1) if precision == ag.MIXED_FIX48 else (0,))
     found = set()
-    for steps in (1, 2, 4, 8, 16):
-        for order in (0, 1, 8, 64):
-            try:
-                B.set_spmv_kernel(precision, ag.JACOBI, steps, order)
-            except ag.AmieError:
-                continue
-            found.add(steps)
-            st = B.solve(precision=precision, rtol=1e-8)
-            assert st["status"] == 0, (steps, order, st)
-            for k in range(len(probs)):
-                np.testing.assert_array_equal(B.solution(k), ref[k])
-    assert len(found) == 3 and tuned["steps"] in found
+    for fmt in (0, 1):
+        for steps in (1, 2, 4, 8, 16):
+            for order in (0, 1, 8, 64):
+                try:
+                    B.set_spmv_kernel(precision, ag.JACOBI, steps, order, fmt)
+                except ag.AmieError:
+                    continue
+                found.add((fmt, steps))
+                st = B.solve(precision=precision, rtol=1e-8)
+                assert st["status"] == 0, (fmt, steps, order, st)
+                for k in range(len(probs)):
+                    np.testing.assert_array_equal(B.solution(k), ref[k])
+    # three step counts per format; the value-indexed matrix with FIX48 only
+    assert len(found) == (6 if precision == ag.MIXED_FIX48 else 3)
+    assert (tuned["format"], tuned["steps"]) in found
     with pytest.raises(ag.AmieError):
         B.set_spmv_kernel(precision, ag.JACOBI, 3, 0)
     with pytest.raises(ag.AmieError):
@@ -470,6 +475,32 @@ def test_spmv_candidates_give_same_bits(ctx, kind, precision):
     assert B2.solve(precision=precision, rtol=1e-8, autotune=1)["status"] == 0
     for k in range(len(probs)):
         np.testing.assert_array_equal(B2.solution(k), ref[k])
+
+
+def test_value_indexed_matrix_follows_assembly(ctx):
+    """The value-indexed matrix is rebuilt at every assembly: after a damage
+    step changes the values, solves with it still match the FIX48 planes bit
+    for bit; a matrix with more distinct blocks than 16-bit indices reach
+    (a damage field over every element) has none, and the candidate is
+    refused."""
+    probs = [configs.rve(n, 990 + n, 0.3) for n in (10, 17)]
+    B = ag.Batch(ctx, probs)
+    B.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, 8, 0, 1)
+    for step in range(2):
+        if step:
+            B.damage_step()   # new element stiffness, new values, re-assembly
+        st = B.solve(precision=ag.MIXED_FIX48, rtol=1e-8)
+        assert st["status"] == 0
+        xd = [B.solution(k).copy() for k in range(len(probs))]
+        B.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, 8, 0, 0)
+        st = B.solve(precision=ag.MIXED_FIX48, rtol=1e-8)
+        for k in range(len(probs)):
+            np.testing.assert_array_equal(B.solution(k), xd[k])
+        B.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, 8, 0, 1)
+    D = ag.Batch(ctx, [configs.config3(40)])   # 384 000 elements, each with its own damage
+    with pytest.raises(ag.AmieError):
+        D.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, 8, 0, 1)
+    assert D.autotune(ag.MIXED_FIX48, ag.JACOBI, iters=2)["format"] == 0
 
 
 DROPIN = os.path.join(os.path.dirname(__file__), "..", "xfem-amie_amd", "host", "dropin_check")

@@ -89,8 +89,8 @@ SIGNATURES = {
     "amie_gpu_batch_get_element_blocks": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_spmv": (c_i32, [vp, c_i32, c_i32, vp, vp]),
     "amie_gpu_batch_time_spmv": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp]),
-    "amie_gpu_batch_autotune": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp, vp]),
-    "amie_gpu_batch_set_spmv_kernel": (c_i32, [vp, c_i32, c_i32, c_i32, c_i32]),
+    "amie_gpu_batch_autotune": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp, vp, vp]),
+    "amie_gpu_batch_set_spmv_kernel": (c_i32, [vp, c_i32, c_i32, c_i32, c_i32, c_i32]),
     "amie_gpu_batch_element_strain": (c_i32, [vp, c_i32, c_i32, vp]),
     "amie_gpu_batch_average_stress": (c_i32, [vp, vp]),
     "amie_gpu_batch_damage_step": (c_i32, [vp, vp, vp]),
@@ -400,16 +400,17 @@ class Batch:
 
     def autotune(self, precision=MIXED_FIX48, precond=JACOBI, iters=5):
         """Picks the fastest SpMV candidate for this batch: returns
-        {"steps", "tile_order", "us"}; later solves of the same precision and
-        preconditioner use it."""
-        st, order, us = c_i32(), c_i32(), c_dbl()
+        {"steps", "tile_order", "format", "us"}; later solves of the same
+        precision and preconditioner use it."""
+        st, order, fmt, us = c_i32(), c_i32(), c_i32(), c_dbl()
         check(lib().amie_gpu_batch_autotune(self.h, precision, precond, iters, ctypes.byref(st), ctypes.byref(order),
-                                            ctypes.byref(us)))
-        return {"steps": st.value, "tile_order": order.value, "us": us.value}
+                                            ctypes.byref(fmt), ctypes.byref(us)))
+        return {"steps": st.value, "tile_order": order.value, "format": fmt.value, "us": us.value}
 
-    def set_spmv_kernel(self, precision, precond, steps, tile_order):
-        """Selects an SpMV candidate directly (blocks per step, tile order)."""
-        check(lib().amie_gpu_batch_set_spmv_kernel(self.h, precision, precond, steps, tile_order))
+    def set_spmv_kernel(self, precision, precond, steps, tile_order, format=0):
+        """Selects an SpMV candidate directly (blocks per step, tile order,
+        format 0 value planes / 1 value-indexed FIX48 matrix)."""
+        check(lib().amie_gpu_batch_set_spmv_kernel(self.h, precision, precond, steps, tile_order, format))
 
     def element_strain(self, p=0, rhs=0):
         pr = self.problems[p]

@@ -76,7 +76,7 @@ Batch::~Batch()
 
 size_t Batch::bytes() const
 {
-    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &slot_off, &col, &dslot, &row_len, &v64,
+    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &slot_off, &dict_ids, &dict_table, &col, &dslot, &row_len, &v64,
                            &v32, &vsp, &vfx, &fix48_bad, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
                            &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
@@ -105,6 +105,7 @@ SolveArgs Batch::args(int mode) const
     a.tile_desc = tile_desc.as<int4>();
     a.slice_desc = slice_desc.as<int4>();
     a.slot_off = slot_off.as<int32_t>();
+    a.dict_table = dict_table.as<double>();
     a.ntiles_all = (int32_t)ntiles;
     a.tile_nslices = tile_nslices.as<int32_t>();
     a.tile_prob = tile_prob.as<int32_t>();
@@ -676,6 +677,7 @@ int Batch::assemble()
     AMIE_HIP(hipMemcpyAsync(&bad, fix48_bad.p, 4, hipMemcpyDeviceToHost, st));
     AMIE_HIP(hipStreamSynchronize(st));
     fix48_ok = bad == 0;
+    if (dict_wanted) return build_dict(*this);
     return AMIE_OK;
 }
 
@@ -704,7 +706,7 @@ static Kern select_kern(int mode, int b, int nr, bool bj)
 
 // ------------------------------------------------------------ autotuner
 // Candidate SpMV kernels: which = 0, 1, 2 -> half, equal and twice the
-// default blocks per step.
+// default blocks per step; format 1: the value-indexed FIX48 matrix.
 typedef void (*KFn)(SolveArgs);
 struct K1Var {
     KFn k1;
@@ -727,8 +729,12 @@ template <typename W, typename M, int Bk> static K1Var k1_nr(int nr, bool bj, in
 }
 
 // the precisions with candidates: FP64, all-FP32 and the fixed-point matrix
-static K1Var k1_variant(int mode, int b, int nr, bool bj, int which)
+// (planes or value-indexed)
+static K1Var k1_variant(int mode, int b, int nr, bool bj, int which, int format = 0)
 {
+    if (format == 1)
+        return mode != AMIE_MIXED_FIX48 ? K1Var{nullptr, 0}
+                                        : (b == 2 ? k1_nr<float, Dict, 2>(nr, bj, which) : k1_nr<float, Dict, 3>(nr, bj, which));
     if (mode == AMIE_MIXED) return b == 2 ? k1_nr<float, float, 2>(nr, bj, which) : k1_nr<float, float, 3>(nr, bj, which);
     if (mode == AMIE_MIXED_FIX48)
         return b == 2 ? k1_nr<float, Fix48, 2>(nr, bj, which) : k1_nr<float, Fix48, 3>(nr, bj, which);
@@ -742,8 +748,18 @@ void Batch::apply_tuned(int mode, bool bj, Kern &K, SolveArgs &a) const
 {
     auto it = k1_tuned.find(tune_key(mode, bj));
     if (it == k1_tuned.end()) return;
+    if (it->second.format == 1) {
+        if (!dict_ok) return;   // this assembly has no dictionary: the default kernel
+        a.vw = dict_ids.p;
+    }
     K.k1 = it->second.k1;
     a.tile_order = it->second.tile_order;
+}
+
+int Batch::k1_format(int mode, bool bj) const
+{
+    auto it = k1_tuned.find(tune_key(mode, bj));
+    return it != k1_tuned.end() && it->second.format == 1 && dict_ok ? 1 : 0;
 }
 
 static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs &a)
@@ -754,8 +770,10 @@ static void launch0(void (*k)(SolveArgs), int g, hipStream_t st, const SolveArgs
 // Algorithmic bytes of one SpMV launch (K1) over problem p: matrix values
 // and block column indices, row pointer, the gathered vector p read once,
 // r read for the dots, q written, per right-hand side (+ the block-Jacobi
-// inverses).  Padding blocks are not counted.
-static double spmv_bytes(const Batch &B, const ProbHost &P, int mode, bool bj)
+// inverses).  Padding blocks are not counted.  The value-indexed matrix
+// (format 1) reads a 2-byte index per block, plus its share (by blocks) of
+// the table once.
+static double spmv_bytes(const Batch &B, const ProbHost &P, int mode, bool bj, int format = 0)
 {
     const double sm = (mode == AMIE_FP64 || mode == AMIE_MIXED_K64) ? 8.
                       : ((mode == AMIE_MIXED_SPLIT || mode == AMIE_MIXED_FIX48) ? 6. : 4.);
@@ -766,7 +784,12 @@ static double spmv_bytes(const Batch &B, const ProbHost &P, int mode, bool bj)
     // K1 loads no column index for blocks at a position with a shared
     // offset, but one 4-byte offset per position
     const double cols = nnzb - (double)P.offset_blocks + (double)P.positions;
-    return nnzb * b * b * sm + 4. * cols + 4. * (P.nrows + 1) + 3. * n * sv * B.nrhs + (bj ? n * b * sv : 0.);
+    double vals = nnzb * b * b * sm;
+    if (format == 1) {
+        const double tbytes = (double)B.dict_n * (b == 2 ? DictEntry<2>::N : DictEntry<3>::N) * 8.;
+        vals = nnzb * 2. + tbytes * nnzb / std::max(1., (double)B.nslots);
+    }
+    return vals + 4. * cols + 4. * (P.nrows + 1) + 3. * n * sv * B.nrhs + (bj ? n * b * sv : 0.);
 }
 
 int Batch::solve_begin(const amie_solve_opts &o)
@@ -940,7 +963,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
     int64_t maxit_all = 0;
     for (int p = 0; p < nprob; p++) {
         if (!subset[p]) continue;
-        R.pbytes[p] = spmv_bytes(*this, ph[p], mode, R.bj);
+        R.pbytes[p] = spmv_bytes(*this, ph[p], mode, R.bj, k1_format(mode, R.bj));
         R.pbytes_k64[p] = spmv_bytes(*this, ph[p], AMIE_MIXED_K64, R.bj);
         R.was_running[p] = 1;
         maxit_all = std::max<int64_t>(maxit_all, h_prob[p].maxit);
@@ -1513,19 +1536,29 @@ int Batch::autotune(int mode, bool bj, int iters)
         set_error("AMIE_MIXED_FIX48: a scaled matrix entry is outside (-2, 2) (K not symmetric positive definite?)");
         return AMIE_ERR_ARG;
     }
+    if (mode == AMIE_MIXED_FIX48 && !dict_wanted) {
+        // from now on every assembly also builds the value-indexed matrix
+        dict_wanted = true;
+        const int rc = build_dict(*this);
+        if (rc) return rc;
+    }
     K1Choice best;
     best.us = DBL_MAX;
     static const int orders[] = {0, 1, 8, 64};
-    for (int which = 0; which < 3; which++)
-        for (int order : orders) {
-            const K1Var v = k1_variant(mode, et.b, nrhs, bj, which);
-            SolveArgs a = args(mode);
-            a.tile_order = order;
-            double us = 0.;
-            const int rc = time_k1(*this, v.k1, a, iters, &us);
-            if (rc) return rc;
-            if (us < best.us) best = {v.k1, v.steps, order, us};
-        }
+    for (int format = 0; format < 2; format++) {
+        if (format == 1 && !(mode == AMIE_MIXED_FIX48 && dict_ok)) continue;
+        for (int which = 0; which < 3; which++)
+            for (int order : orders) {
+                const K1Var v = k1_variant(mode, et.b, nrhs, bj, which, format);
+                SolveArgs a = args(mode);
+                a.tile_order = order;
+                if (format == 1) a.vw = dict_ids.p;
+                double us = 0.;
+                const int rc = time_k1(*this, v.k1, a, iters, &us);
+                if (rc) return rc;
+                if (us < best.us) best = {v.k1, v.steps, format, order, us};
+            }
+    }
     k1_tuned[tune_key(mode, bj)] = best;
     return AMIE_OK;
 }
@@ -1547,14 +1580,15 @@ int amie_gpu_batch_time_spmv(amie_gpu_batch *bb, int32_t precision, int32_t prec
     if (rc) return rc;
     if (bytes_per_launch) {
         double s = 0.;
-        for (int p = 0; p < B.nprob; p++) s += spmv_bytes(B, B.ph[p], precision, bj);
+        const int fmt = B.k1_format(precision, bj);
+        for (int p = 0; p < B.nprob; p++) s += spmv_bytes(B, B.ph[p], precision, bj, fmt);
         *bytes_per_launch = s;
     }
     return AMIE_OK;
 }
 
 int amie_gpu_batch_autotune(amie_gpu_batch *bb, int32_t precision, int32_t precond, int32_t iters, int32_t *steps,
-                            int32_t *tile_order, double *us_per_launch)
+                            int32_t *tile_order, int32_t *format, double *us_per_launch)
 {
     if (!bb || !bb->b || iters <= 0) {
         set_error("bad arguments");
@@ -1568,24 +1602,39 @@ int amie_gpu_batch_autotune(amie_gpu_batch *bb, int32_t precision, int32_t preco
     const K1Choice &c = B.k1_tuned[tune_key(precision, bj)];
     if (steps) *steps = c.steps;
     if (tile_order) *tile_order = c.tile_order;
+    if (format) *format = c.format;
     if (us_per_launch) *us_per_launch = c.us;
     return AMIE_OK;
 }
 
 int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *bb, int32_t precision, int32_t precond, int32_t steps,
-                                   int32_t tile_order)
+                                   int32_t tile_order, int32_t format)
 {
-    if (!bb || !bb->b || tile_order < 0 || tile_order > 4096) {
+    if (!bb || !bb->b || tile_order < 0 || tile_order > 4096 || format < 0 || format > 1) {
         set_error("bad arguments");
         return AMIE_ERR_ARG;
     }
     Batch &B = *bb->b;
     if (B.run.started) { set_error("a solve is in progress"); return AMIE_ERR_BUSY; }
     const bool bj = precond == AMIE_BLOCK_JACOBI;
+    if (format == 1) {
+        if (precision != AMIE_MIXED_FIX48) {
+            set_error("the value-indexed matrix exists for AMIE_MIXED_FIX48 only");
+            return AMIE_ERR_ARG;
+        }
+        if (!B.dict_wanted) {
+            B.dict_wanted = true;
+            if (int rc = build_dict(B)) return rc;
+        }
+        if (!B.dict_ok) {
+            set_error("no value-indexed matrix: more than %d distinct blocks, or FIX48 out of range", DICT_MAX);
+            return AMIE_ERR_ARG;
+        }
+    }
     for (int which = 0; which < 3; which++) {
-        const K1Var v = k1_variant(precision, B.et.b, B.nrhs, bj, which);
+        const K1Var v = k1_variant(precision, B.et.b, B.nrhs, bj, which, format);
         if (v.k1 && v.steps == steps) {
-            B.k1_tuned[tune_key(precision, bj)] = {v.k1, v.steps, tile_order, 0.};
+            B.k1_tuned[tune_key(precision, bj)] = {v.k1, v.steps, format, tile_order, 0.};
             return AMIE_OK;
         }
     }

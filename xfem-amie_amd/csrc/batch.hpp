@@ -67,6 +67,7 @@ struct Kern {
 struct K1Choice {
     void (*k1)(SolveArgs) = nullptr;
     int steps = 0;           // blocks per step of the row loop
+    int format = 0;          // 0: the precision's value planes, 1: the value-indexed FIX48 matrix
     int tile_order = 0;      // SolveArgs::tile_order
     double us = 0.;          // its time per launch with every tile active
 };
@@ -127,6 +128,10 @@ struct Batch {
     DevBuf slice_ptr, slice_width, slice_rows, col, dslot, row_len;
     DevBuf tile_desc, slice_desc;     // packed {slice0, nslices, prob} / {slot ptr lo, hi, width, rows}
     DevBuf slot_off;                  // per 64 slots: column - row shared by the position's 64 lanes, or NO_OFFSET
+    DevBuf dict_ids, dict_table;      // value-indexed FIX48 matrix: 16-bit index per slot, distinct blocks (FP64)
+    bool dict_wanted = false;         // rebuild the dictionary at every assembly (set by the autotuner)
+    bool dict_ok = false;             // the dictionary matches the current FIX48 planes
+    int64_t dict_n = 0;               // its distinct blocks
     DevBuf v64, v32, vsp, vfx;   // FP64 unscaled K; FP32, split FP32+bf16 and 48-bit fixed-point scaled Ks
     DevBuf fix48_bad;            // int flag: a scaled value outside the fixed-point range
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active, active_prob, n_active_prob;
@@ -165,8 +170,13 @@ struct Batch {
     int set_active(const std::vector<char> &probs);
     int autotune(int mode, bool bj, int iters);
     void apply_tuned(int mode, bool bj, Kern &K, SolveArgs &a) const;
+    int k1_format(int mode, bool bj) const;
     int64_t vplane() const { return nrows_pad * et.b; }
 };
+
+// Builds the value-indexed FIX48 matrix of the current assembly (dict_ok
+// tells whether it exists: at most DICT_MAX distinct blocks).
+int build_dict(Batch &B);
 
 // FP64 plane vector of problem p <-> original DOF order, one column of
 // nrows*b entries per right-hand side.

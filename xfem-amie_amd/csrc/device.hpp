@@ -108,6 +108,7 @@ struct SolveArgs {
     const int4 *tile_desc;   // per tile {first slice, slices, problem, 0}
     const int4 *slice_desc;  // per slice {slot pointer lo, hi, width, rows}
     const int32_t *slot_off; // per 64 slots (one block position of a slice): column - row shared by all lanes, or NO_OFFSET
+    const double *dict_table; // value-indexed FIX48 matrix: the distinct blocks, decoded (b*b doubles each)
     int32_t ntiles_all;
     int32_t tile_order;      // 0: XCD-contiguous tile ranges (xcd_index); C > 0: chunks of C tiles dealt
                              // round-robin over the XCDs (1: launch order)
@@ -515,6 +516,27 @@ template <> struct MatKind<Fix48> {
     static constexpr bool onfly = false, acc64 = true;
 };
 
+// Value-indexed FIX48 matrix: structured meshes hold few distinct blocks
+// (the 512-RVE batch: about 10^3 among 9 x 10^7), so each slot stores a
+// 16-bit index into a table of the distinct FIX48 blocks, decoded to FP64.
+// The iteration then reads 2 bytes a block from HBM instead of 24 (2x2) or
+// 54 (3x3); the table stays in the caches.  The values are bitwise those of
+// the FIX48 planes, so are the results.
+struct Dict {};
+template <int B> struct Blk<Dict, B> {
+    typedef double T;
+    static constexpr int N = B * B;
+    static constexpr size_t bytes_per_slot = 2;
+};
+template <> struct MatKind<Dict> {
+    static constexpr bool onfly = false, acc64 = true;
+};
+constexpr int DICT_MAX = 65536;   // table entries addressable by a 16-bit index
+// doubles per table entry: b*b, padded to a 16-byte multiple
+template <int B> struct DictEntry {
+    static constexpr int N = B == 2 ? 4 : 10;
+};
+
 // ------------------------------------------------------------ reductions
 __device__ inline double wave_sum(double v)
 {
@@ -776,7 +798,8 @@ template <typename A, typename M, int B, int NR, int U, typename G>
 __device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t sp0, int w, int lane, int64_t row,
                                   A (*acc)[B], G gather)
 {
-    typedef BlkRaw<M, B> BR;
+    constexpr bool IS_DICT = std::is_same<M, Dict>::value;
+    typedef BlkRaw<typename std::conditional<IS_DICT, double, M>::type, B> BR;
     const int64_t sp = sp0 + lane;
     const int32_t *offs = a.slot_off + (sp0 >> 6);
     for (int k = 0; k < w; k += U) {
@@ -787,7 +810,7 @@ __device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t 
             o[u] = k + u < w ? offs[k + u] : 0;
             direct &= o[u] != NO_OFFSET;
         }
-        uint32_t c[U];
+        uint32_t c[U], id[U];
         typename BR::Raw blk[U];
         if (direct) {
 #pragma unroll
@@ -799,17 +822,35 @@ __device__ inline void spmv_slice(const SolveArgs &a, const void *vals, int64_t 
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
-            if (k + u < w) BR::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
+            if (k + u < w) {
+                if constexpr (IS_DICT)
+                    id[u] = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(vals) + sp + (int64_t)(k + u) * WAVE);
+                else
+                    BR::load(vals, a.nslots, sp + (int64_t)(k + u) * WAVE, blk[u]);
+            }
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (k + u < w) {
                 A xv[NR][B];
                 gather(c[u], xv);
+                double dv[IS_DICT ? B * B : 1];
+                if constexpr (IS_DICT) {
+                    const d2v *e = reinterpret_cast<const d2v *>(a.dict_table + (size_t)id[u] * DictEntry<B>::N);
+#pragma unroll
+                    for (int h = 0; h < B * B / 2; h++) {
+                        const d2v t = e[h];
+                        dv[2 * h] = t.x;
+                        dv[2 * h + 1] = t.y;
+                    }
+                    if constexpr (B * B % 2) dv[B * B - 1] = reinterpret_cast<const double *>(e)[B * B - 1];
+                }
 #pragma unroll
                 for (int j = 0; j < B; j++)
 #pragma unroll
                     for (int i = 0; i < B; i++) {
-                        const A m = (A)BR::at(blk[u], j * B + i);
+                        A m;
+                        if constexpr (IS_DICT) m = (A)dv[j * B + i];
+                        else m = (A)BR::at(blk[u], j * B + i);
 #pragma unroll
                         for (int r = 0; r < NR; r++) acc[r][i] += m * xv[r][j];
                     }
