@@ -186,8 +186,8 @@ def main():
                     help="serving legs as RATE:SECONDS (Poisson arrivals in req/s over an arrival window)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the P2 400^2 and Tet4 64^3 legs")
-    ap.add_argument("--spmv-kernel", default="", help="STEPS,ORDER: set the batch's FIX48 SpMV candidate instead "
-                                                      "of autotuning (profiling runs)")
+    ap.add_argument("--spmv-kernel", default="", help="STEPS,ORDER[,FORMAT]: set the batch's FIX48 SpMV candidate "
+                                                      "instead of autotuning (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -208,9 +208,9 @@ def main():
     hbm_bytes = B.nbytes
     # set-up: the autotuner picks the SpMV kernel of each precision once
     if args.spmv_kernel:
-        steps_, order_ = (int(v) for v in args.spmv_kernel.split(","))
-        B.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, steps_, order_)
-        tuned = {"fix48": {"steps": steps_, "tile_order": order_, "us": None}}
+        sk = [int(v) for v in args.spmv_kernel.split(",")] + [0]
+        B.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, sk[0], sk[1], sk[2])
+        tuned = {"fix48": {"steps": sk[0], "tile_order": sk[1], "format": sk[2], "us": None}}
     else:
         tuned = {name: B.autotune(prec, ag.JACOBI) for name, prec in
                  (("fix48", ag.MIXED_FIX48), ("fp32", ag.MIXED), ("fp64", ag.FP64))}
@@ -270,6 +270,15 @@ def main():
     log(f"stream copy: {copy_gbs:.0f} GB/s")
     us_full, by_full = B.time_spmv(headline, ag.JACOBI, 20)
     log(f"k_spmv with every tile active: {us_full:.1f} us, {by_full / 1e9:.3f} GB, {by_full / us_full / 1e3:.0f} GB/s")
+    tf = tuned["fix48"]
+    planes = None
+    if tf.get("format") == 1:
+        # the same product from the FIX48 value planes, for comparison
+        B.set_spmv_kernel(headline, ag.JACOBI, 4, 8, 0)
+        us_pl, by_pl = B.time_spmv(headline, ag.JACOBI, 20)
+        B.set_spmv_kernel(headline, ag.JACOBI, tf["steps"], tf["tile_order"], 1)
+        planes = {"us": us_pl, "bytes": by_pl, "gbs": by_pl / us_pl / 1e3}
+        log(f"k_spmv from the FIX48 planes: {us_pl:.1f} us, {by_pl / 1e9:.3f} GB, {by_pl / us_pl / 1e3:.0f} GB/s")
     B.close()
     single = None if args.no_single else single_problem_legs(ag, configs, ctx, args.rtol, args.maxit)
     serve = None
@@ -312,12 +321,15 @@ def main():
                                         f"({os.path.relpath(TRAFFIC_PROFILE, ROOT)}), times avg_bytes_per_launch"
                                         if tr else None),
                      "stream_copy_gbs": copy_gbs,
-                     "kernel": "k_spmv (block SpMV q = Ks p, 48-bit fixed-point values, fused p.q, q.z, q.Mq)",
+                     "kernel": "k_spmv (block SpMV q = Ks p, 48-bit fixed-point values"
+                               + (", value-indexed: 16-bit index per block into a table of the distinct blocks"
+                                  if tf.get("format") == 1 else "") + ", fused p.q, q.z, q.Mq)",
                      "autotuned": tuned,
                      "launches": launches,
                      "avg_launch_us": 1e3 * spmv_ms / max(1, launches),
                      "avg_bytes_per_launch": avg_bytes,
-                     "full_occupancy": {"us": us_full, "bytes": by_full, "gbs": by_full / us_full / 1e3}},
+                     "full_occupancy": {"us": us_full, "bytes": by_full, "gbs": by_full / us_full / 1e3},
+                     "fix48_planes_full_occupancy": planes},
         "cpu_baseline": cpu,
         "fp64_batched": fp64,
         "fp32_inner_batched": fp32,
