@@ -83,8 +83,9 @@ def cpu_baseline(problems, budget_s=15.0, rtol=1e-8):
 
 def serving(ag, ctx, reqs, precision, rtol, limit_s=90.0):
     """Replays a request trace against the server in wall-clock time and
-    returns p50/p99 request latency (arrival to result) and solves/s."""
-    S = ag.Server(ctx, 200 << 30, 64, precision=precision, rtol=rtol)
+    returns p50/p99 request latency (arrival to result) and solves/s.  The
+    server autotunes each group's SpMV on its build thread."""
+    S = ag.Server(ctx, 200 << 30, 64, precision=precision, rtol=rtol, autotune=1)
     t0 = time.perf_counter()
     lat, solves, nsub, bad = [], 0, 0, 0
     nrhs = {}

@@ -378,9 +378,10 @@ def test_bad_input_is_an_error(ctx):
         ag.Batch(ctx, [ag.Problem(ag.P1, coords, bad, ag.cauchy_green(1e10, 0.2)[None])])
 
 
-def test_server_result_same_alone_and_in_mix(ctx):
+@pytest.mark.parametrize("precision,tune", [(ag.MIXED, 0), (ag.MIXED_FIX48, 1)])
+def test_server_result_same_alone_and_in_mix(ctx, precision, tune):
     probs = [configs.rve(n, 500 + n, 0.3) for n in (8, 12, 16, 10)]
-    S = ag.Server(ctx, 8 << 30, 8, precision=ag.MIXED, rtol=1e-8)
+    S = ag.Server(ctx, 8 << 30, 8, precision=precision, rtol=1e-8, autotune=tune)
     t = S.submit(probs[1], 0.0)
     res = []
     t_end = time.monotonic() + 60.0   # batches are built on worker threads: steps return at once

@@ -302,8 +302,9 @@ struct Server {
                 std::vector<amie_problem_desc> descs;
                 for (auto &r : g->reqs) descs.push_back(r.prob.d);
                 amie_gpu_ctx *cx = ctx;
+                const amie_solve_opts so = opts;
                 Building b;
-                b.fut = std::async(std::launch::async, [cx, descs]() {
+                b.fut = std::async(std::launch::async, [cx, descs, so]() {
                     Built o;
                     if (hipSetDevice(cx->device) != hipSuccess) {
                         o.rc = AMIE_ERR_HIP;
@@ -311,7 +312,17 @@ struct Server {
                         return o;
                     }
                     o.rc = batch_create(cx, (int)descs.size(), descs.data(), &o.batch);
-                    if (o.rc) o.err = amie_gpu_last_error();
+                    // the SpMV candidates are timed here too, off the serving thread
+                    // (under the load of the groups in flight)
+                    const bool tunable = so.precision == AMIE_FP64 || so.precision == AMIE_MIXED ||
+                                         so.precision == AMIE_MIXED_FIX48;
+                    if (o.rc == AMIE_OK && so.autotune && tunable)
+                        o.rc = o.batch->autotune(so.precision, so.precond == AMIE_BLOCK_JACOBI, 3);
+                    if (o.rc) {
+                        o.err = amie_gpu_last_error();
+                        delete o.batch;
+                        o.batch = nullptr;
+                    }
                     return o;
                 });
                 b.g = std::move(g);
