@@ -282,6 +282,21 @@ int amie_gpu_batch_average_stress(amie_gpu_batch *b, double *out);
  * d0 (nullable, per problem element count) seeds the damage on first call.
  * n_grown (nullable) returns the number of elements whose damage grew. */
 int amie_gpu_batch_damage_step(amie_gpu_batch *b, const amie_damage_params *prm, int64_t *n_grown);
+/* Non-local damage driving strain (linear elements, P1 and Tet4): restates
+ * the reference's smoothed fields, Mesh::getSmoothedFields
+ * (mesher/mesh.h:1384-1486) over the neighbourhoods of Mesh::generateCache
+ * (mesher/mesh.h:546-640) as FractureCriterion::initialiseCache builds them
+ * (physics/fracturecriteria/fracturecriterion.cpp:171-190).  radius is the
+ * material characteristic radius (the reference default is 0.008; <= 0
+ * turns the smoothing off), kernel 0 QUARTIC_COMPACT (the reference
+ * default) or 1 GAUSSIAN_NONCOMPACT, elements with damage >= threshold count
+ * as fractured and are left out (the reference default threshold is 1).
+ * Once set, amie_gpu_batch_damage_step drives the damage law with the
+ * smoothed strains.  The reference's event-driven damage iteration
+ * (physics/damagemodels/damagemodel.cpp:23-320) is not restated. */
+int amie_gpu_batch_set_nonlocal(amie_gpu_batch *b, double radius, int32_t kernel, double threshold);
+/* The smoothed strains of problem p for right-hand side rhs (nelem x 3|6). */
+int amie_gpu_batch_smoothed_strain(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain);
 int amie_gpu_batch_get_damage(amie_gpu_batch *b, int32_t p, double *d, double *kappa);
 int amie_gpu_batch_set_damage(amie_gpu_batch *b, int32_t p, const double *d);
 /* Scale the Dirichlet values of every problem (incremental loading). */

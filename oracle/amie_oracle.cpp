@@ -25,6 +25,9 @@
 //   or_damage_update     closed form of the law; principal strains of
 //                        numpy.linalg.eigvalsh; monotonicity
 //   or_average_stress    homogeneous fields; Voigt/Reuss bounds
+//   or_smoothed_strain   constant fields reproduced; kernel values at known
+//                        distances; a lone element returns its own strain;
+//                        phase and fracture exclusions; neighbourhood radius
 
 #include "amie_oracle.h"
 
@@ -1000,6 +1003,107 @@ void or_average_stress(int etype, int64_t nelem, const int32_t *conn, const doub
     }
     for (int i = 0; i < s; i++) out[i] = acc[i] / vol;
     if (measure_out) *measure_out = vol;
+}
+
+} // extern "C"
+
+namespace {
+
+// centroid, circumradius and measure of a linear element
+struct ElemGeo {
+    double c[3], r, meas;
+};
+
+ElemGeo elem_geo(int dim, int nper, const int32_t *ids, const double *coords)
+{
+    ElemGeo g{};
+    double X[4][3] = {};
+    for (int i = 0; i < nper; i++)
+        for (int d = 0; d < dim; d++) X[i][d] = coords[(size_t)ids[i] * dim + d];
+    for (int d = 0; d < 3; d++) {
+        double s = 0.;
+        for (int i = 0; i < nper; i++) s += X[i][d];
+        g.c[d] = s / nper;
+    }
+    if (dim == 2) {
+        // circumcentre: intersection of the perpendicular bisectors
+        const double ax = X[0][0], ay = X[0][1], bx = X[1][0], by = X[1][1], cx = X[2][0], cy = X[2][1];
+        const double dd = 2. * (ax * (by - cy) + bx * (cy - ay) + cx * (ay - by));
+        const double a2 = ax * ax + ay * ay, b2 = bx * bx + by * by, c2 = cx * cx + cy * cy;
+        const double ux = (a2 * (by - cy) + b2 * (cy - ay) + c2 * (ay - by)) / dd;
+        const double uy = (a2 * (cx - bx) + b2 * (ax - cx) + c2 * (bx - ax)) / dd;
+        // Triangle::Triangle (geometry_2D.cpp:475-503): mean distance of the
+        // first two points to the circumcentre
+        g.r = (std::sqrt((bx - ux) * (bx - ux) + (by - uy) * (by - uy)) +
+               std::sqrt((ax - ux) * (ax - ux) + (ay - uy) * (ay - uy))) / 2.;
+        g.meas = 0.5 * std::abs((bx - ax) * (cy - ay) - (by - ay) * (cx - ax));
+    } else {
+        double u[3], v[3], w[3];
+        for (int d = 0; d < 3; d++) { u[d] = X[1][d] - X[0][d]; v[d] = X[2][d] - X[0][d]; w[d] = X[3][d] - X[0][d]; }
+        const double det = u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) + u[2] * (v[0] * w[1] - v[1] * w[0]);
+        // circumcentre relative to X0: solve 2 [u; v; w] x = [|u|^2, |v|^2, |w|^2]
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2], vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2],
+                     ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        double vxw[3] = {v[1] * w[2] - v[2] * w[1], v[2] * w[0] - v[0] * w[2], v[0] * w[1] - v[1] * w[0]};
+        double wxu[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
+        double uxv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        double x[3];
+        for (int d = 0; d < 3; d++) x[d] = (uu * vxw[d] + vv * wxu[d] + ww * uxv[d]) / (2. * det);
+        // Tetrahedron::Tetrahedron (geometry_3D.cpp:254-281): distance of the
+        // first point to the circumcentre
+        g.r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        g.meas = std::abs(det) / 6.;
+    }
+    return g;
+}
+
+double smoothing_kernel(int kernel, double rrn)
+{
+    if (kernel == 1) return std::exp(-rrn);
+    const double t = rrn / 2.;
+    return 1. - t >= 0. ? (t - 1.) * (t - 1.) : 0.;
+}
+
+} // namespace
+
+extern "C" {
+
+void or_smoothed_strain(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,
+                        const double *damage, double threshold, double radius, int kernel, const double *strain,
+                        double *out)
+{
+    Shape sh = shape_of(etype);
+    const int s = sh.dim == 2 ? 3 : 6;
+    std::vector<ElemGeo> geo(nelem);
+    for (int64_t e = 0; e < nelem; e++) geo[e] = elem_geo(sh.dim, sh.nper, conn + e * sh.nper, coords);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t e = 0; e < nelem; e++) {
+        const ElemGeo &ge = geo[e];
+        const double R = std::max(radius, ge.r * 1.1);
+        const double rho = R * 2.5 + ge.r * 1.5;
+        std::vector<double> acc(s, 0.);
+        double sum = 0.;
+        for (int64_t i = 0; i < nelem; i++) {
+            if (phase && phase[i] != phase[e]) continue;
+            if (damage && damage[i] >= threshold) continue;
+            const ElemGeo &gi = geo[i];
+            double d2 = 0.;
+            for (int d = 0; d < sh.dim; d++) d2 += (gi.c[d] - ge.c[d]) * (gi.c[d] - ge.c[d]);
+            const bool in = sh.dim == 2 ? d2 < rho * rho : d2 < rho * rho + 2. * rho * 1e-12 + 1e-24;
+            if (!in) continue;
+            double coef = 0.;
+            for (int v = 0; v < sh.nper; v++) {
+                const double *X = coords + (size_t)conn[i * sh.nper + v] * sh.dim;
+                double r2 = 0.;
+                for (int d = 0; d < sh.dim; d++) r2 += (X[d] - ge.c[d]) * (X[d] - ge.c[d]);
+                coef += std::max(smoothing_kernel(kernel, r2 / (R * R)) * gi.meas, 0.);
+            }
+            coef /= sh.nper;
+            for (int q = 0; q < s; q++) acc[q] += coef * strain[i * s + q];
+            sum += coef;
+        }
+        for (int q = 0; q < s; q++) out[e * s + q] = sum > 1e-12 ? acc[q] / sum : 0.;
+    }
 }
 
 } // extern "C"

@@ -124,6 +124,28 @@ void or_average_stress(int etype, int64_t nelem, const int32_t *conn, const doub
                        const double *cmat, const int32_t *phase, const double *strain,
                        double *out, double *measure_out);
 
+/* Non-local (smoothed) element strains of linear elements (P1, Tet4): the
+ * reference's Mesh::getSmoothedFields, non-space-time branch
+ * (mesher/mesh.h:1384-1486), over the neighbourhoods and weights of
+ * Mesh::generateCache (mesher/mesh.h:546-640) as FractureCriterion::
+ * initialiseCache sets them up (physics/fracturecriteria/fracturecriterion.cpp:
+ * 171-190).  For element e (centroid c_e, circumradius r_e):
+ *   R_e = max(radius, 1.1 r_e); locus: circle / sphere around c_e of radius
+ *   2.5 R_e + 1.5 r_e (2.5 = getSmoothingKernelSize, :28-41), containing the
+ *   centroids of the neighbours (geometry_2D.cpp:1472-1484 strict, 3D with
+ *   the 1e-12 tolerance of geometry_3D.cpp:1551-1566);
+ *   kernel (:43-61) on rrn = |x - c_e|^2 / R_e^2: 0 QUARTIC_COMPACT
+ *   ((rrn/2 - 1)^2 where rrn/2 <= 1), 1 GAUSSIAN_NONCOMPACT (exp(-rrn));
+ *   weight of neighbour i (one Gauss point): mean over its vertices v of
+ *   max(w(v) * measure_i, 0);
+ *   neighbours of another phase (behaviour source) or fractured
+ *   (damage >= threshold, damagemodels/isotropiclineardamage.cpp:57-62)
+ *   left out; smoothed = sum w_i eps_i / sum w_i, 0 when sum w_i <= 1e-12.
+ * Brute force over all element pairs.  strain, out: nelem x 3|6. */
+void or_smoothed_strain(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,
+                        const double *damage, double threshold, double radius, int kernel, const double *strain,
+                        double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,7 @@ def lib():
         L.or_damage_update.argtypes = [i64, i32, vp, vp, dbl, dbl, dbl, vp, vp]
         L.or_damage_update.restype = i64
         L.or_average_stress.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.or_smoothed_strain.argtypes = [i32, i64, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
         _LIB = L
     return _LIB
 
@@ -297,3 +298,21 @@ def average_stress(etype, conn, coords, cmats, phase, strain):
     lib().or_average_stress(etype, conn.shape[0], _p(conn), _p(coords), _p(cmats), _p(phase), _p(strain),
                             _p(out), _p(vol))
     return out, float(vol[0])
+
+
+QUARTIC_COMPACT, GAUSSIAN_NONCOMPACT = 0, 1
+
+
+def smoothed_strain(etype, conn, coords, strain, radius, kernel=QUARTIC_COMPACT, phase=None, damage=None,
+                    threshold=1.0):
+    """Non-local element strains (linear elements): the reference's
+    getSmoothedFields over its generateCache neighbourhoods."""
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    strain = _c(strain, np.float64)
+    phase = _c(phase, np.int32)
+    damage = _c(damage, np.float64)
+    out = np.zeros_like(strain)
+    lib().or_smoothed_strain(etype, conn.shape[0], _p(conn), _p(coords), _p(phase), _p(damage), threshold, radius,
+                             kernel, _p(strain), _p(out))
+    return out

@@ -336,6 +336,54 @@ def test_stiffness_behaviour_fixture(ctx, n, precision):
     np.testing.assert_allclose(eps, np.tile([-nu * smax / E, smax / E, 0.], (len(eps), 1)), atol=1e-6 * smax / E)
 
 
+@pytest.mark.parametrize("case,kernel", [("rve", 0), ("rve", 1), ("tet", 0)])
+def test_smoothed_strain_vs_oracle(ctx, case, kernel):
+    """Non-local strains on the device against the oracle's brute force, from
+    the device's own local strains: a P1 RVE batch with two phases, and a
+    Tet4 block whose damage field leaves the elements above the threshold
+    out."""
+    print("case", case, kernel, flush=True)
+    if case == "rve":
+        probs = [configs.rve(n, 600 + n, 0.3) for n in (12, 20)]
+        radius, threshold, etype = 0.0377, 1.0, ag.P1
+    else:
+        probs = [configs.config3(6)]
+        radius, threshold, etype = 0.09, 0.25, ag.TET4
+    B = ag.Batch(ctx, probs)
+    assert B.solve(precision=ag.FP64, rtol=1e-10)["status"] == 0
+    B.set_nonlocal(radius, kernel, threshold)
+    for k, p in enumerate(probs):
+        d = B.damage(k)[0] if p.damage is not None else None
+        for rhs in range(p.nrhs):
+            eps = B.element_strain(k, rhs)
+            dev = B.smoothed_strain(k, rhs)
+            ora = orc.smoothed_strain(etype, p.conn, p.coords, eps, radius, kernel=kernel, phase=p.phase, damage=d,
+                                      threshold=threshold)
+            np.testing.assert_allclose(dev, ora, rtol=1e-12, atol=1e-12 * np.abs(ora).max())
+            if p.damage is not None:
+                assert (d >= threshold).any() and (d < threshold).any()
+
+
+def test_nonlocal_damage_step_vs_oracle(ctx):
+    """A damage step driven by the non-local strains: the oracle's law on the
+    oracle's smoothed strains of the device's local strains."""
+    probs = [configs.rve(n, 700 + n, 0.3) for n in (10, 16)]
+    B = ag.Batch(ctx, probs)
+    assert B.solve(precision=ag.FP64, rtol=1e-10)["status"] == 0
+    B.set_nonlocal(0.05, 0, 1.0)
+    before = [(B.element_strain(k, 0), B.damage(k)) for k in range(len(probs))]
+    B.damage_step(kappa0=1e-4, kappaf=1e-3)
+    for k, p in enumerate(probs):
+        eps, (d0, kap0) = before[k]
+        sm = orc.smoothed_strain(ag.P1, p.conn, p.coords, eps, 0.05, phase=p.phase, damage=d0, threshold=1.0)
+        d, kap = d0.copy(), kap0.copy()
+        orc.damage_update(sm, kap, d, kappa0=1e-4, kappaf=1e-3)
+        dg, kg = B.damage(k)
+        np.testing.assert_allclose(kg, kap, rtol=1e-12, atol=0)
+        np.testing.assert_allclose(dg, d, rtol=1e-12, atol=1e-15)
+        assert (dg > 0).any()
+
+
 def test_damage_step_vs_oracle(ctx):
     p = configs.config3(6, step=10)
     B = ag.Batch(ctx, [p])

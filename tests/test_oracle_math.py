@@ -362,3 +362,100 @@ def test_homogeneous_rve_reproduces_c():
         s, vol = orc.average_stress(P1, p.conn, p.coords, cm, p.phase, eps)
         ceff[lc] = s / 1e-3
     np.testing.assert_allclose(ceff, C, rtol=1e-12, atol=1e-12 * np.abs(C).max())
+
+
+# ---------------------------------------------------------------- non-local strains
+def _mesh(etype, n):
+    coords, conn = ag.structured_mesh(etype, n)
+    return coords, conn
+
+
+def _measures(etype, conn, coords):
+    X = coords[conn]
+    if etype == P1:
+        a, b, c = X[:, 0], X[:, 1], X[:, 2]
+        return 0.5 * np.abs((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+    u, v, w = X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]
+    return np.abs(np.einsum("ij,ij->i", u, np.cross(v, w))) / 6.
+
+
+@pytest.mark.parametrize("etype,n", [(P1, 12), (orc.TET4, 4)])
+def test_smoothed_strain_constant_per_phase(etype, n):
+    """Each phase's own constant field comes back: weights are normalised
+    and neighbours of the other phase are left out."""
+    coords, conn = _mesh(etype, n)
+    ne = conn.shape[0]
+    s = 3 if etype == P1 else 6
+    phase = (np.arange(ne) % 3 == 0).astype(np.int32)
+    vals = np.array([np.linspace(1, 2, s), np.linspace(-3, 5, s)]) * 1e-4
+    eps = vals[phase]
+    out = orc.smoothed_strain(etype, conn, coords, eps, 0.05, phase=phase)
+    np.testing.assert_allclose(out, eps, rtol=1e-14, atol=1e-19)
+
+
+@pytest.mark.parametrize("kernel", [orc.QUARTIC_COMPACT, orc.GAUSSIAN_NONCOMPACT])
+@pytest.mark.parametrize("etype,n", [(P1, 10), (orc.TET4, 3)])
+def test_smoothed_strain_wide_kernel_is_the_phase_average(etype, n, kernel):
+    """A radius far above the mesh takes every same-phase element with a
+    weight that tends to its measure: the measure-weighted phase average."""
+    coords, conn = _mesh(etype, n)
+    ne = conn.shape[0]
+    s = 3 if etype == P1 else 6
+    rng = np.random.default_rng(7)
+    eps = rng.normal(size=(ne, s))
+    phase = (rng.random(ne) < 0.3).astype(np.int32)
+    out = orc.smoothed_strain(etype, conn, coords, eps, 1e3, kernel=kernel, phase=phase)
+    meas = _measures(etype, conn, coords)
+    for ph in (0, 1):
+        m = phase == ph
+        avg = (meas[m, None] * eps[m]).sum(0) / meas[m].sum()
+        np.testing.assert_allclose(out[m], np.tile(avg, (m.sum(), 1)), rtol=0, atol=1e-5 * np.abs(eps).max())
+
+
+@pytest.mark.parametrize("etype,n", [(P1, 6), (orc.TET4, 2)])
+def test_smoothed_strain_lone_elements(etype, n):
+    """An element with no same-phase neighbour keeps its own strain (its own
+    weight is positive: its vertices lie within R = 1.1 r of the centroid)."""
+    coords, conn = _mesh(etype, n)
+    ne = conn.shape[0]
+    s = 3 if etype == P1 else 6
+    eps = np.random.default_rng(3).normal(size=(ne, s))
+    out = orc.smoothed_strain(etype, conn, coords, eps, 0.01, phase=np.arange(ne, dtype=np.int32))
+    np.testing.assert_allclose(out, eps, rtol=4e-16, atol=0)   # w * eps / w: one rounding
+
+
+def test_smoothed_strain_fractured_as_excluded_phase():
+    """Leaving fractured elements (damage >= threshold) out is the same as
+    putting them in a phase of their own, for every other element."""
+    coords, conn = _mesh(P1, 10)
+    ne = conn.shape[0]
+    rng = np.random.default_rng(11)
+    eps = rng.normal(size=(ne, 3))
+    frac = rng.random(ne) < 0.2
+    d = np.where(frac, 0.95, 0.1)
+    a = orc.smoothed_strain(P1, conn, coords, eps, 0.12, damage=d, threshold=0.9)
+    b = orc.smoothed_strain(P1, conn, coords, eps, 0.12, phase=frac.astype(np.int32))
+    np.testing.assert_allclose(a[~frac], b[~frac], rtol=1e-14, atol=0)
+
+
+def test_smoothed_strain_locus_radius():
+    """Two triangles: the second counts exactly when its centroid lies within
+    2.5 R + 1.5 r of the first's, with r the circumradius (here from
+    r = abc / (4 area)) and R = max(radius, 1.1 r); with the Gaussian kernel
+    its weight is positive, so the inclusion shows."""
+    tri = np.array([[0., 0.], [1., 0.], [0.3, 0.8]])
+    a, b, c = (np.linalg.norm(tri[i] - tri[j]) for i, j in ((0, 1), (1, 2), (2, 0)))
+    area = 0.5 * abs((tri[1, 0] - tri[0, 0]) * (tri[2, 1] - tri[0, 1]) - (tri[1, 1] - tri[0, 1]) * (tri[2, 0] - tri[0, 0]))
+    r = a * b * c / (4. * area)
+    radius = 0.2
+    R = max(radius, 1.1 * r)
+    rho = 2.5 * R + 1.5 * r
+    cen = tri.mean(0)
+    for dist, inside in ((rho * (1 - 1e-9), True), (rho * (1 + 1e-9), False)):
+        shift = np.array([dist, 0.])
+        coords = np.vstack([tri, tri + shift])
+        conn = np.array([[0, 1, 2], [3, 4, 5]], dtype=np.int32)
+        eps = np.array([[0., 0., 0.], [1., 1., 1.]])
+        out = orc.smoothed_strain(P1, conn, coords, eps, radius, kernel=orc.GAUSSIAN_NONCOMPACT)
+        assert (out[0, 0] > 0) == inside, (dist, rho, out[0])
+        assert np.allclose(np.linalg.norm((tri + shift).mean(0) - cen), dist)

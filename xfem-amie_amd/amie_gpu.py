@@ -90,6 +90,8 @@ SIGNATURES = {
     "amie_gpu_batch_spmv": (c_i32, [vp, c_i32, c_i32, vp, vp]),
     "amie_gpu_batch_time_spmv": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp]),
     "amie_gpu_batch_autotune": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp, vp, vp]),
+    "amie_gpu_batch_set_nonlocal": (c_i32, [vp, c_dbl, c_i32, c_dbl]),
+    "amie_gpu_batch_smoothed_strain": (c_i32, [vp, c_i32, c_i32, vp]),
     "amie_gpu_batch_set_spmv_kernel": (c_i32, [vp, c_i32, c_i32, c_i32, c_i32, c_i32]),
     "amie_gpu_batch_element_strain": (c_i32, [vp, c_i32, c_i32, vp]),
     "amie_gpu_batch_average_stress": (c_i32, [vp, vp]),
@@ -416,6 +418,17 @@ class Batch:
         pr = self.problems[p]
         out = np.zeros((pr.conn.shape[0], VOIGT[self.etype]))
         check(lib().amie_gpu_batch_element_strain(self.h, p, rhs, _p(out)))
+        return out
+
+    def set_nonlocal(self, radius, kernel=0, threshold=1.0):
+        """Non-local damage driving strain (linear elements); radius <= 0
+        turns it off."""
+        check(lib().amie_gpu_batch_set_nonlocal(self.h, radius, kernel, threshold))
+
+    def smoothed_strain(self, p=0, rhs=0):
+        pr = self.problems[p]
+        out = np.zeros((pr.conn.shape[0], VOIGT[self.etype]))
+        check(lib().amie_gpu_batch_smoothed_strain(self.h, p, rhs, _p(out)))
         return out
 
     def average_stress(self):

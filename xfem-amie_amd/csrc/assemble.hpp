@@ -559,12 +559,17 @@ __device__ inline double mazars_equivalent(int dim, const double *s)
 
 template <int ET>
 __global__ __launch_bounds__(256) void k_damage(StrainArgs a, int n, const double *d0, double *kappa, double *d,
-                                               double k0, double kf, double dmax, unsigned long long *grown)
+                                               double k0, double kf, double dmax, unsigned long long *grown,
+                                               const double *eps_in)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double eps[6];
-    element_strain<ET>(a, e, 0, eps);
+    // eps_in: the non-local strains (else the element's own)
+    if (eps_in)
+        for (int q = 0; q < Elem<ET>::S; q++) eps[q] = eps_in[(int64_t)e * Elem<ET>::S + q];
+    else
+        element_strain<ET>(a, e, 0, eps);
     double eq = mazars_equivalent(Elem<ET>::DIM, eps);
     double kap = fmax(kappa[e], eq);
     kappa[e] = kap;
@@ -572,6 +577,96 @@ __global__ __launch_bounds__(256) void k_damage(StrainArgs a, int n, const doubl
     double dn = fmin(dmax, fmax(d0 ? d0[e] : 0., fmax(d[e], dl)));
     if (dn > d[e]) atomicAdd(grown, 1ull);
     d[e] = dn;
+}
+
+// Non-local (smoothed) strains of linear elements (P1, Tet4), restating
+// Mesh::getSmoothedFields (mesher/mesh.h:1384-1486, non-space-time branch)
+// over the neighbourhoods of Mesh::generateCache (mesher/mesh.h:546-640) set
+// up by FractureCriterion::initialiseCache
+// (physics/fracturecriteria/fracturecriterion.cpp:171-190): element e
+// averages the strains of the same-phase, unfractured elements whose
+// centroid lies within rho_e = 2.5 R_e + 1.5 r_e of its centroid (R_e =
+// max(radius, 1.1 r_e), r_e its circumradius), each weighted by the mean over
+// its vertices of max(w(|v - c_e|^2 / R_e^2) * measure, 0) with the
+// smoothing kernel w (fracturecriterion.cpp:43-61).  The candidates come
+// from a uniform grid of element centroids per problem (cell = the largest
+// rho of the problem).
+struct NonlocalArgs {
+    const double *geo;         // per element: cx, cy, cz, R, rho, measure, -, -
+    const int32_t *econn;
+    const double *coords;
+    const int32_t *ephase;
+    const double *edamage;     // nullable
+    double threshold;          // fractured: damage >= threshold
+    const int32_t *eprob;
+    const double *grid;        // per problem: origin x, y, z, cell size
+    const int32_t *gdim;       // per problem: cells along x, y, z, first cell
+    const int32_t *cell_start; // per cell (all problems): first entry in cell_elem; one more at the end
+    const int32_t *cell_elem;  // element ids, cell by cell
+    int kernel;                // 0 QUARTIC_COMPACT, 1 GAUSSIAN_NONCOMPACT
+};
+constexpr int GEO_STRIDE = 8;
+
+__device__ inline double smoothing_weight(int kernel, double rrn)
+{
+    if (kernel == 1) return exp(-rrn);
+    const double t = rrn / 2.;
+    return 1. - t >= 0. ? (t - 1.) * (t - 1.) : 0.;
+}
+
+template <int ET>
+__global__ __launch_bounds__(256) void k_smooth(NonlocalArgs a, const double *strain, int n, double *out)
+{
+    AMIE_NO_CONTRACT
+    using E = Elem<ET>;
+    constexpr int S = E::S, NP = E::NPER, D = E::DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double *ge = a.geo + (int64_t)e * GEO_STRIDE;
+    const double c[3] = {ge[0], ge[1], ge[2]};
+    const double R = ge[3], rho = ge[4];
+    const int p = a.eprob[e];
+    const int ph = a.ephase ? a.ephase[e] : 0;
+    const double *g = a.grid + (int64_t)p * 4;
+    const int *gd = a.gdim + (int64_t)p * 4;
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+        lo[d] = max(0, (int)floor((c[d] - rho - g[d]) / g[3]));
+        hi[d] = min(gd[d] - 1, (int)floor((c[d] + rho - g[d]) / g[3]));
+    }
+    double acc[S], sum = 0.;
+    for (int q = 0; q < S; q++) acc[q] = 0.;
+    for (int z = lo[2]; z <= hi[2]; z++)
+        for (int y = lo[1]; y <= hi[1]; y++)
+            for (int x = lo[0]; x <= hi[0]; x++) {
+                const int cell = gd[3] + (z * gd[1] + y) * gd[0] + x;
+                for (int j = a.cell_start[cell]; j < a.cell_start[cell + 1]; j++) {
+                    const int i = a.cell_elem[j];
+                    if (a.ephase && a.ephase[i] != ph) continue;
+                    if (a.edamage && a.edamage[i] >= a.threshold) continue;
+                    const double *gi = a.geo + (int64_t)i * GEO_STRIDE;
+                    double d2 = 0.;
+                    for (int d = 0; d < D; d++) d2 += (gi[d] - c[d]) * (gi[d] - c[d]);
+                    // Circle::in strict (geometry_2D.cpp:1472-1484); Sphere::in
+                    // with its 1e-12 tolerance (geometry_3D.cpp:1551-1566)
+                    const bool in = D == 2 ? d2 < rho * rho : d2 < rho * rho + 2. * rho * 1e-12 + 1e-24;
+                    if (!in) continue;
+                    double coef = 0.;
+                    for (int v = 0; v < NP; v++) {
+                        const int nd = a.econn[(int64_t)i * NP + v];
+                        double r2 = 0.;
+                        for (int d = 0; d < D; d++) {
+                            const double t = a.coords[(int64_t)nd * D + d] - c[d];
+                            r2 += t * t;
+                        }
+                        coef += fmax(smoothing_weight(a.kernel, r2 / (R * R)) * gi[5], 0.);
+                    }
+                    coef /= NP;
+                    for (int q = 0; q < S; q++) acc[q] += coef * strain[(int64_t)i * S + q];
+                    sum += coef;
+                }
+            }
+    for (int q = 0; q < S; q++) out[(int64_t)e * S + q] = sum > 1e-12 ? acc[q] / sum : 0.;
 }
 
 // Volume average of C_e (1 - d_e) eps_e per problem and right-hand side:

@@ -76,7 +76,8 @@ Batch::~Batch()
 
 size_t Batch::bytes() const
 {
-    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &slot_off, &dict_ids, &dict_table, &col, &dslot, &row_len, &v64,
+    const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &slot_off, &dict_ids, &dict_table,
+                           &egeo, &ngrid, &ngdim, &cell_start, &cell_elem, &col, &dslot, &row_len, &v64,
                            &v32, &vsp, &vfx, &fix48_bad, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
                            &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
@@ -1701,6 +1702,45 @@ int amie_gpu_batch_average_stress(amie_gpu_batch *bb, double *outp)
     return AMIE_OK;
 }
 
+static NonlocalArgs nonlocal_args(Batch &B)
+{
+    NonlocalArgs a{};
+    a.geo = B.egeo.as<double>();
+    a.econn = B.econn.as<int32_t>();
+    a.coords = B.coords.as<double>();
+    a.ephase = B.ephase.as<int32_t>();
+    a.edamage = B.edamage.p ? B.edamage.as<double>() : nullptr;
+    a.threshold = B.nl_threshold;
+    a.eprob = B.eprob.as<int32_t>();
+    a.grid = B.ngrid.as<double>();
+    a.gdim = B.ngdim.as<int32_t>();
+    a.cell_start = B.cell_start.as<int32_t>();
+    a.cell_elem = B.cell_elem.as<int32_t>();
+    a.kernel = B.nl_kernel;
+    return a;
+}
+
+// Non-local strains of every element for right-hand side rhs into out
+// (nelem x S); tmp receives the local strains.
+static int smoothed_all(Batch &B, int rhs, DevBuf &tmp, DevBuf &out)
+{
+    const int n = (int)B.nelem, S = B.et.s;
+    if (!tmp.alloc((size_t)n * S * 8) || !out.alloc((size_t)n * S * 8)) return AMIE_ERR_NOMEM;
+    StrainArgs s = strain_args(B);
+    NonlocalArgs a = nonlocal_args(B);
+    dim3 gr((n + 255) / 256), bl(256);
+    hipStream_t st = B.stream;
+    if (B.et.et == 1) {
+        hipLaunchKernelGGL(k_strain<1>, gr, bl, 0, st, s, 0, n, rhs, tmp.as<double>());
+        hipLaunchKernelGGL(k_smooth<1>, gr, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
+    } else {
+        hipLaunchKernelGGL(k_strain<3>, gr, bl, 0, st, s, 0, n, rhs, tmp.as<double>());
+        hipLaunchKernelGGL(k_smooth<3>, gr, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
+    }
+    AMIE_HIP(hipGetLastError());
+    return AMIE_OK;
+}
+
 int amie_gpu_batch_damage_step(amie_gpu_batch *bb, const amie_damage_params *prm, int64_t *n_grown)
 {
     if (!bb || !bb->b || !prm) return AMIE_ERR_ARG;
@@ -1711,9 +1751,13 @@ int amie_gpu_batch_damage_step(amie_gpu_batch *bb, const amie_damage_params *prm
     StrainArgs s = strain_args(B);
     const int n = (int)B.nelem;
     dim3 gr((n + 255) / 256), bl(256);
+    DevBuf tmp, eps_nl;
+    if (B.nonlocal_on)
+        if (int rc = smoothed_all(B, 0, tmp, eps_nl)) return rc;
+    const double *eps_in = B.nonlocal_on ? eps_nl.as<double>() : nullptr;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, gr, bl, 0, st, s, n, B.ed0.as<double>(), B.ekappa.as<double>(), B.edamage.as<double>(),
-                           prm->kappa0, prm->kappaf, prm->dmax, B.grown.as<unsigned long long>());
+                           prm->kappa0, prm->kappaf, prm->dmax, B.grown.as<unsigned long long>(), eps_in);
     };
     if (B.et.et == 1) go(k_damage<1>);
     else if (B.et.et == 2) go(k_damage<2>);
@@ -1725,6 +1769,147 @@ int amie_gpu_batch_damage_step(amie_gpu_batch *bb, const amie_damage_params *prm
     if (n_grown) *n_grown = (int64_t)g;
     B.has_damage_state = true;
     return B.assemble();
+}
+
+// Element geometry for the non-local search: centroid, circumradius (the
+// reference's Triangle / Tetrahedron constructors, geometry_2D.cpp:475-503,
+// geometry_3D.cpp:254-281), measure.
+static void nonlocal_geometry(int dim, int nper, const double *X /*nper x dim*/, double *c, double &r, double &meas)
+{
+    for (int d = 0; d < 3; d++) {
+        double s = 0.;
+        if (d < dim)
+            for (int i = 0; i < nper; i++) s += X[i * dim + d];
+        c[d] = d < dim ? s / nper : 0.;
+    }
+    if (dim == 2) {
+        const double ax = X[0], ay = X[1], bx = X[2], by = X[3], cx = X[4], cy = X[5];
+        const double dd = 2. * (ax * (by - cy) + bx * (cy - ay) + cx * (ay - by));
+        const double a2 = ax * ax + ay * ay, b2 = bx * bx + by * by, c2 = cx * cx + cy * cy;
+        const double ux = (a2 * (by - cy) + b2 * (cy - ay) + c2 * (ay - by)) / dd;
+        const double uy = (a2 * (cx - bx) + b2 * (ax - cx) + c2 * (bx - ax)) / dd;
+        r = (std::sqrt((bx - ux) * (bx - ux) + (by - uy) * (by - uy)) +
+             std::sqrt((ax - ux) * (ax - ux) + (ay - uy) * (ay - uy))) / 2.;
+        meas = 0.5 * std::fabs((bx - ax) * (cy - ay) - (by - ay) * (cx - ax));
+    } else {
+        double u[3], v[3], w[3];
+        for (int d = 0; d < 3; d++) { u[d] = X[3 + d] - X[d]; v[d] = X[6 + d] - X[d]; w[d] = X[9 + d] - X[d]; }
+        const double det = u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) + u[2] * (v[0] * w[1] - v[1] * w[0]);
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2], vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2],
+                     ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        const double vxw[3] = {v[1] * w[2] - v[2] * w[1], v[2] * w[0] - v[0] * w[2], v[0] * w[1] - v[1] * w[0]};
+        const double wxu[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
+        const double uxv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        double x[3];
+        for (int d = 0; d < 3; d++) x[d] = (uu * vxw[d] + vv * wxu[d] + ww * uxv[d]) / (2. * det);
+        r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        meas = std::fabs(det) / 6.;
+    }
+}
+
+int amie_gpu_batch_set_nonlocal(amie_gpu_batch *bb, double radius, int32_t kernel, double threshold)
+{
+    if (!bb || !bb->b || (kernel != 0 && kernel != 1)) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    Batch &B = *bb->b;
+    if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
+    if (!(radius > 0.)) {
+        B.nonlocal_on = false;
+        return AMIE_OK;
+    }
+    if (B.et.et == 2) { set_error("non-local smoothing: linear elements (P1, Tet4) only"); return AMIE_ERR_ARG; }
+    const int D = B.et.dim, NP = B.et.nper;
+    const int64_t ne = B.nelem;
+    std::vector<double> xy((size_t)B.nnodes_geo * D);
+    std::vector<int32_t> conn((size_t)ne * NP), eprob(ne);
+    AMIE_HIP(hipMemcpy(xy.data(), B.coords.p, xy.size() * 8, hipMemcpyDeviceToHost));
+    AMIE_HIP(hipMemcpy(conn.data(), B.econn.p, conn.size() * 4, hipMemcpyDeviceToHost));
+    AMIE_HIP(hipMemcpy(eprob.data(), B.eprob.p, eprob.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<double> geo((size_t)ne * GEO_STRIDE, 0.);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+        double X[12];
+        for (int i = 0; i < NP; i++)
+            for (int d = 0; d < D; d++) X[i * D + d] = xy[(size_t)conn[e * NP + i] * D + d];
+        double *g = &geo[(size_t)e * GEO_STRIDE];
+        double r, meas;
+        nonlocal_geometry(D, NP, X, g, r, meas);
+        g[3] = std::max(radius, r * 1.1);   // R_e
+        g[4] = g[3] * 2.5 + r * 1.5;        // rho_e
+        g[5] = meas;
+    }
+    // per problem: a grid of cells of the largest rho over the centroids
+    std::vector<double> grid((size_t)B.nprob * 4);
+    std::vector<int32_t> gdim((size_t)B.nprob * 4);
+    int64_t ncells = 0;
+    for (int p = 0; p < B.nprob; p++) {
+        const ProbHost &P = B.ph[p];
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, h = 0.;
+        for (int d = 0; d < 3; d++) { lo[d] = DBL_MAX; hi[d] = -DBL_MAX; }
+        for (int64_t e = P.elem0; e < P.elem0 + P.nelem; e++) {
+            const double *g = &geo[(size_t)e * GEO_STRIDE];
+            for (int d = 0; d < D; d++) { lo[d] = std::min(lo[d], g[d]); hi[d] = std::max(hi[d], g[d]); }
+            h = std::max(h, g[4]);
+        }
+        if (P.nelem == 0) { for (int d = 0; d < 3; d++) lo[d] = hi[d] = 0.; h = 1.; }
+        int64_t nc;
+        int dims[3];
+        for (;;) {   // at most 2^24 cells per problem
+            nc = 1;
+            for (int d = 0; d < 3; d++) {
+                dims[d] = d < D ? (int)std::floor((hi[d] - lo[d]) / h) + 1 : 1;
+                nc *= dims[d];
+            }
+            if (nc <= (1 << 24)) break;
+            h *= 2.;
+        }
+        for (int d = 0; d < 3; d++) grid[p * 4 + d] = d < D ? lo[d] : 0.;
+        grid[p * 4 + 3] = h;
+        for (int d = 0; d < 3; d++) gdim[p * 4 + d] = dims[d];
+        gdim[p * 4 + 3] = (int32_t)ncells;
+        ncells += nc;
+    }
+    if (ncells >= INT32_MAX || ne >= INT32_MAX) { set_error("non-local grid too large"); return AMIE_ERR_ARG; }
+    std::vector<int32_t> ecell(ne), start(ncells + 1, 0), order(ne);
+    for (int64_t e = 0; e < ne; e++) {
+        const int p = eprob[e];
+        const double *g = &geo[(size_t)e * GEO_STRIDE];
+        int idx[3] = {0, 0, 0};
+        for (int d = 0; d < D; d++)
+            idx[d] = std::min(gdim[p * 4 + d] - 1, std::max(0, (int)std::floor((g[d] - grid[p * 4 + d]) / grid[p * 4 + 3])));
+        ecell[e] = gdim[p * 4 + 3] + (idx[2] * gdim[p * 4 + 1] + idx[1]) * gdim[p * 4 + 0] + idx[0];
+        start[ecell[e] + 1]++;
+    }
+    for (int64_t c = 0; c < ncells; c++) start[c + 1] += start[c];
+    {
+        std::vector<int32_t> fill(start.begin(), start.end() - 1);
+        for (int64_t e = 0; e < ne; e++) order[fill[ecell[e]]++] = (int32_t)e;
+    }
+    int rc;
+    if ((rc = alloc_and_upload(B.egeo, geo.data(), geo.size() * 8))) return rc;
+    if ((rc = alloc_and_upload(B.ngrid, grid.data(), grid.size() * 8))) return rc;
+    if ((rc = alloc_and_upload(B.ngdim, gdim.data(), gdim.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.cell_start, start.data(), start.size() * 4))) return rc;
+    if ((rc = alloc_and_upload(B.cell_elem, order.data(), order.size() * 4))) return rc;
+    B.nl_radius = radius;
+    B.nl_kernel = kernel;
+    B.nl_threshold = threshold;
+    B.nonlocal_on = true;
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_smoothed_strain(amie_gpu_batch *bb, int32_t p, int32_t rhs, double *strain)
+{
+    if (!bb || check_p(bb->b, p) || !strain) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    if (!B.nonlocal_on) { set_error("no non-local smoothing set (amie_gpu_batch_set_nonlocal)"); return AMIE_ERR_STATE; }
+    if (rhs < 0 || rhs >= B.nrhs) { set_error("bad rhs"); return AMIE_ERR_ARG; }
+    DevBuf tmp, out;
+    if (int rc = smoothed_all(B, rhs, tmp, out)) return rc;
+    AMIE_HIP(hipStreamSynchronize(B.stream));
+    const ProbHost &P = B.ph[p];
+    AMIE_HIP(hipMemcpy(strain, out.as<double>() + (size_t)P.elem0 * B.et.s, (size_t)P.nelem * B.et.s * 8,
+                       hipMemcpyDeviceToHost));
+    return AMIE_OK;
 }
 
 int amie_gpu_batch_get_damage(amie_gpu_batch *bb, int32_t p, double *d, double *kappa)

@@ -143,6 +143,11 @@ struct Batch {
     // elements
     DevBuf coords, econn, erow, eslot, ephase, edamage, ekappa, ed0, eprob, cmat, cmat_off, macro, colour_elems,
         prob_e0, prob_ne, grown;
+    // non-local strain smoothing (amie_gpu_batch_set_nonlocal)
+    DevBuf egeo, ngrid, ngdim, cell_start, cell_elem;
+    bool nonlocal_on = false;
+    int nl_kernel = 0;
+    double nl_radius = 0., nl_threshold = 1.;
     std::vector<int32_t> h_active, h_active_prob;
     std::vector<ProbDev> h_prob;
     ProbDev *h_prob_pinned = nullptr;
