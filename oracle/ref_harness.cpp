@@ -20,6 +20,8 @@
 #include "elements/elements.h"
 #include "mesher/delaunay.h"
 #include "mesher/delaunay_3d.h"
+#include "physics/fracturecriteria/fracturecriterion.h"
+#include "polynomial/vm_function_extra.h"
 #include "physics/stiffness.h"
 #include "solvers/assembly.h"
 #include "solvers/conjugategradient.h"
@@ -32,6 +34,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -361,6 +364,95 @@ static void golden_system_tet(const std::string &tag, int n, double E, double nu
     npy_i64(tag + "_cg_info", {(int64_t)ok, (int64_t)cg.nit, 32});
 }
 
+namespace Amie {
+// physics/fracturecriteria/fracturecriterion.cpp:43-61 (not declared in a header)
+Function getSmoothingKernelFunction(SmoothingFunctionType type, Function &rrn);
+}
+
+// The coefficients of Mesh::generateCache are a protected member and
+// DelaunayTree is final: they are read through a pointer to member formed in
+// an explicit instantiation (where access is not checked).
+typedef Mesh<DelaunayTriangle, DelaunayTreeItem> TriMesh;
+struct CoefsTag {
+    typedef std::vector<std::vector<std::vector<double>>> TriMesh::*type;
+    friend type member(CoefsTag);
+};
+template <typename Tag, typename Tag::type M> struct Expose {
+    friend typename Tag::type member(Tag) { return M; }
+};
+template struct Expose<CoefsTag, &TriMesh::coefs>;
+
+// Non-local neighbourhoods and weights: Mesh::generateCache
+// (mesher/mesh.h:546-640) with the locus, radius and kernel that
+// FractureCriterion::initialiseCache uses (fracturecriterion.cpp:171-190), on
+// a Delaunay mesh of seeded points in the unit right triangle.  No element has
+// a behaviour, so with a non-null source none is skipped and every weight
+// comes from the smoothing kernel.
+static void golden_smoothing(const std::string &tag, int npts, double radius, SmoothingFunctionType type)
+{
+    TriElement father(LINEAR);
+    Point *c0 = new Point(0., 0.), *c1 = new Point(1., 0.), *c2 = new Point(0., 1.);
+    DelaunayTree tree(c0, c1, c2);
+    std::mt19937_64 rng(2024);
+    std::uniform_real_distribution<double> U(0.02, 0.98);
+    std::vector<Point *> pts = {c0, c1, c2};
+    while ((int)pts.size() < npts + 3) {
+        const double x = U(rng), y = U(rng);
+        if (x + y > 0.97) continue;
+        Point *p = new Point(x, y);
+        pts.push_back(p);
+        tree.insert(p, 0.);
+    }
+    std::vector<DelaunayTriangle *> tris = tree.getTriangles();
+    std::map<const Point *, int64_t> pid;
+    std::map<int, int64_t> pos;   // tree index -> position in tris
+    std::vector<double> xy;
+    std::vector<int64_t> conn;
+    for (size_t e = 0; e < tris.size(); e++) {
+        tris[e]->refresh(&father);
+        pos[tris[e]->index] = (int64_t)e;
+        for (int k = 0; k < 3; k++) {
+            const Point *p = &tris[e]->getBoundingPoint(k);
+            auto it = pid.find(p);
+            if (it == pid.end()) {
+                it = pid.emplace(p, (int64_t)pid.size()).first;
+                xy.push_back(p->getX());
+                xy.push_back(p->getY());
+            }
+            conn.push_back(it->second);
+        }
+    }
+    Circle source(1., 0., 0.);
+    std::vector<int64_t> ptr = {0}, idx;
+    std::vector<double> w;
+    for (size_t e = 0; e < tris.size(); e++) {
+        DelaunayTriangle *t = tris[e];
+        const double R = std::max(radius, t->getRadius() * 1.1);
+        Function x = Function("x") - t->getCenter().getX();
+        Function y = Function("y") - t->getCenter().getY();
+        Function rr = x * x + y * y;
+        Function rrn = rr / (R * R);
+        Function smooth = getSmoothingKernelFunction(type, rrn);
+        Circle locus(R * 2.5 + t->getRadius() * 1.5, t->getCenter());
+        const unsigned id = tree.generateCache(&locus, &source, smooth);
+        std::vector<int> &cache = tree.getCache(id);
+        std::vector<std::vector<double>> &cw = (static_cast<TriMesh &>(tree).*member(CoefsTag()))[id];
+        for (size_t k = 0; k < cache.size(); k++) {
+            idx.push_back(pos.at(cache[k]));
+            w.push_back(cw[k].at(0));
+        }
+        ptr.push_back((int64_t)idx.size());
+        cache.clear();
+        cw.clear();
+    }
+    npy_f64(tag + "_xy", xy, {xy.size() / 2, 2});
+    npy_i64(tag + "_conn", conn, {tris.size(), 3});
+    npy_i64(tag + "_ptr", ptr);
+    npy_i64(tag + "_idx", idx);
+    npy_f64(tag + "_w", w);
+    npy_f64(tag + "_radius", {radius});
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
@@ -382,6 +474,8 @@ int main(int argc, char **argv)
     golden_system_p1("sys_p1_12x10_mix", {12, 10, 1.0, 1.0, 12e9, 0.3, 60e9, 0.2}, 32, true);
     golden_system_tet("sys_tet_2", 2, 30e9, 0.2);
     golden_system_tet("sys_tet_3", 3, 30e9, 0.2);
+    golden_smoothing("smooth_quartic", 300, 0.05, QUARTIC_COMPACT);
+    golden_smoothing("smooth_gauss", 300, 0.05, GAUSSIAN_NONCOMPACT);
     printf("golden vectors written to %s\n", g_out.c_str());
     return 0;
 }

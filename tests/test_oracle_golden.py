@@ -179,3 +179,23 @@ def test_cg_reference_golden(tag):
     # expression path in the reference)
     xref = g(tag + "_cg_x")
     assert np.abs(x - xref).max() <= 1e-12 * np.abs(xref).max()
+
+
+@pytest.mark.parametrize("tag,kernel", [("smooth_quartic", orc.QUARTIC_COMPACT), ("smooth_gauss", orc.GAUSSIAN_NONCOMPACT)])
+def test_smoothed_strain_vs_reference_caches(tag, kernel):
+    """Non-local strains against the neighbourhoods and weights the compiled
+    reference's Mesh::generateCache builds (mesher/mesh.h:546-640, with the
+    locus and kernel of FractureCriterion::initialiseCache) on a Delaunay
+    mesh of 300 seeded points: sum w_i eps_i / sum w_i over the reference's
+    cache equals the oracle's smoothing for a random strain field."""
+    xy, conn = g(tag + "_xy"), g(tag + "_conn").astype(np.int32)
+    ptr, idx, w = g(tag + "_ptr"), g(tag + "_idx"), g(tag + "_w")
+    radius = float(g(tag + "_radius")[0])
+    eps = np.random.default_rng(5).normal(size=(conn.shape[0], 3))
+    ref = np.zeros_like(eps)
+    for e in range(conn.shape[0]):
+        sl = slice(ptr[e], ptr[e + 1])
+        ref[e] = (w[sl, None] * eps[idx[sl]]).sum(0) / w[sl].sum()
+    out = orc.smoothed_strain(orc.P1, conn, xy, eps, radius, kernel=kernel)
+    np.testing.assert_allclose(out, ref, rtol=1e-11, atol=1e-12)
+
