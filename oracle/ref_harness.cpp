@@ -474,8 +474,8 @@ int main(int argc, char **argv)
     golden_system_p1("sys_p1_12x10_mix", {12, 10, 1.0, 1.0, 12e9, 0.3, 60e9, 0.2}, 32, true);
     golden_system_tet("sys_tet_2", 2, 30e9, 0.2);
     golden_system_tet("sys_tet_3", 3, 30e9, 0.2);
-    golden_smoothing("smooth_quartic", 300, 0.05, QUARTIC_COMPACT);
-    golden_smoothing("smooth_gauss", 300, 0.05, GAUSSIAN_NONCOMPACT);
+    golden_smoothing("smooth_quartic", 150, 0.05, QUARTIC_COMPACT);
+    golden_smoothing("smooth_gauss", 150, 0.05, GAUSSIAN_NONCOMPACT);
     printf("golden vectors written to %s\n", g_out.c_str());
     return 0;
 }

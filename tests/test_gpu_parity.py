@@ -426,6 +426,40 @@ def test_bad_input_is_an_error(ctx):
         ag.Batch(ctx, [ag.Problem(ag.P1, coords, bad, ag.cauchy_green(1e10, 0.2)[None])])
 
 
+def test_fix48_refuses_a_matrix_outside_its_range(ctx):
+    """The 48-bit format holds a scaled SPD matrix (|Ks_ij| <= 1); a matrix
+    whose diagonal is negative (a negative modulus) is refused with an error,
+    not solved with clipped values; the value-indexed candidate too."""
+    coords, conn = ag.structured_mesh(ag.P1, 4)
+    p = ag.Problem(ag.P1, coords, conn, ag.cauchy_green(-1e10, 0.2)[None],
+                   bc_dofs=np.array([0, 1, 3], dtype=np.int64), bc_vals=np.array([0., 0., 1e-3]))
+    B = ag.Batch(ctx, [p])
+    with pytest.raises(ag.AmieError):
+        B.solve(precision=ag.MIXED_FIX48)
+    with pytest.raises(ag.AmieError):
+        B.set_spmv_kernel(ag.MIXED_FIX48, ag.JACOBI, 8, 0, 1)
+
+
+def test_iteration_cap_reports_not_converged(ctx):
+    p = configs.config2(10)
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=ag.MIXED_FIX48, rtol=1e-10, maxit=5)
+    assert st["status"] == 1 and st["nproblems_converged"] == 0
+    st = B.solve(precision=ag.MIXED_FIX48, rtol=1e-10)
+    assert st["status"] == 0
+
+
+def test_nan_in_the_input_is_reported(ctx):
+    coords, conn = ag.structured_mesh(ag.P1, 6)
+    f = np.zeros(2 * len(coords))
+    f[-1] = np.nan
+    p = ag.Problem(ag.P1, coords, conn, ag.cauchy_green(1e10, 0.2)[None],
+                   bc_dofs=np.array([0, 1, 3], dtype=np.int64), bc_vals=np.zeros(3), forces=f)
+    B = ag.Batch(ctx, [p])
+    st = B.solve(precision=ag.FP64)
+    assert st["status"] == -4, st   # AMIE_ERR_NAN: the reference exits the process instead
+
+
 @pytest.mark.parametrize("precision,tune", [(ag.MIXED, 0), (ag.MIXED_FIX48, 1)])
 def test_server_result_same_alone_and_in_mix(ctx, precision, tune):
     probs = [configs.rve(n, 500 + n, 0.3) for n in (8, 12, 16, 10)]

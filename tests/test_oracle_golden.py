@@ -186,7 +186,7 @@ def test_smoothed_strain_vs_reference_caches(tag, kernel):
     """Non-local strains against the neighbourhoods and weights the compiled
     reference's Mesh::generateCache builds (mesher/mesh.h:546-640, with the
     locus and kernel of FractureCriterion::initialiseCache) on a Delaunay
-    mesh of 300 seeded points: sum w_i eps_i / sum w_i over the reference's
+    mesh of 150 seeded points: sum w_i eps_i / sum w_i over the reference's
     cache equals the oracle's smoothing for a random strain field."""
     xy, conn = g(tag + "_xy"), g(tag + "_conn").astype(np.int32)
     ptr, idx, w = g(tag + "_ptr"), g(tag + "_idx"), g(tag + "_w")
