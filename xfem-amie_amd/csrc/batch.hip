@@ -89,6 +89,21 @@ size_t Batch::bytes() const
     return s;
 }
 
+// The value-indexed matrix is an option: when its build runs out of device
+// memory (it needs 32 bytes a slot while it runs) the batch keeps the value
+// planes instead of failing.
+int build_dict_or_drop(Batch &B)
+{
+    const int rc = build_dict(B);
+    if (rc == AMIE_ERR_NOMEM) {
+        (void)hipGetLastError();
+        B.dict_ok = false;
+        B.dict_wanted = false;
+        return AMIE_OK;
+    }
+    return rc;
+}
+
 SolveArgs Batch::args(int mode) const
 {
     const bool m32 = mode == AMIE_MIXED || mode == AMIE_MIXED64;      // FP32 matrix planes
@@ -678,7 +693,7 @@ int Batch::assemble()
     AMIE_HIP(hipMemcpyAsync(&bad, fix48_bad.p, 4, hipMemcpyDeviceToHost, st));
     AMIE_HIP(hipStreamSynchronize(st));
     fix48_ok = bad == 0;
-    if (dict_wanted) return build_dict(*this);
+    if (dict_wanted) return build_dict_or_drop(*this);
     return AMIE_OK;
 }
 
@@ -1540,7 +1555,7 @@ int Batch::autotune(int mode, bool bj, int iters)
     if (mode == AMIE_MIXED_FIX48 && !dict_wanted) {
         // from now on every assembly also builds the value-indexed matrix
         dict_wanted = true;
-        const int rc = build_dict(*this);
+        const int rc = build_dict_or_drop(*this);
         if (rc) return rc;
     }
     K1Choice best;
@@ -1625,7 +1640,7 @@ int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *bb, int32_t precision, int32_
         }
         if (!B.dict_wanted) {
             B.dict_wanted = true;
-            if (int rc = build_dict(B)) return rc;
+            if (int rc = build_dict_or_drop(B)) return rc;
         }
         if (!B.dict_ok) {
             set_error("no value-indexed matrix: more than %d distinct blocks, or FIX48 out of range", DICT_MAX);

@@ -182,6 +182,7 @@ struct Batch {
 // Builds the value-indexed FIX48 matrix of the current assembly (dict_ok
 // tells whether it exists: at most DICT_MAX distinct blocks).
 int build_dict(Batch &B);
+int build_dict_or_drop(Batch &B);   // a build out of device memory leaves the batch without one
 
 // FP64 plane vector of problem p <-> original DOF order, one column of
 // nrows*b entries per right-hand side.
