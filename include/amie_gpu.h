@@ -71,6 +71,7 @@ enum { AMIE_P1 = 1, AMIE_P2 = 2, AMIE_TET4 = 3 };
 enum { AMIE_FP64 = 0, AMIE_MIXED = 1, AMIE_MIXED64 = 2, AMIE_MIXED_K64 = 3, AMIE_MIXED_SPLIT = 4, AMIE_MIXED_FIX48 = 5 };
 enum { AMIE_JACOBI = 0, AMIE_BLOCK_JACOBI = 1 };      /* preconditioner */
 enum { AMIE_REFINE_REPLACE = 0, AMIE_REFINE_RESTART = 1 };
+enum { AMIE_ASSEMBLE_GATHER = 0, AMIE_ASSEMBLE_COLOURED = 1 };  /* amie_gpu_batch_set_assembly */
 
 typedef struct amie_gpu_ctx amie_gpu_ctx;
 typedef struct amie_gpu_batch amie_gpu_batch;
@@ -215,9 +216,18 @@ int amie_gpu_batch_from_bcsr(amie_gpu_ctx *ctx, int32_t stride, int64_t nrows, c
 void amie_gpu_batch_destroy(amie_gpu_batch *b);
 /* HBM bytes held by the batch. */
 size_t amie_gpu_batch_bytes(const amie_gpu_batch *b);
-/* Element integration + coloured scatter + Dirichlet elimination + scaling,
- * all on the device (re-run after damage changes). */
+/* Element integration + scatter + Dirichlet elimination + scaling, all on
+ * the device (re-run after damage changes). */
 int amie_gpu_batch_assemble(amie_gpu_batch *b);
+/* Scatter method of the assembly (solvers/assembly.cpp:657-735 is the serial
+ * loop both replace), then re-assembles.  AMIE_ASSEMBLE_GATHER (default): one
+ * thread per block, its element contributions added in the reference's order
+ * with its Kahan compensation; every block written once.  AMIE_ASSEMBLE_COLOURED:
+ * one launch per element colour, each element adding its blocks (no two
+ * elements of a colour share a block); the sums are ordered by colour.  Both
+ * are deterministic; the gather needs fewer than 2^32 element blocks in the
+ * batch. */
+int amie_gpu_batch_set_assembly(amie_gpu_batch *b, int32_t method);
 int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_solve_stats *stats);
 /* Outcome of problem p's last solve: status (AMIE_OK, AMIE_NOT_CONVERGED,
  * AMIE_ERR_NAN), iterations, refinements and the largest final relative

@@ -3,9 +3,10 @@ itself pinned to the compiled reference, tests/test_oracle_golden.py) and
 against the reference's golden vectors directly.
 
 Tolerances: pattern, element->block maps and DOF maps are compared bit for
-bit; FP64 matrices and right-hand sides to 1e-14 of their largest entry
-(summation order differs from the reference's serial Kahan scatter); the
-FP64 SpMV to 1e-13; the scaled FP32 SpMV to 2e-6; solutions through the
+bit; FP64 matrices of the default (gather) assembly bit for bit where the
+element matrices are (P1, Tet4), and matrices and right-hand sides otherwise
+to 1e-14 of their largest entry (the coloured scatter sums in colour order,
+not the reference's serial Kahan order); the FP64 SpMV to 1e-13; the scaled FP32 SpMV to 2e-6; solutions through the
 relative residual they reach plus the displacement error against the FP64
 oracle solution.
 """
@@ -53,16 +54,25 @@ def golden_problem(tag):
 SYSTEMS = ["sys_p1_4x3", "sys_p1_8x8", "sys_p1_12x10_mix", "sys_tet_2", "sys_tet_3"]
 
 
+@pytest.mark.parametrize("method", [ag.ASSEMBLE_GATHER, ag.ASSEMBLE_COLOURED])
 @pytest.mark.parametrize("tag", SYSTEMS)
-def test_assembly_matches_reference_golden(ctx, tag):
-    print("case", tag, flush=True)
+def test_assembly_matches_reference_golden(ctx, tag, method):
+    """The gather assembly adds each block's contributions in the reference's
+    order with its Kahan compensation, so its matrix is the reference's own,
+    bit for bit; the coloured scatter sums in colour order (to 1e-14)."""
+    print("case", tag, method, flush=True)
     B = ag.Batch(ctx, [golden_problem(tag)])
+    if method != ag.ASSEMBLE_GATHER:
+        B.set_assembly(method)
     rs, ci, ar, rhs = B.matrix()
     np.testing.assert_array_equal(rs, g(tag + "_row_size"))
     np.testing.assert_array_equal(ci, g(tag + "_column_index"))
     ref = g(tag + "_array")
-    assert np.abs(ar - ref).max() <= 1e-14 * np.abs(ref).max()
     fref = g(tag + "_forces")
+    if method == ag.ASSEMBLE_GATHER:
+        np.testing.assert_array_equal(ar, ref)
+        np.testing.assert_array_equal(rhs, fref)
+    assert np.abs(ar - ref).max() <= 1e-14 * np.abs(ref).max()
     assert np.abs(rhs - fref).max() <= 1e-13 * np.abs(fref).max()
 
 
@@ -101,18 +111,28 @@ def test_periodic_block_map_bit_exact(ctx):
     np.testing.assert_array_equal(B.element_blocks(), element_blocks_oracle(ag.P1, p.conn, p.dofmap))
 
 
+@pytest.mark.parametrize("method", [ag.ASSEMBLE_GATHER, ag.ASSEMBLE_COLOURED])
 @pytest.mark.parametrize("make", ["p1", "p2", "tet"])
-def test_assembly_vs_oracle(ctx, make):
-    print("case", make, flush=True)
+def test_assembly_vs_oracle(ctx, make, method):
+    """Damaged Tet4, two-phase P1 and P2 against the oracle's serial Kahan
+    assembly: bit for bit with the gather for P1 and Tet4 (whose element
+    matrices are bit-exact); P2 element matrices differ by an ulp, so P2 and
+    the coloured scatter are held to 1e-14."""
+    print("case", make, method, flush=True)
     p = {"p1": lambda: configs.config1(16), "p2": lambda: configs.config2(10),
          "tet": lambda: configs.config3(5)}[make]()
     B = ag.Batch(ctx, [p])
+    if method != ag.ASSEMBLE_GATHER:
+        B.set_assembly(method)
     rs, ci, ar, rhs = B.matrix()
     A = orc.assemble(p.etype, p.conn, p.coords, p.cmats, p.phase, p.damage)
     b = np.zeros(A.n)
     orc.apply_dirichlet(A, b, p.bc_dofs, p.bc_vals)
     np.testing.assert_array_equal(rs, A.row_size)
     np.testing.assert_array_equal(ci, A.column_index)
+    if method == ag.ASSEMBLE_GATHER and make != "p2":
+        np.testing.assert_array_equal(ar, A.array)
+        np.testing.assert_array_equal(rhs, b)
     err = np.abs(ar - A.array)
     if err.max() > 1e-14 * np.abs(A.array).max():
         b2 = p.coords.shape[1] * (p.coords.shape[1] + p.coords.shape[1] % 2)
@@ -259,6 +279,34 @@ def test_reference_absolute_stop(ctx, precision):
     assert meas >= 1e-4 * eps
     xref = g(tag + "_cg_x")
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
+
+
+@pytest.mark.parametrize("method", [ag.ASSEMBLE_GATHER, ag.ASSEMBLE_COLOURED])
+def test_rve_macro_load_vs_oracle(ctx, method):
+    """Periodic RVE batch: matrices and the three macro-strain loads of each
+    problem against the oracle, bit for bit with the gather (loads summed per
+    row in element order, each element row over its DOFs in order), to 1e-14
+    with the coloured scatter."""
+    probs = [configs.rve(n, 100 + n, f) for n, f in [(8, 0.25), (12, 0.3), (16, 0.4)]]
+    B = ag.Batch(ctx, probs)
+    if method != ag.ASSEMBLE_GATHER:
+        B.set_assembly(method)
+    for k, p in enumerate(probs):
+        print("case", method, k, flush=True)
+        rs, ci, ar, rhs = B.matrix(k)
+        A = orc.assemble_periodic(ag.P1, p.conn, p.dofmap, p.coords, p.cmats, p.phase)
+        loads = [orc.macro_rhs(ag.P1, p.conn, p.dofmap, p.coords, p.cmats, p.phase, p.macro[lc], A.nrows)
+                 for lc in range(3)]
+        orc.apply_dirichlet(A, np.zeros(A.n), p.bc_dofs, p.bc_vals)
+        if method == ag.ASSEMBLE_GATHER:
+            np.testing.assert_array_equal(ar, A.array)
+        assert np.abs(ar - A.array).max() <= 1e-14 * np.abs(A.array).max()
+        for lc in range(3):
+            f = loads[lc]
+            f[p.bc_dofs] = p.bc_vals
+            if method == ag.ASSEMBLE_GATHER:
+                np.testing.assert_array_equal(rhs[lc], f)
+            assert np.abs(rhs[lc] - f).max() <= 1e-13 * np.abs(f).max()
 
 
 @pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT, ag.MIXED_FIX48])

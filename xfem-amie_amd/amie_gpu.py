@@ -16,6 +16,7 @@ P1, P2, TET4 = 1, 2, 3
 FP64, MIXED, MIXED64, MIXED_K64, MIXED_SPLIT, MIXED_FIX48 = 0, 1, 2, 3, 4, 5
 JACOBI, BLOCK_JACOBI = 0, 1
 REFINE_REPLACE, REFINE_RESTART = 0, 1
+ASSEMBLE_GATHER, ASSEMBLE_COLOURED = 0, 1
 NPER = {P1: 3, P2: 6, TET4: 4}
 DIM = {P1: 2, P2: 2, TET4: 3}
 VOIGT = {P1: 3, P2: 3, TET4: 6}
@@ -80,6 +81,7 @@ SIGNATURES = {
     "amie_gpu_batch_destroy": (None, [vp]),
     "amie_gpu_batch_bytes": (ctypes.c_size_t, [vp]),
     "amie_gpu_batch_assemble": (c_i32, [vp]),
+    "amie_gpu_batch_set_assembly": (c_i32, [vp, c_i32]),
     "amie_gpu_batch_solve": (c_i32, [vp, vp, vp]),
     "amie_gpu_batch_problem_info": (c_i32, [vp, c_i32, vp, vp, vp, vp]),
     "amie_gpu_batch_get_solution": (c_i32, [vp, c_i32, vp]),
@@ -344,6 +346,10 @@ class Batch:
 
     def assemble(self):
         check(lib().amie_gpu_batch_assemble(self.h))
+
+    def set_assembly(self, method):
+        """ASSEMBLE_GATHER (default) or ASSEMBLE_COLOURED; re-assembles."""
+        check(lib().amie_gpu_batch_set_assembly(self.h, method))
 
     def solve(self, **kw):
         o = default_opts(**kw)

@@ -9,9 +9,10 @@
 //   first Gauss point as a plain product scaled by w and later points added
 //   with Kahan compensation (polynomial/vm_base.cpp:2974-2989,
 //   utilities/matrixops.h:462-507), K_ji = K_ij^T (mesher/delaunay.cpp:2457-2468).
-// One thread integrates one element and adds its blocks straight into the
-// FP64 matrix; elements of one colour share no DOF node, so the adds need no
-// atomics and their order per block is fixed by the colour order.
+// Two scatters into the FP64 matrix, both without atomics: the gather (one
+// thread per block, its contributions in the reference's order) and the
+// coloured one (one thread per element adding its blocks; elements of one
+// colour share no DOF node, so the order per block is the colour order).
 #pragma once
 #include "device.hpp"
 
@@ -180,17 +181,18 @@ template <int ET> AMIE_HD inline void btcb(const double *gi, const double *gj, c
 
 // Dense element matrix (nper*b)^2, dof = local_node*b + axis: the exact
 // formulation the assembly kernel adds into the global matrix.
-template <int ET> AMIE_HD inline void element_block(const double (*G)[6][3], const double *w, const double *C, int i,
-                                                    int j, double *K)
+// gi[g], gj[g]: gradients of local nodes i and j at Gauss point g.
+template <int ET> AMIE_HD inline void element_block_pair(const double (*gi)[3], const double (*gj)[3], const double *w,
+                                                         const double *C, double *K)
 {
     AMIE_NO_CONTRACT
     using E = Elem<ET>;
     constexpr int B = E::B;
     double t[B * B], comp[B * B];
-    btcb<ET>(G[0][i], G[0][j], C, K);
+    btcb<ET>(gi[0], gj[0], C, K);
     for (int q = 0; q < B * B; q++) { K[q] *= w[0]; comp[q] = 0.; }
     for (int g = 1; g < E::NGP; g++) {
-        btcb<ET>(G[g][i], G[g][j], C, t);
+        btcb<ET>(gi[g], gj[g], C, t);
         for (int q = 0; q < B * B; q++) {
             double yk = t[q] * w[g] - comp[q];
             double tt = K[q] + yk;
@@ -198,6 +200,27 @@ template <int ET> AMIE_HD inline void element_block(const double (*G)[6][3], con
             K[q] = tt;
         }
     }
+}
+
+template <int ET> AMIE_HD inline void element_block(const double (*G)[6][3], const double *w, const double *C, int i,
+                                                    int j, double *K)
+{
+    using E = Elem<ET>;
+    double gi[E::NGP][3], gj[E::NGP][3];
+    for (int g = 0; g < E::NGP; g++)
+        for (int d = 0; d < 3; d++) { gi[g][d] = G[g][i][d]; gj[g][d] = G[g][j][d]; }
+    element_block_pair<ET>(gi, gj, w, C, K);
+}
+
+// Gradients of local node n (a run-time index) picked from G with
+// compile-time indexing only, so that G stays in registers.
+template <int ET> __device__ inline void pick_grads(const double (*G)[6][3], int n, double (*gn)[3])
+{
+    using E = Elem<ET>;
+    for (int m = 0; m < E::NPER; m++)
+        for (int g = 0; g < E::NGP; g++)
+            for (int d = 0; d < 3; d++)
+                if (m == n) gn[g][d] = G[g][m][d];
 }
 
 template <int ET> AMIE_HD inline void element_matrix(const double (*X)[3], const double *C, double *Ke)
@@ -328,6 +351,135 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a, const int32_t *elem
     }
 }
 
+// ------------------------------------------------- gather assembly
+// One thread per block slot: the slot's contributions, listed at batch build
+// as codes (e*NPER + i)*NPER + j (element e, block row node i, column node j)
+// in the reference's own order (solvers/assembly.cpp:657-735: elements in
+// order, and inside one element (j,j), then (j,k), (k,j) for k > j), are
+// recomputed and added with the same Kahan compensation, so each entry is
+// rounded exactly as the serial loop rounds it.  Every slot is written once
+// (padding slots get zeros): no read-modify-write, coalesced stores.
+template <int ET> __device__ inline void load_element(const AsmArgs &a, int e, double (*X)[3], double *C)
+{
+    using E = Elem<ET>;
+    constexpr int NP = E::NPER, S = E::S, D = E::DIM;
+    const int p = a.eprob[e];
+    for (int i = 0; i < NP; i++) {
+        const int nd = a.econn[(int64_t)e * NP + i];
+        for (int d = 0; d < D; d++) X[i][d] = a.coords[(int64_t)nd * D + d];
+        if (D == 2) X[i][2] = 0.;
+    }
+    const double *Cp = a.cmat + (int64_t)(a.cmat_off[p] + a.ephase[e]) * S * S;
+    const double f = a.edamage ? 1. - a.edamage[e] : 1.;
+    for (int k = 0; k < S * S; k++) C[k] = a.edamage ? Cp[k] * f : Cp[k];
+}
+
+template <int ET>
+__global__ __launch_bounds__(256) void k_assemble_gather(AsmArgs a, const uint32_t *cptr, const uint32_t *contrib)
+{
+    AMIE_NO_CONTRACT
+    using E = Elem<ET>;
+    constexpr int NP = E::NPER, B = E::B, S = E::S;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nslots) return;
+    double acc[B * B], comp[B * B];  // plane order: entry (r,c) at c*B + r
+    for (int q = 0; q < B * B; q++) { acc[q] = 0.; comp[q] = 0.; }
+    const uint32_t q1 = cptr[s + 1];
+    for (uint32_t q = cptr[s]; q < q1; q++) {
+        const uint32_t code = contrib[q];
+        const int e = (int)(code / (NP * NP));
+        const int i = (int)(code % (NP * NP)) / NP, j = (int)(code % NP);
+        double X[NP][3], C[S * S];
+        load_element<ET>(a, e, X, C);
+        double G[E::NGP][6][3], w[E::NGP];
+        element_grads<ET>(X, G, w);
+        double ga[E::NGP][3], gb[E::NGP][3];
+        pick_grads<ET>(G, i >= j ? i : j, ga);
+        pick_grads<ET>(G, i >= j ? j : i, gb);
+        double K[B * B];
+        element_block_pair<ET>(ga, gb, w, C, K);
+        for (int r = 0; r < B; r++)
+            for (int c = 0; c < B; c++) {
+                const double v = i >= j ? K[r * B + c] : K[c * B + r];
+                double &x = acc[c * B + r], &cc = comp[c * B + r];
+                const double y = v - cc;
+                const double t = x + y;
+                cc = (t - x) - y;
+                x = t;
+            }
+    }
+    double2 *p = reinterpret_cast<double2 *>(a.v64);
+    constexpr int NPL = (B * B) / 2;
+    for (int k = 0; k < NPL; k++) p[(int64_t)k * a.nslots + s] = make_double2(acc[2 * k], acc[2 * k + 1]);
+    if constexpr ((B * B) % 2) reinterpret_cast<double *>(p + (int64_t)NPL * a.nslots)[s] = acc[B * B - 1];
+}
+
+// Macro-strain right-hand side, one thread per internal block row: the row's
+// (element, local node) pairs in element order, f -= Ke(row, :) u_macro with
+// the element row summed over its DOFs in order (the oracle's restatement of
+// the homogenisation load).
+template <int ET, int NR>
+__global__ __launch_bounds__(256) void k_macro_rhs_gather(AsmArgs a, const uint32_t *rptr, const uint32_t *rlist,
+                                                          int64_t nrows)
+{
+    AMIE_NO_CONTRACT
+    using E = Elem<ET>;
+    constexpr int NP = E::NPER, B = E::B, S = E::S, D = E::DIM;
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const uint32_t q0 = rptr[row], q1 = rptr[row + 1];
+    if (q0 == q1) return;
+    double f[NR][B];
+    for (int k = 0; k < NR; k++)
+        for (int r = 0; r < B; r++) f[k][r] = a.b64[k * a.vplane + row * B + r];
+    for (uint32_t q = q0; q < q1; q++) {
+        const uint32_t code = rlist[q];
+        const int e = (int)(code / NP), i = (int)(code % NP);
+        double X[NP][3], C[S * S];
+        load_element<ET>(a, e, X, C);
+        double G[E::NGP][6][3], w[E::NGP];
+        element_grads<ET>(X, G, w);
+        const int p = a.eprob[e];
+        double U[NR][NP][B];
+        for (int k = 0; k < NR; k++) {
+            const double *m = a.macro + ((int64_t)p * NR + k) * S;
+            for (int n = 0; n < NP; n++) {
+                if constexpr (D == 2) {
+                    U[k][n][0] = m[0] * X[n][0] + 0.5 * m[2] * X[n][1];
+                    U[k][n][1] = 0.5 * m[2] * X[n][0] + m[1] * X[n][1];
+                } else {
+                    U[k][n][0] = m[0] * X[n][0] + 0.5 * m[5] * X[n][1] + 0.5 * m[4] * X[n][2];
+                    U[k][n][1] = 0.5 * m[5] * X[n][0] + m[1] * X[n][1] + 0.5 * m[3] * X[n][2];
+                    U[k][n][2] = 0.5 * m[4] * X[n][0] + 0.5 * m[3] * X[n][1] + m[2] * X[n][2];
+                }
+            }
+        }
+        double t[NR][B];
+        for (int k = 0; k < NR; k++)
+            for (int r = 0; r < B; r++) t[k][r] = 0.;
+        double gi[E::NGP][3];
+        pick_grads<ET>(G, i, gi);
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            double gj[E::NGP][3];
+            for (int g = 0; g < E::NGP; g++)
+                for (int d = 0; d < 3; d++) gj[g][d] = G[g][j][d];
+            double K[B * B];
+            if (i >= j) element_block_pair<ET>(gi, gj, w, C, K);
+            else element_block_pair<ET>(gj, gi, w, C, K);
+            for (int r = 0; r < B; r++)
+                for (int c = 0; c < B; c++) {
+                    const double v = i >= j ? K[r * B + c] : K[c * B + r];
+                    for (int k = 0; k < NR; k++) t[k][r] = t[k][r] + v * U[k][j][c];
+                }
+        }
+        for (int k = 0; k < NR; k++)
+            for (int r = 0; r < B; r++) f[k][r] = f[k][r] - t[k][r];
+    }
+    for (int k = 0; k < NR; k++)
+        for (int r = 0; r < B; r++) a.b64[k * a.vplane + row * B + r] = f[k][r];
+}
+
 // ------------------------------------------------- Dirichlet elimination
 // solvers/assembly.cpp:125-253 restated per block row: for a free row i,
 // b_i -= g_j A(i,j) and A(i,j) = 0 for every constrained column j (blocks in
@@ -336,6 +488,7 @@ template <int B, int NR>
 __global__ __launch_bounds__(WG) void k_apply_bc(SolveArgs a, double *v64, double *b64, const uint8_t *bcmask,
                                                  const double *bcval, const int32_t *row_len, int ntiles)
 {
+    AMIE_NO_CONTRACT  // b -= g A as the reference rounds it: a product, then a subtraction
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         if (wave >= a.tile_nslices[t]) continue;

@@ -83,7 +83,8 @@ size_t Batch::bytes() const
                            &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
                            &bcval, &coords,
                            &econn, &erow, &eslot, &ephase, &edamage, &ekappa, &ed0, &eprob, &cmat, &cmat_off,
-                           &macro, &colour_elems, &prob_e0, &prob_ne, &grown};
+                           &macro, &colour_elems, &prob_e0, &prob_ne, &grown, &slot_cptr, &slot_contrib,
+                           &row_cptr, &row_contrib};
     size_t s = 0;
     for (auto b : all) s += b->n;
     return s;
@@ -523,6 +524,36 @@ int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, B
         }
     }
     if (slot_fail >= 0) { set_error("internal: element block missing from pattern (problem %d)", slot_fail); return AMIE_ERR_STATE; }
+    // gather assembly lists, counting-sorted by slot (and by row for the
+    // macro load) with the reference's summation order kept inside each list
+    if ((uint64_t)ne * nper * nper < (1ull << 32)) {
+        std::vector<uint32_t> cptr(B.nslots + 1, 0), contrib((size_t)ne * nper * nper), fill;
+        for (uint32_t s : h_eslot) cptr[s + 1]++;
+        for (int64_t s = 0; s < B.nslots; s++) cptr[s + 1] += cptr[s];
+        fill.assign(cptr.begin(), cptr.end() - 1);
+        for (int64_t e = 0; e < ne; e++) {
+            const uint32_t *es = &h_eslot[(size_t)e * nper * nper];
+            auto put = [&](int i, int j) { contrib[fill[es[i * nper + j]]++] = (uint32_t)((e * nper + i) * nper + j); };
+            for (int j = 0; j < nper; j++) {
+                put(j, j);
+                for (int k = j + 1; k < nper; k++) { put(j, k); put(k, j); }
+            }
+        }
+        if ((rc = alloc_and_upload(B.slot_cptr, cptr.data(), cptr.size() * 4))) return rc;
+        if ((rc = alloc_and_upload(B.slot_contrib, contrib.data(), contrib.size() * 4))) return rc;
+        if (B.has_macro) {
+            std::vector<uint32_t> rptr(B.nrows_pad + 1, 0), rlist((size_t)ne * nper);
+            for (int32_t r : h_erow) rptr[r + 1]++;
+            for (int64_t r = 0; r < B.nrows_pad; r++) rptr[r + 1] += rptr[r];
+            fill.assign(rptr.begin(), rptr.end() - 1);
+            for (int64_t q = 0; q < ne * nper; q++) rlist[fill[h_erow[q]]++] = (uint32_t)q;
+            if ((rc = alloc_and_upload(B.row_cptr, rptr.data(), rptr.size() * 4))) return rc;
+            if ((rc = alloc_and_upload(B.row_contrib, rlist.data(), rlist.size() * 4))) return rc;
+        }
+        B.asm_method = AMIE_ASSEMBLE_GATHER;
+    } else {
+        B.asm_method = AMIE_ASSEMBLE_COLOURED;
+    }
     for (int p = 0; p < nprob; p++)
         for (int64_t e = 0; e < probs[p].nelem; e++) col_elems[tmp[p].colour[e]].push_back((int32_t)(B.ph[p].elem0 + e));
     std::vector<int32_t> h_colour_elems;
@@ -636,6 +667,15 @@ template <int ET, int NR> static void launch_assemble(Batch &B, const AsmArgs &a
     }
 }
 
+template <int ET, int NR> static void launch_assemble_gather(Batch &B, const AsmArgs &a)
+{
+    hipLaunchKernelGGL((k_assemble_gather<ET>), dim3((unsigned)((B.nslots + 255) / 256)), dim3(256), 0, B.stream, a,
+                       B.slot_cptr.as<uint32_t>(), B.slot_contrib.as<uint32_t>());
+    if (a.macro)
+        hipLaunchKernelGGL((k_macro_rhs_gather<ET, NR>), dim3((unsigned)((B.nrows_pad + 255) / 256)), dim3(256), 0,
+                           B.stream, a, B.row_cptr.as<uint32_t>(), B.row_contrib.as<uint32_t>(), B.nrows_pad);
+}
+
 template <int Bk> static void launch_post(Batch &B)
 {
     SolveArgs a = B.args(AMIE_MIXED);
@@ -664,7 +704,8 @@ int Batch::assemble()
 {
     hipStream_t st = stream;
     if (has_elements) {
-        AMIE_HIP(hipMemsetAsync(v64.p, 0, v64.n, st));
+        const bool gather = asm_method == AMIE_ASSEMBLE_GATHER;
+        if (!gather) AMIE_HIP(hipMemsetAsync(v64.p, 0, v64.n, st));
         AMIE_HIP(hipMemcpyAsync(b64.p, fext.p, b64.n, hipMemcpyDeviceToDevice, st));
         AsmArgs a{};
         a.coords = coords.as<double>();
@@ -681,9 +722,15 @@ int Batch::assemble()
         a.nslots = nslots;
         a.b64 = b64.as<double>();
         a.vplane = vplane();
-        if (et.et == 1) { if (nrhs == 1) launch_assemble<1, 1>(*this, a); else launch_assemble<1, 3>(*this, a); }
-        else if (et.et == 2) { if (nrhs == 1) launch_assemble<2, 1>(*this, a); else launch_assemble<2, 3>(*this, a); }
-        else { if (nrhs == 1) launch_assemble<3, 1>(*this, a); else launch_assemble<3, 3>(*this, a); }
+        if (gather) {
+            if (et.et == 1) { if (nrhs == 1) launch_assemble_gather<1, 1>(*this, a); else launch_assemble_gather<1, 3>(*this, a); }
+            else if (et.et == 2) { if (nrhs == 1) launch_assemble_gather<2, 1>(*this, a); else launch_assemble_gather<2, 3>(*this, a); }
+            else { if (nrhs == 1) launch_assemble_gather<3, 1>(*this, a); else launch_assemble_gather<3, 3>(*this, a); }
+        } else {
+            if (et.et == 1) { if (nrhs == 1) launch_assemble<1, 1>(*this, a); else launch_assemble<1, 3>(*this, a); }
+            else if (et.et == 2) { if (nrhs == 1) launch_assemble<2, 1>(*this, a); else launch_assemble<2, 3>(*this, a); }
+            else { if (nrhs == 1) launch_assemble<3, 1>(*this, a); else launch_assemble<3, 3>(*this, a); }
+        }
     }
     AMIE_HIP(hipMemsetAsync(fix48_bad.p, 0, 4, st));
     if (et.b == 2) launch_post<2>(*this);
@@ -1383,6 +1430,20 @@ int amie_gpu_batch_assemble(amie_gpu_batch *b)
 {
     if (!b || !b->b) { set_error("batch is NULL"); return AMIE_ERR_ARG; }
     return b->b->assemble();
+}
+
+int amie_gpu_batch_set_assembly(amie_gpu_batch *b, int32_t method)
+{
+    if (!b || !b->b) { set_error("batch is NULL"); return AMIE_ERR_ARG; }
+    Batch &B = *b->b;
+    if (method != AMIE_ASSEMBLE_GATHER && method != AMIE_ASSEMBLE_COLOURED) { set_error("unknown assembly method %d", method); return AMIE_ERR_ARG; }
+    if (!B.has_elements) { set_error("the batch holds an assembled matrix, not elements"); return AMIE_ERR_STATE; }
+    if (method == AMIE_ASSEMBLE_GATHER && !B.slot_cptr.p) {
+        set_error("more than 2^32 element blocks: only the coloured assembly is available");
+        return AMIE_ERR_ARG;
+    }
+    B.asm_method = method;
+    return B.assemble();
 }
 
 int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_solve_stats *stats)

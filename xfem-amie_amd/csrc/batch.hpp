@@ -143,6 +143,11 @@ struct Batch {
     // elements
     DevBuf coords, econn, erow, eslot, ephase, edamage, ekappa, ed0, eprob, cmat, cmat_off, macro, colour_elems,
         prob_e0, prob_ne, grown;
+    // gather assembly: per slot its contributions (e*nper + i)*nper + j, per
+    // internal row its (element, local node) pairs e*nper + i, in the
+    // reference's summation order
+    DevBuf slot_cptr, slot_contrib, row_cptr, row_contrib;
+    int asm_method = 0;               // AMIE_ASSEMBLE_GATHER | AMIE_ASSEMBLE_COLOURED
     // non-local strain smoothing (amie_gpu_batch_set_nonlocal)
     DevBuf egeo, ngrid, ngdim, cell_start, cell_elem;
     bool nonlocal_on = false;
