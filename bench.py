@@ -81,6 +81,48 @@ def cpu_baseline(problems, budget_s=15.0, rtol=1e-8):
                       f"oracle FP64 Jacobi PCG to rel. res {rtol:g}, {threads} OpenMP threads, assembly excluded"}
 
 
+def ingest_input(ag, n):
+    """Structured P1 mesh of 2 n^2 triangles, nodes relabelled and triangles
+    shuffled (seed 13), as a mesher's element list would arrive."""
+    coords, conn = ag.structured_mesh(ag.P1, n)
+    rng = np.random.default_rng(13)
+    perm = rng.permutation(coords.shape[0])
+    xy = np.empty_like(coords)
+    xy[perm] = coords
+    return xy, perm[conn][rng.permutation(conn.shape[0])].astype(np.int32)
+
+
+def ingest_leg(ag, ctx, n):
+    """Mesh input on the device: P2 mid-edge nodes (DelaunayTree::
+    addSharedNodes) then FeatureTree's first-touch renumbering of the P2 mesh,
+    device time with the inputs in HBM (best of 3)."""
+    xy, conn3 = ingest_input(ag, n)
+    best_mid, best_ren = float("inf"), float("inf")
+    for _ in range(3):
+        c6, mid, ms_mid = ag.add_midpoints_gpu(ctx, xy, conn3, timed=True)
+        ids, count, ms_ren = ag.renumber_gpu(ctx, c6, xy.shape[0] + mid.shape[0], timed=True)
+        best_mid, best_ren = min(best_mid, ms_mid), min(best_ren, ms_ren)
+    ntri = conn3.shape[0]
+    return {"workload": f"structured P1 {n}x{n} ({ntri} triangles, {xy.shape[0]} vertices), nodes relabelled and "
+                        f"triangles shuffled (seed 13): P2 mid-edge nodes, then first-touch renumbering of the "
+                        f"{count} P2 nodes",
+            "triangles": ntri, "p2_nodes": int(count), "add_midpoints_ms": best_mid, "renumber_ms": best_ren,
+            "triangles_per_s": ntri / ((best_mid + best_ren) * 1e-3)}
+
+
+def cpu_ingest(ag, n):
+    """The oracle's serial add_midpoints + renumber on the same input."""
+    import oracle as orc
+    xy, conn3 = ingest_input(ag, n)
+    t0 = time.perf_counter()
+    c6, mid = orc.add_midpoints(xy, conn3)
+    t1 = time.perf_counter()
+    orc.renumber(c6, xy.shape[0] + mid.shape[0])
+    t2 = time.perf_counter()
+    return {"add_midpoints_ms": 1e3 * (t1 - t0), "renumber_ms": 1e3 * (t2 - t1),
+            "triangles_per_s": conn3.shape[0] / (t2 - t0), "cores": 1, "kind": "port"}
+
+
 def serving(ag, ctx, reqs, precision, rtol, limit_s=90.0):
     """Replays a request trace against the server in wall-clock time and
     returns p50/p99 request latency (arrival to result) and solves/s.  The
@@ -187,6 +229,7 @@ def main():
                     help="serving legs as RATE:SECONDS (Poisson arrivals in req/s over an arrival window)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the P2 400^2 and Tet4 64^3 legs")
+    ap.add_argument("--ingest-n", type=int, default=2048, help="mesh-input leg: structured P1 n x n (0: skip)")
     ap.add_argument("--spmv-kernel", default="", help="STEPS,ORDER[,FORMAT]: set the batch's FIX48 SpMV candidate "
                                                       "instead of autotuning (profiling runs)")
     args = ap.parse_args()
@@ -282,6 +325,10 @@ def main():
         log(f"k_spmv from the FIX48 planes: {us_pl:.1f} us, {by_pl / 1e9:.3f} GB, {by_pl / us_pl / 1e3:.0f} GB/s")
     B.close()
     single = None if args.no_single else single_problem_legs(ag, configs, ctx, args.rtol, args.maxit)
+    ingest = None
+    if args.ingest_n > 0:
+        ingest = ingest_leg(ag, ctx, args.ingest_n)
+        log(f"mesh input: {ingest}")
     serve = None
     if not args.no_serve:
         serve = {"workload": f"served mix (seed 5): Poisson arrivals, 1/3 each 2D P1 RVE n~U{{64..512}} (3 load "
@@ -297,6 +344,9 @@ def main():
                 log(f"serving {rate:g} req/s {name}: {row[name]}")
             serve["rates"].append(row)
     cpu = None if args.no_cpu else cpu_baseline(problems, args.cpu_budget, args.rtol)
+    if ingest and not args.no_cpu:
+        ingest["cpu_baseline"] = cpu_ingest(ag, args.ingest_n)
+        log(f"mesh input, oracle: {ingest['cpu_baseline']}")
     out = {
         "metric": "RVE solves/s + p50/p99 latency @rel.res 1e-8; mixed vs FP64 CG; SpMV GB/s",
         "value": value,
@@ -338,6 +388,7 @@ def main():
         "accuracy": accuracy,
         "single_problems": single,
         "serving": serve,
+        "mesh_input": ingest,
     }
     print(json.dumps(out), flush=True)
 

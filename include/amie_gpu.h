@@ -203,6 +203,30 @@ void amie_cauchy_green(double E, double nu, int32_t dim, double *C);
 int amie_gpu_element_matrices(amie_gpu_ctx *ctx, int32_t elem_type, int64_t nelem, const double *coords,
                               const double *cmat, double *Ke);
 
+/* --------------------------------------------------- mesh input (device)
+ * Meshes as the reference's FeatureTree hands them over: elements in its
+ * mesh iteration order.  Host arrays in and out; device_ms (nullable) is the
+ * device time of the work with the inputs already in HBM (HIP events).
+ *
+ * FeatureTree::renumber (features/features.cpp:603-729): walking the
+ * elements in order, each node gets the next id the first time an element
+ * names it; ids[nnodes] (-1: named by no element), count = ids given. */
+int amie_gpu_renumber(amie_gpu_ctx *ctx, int64_t nnodes, int64_t nelem, int32_t nper, const int32_t *conn,
+                      int32_t *ids, int64_t *count, double *device_ms);
+/* DelaunayTree::addSharedNodes(1, 1, ...) (mesher/delaunay.cpp:260-355), the
+ * mid-edge nodes FeatureTree::stitch adds for QUADRATIC triangles: side s of
+ * triangle t (vertices conn3[t][s], conn3[t][(s+1)%3]) takes the mid-edge
+ * node an earlier triangle made on the same edge, else a new one at
+ * a*0.5 + b*0.5.  New nodes are numbered nnodes, nnodes + 1, ... as they are
+ * made; conn6[t] = [v0, m01, v1, m12, v2, m20]; mid_xy needs room for
+ * 3*ntri nodes (2 doubles each), nmid = nodes made.  Then amie_gpu_renumber
+ * over conn6 gives FeatureTree's final numbering.  Edges are identified by
+ * their end nodes (the reference compares positions, which is the same on a
+ * valid mesh); the projection of mid-edge nodes onto curved feature
+ * boundaries (FeatureTree::projectTrianglesOnBoundaries) is not applied. */
+int amie_gpu_add_midpoints(amie_gpu_ctx *ctx, int64_t nnodes, const double *xy, int64_t ntri, const int32_t *conn3,
+                           int32_t *conn6, double *mid_xy, int64_t *nmid, double *device_ms);
+
 /* ------------------------------------------------------------- batches
  * A batch holds one or more problems resident in HBM: pattern, element->
  * block maps, FP64 and FP32 matrices, damage state and Krylov vectors. */

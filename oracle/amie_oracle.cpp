@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1107,3 +1108,45 @@ void or_smoothed_strain(int etype, int64_t nelem, const int32_t *conn, const dou
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------- mesh input
+int64_t or_renumber(int64_t nnodes, int64_t nelem, int nper, const int32_t *conn, int32_t *ids)
+{
+    // features/features.cpp:603-633: ids reset to -1, then every bounding
+    // point of every element, in iteration order, gets count++ if still -1
+    std::fill(ids, ids + nnodes, -1);
+    int64_t count = 0;
+    for (int64_t q = 0; q < nelem * nper; q++)
+        if (ids[conn[q]] == -1) ids[conn[q]] = (int32_t)count++;
+    return count;
+}
+
+int64_t or_add_midpoints(int64_t nnodes, const double *xy, int64_t ntri, const int32_t *conn3, int32_t *conn6,
+                         double *mid_xy)
+{
+    // mesher/delaunay.cpp:280-351 with nodes_per_side = 1: fraction 0 finds
+    // the side's first vertex, fraction 1/2 the mid-edge point, looked up in
+    // the already visited neighbours (an edge's other triangle) by position;
+    // a miss makes a new point a*(1 - 0.5) + b*0.5
+    std::unordered_map<uint64_t, int32_t> edge;
+    edge.reserve((size_t)ntri * 2);
+    int64_t nmid = 0;
+    for (int64_t t = 0; t < ntri; t++)
+        for (int s = 0; s < 3; s++) {
+            const int32_t a = conn3[t * 3 + s], b = conn3[t * 3 + (s + 1) % 3];
+            const uint64_t key = ((uint64_t)(uint32_t)std::min(a, b) << 32) | (uint32_t)std::max(a, b);
+            auto it = edge.find(key);
+            int32_t id;
+            if (it != edge.end()) {
+                id = it->second;
+            } else {
+                id = (int32_t)(nnodes + nmid);
+                for (int d = 0; d < 2; d++) mid_xy[nmid * 2 + d] = xy[(int64_t)a * 2 + d] * (1. - .5) + xy[(int64_t)b * 2 + d] * .5;
+                edge.emplace(key, id);
+                nmid++;
+            }
+            conn6[t * 6 + 2 * s] = a;
+            conn6[t * 6 + 2 * s + 1] = id;
+        }
+    return nmid;
+}

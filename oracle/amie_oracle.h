@@ -146,6 +146,21 @@ void or_smoothed_strain(int etype, int64_t nelem, const int32_t *conn, const dou
                         const double *damage, double threshold, double radius, int kernel, const double *strain,
                         double *out);
 
+/* FeatureTree::renumber (features/features.cpp:603-729, one time plane):
+ * the elements are walked in order and each node takes the next id the first
+ * time one of them names it; nodes no element names keep -1.  Returns the
+ * number of ids given. */
+int64_t or_renumber(int64_t nnodes, int64_t nelem, int nper, const int32_t *conn, int32_t *ids);
+
+/* DelaunayTree::addSharedNodes(1, 1, ...) (mesher/delaunay.cpp:260-355):
+ * triangles in order, sides (v0,v1), (v1,v2), (v2,v0); a side takes the
+ * mid-edge node an earlier triangle made on the same edge, else a new one at
+ * a*0.5 + b*0.5 (a, b in the side's order), numbered nnodes, nnodes + 1, ...
+ * as they are made.  conn6 = [v0, m01, v1, m12, v2, m20] per triangle;
+ * mid_xy (2 per new node, room for 3*ntri).  Returns the number made. */
+int64_t or_add_midpoints(int64_t nnodes, const double *xy, int64_t ntri, const int32_t *conn3, int32_t *conn6,
+                         double *mid_xy);
+
 #ifdef __cplusplus
 }
 #endif

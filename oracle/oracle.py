@@ -53,6 +53,10 @@ def lib():
         L.or_damage_update.restype = i64
         L.or_average_stress.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp]
         L.or_smoothed_strain.argtypes = [i32, i64, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
+        L.or_renumber.argtypes = [i64, i64, i32, vp, vp]
+        L.or_renumber.restype = i64
+        L.or_add_midpoints.argtypes = [i64, vp, i64, vp, vp, vp]
+        L.or_add_midpoints.restype = i64
         _LIB = L
     return _LIB
 
@@ -316,3 +320,25 @@ def smoothed_strain(etype, conn, coords, strain, radius, kernel=QUARTIC_COMPACT,
     lib().or_smoothed_strain(etype, conn.shape[0], _p(conn), _p(coords), _p(phase), _p(damage), threshold, radius,
                              kernel, _p(strain), _p(out))
     return out
+
+
+def renumber(conn, nnodes=None):
+    """FeatureTree::renumber: first-touch ids over the elements in order
+    (-1 for nodes no element names) and their count."""
+    conn = _c(conn, np.int32)
+    n = int(conn.max()) + 1 if nnodes is None else int(nnodes)
+    ids = np.empty(n, dtype=np.int32)
+    count = lib().or_renumber(n, conn.shape[0], conn.shape[1], _p(conn), _p(ids))
+    return ids, int(count)
+
+
+def add_midpoints(xy, conn3):
+    """DelaunayTree::addSharedNodes(1, 1): P2 triangles [v0, m01, v1, m12,
+    v2, m20] and the new mid-edge nodes' coordinates (numbered from
+    len(xy) in the order they are made)."""
+    xy = _c(xy, np.float64)
+    conn3 = _c(conn3, np.int32)
+    conn6 = np.empty((conn3.shape[0], 6), dtype=np.int32)
+    mid = np.empty((3 * conn3.shape[0], 2))
+    n = lib().or_add_midpoints(xy.shape[0], _p(xy), conn3.shape[0], _p(conn3), _p(conn6), _p(mid))
+    return conn6, mid[:n].copy()

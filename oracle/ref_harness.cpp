@@ -18,6 +18,10 @@
 // Usage: ref_harness <outdir>
 
 #include "elements/elements.h"
+#include "features/features.h"
+#include "features/inclusion.h"
+#include "features/sample.h"
+#include "features/sample3d.h"
 #include "mesher/delaunay.h"
 #include "mesher/delaunay_3d.h"
 #include "physics/fracturecriteria/fracturecriterion.h"
@@ -37,6 +41,7 @@
 #include <map>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace Amie;
@@ -453,6 +458,88 @@ static void golden_smoothing(const std::string &tag, int npts, double radius, Sm
     npy_f64(tag + "_radius", {radius});
 }
 
+// FeatureTree's own mesh pipeline up to renumbering: the state machine of
+// features/features.cpp:5767-5920 (sample, generateElements, stitch -- which
+// for QUADRATIC adds the mid-edge nodes, mesher/delaunay.cpp:260-355 --,
+// element behaviours, renumber, features/features.cpp:603-729).  FeatureTree's
+// `state` and its StateType are protected: the member pointer is formed in an
+// explicit instantiation (access not checked there) and the enumerator is
+// passed by value (RENUMBERED = 3, features/features.h:58-72).
+auto feature_tree_state();
+template <auto M> struct ExposeAuto {
+    friend auto feature_tree_state() { return M; }
+};
+template struct ExposeAuto<&FeatureTree::state>;
+
+// setStateTo's parameter type deduced from its member pointer, so that the
+// protected enumeration need not be named.
+template <typename S, typename E> static void call_set_state(S &st, void (S::*f)(E, bool), int v)
+{
+    (st.*f)(static_cast<E>(v), false);
+}
+
+template <typename T> static void renumber_tree(T &F)
+{
+    auto &st = F.*feature_tree_state();
+    call_set_state(st, &std::remove_reference_t<decltype(st)>::setStateTo, 3 /* RENUMBERED */);
+}
+
+// Elements in the mesh's iteration order (the order renumber() walks), each
+// with the ids renumber() gave its bounding points, and the coordinates by id.
+template <typename MeshT> static void dump_mesh(const std::string &tag, MeshT *m, int dim)
+{
+    std::vector<int64_t> conn;
+    std::map<int, std::vector<double>> xyz;
+    size_t nper = 0;
+    for (auto i = m->begin(); i != m->end(); i++) {
+        nper = i->getBoundingPoints().size();
+        for (size_t j = 0; j < nper; j++) {
+            const Point &p = i->getBoundingPoint(j);
+            conn.push_back(p.getId());
+            std::vector<double> c = {p.getX(), p.getY()};
+            if (dim == 3) c.push_back(p.getZ());
+            xyz[p.getId()] = c;
+        }
+    }
+    std::vector<double> coords;
+    int expect = 0;
+    for (auto &kv : xyz) {
+        if (kv.first != expect++) { fprintf(stderr, "%s: ids not contiguous\n", tag.c_str()); exit(1); }
+        coords.insert(coords.end(), kv.second.begin(), kv.second.end());
+    }
+    npy_i64(tag + "_conn", conn, {conn.size() / nper, nper});
+    npy_f64(tag + "_coords", coords, {xyz.size(), (size_t)dim});
+}
+
+static void golden_ingest_2d(const std::string &tag, double sampling, Order order, bool inclusion)
+{
+    RectangularFeature sample(nullptr, 1., 1., 0.5, 0.5);
+    sample.setBehaviour(new Stiffness(1e10, 0.2));
+    FeatureTree F(&sample);
+    Inclusion *inc = nullptr;
+    if (inclusion) {
+        inc = new Inclusion(&sample, 0.21, 0.45, 0.55);
+        inc->setBehaviour(new Stiffness(6e10, 0.2));
+        F.addFeature(&sample, inc);
+    }
+    F.setSamplingNumber(sampling);
+    F.setOrder(order);
+    F.setProjectionOnBoundaries(false);
+    renumber_tree(F);
+    dump_mesh(tag, F.get2DMesh(), 2);
+}
+
+static void golden_ingest_3d(const std::string &tag, double sampling)
+{
+    Sample3D sample(nullptr, 1., 1., 1., 0.5, 0.5, 0.5);
+    sample.setBehaviour(new Stiffness(1e10, 0.2, SPACE_THREE_DIMENSIONAL));
+    FeatureTree F(&sample);
+    F.setSamplingNumber(sampling);
+    F.setOrder(LINEAR);
+    renumber_tree(F);
+    dump_mesh(tag, F.get3DMesh(), 3);
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
@@ -476,6 +563,10 @@ int main(int argc, char **argv)
     golden_system_tet("sys_tet_3", 3, 30e9, 0.2);
     golden_smoothing("smooth_quartic", 150, 0.05, QUARTIC_COMPACT);
     golden_smoothing("smooth_gauss", 150, 0.05, GAUSSIAN_NONCOMPACT);
+    golden_ingest_2d("ingest_p1", 16, LINEAR, false);
+    golden_ingest_2d("ingest_p2", 16, QUADRATIC, false);
+    golden_ingest_2d("ingest_p2_inc", 24, QUADRATIC, true);
+    golden_ingest_3d("ingest_tet", 4);
     printf("golden vectors written to %s\n", g_out.c_str());
     return 0;
 }

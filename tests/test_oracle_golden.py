@@ -199,3 +199,70 @@ def test_smoothed_strain_vs_reference_caches(tag, kernel):
     out = orc.smoothed_strain(orc.P1, conn, xy, eps, radius, kernel=kernel)
     np.testing.assert_allclose(out, ref, rtol=1e-11, atol=1e-12)
 
+
+
+def relabel_vertices(conn6, xy, seed):
+    """P1 input of a reference P2 mesh: the vertex columns, with the vertices
+    given random labels (first-touch numbering must not depend on them)."""
+    verts = np.unique(conn6[:, ::2])
+    lab = np.full(xy.shape[0], -1, dtype=np.int64)
+    lab[verts] = np.random.default_rng(seed).permutation(len(verts))
+    vxy = np.empty((len(verts), 2))
+    vxy[lab[verts]] = xy[verts]
+    return lab[conn6[:, ::2]].astype(np.int32), vxy
+
+
+@pytest.mark.parametrize("tag", ["ingest_p1", "ingest_tet"])
+def test_renumber_vs_reference_meshes(tag):
+    """Meshes FeatureTree made itself (RectangularFeature / Sample3D, its
+    sampling and Delaunay mesher, then FeatureTree::renumber): with the nodes
+    relabelled at random, the oracle's first-touch numbering gives back the
+    reference's ids, element by element."""
+    conn, coords = g(tag + "_conn"), g(tag + "_coords")
+    n = coords.shape[0]
+    perm = np.random.default_rng(11).permutation(n)
+    ids, count = orc.renumber(perm[conn].astype(np.int32), n)
+    assert count == n
+    np.testing.assert_array_equal(ids[perm[conn]], conn)
+
+
+@pytest.mark.parametrize("tag", ["ingest_p2", "ingest_p2_inc"])
+def test_midpoints_and_renumber_vs_reference_meshes(tag):
+    """FeatureTree with QUADRATIC elements (its stitch() adds the mid-edge
+    nodes through DelaunayTree::addSharedNodes, then renumber()), on a box and
+    on a box with an inclusion (projection on boundaries off): from the
+    vertex-only triangles with random vertex labels, add_midpoints then
+    renumber gives the reference's P2 connectivity and, bit for bit, its node
+    coordinates."""
+    conn6, xy = g(tag + "_conn"), g(tag + "_coords")
+    conn3, vxy = relabel_vertices(conn6, xy, 12)
+    c6, mid = orc.add_midpoints(vxy, conn3)
+    allxy = np.vstack([vxy, mid])
+    ids, count = orc.renumber(c6, allxy.shape[0])
+    assert count == xy.shape[0] == allxy.shape[0]
+    np.testing.assert_array_equal(ids[c6], conn6)
+    out = np.empty_like(allxy)
+    out[ids] = allxy
+    np.testing.assert_array_equal(out, xy)
+
+
+def test_midpoints_brute_force():
+    """Small random triangle soups: one mid-edge node per distinct edge, made
+    by the first triangle (in order) that has the edge, shared by every later
+    one, at the exact half-sum of the edge's ends."""
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        nv = int(rng.integers(3, 12))
+        conn3 = np.array([rng.choice(nv, 3, replace=False) for _ in range(int(rng.integers(1, 15)))], dtype=np.int32)
+        xy = rng.normal(size=(nv, 2))
+        c6, mid = orc.add_midpoints(xy, conn3)
+        first = {}
+        for t, tri in enumerate(conn3):
+            for s in range(3):
+                a, b = int(tri[s]), int(tri[(s + 1) % 3])
+                e = (min(a, b), max(a, b))
+                if e not in first:
+                    first[e] = nv + len(first)
+                    np.testing.assert_array_equal(mid[first[e] - nv], xy[a] * 0.5 + xy[b] * 0.5)
+                assert c6[t, 2 * s] == a and c6[t, 2 * s + 1] == first[e]
+        assert len(mid) == len(first)

@@ -75,6 +75,8 @@ SIGNATURES = {
     "amie_cauchy_green": (None, [c_dbl, c_dbl, c_i32, vp]),
     "amie_element_matrix": (c_i32, [c_i32, vp, vp, vp]),
     "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
+    "amie_gpu_renumber": (c_i32, [vp, c_i64, c_i64, c_i32, vp, vp, vp, vp]),
+    "amie_gpu_add_midpoints": (c_i32, [vp, c_i64, vp, c_i64, vp, vp, vp, vp, vp]),
     "amie_gpu_stream_copy": (c_i32, [vp, ctypes.c_size_t, c_i32, vp]),
     "amie_gpu_batch_create": (c_i32, [vp, c_i32, vp, vp]),
     "amie_gpu_batch_from_bcsr": (c_i32, [vp, c_i32, c_i64, vp, vp, vp, vp, vp]),
@@ -290,6 +292,33 @@ def element_matrices_gpu(ctx, etype, coords, cmats):
     K = np.zeros((ne, n, n))
     check(lib().amie_gpu_element_matrices(ctx.h, etype, ne, _p(coords), _p(cmats), _p(K)))
     return K
+
+
+def renumber_gpu(ctx, conn, nnodes, timed=False):
+    """FeatureTree::renumber on the device: first-touch ids over the elements
+    in order (-1 for nodes no element names), their count, and (timed) the
+    device milliseconds."""
+    conn = _c(conn, np.int32)
+    ids = np.empty(int(nnodes), dtype=np.int32)
+    cnt, ms = c_i64(), c_dbl()
+    check(lib().amie_gpu_renumber(ctx.h, int(nnodes), conn.shape[0], conn.shape[1], _p(conn), _p(ids),
+                                  ctypes.byref(cnt), ctypes.byref(ms) if timed else None))
+    return (ids, cnt.value, ms.value) if timed else (ids, cnt.value)
+
+
+def add_midpoints_gpu(ctx, xy, conn3, timed=False):
+    """DelaunayTree::addSharedNodes(1, 1) on the device: P2 triangles [v0,
+    m01, v1, m12, v2, m20], the new mid-edge nodes' coordinates (numbered
+    from len(xy)), and (timed) the device milliseconds."""
+    xy = _c(xy, np.float64)
+    conn3 = _c(conn3, np.int32)
+    conn6 = np.empty((conn3.shape[0], 6), dtype=np.int32)
+    mid = np.empty((3 * conn3.shape[0], 2))
+    n, ms = c_i64(), c_dbl()
+    check(lib().amie_gpu_add_midpoints(ctx.h, xy.shape[0], _p(xy), conn3.shape[0], _p(conn3), _p(conn6), _p(mid),
+                                       ctypes.byref(n), ctypes.byref(ms) if timed else None))
+    out = (conn6, mid[:n.value].copy())
+    return out + (ms.value,) if timed else out
 
 
 def default_opts(**kw):
