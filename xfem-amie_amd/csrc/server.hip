@@ -68,12 +68,14 @@ struct OwnedProblem {
         d.forces = s.forces ? forces.data() : nullptr;
         d.macro_strain = s.macro_strain ? macro.data() : nullptr;
         // HBM estimate: blocks per row of the element type, FP64 + FP32
-        // values, column index, 11 vectors, element data
+        // values, column index, 11 vectors, element data, the gather
+        // assembly's lists (per slot, per element block, per row)
         const int64_t rows = s.dofmap ? s.ndofnodes : s.nnodes;
         const int bpr = et.et == 1 ? 7 : (et.et == 2 ? 12 : 15);
         const int64_t slots = rows * bpr * 5 / 4;
-        footprint = slots * (int64_t)(8 * et.b * et.b + 4 * et.b * et.b + 4) + rows * et.b * s.nrhs * 8 * 11 +
-                    s.nelem * (int64_t)(et.nper * et.nper * 4 + et.nper * 8 + 40) + s.nnodes * et.dim * 8 + (1 << 20);
+        footprint = slots * (int64_t)(8 * et.b * et.b + 4 * et.b * et.b + 4 + 4) + rows * et.b * s.nrhs * 8 * 11 +
+                    s.nelem * (int64_t)(2 * et.nper * et.nper * 4 + et.nper * 8 + 40) + s.nnodes * et.dim * 8 +
+                    (s.macro_strain ? rows * 4 + s.nelem * et.nper * 4 : 0) + (1 << 20);
     }
     int cls() const { return d.elem_type * 64 + d.nrhs * 2 + (d.macro_strain ? 1 : 0); }
 };
