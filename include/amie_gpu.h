@@ -227,6 +227,23 @@ int amie_gpu_renumber(amie_gpu_ctx *ctx, int64_t nnodes, int64_t nelem, int32_t 
 int amie_gpu_add_midpoints(amie_gpu_ctx *ctx, int64_t nnodes, const double *xy, int64_t ntri, const int32_t *conn3,
                            int32_t *conn6, double *mid_xy, int64_t *nmid, double *device_ms);
 
+/* ------------------------------------------------------- result files
+ * TriangleWriter text output (utilities/writer/triangle_writer.cpp:31-86,
+ * 407-445, 578-608, 1024-1075, 1461-1475), for the reference's viewers:
+ * "TRIANGLES", ntri, 3, 2 + total field components, then one line per
+ * triangle: vertex coordinates x0 y0 x1 y1 x2 y2, displacements u0x u1x u2x
+ * u0y u1y u2y, then for each field, component by component, its values at
+ * vertices 0, 1, 2; every value as a C++ stream prints a double, followed by
+ * a space.  conn: ntri x nper (vertices at local 0, nper/3, 2nper/3), xy: 2
+ * per node; the displacements uv and every field are values at the
+ * vertices, as the reference evaluates them there: uv ntri x 3 x 2,
+ * fields[f] ntri x 3 x sizes[f] (triangle, vertex, component), e.g. the
+ * nodal solution at each vertex, and amie_gpu_batch_element_strain's
+ * constant strain repeated at the three vertices for TOTAL_STRAIN_FIELD.
+ * Host only. */
+int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, const double *xy,
+                         const double *uv, int32_t nfield, const int32_t *sizes, const double *const *fields);
+
 /* ------------------------------------------------------------- batches
  * A batch holds one or more problems resident in HBM: pattern, element->
  * block maps, FP64 and FP32 matrices, damage state and Krylov vectors. */

@@ -22,6 +22,8 @@
 #include "features/inclusion.h"
 #include "features/sample.h"
 #include "features/sample3d.h"
+#include "features/boundarycondition.h"
+#include "utilities/writer/triangle_writer.h"
 #include "mesher/delaunay.h"
 #include "mesher/delaunay_3d.h"
 #include "physics/fracturecriteria/fracturecriterion.h"
@@ -540,6 +542,47 @@ static void golden_ingest_3d(const std::string &tag, double sampling)
     dump_mesh(tag, F.get3DMesh(), 3);
 }
 
+// TriangleWriter (utilities/writer/triangle_writer.cpp:31-86, 407-445,
+// 578-608, 1024-1075, 1461-1475) after a FeatureTree solve of a unit box
+// (u_x = 0 left, u_y = 0 bottom, u_x = 1e-3 right): the file it writes, and
+// the inputs it read -- mesh, DOF displacements, and the displacement,
+// strain and stress the element states give at each element's three
+// vertices (interpolated there: a fixed DOF can read back as 1e-20).
+static void golden_writer(const std::string &tag, double sampling)
+{
+    RectangularFeature sample(nullptr, 1., 1., 0.5, 0.5);
+    sample.setBehaviour(new Stiffness(1e10, 0.2));
+    FeatureTree F(&sample);
+    F.setSamplingNumber(sampling);
+    F.setOrder(LINEAR);
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_XI, LEFT));
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_ETA, BOTTOM));
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(SET_ALONG_XI, RIGHT, 1e-3));
+    F.step();
+    TriangleWriter trg(g_out + "/" + tag + "_trg.txt", &F, 1);
+    trg.getField(TOTAL_STRAIN_FIELD);
+    trg.getField(REAL_STRESS_FIELD);
+    trg.write();
+    dump_mesh(tag, F.get2DMesh(), 2);
+    npy_f64(tag + "_u", vec(F.getDisplacements()));
+    std::vector<double> eps, sig, uv;
+    VirtualMachine vm;
+    auto *m = F.get2DMesh();
+    for (auto i = m->begin(); i != m->end(); i++)
+        for (int k = 0; k < 3; k++) {
+            Vector e(3), t(3), d(2);
+            i->getState().getField(TOTAL_STRAIN_FIELD, i->getBoundingPoint(k), e, false, &vm);
+            i->getState().getField(REAL_STRESS_FIELD, i->getBoundingPoint(k), t, false, &vm);
+            i->getState().getField(DISPLACEMENT_FIELD, i->getBoundingPoint(k), d, false, &vm, 0);
+            for (int c = 0; c < 3; c++) { eps.push_back(e[c]); sig.push_back(t[c]); }
+            uv.push_back(d[0]);
+            uv.push_back(d[1]);
+        }
+    npy_f64(tag + "_uvertex", uv, {uv.size() / 6, 3, 2});
+    npy_f64(tag + "_strain", eps, {eps.size() / 9, 3, 3});
+    npy_f64(tag + "_stress", sig, {sig.size() / 9, 3, 3});
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
@@ -567,6 +610,7 @@ int main(int argc, char **argv)
     golden_ingest_2d("ingest_p2", 16, QUADRATIC, false);
     golden_ingest_2d("ingest_p2_inc", 24, QUADRATIC, true);
     golden_ingest_3d("ingest_tet", 4);
+    golden_writer("writer_p1", 8);
     printf("golden vectors written to %s\n", g_out.c_str());
     return 0;
 }

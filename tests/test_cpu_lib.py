@@ -164,3 +164,25 @@ def test_cpp_dropin_built_and_fails_loudly_without_gpu(tmp_path):
     r = subprocess.run([DROPIN, "2", *files, "1e-10", "4", str(tmp_path / "x.bin")], capture_output=True, text=True)
     assert r.returncode == 2
     assert "GPU CG" in r.stderr
+
+
+def test_triangle_writer_matches_reference_file(tmp_path):
+    """The file the compiled reference's TriangleWriter wrote after a
+    FeatureTree solve (coordinates, displacements, TOTAL_STRAIN_FIELD,
+    REAL_STRESS_FIELD; tests/golden/ref/writer_p1_*), rewritten from the
+    inputs that writer read (the element states' values at the vertices):
+    byte for byte the same."""
+    ref = os.path.join(ROOT, "tests", "golden", "ref")
+    g = lambda n: np.load(os.path.join(ref, "writer_p1_" + n + ".npy"))
+    out = tmp_path / "trg.txt"
+    ag.write_triangles(out, g("conn"), g("coords"), g("uvertex"), [g("strain"), g("stress")])
+    assert out.read_bytes() == open(os.path.join(ref, "writer_p1_trg.txt"), "rb").read()
+
+
+def test_triangle_writer_argument_errors(tmp_path):
+    conn = np.array([[0, 1, 2]], dtype=np.int32)
+    xy = np.zeros((3, 2))
+    with pytest.raises(ag.AmieError):
+        ag.write_triangles(tmp_path / "no_such_dir" / "t.txt", conn, xy, np.zeros(6))
+    with pytest.raises(ag.AmieError):
+        ag.write_triangles(tmp_path / "t.txt", np.zeros((1, 4), dtype=np.int32), xy, np.zeros(6))

@@ -661,3 +661,36 @@ def test_cpp_linear_solver_dropin(tmp_path, tag, precision):
     assert np.sqrt(np.dot(res, orc.inverse_diagonal(A) * res)) <= 1e-10
     xref = g(tag + "_cg_x")
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
+
+
+def test_triangle_writer_from_device_solution(ctx, tmp_path):
+    """The reference's own solve written by its TriangleWriter
+    (tests/golden/ref/writer_p1_trg.txt: FeatureTree mesh of a unit box,
+    u_x = 0 left, u_y = 0 bottom, u_x = 1e-3 right) against the same problem
+    solved on the device and written by amie_write_triangles: same header,
+    same coordinates text, displacements, strains and stresses equal to the
+    printed 6 digits (within 1e-5 of each field's largest value)."""
+    ref = lambda n: g("writer_p1_" + n)
+    conn, xy = ref("conn").astype(np.int32), ref("coords")
+    x, y = xy[:, 0], xy[:, 1]
+    left, bottom, right = np.where(x == x.min())[0], np.where(y == y.min())[0], np.where(x == x.max())[0]
+    bc_dofs = np.concatenate([2 * left, 2 * bottom + 1, 2 * right])
+    bc_vals = np.concatenate([np.zeros(len(left)), np.zeros(len(bottom)), np.full(len(right), 1e-3)])
+    C = ag.cauchy_green(1e10, 0.2)
+    p = ag.Problem(ag.P1, xy, conn, C[None], bc_dofs=bc_dofs, bc_vals=bc_vals)
+    B = ag.Batch(ctx, [p])
+    assert B.solve(precision=ag.FP64, rtol=1e-12)["status"] == 0
+    u = B.solution(0)
+    eps = B.element_strain(0)
+    out = tmp_path / "trg.txt"
+    ag.write_triangles(out, conn, xy, u, [eps, eps @ C.T])
+    got = out.read_text().splitlines()
+    want = open(os.path.join(REF, "writer_p1_trg.txt")).read().splitlines()
+    assert got[:4] == want[:4] and len(got) == len(want)
+    a = np.array([[float(v) for v in line.split()] for line in got[4:]])
+    b = np.array([[float(v) for v in line.split()] for line in want[4:]])
+    assert [line.split()[:6] for line in got[4:]] == [line.split()[:6] for line in want[4:]]
+    for lo, hi in ((6, 12), (12, 21), (21, 30)):
+        scale = np.abs(b[:, lo:hi]).max()
+        assert np.abs(a[:, lo:hi] - b[:, lo:hi]).max() <= 1e-5 * scale, (lo, hi)
+

@@ -76,6 +76,7 @@ SIGNATURES = {
     "amie_element_matrix": (c_i32, [c_i32, vp, vp, vp]),
     "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
     "amie_gpu_renumber": (c_i32, [vp, c_i64, c_i64, c_i32, vp, vp, vp, vp]),
+    "amie_write_triangles": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, vp, vp, c_i32, vp, vp]),
     "amie_gpu_add_midpoints": (c_i32, [vp, c_i64, vp, c_i64, vp, vp, vp, vp, vp]),
     "amie_gpu_stream_copy": (c_i32, [vp, ctypes.c_size_t, c_i32, vp]),
     "amie_gpu_batch_create": (c_i32, [vp, c_i32, vp, vp]),
@@ -319,6 +320,30 @@ def add_midpoints_gpu(ctx, xy, conn3, timed=False):
                                        ctypes.byref(n), ctypes.byref(ms) if timed else None))
     out = (conn6, mid[:n.value].copy())
     return out + (ms.value,) if timed else out
+
+
+def write_triangles(path, conn, xy, u, fields=()):
+    """TriangleWriter text file: conn ntri x nper, xy nnodes x 2; u either
+    ntri x 3 x 2 (values at the vertices) or nodal (nnodes x 2, or flat by
+    DOF); fields a sequence of ntri x 3 x size arrays (values at the
+    vertices) or ntri x size arrays (constant over the element)."""
+    conn = _c(conn, np.int32)
+    xy = _c(xy, np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    if u.ndim != 3:
+        n = conn.shape[1] // 3
+        u = u.reshape(-1, 2)[conn[:, [0, n, 2 * n]]]
+    u = np.ascontiguousarray(u)
+    fs = []
+    for f in fields:
+        f = np.asarray(f, dtype=np.float64)
+        if f.ndim == 2:
+            f = np.repeat(f[:, None, :], 3, axis=1)
+        fs.append(np.ascontiguousarray(f))
+    sizes = np.array([f.shape[2] for f in fs], dtype=np.int32)
+    ptrs = (ctypes.c_void_p * max(1, len(fs)))(*[f.ctypes.data for f in fs])
+    check(lib().amie_write_triangles(str(path).encode(), conn.shape[0], conn.shape[1], _p(conn), _p(xy), _p(u),
+                                     len(fs), _p(sizes), ptrs))
 
 
 def default_opts(**kw):

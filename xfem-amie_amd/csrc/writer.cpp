@@ -1,0 +1,56 @@
+// Result files in the reference's TriangleWriter text format, for its
+// viewers.  Host code: the fields are computed on the device and copied out
+// first (amie_gpu_batch_get_solution, amie_gpu_batch_element_strain, ...).
+#include <cstdint>
+#include <fstream>
+#include <vector>
+
+#include "amie_gpu.h"
+
+namespace amie {
+void set_error(const char *fmt, ...);
+}
+
+// utilities/writer/triangle_writer.cpp: the header (:1461-1475) counts the
+// columns after the six coordinates in threes; the constructor (:31-86)
+// collects the coordinates and the displacements, each field adds its
+// columns (:578-608) component by component, vertex 0, 1, 2 within a
+// component (:1024-1075, reversed at :589), and write() (:407-445) prints
+// every column of a triangle as a C++ stream prints a double, each followed
+// by a space, one triangle a line.
+int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, const double *xy,
+                         const double *uv, int32_t nfield, const int32_t *sizes, const double *const *fields)
+{
+    if (!path || ntri <= 0 || nper <= 0 || nper % 3 || !conn || !xy || !uv || nfield < 0 || (nfield && (!sizes || !fields))) {
+        amie::set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    int64_t cols = 0;
+    for (int f = 0; f < nfield; f++) {
+        if (sizes[f] <= 0 || !fields[f]) { amie::set_error("field %d: bad size or data", f); return AMIE_ERR_ARG; }
+        cols += sizes[f];
+    }
+    std::ofstream out(path, std::ios::out);
+    if (!out) { amie::set_error("cannot open %s", path); return AMIE_ERR_ARG; }
+    out << "TRIANGLES" << std::endl;
+    out << (int)ntri << std::endl;
+    out << 3 << std::endl;
+    out << (int)(2 + cols) << std::endl;
+    const int n = nper / 3;  // vertices at local 0, n, 2n
+    for (int64_t t = 0; t < ntri; t++) {
+        const int32_t v[3] = {conn[t * nper], conn[t * nper + n], conn[t * nper + 2 * n]};
+        for (int k = 0; k < 3; k++) out << xy[(int64_t)v[k] * 2] << " " << xy[(int64_t)v[k] * 2 + 1] << " ";
+        for (int c = 0; c < 2; c++)
+            for (int k = 0; k < 3; k++) out << uv[t * 6 + k * 2 + c] << " ";
+        for (int f = 0; f < nfield; f++) {
+            const int s = sizes[f];
+            const double *d = fields[f] + t * 3 * s;
+            for (int c = 0; c < s; c++)
+                for (int k = 0; k < 3; k++) out << d[k * s + c] << " ";
+        }
+        out << std::endl;
+    }
+    out.close();
+    if (!out) { amie::set_error("write to %s failed", path); return AMIE_ERR_ARG; }
+    return AMIE_OK;
+}
