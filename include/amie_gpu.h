@@ -176,6 +176,16 @@ int amie_mesh_structured_size(int32_t elem_type, int32_t n, int64_t *nnodes, int
 int amie_mesh_structured(int32_t elem_type, int32_t n, double *coords, int32_t *conn);
 /* Periodic DOF map of the structured P1 square: node (i,j) -> (i mod n, j mod n). */
 int amie_mesh_periodic_dofmap(int32_t n, int32_t *dofmap);
+/* placement2D (utilities/placement.cpp:87-162) for circular inclusions in an
+ * axis-aligned rectangle (width, height, centre), no pre-placed inclusions
+ * or exclusion zones: radii in the order given (the reference's callers sort
+ * them largest first), gap min_dist, one budget of tries_max tries for the
+ * whole list, rotation range orientation (drawn, ignored by circles), seed
+ * of std::default_random_engine.  Same placements as the reference, bit for
+ * bit: placed_index (n), placed_xyr (3n: centre, final radius), nplaced. */
+int amie_place_inclusions_2d(double width, double height, double cx, double cy, int64_t n, const double *radii,
+                             double min_dist, int64_t tries_max, double orientation, uint64_t seed,
+                             int64_t *placed_index, double *placed_xyr, int64_t *nplaced);
 /* Seeded rejection sampling of circular inclusions in the unit square
  * (fully inside, pairwise gap >= gap) until area fraction >= target or the
  * try cap; returns the number placed in *ncircles (xyr: 3 per circle). */

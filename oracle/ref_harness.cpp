@@ -24,6 +24,7 @@
 #include "features/sample3d.h"
 #include "features/boundarycondition.h"
 #include "utilities/writer/triangle_writer.h"
+#include "utilities/placement.h"
 #include "mesher/delaunay.h"
 #include "mesher/delaunay_3d.h"
 #include "physics/fracturecriteria/fracturecriterion.h"
@@ -37,6 +38,8 @@
 #include "utilities/tensor.h"
 
 #include <omp.h>
+#include <algorithm>
+#include <functional>
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
@@ -542,6 +545,37 @@ static void golden_ingest_3d(const std::string &tag, double sampling)
     dump_mesh(tag, F.get3DMesh(), 3);
 }
 
+// placement2D (utilities/placement.cpp:87-162) of circular inclusions in a
+// rectangle: radii drawn from a seeded list (largest first), placed with the
+// given gap, tries budget and seed; which were placed, where, and with what
+// radius after the gap scaling is undone.
+static void golden_placement(const std::string &tag, int n, double rmin, double rmax, double min_dist, int tries,
+                             size_t seed)
+{
+    RectangularFeature sample(nullptr, 1., 0.8, 0.5, 0.4);
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> ur(rmin, rmax);
+    std::vector<double> radii(n);
+    for (auto &r : radii) r = ur(rng);
+    std::sort(radii.begin(), radii.end(), std::greater<double>());
+    std::vector<Feature *> incs;
+    for (double r : radii) incs.push_back(new Inclusion(r, 0., 0.));
+    std::vector<Feature *> placed = placement2D(sample.getPrimitive(), incs, min_dist, 0, tries, M_PI,
+                                                std::vector<Geometry *>(), seed);
+    std::vector<int64_t> idx;
+    std::vector<double> out;
+    for (Feature *f : placed) {
+        idx.push_back(std::find(incs.begin(), incs.end(), f) - incs.begin());
+        out.push_back(f->getCenter().getX());
+        out.push_back(f->getCenter().getY());
+        out.push_back(f->getRadius());
+    }
+    npy_f64(tag + "_radii", radii);
+    npy_i64(tag + "_placed", idx);
+    npy_f64(tag + "_out", out, {idx.size(), 3});
+    npy_f64(tag + "_params", {min_dist, (double)tries, (double)seed});
+}
+
 // TriangleWriter (utilities/writer/triangle_writer.cpp:31-86, 407-445,
 // 578-608, 1024-1075, 1461-1475) after a FeatureTree solve of a unit box
 // (u_x = 0 left, u_y = 0 bottom, u_x = 1e-3 right): the file it writes, and
@@ -611,6 +645,8 @@ int main(int argc, char **argv)
     golden_ingest_2d("ingest_p2_inc", 24, QUADRATIC, true);
     golden_ingest_3d("ingest_tet", 4);
     golden_writer("writer_p1", 8);
+    golden_placement("place_gap", 60, 0.01, 0.05, 0.005, 6400, 1);
+    golden_placement("place_nogap", 200, 0.02, 0.08, 0., 3000, 5);
     printf("golden vectors written to %s\n", g_out.c_str());
     return 0;
 }

@@ -186,3 +186,29 @@ def test_triangle_writer_argument_errors(tmp_path):
         ag.write_triangles(tmp_path / "no_such_dir" / "t.txt", conn, xy, np.zeros(6))
     with pytest.raises(ag.AmieError):
         ag.write_triangles(tmp_path / "t.txt", np.zeros((1, 4), dtype=np.int32), xy, np.zeros(6))
+
+
+@pytest.mark.parametrize("tag", ["place_gap", "place_nogap"])
+def test_inclusion_placement_matches_reference(tag):
+    """placement2D of the compiled reference (utilities/placement.cpp:87-162)
+    on a 1 x 0.8 rectangle: 60 radii with a 0.005 gap (all placed), and 200
+    larger radii without a gap (the 3000-try budget runs out after 21).  The
+    same inclusions are placed, at the same centres with the same radii, bit
+    for bit."""
+    ref = os.path.join(ROOT, "tests", "golden", "ref")
+    g = lambda n: np.load(os.path.join(ref, tag + "_" + n + ".npy"))
+    gap, tries, seed = g("params")
+    idx, xyr = ag.place_inclusions_2d(g("radii"), 1.0, 0.8, (0.5, 0.4), min_dist=gap, tries=int(tries), seed=int(seed))
+    np.testing.assert_array_equal(idx, g("placed"))
+    np.testing.assert_array_equal(xyr, g("out"))
+
+
+def test_inclusion_placement_properties():
+    """Placed circles lie in the rectangle and keep the gap between them."""
+    radii = np.sort(np.random.default_rng(3).uniform(0.01, 0.04, 80))[::-1]
+    idx, xyr = ag.place_inclusions_2d(radii, 2.0, 1.0, (1.0, 0.5), min_dist=0.01, tries=20000, seed=9)
+    assert len(idx) > 40
+    x, y, r = xyr.T
+    assert (x - r >= -1e-12).all() and (x + r <= 2 + 1e-12).all() and (y - r >= -1e-12).all() and (y + r <= 1 + 1e-12).all()
+    d = np.hypot(x[:, None] - x[None], y[:, None] - y[None]) + np.eye(len(x)) * 10
+    assert (d >= np.maximum(r[:, None], r[None]) + np.minimum(r[:, None], r[None]) + 0.01 - 1e-12).all()

@@ -77,6 +77,8 @@ SIGNATURES = {
     "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
     "amie_gpu_renumber": (c_i32, [vp, c_i64, c_i64, c_i32, vp, vp, vp, vp]),
     "amie_write_triangles": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, vp, vp, c_i32, vp, vp]),
+    "amie_place_inclusions_2d": (c_i32, [c_dbl, c_dbl, c_dbl, c_dbl, c_i64, vp, c_dbl, c_i64, c_dbl, c_u64,
+                                         vp, vp, vp]),
     "amie_gpu_add_midpoints": (c_i32, [vp, c_i64, vp, c_i64, vp, vp, vp, vp, vp]),
     "amie_gpu_stream_copy": (c_i32, [vp, ctypes.c_size_t, c_i32, vp]),
     "amie_gpu_batch_create": (c_i32, [vp, c_i32, vp, vp]),
@@ -320,6 +322,20 @@ def add_midpoints_gpu(ctx, xy, conn3, timed=False):
                                        ctypes.byref(n), ctypes.byref(ms) if timed else None))
     out = (conn6, mid[:n.value].copy())
     return out + (ms.value,) if timed else out
+
+
+def place_inclusions_2d(radii, width=1.0, height=1.0, center=(0.5, 0.5), min_dist=0.0, tries=6400,
+                        orientation=np.pi, seed=1):
+    """The reference's placement2D for circles in a rectangle: indices of
+    the placed radii, and their centres and final radii (k x 3)."""
+    radii = _c(radii, np.float64)
+    n = radii.shape[0]
+    idx = np.zeros(max(1, n), dtype=np.int64)
+    xyr = np.zeros((max(1, n), 3))
+    k = c_i64()
+    check(lib().amie_place_inclusions_2d(width, height, center[0], center[1], n, _p(radii), min_dist, tries,
+                                         orientation, seed, _p(idx), _p(xyr), ctypes.byref(k)))
+    return idx[:k.value].copy(), xyr[:k.value].copy()
 
 
 def write_triangles(path, conn, xy, u, fields=()):

@@ -299,4 +299,122 @@ int amie_damage_field_3d(uint64_t seed, int32_t n, double ell, double lo, double
     return AMIE_OK;
 }
 
+// placement2D (utilities/placement.cpp:87-162) for circular inclusions in an
+// axis-aligned rectangle, without pre-placed inclusions or exclusion zones.
+// Restated step by step:
+//  * the generator is std::default_random_engine(seed) with uniform
+//    distributions over the rectangle's corner coordinates, x from the top
+//    left to the bottom right corner's x, y from the top left's y to the
+//    bottom right's (a decreasing range), and a rotation in
+//    [-orientation, orientation] that circles ignore (geometry_base.cpp:
+//    588-660); one try draws the rotation first, then y, then x, the order
+//    in which the reference's g++ build evaluates the arguments of
+//    transform2D(inc, x(), y(), r());
+//  * with a gap, the radius is scaled by (r + gap)/r for the placement and
+//    back by 1/scale once placed (Circle::setRadius, geometry_2D.cpp:1256);
+//  * a candidate must lie in the rectangle with its bounding box
+//    (Rectangle::in, geometry_2D.cpp:993-1008, 1e-12 tolerance), then neither
+//    overlap a placed circle (Geometry::intersects, circle branch,
+//    geometry_base.cpp:1430-1445) nor have its centre inside one
+//    (Circle::in, geometry_2D.cpp:1472-1484); placed circles keep their
+//    unscaled radii.  The reference finds the placed circles to test through
+//    a pixel grid (utilities/grid.cpp:959-991, 1090-1105); every circle the
+//    grid returns is tested the same way and a circle it does not return
+//    cannot overlap, so testing all placed circles decides the same;
+//  * one try budget for the whole list: an inclusion still searching when it
+//    runs out is not placed and placement stops.
+// Outputs the placed inclusions' indices, centres and final radii in order.
+int amie_place_inclusions_2d(double width, double height, double cx, double cy, int64_t n, const double *radii,
+                             double min_dist, int64_t tries_max, double orientation, uint64_t seed,
+                             int64_t *placed_index, double *placed_xyr, int64_t *nplaced)
+{
+#pragma clang fp contract(off)
+    if (n < 0 || (n && (!radii || !placed_index || !placed_xyr)) || !nplaced || width <= 0 || height <= 0) {
+        amie::set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    for (int64_t i = 0; i < n; i++)
+        if (!(radii[i] > 0)) { amie::set_error("radius %lld is not positive", (long long)i); return AMIE_ERR_ARG; }
+    const double tol = 1e-12;
+    const double left = cx - 0.5 * width, right = cx + 0.5 * width, top = cy + 0.5 * height, bottom = cy - 0.5 * height;
+    auto box_in = [&](double x, double y) {
+        if (x < cx - 0.5 * width - tol) return false;
+        if (x > cx + 0.5 * width + tol) return false;
+        if (y > cy + 0.5 * height + tol) return false;
+        if (y < cy - 0.5 * height - tol) return false;
+        return true;
+    };
+    std::default_random_engine gen(seed);
+    std::uniform_real_distribution<double> xd(left, right), yd(top, bottom), rd(-orientation, orientation);
+    struct Circ { double x, y, r, r2; };
+    std::vector<Circ> placed;
+    double x = 0., y = 0.;
+    auto draw = [&]() {
+        (void)rd(gen);
+        y = yd(gen);
+        x = xd(gen);
+    };
+    auto fits_box = [&](double r) {
+        return box_in(x, y) && box_in(x + (-r), y + r) && box_in(x + r, y + r) && box_in(x + r, y + (-r)) &&
+               box_in(x + (-r), y + (-r));
+    };
+    auto free_of_others = [&](double r) {
+        for (const Circ &t : placed) {
+            const double birad = r + t.r;
+            bool inter = true;
+            if (x + birad < t.x) inter = false;
+            else if (x - birad > t.x) inter = false;
+            else if (y + birad < t.y) inter = false;
+            else if (y - birad > t.y) inter = false;
+            if (inter) {
+                const double dx = x - t.x, dy = y - t.y;
+                if (dx * dx + dy * dy < birad * birad) return false;
+            }
+            if (!(x < t.x - t.r * 1.01) && !(x > t.x + t.r * 1.01) && !(y < t.y - t.r * 1.01) && !(y > t.y + t.r * 1.01)) {
+                const double dx = x - t.x, dy = y - t.y;
+                if (dx * dx + dy * dy < t.r2) return false;
+            }
+        }
+        return true;
+    };
+    int64_t tries = 0, np = 0;
+    for (int64_t i = 0; i < n && tries < tries_max; i++) {
+        tries++;
+        double r = radii[i], scale = 1.;
+        if (min_dist > tol) {
+            scale = (radii[i] + min_dist) / radii[i];
+            r = r * scale;
+        }
+        draw();
+        while (!fits_box(r) && tries < tries_max) {
+            tries++;
+            draw();
+        }
+        bool added;
+        while (!(added = free_of_others(r)) && tries < tries_max) {
+            tries++;
+            draw();
+            while (!fits_box(r) && tries < tries_max) {
+                tries++;
+                draw();
+            }
+        }
+        if (added && tries < tries_max) {
+            if (scale > 1.) r = r * (1. / scale);
+            placed.push_back({x, y, r, r * r});
+            placed_index[np] = i;
+            placed_xyr[np * 3] = x;
+            placed_xyr[np * 3 + 1] = y;
+            placed_xyr[np * 3 + 2] = r;
+            np++;
+        } else if (added) {
+            // added to the reference's grid on the last try, not returned
+            if (scale > 1.) r = r * (1. / scale);
+            placed.push_back({x, y, r, r * r});
+        }
+    }
+    *nplaced = np;
+    return AMIE_OK;
+}
+
 } // extern "C"
