@@ -276,7 +276,7 @@ int amie_gpu_batch_assemble(amie_gpu_batch *b);
  * with its Kahan compensation; every block written once.  AMIE_ASSEMBLE_COLOURED:
  * one launch per element colour, each element adding its blocks (no two
  * elements of a colour share a block); the sums are ordered by colour.  Both
- * are deterministic; the gather needs fewer than 2^32 element blocks in the
+ * are deterministic; the gather needs fewer than 2^31 element blocks in the
  * batch. */
 int amie_gpu_batch_set_assembly(amie_gpu_batch *b, int32_t method);
 int amie_gpu_batch_solve(amie_gpu_batch *b, const amie_solve_opts *opts, amie_solve_stats *stats);

@@ -524,36 +524,6 @@ int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, B
         }
     }
     if (slot_fail >= 0) { set_error("internal: element block missing from pattern (problem %d)", slot_fail); return AMIE_ERR_STATE; }
-    // gather assembly lists, counting-sorted by slot (and by row for the
-    // macro load) with the reference's summation order kept inside each list
-    if ((uint64_t)ne * nper * nper < (1ull << 32)) {
-        std::vector<uint32_t> cptr(B.nslots + 1, 0), contrib((size_t)ne * nper * nper), fill;
-        for (uint32_t s : h_eslot) cptr[s + 1]++;
-        for (int64_t s = 0; s < B.nslots; s++) cptr[s + 1] += cptr[s];
-        fill.assign(cptr.begin(), cptr.end() - 1);
-        for (int64_t e = 0; e < ne; e++) {
-            const uint32_t *es = &h_eslot[(size_t)e * nper * nper];
-            auto put = [&](int i, int j) { contrib[fill[es[i * nper + j]]++] = (uint32_t)((e * nper + i) * nper + j); };
-            for (int j = 0; j < nper; j++) {
-                put(j, j);
-                for (int k = j + 1; k < nper; k++) { put(j, k); put(k, j); }
-            }
-        }
-        if ((rc = alloc_and_upload(B.slot_cptr, cptr.data(), cptr.size() * 4))) return rc;
-        if ((rc = alloc_and_upload(B.slot_contrib, contrib.data(), contrib.size() * 4))) return rc;
-        if (B.has_macro) {
-            std::vector<uint32_t> rptr(B.nrows_pad + 1, 0), rlist((size_t)ne * nper);
-            for (int32_t r : h_erow) rptr[r + 1]++;
-            for (int64_t r = 0; r < B.nrows_pad; r++) rptr[r + 1] += rptr[r];
-            fill.assign(rptr.begin(), rptr.end() - 1);
-            for (int64_t q = 0; q < ne * nper; q++) rlist[fill[h_erow[q]]++] = (uint32_t)q;
-            if ((rc = alloc_and_upload(B.row_cptr, rptr.data(), rptr.size() * 4))) return rc;
-            if ((rc = alloc_and_upload(B.row_contrib, rlist.data(), rlist.size() * 4))) return rc;
-        }
-        B.asm_method = AMIE_ASSEMBLE_GATHER;
-    } else {
-        B.asm_method = AMIE_ASSEMBLE_COLOURED;
-    }
     for (int p = 0; p < nprob; p++)
         for (int64_t e = 0; e < probs[p].nelem; e++) col_elems[tmp[p].colour[e]].push_back((int32_t)(B.ph[p].elem0 + e));
     std::vector<int32_t> h_colour_elems;
@@ -584,6 +554,14 @@ int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, B
     AMIE_HIP(hipMemset(B.ekappa.p, 0, B.ekappa.n));
     B.has_damage_state = any_damage;
     if ((rc = alloc_and_upload(B.grown, nullptr, 8))) return rc;
+    // gather assembly lists (device radix sort), unless the batch is too
+    // large for 32-bit list offsets: then the coloured scatter
+    if (gather_fits(ne, nper, B.nslots)) {
+        if ((rc = build_gather_lists(B))) return rc;
+        B.asm_method = AMIE_ASSEMBLE_GATHER;
+    } else {
+        B.asm_method = AMIE_ASSEMBLE_COLOURED;
+    }
     if ((rc = alloc_and_upload(B.bcmask, h_bcmask.data(), h_bcmask.size()))) return rc;
     if ((rc = alloc_and_upload(B.bcval, h_bcval.data(), h_bcval.size() * 8))) return rc;
     if ((rc = alloc_vectors(B))) return rc;
@@ -1439,7 +1417,7 @@ int amie_gpu_batch_set_assembly(amie_gpu_batch *b, int32_t method)
     if (method != AMIE_ASSEMBLE_GATHER && method != AMIE_ASSEMBLE_COLOURED) { set_error("unknown assembly method %d", method); return AMIE_ERR_ARG; }
     if (!B.has_elements) { set_error("the batch holds an assembled matrix, not elements"); return AMIE_ERR_STATE; }
     if (method == AMIE_ASSEMBLE_GATHER && !B.slot_cptr.p) {
-        set_error("more than 2^32 element blocks: only the coloured assembly is available");
+        set_error("more than 2^31 element blocks: only the coloured assembly is available");
         return AMIE_ERR_ARG;
     }
     B.asm_method = method;

@@ -191,6 +191,10 @@ int build_dict_or_drop(Batch &B);   // a build out of device memory leaves the b
 
 // FP64 plane vector of problem p <-> original DOF order, one column of
 // nrows*b entries per right-hand side.
+// gather assembly: whether the batch's element blocks fit 32-bit list
+// offsets, and the device build of the slot and row lists
+bool gather_fits(int64_t nelem, int nper, int64_t nslots);
+int build_gather_lists(Batch &B);
 int download_vec(Batch &B, int p, const DevBuf &buf, double *out);
 int upload_vec(Batch &B, int p, DevBuf &buf, const double *in);
 
