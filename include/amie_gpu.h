@@ -245,14 +245,15 @@ int amie_gpu_add_midpoints(amie_gpu_ctx *ctx, int64_t nnodes, const double *xy, 
  * u0y u1y u2y, then for each field, component by component, its values at
  * vertices 0, 1, 2; every value as a C++ stream prints a double, followed by
  * a space.  conn: ntri x nper (vertices at local 0, nper/3, 2nper/3), xy: 2
- * per node; the displacements uv and every field are values at the
+ * per node for nnodes nodes; the displacements uv and every field are values at the
  * vertices, as the reference evaluates them there: uv ntri x 3 x 2,
  * fields[f] ntri x 3 x sizes[f] (triangle, vertex, component), e.g. the
  * nodal solution at each vertex, and amie_gpu_batch_element_strain's
  * constant strain repeated at the three vertices for TOTAL_STRAIN_FIELD.
  * Host only. */
-int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, const double *xy,
-                         const double *uv, int32_t nfield, const int32_t *sizes, const double *const *fields);
+int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, int64_t nnodes,
+                         const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
+                         const double *const *fields);
 
 /* ------------------------------------------------------------- batches
  * A batch holds one or more problems resident in HBM: pattern, element->

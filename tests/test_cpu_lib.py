@@ -185,7 +185,9 @@ def test_triangle_writer_argument_errors(tmp_path):
     with pytest.raises(ag.AmieError):
         ag.write_triangles(tmp_path / "no_such_dir" / "t.txt", conn, xy, np.zeros(6))
     with pytest.raises(ag.AmieError):
-        ag.write_triangles(tmp_path / "t.txt", np.zeros((1, 4), dtype=np.int32), xy, np.zeros(6))
+        ag.write_triangles(tmp_path / "t.txt", np.zeros((1, 4), dtype=np.int32), xy, np.zeros((1, 3, 2)))
+    with pytest.raises(ag.AmieError, match="outside"):
+        ag.write_triangles(tmp_path / "t.txt", np.array([[0, 1, 3]], dtype=np.int32), xy, np.zeros((1, 3, 2)))
 
 
 @pytest.mark.parametrize("tag", ["place_gap", "place_nogap"])

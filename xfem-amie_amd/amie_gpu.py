@@ -76,7 +76,7 @@ SIGNATURES = {
     "amie_element_matrix": (c_i32, [c_i32, vp, vp, vp]),
     "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
     "amie_gpu_renumber": (c_i32, [vp, c_i64, c_i64, c_i32, vp, vp, vp, vp]),
-    "amie_write_triangles": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, vp, vp, c_i32, vp, vp]),
+    "amie_write_triangles": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, c_i64, vp, vp, c_i32, vp, vp]),
     "amie_place_inclusions_2d": (c_i32, [c_dbl, c_dbl, c_dbl, c_dbl, c_i64, vp, c_dbl, c_i64, c_dbl, c_u64,
                                          vp, vp, vp]),
     "amie_gpu_add_midpoints": (c_i32, [vp, c_i64, vp, c_i64, vp, vp, vp, vp, vp]),
@@ -358,8 +358,8 @@ def write_triangles(path, conn, xy, u, fields=()):
         fs.append(np.ascontiguousarray(f))
     sizes = np.array([f.shape[2] for f in fs], dtype=np.int32)
     ptrs = (ctypes.c_void_p * max(1, len(fs)))(*[f.ctypes.data for f in fs])
-    check(lib().amie_write_triangles(str(path).encode(), conn.shape[0], conn.shape[1], _p(conn), _p(xy), _p(u),
-                                     len(fs), _p(sizes), ptrs))
+    check(lib().amie_write_triangles(str(path).encode(), conn.shape[0], conn.shape[1], _p(conn), xy.shape[0], _p(xy),
+                                     _p(u), len(fs), _p(sizes), ptrs))
 
 
 def default_opts(**kw):

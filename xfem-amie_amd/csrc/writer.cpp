@@ -18,13 +18,17 @@ void set_error(const char *fmt, ...);
 // component (:1024-1075, reversed at :589), and write() (:407-445) prints
 // every column of a triangle as a C++ stream prints a double, each followed
 // by a space, one triangle a line.
-int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, const double *xy,
-                         const double *uv, int32_t nfield, const int32_t *sizes, const double *const *fields)
+int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, int64_t nnodes,
+                         const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
+                         const double *const *fields)
 {
-    if (!path || ntri <= 0 || nper <= 0 || nper % 3 || !conn || !xy || !uv || nfield < 0 || (nfield && (!sizes || !fields))) {
+    if (!path || ntri <= 0 || nper <= 0 || nper % 3 || !conn || nnodes <= 0 || !xy || !uv || nfield < 0 ||
+        (nfield && (!sizes || !fields))) {
         amie::set_error("bad arguments");
         return AMIE_ERR_ARG;
     }
+    for (int64_t q = 0; q < ntri * nper; q++)
+        if (conn[q] < 0 || conn[q] >= nnodes) { amie::set_error("connectivity names a node outside [0, nnodes)"); return AMIE_ERR_ARG; }
     int64_t cols = 0;
     for (int f = 0; f < nfield; f++) {
         if (sizes[f] <= 0 || !fields[f]) { amie::set_error("field %d: bad size or data", f); return AMIE_ERR_ARG; }
