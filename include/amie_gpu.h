@@ -255,6 +255,27 @@ int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int
                          const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
                          const double *const *fields);
 
+/* VoxelWriter sampling on the device (utilities/writer/voxel_writer.cpp:
+ * 88-272) for a Tet4 mesh with one value set per element (a Tet4 field is
+ * constant over the element, e.g. REAL_STRESS_FIELD): an nx x ny x nz grid
+ * spans c -/+ 0.5 s (n-1)/n per axis of the box (box: cx cy cz sx sy sz);
+ * walking the tetrahedra in order, each voxel of the index range of a
+ * tetrahedron's bounding box whose point (or one of the 8 points offset by
+ * +-16e-12) lies in it takes its values, later tetrahedra overwriting
+ * earlier ones (on the device: the largest owning index).  Barycentric test
+ * of geometry_3D.cpp:451-533 evaluated by volumes.  values: ncomp x
+ * (nx*ny*nz), voxel index k + ny*j + ny*nx*i, -1e9 where no tetrahedron
+ * owns the voxel; that index numbers cubic grids only, so nx = ny = nz is
+ * required.  device_ms (nullable): device time with inputs in HBM. */
+int amie_gpu_voxelize(amie_gpu_ctx *ctx, int64_t nnodes, const double *xyz, int64_t ntet, const int32_t *conn,
+                      int32_t ncomp, const double *field, const double *box, int32_t nx, int32_t ny, int32_t nz,
+                      double *values, double *device_ms);
+/* VoxelWriter files (voxel_writer.cpp:52-86, 433-442, 548-571): component c
+ * of values to <base>_<nx>_<c>.vox, header "VOXELS", 1, nx, ny, nz, then one
+ * byte per voxel: the value mapped to [1, 255] over the symmetric range set
+ * by the 1 % and 99 % points of the sorted component.  Host only. */
+int amie_write_voxels(const char *base, int32_t nx, int32_t ny, int32_t nz, int32_t ncomp, const double *values);
+
 /* ------------------------------------------------------------- batches
  * A batch holds one or more problems resident in HBM: pattern, element->
  * block maps, FP64 and FP32 matrices, damage state and Krylov vectors. */

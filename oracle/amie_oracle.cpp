@@ -1150,3 +1150,77 @@ int64_t or_add_midpoints(int64_t nnodes, const double *xy, int64_t ntri, const i
         }
     return nmid;
 }
+
+// ---------------------------------------------------------------- voxels
+namespace {
+double det3(const double *a, const double *b, const double *c)
+{
+    return a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
+}
+
+// barycentric test of geometry_3D.cpp:451-533 (weights of vertices 2, 3, 0),
+// by volumes instead of the reference's 4x4 inverse
+bool in_tet(const double (*P)[3], const double *p)
+{
+    double e[3][3], f[3][3];
+    for (int d = 0; d < 3; d++) {
+        e[0][d] = P[3][d] - P[2][d];
+        e[1][d] = P[0][d] - P[2][d];
+        e[2][d] = P[1][d] - P[2][d];
+    }
+    const double V = det3(e[0], e[1], e[2]);
+    if (V == 0.) return false;
+    double q[3][3];
+    for (int d = 0; d < 3; d++) { q[0][d] = P[3][d] - p[d]; q[1][d] = P[0][d] - p[d]; q[2][d] = P[1][d] - p[d]; }
+    const double l2 = det3(q[0], q[1], q[2]) / V;
+    for (int d = 0; d < 3; d++) f[0][d] = p[d] - P[2][d];
+    const double l3 = det3(f[0], e[1], e[2]) / V;
+    const double l0 = det3(e[0], f[0], e[2]) / V;
+    if (l2 < -1e-12 || l3 < -1e-12 || l0 < -1e-12) return false;
+    return !(l2 + l3 + l0 > 1 + 3. * 1e-12);
+}
+} // namespace
+
+void or_voxel_owner(int64_t nnodes, const double *xyz, int64_t ntet, const int32_t *conn, const double *box,
+                    int nx, int ny, int nz, int32_t *owner)
+{
+    (void)nnodes;
+    const int n[3] = {nx, ny, nz};
+    double bl[3], tr[3];
+    for (int d = 0; d < 3; d++) {
+        const double v = box[3 + d] * (n[d] - 1) / (n[d]) * 0.5;
+        bl[d] = box[d] - v;
+        tr[d] = box[d] + v;
+    }
+    std::fill(owner, owner + (int64_t)nx * ny * nz, -1);
+    for (int64_t t = 0; t < ntet; t++) {
+        double P[4][3], lo[3], hi[3];
+        for (int a = 0; a < 4; a++)
+            for (int d = 0; d < 3; d++) P[a][d] = xyz[(int64_t)conn[t * 4 + a] * 3 + d];
+        for (int d = 0; d < 3; d++) {
+            lo[d] = hi[d] = P[0][d];
+            for (int a = 1; a < 4; a++) { lo[d] = std::min(lo[d], P[a][d]); hi[d] = std::max(hi[d], P[a][d]); }
+        }
+        for (int i = (int)(nx * (lo[0] - bl[0]) / (tr[0] - bl[0])); i < nx * (hi[0] - bl[0]) / (tr[0] - bl[0]); i++) {
+            if (i < 0 || i >= nx) continue;
+            for (int j = (int)(ny * (lo[1] - bl[1]) / (tr[1] - bl[1])); j < ny * (hi[1] - bl[1]) / (tr[1] - bl[1]); j++) {
+                if (j < 0 || j >= ny) continue;
+                for (int k = (int)(nz * (lo[2] - bl[2]) / (tr[2] - bl[2])); k < nz * (hi[2] - bl[2]) / (tr[2] - bl[2]); k++) {
+                    if (k < 0 || k >= nz) continue;
+                    double p[3] = {bl[0], bl[1], bl[2]};
+                    p[0] += (tr[0] - bl[0]) * ((double)i) / (double)(nx - 1);
+                    p[1] += (tr[1] - bl[1]) * ((double)j) / (double)(ny - 1);
+                    p[2] += (tr[2] - bl[2]) * ((double)k) / (double)(nz - 1);
+                    bool in = in_tet(P, p);
+                    for (int m = 0; m < 8 && !in; m++) {
+                        const double q[3] = {p[0] + ((m & 4) ? 16. : -16.) * 1e-12, p[1] + ((m & 2) ? 16. : -16.) * 1e-12,
+                                             p[2] + ((m & 1) ? 16. : -16.) * 1e-12};
+                        in = in_tet(P, q);
+                    }
+                    if (in) owner[k + (int64_t)ny * j + (int64_t)ny * nx * i] = (int32_t)t;
+                }
+            }
+        }
+    }
+}
+

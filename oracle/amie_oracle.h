@@ -161,6 +161,18 @@ int64_t or_renumber(int64_t nnodes, int64_t nelem, int nper, const int32_t *conn
 int64_t or_add_midpoints(int64_t nnodes, const double *xy, int64_t ntri, const int32_t *conn3, int32_t *conn6,
                          double *mid_xy);
 
+/* VoxelWriter sampling (utilities/writer/voxel_writer.cpp:88-272): grid
+ * corners bl/tr = c -/+ 0.5 * (s * (n-1) / n) per axis of the box (centre c,
+ * size s); for every tetrahedron in order, the voxels i of the index range
+ * [(int)(n*(min-bl)/(tr-bl)), n*(max-bl)/(tr-bl)) of its bounding box, at
+ * point bl + (tr-bl)*i/(n-1), whose point or one of the 8 points offset by
+ * +-16e-12 lies in it (barycentric weights of vertices 2, 3, 0 >= -1e-12,
+ * their sum <= 1 + 3e-12, geometry_3D.cpp:451-533) take it as owner, later
+ * tetrahedra overwriting earlier ones.  owner[k + ny*j + ny*nx*i], -1 where
+ * none.  box: cx cy cz sx sy sz. */
+void or_voxel_owner(int64_t nnodes, const double *xyz, int64_t ntet, const int32_t *conn, const double *box,
+                    int nx, int ny, int nz, int32_t *owner);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,7 @@ def lib():
         L.or_renumber.restype = i64
         L.or_add_midpoints.argtypes = [i64, vp, i64, vp, vp, vp]
         L.or_add_midpoints.restype = i64
+        L.or_voxel_owner.argtypes = [i64, vp, i64, vp, vp, i32, i32, i32, vp]
         _LIB = L
     return _LIB
 
@@ -342,3 +343,15 @@ def add_midpoints(xy, conn3):
     mid = np.empty((3 * conn3.shape[0], 2))
     n = lib().or_add_midpoints(xy.shape[0], _p(xy), conn3.shape[0], _p(conn3), _p(conn6), _p(mid))
     return conn6, mid[:n].copy()
+
+
+def voxel_owner(xyz, conn, box, n):
+    """VoxelWriter sampling: owning tetrahedron of every voxel of an n^3 grid
+    (-1 where none), index k + n*j + n*n*i."""
+    xyz = _c(xyz, np.float64)
+    conn = _c(conn, np.int32)
+    box = _c(box, np.float64)
+    owner = np.empty(n ** 3, dtype=np.int32)
+    lib().or_voxel_owner(xyz.shape[0], _p(xyz), conn.shape[0], _p(conn), _p(box), n, n, n, _p(owner))
+    return owner
+

@@ -25,6 +25,7 @@
 #include "features/boundarycondition.h"
 #include "utilities/writer/triangle_writer.h"
 #include "utilities/placement.h"
+#include "utilities/writer/voxel_writer.h"
 #include "mesher/delaunay.h"
 #include "mesher/delaunay_3d.h"
 #include "physics/fracturecriteria/fracturecriterion.h"
@@ -545,6 +546,38 @@ static void golden_ingest_3d(const std::string &tag, double sampling)
     dump_mesh(tag, F.get3DMesh(), 3);
 }
 
+// VoxelWriter (utilities/writer/voxel_writer.cpp:52-272, 433-442, 548-571)
+// after a FeatureTree solve of a unit cube clamped on its left face and
+// sheared on its right (u_y = 1e-3), a non-uniform stress field: the six
+// VWFT_STRESS files it writes, and its inputs -- the tetrahedra in mesh
+// order and each one's stress (constant over a Tet4; taken at its centre).
+static void golden_voxels(const std::string &tag, double sampling, int nvox)
+{
+    Sample3D sample(nullptr, 1., 1., 1., 0.5, 0.5, 0.5);
+    sample.setBehaviour(new Stiffness(1e10, 0.2, SPACE_THREE_DIMENSIONAL));
+    FeatureTree F(&sample);
+    F.setSamplingNumber(sampling);
+    F.setOrder(LINEAR);
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_XI, LEFT));
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_ETA, LEFT));
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_ZETA, LEFT));
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(SET_ALONG_ETA, RIGHT, 1e-3));
+    F.step();
+    VoxelWriter vw(g_out + "/" + tag, nvox);
+    vw.getField(&F, VWFT_STRESS);
+    vw.write();
+    dump_mesh(tag, F.get3DMesh(), 3);
+    std::vector<double> sig;
+    VirtualMachine vm;
+    auto *m = F.get3DMesh();
+    for (auto i = m->begin(); i != m->end(); i++) {
+        Vector t(0., 6);
+        i->getState().getField(REAL_STRESS_FIELD, i->getCenter(), t, false, &vm);
+        for (int c = 0; c < 6; c++) sig.push_back(t[c]);
+    }
+    npy_f64(tag + "_stress", sig, {sig.size() / 6, 6});
+}
+
 // placement2D (utilities/placement.cpp:87-162) of circular inclusions in a
 // rectangle: radii drawn from a seeded list (largest first), placed with the
 // given gap, tries budget and seed; which were placed, where, and with what
@@ -645,6 +678,7 @@ int main(int argc, char **argv)
     golden_ingest_2d("ingest_p2_inc", 24, QUADRATIC, true);
     golden_ingest_3d("ingest_tet", 4);
     golden_writer("writer_p1", 8);
+    golden_voxels("voxels_tet", 4, 24);
     golden_placement("place_gap", 60, 0.01, 0.05, 0.005, 6400, 1);
     golden_placement("place_nogap", 200, 0.02, 0.08, 0., 3000, 5);
     printf("golden vectors written to %s\n", g_out.c_str());

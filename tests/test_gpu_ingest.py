@@ -1,9 +1,11 @@
-"""GPU parity of the mesh input kernels (amie_gpu_renumber,
-amie_gpu_add_midpoints) through the C ABI: against meshes the compiled
-reference's FeatureTree made and numbered itself (tests/golden/ref/ingest_*,
-written by oracle/ref_harness.cpp), and against the oracle's serial
-restatement on larger and irregular inputs.  Integer results and the new
-nodes' coordinates are compared bit for bit."""
+"""GPU parity of the mesh input and output kernels through the C ABI:
+amie_gpu_renumber and amie_gpu_add_midpoints against meshes the compiled
+reference's FeatureTree made and numbered itself (tests/golden/ref/ingest_*),
+amie_gpu_voxelize against the files the reference's VoxelWriter wrote
+(tests/golden/ref/voxels_tet_*), all written by oracle/ref_harness.cpp, and
+against the oracle's serial restatements on larger and irregular inputs.
+Integer results, the new nodes' coordinates and the voxel files are
+compared bit for bit."""
 import os
 
 import numpy as np
@@ -110,3 +112,42 @@ def test_out_of_range_connectivity_is_an_error(ctx):
         ag.renumber_gpu(ctx, conn, 3)
     with pytest.raises(ag.AmieError, match="outside"):
         ag.add_midpoints_gpu(ctx, np.zeros((3, 2)), conn)
+
+
+def test_voxel_sampling_vs_reference_files(ctx, tmp_path):
+    """Device voxel ownership on the reference's mesh and stresses: the same
+    owners as the oracle, and the reference VoxelWriter's six files byte for
+    byte once written."""
+    conn, xyz, sig = g("voxels_tet_conn").astype(np.int32), g("voxels_tet_coords"), g("voxels_tet_stress")
+    box = np.array([0.5, 0.5, 0.5, 1.0, 1.0, 1.0])
+    vals = ag.voxelize_gpu(ctx, xyz, conn, sig, box, 24)
+    own = orc.voxel_owner(xyz, conn, box, 24)
+    np.testing.assert_array_equal(vals, np.where(own[None, :] >= 0, sig[own].T, -1e9))
+    ag.write_voxels(tmp_path / "v", vals, 24)
+    for c in range(6):
+        got = (tmp_path / f"v_24_{c}.vox").read_bytes()
+        assert got == open(os.path.join(REF, f"voxels_tet_24_{c}.vox"), "rb").read(), c
+
+
+def test_voxel_sampling_larger_mesh_vs_oracle(ctx):
+    """A structured Tet4 mesh (30^3 cells) with a random field on a 64^3 grid:
+    device owners and values equal the oracle's; every voxel is owned."""
+    xyz, conn = ag.structured_mesh(ag.TET4, 30)
+    field = np.random.default_rng(4).normal(size=(conn.shape[0], 2))
+    box = np.array([0.5, 0.5, 0.5, 1.0, 1.0, 1.0])
+    vals = ag.voxelize_gpu(ctx, xyz, conn, field, box, 64)
+    own = orc.voxel_owner(xyz, conn, box, 64)
+    assert (own >= 0).all()
+    np.testing.assert_array_equal(vals, field[own].T)
+
+
+def test_voxel_grid_must_be_cubic(ctx):
+    conn = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    with pytest.raises(ag.AmieError, match="cubic"):
+        check = ag.lib().amie_gpu_voxelize
+        vals = np.zeros(2 * 3 * 4)
+        field = np.zeros(1)
+        box = np.array([0.5, 0.5, 0.5, 1, 1, 1.0])
+        ag.check(check(ctx.h, 4, xyz.ctypes.data, 1, conn.ctypes.data, 1, field.ctypes.data, box.ctypes.data, 2, 3, 4,
+                       vals.ctypes.data, None))

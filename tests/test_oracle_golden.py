@@ -266,3 +266,19 @@ def test_midpoints_brute_force():
                     np.testing.assert_array_equal(mid[first[e] - nv], xy[a] * 0.5 + xy[b] * 0.5)
                 assert c6[t, 2 * s] == a and c6[t, 2 * s + 1] == first[e]
         assert len(mid) == len(first)
+
+
+def test_voxel_sampling_vs_reference_files(tmp_path):
+    """The compiled reference's VoxelWriter after a FeatureTree solve of a
+    unit cube (clamped left face, sheared right face; tests/golden/ref/
+    voxels_tet_*): from its tetrahedra in mesh order and their stresses, the
+    oracle's voxel ownership and the product's file writer give the six
+    VWFT_STRESS files byte for byte."""
+    import amie_gpu as ag
+    conn, xyz, sig = g("voxels_tet_conn").astype(np.int32), g("voxels_tet_coords"), g("voxels_tet_stress")
+    own = orc.voxel_owner(xyz, conn, np.array([0.5, 0.5, 0.5, 1.0, 1.0, 1.0]), 24)
+    vals = np.where(own[None, :] >= 0, sig[own].T, -1e9)
+    ag.write_voxels(tmp_path / "v", vals, 24)
+    for c in range(6):
+        got = (tmp_path / f"v_24_{c}.vox").read_bytes()
+        assert got == open(os.path.join(REF, f"voxels_tet_24_{c}.vox"), "rb").read(), c

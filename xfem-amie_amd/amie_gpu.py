@@ -77,6 +77,8 @@ SIGNATURES = {
     "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
     "amie_gpu_renumber": (c_i32, [vp, c_i64, c_i64, c_i32, vp, vp, vp, vp]),
     "amie_write_triangles": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, c_i64, vp, vp, c_i32, vp, vp]),
+    "amie_gpu_voxelize": (c_i32, [vp, c_i64, vp, c_i64, vp, c_i32, vp, vp, c_i32, c_i32, c_i32, vp, vp]),
+    "amie_write_voxels": (c_i32, [ctypes.c_char_p, c_i32, c_i32, c_i32, c_i32, vp]),
     "amie_place_inclusions_2d": (c_i32, [c_dbl, c_dbl, c_dbl, c_dbl, c_i64, vp, c_dbl, c_i64, c_dbl, c_u64,
                                          vp, vp, vp]),
     "amie_gpu_add_midpoints": (c_i32, [vp, c_i64, vp, c_i64, vp, vp, vp, vp, vp]),
@@ -336,6 +338,28 @@ def place_inclusions_2d(radii, width=1.0, height=1.0, center=(0.5, 0.5), min_dis
     check(lib().amie_place_inclusions_2d(width, height, center[0], center[1], n, _p(radii), min_dist, tries,
                                          orientation, seed, _p(idx), _p(xyr), ctypes.byref(k)))
     return idx[:k.value].copy(), xyr[:k.value].copy()
+
+
+def voxelize_gpu(ctx, xyz, conn, field, box, n, timed=False):
+    """VoxelWriter sampling on the device: field (ntet x ncomp, constant per
+    Tet4) on an n^3 grid over box (cx cy cz sx sy sz) -> ncomp x n^3 values
+    (-1e9 where no tetrahedron owns the voxel), and (timed) device ms."""
+    xyz = _c(xyz, np.float64)
+    conn = _c(conn, np.int32)
+    field = _c(np.atleast_2d(np.asarray(field, dtype=np.float64).T).T, np.float64)
+    box = _c(box, np.float64)
+    ncomp = field.shape[1]
+    values = np.empty((ncomp, n ** 3))
+    ms = c_dbl()
+    check(lib().amie_gpu_voxelize(ctx.h, xyz.shape[0], _p(xyz), conn.shape[0], _p(conn), ncomp, _p(field), _p(box),
+                                  n, n, n, _p(values), ctypes.byref(ms) if timed else None))
+    return (values, ms.value) if timed else values
+
+
+def write_voxels(base, values, n):
+    """VoxelWriter files <base>_<n>_<c>.vox from ncomp x n^3 values."""
+    values = _c(np.atleast_2d(values), np.float64)
+    check(lib().amie_write_voxels(str(base).encode(), n, n, n, values.shape[0], _p(values)))
 
 
 def write_triangles(path, conn, xy, u, fields=()):

@@ -1,8 +1,12 @@
 // Result files in the reference's TriangleWriter text format, for its
 // viewers.  Host code: the fields are computed on the device and copied out
 // first (amie_gpu_batch_get_solution, amie_gpu_batch_element_strain, ...).
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <fstream>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "amie_gpu.h"
@@ -58,3 +62,44 @@ int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int
     if (!out) { amie::set_error("write to %s failed", path); return AMIE_ERR_ARG; }
     return AMIE_OK;
 }
+
+// VoxelWriter output (utilities/writer/voxel_writer.cpp:52-86, 433-442,
+// 548-571): each component to <base>_<nx>_<c>.vox, a text header ("VOXELS",
+// 1, nx, ny, nz, one a line) and one byte per voxel in index order.  The
+// byte is the value mapped from [vmin, vmax] to [1, 255] and rounded, where
+// vmax and vmin are the values at the 99 % and 1 % positions of the sorted
+// component, widened to the symmetric range of the larger magnitude
+// (normalizeArray, min 1, max 255); voxels no tetrahedron owns keep the
+// sampling's -1e9 and take part in the percentiles as the reference's do.
+int amie_write_voxels(const char *base, int32_t nx, int32_t ny, int32_t nz, int32_t ncomp, const double *values)
+{
+    if (!base || nx <= 1 || ny <= 1 || nz <= 1 || ncomp <= 0 || !values) {
+        amie::set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<double> sorted(n);
+    std::vector<unsigned char> bytes(n);
+    const double lo = 1., hi = 255.;
+    for (int c = 0; c < ncomp; c++) {
+        const double *val = values + (size_t)c * n;
+        sorted.assign(val, val + n);
+        std::sort(sorted.begin(), sorted.end());
+        double vmax = sorted[std::min(n * 1 - 1, (size_t)(n * .99))];
+        double vmin = sorted[(size_t)(n * .01)];
+        if (std::abs(vmax) > std::abs(vmin)) vmin = -std::abs(vmax);
+        else vmax = std::abs(vmin);
+        for (size_t i = 0; i < n; i++)
+            bytes[i] = (unsigned char)std::min(std::max(std::round(lo + (hi - lo) * ((val[i] - vmin) / (vmax - vmin))), lo), hi);
+        std::ostringstream name;
+        name << base << "_" << nx << "_" << c << ".vox";
+        std::ofstream out(name.str(), std::ios::out | std::ios::binary);
+        if (!out) { amie::set_error("cannot open %s", name.str().c_str()); return AMIE_ERR_ARG; }
+        out << "VOXELS" << std::endl << 1 << std::endl << nx << std::endl << ny << std::endl << nz << std::endl;
+        out.write(reinterpret_cast<const char *>(bytes.data()), (std::streamsize)n);
+        out.close();
+        if (!out) { amie::set_error("write to %s failed", name.str().c_str()); return AMIE_ERR_ARG; }
+    }
+    return AMIE_OK;
+}
+
