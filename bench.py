@@ -110,6 +110,31 @@ def ingest_leg(ag, ctx, n):
             "triangles_per_s": ntri / ((best_mid + best_ren) * 1e-3)}
 
 
+def voxel_leg(ag, ctx, n, nvox):
+    """VoxelWriter sampling on the device: a structured Tet4 mesh of n^3
+    cells (6 n^3 tetrahedra) with a 6-component element field (seed 17) on
+    an nvox^3 grid; device time with the inputs in HBM (best of 3)."""
+    xyz, conn = ag.structured_mesh(ag.TET4, n)
+    field = np.random.default_rng(17).normal(size=(conn.shape[0], 6))
+    box = np.array([0.5, 0.5, 0.5, 1.0, 1.0, 1.0])
+    best = float("inf")
+    for _ in range(3):
+        _, ms = ag.voxelize_gpu(ctx, xyz, conn, field, box, nvox, timed=True)
+        best = min(best, ms)
+    return {"workload": f"structured Tet4 {n}^3 ({conn.shape[0]} tetrahedra), 6-component field, {nvox}^3 voxels",
+            "voxelize_ms": best, "voxels_per_s": nvox ** 3 / (best * 1e-3)}
+
+
+def cpu_voxels(ag, n, nvox):
+    """The oracle's serial voxel ownership on the same input."""
+    import oracle as orc
+    xyz, conn = ag.structured_mesh(ag.TET4, n)
+    t0 = time.perf_counter()
+    orc.voxel_owner(xyz, conn, np.array([0.5, 0.5, 0.5, 1.0, 1.0, 1.0]), nvox)
+    dt = time.perf_counter() - t0
+    return {"voxelize_ms": 1e3 * dt, "voxels_per_s": nvox ** 3 / dt, "cores": 1, "kind": "port"}
+
+
 def cpu_ingest(ag, n):
     """The oracle's serial add_midpoints + renumber on the same input."""
     import oracle as orc
@@ -230,6 +255,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the P2 400^2 and Tet4 64^3 legs")
     ap.add_argument("--ingest-n", type=int, default=2048, help="mesh-input leg: structured P1 n x n (0: skip)")
+    ap.add_argument("--voxel-n", type=int, default=64, help="voxel-output leg: structured Tet4 n^3 on a (2n)^3 grid "
+                                                            "(0: skip)")
     ap.add_argument("--spmv-kernel", default="", help="STEPS,ORDER[,FORMAT]: set the batch's FIX48 SpMV candidate "
                                                       "instead of autotuning (profiling runs)")
     args = ap.parse_args()
@@ -329,6 +356,10 @@ def main():
     if args.ingest_n > 0:
         ingest = ingest_leg(ag, ctx, args.ingest_n)
         log(f"mesh input: {ingest}")
+    voxels = None
+    if args.voxel_n > 0:
+        voxels = voxel_leg(ag, ctx, args.voxel_n, 2 * args.voxel_n)
+        log(f"voxel output: {voxels}")
     serve = None
     if not args.no_serve:
         serve = {"workload": f"served mix (seed 5): Poisson arrivals, 1/3 each 2D P1 RVE n~U{{64..512}} (3 load "
@@ -347,6 +378,9 @@ def main():
     if ingest and not args.no_cpu:
         ingest["cpu_baseline"] = cpu_ingest(ag, args.ingest_n)
         log(f"mesh input, oracle: {ingest['cpu_baseline']}")
+    if voxels and not args.no_cpu:
+        voxels["cpu_baseline"] = cpu_voxels(ag, args.voxel_n, 2 * args.voxel_n)
+        log(f"voxel output, oracle: {voxels['cpu_baseline']}")
     out = {
         "metric": "RVE solves/s + p50/p99 latency @rel.res 1e-8; mixed vs FP64 CG; SpMV GB/s",
         "value": value,
@@ -389,6 +423,7 @@ def main():
         "single_problems": single,
         "serving": serve,
         "mesh_input": ingest,
+        "voxel_output": voxels,
     }
     print(json.dumps(out), flush=True)
 
