@@ -254,6 +254,14 @@ int amie_gpu_add_midpoints(amie_gpu_ctx *ctx, int64_t nnodes, const double *xy, 
 int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, int64_t nnodes,
                          const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
                          const double *const *fields);
+/* BinaryTriangleWriter (triangle_writer.cpp:447-478, 1477-1491), same
+ * arguments: header "BIN_TRIANGLES", the column count, 3, -1 (the
+ * reference's (layers - 6) / 3 for one layer); per triangle the six
+ * coordinates, then every later column normalised over the triangles to
+ * 1..255 as the voxel files are, written as integers.  Host only. */
+int amie_write_triangles_bin(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, int64_t nnodes,
+                             const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
+                             const double *const *fields);
 
 /* VoxelWriter sampling on the device (utilities/writer/voxel_writer.cpp:
  * 88-272) for a Tet4 mesh with one value set per element (a Tet4 field is

@@ -610,7 +610,8 @@ static void golden_placement(const std::string &tag, int n, double rmin, double 
 }
 
 // TriangleWriter (utilities/writer/triangle_writer.cpp:31-86, 407-445,
-// 578-608, 1024-1075, 1461-1475) after a FeatureTree solve of a unit box
+// 578-608, 1024-1075, 1461-1475) and BinaryTriangleWriter (:447-478,
+// 1477-1491) after a FeatureTree solve of a unit box
 // (u_x = 0 left, u_y = 0 bottom, u_x = 1e-3 right): the file it writes, and
 // the inputs it read -- mesh, DOF displacements, and the displacement,
 // strain and stress the element states give at each element's three
@@ -630,6 +631,10 @@ static void golden_writer(const std::string &tag, double sampling)
     trg.getField(TOTAL_STRAIN_FIELD);
     trg.getField(REAL_STRESS_FIELD);
     trg.write();
+    BinaryTriangleWriter btrg(g_out + "/" + tag + "_btrg.txt", &F, 1);
+    btrg.getField(TOTAL_STRAIN_FIELD);
+    btrg.getField(REAL_STRESS_FIELD);
+    btrg.write();
     dump_mesh(tag, F.get2DMesh(), 2);
     npy_f64(tag + "_u", vec(F.getDisplacements()));
     std::vector<double> eps, sig, uv;

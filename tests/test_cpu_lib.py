@@ -179,6 +179,16 @@ def test_triangle_writer_matches_reference_file(tmp_path):
     assert out.read_bytes() == open(os.path.join(ref, "writer_p1_trg.txt"), "rb").read()
 
 
+def test_binary_triangle_writer_matches_reference_file(tmp_path):
+    """The BinaryTriangleWriter file of the same reference solve, rewritten
+    from the same inputs: byte for byte the same."""
+    ref = os.path.join(ROOT, "tests", "golden", "ref")
+    g = lambda n: np.load(os.path.join(ref, "writer_p1_" + n + ".npy"))
+    out = tmp_path / "btrg.txt"
+    ag.write_triangles(out, g("conn"), g("coords"), g("uvertex"), [g("strain"), g("stress")], binary=True)
+    assert out.read_bytes() == open(os.path.join(ref, "writer_p1_btrg.txt"), "rb").read()
+
+
 def test_triangle_writer_argument_errors(tmp_path):
     conn = np.array([[0, 1, 2]], dtype=np.int32)
     xy = np.zeros((3, 2))

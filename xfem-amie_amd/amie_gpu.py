@@ -77,6 +77,7 @@ SIGNATURES = {
     "amie_gpu_element_matrices": (c_i32, [vp, c_i32, c_i64, vp, vp, vp]),
     "amie_gpu_renumber": (c_i32, [vp, c_i64, c_i64, c_i32, vp, vp, vp, vp]),
     "amie_write_triangles": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, c_i64, vp, vp, c_i32, vp, vp]),
+    "amie_write_triangles_bin": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, c_i64, vp, vp, c_i32, vp, vp]),
     "amie_gpu_voxelize": (c_i32, [vp, c_i64, vp, c_i64, vp, c_i32, vp, vp, c_i32, c_i32, c_i32, vp, vp]),
     "amie_write_voxels": (c_i32, [ctypes.c_char_p, c_i32, c_i32, c_i32, c_i32, vp]),
     "amie_place_inclusions_2d": (c_i32, [c_dbl, c_dbl, c_dbl, c_dbl, c_i64, vp, c_dbl, c_i64, c_dbl, c_u64,
@@ -362,8 +363,8 @@ def write_voxels(base, values, n):
     check(lib().amie_write_voxels(str(base).encode(), n, n, n, values.shape[0], _p(values)))
 
 
-def write_triangles(path, conn, xy, u, fields=()):
-    """TriangleWriter text file: conn ntri x nper, xy nnodes x 2; u either
+def write_triangles(path, conn, xy, u, fields=(), binary=False):
+    """TriangleWriter (binary: BinaryTriangleWriter) file: conn ntri x nper, xy nnodes x 2; u either
     ntri x 3 x 2 (values at the vertices) or nodal (nnodes x 2, or flat by
     DOF); fields a sequence of ntri x 3 x size arrays (values at the
     vertices) or ntri x size arrays (constant over the element)."""
@@ -382,8 +383,9 @@ def write_triangles(path, conn, xy, u, fields=()):
         fs.append(np.ascontiguousarray(f))
     sizes = np.array([f.shape[2] for f in fs], dtype=np.int32)
     ptrs = (ctypes.c_void_p * max(1, len(fs)))(*[f.ctypes.data for f in fs])
-    check(lib().amie_write_triangles(str(path).encode(), conn.shape[0], conn.shape[1], _p(conn), xy.shape[0], _p(xy),
-                                     _p(u), len(fs), _p(sizes), ptrs))
+    fn = lib().amie_write_triangles_bin if binary else lib().amie_write_triangles
+    check(fn(str(path).encode(), conn.shape[0], conn.shape[1], _p(conn), xy.shape[0], _p(xy), _p(u), len(fs), _p(sizes),
+             ptrs))
 
 
 def default_opts(**kw):

@@ -15,6 +15,75 @@ namespace amie {
 void set_error(const char *fmt, ...);
 }
 
+namespace {
+
+// normalizeArray (utilities/writer/voxel_writer.cpp:548-571, min 1, max
+// 255, no voids): the values at the 99 % and 1 % positions of the sorted
+// array, widened to the symmetric range of the larger magnitude, map the
+// values to [1, 255], rounded
+void normalize(const double *val, size_t n, size_t stride, std::vector<double> &sorted, unsigned char *out)
+{
+    sorted.resize(n);
+    for (size_t i = 0; i < n; i++) sorted[i] = val[i * stride];
+    std::sort(sorted.begin(), sorted.end());
+    double vmax = sorted[std::min(n * 1 - 1, (size_t)(n * .99))];
+    double vmin = sorted[(size_t)(n * .01)];
+    if (std::abs(vmax) > std::abs(vmin)) vmin = -std::abs(vmax);
+    else vmax = std::abs(vmin);
+    const double lo = 1., hi = 255.;
+    for (size_t i = 0; i < n; i++)
+        out[i] = (unsigned char)std::min(std::max(std::round(lo + (hi - lo) * ((val[i * stride] - vmin) / (vmax - vmin))), lo), hi);
+}
+
+// the columns of a TriangleWriter line: coordinates (6), displacements (6),
+// then each field component over vertices 0, 1, 2
+struct Columns {
+    int64_t ntri;
+    int32_t nper;
+    const int32_t *conn;
+    const double *xy, *uv;
+    int32_t nfield;
+    const int32_t *sizes;
+    const double *const *fields;
+    int64_t count() const
+    {
+        int64_t c = 12;
+        for (int f = 0; f < nfield; f++) c += 3 * sizes[f];
+        return c;
+    }
+    // column `col` of triangle t
+    double at(int64_t t, int64_t col) const
+    {
+        const int n = nper / 3;
+        if (col < 6) return xy[(int64_t)conn[t * nper + (col / 2) * n] * 2 + col % 2];
+        if (col < 12) return uv[t * 6 + ((col - 6) % 3) * 2 + (col - 6) / 3];
+        col -= 12;
+        for (int f = 0; f < nfield; f++) {
+            const int s = sizes[f];
+            if (col < 3 * s) return fields[f][t * 3 * s + (col % 3) * s + col / 3];
+            col -= 3 * s;
+        }
+        return 0.;
+    }
+};
+
+int check_args(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, int64_t nnodes, const double *xy,
+               const double *uv, int32_t nfield, const int32_t *sizes, const double *const *fields)
+{
+    if (!path || ntri <= 0 || nper <= 0 || nper % 3 || !conn || nnodes <= 0 || !xy || !uv || nfield < 0 ||
+        (nfield && (!sizes || !fields))) {
+        amie::set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    for (int64_t q = 0; q < ntri * nper; q++)
+        if (conn[q] < 0 || conn[q] >= nnodes) { amie::set_error("connectivity names a node outside [0, nnodes)"); return AMIE_ERR_ARG; }
+    for (int f = 0; f < nfield; f++)
+        if (sizes[f] <= 0 || !fields[f]) { amie::set_error("field %d: bad size or data", f); return AMIE_ERR_ARG; }
+    return AMIE_OK;
+}
+
+} // namespace
+
 // utilities/writer/triangle_writer.cpp: the header (:1461-1475) counts the
 // columns after the six coordinates in threes; the constructor (:31-86)
 // collects the coordinates and the displacements, each field adds its
@@ -26,36 +95,53 @@ int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int
                          const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
                          const double *const *fields)
 {
-    if (!path || ntri <= 0 || nper <= 0 || nper % 3 || !conn || nnodes <= 0 || !xy || !uv || nfield < 0 ||
-        (nfield && (!sizes || !fields))) {
-        amie::set_error("bad arguments");
-        return AMIE_ERR_ARG;
-    }
-    for (int64_t q = 0; q < ntri * nper; q++)
-        if (conn[q] < 0 || conn[q] >= nnodes) { amie::set_error("connectivity names a node outside [0, nnodes)"); return AMIE_ERR_ARG; }
-    int64_t cols = 0;
-    for (int f = 0; f < nfield; f++) {
-        if (sizes[f] <= 0 || !fields[f]) { amie::set_error("field %d: bad size or data", f); return AMIE_ERR_ARG; }
-        cols += sizes[f];
-    }
+    int rc = check_args(path, ntri, nper, conn, nnodes, xy, uv, nfield, sizes, fields);
+    if (rc) return rc;
+    const Columns C{ntri, nper, conn, xy, uv, nfield, sizes, fields};
+    const int64_t cols = C.count();
     std::ofstream out(path, std::ios::out);
     if (!out) { amie::set_error("cannot open %s", path); return AMIE_ERR_ARG; }
     out << "TRIANGLES" << std::endl;
     out << (int)ntri << std::endl;
     out << 3 << std::endl;
-    out << (int)(2 + cols) << std::endl;
-    const int n = nper / 3;  // vertices at local 0, n, 2n
+    out << (int)((cols - 6) / 3) << std::endl;
     for (int64_t t = 0; t < ntri; t++) {
-        const int32_t v[3] = {conn[t * nper], conn[t * nper + n], conn[t * nper + 2 * n]};
-        for (int k = 0; k < 3; k++) out << xy[(int64_t)v[k] * 2] << " " << xy[(int64_t)v[k] * 2 + 1] << " ";
-        for (int c = 0; c < 2; c++)
-            for (int k = 0; k < 3; k++) out << uv[t * 6 + k * 2 + c] << " ";
-        for (int f = 0; f < nfield; f++) {
-            const int s = sizes[f];
-            const double *d = fields[f] + t * 3 * s;
-            for (int c = 0; c < s; c++)
-                for (int k = 0; k < 3; k++) out << d[k * s + c] << " ";
-        }
+        for (int64_t c = 0; c < cols; c++) out << C.at(t, c) << " ";
+        out << std::endl;
+    }
+    out.close();
+    if (!out) { amie::set_error("write to %s failed", path); return AMIE_ERR_ARG; }
+    return AMIE_OK;
+}
+
+// BinaryTriangleWriter (triangle_writer.cpp:447-478, 1477-1491): the header
+// "BIN_TRIANGLES", the column count, 3, and the reference's (layers - 6) / 3
+// for one layer (-1); per triangle the six coordinates as doubles, then
+// every later column normalised over the triangles to 1..255 (as the voxel
+// files) and written as an integer, each value followed by a space.
+int amie_write_triangles_bin(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, int64_t nnodes,
+                             const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
+                             const double *const *fields)
+{
+    int rc = check_args(path, ntri, nper, conn, nnodes, xy, uv, nfield, sizes, fields);
+    if (rc) return rc;
+    const Columns C{ntri, nper, conn, xy, uv, nfield, sizes, fields};
+    const int64_t cols = C.count();
+    std::vector<double> col((size_t)ntri), sorted;
+    std::vector<unsigned char> norm((size_t)(cols - 6) * ntri);
+    for (int64_t c = 6; c < cols; c++) {
+        for (int64_t t = 0; t < ntri; t++) col[t] = C.at(t, c);
+        normalize(col.data(), (size_t)ntri, 1, sorted, norm.data() + (c - 6) * ntri);
+    }
+    std::ofstream out(path, std::ios::out);
+    if (!out) { amie::set_error("cannot open %s", path); return AMIE_ERR_ARG; }
+    out << "BIN_TRIANGLES" << std::endl;
+    out << (int)cols << std::endl;
+    out << 3 << std::endl;
+    out << ((int)1 - 6) / 3 << std::endl;
+    for (int64_t t = 0; t < ntri; t++) {
+        for (int64_t c = 0; c < 6; c++) out << C.at(t, c) << " ";
+        for (int64_t c = 6; c < cols; c++) out << (unsigned short)norm[(c - 6) * ntri + t] << " ";
         out << std::endl;
     }
     out.close();
@@ -78,19 +164,10 @@ int amie_write_voxels(const char *base, int32_t nx, int32_t ny, int32_t nz, int3
         return AMIE_ERR_ARG;
     }
     const size_t n = (size_t)nx * ny * nz;
-    std::vector<double> sorted(n);
+    std::vector<double> sorted;
     std::vector<unsigned char> bytes(n);
-    const double lo = 1., hi = 255.;
     for (int c = 0; c < ncomp; c++) {
-        const double *val = values + (size_t)c * n;
-        sorted.assign(val, val + n);
-        std::sort(sorted.begin(), sorted.end());
-        double vmax = sorted[std::min(n * 1 - 1, (size_t)(n * .99))];
-        double vmin = sorted[(size_t)(n * .01)];
-        if (std::abs(vmax) > std::abs(vmin)) vmin = -std::abs(vmax);
-        else vmax = std::abs(vmin);
-        for (size_t i = 0; i < n; i++)
-            bytes[i] = (unsigned char)std::min(std::max(std::round(lo + (hi - lo) * ((val[i] - vmin) / (vmax - vmin))), lo), hi);
+        normalize(values + (size_t)c * n, n, 1, sorted, bytes.data());
         std::ostringstream name;
         name << base << "_" << nx << "_" << c << ".vox";
         std::ofstream out(name.str(), std::ios::out | std::ios::binary);
