@@ -182,8 +182,9 @@ def serving(ag, ctx, reqs, precision, rtol, limit_s=90.0):
 def single_problem_legs(ag, configs, ctx, rtol, maxit=0):
     """The single-problem configurations beside the batch: P2 400^2 composite
     (1.28 M DOFs) solved once, and the Tet4 64^3 damage case (824 k DOFs)
-    through 10 load steps (solve + damage update each), FIX48 mixed
-    precision against the plain FP64 CG.  The solves run without per-launch
+    through 10 load steps (solve + damage update each) with the 3x3
+    block-Jacobi preconditioner its configuration names, point Jacobi
+    beside it; FIX48 mixed precision against the plain FP64 CG.  The solves run without per-launch
     events (a problem this size keeps every tile active, so the SpMV is
     timed on its own with amie_gpu_batch_time_spmv)."""
     out = {}
@@ -212,27 +213,34 @@ def single_problem_legs(ag, configs, ctx, rtol, maxit=0):
     B.close()
 
     p3 = configs.config3(64)
-    leg = {"workload": f"Tet4 64^3 damage case (d0 field seed 3), 10 load steps of 2e-5, Jacobi, rel.res {rtol:g}",
+    leg = {"workload": f"Tet4 64^3 damage case (d0 field seed 3), 10 load steps of 2e-5, block-Jacobi (3x3) as the "
+                       f"configuration names it, and point Jacobi beside it, rel.res {rtol:g}",
            "dofs": int(p3.ndof)}
-    for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
+
+    def damage_run(prec, precond):
         B = ag.Batch(ctx, [p3])
-        tuned = B.autotune(prec, ag.JACOBI)
-        us_alone, by_alone = B.time_spmv(prec, ag.JACOBI, 50)
+        tuned = B.autotune(prec, precond)
+        us_alone, by_alone = B.time_spmv(prec, precond, 50)
         t = time.perf_counter()
         its = 0
         for s in range(1, 11):
             if s > 1:
                 B.scale_bc(s / (s - 1))
             B.assemble()
-            st = timed_solve(B, prec, ag.JACOBI, use_x0=1 if s > 1 else 0)
+            st = timed_solve(B, prec, precond, use_x0=1 if s > 1 else 0)
             its += st["iterations_max"]
             B.damage_step()
         dt = time.perf_counter() - t
         d, _ = B.damage()
-        leg[name] = {"ten_steps_ms": 1e3 * dt, "iterations": its, "damage_mean": float(d.mean()), "spmv_kernel": tuned,
-                     "spmv_us": us_alone, "spmv_gbs": by_alone / us_alone / 1e3, "spmv_bytes_per_launch": by_alone}
         B.close()
+        return {"ten_steps_ms": 1e3 * dt, "iterations": its, "damage_mean": float(d.mean()), "spmv_kernel": tuned,
+                "spmv_us": us_alone, "spmv_gbs": by_alone / us_alone / 1e3, "spmv_bytes_per_launch": by_alone}
+
+    for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
+        leg[name] = damage_run(prec, ag.BLOCK_JACOBI)
     leg["mixed_vs_fp64"] = leg["fp64"]["ten_steps_ms"] / leg["mixed"]["ten_steps_ms"]
+    leg["jacobi"] = {name: damage_run(prec, ag.JACOBI) for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64))}
+    leg["jacobi"]["mixed_vs_fp64"] = leg["jacobi"]["fp64"]["ten_steps_ms"] / leg["jacobi"]["mixed"]["ten_steps_ms"]
     out["tet_64_damage"] = leg
     log(f"Tet4 64^3: {leg}")
     return out
@@ -250,7 +258,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--maxit", type=int, default=0, help="cap on CG iterations (profiling runs only)")
-    ap.add_argument("--serve-rates", default="10:8,50:8,200:2",
+    ap.add_argument("--serve-rates", default="10:8,50:8,200:2,2000:0.2",
                     help="serving legs as RATE:SECONDS (Poisson arrivals in req/s over an arrival window)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the P2 400^2 and Tet4 64^3 legs")
