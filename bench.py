@@ -197,17 +197,22 @@ def single_problem_legs(ag, configs, ctx, rtol, maxit=0):
     B = ag.Batch(ctx, [p2])
     leg = {"workload": f"P2 400^2 composite, 35 % inclusions (seed 2), uniaxial, Jacobi, rel.res {rtol:g}",
            "dofs": int(p2.ndof)}
+    x = {}
     for name, prec in (("mixed", ag.MIXED_FIX48), ("fp64", ag.FP64)):
         B.assemble()
         tuned = B.autotune(prec, ag.JACOBI)
         t = time.perf_counter()
         st = timed_solve(B, prec, ag.JACOBI)
         dt = time.perf_counter() - t
+        x[name] = B.solution(0).copy()
         us, by = B.time_spmv(prec, ag.JACOBI, 50)
         leg[name] = {"solve_ms": 1e3 * dt, "iterations": st["iterations_max"], "status": st["status"],
                      "spmv_kernel": tuned, "spmv_us": us, "spmv_gbs": by / (us * 1e-6) / 1e9,
                      "spmv_bytes_per_launch": by}
     leg["mixed_vs_fp64"] = leg["fp64"]["solve_ms"] / leg["mixed"]["solve_ms"]
+    # the mixed solution against the plain FP64 CG's (the FP64 CPU oracle pins
+    # both on smaller meshes in the GPU tests)
+    leg["displacement_rel_err_vs_fp64"] = float(np.linalg.norm(x["mixed"] - x["fp64"]) / np.linalg.norm(x["fp64"]))
     out["p2_400"] = leg
     log(f"P2 400^2: {leg}")
     B.close()
