@@ -213,6 +213,10 @@ def single_problem_legs(ag, configs, ctx, rtol, maxit=0):
     # the mixed solution against the plain FP64 CG's (the FP64 CPU oracle pins
     # both on smaller meshes in the GPU tests)
     leg["displacement_rel_err_vs_fp64"] = float(np.linalg.norm(x["mixed"] - x["fp64"]) / np.linalg.norm(x["fp64"]))
+    st = B.solve(precision=ag.FP64, precond=ag.JACOBI, rtol=1e-12, use_graph=0, time_kernels=0)
+    xt = B.solution(0)
+    leg["displacement_rel_err_vs_fp64_1e-12"] = float(np.linalg.norm(x["mixed"] - xt) / np.linalg.norm(xt))
+    leg["fp64_1e-12_iterations"] = st["iterations_max"]
     out["p2_400"] = leg
     log(f"P2 400^2: {leg}")
     B.close()
@@ -345,6 +349,15 @@ def main():
         errs = [float(np.linalg.norm(x_head[k] - B.solution(k)) / np.linalg.norm(B.solution(k))) for k in sample]
         accuracy = {"displacement_rel_err_vs_fp64_max": max(errs), "sample_rves": len(sample),
                     "ceff_rel_diff_vs_fp64_max": float(np.abs(ceff - ceff64).max() / np.abs(ceff64).max())}
+        # against the FP64 CG solved to rel. res 1e-12 (the converged FP64
+        # solution the 1e-7 displacement target refers to)
+        B.assemble()
+        st_t = B.solve(precision=ag.FP64, rtol=1e-12, use_graph=0, time_kernels=0)
+        ceff_t = B.average_stress()
+        errs_t = [float(np.linalg.norm(x_head[k] - B.solution(k)) / np.linalg.norm(B.solution(k))) for k in sample]
+        accuracy["displacement_rel_err_vs_fp64_1e-12_max"] = max(errs_t)
+        accuracy["ceff_rel_diff_vs_fp64_1e-12_max"] = float(np.abs(ceff - ceff_t).max() / np.abs(ceff_t).max())
+        accuracy["fp64_1e-12_status"] = st_t["status"]
         log(f"accuracy vs FP64 CG: {accuracy}")
 
     tr = measured_traffic()
