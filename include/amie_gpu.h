@@ -254,6 +254,14 @@ int amie_gpu_add_midpoints(amie_gpu_ctx *ctx, int64_t nnodes, const double *xy, 
 int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int32_t *conn, int64_t nnodes,
                          const double *xy, const double *uv, int32_t nfield, const int32_t *sizes,
                          const double *const *fields);
+/* MultiTriangleWriter (triangle_writer.cpp:73-86, 538-573, 1493-1500): the
+ * index file `head` starts with "MULTI"; each append writes <base>_<step>
+ * (step 1, 2, ... as the caller counts) in the TriangleWriter format and
+ * adds that name to the index file.  Host only. */
+int amie_multi_triangles_begin(const char *head);
+int amie_multi_triangles_append(const char *head, const char *base, int32_t step, int64_t ntri, int32_t nper,
+                                const int32_t *conn, int64_t nnodes, const double *xy, const double *uv,
+                                int32_t nfield, const int32_t *sizes, const double *const *fields);
 /* BinaryTriangleWriter (triangle_writer.cpp:447-478, 1477-1491), same
  * arguments: header "BIN_TRIANGLES", the column count, 3, -1 (the
  * reference's (layers - 6) / 3 for one layer); per triangle the six

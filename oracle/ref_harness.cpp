@@ -610,8 +610,8 @@ static void golden_placement(const std::string &tag, int n, double rmin, double 
 }
 
 // TriangleWriter (utilities/writer/triangle_writer.cpp:31-86, 407-445,
-// 578-608, 1024-1075, 1461-1475) and BinaryTriangleWriter (:447-478,
-// 1477-1491) after a FeatureTree solve of a unit box
+// 578-608, 1024-1075, 1461-1475), BinaryTriangleWriter (:447-478,
+// 1477-1491) and MultiTriangleWriter after a FeatureTree solve of a unit box
 // (u_x = 0 left, u_y = 0 bottom, u_x = 1e-3 right): the file it writes, and
 // the inputs it read -- mesh, DOF displacements, and the displacement,
 // strain and stress the element states give at each element's three
@@ -635,6 +635,13 @@ static void golden_writer(const std::string &tag, double sampling)
     btrg.getField(TOTAL_STRAIN_FIELD);
     btrg.getField(REAL_STRESS_FIELD);
     btrg.write();
+    // MultiTriangleWriter (:73-86, 538-573, 1493-1520): an index file and one
+    // TriangleWriter file per append; two appends of the same fields
+    MultiTriangleWriter mtrg(g_out + "/" + tag + "_multi_head.txt", g_out + "/" + tag + "_multi", &F, 1);
+    mtrg.getField(TOTAL_STRAIN_FIELD);
+    mtrg.getField(REAL_STRESS_FIELD);
+    mtrg.append();
+    mtrg.append();
     dump_mesh(tag, F.get2DMesh(), 2);
     npy_f64(tag + "_u", vec(F.getDisplacements()));
     std::vector<double> eps, sig, uv;

@@ -189,6 +189,25 @@ def test_binary_triangle_writer_matches_reference_file(tmp_path):
     assert out.read_bytes() == open(os.path.join(ref, "writer_p1_btrg.txt"), "rb").read()
 
 
+def test_multi_triangle_writer_matches_reference(tmp_path):
+    """MultiTriangleWriter of the same reference solve, appended twice: the
+    index file (the reference's, with its directory replaced by ours) and
+    two step files equal to the reference's TriangleWriter file."""
+    ref = os.path.join(ROOT, "tests", "golden", "ref")
+    g = lambda n: np.load(os.path.join(ref, "writer_p1_" + n + ".npy"))
+    want_head = open(os.path.join(ref, "writer_p1_multi_head.txt")).read().splitlines()
+    ref_base = want_head[1][: -len("_1")]
+    base = str(tmp_path / "writer_p1_multi")
+    w = ag.MultiTriangleWriter(tmp_path / "head.txt", base)
+    for _ in range(2):
+        w.append(g("conn"), g("coords"), g("uvertex"), [g("strain"), g("stress")])
+    got = (tmp_path / "head.txt").read_text().splitlines()
+    assert got == [line.replace(ref_base, base) for line in want_head]
+    trg = open(os.path.join(ref, "writer_p1_trg.txt"), "rb").read()
+    for step in (1, 2):
+        assert open(f"{base}_{step}", "rb").read() == trg
+
+
 def test_triangle_writer_argument_errors(tmp_path):
     conn = np.array([[0, 1, 2]], dtype=np.int32)
     xy = np.zeros((3, 2))

@@ -78,6 +78,9 @@ SIGNATURES = {
     "amie_gpu_renumber": (c_i32, [vp, c_i64, c_i64, c_i32, vp, vp, vp, vp]),
     "amie_write_triangles": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, c_i64, vp, vp, c_i32, vp, vp]),
     "amie_write_triangles_bin": (c_i32, [ctypes.c_char_p, c_i64, c_i32, vp, c_i64, vp, vp, c_i32, vp, vp]),
+    "amie_multi_triangles_begin": (c_i32, [ctypes.c_char_p]),
+    "amie_multi_triangles_append": (c_i32, [ctypes.c_char_p, ctypes.c_char_p, c_i32, c_i64, c_i32, vp, c_i64, vp, vp,
+                                            c_i32, vp, vp]),
     "amie_gpu_voxelize": (c_i32, [vp, c_i64, vp, c_i64, vp, c_i32, vp, vp, c_i32, c_i32, c_i32, vp, vp]),
     "amie_write_voxels": (c_i32, [ctypes.c_char_p, c_i32, c_i32, c_i32, c_i32, vp]),
     "amie_place_inclusions_2d": (c_i32, [c_dbl, c_dbl, c_dbl, c_dbl, c_i64, vp, c_dbl, c_i64, c_dbl, c_u64,
@@ -363,11 +366,23 @@ def write_voxels(base, values, n):
     check(lib().amie_write_voxels(str(base).encode(), n, n, n, values.shape[0], _p(values)))
 
 
-def write_triangles(path, conn, xy, u, fields=(), binary=False):
-    """TriangleWriter (binary: BinaryTriangleWriter) file: conn ntri x nper, xy nnodes x 2; u either
-    ntri x 3 x 2 (values at the vertices) or nodal (nnodes x 2, or flat by
-    DOF); fields a sequence of ntri x 3 x size arrays (values at the
-    vertices) or ntri x size arrays (constant over the element)."""
+class MultiTriangleWriter:
+    """MultiTriangleWriter: an index file `head` and one TriangleWriter file
+    <base>_<step> per append."""
+
+    def __init__(self, head, base):
+        self.head, self.base, self.step = str(head), str(base), 0
+        check(lib().amie_multi_triangles_begin(self.head.encode()))
+
+    def append(self, conn, xy, u, fields=()):
+        self.step += 1
+        conn, xy, u, sizes, ptrs, keep = _triangle_args(conn, xy, u, fields)
+        check(lib().amie_multi_triangles_append(self.head.encode(), self.base.encode(), self.step, conn.shape[0],
+                                                conn.shape[1], _p(conn), xy.shape[0], _p(xy), _p(u), len(keep),
+                                                _p(sizes), ptrs))
+
+
+def _triangle_args(conn, xy, u, fields):
     conn = _c(conn, np.int32)
     xy = _c(xy, np.float64)
     u = np.asarray(u, dtype=np.float64)
@@ -383,9 +398,19 @@ def write_triangles(path, conn, xy, u, fields=(), binary=False):
         fs.append(np.ascontiguousarray(f))
     sizes = np.array([f.shape[2] for f in fs], dtype=np.int32)
     ptrs = (ctypes.c_void_p * max(1, len(fs)))(*[f.ctypes.data for f in fs])
+    return conn, xy, u, sizes, ptrs, fs
+
+
+def write_triangles(path, conn, xy, u, fields=(), binary=False):
+    """TriangleWriter (binary: BinaryTriangleWriter) file: conn ntri x nper,
+    xy nnodes x 2; u either ntri x 3 x 2 (values at the vertices) or nodal
+    (nnodes x 2, or flat by DOF); fields a sequence of ntri x 3 x size arrays
+    (values at the vertices) or ntri x size arrays (constant over the
+    element)."""
+    conn, xy, u, sizes, ptrs, keep = _triangle_args(conn, xy, u, fields)
     fn = lib().amie_write_triangles_bin if binary else lib().amie_write_triangles
-    check(fn(str(path).encode(), conn.shape[0], conn.shape[1], _p(conn), xy.shape[0], _p(xy), _p(u), len(fs), _p(sizes),
-             ptrs))
+    check(fn(str(path).encode(), conn.shape[0], conn.shape[1], _p(conn), xy.shape[0], _p(xy), _p(u), len(keep),
+             _p(sizes), ptrs))
 
 
 def default_opts(**kw):

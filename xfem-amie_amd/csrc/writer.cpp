@@ -114,6 +114,37 @@ int amie_write_triangles(const char *path, int64_t ntri, int32_t nper, const int
     return AMIE_OK;
 }
 
+// MultiTriangleWriter (triangle_writer.cpp:73-86, 538-573, 1493-1500): the
+// index file starts with "MULTI"; every append writes <base>_<step> in the
+// TriangleWriter format (step counting 1, 2, ... from the first append) and
+// adds that name to the index file.
+int amie_multi_triangles_begin(const char *head)
+{
+    if (!head) { amie::set_error("bad arguments"); return AMIE_ERR_ARG; }
+    std::ofstream out(head, std::ios::out);
+    if (!out) { amie::set_error("cannot open %s", head); return AMIE_ERR_ARG; }
+    out << "MULTI" << std::endl;
+    out.close();
+    if (!out) { amie::set_error("write to %s failed", head); return AMIE_ERR_ARG; }
+    return AMIE_OK;
+}
+
+int amie_multi_triangles_append(const char *head, const char *base, int32_t step, int64_t ntri, int32_t nper,
+                                const int32_t *conn, int64_t nnodes, const double *xy, const double *uv,
+                                int32_t nfield, const int32_t *sizes, const double *const *fields)
+{
+    if (!head || !base || step < 1) { amie::set_error("bad arguments"); return AMIE_ERR_ARG; }
+    const std::string name = std::string(base) + "_" + std::to_string(step);
+    int rc = amie_write_triangles(name.c_str(), ntri, nper, conn, nnodes, xy, uv, nfield, sizes, fields);
+    if (rc) return rc;
+    std::ofstream out(head, std::ios::out | std::ios::app);
+    if (!out) { amie::set_error("cannot open %s", head); return AMIE_ERR_ARG; }
+    out << name << std::endl;
+    out.close();
+    if (!out) { amie::set_error("write to %s failed", head); return AMIE_ERR_ARG; }
+    return AMIE_OK;
+}
+
 // BinaryTriangleWriter (triangle_writer.cpp:447-478, 1477-1491): the header
 // "BIN_TRIANGLES", the column count, 3, and the reference's (layers - 6) / 3
 // for one layer (-1); per triangle the six coordinates as doubles, then
