@@ -51,34 +51,82 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(problems, budget_s=15.0, rtol=1e-8):
-    """Oracle FP64 Jacobi PCG (OpenMP) on RVEs drawn in a seeded order until
-    the time budget is spent.  Returns solves/s and the sample description."""
+def _write_system(path, A, f, eps, maxit):
+    """A system in the layout the reference harness's --cg mode reads."""
+    with open(path, "wb") as fh:
+        fh.write(np.int32(A.stride).tobytes())
+        fh.write(np.int64(A.nrows).tobytes())
+        fh.write(np.ascontiguousarray(A.row_size, dtype=np.uint32).tobytes())
+        fh.write(np.int64(len(A.column_index)).tobytes())
+        fh.write(np.ascontiguousarray(A.column_index, dtype=np.uint32).tobytes())
+        fh.write(np.ascontiguousarray(A.array, dtype=np.float64).tobytes())
+        fh.write(np.ascontiguousarray(f, dtype=np.float64).tobytes())
+        fh.write(np.float64(eps).tobytes())
+        fh.write(np.int64(maxit).tobytes())
+
+
+def cpu_baseline(problems, budget_s=20.0, rtol=1e-8):
+    """The reference's own threaded ConjugateGradient::solve (compiled from
+    its sources into oracle/_ref/ref_harness, --cg mode: InverseDiagonal,
+    nssor 32) on RVEs drawn in a seeded order (n <= 128, so that one solve
+    stays within seconds) until the time budget is spent; its absolute stop
+    on sqrt(r.z) is set to rtol * sqrt(b.D^-1 b), the relative residual it
+    reached is checked.  The oracle's OpenMP PCG is timed on the same sample
+    beside it.  Without the compiled reference, the oracle alone (kind
+    "port")."""
+    import subprocess
+    import tempfile
     import oracle as orc
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    use_ref = os.path.exists(ref)
     order = np.random.default_rng(11).permutation(len(problems))
-    solved, t_solve, ndof = 0, 0.0, 0
-    used = []
-    for k in order:
-        p = problems[k]
-        A = orc.assemble_periodic(1, p.conn, p.dofmap, p.coords, p.cmats, p.phase)
-        loads = [orc.macro_rhs(1, p.conn, p.dofmap, p.coords, p.cmats, p.phase, p.macro[lc], A.nrows)
-                 for lc in range(p.nrhs)]
-        orc.apply_dirichlet(A, np.zeros(A.n), p.bc_dofs, p.bc_vals)
-        t0 = time.perf_counter()
-        for f in loads:
-            f[p.bc_dofs] = p.bc_vals
-            x, it, rr, ok = orc.pcg(A, f, rtol=rtol, maxit=200000)
-        t_solve += time.perf_counter() - t0
-        solved += len(loads)
-        ndof += A.n
-        used.append(int(np.sqrt(p.ndofnodes)))
-        log(f"cpu baseline: rve n={used[-1]} {len(loads)} solves, total {t_solve:.1f}s")
-        if t_solve >= budget_s:
-            break
-    threads = orc.get_threads()
-    return {"value": solved / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{len(used)} of the {len(problems)} RVEs (n = {sorted(used)}), {solved} load-case solves, "
-                      f"oracle FP64 Jacobi PCG to rel. res {rtol:g}, {threads} OpenMP threads, assembly excluded"}
+    solved, t_ref, t_port, used, worst, threads, iters = 0, 0.0, 0.0, [], 0.0, 0, 0
+    with tempfile.TemporaryDirectory() as tmp:
+        sysfile = os.path.join(tmp, "system.bin")
+        for k in order:
+            p = problems[k]
+            n = int(np.sqrt(p.ndofnodes))
+            if use_ref and n > 128:
+                continue
+            A = orc.assemble_periodic(1, p.conn, p.dofmap, p.coords, p.cmats, p.phase)
+            loads = [orc.macro_rhs(1, p.conn, p.dofmap, p.coords, p.cmats, p.phase, p.macro[lc], A.nrows)
+                     for lc in range(p.nrhs)]
+            orc.apply_dirichlet(A, np.zeros(A.n), p.bc_dofs, p.bc_vals)
+            dinv = orc.inverse_diagonal(A)
+            for f in loads:
+                f[p.bc_dofs] = p.bc_vals
+                t0 = time.perf_counter()
+                orc.pcg(A, f, rtol=rtol, maxit=200000)
+                t_port += time.perf_counter() - t0
+                if use_ref:
+                    _write_system(sysfile, A, f, rtol * np.sqrt(f @ (dinv * f)), 10 * A.n)
+                    out = subprocess.run([ref, "--cg", sysfile], capture_output=True, text=True, timeout=600)
+                    if out.returncode != 0:
+                        raise RuntimeError(f"reference CG failed: {out.stderr[-300:]}")
+                    it, secs, conv, thr = out.stdout.strip().split("\n")[-1].split()
+                    x = np.fromfile(sysfile + ".x")
+                    worst = max(worst, float(np.linalg.norm(f - orc.spmv(A, x)) / np.linalg.norm(f)))
+                    t_ref += float(secs)
+                    iters += int(it)
+                    threads = int(thr)
+            solved += len(loads)
+            used.append(n)
+            spent = t_ref if use_ref else t_port
+            log(f"cpu baseline: rve n={n} {len(loads)} solves, reference {t_ref:.1f}s, oracle {t_port:.1f}s")
+            if spent >= budget_s:
+                break
+    port = {"value": solved / t_port, "unit": "solves/s", "cores": orc.get_threads(), "kind": "port",
+            "sample": "the same solves with the oracle's FP64 Jacobi PCG (OpenMP)"}
+    if not use_ref:
+        port["sample"] = (f"{len(used)} of the {len(problems)} RVEs (n = {sorted(used)}), {solved} load-case solves, "
+                          f"oracle FP64 Jacobi PCG to rel. res {rtol:g}, assembly excluded")
+        return port
+    return {"value": solved / t_ref, "unit": "solves/s", "cores": threads, "kind": "reference",
+            "sample": f"{len(used)} of the {len(problems)} RVEs with n <= 128 (n = {sorted(used)}), {solved} load-case "
+                      f"solves by the reference's ConjugateGradient::solve (InverseDiagonal, nssor 32, {iters} "
+                      f"iterations in all), stop sqrt(r.z) <= {rtol:g} sqrt(b.D^-1 b), worst reached rel. res "
+                      f"{worst:.2e}, {threads} OpenMP threads, assembly and file transfer excluded",
+            "port": port}
 
 
 def ingest_input(ag, n):
@@ -265,7 +313,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--maxit", type=int, default=0, help="cap on CG iterations (profiling runs only)")
     ap.add_argument("--serve-rates", default="10:8,50:8,200:2,2000:0.2",
                     help="serving legs as RATE:SECONDS (Poisson arrivals in req/s over an arrival window)")

@@ -662,9 +662,57 @@ static void golden_writer(const std::string &tag, double sampling)
     npy_f64(tag + "_stress", sig, {sig.size() / 9, 3, 3});
 }
 
+// The reference's own ConjugateGradient::solve (solvers/conjugategradient.cpp:
+// 69-318, default InverseDiagonal preconditioner, nssor 32 as
+// Assembly::cgsolve sets it, solvers/assembly.h:262 and assembly.cpp:1842)
+// on a system read from a file, with every OpenMP thread the environment
+// gives; the CPU baseline of the benchmark.  File (native byte order):
+// int32 stride, int64 nrows, uint32 row_size[nrows], int64 nnzb,
+// uint32 column_index[nnzb], double array[nnzb*stride*colLen],
+// double rhs[nrows*stride], double eps (absolute, on sqrt(r.z)), int64 maxit.
+// Writes the solution to <file>.x and prints "iterations seconds converged
+// threads" on one line; seconds cover the solve only.
+static int run_cg(const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) { fprintf(stderr, "cannot open %s\n", path); return 2; }
+    int32_t stride = 0;
+    int64_t nrows = 0, nnzb = 0, maxit = 0;
+    bool ok = fread(&stride, 4, 1, f) == 1 && fread(&nrows, 8, 1, f) == 1;
+    std::valarray<unsigned int> rs(ok ? (size_t)nrows : 0);
+    ok = ok && fread(&rs[0], 4, nrows, f) == (size_t)nrows && fread(&nnzb, 8, 1, f) == 1;
+    std::valarray<unsigned int> ci(ok ? (size_t)nnzb : 0);
+    ok = ok && fread(&ci[0], 4, nnzb, f) == (size_t)nnzb;
+    const size_t cl = stride + stride % 2, nval = (size_t)nnzb * stride * cl, n = (size_t)nrows * stride;
+    std::vector<double> vals(ok ? nval : 0), rhs(ok ? n : 0);
+    double eps = 0.;
+    ok = ok && fread(vals.data(), 8, nval, f) == nval && fread(rhs.data(), 8, n, f) == n && fread(&eps, 8, 1, f) == 1 &&
+         fread(&maxit, 8, 1, f) == 1;
+    fclose(f);
+    if (!ok) { fprintf(stderr, "short file %s\n", path); return 2; }
+    Assembly K;
+    K.coordinateIndexedMatrix = new CoordinateIndexedSparseMatrix(rs, ci, stride);
+    for (size_t q = 0; q < nval; q++) K.coordinateIndexedMatrix->array[q] = vals[q];
+    K.externalForces.resize(n);
+    for (size_t q = 0; q < n; q++) K.externalForces[q] = rhs[q];
+    ConjugateGradient cg(&K);
+    cg.nssor = 32;
+    Vector x0(0., n);
+    const double t0 = omp_get_wtime();
+    const bool conv = cg.solve(x0, nullptr, eps, (int)maxit, false);
+    const double dt = omp_get_wtime() - t0;
+    std::string out = std::string(path) + ".x";
+    FILE *g = fopen(out.c_str(), "wb");
+    if (!g || fwrite(&cg.x[0], 8, n, g) != n) { fprintf(stderr, "cannot write %s\n", out.c_str()); return 2; }
+    fclose(g);
+    printf("%lld %.9f %d %d\n", (long long)cg.nit, dt, (int)conv, omp_get_max_threads());
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
-    if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
+    if (argc == 3 && std::string(argv[1]) == "--cg") return run_cg(argv[2]);
+    if (argc < 2) { fprintf(stderr, "usage: %s outdir | --cg system.bin\n", argv[0]); return 2; }
     g_out = argv[1];
     omp_set_num_threads(1);
     golden_cauchy_green();
