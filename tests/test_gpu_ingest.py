@@ -151,3 +151,16 @@ def test_voxel_grid_must_be_cubic(ctx):
         box = np.array([0.5, 0.5, 0.5, 1, 1, 1.0])
         ag.check(check(ctx.h, 4, xyz.ctypes.data, 1, conn.ctypes.data, 1, field.ctypes.data, box.ctypes.data, 2, 3, 4,
                        vals.ctypes.data, None))
+
+
+def test_many_triangles_on_one_edge_vs_oracle(ctx):
+    """A fan of 20 000 triangles on one edge (0, 1): one run of 20 000 sides
+    after the sort, numbered in linear time."""
+    k = np.arange(2, 20002, dtype=np.int32)
+    conn3 = np.stack([np.zeros_like(k), np.ones_like(k), k], axis=1).astype(np.int32)
+    xy = np.random.default_rng(2).normal(size=(20002, 2))
+    c6, mid = ag.add_midpoints_gpu(ctx, xy, conn3)
+    o6, omid = orc.add_midpoints(xy, conn3)
+    np.testing.assert_array_equal(c6, o6)
+    np.testing.assert_array_equal(mid, omid)
+    assert (c6[:, 1] == xy.shape[0]).all()

@@ -69,18 +69,29 @@ __global__ void k_side_keys(const int32_t *conn3, int64_t nsides, int64_t nnodes
     val[q] = (uint32_t)q;
 }
 
-// after the stable sort: every side learns the first side of its edge (runs
-// hold ascending positions, so the run head is the smallest)
-__global__ void k_side_first(const uint64_t *skey, const uint32_t *sval, int64_t nsides, uint32_t *first,
-                             uint32_t *makes)
+// after the stable sort: a run of one edge starts where the key changes;
+// head[i] = i there, 0 elsewhere, and a running maximum gives every sorted
+// side the start of its run
+__global__ void k_run_heads(const uint64_t *skey, int64_t nsides, uint32_t *head)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsides) head[i] = (i == 0 || skey[i] != skey[i - 1]) ? (uint32_t)i : 0u;
+}
+
+// every side learns the first side of its edge (runs hold ascending
+// positions, so the run head is the smallest)
+__global__ void k_side_first(const uint32_t *sval, const uint32_t *hmax, int64_t nsides, uint32_t *first, uint32_t *makes)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsides) return;
-    int64_t h = i;
-    while (h > 0 && skey[h - 1] == skey[i]) h--;
+    const uint32_t h = hmax[i];
     first[sval[i]] = sval[h];
-    makes[sval[i]] = h == i;
+    makes[sval[i]] = h == (uint32_t)i;
 }
+
+struct MaxOp {
+    __device__ __host__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
 
 __global__ void k_p2_conn(const int32_t *conn3, int64_t nsides, const uint32_t *first, const uint32_t *rank,
                           int64_t nnodes, const double *xy, int32_t *conn6, double *mid)
@@ -189,20 +200,23 @@ int amie_gpu_add_midpoints(amie_gpu_ctx *ctx, int64_t nnodes, const double *xy, 
     if (!conn_in_range(conn3, ns, nnodes)) { set_error("connectivity names a node outside [0, nnodes)"); return AMIE_ERR_ARG; }
     AMIE_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    DevBuf dconn, dxy, key, key2, val, val2, first, makes, rank, dconn6, dmid, tmp;
+    DevBuf dconn, dxy, key, key2, val, val2, first, makes, rank, dconn6, dmid, tmp, head, hmax;
     if (!dconn.alloc(ns * 4) || !dxy.alloc(nnodes * 16) || !key.alloc(ns * 8) || !key2.alloc(ns * 8) ||
         !val.alloc(ns * 4) || !val2.alloc(ns * 4) || !first.alloc(ns * 4) || !makes.alloc(ns * 4) ||
-        !rank.alloc(ns * 4) || !dconn6.alloc(ntri * 24) || !dmid.alloc(ns * 16))
+        !rank.alloc(ns * 4) || !dconn6.alloc(ntri * 24) || !dmid.alloc(ns * 16) || !head.alloc(ns * 4) ||
+        !hmax.alloc(ns * 4))
         return AMIE_ERR_NOMEM;
     // edge keys < nnodes^2: sort only the bits that can be set
     const uint64_t kmax = (uint64_t)nnodes * (uint64_t)nnodes - 1;
     int end_bit = 1;
     while (end_bit < 64 && (kmax >> end_bit)) end_bit++;
-    size_t need_sort = 0, need_scan = 0;
+    size_t need_sort = 0, need_scan = 0, need_max = 0;
     AMIE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need_sort, key.as<uint64_t>(), key2.as<uint64_t>(),
                                                 val.as<uint32_t>(), val2.as<uint32_t>(), (int)ns, 0, end_bit, st));
     AMIE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need_scan, makes.as<uint32_t>(), rank.as<uint32_t>(), (int)ns, st));
-    const size_t need = std::max(need_sort, need_scan);
+    AMIE_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, need_max, head.as<uint32_t>(), hmax.as<uint32_t>(), MaxOp(),
+                                               (int)ns, st));
+    const size_t need = std::max(std::max(need_sort, need_scan), need_max);
     if (!tmp.alloc(need ? need : 4)) return AMIE_ERR_NOMEM;
     AMIE_HIP(hipMemcpyAsync(dconn.p, conn3, ns * 4, hipMemcpyHostToDevice, st));
     AMIE_HIP(hipMemcpyAsync(dxy.p, xy, nnodes * 16, hipMemcpyHostToDevice, st));
@@ -217,7 +231,10 @@ int amie_gpu_add_midpoints(amie_gpu_ctx *ctx, int64_t nnodes, const double *xy, 
     size_t have = need;
     AMIE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, have, key.as<uint64_t>(), key2.as<uint64_t>(), val.as<uint32_t>(),
                                                 val2.as<uint32_t>(), (int)ns, 0, end_bit, st));
-    hipLaunchKernelGGL(k_side_first, dim3(blocks(ns)), dim3(256), 0, st, key2.as<uint64_t>(), val2.as<uint32_t>(), ns,
+    hipLaunchKernelGGL(k_run_heads, dim3(blocks(ns)), dim3(256), 0, st, key2.as<uint64_t>(), ns, head.as<uint32_t>());
+    have = need;
+    AMIE_HIP(hipcub::DeviceScan::InclusiveScan(tmp.p, have, head.as<uint32_t>(), hmax.as<uint32_t>(), MaxOp(), (int)ns, st));
+    hipLaunchKernelGGL(k_side_first, dim3(blocks(ns)), dim3(256), 0, st, val2.as<uint32_t>(), hmax.as<uint32_t>(), ns,
                        first.as<uint32_t>(), makes.as<uint32_t>());
     have = need;
     AMIE_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, have, makes.as<uint32_t>(), rank.as<uint32_t>(), (int)ns, st));
