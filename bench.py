@@ -3,7 +3,7 @@
 Workload (one "step"): 512 periodic P1 RVEs (n ~ U{32..256}, seed 4, 2k-130k
 DOFs each, 20-45 % circular inclusions, 12/60 GPa), three macro-strain load
 cases each (1536 right-hand sides), resident in HBM.  A step runs the whole
-hot path on the device: element integration + coloured scatter + Dirichlet
+hot path on the device: element integration + gather assembly + Dirichlet
 pin + scaling (assembly), the mixed-precision CG (FP32 vectors, the scaled
 matrix in 48-bit fixed point with FP64 accumulation, FP64 refinement) of all 1536
 right-hand sides to relative residual 1e-8 in one batch, and the
@@ -14,8 +14,11 @@ roofline = the SpMV kernel (k_spmv), timed with HIP events on the library
 stream during the timed steps.
 Beside it: the same step with the all-FP32 inner iteration, and with the
 plain FP64 CG (same kernels, FP64 matrix and vectors).
-cpu_baseline = the oracle's threaded FP64 Jacobi PCG (rtol 1e-8) on a bounded
-sample of the same RVEs.
+cpu_baseline = the reference's own threaded ConjugateGradient (compiled from
+its sources into oracle/_ref/ref_harness) on a bounded sample of the same
+RVEs, the oracle's threaded FP64 Jacobi PCG beside it.
+Side legs: P2 400^2 and the Tet4 64^3 damage case, the served request mix,
+mesh input (mid-edge nodes + renumbering) and voxel output on the device.
 """
 import argparse
 import json
