@@ -1,4 +1,4 @@
-// Element integration, coloured scatter, Dirichlet elimination, scaling,
+// Element integration, gather and coloured assembly, Dirichlet elimination, scaling,
 // strain / damage kernels (device side).
 //
 // Element matrices follow the reference quadrature exactly in structure:

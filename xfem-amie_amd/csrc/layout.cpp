@@ -1,6 +1,6 @@
 // Host-side layout builder: sparsity pattern in the reference order, the
 // internal SELL-64 block layout, element -> block maps and the element
-// colouring used by the deterministic scatter.
+// colouring used by the coloured scatter.
 //
 // Pattern: the reference inserts every (ids[j], ids[k]) node pair of every
 // element into a std::set and fills missing diagonals
