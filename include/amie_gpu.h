@@ -125,6 +125,9 @@ typedef struct {
                                 (solvers/conjugategradient.cpp:101,218,305) */
     int32_t autotune;        /* 1: the first solve of a precision / preconditioner on this batch
                                 runs amie_gpu_batch_autotune first (default 0) */
+    int64_t stall_iters;     /* FP32-vector modes with escalate: a right-hand side whose FP64
+                                residual has not halved in this many iterations stalls its problem,
+                                which then continues in AMIE_FP64 (<= 0: 8192) */
 } amie_solve_opts;
 
 typedef struct {
@@ -400,12 +403,23 @@ int amie_gpu_batch_get_damage(amie_gpu_batch *b, int32_t p, double *d, double *k
 int amie_gpu_batch_set_damage(amie_gpu_batch *b, int32_t p, const double *d);
 /* Scale the Dirichlet values of every problem (incremental loading). */
 int amie_gpu_batch_scale_bc(amie_gpu_batch *b, double factor);
+/* New values for Dirichlet conditions of problem p, each DOF one that the
+ * problem was created with (Assembly::setPointAlong on an existing
+ * condition, solvers/assembly.cpp:1590-1634: the new value replaces the
+ * old); they take effect at the next assembly. */
+int amie_gpu_batch_set_bc_values(amie_gpu_batch *b, int32_t p, int64_t nbc, const int64_t *bc_dofs,
+                                 const double *bc_vals);
 
 /* -------------------------------------------------------------- serving
  * Continuous batching of solve requests: each request is admitted when its
  * HBM footprint fits the budget and a slot is free; admitted requests share
  * every kernel launch of the solve loop, and new ones join between
  * iteration chunks while others are in flight. */
+/* status: AMIE_OK, AMIE_NOT_CONVERGED or AMIE_ERR_NAN for a solve that ran;
+ * a request that did not run: AMIE_ERR_ARG (a persistent-key request whose
+ * mesh or Dirichlet DOFs differ from the resident problem's), AMIE_ERR_NOMEM
+ * (its footprint does not fit next to the idle resident problems), or the
+ * error of its batch build (amie_gpu_last_error tells which). */
 typedef struct {
     uint64_t ticket;
     int32_t status;
@@ -420,10 +434,33 @@ int amie_gpu_server_create(amie_gpu_ctx *ctx, size_t hbm_budget, int32_t max_slo
                            amie_gpu_server **out);
 void amie_gpu_server_destroy(amie_gpu_server *s);
 /* Queue a request (copied).  persistent_key != 0 keeps the problem's
- * matrix and damage state resident between requests carrying that key
- * (damage steps); the request then runs one damage step + solve. */
+ * matrix, damage state and solution resident between requests carrying
+ * that key; each such request is one load step in the order of
+ * FeatureTree::stepInternal (features/features.cpp:6150-6373): the
+ * request's Dirichlet values replace the resident ones (same mesh and
+ * Dirichlet DOFs required), the matrix is re-assembled with the damage so
+ * far, the solve starts from the previous solution, then the damage law
+ * updates every element (physics/fracturecriteria/mazars.cpp:113-139, here
+ * with kappa0 1e-4, kappaf 1e-3, dmax 1 - 1e-6; amie_gpu_server_submit_step
+ * takes other parameters).  The first request on a key builds the problem
+ * from its description (its damage field as the initial damage).  A request
+ * that cannot run completes with a negative status (see
+ * amie_request_result). */
 int amie_gpu_server_submit(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t persistent_key,
                            double submit_ms, uint64_t *ticket);
+/* A load step on a persistent key with its own damage law parameters. */
+int amie_gpu_server_submit_step(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t persistent_key,
+                                const amie_damage_params *prm, double submit_ms, uint64_t *ticket);
+/* HBM the server sets aside for a request at admission (an estimate from
+ * the mesh size; the built batch's own size replaces it once built). */
+int amie_gpu_server_request_bytes(const amie_problem_desc *desc, int64_t *bytes);
+/* Frees the resident problem of a key (its HBM leaves the budget's use);
+ * AMIE_ERR_BUSY while a request on it is queued or in flight. */
+int amie_gpu_server_release(amie_gpu_server *s, uint64_t persistent_key);
+/* Damage and kappa history of a key's resident problem (nelem each), as
+ * left by its last load step; AMIE_ERR_BUSY while it is in flight. */
+int amie_gpu_server_get_damage(amie_gpu_server *s, uint64_t persistent_key, double *d, double *kappa,
+                               int64_t nelem);
 /* Advance the solve loop by one chunk (admits, iterates, retires). */
 int amie_gpu_server_step(amie_gpu_server *s, double now_ms, int32_t *n_active, int32_t *n_queued);
 /* Completed requests since the last poll. */

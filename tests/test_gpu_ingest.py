@@ -141,6 +141,27 @@ def test_voxel_sampling_larger_mesh_vs_oracle(ctx):
     np.testing.assert_array_equal(vals, field[own].T)
 
 
+def test_voxel_sampling_ignores_tetrahedra_outside_the_grid(ctx):
+    """Tetrahedra far outside the grid (coordinates near +-1e300, where the
+    reference's index range would not fit an int) own no voxel and leave
+    the others' owners as they are: their index ranges are clamped to the
+    grid before the loops."""
+    xyz, conn = ag.structured_mesh(ag.TET4, 12)
+    field = np.random.default_rng(5).normal(size=(conn.shape[0], 1))
+    box = np.array([0.5, 0.5, 0.5, 1.0, 1.0, 1.0])
+    own = orc.voxel_owner(xyz, conn, box, 24)
+    far = np.array([[1e300, 0, 0], [1e300, 1, 0], [1e300, 0, 1], [1.1e300, 0, 0],
+                    [-1e300, -1e300, -1e300], [-1e300, -1e300, -0.9e300], [-1e300, -0.9e300, -1e300],
+                    [-0.9e300, -1e300, -1e300]])
+    xyz2 = np.vstack([xyz, far])
+    n0 = xyz.shape[0]
+    conn2 = np.vstack([conn, [[n0, n0 + 1, n0 + 2, n0 + 3], [n0 + 4, n0 + 5, n0 + 6, n0 + 7]]]).astype(np.int32)
+    field2 = np.vstack([field, [[7.0], [8.0]]])
+    vals = ag.voxelize_gpu(ctx, xyz2, conn2, field2, box, 24)
+    assert (own >= 0).all()
+    np.testing.assert_array_equal(vals, field[own].T)
+
+
 def test_voxel_grid_must_be_cubic(ctx):
     conn = np.array([[0, 1, 2, 3]], dtype=np.int32)
     xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])

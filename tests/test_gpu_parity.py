@@ -19,6 +19,7 @@ import pytest
 
 import amie_gpu as ag
 import configs
+import homogenisation as hm
 import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -335,6 +336,77 @@ def test_rve_batch_effective_stiffness_vs_oracle(ctx, precision):
         np.testing.assert_allclose(ceff[k], ref, rtol=1e-7, atol=1e-7 * np.abs(ref).max())
         C = ceff[k] / 1e-3
         assert np.abs(C - C.T).max() <= 1e-5 * np.abs(C).max()
+        # strictly inside the Voigt / Reuss bracket (a sign error in the
+        # load would give 2 <C> - C_eff, outside it)
+        voigt, reuss = hm.voigt_reuss(ag.P1, p)
+        sym = 0.5 * (C + C.T)
+        assert np.linalg.eigvalsh(voigt - sym).min() > 1e-4 * np.abs(voigt).max()
+        assert np.linalg.eigvalsh(sym - reuss).min() > 1e-4 * np.abs(voigt).max()
+
+
+@pytest.mark.parametrize("precision", [ag.FP64, ag.MIXED_FIX48])
+def test_rve3d_effective_stiffness_vs_oracle(ctx, precision):
+    """Periodic Tet4 RVE under all six unit macro strains (two problems of
+    three load cases each): matrices and loads bit for bit against the
+    oracle, solutions and the 6x6 C_eff against the oracle's (scipy direct
+    solve), and C_eff strictly inside the Voigt / Reuss bracket."""
+    E = np.eye(6) * 1e-3
+    probs = [configs.rve3d(6, 41, 0.35, macro=E[:3]), configs.rve3d(6, 41, 0.35, macro=E[3:])]
+    B = ag.Batch(ctx, probs)
+    ref, _, xs = hm.effective_stiffness(ag.TET4, probs[0], list(E), solutions=True)
+    for k, p in enumerate(probs):
+        print("case", precision, k, flush=True)
+        rs, ci, ar, rhs = B.matrix(k)
+        A = orc.assemble_periodic(ag.TET4, p.conn, p.dofmap, p.coords, p.cmats, p.phase)
+        loads = [orc.macro_rhs(ag.TET4, p.conn, p.dofmap, p.coords, p.cmats, p.phase, p.macro[lc], A.nrows)
+                 for lc in range(3)]
+        orc.apply_dirichlet(A, np.zeros(A.n), p.bc_dofs, p.bc_vals)
+        np.testing.assert_array_equal(ar, A.array)
+        for lc in range(3):
+            f = loads[lc]
+            f[p.bc_dofs] = p.bc_vals
+            np.testing.assert_array_equal(rhs[lc], f)
+    st = B.solve(precision=precision, rtol=1e-10)
+    assert st["status"] == 0 and st["relres_max"] <= 1e-10, st
+    got = B.average_stress()             # [problem][load][stress]
+    C = np.concatenate([got[0], got[1]]) / 1e-3
+    for k in range(2):
+        for lc in range(3):
+            x = xs[3 * k + lc]
+            assert np.linalg.norm(B.solution(k)[lc] - x) <= 1e-7 * np.linalg.norm(x)
+    np.testing.assert_allclose(C, ref / 1e-3, rtol=0, atol=1e-7 * np.abs(ref / 1e-3).max())
+    voigt, reuss = hm.voigt_reuss(ag.TET4, probs[0])
+    sym = 0.5 * (C + C.T)
+    assert np.abs(C - C.T).max() <= 1e-6 * np.abs(C).max()
+    assert np.linalg.eigvalsh(voigt - sym).min() > 1e-4 * np.abs(voigt).max()
+    assert np.linalg.eigvalsh(sym - reuss).min() > 1e-4 * np.abs(voigt).max()
+
+
+@pytest.mark.parametrize("etype", [ag.P1, ag.TET4])
+def test_laminate_effective_stiffness_on_device(ctx, etype):
+    """A laminate RVE's C_eff has a closed form that P1 / Tet4 reproduce
+    exactly (piecewise-linear fluctuation): the device solve, mixed
+    precision to rel. res 1e-10, matches it to 1e-7."""
+    dim = ag.DIM[etype]
+    cm = np.stack([ag.cauchy_green(12e9, 0.3, dim=dim), ag.cauchy_green(60e9, 0.2, dim=dim)])
+    if etype == ag.P1:
+        n, pattern = 16, [0, 1, 1, 0, 0, 0, 1, 0, 1, 1, 1, 0, 0, 1, 0, 0]
+        ph = hm.layer_phases(etype, n, pattern)
+        probs = [hm.periodic_problem(etype, n, ph, cm, macro=np.eye(3) * 1e-3)]
+        t, nn = [0], [1, 2]
+    else:
+        n, pattern = 6, [1, 0, 0, 1, 1, 0]
+        ph = hm.layer_phases(etype, n, pattern)
+        probs = [hm.periodic_problem(etype, n, ph, cm, macro=np.eye(6)[:3] * 1e-3),
+                 hm.periodic_problem(etype, n, ph, cm, macro=np.eye(6)[3:] * 1e-3)]
+        t, nn = [0, 1, 5], [2, 3, 4]
+    B = ag.Batch(ctx, probs)
+    st = B.solve(precision=ag.MIXED_FIX48, rtol=1e-10)
+    assert st["status"] == 0, st
+    C = np.concatenate(list(B.average_stress())).T / 1e-3     # column k: stress of unit strain k
+    f1 = float(np.mean(pattern))
+    exact = hm.laminate_stiffness([cm[0], cm[1]], [1. - f1, f1], t, nn)
+    np.testing.assert_allclose(C, exact, rtol=0, atol=1e-7 * np.abs(exact).max())
 
 
 @pytest.mark.parametrize("precision", [ag.MIXED, ag.MIXED_SPLIT, ag.MIXED_FIX48])
@@ -497,6 +569,37 @@ def test_iteration_cap_reports_not_converged(ctx):
     assert st["status"] == 0
 
 
+def test_escalation_keeps_per_problem_counts(ctx):
+    """A tiny stall window (16 iterations without halving the FP64
+    residual, refinements every 8) stalls the larger RVE, which continues in
+    FP64; the small one converges first.  Every problem reports its own
+    iterations and refinements: the same as when solved alone, and the
+    batch totals are their sum and maximum."""
+    small = configs.rve(3, 601, 0.3)
+    large = configs.rve(48, 602, 0.4)
+    kw = dict(precision=ag.MIXED_FIX48, rtol=1e-10, stall_iters=16, ckpt_interval=8)
+    B = ag.Batch(ctx, [small, large])
+    st = B.solve(**kw)
+    info = [B.problem_info(k) for k in range(2)]
+    alone = []
+    for p in (small, large):
+        A = ag.Batch(ctx, [p])
+        sa = A.solve(**kw)
+        alone.append((sa, A.problem_info(0)))
+        A.close()
+    print("batch", st, info, "alone", alone, flush=True)
+    assert st["status"] == 0 and st["relres_max"] <= 1e-10, st
+    assert alone[0][0]["nproblems_escalated"] == 0 and alone[1][0]["nproblems_escalated"] == 1
+    assert st["nproblems_escalated"] == 1
+    for k in range(2):
+        assert info[k]["iterations"] == alone[k][1]["iterations"], k
+        assert info[k]["refinements"] == alone[k][1]["refinements"], k
+        assert info[k]["status"] == 0
+    assert st["iterations_sum"] == sum(i["iterations"] for i in info)
+    assert st["iterations_max"] == max(i["iterations"] for i in info)
+    assert st["refinements"] == sum(i["refinements"] for i in info)
+
+
 def test_nan_in_the_input_is_reported(ctx):
     coords, conn = ag.structured_mesh(ag.P1, 6)
     f = np.zeros(2 * len(coords))
@@ -530,6 +633,125 @@ def test_server_result_same_alone_and_in_mix(ctx, precision, tune):
             break
     assert len(got) == len(probs)
     np.testing.assert_array_equal(got[tickets[1]], x_alone)
+    S.close()
+
+
+def serve(S, reqs, limit_s=120.0):
+    """Submits (problem, key, damage) requests at t = 0 and steps the server
+    until every one has a result: {ticket: (result, solution)}."""
+    tickets = [S.submit(p, 0.0, key=key, damage=dmg) for p, key, dmg in reqs]
+    ndof = {t: p.ndof * p.nrhs for t, (p, _, _) in zip(tickets, reqs)}
+    got = {}
+    t_end = time.monotonic() + limit_s
+    while len(got) < len(tickets) and time.monotonic() < t_end:
+        S.step(1.0)
+        for r in S.poll():
+            x = S.solution(r["ticket"], ndof[r["ticket"]]) if r["status"] == 0 else None
+            got[r["ticket"]] = (r, x)
+    assert len(got) == len(tickets), (len(got), len(tickets))
+    return [got[t] for t in tickets]
+
+
+def oracle_load_steps(p_of_step, steps, prm):
+    """The incremental-loading loop in the reference's step order
+    (features/features.cpp:6150-6373): loads of step s, assembly with the
+    damage so far, solve (scipy direct), element strains, damage update."""
+    import scipy.sparse.linalg as spla
+    p = p_of_step(1)
+    d0 = p.damage.copy()
+    d, kap = d0.copy(), np.zeros(len(d0))
+    xs = []
+    for s in range(1, steps + 1):
+        ps = p_of_step(s)
+        A = orc.assemble(ag.TET4, ps.conn, ps.coords, ps.cmats, None, d)
+        b = np.zeros(A.n)
+        orc.apply_dirichlet(A, b, ps.bc_dofs, ps.bc_vals)
+        x = spla.spsolve(A.to_scipy().tocsc(), b)
+        eps = orc.element_strain(ag.TET4, ps.conn, ps.coords, x)
+        orc.damage_update(eps, kap, d, d0=d0, kappa0=prm[0], kappaf=prm[1], dmax=prm[2])
+        xs.append(x)
+    return xs, d, kap
+
+
+def test_served_mix_vs_oracle(ctx):
+    """The served mix: 2D P1 RVEs (three load cases), 3D Tet4 RVEs (three
+    load cases) and four load steps on a persistent damage key, submitted
+    together.  Every RVE solution and C_eff matches the oracle's (scipy
+    direct solve); each load step's solution and the damage after the four
+    steps match the oracle's sequential loop; and the 3D RVEs and the load
+    steps give bitwise the same results served alone.  Solved to rel. res
+    1e-10: displacements to 1e-6, C_eff to 1e-7, damage to 1e-6."""
+    E6 = np.eye(6)[:3] * 1e-3
+    rve2 = [configs.rve(n, 900 + n, 0.3) for n in (12, 20)]
+    rve3 = [configs.rve3d(n, 950 + n, 0.3, macro=E6) for n in (4, 6)]
+    prm = (1e-4, 1e-3, 1 - 1e-6)
+
+    def dmg_step(s):
+        return configs.config3(8, seed=301, step=s, du=1e-4)
+
+    steps = [(dmg_step(s), 7, prm) for s in range(1, 5)]
+    S = ag.Server(ctx, 8 << 30, 8, precision=ag.MIXED_FIX48, rtol=1e-10)
+    mix = serve(S, [(p, 0, None) for p in rve2 + rve3] + steps)
+    assert all(r["status"] == 0 for r, _ in mix), [r for r, _ in mix]
+    d_mix, k_mix = S.damage(7, dmg_step(1).conn.shape[0])
+    S.close()
+    # RVEs against the oracle
+    for (r, x), p in zip(mix[:4], rve2 + rve3):
+        et = p.etype
+        rows, _, xs = hm.effective_stiffness(et, p, list(p.macro), solutions=True)
+        xg = x.reshape(p.nrhs, -1)
+        for lc in range(p.nrhs):
+            assert np.linalg.norm(xg[lc] - xs[lc]) <= 1e-6 * np.linalg.norm(xs[lc]), (et, lc)
+            eps = orc.element_strain_dofmap(et, p.conn, p.dofmap, p.coords, xg[lc], p.macro[lc])
+            sg, _ = orc.average_stress(et, p.conn, p.coords, p.cmats, p.phase, eps)
+            np.testing.assert_allclose(sg, rows[lc], rtol=0, atol=1e-7 * np.abs(rows[lc]).max())
+    # load steps against the oracle's loop
+    xs, d_ref, k_ref = oracle_load_steps(dmg_step, 4, prm)
+    for (r, x), xr in zip(mix[4:], xs):
+        assert np.linalg.norm(x - xr) <= 1e-6 * np.linalg.norm(xr)
+    d_init = dmg_step(1).damage
+    assert (d_ref > d_init + 1e-3).any()          # the damage grew
+    np.testing.assert_allclose(k_mix, k_ref, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(d_mix, d_ref, rtol=0, atol=1e-6)
+    # alone = in the mix
+    S = ag.Server(ctx, 8 << 30, 8, precision=ag.MIXED_FIX48, rtol=1e-10)
+    alone = [serve(S, [(p, 0, None)])[0] for p in rve3] + serve(S, steps)
+    d_alone, _ = S.damage(7, dmg_step(1).conn.shape[0])
+    S.close()
+    for (ra, xa), (rm, xm) in zip(alone, mix[2:]):
+        np.testing.assert_array_equal(xa, xm)
+        assert ra["iterations"] == rm["iterations"]
+    np.testing.assert_array_equal(d_alone, d_mix)
+
+
+def test_server_keys_release_and_errors(ctx):
+    """A request that no longer fits next to idle resident problems
+    completes with AMIE_ERR_NOMEM instead of waiting forever; releasing the
+    key frees its memory; a load step whose Dirichlet DOFs differ from the
+    resident problem's completes with AMIE_ERR_ARG; the queue drains."""
+    p = configs.config3(6, seed=302, step=1)
+    big = configs.config3(6, seed=303, step=1)
+    B = ag.Batch(ctx, [p])
+    resident = B.nbytes            # what the key holds once built
+    B.close()
+    need = ag.Server.request_bytes(big)
+    assert need <= 2 * resident
+    # a budget that holds the resident key or the other problem, not both
+    S = ag.Server(ctx, resident + need // 2, 4, precision=ag.MIXED_FIX48, rtol=1e-8)
+    (r1, _), = serve(S, [(p, 11, None)])
+    assert r1["status"] == 0
+    (r2, _), = serve(S, [(big, 0, None)])
+    assert r2["status"] == -3, r2             # AMIE_ERR_NOMEM: the key still holds its memory
+    bad = configs.config3(6, seed=302, step=2)
+    bad = ag.Problem(ag.TET4, bad.coords, bad.conn, bad.cmats, damage=bad.damage, bc_dofs=bad.bc_dofs[:-1],
+                     bc_vals=bad.bc_vals[:-1])
+    (r3, _), = serve(S, [(bad, 11, None)])
+    assert r3["status"] == -1, r3             # AMIE_ERR_ARG: other Dirichlet DOFs
+    S.release(11)
+    with pytest.raises(ag.AmieError):
+        S.release(11)
+    (r4, _), = serve(S, [(big, 0, None)])
+    assert r4["status"] == 0, r4
     S.close()
 
 

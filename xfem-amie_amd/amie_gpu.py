@@ -36,7 +36,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", c_i32), ("precond", c_i32), ("refine", c_i32), ("rtol", c_dbl),
                 ("refine_drop", c_dbl), ("ckpt_interval", c_i64), ("maxit", c_i64), ("use_x0", c_i32), ("use_graph", c_i32),
                 ("chunk", c_i32), ("time_kernels", c_i32), ("escalate", c_i32),
-                ("max_grid", c_i32), ("ref_eps", c_dbl), ("autotune", c_i32)]
+                ("max_grid", c_i32), ("ref_eps", c_dbl), ("autotune", c_i32), ("stall_iters", c_i64)]
 
 
 class SolveStats(ctypes.Structure):
@@ -118,6 +118,11 @@ SIGNATURES = {
     "amie_gpu_server_step": (c_i32, [vp, c_dbl, vp, vp]),
     "amie_gpu_server_poll": (c_i32, [vp, vp, c_i32, vp]),
     "amie_gpu_server_get_solution": (c_i32, [vp, c_u64, vp, c_i64]),
+    "amie_gpu_server_submit_step": (c_i32, [vp, vp, c_u64, vp, c_dbl, vp]),
+    "amie_gpu_server_release": (c_i32, [vp, c_u64]),
+    "amie_gpu_server_request_bytes": (c_i32, [vp, vp]),
+    "amie_gpu_server_get_damage": (c_i32, [vp, c_u64, vp, vp, c_i64]),
+    "amie_gpu_batch_set_bc_values": (c_i32, [vp, c_i32, c_i64, vp, vp]),
 }
 
 _LIB = None
@@ -578,6 +583,12 @@ class Batch:
     def scale_bc(self, f):
         check(lib().amie_gpu_batch_scale_bc(self.h, f))
 
+    def set_bc_values(self, dofs, vals, p=0):
+        """New values of existing Dirichlet conditions (next assembly)."""
+        dofs = _c(dofs, np.int64)
+        vals = _c(vals, np.float64)
+        check(lib().amie_gpu_batch_set_bc_values(self.h, p, len(dofs), _p(dofs), _p(vals)))
+
 
 class Server:
     def __init__(self, ctx, hbm_budget, max_slots, **opts):
@@ -588,10 +599,32 @@ class Server:
                                            ctypes.byref(self.h)))
         self._keep = {}
 
-    def submit(self, problem, t_ms, key=0):
+    def submit(self, problem, t_ms, key=0, damage=None):
+        """Queues a request; key != 0 makes it a load step on that key's
+        resident problem, damage = (kappa0, kappaf, dmax) its damage law."""
         t = c_u64()
-        check(lib().amie_gpu_server_submit(self.h, ctypes.byref(problem.desc), key, t_ms, ctypes.byref(t)))
+        if damage is None:
+            check(lib().amie_gpu_server_submit(self.h, ctypes.byref(problem.desc), key, t_ms, ctypes.byref(t)))
+        else:
+            prm = DamageParams(*damage)
+            check(lib().amie_gpu_server_submit_step(self.h, ctypes.byref(problem.desc), key, ctypes.byref(prm), t_ms,
+                                                    ctypes.byref(t)))
         return t.value
+
+    @staticmethod
+    def request_bytes(problem):
+        """HBM set aside for a request at admission (estimate)."""
+        b = c_i64()
+        check(lib().amie_gpu_server_request_bytes(ctypes.byref(problem.desc), ctypes.byref(b)))
+        return b.value
+
+    def release(self, key):
+        check(lib().amie_gpu_server_release(self.h, key))
+
+    def damage(self, key, nelem):
+        d, k = np.zeros(nelem), np.zeros(nelem)
+        check(lib().amie_gpu_server_get_damage(self.h, key, _p(d), _p(k), nelem))
+        return d, k
 
     def step(self, now_ms):
         na, nq = c_i32(), c_i32()

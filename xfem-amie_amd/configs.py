@@ -77,9 +77,10 @@ def config4(nrve=512, seed=4, nmin=32, nmax=256):
     return [rve(int(n), int(s), float(f)) for n, s, f in zip(ns, seeds, fr)]
 
 
-def rve3d(n, seed, fraction=0.3):
-    """Periodic Tet4 RVE: spherical-free simple variant with phases by a
-    seeded voxel field thresholded at the requested fraction."""
+def rve3d(n, seed, fraction=0.3, macro=None):
+    """Periodic Tet4 RVE: phases by a seeded voxel field thresholded at the
+    requested fraction; macro (nrhs x 6, nrhs 1 or 3) defaults to one load
+    case exx = 1e-3."""
     coords, conn = ag.structured_mesh(ag.TET4, n)
     field = ag.damage_field_3d(seed, n, 0.08, 0.0, 1.0)
     thr = np.quantile(field, 1.0 - fraction)
@@ -88,9 +89,10 @@ def rve3d(n, seed, fraction=0.3):
     i, j, k = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
     dofmap = (((i % n) * n + (j % n)) * n + (k % n)).ravel().astype(np.int32)
     C = np.stack([ag.cauchy_green(12e9, 0.3, dim=3), ag.cauchy_green(60e9, 0.2, dim=3)])
-    macro = np.array([[1e-3, 0, 0, 0, 0, 0]], dtype=np.float64)
+    macro = np.array([[1e-3, 0, 0, 0, 0, 0]], dtype=np.float64) if macro is None else np.atleast_2d(macro)
     return ag.Problem(ag.TET4, coords, conn, C, phase=phase, dofmap=dofmap, ndofnodes=n ** 3,
-                      bc_dofs=np.array([0, 1, 2], dtype=np.int64), bc_vals=np.zeros(3), macro=macro, nrhs=1)
+                      bc_dofs=np.array([0, 1, 2], dtype=np.int64), bc_vals=np.zeros(3), macro=macro,
+                      nrhs=macro.shape[0])
 
 
 def config5(rate=20.0, duration=10.0, seed=5, n2d=(64, 512), n3d=(16, 48), ndamage=32, damage_keys=4):

@@ -839,6 +839,7 @@ int Batch::solve_begin(const amie_solve_opts &o)
     R = SolveRun();
     R.o = o;
     R.iters_done.assign(nprob, 0);
+    R.nckpt_done.assign(nprob, 0);
     R.started = true;
     hipStream_t s = stream;
     if (!o.use_x0) AMIE_HIP(hipMemsetAsync(x64.p, 0, x64.n, s));
@@ -944,7 +945,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         P.drop = R.mixed ? (o.refine_drop > 0 ? o.refine_drop : 1e-2) : 0.;
         P.maxit = o.maxit > 0 ? o.maxit : std::max<int64_t>(2 * ph[p].nrows * et.b, 1000);
         P.ckpt_interval = R.mixed ? (o.ckpt_interval > 0 ? o.ckpt_interval : 128) : ((int64_t)1 << 62);
-        P.stall_iters = (fp32vec && o.escalate) ? 8192 : 0;
+        P.stall_iters = (fp32vec && o.escalate) ? (o.stall_iters > 0 ? o.stall_iters : 8192) : 0;
         P.ladder = mode == AMIE_MIXED_SPLIT ? 1 : 0;
         P.kmode = 0;
         P.iters_k64 = 0;
@@ -1089,11 +1090,15 @@ int Batch::solve_poll(bool wait, bool *finished)
         for (int p = 0; p < nprob; p++)
             if (h_prob_pinned[p].status == ST_STALLED) stalled[p] = 1, any_stalled = true;
         if (any_stalled && R.mode != AMIE_FP64 && R.o.escalate) {
+            // only the escalated problems restart their counters in the new
+            // pass; the others keep theirs on the device and in h_prob
             std::memcpy(h_prob.data(), h_prob_pinned, nprob * sizeof(ProbDev));
-            for (int p = 0; p < nprob; p++) {
-                R.iters_done[p] += h_prob[p].iters;
-                if (stalled[p]) R.stats.nproblems_escalated++;
-            }
+            for (int p = 0; p < nprob; p++)
+                if (stalled[p]) {
+                    R.iters_done[p] += h_prob[p].iters;
+                    R.nckpt_done[p] += h_prob[p].nckpt;
+                    R.stats.nproblems_escalated++;
+                }
             return start_pass(AMIE_FP64, stalled);
         }
         AMIE_HIP(hipEventRecord(R.e1, stream));
@@ -1123,6 +1128,7 @@ int Batch::solve_end(amie_solve_stats *st)
     for (int p = 0; p < nprob; p++) {
         ProbDev &P = h_prob[p];
         P.iters += R.iters_done[p];   // iterations of every pass
+        P.nckpt += R.nckpt_done[p];
         stats.iterations_max = std::max<int64_t>(stats.iterations_max, P.iters);
         stats.iterations_sum += P.iters;
         stats.refinements += P.nckpt;
@@ -1798,7 +1804,19 @@ static int smoothed_all(Batch &B, int rhs, DevBuf &tmp, DevBuf &out)
 int amie_gpu_batch_damage_step(amie_gpu_batch *bb, const amie_damage_params *prm, int64_t *n_grown)
 {
     if (!bb || !bb->b || !prm) return AMIE_ERR_ARG;
-    Batch &B = *bb->b;
+    if (int rc = bb->b->damage_update(*prm, n_grown)) return rc;
+    return bb->b->assemble();
+}
+
+} // extern "C"
+
+// Strain of the current solution (right-hand side 0, or its non-local
+// average), kappa history and damage of every element; the matrix is not
+// re-assembled.
+int Batch::damage_update(const amie_damage_params &prm_, int64_t *n_grown)
+{
+    Batch &B = *this;
+    const amie_damage_params *prm = &prm_;
     if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
     hipStream_t st = B.stream;
     AMIE_HIP(hipMemsetAsync(B.grown.p, 0, 8, st));
@@ -1822,8 +1840,10 @@ int amie_gpu_batch_damage_step(amie_gpu_batch *bb, const amie_damage_params *prm
     AMIE_HIP(hipStreamSynchronize(st));
     if (n_grown) *n_grown = (int64_t)g;
     B.has_damage_state = true;
-    return B.assemble();
+    return AMIE_OK;
 }
+
+extern "C" {
 
 // Element geometry for the non-local search: centroid, circumradius (the
 // reference's Triangle / Tetrahedron constructors, geometry_2D.cpp:475-503,
@@ -1983,6 +2003,37 @@ int amie_gpu_batch_set_damage(amie_gpu_batch *bb, int32_t p, const double *d)
     const ProbHost &P = B.ph[p];
     AMIE_HIP(hipMemcpy(B.edamage.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
     AMIE_HIP(hipMemcpy(B.ed0.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
+    return AMIE_OK;
+}
+
+int amie_gpu_batch_set_bc_values(amie_gpu_batch *bb, int32_t p, int64_t nbc, const int64_t *dofs, const double *vals)
+{
+    if (!bb || !bb->b || check_p(bb->b, p) || nbc < 0 || (nbc && (!dofs || !vals))) {
+        set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    Batch &B = *bb->b;
+    const ProbHost &P = B.ph[p];
+    const int b = B.et.b;
+    const int64_t ndof = P.nrows * b;
+    std::vector<int64_t> at((size_t)nbc);
+    for (int64_t i = 0; i < nbc; i++) {
+        if (dofs[i] < 0 || dofs[i] >= ndof) { set_error("Dirichlet DOF %lld out of range", (long long)dofs[i]); return AMIE_ERR_ARG; }
+        at[i] = (P.row0 + P.iperm[dofs[i] / b]) * b + dofs[i] % b;
+    }
+    std::vector<uint8_t> mask(B.bcmask.n);
+    std::vector<double> h(B.bcval.n / 8);
+    AMIE_HIP(hipStreamSynchronize(B.stream));
+    AMIE_HIP(hipMemcpy(mask.data(), B.bcmask.p, B.bcmask.n, hipMemcpyDeviceToHost));
+    AMIE_HIP(hipMemcpy(h.data(), B.bcval.p, B.bcval.n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nbc; i++) {
+        if (!mask[at[i]]) {
+            set_error("DOF %lld of problem %d has no Dirichlet condition", (long long)dofs[i], p);
+            return AMIE_ERR_ARG;
+        }
+        h[at[i]] = vals[i];   // a later entry for the same DOF wins, as at creation
+    }
+    AMIE_HIP(hipMemcpy(B.bcval.p, h.data(), B.bcval.n, hipMemcpyHostToDevice));
     return AMIE_OK;
 }
 

@@ -77,6 +77,7 @@ struct SolveRun {
     int mode = 0;            // precision of the current pass
     bool mixed = false, bj = false;
     std::vector<int64_t> iters_done;   // iterations of earlier passes, per problem
+    std::vector<int64_t> nckpt_done;   // FP64 refinements of earlier passes, per problem
     int ce = 4;              // iterations between checkpoint opportunities
     int per_check = 8;       // checkpoint periods per host status check
     Kern K{};
@@ -179,6 +180,7 @@ struct Batch {
     int set_active_all();
     int set_active(const std::vector<char> &probs);
     int autotune(int mode, bool bj, int iters);
+    int damage_update(const amie_damage_params &prm, int64_t *n_grown);
     void apply_tuned(int mode, bool bj, Kern &K, SolveArgs &a) const;
     int k1_format(int mode, bool bj) const;
     int64_t vplane() const { return nrows_pad * et.b; }

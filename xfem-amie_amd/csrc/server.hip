@@ -15,8 +15,18 @@
 // with its own scalars and reductions: a problem's result does not depend
 // on which other problems share its group.
 // Requests with a persistent key keep their batch (matrix, damage state and
-// solution) resident between requests: such a request runs one damage step
-// on the resident state and a solve.
+// solution) resident between requests.  Each such request is one load step
+// in the order of the reference's FeatureTree::stepInternal
+// (features/features.cpp:6150-6373: boundary conditions, assembly, solve,
+// then the element state update): the request's Dirichlet values become the
+// resident problem's, the matrix is re-assembled with the damage so far,
+// the solve starts from the previous solution, and after it the damage law
+// (the request's parameters) updates the damage for the next step.  A
+// request that cannot be run completes with a negative status instead of
+// stopping the server: a mesh or Dirichlet DOFs that differ from the
+// resident problem's, a batch build that fails, or a footprint larger than
+// what the budget leaves next to idle resident problems (release keys with
+// amie_gpu_server_release).
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -84,6 +94,7 @@ struct Request {
     uint64_t ticket = 0;
     uint64_t key = 0;
     OwnedProblem prob;
+    amie_damage_params prm{1e-4, 1e-3, 1. - 1e-6};   // the damage law of the incremental-loading case
     double submit_ms = 0., start_ms = 0.;
 };
 
@@ -93,6 +104,10 @@ struct Group {
     hipStream_t stream = nullptr;
     bool persistent = false;
     size_t bytes = 0;
+    // persistent problems: what later requests on the key must share
+    int32_t elem_type = 0, nrhs = 0;
+    int64_t nnodes = 0, nelem = 0;
+    std::vector<int64_t> bc_dofs;
 };
 
 // A batch built on a worker thread.
@@ -159,6 +174,38 @@ struct Server {
         return AMIE_OK;
     }
 
+    // a request that is not run: its result carries the error status
+    void fail(const Request &r, int status, double now)
+    {
+        amie_request_result o{};
+        o.ticket = r.ticket;
+        o.status = status;
+        o.iterations = 0;
+        o.relres = 0.;
+        o.submit_ms = r.submit_ms;
+        o.start_ms = now;
+        o.done_ms = now;
+        done.push_back(o);
+    }
+
+    void release_group(std::unique_ptr<Group> g)
+    {
+        in_use -= g->bytes;
+        const int dev = ctx->device;
+        Group *gp = g.release();
+        releasing.push_back(std::async(std::launch::async, [gp, dev]() {
+            (void)hipSetDevice(dev);
+            delete gp;
+        }));
+    }
+
+    static bool same_problem(const Group &g, const Request &r)
+    {
+        const amie_problem_desc &d = r.prob.d;
+        return d.elem_type == g.elem_type && d.nnodes == g.nnodes && d.nelem == g.nelem && d.nrhs == g.nrhs &&
+               r.prob.bc_dofs == g.bc_dofs;
+    }
+
     int retire(std::unique_ptr<Group> &g, double now)
     {
         amie_solve_stats st{};
@@ -186,17 +233,15 @@ struct Server {
         }
         used_slots -= (int)g->reqs.size();
         if (g->persistent) {
+            // the element state update closes the load step; the matrix is
+            // re-assembled when the next step's loads are known
             const uint64_t key = g->reqs[0].key;
+            int64_t grown = 0;
+            if (int rc = g->batch->damage_update(g->reqs[0].prm, &grown)) return rc;
             g->reqs.clear();
             resident[key] = std::move(g);   // stays in HBM, still counted in in_use
         } else {
-            in_use -= g->bytes;
-            const int dev = ctx->device;
-            Group *gp = g.release();
-            releasing.push_back(std::async(std::launch::async, [gp, dev]() {
-                (void)hipSetDevice(dev);
-                delete gp;
-            }));
+            release_group(std::move(g));
         }
         return AMIE_OK;
     }
@@ -213,10 +258,12 @@ struct Server {
             std::unique_ptr<Group> g = std::move(building[i].g);
             building.erase(building.begin() + i);
             if (b.rc) {
+                // the group's requests complete with the build's error
                 in_use -= g->bytes;
                 used_slots -= (int)g->reqs.size();
                 set_error("%s", b.err.c_str());
-                return b.rc;
+                for (auto &r : g->reqs) fail(r, b.rc, now);
+                continue;
             }
             g->batch.reset(b.batch);
             b.batch->stream = pick_stream();
@@ -265,12 +312,21 @@ struct Server {
             if (it != resident.end() && used_slots < max_slots) {
                 std::unique_ptr<Group> g = std::move(it->second);
                 resident.erase(it);
-                g->reqs.push_back(std::move(r));
+                Request req = std::move(r);
                 queue.erase(queue.begin() + i);
-                amie_damage_params prm{1e-4, 1e-3, 1. - 1e-6};
+                if (!same_problem(*g, req)) {
+                    set_error("request %llu: its mesh or Dirichlet DOFs differ from the resident problem of key %llu",
+                              (unsigned long long)req.ticket, (unsigned long long)req.key);
+                    fail(req, AMIE_ERR_ARG, now);
+                    resident[req.key] = std::move(g);
+                    continue;
+                }
+                // this step's loads, then the matrix with the damage so far
                 amie_gpu_batch wrap{g->batch.get()};
-                int64_t grown = 0;
-                if ((rc = amie_gpu_batch_damage_step(&wrap, &prm, &grown))) return rc;
+                const amie_problem_desc &d = req.prob.d;
+                if ((rc = amie_gpu_batch_set_bc_values(&wrap, 0, d.nbc, d.bc_dofs, d.bc_vals))) return rc;
+                if ((rc = g->batch->assemble())) return rc;
+                g->reqs.push_back(std::move(req));
                 used_slots += 1;
                 if ((rc = start_group(std::move(g), now))) return rc;
             } else {
@@ -286,7 +342,19 @@ struct Server {
                 Request &r = queue[i];
                 const bool resident_key = r.key && key_busy(r.key);
                 if (r.prob.cls() != cls || (r.key != 0) != pers || resident_key) { i++; continue; }
-                if (in_use + bytes + (size_t)r.prob.footprint > budget) break;
+                if (in_use + bytes + (size_t)r.prob.footprint > budget) {
+                    // nothing in flight or being built will free memory: what
+                    // is in use belongs to idle resident problems, and this
+                    // request cannot run until keys are released
+                    if (pick.empty() && inflight.empty() && building.empty()) {
+                        set_error("request %llu needs %lld bytes; idle resident problems hold %zu of the %zu budget",
+                                  (unsigned long long)r.ticket, (long long)r.prob.footprint, in_use, budget);
+                        fail(r, AMIE_ERR_NOMEM, now);
+                        queue.erase(queue.begin() + i);
+                        continue;
+                    }
+                    break;
+                }
                 bytes += (size_t)r.prob.footprint;
                 pick.push_back(std::move(r));
                 queue.erase(queue.begin() + i);
@@ -299,6 +367,14 @@ struct Server {
                 in_use += bytes;
                 used_slots += (int)pick.size();
                 g->reqs = std::move(pick);
+                if (pers) {
+                    const Request &r = g->reqs[0];
+                    g->elem_type = r.prob.d.elem_type;
+                    g->nrhs = r.prob.d.nrhs;
+                    g->nnodes = r.prob.d.nnodes;
+                    g->nelem = r.prob.d.nelem;
+                    g->bc_dofs = r.prob.bc_dofs;
+                }
                 // the descriptions point into the requests' own arrays, which
                 // stay in place while the group waits for its batch
                 std::vector<amie_problem_desc> descs;
@@ -382,21 +458,83 @@ void amie_gpu_server_destroy(amie_gpu_server *s)
     ctx_release(ctx);
 }
 
-int amie_gpu_server_submit(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t key, double submit_ms,
-                           uint64_t *ticket)
+static int submit(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t key, const amie_damage_params *prm,
+                  double submit_ms, uint64_t *ticket)
 {
     if (!s || !desc) { set_error("bad arguments"); return AMIE_ERR_ARG; }
     const EType et = etype_info(desc->elem_type);
     if (!et.et || !desc->coords || !desc->conn || !desc->cmat) { set_error("bad problem"); return AMIE_ERR_ARG; }
+    if (prm && !(prm->kappa0 > 0. && prm->kappaf > 0. && prm->dmax >= 0. && prm->dmax < 1.)) {
+        set_error("damage parameters need kappa0 > 0, kappaf > 0, 0 <= dmax < 1");
+        return AMIE_ERR_ARG;
+    }
     Request r;
-    r.ticket = s->s->next_ticket++;
     r.key = key;
     r.submit_ms = submit_ms;
+    if (prm) r.prm = *prm;
     r.prob.copy(*desc);
     if ((size_t)r.prob.footprint > s->s->budget) { set_error("request larger than the HBM budget"); return AMIE_ERR_NOMEM; }
+    r.ticket = s->s->next_ticket++;
     if (ticket) *ticket = r.ticket;
     s->s->queue.push_back(std::move(r));
     return AMIE_OK;
+}
+
+int amie_gpu_server_submit(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t key, double submit_ms,
+                           uint64_t *ticket)
+{
+    return submit(s, desc, key, nullptr, submit_ms, ticket);
+}
+
+int amie_gpu_server_submit_step(amie_gpu_server *s, const amie_problem_desc *desc, uint64_t key,
+                                const amie_damage_params *prm, double submit_ms, uint64_t *ticket)
+{
+    if (key == 0 || !prm) { set_error("a load step needs a persistent key and damage parameters"); return AMIE_ERR_ARG; }
+    return submit(s, desc, key, prm, submit_ms, ticket);
+}
+
+int amie_gpu_server_request_bytes(const amie_problem_desc *desc, int64_t *bytes)
+{
+    if (!desc || !bytes) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    const EType et = etype_info(desc->elem_type);
+    if (!et.et || !desc->coords || !desc->conn || !desc->cmat) { set_error("bad problem"); return AMIE_ERR_ARG; }
+    OwnedProblem o;
+    o.copy(*desc);
+    *bytes = o.footprint;
+    return AMIE_OK;
+}
+
+int amie_gpu_server_release(amie_gpu_server *s, uint64_t key)
+{
+    if (!s || key == 0) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    Server &S = *s->s;
+    for (auto &r : S.queue)
+        if (r.key == key) { set_error("key %llu has queued requests", (unsigned long long)key); return AMIE_ERR_BUSY; }
+    auto it = S.resident.find(key);
+    if (it == S.resident.end()) {
+        if (S.key_busy(key)) { set_error("key %llu is in flight", (unsigned long long)key); return AMIE_ERR_BUSY; }
+        set_error("no resident problem for key %llu", (unsigned long long)key);
+        return AMIE_ERR_ARG;
+    }
+    std::unique_ptr<Group> g = std::move(it->second);
+    S.resident.erase(it);
+    S.release_group(std::move(g));
+    return AMIE_OK;
+}
+
+int amie_gpu_server_get_damage(amie_gpu_server *s, uint64_t key, double *d, double *kappa, int64_t nelem)
+{
+    if (!s || key == 0) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    Server &S = *s->s;
+    auto it = S.resident.find(key);
+    if (it == S.resident.end()) {
+        if (S.key_busy(key)) { set_error("key %llu is in flight", (unsigned long long)key); return AMIE_ERR_BUSY; }
+        set_error("no resident problem for key %llu", (unsigned long long)key);
+        return AMIE_ERR_ARG;
+    }
+    if (nelem != it->second->nelem) { set_error("the resident problem has %lld elements", (long long)it->second->nelem); return AMIE_ERR_ARG; }
+    amie_gpu_batch wrap{it->second->batch.get()};
+    return amie_gpu_batch_get_damage(&wrap, 0, d, kappa);
 }
 
 int amie_gpu_server_step(amie_gpu_server *s, double now_ms, int32_t *n_active, int32_t *n_queued)

@@ -55,6 +55,14 @@ __global__ void k_fill_owner(int32_t *owner, int64_t n)
     if (i < n) owner[i] = -1;
 }
 
+// First index of a voxel range that starts at s (the reference truncates s
+// to an int; indices below 0 are skipped there): clamped to [0, n] before
+// the conversion, so a tetrahedron far outside the grid loops over nothing.
+__device__ inline int first_index(double s, int n)
+{
+    return !(s > 0.) ? 0 : (s >= (double)n ? n : (int)s);
+}
+
 __global__ void k_voxel_owner(const double *xyz, const int32_t *conn, int64_t ntet, Grid g, int32_t *owner)
 {
 #pragma clang fp contract(off)
@@ -71,12 +79,15 @@ __global__ void k_voxel_owner(const double *xyz, const int32_t *conn, int64_t nt
         }
     }
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    for (int i = (int)(nx * (lo[0] - g.bl[0]) / (g.tr[0] - g.bl[0])); i < nx * (hi[0] - g.bl[0]) / (g.tr[0] - g.bl[0]); i++) {
-        if (i < 0 || i >= nx) continue;
-        for (int j = (int)(ny * (lo[1] - g.bl[1]) / (g.tr[1] - g.bl[1])); j < ny * (hi[1] - g.bl[1]) / (g.tr[1] - g.bl[1]); j++) {
-            if (j < 0 || j >= ny) continue;
-            for (int k = (int)(nz * (lo[2] - g.bl[2]) / (g.tr[2] - g.bl[2])); k < nz * (hi[2] - g.bl[2]) / (g.tr[2] - g.bl[2]); k++) {
-                if (k < 0 || k >= nz) continue;
+    const double ei = nx * (hi[0] - g.bl[0]) / (g.tr[0] - g.bl[0]);
+    const double ej = ny * (hi[1] - g.bl[1]) / (g.tr[1] - g.bl[1]);
+    const double ek = nz * (hi[2] - g.bl[2]) / (g.tr[2] - g.bl[2]);
+    const int i0 = first_index(nx * (lo[0] - g.bl[0]) / (g.tr[0] - g.bl[0]), nx);
+    const int j0 = first_index(ny * (lo[1] - g.bl[1]) / (g.tr[1] - g.bl[1]), ny);
+    const int k0 = first_index(nz * (lo[2] - g.bl[2]) / (g.tr[2] - g.bl[2]), nz);
+    for (int i = i0; i < nx && i < ei; i++) {
+        for (int j = j0; j < ny && j < ej; j++) {
+            for (int k = k0; k < nz && k < ek; k++) {
                 double p[3] = {g.bl[0], g.bl[1], g.bl[2]};
                 p[0] += (g.tr[0] - g.bl[0]) * ((double)i) / (double)(nx - 1);
                 p[1] += (g.tr[1] - g.bl[1]) * ((double)j) / (double)(ny - 1);
