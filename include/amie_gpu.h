@@ -289,6 +289,23 @@ int amie_write_triangles_bin(const char *path, int64_t ntri, int32_t nper, const
 int amie_gpu_voxelize(amie_gpu_ctx *ctx, int64_t nnodes, const double *xyz, int64_t ntet, const int32_t *conn,
                       int32_t ncomp, const double *field, const double *box, int32_t nx, int32_t ny, int32_t nz,
                       double *values, double *device_ms);
+/* amie_gpu_voxelize on device arrays, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  xyz (nnodes x 3), conn (ntet x 4), field (ntet x ncomp) and
+ * values (ncomp x nx*ny*nz) are device pointers the caller owns; box is a
+ * host array.  The voxel ownership goes to a static device array of
+ * AMIE_VOXEL_SCRATCH_MAX int32 entries (16 MiB) that each GPU the library
+ * runs on holds until the process ends; larger grids are refused with
+ * AMIE_ERR_ARG.  Work is queued on `stream` (a hipStream_t; NULL: the
+ * context's stream) and the call returns without waiting: values are ready
+ * when the stream reaches that point.  Every call on one GPU shares its
+ * array (each call fills it before use), so calls that overlap on one GPU
+ * race.  Tetrahedra naming a node outside [0, nnodes) own no voxel. */
+#define AMIE_VOXEL_SCRATCH_MAX (1 << 22)
+int amie_gpu_voxelize_noalloc(amie_gpu_ctx *ctx, void *stream, int64_t nnodes, const double *xyz, int64_t ntet,
+                              const int32_t *conn, int32_t ncomp, const double *field, const double *box, int32_t nx,
+                              int32_t ny, int32_t nz, double *values);
 /* VoxelWriter files (voxel_writer.cpp:52-86, 433-442, 548-571): component c
  * of values to <base>_<nx>_<c>.vox, header "VOXELS", 1, nx, ny, nz, then one
  * byte per voxel: the value mapped to [1, 255] over the symmetric range set

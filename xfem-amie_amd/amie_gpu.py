@@ -82,6 +82,7 @@ SIGNATURES = {
     "amie_multi_triangles_append": (c_i32, [ctypes.c_char_p, ctypes.c_char_p, c_i32, c_i64, c_i32, vp, c_i64, vp, vp,
                                             c_i32, vp, vp]),
     "amie_gpu_voxelize": (c_i32, [vp, c_i64, vp, c_i64, vp, c_i32, vp, vp, c_i32, c_i32, c_i32, vp, vp]),
+    "amie_gpu_voxelize_noalloc": (c_i32, [vp, vp, c_i64, vp, c_i64, vp, c_i32, vp, vp, c_i32, c_i32, c_i32, vp]),
     "amie_write_voxels": (c_i32, [ctypes.c_char_p, c_i32, c_i32, c_i32, c_i32, vp]),
     "amie_place_inclusions_2d": (c_i32, [c_dbl, c_dbl, c_dbl, c_dbl, c_i64, vp, c_dbl, c_i64, c_dbl, c_u64,
                                          vp, vp, vp]),

@@ -1275,6 +1275,7 @@ int amie_gpu_create(amie_gpu_ctx **ctx, int device)
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
     c->total_mem = prop.totalGlobalMem;
+    if (int rc = voxel_scratch_address(device, &c->voxel_scratch)) return rc;
     AMIE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *ctx = c.release();
     return AMIE_OK;

@@ -18,6 +18,7 @@ struct amie_gpu_ctx {
     size_t total_mem = 0;
     std::atomic<int> children{0};   // live batches and servers (created and destroyed on any host thread)
     bool closing = false;    // amie_gpu_destroy called while children live
+    int32_t *voxel_scratch = nullptr;   // this device's copy of the voxel ownership array (voxels.hip)
 };
 
 namespace amie {
@@ -201,6 +202,9 @@ int download_vec(Batch &B, int p, const DevBuf &buf, double *out);
 int upload_vec(Batch &B, int p, DevBuf &buf, const double *in);
 
 int batch_create(amie_gpu_ctx *ctx, int nprob, const amie_problem_desc *probs, Batch **out);
+// Device address of the voxel ownership array on the current device (set up
+// once per device; the device must be current).
+int voxel_scratch_address(int device, int32_t **ptr);
 void ctx_retain(amie_gpu_ctx *ctx);
 void ctx_release(amie_gpu_ctx *ctx);
 int batch_from_bcsr(amie_gpu_ctx *ctx, int stride, int64_t nrows, const uint32_t *row_size,

@@ -5,9 +5,41 @@
 // thread per tetrahedron visits the same index range and raises the voxel's
 // owner to its index with an atomic maximum (order-free, the same winner);
 // a second pass gathers the owner's values.
+#include <mutex>
+
 #include "batch.hpp"
 
 namespace amie {
+
+// Ownership array of amie_gpu_voxelize_noalloc: one int32 per voxel, a
+// static device array of AMIE_VOXEL_SCRATCH_MAX entries, so that call
+// allocates nothing.  Each GPU holds its own copy at its own address until
+// the process ends; host code uses the address hipGetSymbolAddress gives on
+// that device (voxel_scratch_address), never the array's name.
+static __device__ __align__(256) int32_t scratch[AMIE_VOXEL_SCRATCH_MAX];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+int32_t *scratch_ptr[MAX_DEVICES] = {};
+} // namespace
+
+int voxel_scratch_address(int device, int32_t **ptr)
+{
+    if (device < 0 || device >= MAX_DEVICES) { set_error("device %d: at most %d devices", device, MAX_DEVICES); return AMIE_ERR_ARG; }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress of the voxel scratch array: %s", hipGetErrorString(e));
+            return AMIE_ERR_HIP;
+        }
+        scratch_ptr[device] = static_cast<int32_t *>(p);
+    }
+    *ptr = scratch_ptr[device];
+    return AMIE_OK;
+}
 
 namespace {
 
@@ -63,12 +95,18 @@ __device__ inline int first_index(double s, int n)
     return !(s > 0.) ? 0 : (s >= (double)n ? n : (int)s);
 }
 
-__global__ void k_voxel_owner(const double *xyz, const int32_t *conn, int64_t ntet, Grid g, int32_t *owner)
+// Tetrahedra that name a node outside [0, nnodes) own nothing (the host
+// entry point refuses them before the launch; the device-array entry point
+// cannot read the connectivity on the host).
+__global__ void k_voxel_owner(const double *xyz, const int32_t *conn, int64_t ntet, int64_t nnodes, Grid g,
+                              int32_t *owner)
 {
 #pragma clang fp contract(off)
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntet) return;
     double P[4][3], lo[3], hi[3];
+    for (int a = 0; a < 4; a++)
+        if (conn[t * 4 + a] < 0 || conn[t * 4 + a] >= nnodes) return;
     for (int a = 0; a < 4; a++)
         for (int d = 0; d < 3; d++) P[a][d] = xyz[(int64_t)conn[t * 4 + a] * 3 + d];
     for (int d = 0; d < 3; d++) {
@@ -114,6 +152,19 @@ __global__ void k_voxel_values(const int32_t *owner, int64_t nvox, int ncomp, co
 
 unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// The grid of voxel_writer.cpp:88-104: corners c -/+ 0.5 s (n-1)/n.
+int make_grid(const double *box, int nx, int ny, int nz, Grid &g)
+{
+    g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
+    for (int d = 0; d < 3; d++) {
+        if (!(box[3 + d] > 0)) { set_error("box size must be positive"); return AMIE_ERR_ARG; }
+        const double v = box[3 + d] * (g.n[d] - 1) / (g.n[d]) * 0.5;
+        g.bl[d] = box[d] - v;
+        g.tr[d] = box[d] + v;
+    }
+    return AMIE_OK;
+}
+
 } // namespace
 
 } // namespace amie
@@ -135,13 +186,7 @@ int amie_gpu_voxelize(amie_gpu_ctx *ctx, int64_t nnodes, const double *xyz, int6
     for (int64_t q = 0; q < ntet * 4; q++)
         if (conn[q] < 0 || conn[q] >= nnodes) { set_error("connectivity names a node outside [0, nnodes)"); return AMIE_ERR_ARG; }
     Grid g;
-    g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
-    for (int d = 0; d < 3; d++) {
-        if (!(box[3 + d] > 0)) { set_error("box size must be positive"); return AMIE_ERR_ARG; }
-        const double v = box[3 + d] * (g.n[d] - 1) / (g.n[d]) * 0.5;
-        g.bl[d] = box[d] - v;
-        g.tr[d] = box[d] + v;
-    }
+    if (int rc = make_grid(box, nx, ny, nz, g)) return rc;
     AMIE_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int64_t nvox = (int64_t)nx * ny * nz;
@@ -167,8 +212,8 @@ int amie_gpu_voxelize(amie_gpu_ctx *ctx, int64_t nnodes, const double *xyz, int6
         AMIE_HIP(hipEventRecord(e0, st));
     }
     hipLaunchKernelGGL(k_fill_owner, dim3(blocks(nvox)), dim3(256), 0, st, owner.as<int32_t>(), nvox);
-    hipLaunchKernelGGL(k_voxel_owner, dim3(blocks(ntet)), dim3(256), 0, st, dxyz.as<double>(), dconn.as<int32_t>(), ntet, g,
-                       owner.as<int32_t>());
+    hipLaunchKernelGGL(k_voxel_owner, dim3(blocks(ntet)), dim3(256), 0, st, dxyz.as<double>(), dconn.as<int32_t>(), ntet,
+                       nnodes, g, owner.as<int32_t>());
     hipLaunchKernelGGL(k_voxel_values, dim3(blocks(nvox)), dim3(256), 0, st, owner.as<int32_t>(), nvox, (int)ncomp,
                        dfield.as<double>(), dval.as<double>());
     AMIE_HIP(hipGetLastError());
@@ -180,5 +225,34 @@ int amie_gpu_voxelize(amie_gpu_ctx *ctx, int64_t nnodes, const double *xyz, int6
         AMIE_HIP(hipEventElapsedTime(&ms, e0, e1));
         *device_ms = ms;
     }
+    return AMIE_OK;
+}
+
+int amie_gpu_voxelize_noalloc(amie_gpu_ctx *ctx, void *stream, int64_t nnodes, const double *xyz, int64_t ntet,
+                              const int32_t *conn, int32_t ncomp, const double *field, const double *box, int32_t nx,
+                              int32_t ny, int32_t nz, double *values)
+{
+    if (!ctx || !xyz || !conn || !field || !box || !values || nnodes <= 0 || ntet <= 0 || ntet >= INT32_MAX ||
+        ncomp <= 0 || nx <= 1 || ny <= 1 || nz <= 1) {
+        set_error("bad arguments");
+        return AMIE_ERR_ARG;
+    }
+    if (nx != ny || ny != nz) { set_error("the voxel grid must be cubic (nx = ny = nz)"); return AMIE_ERR_ARG; }
+    const int64_t nvox = (int64_t)nx * ny * nz;
+    if (nvox > AMIE_VOXEL_SCRATCH_MAX) {
+        set_error("%lld voxels: amie_gpu_voxelize_noalloc takes at most %lld (AMIE_VOXEL_SCRATCH_MAX)", (long long)nvox,
+                  (long long)AMIE_VOXEL_SCRATCH_MAX);
+        return AMIE_ERR_ARG;
+    }
+    if (!ctx->voxel_scratch) { set_error("the context has no voxel scratch array"); return AMIE_ERR_STATE; }
+    Grid g;
+    if (int rc = make_grid(box, nx, ny, nz, g)) return rc;
+    AMIE_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    int32_t *owner = ctx->voxel_scratch;
+    hipLaunchKernelGGL(k_fill_owner, dim3(blocks(nvox)), dim3(256), 0, st, owner, nvox);
+    hipLaunchKernelGGL(k_voxel_owner, dim3(blocks(ntet)), dim3(256), 0, st, xyz, conn, ntet, nnodes, g, owner);
+    hipLaunchKernelGGL(k_voxel_values, dim3(blocks(nvox)), dim3(256), 0, st, owner, nvox, (int)ncomp, field, values);
+    AMIE_HIP(hipGetLastError());
     return AMIE_OK;
 }
