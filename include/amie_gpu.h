@@ -322,6 +322,13 @@ int amie_gpu_batch_create(amie_gpu_ctx *ctx, int32_t nprob, const amie_problem_d
 int amie_gpu_batch_from_bcsr(amie_gpu_ctx *ctx, int32_t stride, int64_t nrows, const uint32_t *row_size,
                              const uint32_t *column_index, const double *array, const double *rhs,
                              amie_gpu_batch **out);
+/* New matrix values and right-hand side for a batch made by
+ * amie_gpu_batch_from_bcsr, in the same pattern (row_size and column_index
+ * as then): the values are re-uploaded and re-scaled, the layout and the
+ * device buffers are kept.  What a solver reused across
+ * Assembly::mgsolve calls needs when only values change (damage steps,
+ * new loads: solvers/assembly.cpp:1955-1969). */
+int amie_gpu_batch_update_bcsr(amie_gpu_batch *b, const double *array, const double *rhs);
 void amie_gpu_batch_destroy(amie_gpu_batch *b);
 /* HBM bytes held by the batch. */
 size_t amie_gpu_batch_bytes(const amie_gpu_batch *b);

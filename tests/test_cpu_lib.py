@@ -243,3 +243,15 @@ def test_inclusion_placement_properties():
     assert (x - r >= -1e-12).all() and (x + r <= 2 + 1e-12).all() and (y - r >= -1e-12).all() and (y + r <= 1 + 1e-12).all()
     d = np.hypot(x[:, None] - x[None], y[:, None] - y[None]) + np.eye(len(x)) * 10
     assert (d >= np.maximum(r[:, None], r[None]) + np.minimum(r[:, None], r[None]) + 0.01 - 1e-12).all()
+
+
+def test_integration_shows_the_compiled_dropin_file_verbatim():
+    """INTEGRATION.md's reference-side LinearSolver is the file that
+    oracle/ref_dropin.cpp compiles against the reference headers."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "INTEGRATION.md")).read()
+    hdr = open(os.path.join(root, "xfem-amie_amd", "host", "gpuconjugategradient.h")).read()
+    a = text.index("This is that file, verbatim")
+    b = text.index("```cpp\n", a) + len("```cpp\n")
+    c = text.index("```\n", b)
+    assert text[b:c] == hdr

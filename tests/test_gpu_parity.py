@@ -724,6 +724,47 @@ def test_served_mix_vs_oracle(ctx):
     np.testing.assert_array_equal(d_alone, d_mix)
 
 
+@pytest.mark.parametrize("du,precision,precond", [(2e-5, ag.MIXED_FIX48, ag.BLOCK_JACOBI),
+                                                   (1e-4, ag.MIXED_FIX48, ag.BLOCK_JACOBI),
+                                                   (1e-4, ag.FP64, ag.JACOBI)])
+def test_incremental_damage_trajectory_vs_oracle(ctx, du, precision, precond):
+    """The incremental-loading loop of the Tet4 damage case as the benchmark
+    runs it (load scaled step by step, re-assembly, a warm-started solve,
+    then the damage step), five steps on 8^3, against the oracle's loop in
+    the reference's step order (scipy direct solves): every step's
+    displacements to 1e-6 (solved to rel. res 1e-10), kappa to 1e-6
+    relative and the damage to 1e-6 after every step.  du = 2e-5 is the
+    configured increment; 1e-4 makes the damage grow from the first step."""
+    print("case", du, precision, precond, flush=True)
+    prm = (1e-4, 1e-3, 1 - 1e-6)
+
+    def at(s):
+        return configs.config3(8, seed=3, step=s, du=du)
+
+    xs, _, _ = oracle_load_steps(at, 5, prm)
+    B = ag.Batch(ctx, [at(1)])
+    d = at(1).damage.copy()
+    kap = np.zeros(len(d))
+    for s in range(1, 6):
+        if s > 1:
+            B.scale_bc(s / (s - 1))
+        B.assemble()
+        st = B.solve(precision=precision, precond=precond, rtol=1e-10, use_x0=1 if s > 1 else 0)
+        assert st["status"] == 0, (s, st)
+        x = B.solution()
+        assert np.linalg.norm(x - xs[s - 1]) <= 1e-6 * np.linalg.norm(xs[s - 1]), s
+        B.damage_step(*prm)
+        # the oracle's update from the device's own solution of this step
+        eps = orc.element_strain(ag.TET4, at(s).conn, at(s).coords, x)
+        orc.damage_update(eps, kap, d, d0=at(1).damage, kappa0=prm[0], kappaf=prm[1], dmax=prm[2])
+        dg, kg = B.damage()
+        np.testing.assert_allclose(kg, kap, rtol=1e-6, atol=0)
+        np.testing.assert_allclose(dg, d, rtol=0, atol=1e-6)
+    if du > 5e-5:
+        assert (d > at(1).damage + 1e-3).any()
+    B.close()
+
+
 def test_server_keys_release_and_errors(ctx):
     """A request that no longer fits next to idle resident problems
     completes with AMIE_ERR_NOMEM instead of waiting forever; releasing the
@@ -883,6 +924,26 @@ def test_cpp_linear_solver_dropin(tmp_path, tag, precision):
     assert np.sqrt(np.dot(res, orc.inverse_diagonal(A) * res)) <= 1e-10
     xref = g(tag + "_cg_x")
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-6
+
+
+REF_DROPIN = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "ref_dropin")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_DROPIN), reason="oracle/_ref/ref_dropin is built only where the "
+                                                           "reference sources are (oracle/Makefile.ref)")
+@pytest.mark.parametrize("precision", [ag.MIXED_FIX48, ag.FP64])
+def test_reference_assembly_mgsolve_dropin(precision):
+    """The reference's own Assembly drives the drop-in: the reference-side
+    Amie::GpuConjugateGradient : LinearSolver (xfem-amie_amd/host/
+    gpuconjugategradient.h, compiled against the reference headers) through
+    Assembly::mgprepare + mgsolve over three load steps of a P1 and a Tet4
+    case; every step's displacements match Assembly::cgsolve's to 1e-6, one
+    device system serves all three steps (oracle/ref_dropin.cpp)."""
+    print("case", precision, flush=True)
+    r = subprocess.run([REF_DROPIN, str(precision)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr[-2000:]
+    assert "ALL OK" in r.stdout
 
 
 def test_triangle_writer_from_device_solution(ctx, tmp_path):

@@ -91,6 +91,7 @@ SIGNATURES = {
     "amie_gpu_batch_create": (c_i32, [vp, c_i32, vp, vp]),
     "amie_gpu_batch_from_bcsr": (c_i32, [vp, c_i32, c_i64, vp, vp, vp, vp, vp]),
     "amie_gpu_batch_destroy": (None, [vp]),
+    "amie_gpu_batch_update_bcsr": (c_i32, [vp, vp, vp]),
     "amie_gpu_batch_bytes": (ctypes.c_size_t, [vp]),
     "amie_gpu_batch_assemble": (c_i32, [vp]),
     "amie_gpu_batch_set_assembly": (c_i32, [vp, c_i32]),
@@ -450,6 +451,12 @@ class Batch:
             self.etype = P1 if stride == 2 else TET4
             self.nrhs = 1
             self.b = stride
+
+    def update_bcsr(self, array, rhs):
+        """New values and right-hand side, same pattern (drop-in batches)."""
+        ar = _c(array, np.float64)
+        b = _c(rhs, np.float64)
+        check(lib().amie_gpu_batch_update_bcsr(self.h, _p(ar), _p(b)))
 
     def close(self):
         if self.h:

@@ -69,6 +69,11 @@ static size_t slot_bytes(bool fp64, int b)
 
 Batch::~Batch()
 {
+    // work still queued on the stream (a solve left by an error, a status
+    // copy into the pinned buffers) must finish before the graphs and the
+    // pinned buffers it uses go away; the device buffers' hipFree comes
+    // after this body
+    if (stream) (void)hipStreamSynchronize(stream);
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_prob_pinned) (void)hipHostFree(h_prob_pinned);
     if (h_nactive_pinned) (void)hipHostFree(h_nactive_pinned);
@@ -609,8 +614,20 @@ int batch_from_bcsr(amie_gpu_ctx *ctx, int stride, int64_t nrows, const uint32_t
     int rc = build_sell(B, tmp);
     if (rc) return rc;
     if ((rc = alloc_vectors(B))) return rc;
-    // reference padded column-major blocks -> FP64 planes
-    const int b = stride, cl = stride + stride % 2;
+    if ((rc = upload_bcsr_values(B, array, rhs))) return rc;
+    *out = Bp.release();
+    return AMIE_OK;
+}
+
+// Values of a block-CSR system (the pattern the batch was made from) and its
+// right-hand side: reference padded column-major blocks -> FP64 planes, the
+// right-hand side to the internal row order, then scaling and the casts of
+// the mixed-precision forms.
+int upload_bcsr_values(Batch &B, const double *array, const double *rhs)
+{
+    const ProbHost &P = B.ph[0];
+    const int b = B.et.b, cl = b + b % 2;
+    const int64_t nnzb = (int64_t)P.column_index.size(), nrows = P.nrows;
     std::vector<double> planes(B.nslots * b * b, 0.);
     const int np = (b * b) / 2;
     for (int64_t q = 0; q < nnzb; q++) {
@@ -624,14 +641,13 @@ int batch_from_bcsr(amie_gpu_ctx *ctx, int stride, int64_t nrows, const uint32_t
         }
         if ((b * b) % 2) planes[np * B.nslots * 2 + s] = a[b * b - 1];
     }
+    AMIE_HIP(hipStreamSynchronize(B.stream));
     AMIE_HIP(hipMemcpy(B.v64.p, planes.data(), B.nslots * slot_bytes(true, b), hipMemcpyHostToDevice));
     std::vector<double> h_b((size_t)B.nrows_pad * b, 0.);
     for (int64_t dof = 0; dof < nrows * b; dof++) h_b[(size_t)P.iperm[dof / b] * b + dof % b] = rhs[dof];
     AMIE_HIP(hipMemcpy(B.b64.p, h_b.data(), h_b.size() * 8, hipMemcpyHostToDevice));
     AMIE_HIP(hipMemcpy(B.fext.p, h_b.data(), h_b.size() * 8, hipMemcpyHostToDevice));
-    if ((rc = B.assemble())) return rc;
-    *out = Bp.release();
-    return AMIE_OK;
+    return B.assemble();
 }
 
 // ---------------------------------------------------------- assembly
@@ -2005,6 +2021,17 @@ int amie_gpu_batch_set_damage(amie_gpu_batch *bb, int32_t p, const double *d)
     AMIE_HIP(hipMemcpy(B.edamage.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
     AMIE_HIP(hipMemcpy(B.ed0.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
     return AMIE_OK;
+}
+
+int amie_gpu_batch_update_bcsr(amie_gpu_batch *bb, const double *array, const double *rhs)
+{
+    if (!bb || !bb->b || !array || !rhs) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    Batch &B = *bb->b;
+    if (B.has_elements || B.nprob != 1) {
+        set_error("amie_gpu_batch_update_bcsr: the batch was not made by amie_gpu_batch_from_bcsr");
+        return AMIE_ERR_STATE;
+    }
+    return upload_bcsr_values(B, array, rhs);
 }
 
 int amie_gpu_batch_set_bc_values(amie_gpu_batch *bb, int32_t p, int64_t nbc, const int64_t *dofs, const double *vals)

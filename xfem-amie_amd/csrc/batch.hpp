@@ -209,6 +209,7 @@ void ctx_retain(amie_gpu_ctx *ctx);
 void ctx_release(amie_gpu_ctx *ctx);
 int batch_from_bcsr(amie_gpu_ctx *ctx, int stride, int64_t nrows, const uint32_t *row_size,
                     const uint32_t *column_index, const double *array, const double *rhs, Batch **out);
+int upload_bcsr_values(Batch &B, const double *array, const double *rhs);
 
 } // namespace amie
 
