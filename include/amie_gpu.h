@@ -128,6 +128,13 @@ typedef struct {
     int64_t stall_iters;     /* FP32-vector modes with escalate: a right-hand side whose FP64
                                 residual has not halved in this many iterations stalls its problem,
                                 which then continues in AMIE_FP64 (<= 0: 8192) */
+    int32_t persistent;      /* > 0: AMIE_FP64 and AMIE_MIXED_FIX48 run each solve pass as one
+                                persistent launch of at most this many workgroups (all resident),
+                                the iteration phases separated by device-wide barriers instead of
+                                kernel boundaries; the same results bit for bit.  For small
+                                batches and served groups: a barrier's cost grows with the
+                                workgroups (about 2-3 us at 64).  0 (default): one launch per
+                                phase. */
 } amie_solve_opts;
 
 typedef struct {

@@ -569,6 +569,53 @@ def test_iteration_cap_reports_not_converged(ctx):
     assert st["status"] == 0
 
 
+@pytest.mark.parametrize("case", ["rve_fix48", "rve_fix48_dict", "rve_fp64", "tet_fix48_bj", "tet_fp64_bj"])
+@pytest.mark.parametrize("grid", [16, 256])
+def test_persistent_solve_matches_launches(ctx, case, grid):
+    """One persistent launch per solve pass (device-wide barriers between the
+    phases) gives the launch path's results bit for bit: solutions,
+    iterations and refinements, for 2D RVE batches (value planes and the
+    value-indexed matrix) and a damaged Tet4 block with block-Jacobi."""
+    print("case", case, grid, flush=True)
+    if case.startswith("rve"):
+        probs = [configs.rve(n, 1100 + n, 0.3) for n in (8, 20, 33)]
+        precond = ag.JACOBI
+    else:
+        probs = [configs.config3(7, seed=31), configs.config3(5, seed=32)]
+        precond = ag.BLOCK_JACOBI
+    prec = ag.FP64 if "fp64" in case else ag.MIXED_FIX48
+    B = ag.Batch(ctx, probs)
+    if case == "rve_fix48_dict":
+        B.set_spmv_kernel(prec, precond, 8, 8, 1)
+    ref = B.solve(precision=prec, precond=precond, rtol=1e-10, use_graph=0)
+    xr = [B.solution(k).copy() for k in range(len(probs))]
+    ir = [B.problem_info(k) for k in range(len(probs))]
+    got = B.solve(precision=prec, precond=precond, rtol=1e-10, persistent=grid)
+    assert ref["status"] == 0 and got["status"] == 0, (ref, got)
+    for k in range(len(probs)):
+        np.testing.assert_array_equal(B.solution(k), xr[k])
+        assert B.problem_info(k) == ir[k]
+    assert got["iterations_max"] == ref["iterations_max"] and got["refinements"] == ref["refinements"]
+    B.close()
+
+
+def test_persistent_solve_escalates_like_launches(ctx):
+    """The escalation to FP64 of a stalled problem runs as a second
+    persistent pass, with the launch path's results and counts."""
+    probs = [configs.rve(3, 601, 0.3), configs.rve(48, 602, 0.4)]
+    kw = dict(precision=ag.MIXED_FIX48, rtol=1e-10, stall_iters=16, ckpt_interval=8)
+    B = ag.Batch(ctx, probs)
+    ref = B.solve(use_graph=0, **kw)
+    xr = [B.solution(k).copy() for k in range(2)]
+    got = B.solve(persistent=64, **kw)
+    assert ref["nproblems_escalated"] == 1 and got["nproblems_escalated"] == 1
+    for k in range(2):
+        np.testing.assert_array_equal(B.solution(k), xr[k])
+    for key in ("iterations_max", "iterations_sum", "refinements", "status"):
+        assert got[key] == ref[key], key
+    B.close()
+
+
 def test_escalation_keeps_per_problem_counts(ctx):
     """A tiny stall window (16 iterations without halving the FP64
     residual, refinements every 8) stalls the larger RVE, which continues in

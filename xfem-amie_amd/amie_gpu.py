@@ -36,7 +36,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", c_i32), ("precond", c_i32), ("refine", c_i32), ("rtol", c_dbl),
                 ("refine_drop", c_dbl), ("ckpt_interval", c_i64), ("maxit", c_i64), ("use_x0", c_i32), ("use_graph", c_i32),
                 ("chunk", c_i32), ("time_kernels", c_i32), ("escalate", c_i32),
-                ("max_grid", c_i32), ("ref_eps", c_dbl), ("autotune", c_i32), ("stall_iters", c_i64)]
+                ("max_grid", c_i32), ("ref_eps", c_dbl), ("autotune", c_i32), ("stall_iters", c_i64),
+                ("persistent", c_i32)]
 
 
 class SolveStats(ctypes.Structure):

@@ -77,6 +77,7 @@ Batch::~Batch()
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_prob_pinned) (void)hipHostFree(h_prob_pinned);
     if (h_nactive_pinned) (void)hipHostFree(h_nactive_pinned);
+    if (h_persist_pinned) (void)hipHostFree(h_persist_pinned);
 }
 
 size_t Batch::bytes() const
@@ -89,7 +90,7 @@ size_t Batch::bytes() const
                            &bcval, &coords,
                            &econn, &erow, &eslot, &ephase, &edamage, &ekappa, &ed0, &eprob, &cmat, &cmat_off,
                            &macro, &colour_elems, &prob_e0, &prob_ne, &grown, &slot_cptr, &slot_contrib,
-                           &row_cptr, &row_contrib};
+                           &row_cptr, &row_contrib, &persist};
     size_t s = 0;
     for (auto b : all) s += b->n;
     return s;
@@ -256,6 +257,8 @@ static int alloc_vectors(Batch &B)
     if (!B.n_active_prob.alloc(4)) return AMIE_ERR_NOMEM;
     AMIE_HIP(hipHostMalloc((void **)&B.h_prob_pinned, B.nprob * sizeof(ProbDev), hipHostMallocDefault));
     AMIE_HIP(hipHostMalloc((void **)&B.h_nactive_pinned, 2 * sizeof(int32_t), hipHostMallocDefault));
+    if (!B.persist.alloc(16)) return AMIE_ERR_NOMEM;
+    AMIE_HIP(hipHostMalloc((void **)&B.h_persist_pinned, 2 * sizeof(int32_t), hipHostMallocDefault));
     B.h_prob.resize(B.nprob);
     for (int p = 0; p < B.nprob; p++) {
         ProbDev &P = B.h_prob[p];
@@ -739,10 +742,35 @@ int Batch::assemble()
 }
 
 // -------------------------------------------------------------- solve
+// The persistent solve is built for the plain FP64 CG and the FIX48 mixed
+// precision (value planes and value-indexed matrix); the other precisions
+// run the launch path.
+template <typename W, typename M, int Bk, int NR, bool BJ> static void (*persist_for())(SolveArgs, PersistArgs)
+{
+    if constexpr ((std::is_same<W, double>::value && std::is_same<M, double>::value) ||
+                  (std::is_same<W, float>::value && (std::is_same<M, Fix48>::value || std::is_same<M, Dict>::value)))
+        return k_persistent<W, M, Bk, NR, BJ>;
+    else
+        return nullptr;
+}
+
 template <typename W, typename M, int Bk, int NR, bool BJ> static Kern kern_for()
 {
-    return {k_spmv<W, M, Bk, NR, BJ>, k_reduce<NR>, k_update<W, Bk, NR, BJ>, k_ckpt_x<W, Bk, NR>,
-            k_ckpt_r<W, Bk, NR, BJ>, k_rhs_norms<Bk, NR>, k_norms_final<NR>};
+    Kern K{k_spmv<W, M, Bk, NR, BJ>, k_reduce<NR>, k_update<W, Bk, NR, BJ>, k_ckpt_x<W, Bk, NR>,
+           k_ckpt_r<W, Bk, NR, BJ>, k_rhs_norms<Bk, NR>, k_norms_final<NR>};
+    K.kp = persist_for<W, M, Bk, NR, BJ>();
+    return K;
+}
+
+// the persistent solve on the value-indexed FIX48 matrix
+static void (*persist_dict(int b, int nr, bool bj))(SolveArgs, PersistArgs)
+{
+    if (b == 2) {
+        if (nr == 1) return bj ? persist_for<float, Dict, 2, 1, true>() : persist_for<float, Dict, 2, 1, false>();
+        return bj ? persist_for<float, Dict, 2, 3, true>() : persist_for<float, Dict, 2, 3, false>();
+    }
+    if (nr == 1) return bj ? persist_for<float, Dict, 3, 1, true>() : persist_for<float, Dict, 3, 1, false>();
+    return bj ? persist_for<float, Dict, 3, 3, true>() : persist_for<float, Dict, 3, 3, false>();
 }
 
 template <typename W, typename M, int Bk> static Kern kern_nr(int nr, bool bj)
@@ -808,6 +836,7 @@ void Batch::apply_tuned(int mode, bool bj, Kern &K, SolveArgs &a) const
     if (it->second.format == 1) {
         if (!dict_ok) return;   // this assembly has no dictionary: the default kernel
         a.vw = dict_ids.p;
+        if (K.kp) K.kp = persist_dict(et.b, nrhs, bj);
     }
     K.k1 = it->second.k1;
     a.tile_order = it->second.tile_order;
@@ -997,7 +1026,16 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
     launch0(R.K.cx, R.gt, s, R.a);
     launch0(R.K.cr, R.gt, s, R.a);
     R.exec = nullptr;
-    if (o.use_graph && !o.time_kernels) {
+    R.pgrid = 0;
+    R.persist_launched = false;
+    if (o.persistent > 0 && R.K.kp) {
+        // every workgroup must be resident at once (cooperative launch)
+        int per_cu = 0;
+        AMIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)R.K.kp, WG, 0));
+        const int64_t cap = (int64_t)std::max(1, per_cu) * std::max(1, ctx->num_cu);
+        R.pgrid = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)o.persistent, cap, (int64_t)R.gt}));
+    }
+    if (o.use_graph && !o.time_kernels && !R.pgrid) {
         // the captured grids cover this pass's problems; problems that finish
         // leave the device lists and their workgroups return at once
         const int key = mode | (R.bj ? 4 : 0) | (R.ce << 4);
@@ -1039,6 +1077,27 @@ int Batch::solve_advance()
     if (!R.started || R.finished || R.in_flight) return AMIE_OK;
     hipStream_t s = stream;
     R.nt = 0;
+    if (R.pgrid) {
+        // the whole pass in one launch; the status copy follows it
+        if (!R.persist_launched) {
+            AMIE_HIP(hipMemsetAsync(persist.p, 0, persist.n, s));
+            PersistArgs pa;
+            pa.bar = persist.as<unsigned long long>();
+            pa.err = reinterpret_cast<int *>(persist.as<char>() + 8);
+            pa.periods = reinterpret_cast<int *>(persist.as<char>() + 12);
+            pa.ce = R.ce;
+            pa.max_periods = (int)std::min<int64_t>(INT32_MAX, std::max<int64_t>(0, R.max_periods - R.periods));
+            void *kargs[] = {&R.a, &pa};
+            AMIE_HIP(hipLaunchCooperativeKernel((const void *)R.K.kp, dim3(R.pgrid), dim3(WG), kargs, 0, s));
+            AMIE_HIP(hipMemcpyAsync(h_persist_pinned, persist.as<char>() + 8, 8, hipMemcpyDeviceToHost, s));
+            R.persist_launched = true;
+            R.periods = R.max_periods;
+        }
+        AMIE_HIP(hipMemcpyAsync(h_prob_pinned, prob.p, nprob * sizeof(ProbDev), hipMemcpyDeviceToHost, s));
+        AMIE_HIP(hipEventRecord(R.ev_status, s));
+        R.in_flight = true;
+        return AMIE_OK;
+    }
     for (int c = 0; c < R.per_check; c++, R.periods++) {
         if (R.exec) AMIE_HIP(hipGraphLaunch(R.exec, s));
         else queue_period(*this, R, s);
@@ -1063,6 +1122,10 @@ int Batch::solve_poll(bool wait, bool *finished)
         if (!hip_ok(q, "hipEventQuery")) return AMIE_ERR_HIP;
     }
     R.in_flight = false;
+    if (R.pgrid && h_persist_pinned[0]) {
+        set_error("persistent solve: a barrier wait gave up (workgroups not all resident?)");
+        return AMIE_ERR_HIP;
+    }
     for (int i = 0; i < R.nt; i++) {
         float ms = 0.f;
         AMIE_HIP(hipEventElapsedTime(&ms, R.tev[2 * i], R.tev[2 * i + 1]));
@@ -1338,6 +1401,7 @@ void amie_gpu_default_opts(amie_solve_opts *o)
     o->chunk = 32;
     o->time_kernels = 0;
     o->escalate = 1;
+    o->persistent = 0;
 }
 
 int amie_gpu_stream_copy(amie_gpu_ctx *ctx, size_t bytes, int32_t iters, double *ms_per_copy)

@@ -61,6 +61,7 @@ struct Kern {
     void (*cr)(SolveArgs);          // refinement: FP64 residual, p
     void (*norms)(SolveArgs);
     void (*norms_final)(SolveArgs);
+    void (*kp)(SolveArgs, PersistArgs) = nullptr;   // the persistent solve of these kernels, where built
 };
 
 // State of one asynchronous solve (begin / advance / poll / end).
@@ -93,6 +94,8 @@ struct SolveRun {
     int64_t periods = 0, max_periods = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr, ev_status = nullptr;
     bool started = false, in_flight = false, finished = false;
+    int pgrid = 0;                   // > 0: this pass runs as one persistent launch of pgrid workgroups
+    bool persist_launched = false;
     amie_solve_stats stats{};
 };
 
@@ -159,6 +162,8 @@ struct Batch {
     std::vector<ProbDev> h_prob;
     ProbDev *h_prob_pinned = nullptr;
     int32_t *h_nactive_pinned = nullptr;   // [0] tiles, [1] problems
+    DevBuf persist;                        // persistent solve: barrier count (8 bytes), error flag, periods
+    int32_t *h_persist_pinned = nullptr;   // [0] error flag, [1] periods run
 
     // captured iteration chunks, keyed by solve variant
     struct GraphEntry {

@@ -948,10 +948,8 @@ __device__ inline void spmv_tile(const SolveArgs &a, const void *vals, int t, co
 // q = S K (S p).  A problem on the split matrix whose reduce kernel found its
 // accuracy floor (kmode) runs the same product from the FP64 matrix.
 template <typename W, typename M, int B, int NR, bool BJ, int U = Unroll<M, B>::U>
-__global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
+__device__ inline void k1_body(const SolveArgs &a, int t)
 {
-    const int t = my_tile(a);
-    if (t < 0) return;
     const int4 td = a.tile_desc[t];
     const ProbDev &P = a.prob[td.z];
     if (P.status != ST_RUNNING) return;
@@ -961,15 +959,22 @@ __global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
     else spmv_tile<W, M, B, NR, BJ, U>(a, a.vw, t, td, wave, lane);
 }
 
+template <typename W, typename M, int B, int NR, bool BJ, int U = Unroll<M, B>::U>
+__global__ __launch_bounds__(WG) void k_spmv(SolveArgs a)
+{
+    const int t = my_tile(a);
+    if (t < 0) return;
+    k1_body<W, M, B, NR, BJ, U>(a, t);
+}
+
 // ------------------------------------------- R: per-problem CG scalars
 // One workgroup per active problem.  Consumes the residual partials of the
 // previous update (or refinement) and this iteration's SpMV partials.
 template <int NR>
-__global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
+__device__ inline void reduce_body(const SolveArgs &a, int pi)
 {
     __shared__ double lds2[TILE_SLICES][2 * MAXQ];
-    if ((int)blockIdx.x >= *a.n_active_prob) return;
-    ProbDev &P = a.prob[a.active_prob[blockIdx.x]];
+    ProbDev &P = a.prob[a.active_prob[pi]];
     if (P.status != ST_RUNNING) return;   // read by every thread before thread 0 writes (barriers below)
     const bool ck = P.src_ckpt != 0;
     double s1[3 * NR], s0[4 * NR];
@@ -1047,15 +1052,20 @@ __global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
     if (need) P.ckpt_pending = 1;
 }
 
+template <int NR>
+__global__ __launch_bounds__(WG) void k_reduce(SolveArgs a)
+{
+    if ((int)blockIdx.x >= *a.n_active_prob) return;
+    reduce_body<NR>(a, (int)blockIdx.x);
+}
+
 // ------------------------------------ K2: y, r, p update + dots (streaming)
 // slot != 0: a checkpoint follows this iteration; a problem with a pending
 // refinement then only completes its y update (C2 replaces r and forms p).
 template <typename W, int B, int NR, bool BJ>
-__global__ __launch_bounds__(WG) void k_update(SolveArgs a, int slot)
+__device__ inline void k2_body(const SolveArgs &a, int t, int slot)
 {
     __shared__ double lds[TILE_SLICES][MAXQ];
-    const int t = my_tile(a);
-    if (t < 0) return;
     const ProbDev &P = a.prob[a.tile_prob[t]];
     if (P.status != ST_RUNNING) return;
     const bool ck = slot && P.ckpt_pending;
@@ -1103,12 +1113,18 @@ __global__ __launch_bounds__(WG) void k_update(SolveArgs a, int slot)
     if (!ck) write_partials<2 * NR>(part, a.part2, t, lds);
 }
 
-// ------------------------------------------- refinement 1: x64 += S y, y = 0
-template <typename W, int B, int NR>
-__global__ __launch_bounds__(WG) void k_ckpt_x(SolveArgs a)
+template <typename W, int B, int NR, bool BJ>
+__global__ __launch_bounds__(WG) void k_update(SolveArgs a, int slot)
 {
     const int t = my_tile(a);
     if (t < 0) return;
+    k2_body<W, B, NR, BJ>(a, t, slot);
+}
+
+// ------------------------------------------- refinement 1: x64 += S y, y = 0
+template <typename W, int B, int NR>
+__device__ inline void cx_body(const SolveArgs &a, int t)
+{
     const int pi = a.tile_prob[t];
     const ProbDev &P = a.prob[pi];
     if (P.status != ST_RUNNING || !P.ckpt_pending) return;
@@ -1138,13 +1154,19 @@ __global__ __launch_bounds__(WG) void k_ckpt_x(SolveArgs a)
     }
 }
 
-// ------------- refinement 2: r64 = b - K x64 (FP64), r = S r64, p = M r + beta p
-template <typename W, int B, int NR, bool BJ>
-__global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
+template <typename W, int B, int NR>
+__global__ __launch_bounds__(WG) void k_ckpt_x(SolveArgs a)
 {
-    __shared__ double lds[TILE_SLICES][MAXQ];
     const int t = my_tile(a);
     if (t < 0) return;
+    cx_body<W, B, NR>(a, t);
+}
+
+// ------------- refinement 2: r64 = b - K x64 (FP64), r = S r64, p = M r + beta p
+template <typename W, int B, int NR, bool BJ>
+__device__ inline void cr_body(const SolveArgs &a, int t)
+{
+    __shared__ double lds[TILE_SLICES][MAXQ];
     const ProbDev &P = a.prob[a.tile_prob[t]];
     if (P.status != ST_RUNNING || !P.ckpt_pending) return;
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
@@ -1200,6 +1222,99 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
         }
     }
     write_partials<4 * NR>(part, a.partc, t, lds);
+}
+
+template <typename W, int B, int NR, bool BJ>
+__global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
+{
+    const int t = my_tile(a);
+    if (t < 0) return;
+    cr_body<W, B, NR, BJ>(a, t);
+}
+
+// ------------------------------------------------------ persistent solve
+// One launch runs whole checkpoint periods: the phases of queue_period (K1,
+// R, K2 per iteration, then C1 and C2) as loops over the same per-tile and
+// per-problem bodies, separated by device-wide barriers instead of kernel
+// boundaries.  The results are those of the launch path bit for bit: the
+// same bodies in the same order, the same sums.  Meant for small groups:
+// a barrier among G workgroups costs about 2-3 us at G = 64 and grows with
+// G (one arrival counter), while a kernel boundary costs about the same at
+// any size.
+struct PersistArgs {
+    unsigned long long *bar;   // barrier arrivals (only grows), zeroed before the launch
+    int *err;                  // set when a barrier wait passes its bound; every workgroup then returns
+    int *periods;              // checkpoint periods run (workgroup 0)
+    int ce;                    // iterations per checkpoint period
+    int max_periods;
+};
+
+// bound of a barrier wait, in polls (each an L2 round trip plus a short
+// sleep): seconds, far beyond any phase of a group small enough for the
+// persistent path; the wait then gives up instead of hanging the device
+constexpr unsigned PERSIST_SPIN_MAX = 1u << 22;
+
+// Device-wide barrier of the persistent solve: a release (the workgroup's
+// writes visible device-wide: L2 write-back on CDNA, whose XCD L2s are not
+// coherent), one arrival on the counter, a bounded wait until all gridDim.x
+// workgroups of this round arrived, an acquire.  Returns false (every
+// thread) when the wait gave up or another workgroup's did.
+__device__ inline bool persist_barrier(const PersistArgs &pa, unsigned long long &round)
+{
+    __shared__ int ok_s;
+    round++;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(pa.bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long target = round * gridDim.x;
+        unsigned spins = 0;
+        while (__hip_atomic_load(pa.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > PERSIST_SPIN_MAX || __hip_atomic_load(pa.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(pa.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok_s = ok;
+    }
+    __syncthreads();
+    return ok_s != 0;
+}
+
+// Workgroup b takes active tiles b, b + G, ... and active problems b, b + G,
+// ...; every workgroup leaves together once no problem runs (the statuses
+// read after the period's last barrier are the same for all) or after
+// max_periods.  Launched cooperatively (all workgroups resident).
+template <typename W, typename M, int B, int NR, bool BJ>
+__global__ __launch_bounds__(WG) void k_persistent(SolveArgs a, PersistArgs pa)
+{
+    unsigned long long round = 0;
+    const int G = gridDim.x, b = blockIdx.x;
+    const int nt = *a.n_active, np = *a.n_active_prob;
+    int period = 0;
+    while (period < pa.max_periods) {
+        for (int i = 0; i < pa.ce; i++) {
+            for (int it = b; it < nt; it += G) k1_body<W, M, B, NR, BJ>(a, a.active[it]);
+            if (!persist_barrier(pa, round)) return;
+            for (int pi = b; pi < np; pi += G) reduce_body<NR>(a, pi);
+            if (!persist_barrier(pa, round)) return;
+            for (int it = b; it < nt; it += G) k2_body<W, B, NR, BJ>(a, a.active[it], i == pa.ce - 1 ? 1 : 0);
+            if (!persist_barrier(pa, round)) return;
+        }
+        for (int it = b; it < nt; it += G) cx_body<W, B, NR>(a, a.active[it]);
+        if (!persist_barrier(pa, round)) return;
+        for (int it = b; it < nt; it += G) cr_body<W, B, NR, BJ>(a, a.active[it]);
+        if (!persist_barrier(pa, round)) return;
+        period++;
+        bool any = false;
+        for (int pi = 0; pi < np; pi++) any |= a.prob[a.active_prob[pi]].status == ST_RUNNING;
+        if (!any) break;
+    }
+    if (b == 0 && threadIdx.x == 0) *pa.periods = period;
 }
 
 // ------------------------------------------ norms of the right-hand sides
