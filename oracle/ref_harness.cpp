@@ -709,9 +709,68 @@ static int run_cg(const char *path)
     return 0;
 }
 
+// --cg-rate FILE K: the work of K iterations of ConjugateGradient::solve's
+// inner loop (solvers/conjugategradient.cpp:212-253) on the system in FILE
+// (layout of --cg), through the reference's own operations: InverseDiagonal
+// precondition, parallel_inner_product_restricted, the p update, the block
+// SpMV assign(q, A*p), and the Kahan-compensated x and r updates.  Prints
+// "K seconds threads".  (The solver's own maxit is only checked between its
+// restarts, so it cannot bound a solve of a large system to K iterations.)
+static int run_cg_rate(const char *path, int iters)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) { fprintf(stderr, "cannot open %s\n", path); return 2; }
+    int32_t stride = 0;
+    int64_t nrows = 0, nnzb = 0, maxit = 0;
+    bool ok = fread(&stride, 4, 1, f) == 1 && fread(&nrows, 8, 1, f) == 1;
+    std::valarray<unsigned int> rs(ok ? (size_t)nrows : 0);
+    ok = ok && fread(&rs[0], 4, nrows, f) == (size_t)nrows && fread(&nnzb, 8, 1, f) == 1;
+    std::valarray<unsigned int> ci(ok ? (size_t)nnzb : 0);
+    ok = ok && fread(&ci[0], 4, nnzb, f) == (size_t)nnzb;
+    const size_t cl = stride + stride % 2, nval = (size_t)nnzb * stride * cl, n = (size_t)nrows * stride;
+    std::vector<double> vals(ok ? nval : 0), rhs(ok ? n : 0);
+    double eps = 0.;
+    ok = ok && fread(vals.data(), 8, nval, f) == nval && fread(rhs.data(), 8, n, f) == n && fread(&eps, 8, 1, f) == 1 &&
+         fread(&maxit, 8, 1, f) == 1;
+    fclose(f);
+    if (!ok) { fprintf(stderr, "short file %s\n", path); return 2; }
+    CoordinateIndexedSparseMatrix A(rs, ci, stride);
+    for (size_t q = 0; q < nval; q++) A.array[q] = vals[q];
+    InverseDiagonal P(A);
+    Vector r(n), z(n), p(0., n), q(n), x(0., n), rc(0., n), xc(0., n);
+    for (size_t i = 0; i < n; i++) r[i] = rhs[i];
+    double last_rho = 1.;
+    const int vsize = (int)n;
+    const double t0 = omp_get_wtime();
+    for (int it = 0; it < iters; it++) {
+        P.precondition(r, z);
+        const double rho = parallel_inner_product_restricted(&r[0], &z[0], vsize);
+        const double beta = rho / last_rho;
+#pragma omp parallel for
+        for (int i = 0; i < vsize; i++) p[i] = p[i] * beta + z[i];
+        assign(q, A * p, 0, 0);
+        const double pq = parallel_inner_product_restricted(&q[0], &p[0], vsize);
+        const double alpha = pq != 0. ? rho / pq : 0.;
+#pragma omp parallel for
+        for (int i = 0; i < vsize; i++) {
+            const double yr = -q[i] * alpha - rc[i], yx = p[i] * alpha - xc[i];
+            const double rt = r[i] + yr, xt = x[i] + yx;
+            rc[i] = (rt - r[i]) - yr;
+            xc[i] = (xt - x[i]) - yx;
+            r[i] = rt;
+            x[i] = xt;
+        }
+        last_rho = rho;
+    }
+    const double dt = omp_get_wtime() - t0;
+    printf("%d %.9f %d\n", iters, dt, omp_get_max_threads());
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc == 3 && std::string(argv[1]) == "--cg") return run_cg(argv[2]);
+    if (argc == 4 && std::string(argv[1]) == "--cg-rate") return run_cg_rate(argv[2], atoi(argv[3]));
     if (argc < 2) { fprintf(stderr, "usage: %s outdir | --cg system.bin\n", argv[0]); return 2; }
     g_out = argv[1];
     omp_set_num_threads(1);
