@@ -489,6 +489,15 @@ void or_assemble_dofmap(int etype, int64_t nelem, const int32_t *conn, const int
                         const double *cmat, const int32_t *phase, const double *damage, int64_t nrows,
                         const uint32_t *row_size, const uint32_t *column_index, double *array)
 {
+    or_assemble_fractured(etype, nelem, conn, dofmap, coords, cmat, phase, damage, HUGE_VAL, 0., nrows, row_size,
+                          column_index, array);
+}
+
+void or_assemble_fractured(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap, const double *coords,
+                           const double *cmat, const int32_t *phase, const double *damage, double fractured_at,
+                           double residual, int64_t nrows, const uint32_t *row_size, const uint32_t *column_index,
+                           double *array)
+{
     Shape sh = shape_of(etype);
     int b = sh.b, cl = col_len(b), s = (b == 2) ? 3 : 6, n = sh.nper * b;
     std::vector<uint32_t> start = row_starts(nrows, row_size);
@@ -503,8 +512,11 @@ void or_assemble_dofmap(int etype, int64_t nelem, const int32_t *conn, const int
         for (int i = 0; i < sh.nper; i++)
             for (int d = 0; d < sh.dim; d++) xy[i * sh.dim + d] = coords[(size_t)geo[i] * sh.dim + d];
         const double *Cp = cmat + (size_t)(phase ? phase[e] : 0) * s * s;
-        // physics/damagemodels/isotropiclineardamage.cpp:39-46: C * (1 - d)
-        for (int q = 0; q < s * s; q++) C[q] = damage ? Cp[q] * (1. - damage[e]) : Cp[q];
+        // physics/damagemodels/isotropiclineardamage.cpp:39-46: C * (1 - d);
+        // a fractured element keeps C * residual
+        // (fiberbasedisotropiclineardamage.cpp:37-46)
+        const double f = damage ? (damage[e] >= fractured_at ? residual : 1. - damage[e]) : 1.;
+        for (int q = 0; q < s * s; q++) C[q] = damage ? Cp[q] * f : Cp[q];
         or_element_matrix(etype, xy.data(), C.data(), Ke.data());
         // solvers/assembly.cpp:657-735: Kahan adds, (j,j) then (j,k),(k,j) k>j.
         auto kadd = [&](double *a, double *c, double y0) {
@@ -1082,7 +1094,7 @@ void or_smoothed_strain(int etype, int64_t nelem, const int32_t *conn, const dou
         const ElemGeo &ge = geo[e];
         const double R = std::max(radius, ge.r * 1.1);
         const double rho = R * 2.5 + ge.r * 1.5;
-        std::vector<double> acc(s, 0.);
+        std::vector<double> acc(s, 0.), cor(s, 0.);
         double sum = 0.;
         for (int64_t i = 0; i < nelem; i++) {
             if (phase && phase[i] != phase[e]) continue;
@@ -1100,11 +1112,182 @@ void or_smoothed_strain(int etype, int64_t nelem, const int32_t *conn, const dou
                 coef += std::max(smoothing_kernel(kernel, r2 / (R * R)) * gi.meas, 0.);
             }
             coef /= sh.nper;
-            for (int q = 0; q < s; q++) acc[q] += coef * strain[i * s + q];
-            sum += coef;
+            // getAverageField returns the element's measure times its weight
+            // (elements/integrable_entity.cpp:2026-2133); the strains are
+            // Kahan-added with it (mesher/mesh.h:1453-1462)
+            const double v = gi.meas * coef;
+            for (int q = 0; q < s; q++) {
+                const double y = strain[i * s + q] * v - cor[q];
+                const double t = acc[q] + y;
+                cor[q] = (t - acc[q]) - y;
+                acc[q] = t;
+            }
+            sum += v;
         }
         for (int q = 0; q < s; q++) out[e * s + q] = sum > 1e-12 ? acc[q] / sum : 0.;
     }
+}
+
+
+// toPrincipal (elements/integrable_entity.cpp:475-560): principal values of
+// a stress (doubled = 0, SINGLE_OFF_DIAGONAL_VALUES) or an engineering
+// strain (doubled = 1: the shear entries are 2 e_ij).  2D: (tr -/+ delta)/2
+// in that order (cos of half the atan2 angle is never negative); 3D: the
+// trigonometric cubic solution in the reference's operation order.
+void or_principal(int dim, int doubled, const double *v, double *out)
+{
+    if (dim == 2) {
+        const double tr = v[0] + v[1];
+        const double det = doubled ? v[0] * v[1] - 0.25 * v[2] * v[2] : v[0] * v[1] - v[2] * v[2];
+        const double delta = std::sqrt(tr * tr - 4. * det);
+        const double angle = doubled ? 0.5 * std::atan2(0.5 * v[2], v[0] - v[1]) : 0.5 * std::atan2(v[2], v[0] - v[1]);
+        if (std::cos(angle) < 0) {
+            out[0] = (tr + delta) * .5;
+            out[1] = (tr - delta) * .5;
+        } else {
+            out[0] = (tr - delta) * .5;
+            out[1] = (tr + delta) * .5;
+        }
+        return;
+    }
+    // makeStressOrStrainMatrix (utilities/tensor.cpp): [xx xy xz; yx yy yz;
+    // zx zy zz] from [xx yy zz yz xz xy], shear halved for strains
+    const double h = doubled ? 0.5 : 1.;
+    const double m00 = v[0], m11 = v[1], m22 = v[2], m12 = v[3] * h, m02 = v[4] * h, m01 = v[5] * h;
+    const double trmat = -(m00 + m11 + m22);
+    const double detmat = -2.0 * m01 * m02 * m12 + m00 * m12 * m12 + m11 * m02 * m02 + m22 * m01 * m01 - m00 * m11 * m22;
+    const double m2mat = (m00 * m11 + m11 * m22 + m22 * m00) - m02 * m02 - m01 * m01 - m12 * m12;
+    const double q = m2mat / 3. - trmat * trmat / 9.;
+    const double r = (trmat * m2mat - 3. * detmat) / 6. - trmat * trmat * trmat / 27.;
+    const double d = q * q * q + r * r;
+    const double r0 = std::pow(r * r - d, 1. / 6.);
+    double phi = std::atan2(std::sqrt(-1. * d), r) / 3.;
+    if (std::abs(phi) < 1e-12) phi = 0.;
+    if (phi < 0.) phi += M_PI;
+    const double som = r0 * std::cos(phi), dif = r0 * std::sin(phi);
+    out[0] = 2. * som - trmat / 3.;
+    out[1] = -som - trmat / 3. - dif * std::sqrt(3.);
+    out[2] = -som - trmat / 3. + dif * std::sqrt(3.);
+}
+
+// NonLocalMazars::grade, tension branch (physics/fracturecriteria/
+// mazars.cpp:24-31,73-151, the constructor without compressive parameters:
+// tensionOnly), with FractureCriterion::step (fracturecriterion.cpp:534-560:
+// a fractured element scores -1).  Per element of a phase with a criterion
+// (crit[phase] != 0): the smoothed strain sbar (non-local, see
+// or_smoothed_strain) gives the stress sbar.(C (1 - d)) and both principal
+// sets; with e_zz = -nu (e1 + e2) / (1 - nu) in plane stress,
+//   kappa = gamma * max(0, sqrt(sum <e_i>+^2)),  gamma != 1 only when every
+//   principal stress is compressive,
+//   pseudo = min(1 - (k0 / max(k0, kappa)) exp(-B_t (max(k0, kappa) - k0)), 1),
+//   B_t = 0.1 E k0 / (Gf - 0.1 E k0^2 / 2),
+//   score = pseudo - d.
+// prm per phase: {k0, E, nu, Gf}.  Elements of a phase without a criterion
+// score 0 and are never damaged.
+void or_mazars_score(int dim, int64_t nelem, const int32_t *phase, const int32_t *crit, const double *prm,
+                     const double *cmat, const double *sbar, const double *damage, double fractured_at, double *score)
+{
+    const int s = dim == 2 ? 3 : 6;
+    for (int64_t e = 0; e < nelem; e++) {
+        const int ph = phase ? phase[e] : 0;
+        if (!crit[ph]) { score[e] = 0.; continue; }
+        const double d = damage[e];
+        if (d >= fractured_at) { score[e] = -1.; continue; }
+        const double k0 = std::abs(prm[4 * ph]), E = prm[4 * ph + 1], nu = prm[4 * ph + 2], Gf = prm[4 * ph + 3];
+        const double *eps = sbar + e * s;
+        const double *Cp = cmat + (size_t)ph * s * s;
+        double sig[6], ps[3] = {}, pe[3] = {};
+        for (int j = 0; j < s; j++) {
+            double t = 0.;
+            for (int i = 0; i < s; i++) t += eps[i] * (Cp[i * s + j] * (1. - d));
+            sig[j] = t;
+        }
+        or_principal(dim, 0, sig, ps);
+        or_principal(dim, 1, eps, pe);
+        const int np = dim == 2 ? 2 : 3;
+        double trt = 0., trc = 0., sc2 = 0.;
+        for (int i = 0; i < np; i++) {
+            trt += 0.5 * (std::abs(ps[i]) + ps[i]);
+            trc += std::min(0.0, ps[i]);
+            sc2 += std::min(0.0, ps[i]) * std::min(0.0, ps[i]);
+        }
+        double gamma = 1.0;
+        if (trc < -1e-12 && std::abs(trt / trc) < 1e-8) gamma = std::sqrt(sc2) / (-trc);
+        double kappa;
+        auto pos = [](double x) { return 0.5 * (std::abs(x) + x); };
+        if (dim == 2) {
+            const double ezz = -nu * (pe[0] + pe[1]) / (1. - nu);
+            kappa = gamma * std::max(0.0, std::sqrt(std::pow(pos(pe[0]), 2.0) + std::pow(pos(pe[1]), 2.0) +
+                                                    std::pow(pos(ezz), 2.0)));
+        } else {
+            kappa = gamma * std::max(0., std::sqrt(std::pow(pos(pe[0]), 2.0) + std::pow(pos(pe[1]), 2.0) +
+                                                   std::pow(pos(pe[2]), 2.0)));
+        }
+        const double Bt = (0.1 * E * k0) / (Gf - E * 0.1 * k0 * k0 * 0.5);
+        const double kt = std::max(k0, kappa);
+        const double pseudo = std::min(1. * (1. - (k0 / kt) * (std::exp(-Bt * (kt - k0)))), 1.0);
+        score[e] = pseudo - d;
+    }
+}
+
+// One pass of FiberBasedIsotropicLinearDamage::step over the elements
+// (physics/damagemodels/fiberbasedisotropiclineardamage.cpp:58-92, called
+// for every element by FeatureTree::stepElements, features/features.cpp:
+// 5047-5140): an element of a criterion phase that is not fractured, scores
+// above 0 and is within tol (FractureCriterion::scoreTolerance, 2e-3) of the
+// largest score in its neighbourhood takes d += fraction (capped at 1).  The
+// neighbourhood is FractureCriterion::initialiseCache's second cache
+// (fracturecriterion.cpp:171-190): same-phase elements whose centroid lies
+// within max(6 R, 4 r) 2.5 + r of its centroid, R = max(radius, 1.1 r), r
+// the circumradius; getMaxScoreInNeighbourhood (:251-280) takes the maximum
+// over those that are not fractured when the element is visited, its own
+// score included.  Elements are visited in the order the reference's pass
+// runs them with one OpenMP thread: positions >= first_deferred, then
+// 0 .. first_deferred-1 (libgomp runs the first 65 tasks of a single region
+// last); damage written in the pass is seen by the elements visited after.
+// first_deferred < 0: every element sees the damage from before the pass
+// (an order-free pass, for comparison only).
+// changed[e] = 1 for the elements incremented; nbr_max (nullable) receives
+// the neighbourhood maximum each criterion element sees.  Returns the
+// number incremented.
+int64_t or_fiber_pass(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,
+                      const int32_t *crit, const double *score, double *damage, double radius, double fraction,
+                      double fractured_at, double tol, int64_t first_deferred, int32_t *changed, double *nbr_max)
+{
+    Shape sh = shape_of(etype);
+    std::vector<ElemGeo> geo(nelem);
+    for (int64_t e = 0; e < nelem; e++) geo[e] = elem_geo(sh.dim, sh.nper, conn + e * sh.nper, coords);
+    int64_t nch = 0;
+    const int64_t fd = std::min<int64_t>(std::max<int64_t>(first_deferred, 0), nelem);
+    std::vector<double> before(damage, damage + nelem);
+    const double *seen = first_deferred < 0 ? before.data() : damage;
+    for (int64_t t = 0; t < nelem; t++) {
+        const int64_t e = t < nelem - fd ? t + fd : t - (nelem - fd);
+        changed[e] = 0;
+        const int ph = phase ? phase[e] : 0;
+        if (!crit[ph]) continue;
+        if (!(score[e] > 0.) && !nbr_max) continue;
+        const ElemGeo &ge = geo[e];
+        const double R = std::max(radius, ge.r * 1.1);
+        const double rho = std::max(R * 6., ge.r * 4.) * 2.5 + ge.r;
+        double mx = score[e];
+        for (int64_t i = 0; i < nelem; i++) {
+            if ((phase ? phase[i] : 0) != ph) continue;
+            const ElemGeo &gi = geo[i];
+            double d2 = 0.;
+            for (int d = 0; d < sh.dim; d++) d2 += (gi.c[d] - ge.c[d]) * (gi.c[d] - ge.c[d]);
+            const bool in = sh.dim == 2 ? d2 < rho * rho : d2 < rho * rho + 2. * rho * 1e-12 + 1e-24;
+            if (!in || seen[i] >= fractured_at) continue;
+            mx = std::max(mx, score[i]);
+        }
+        if (nbr_max) nbr_max[e] = mx;
+        if (score[e] > 0. && damage[e] < fractured_at && std::abs(score[e] - mx) < tol) {
+            damage[e] = std::min(damage[e] + fraction, 1.);
+            changed[e] = 1;
+            nch++;
+        }
+    }
+    return nch;
 }
 
 } // extern "C"

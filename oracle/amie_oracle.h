@@ -57,6 +57,13 @@ void or_assemble(int etype, int64_t nelem, const int32_t *conn, const double *co
 void or_assemble_dofmap(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap, const double *coords,
                         const double *cmat, const int32_t *phase, const double *damage, int64_t nrows,
                         const uint32_t *row_size, const uint32_t *column_index, double *array);
+/* or_assemble_dofmap where an element with damage >= fractured_at keeps
+   C * residual instead of C * (1 - d)
+   (physics/damagemodels/fiberbasedisotropiclineardamage.cpp:37-46). */
+void or_assemble_fractured(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap, const double *coords,
+                           const double *cmat, const int32_t *phase, const double *damage, double fractured_at,
+                           double residual, int64_t nrows, const uint32_t *row_size, const uint32_t *column_index,
+                           double *array);
 
 /* Load of an imposed macro strain E (Voigt, engineering shear): u_E = E.x at
  * every node, f -= sum_e K_e u_E,e scattered through the DOF map. */
@@ -136,15 +143,32 @@ void or_average_stress(int etype, int64_t nelem, const int32_t *conn, const doub
  *   the 1e-12 tolerance of geometry_3D.cpp:1551-1566);
  *   kernel (:43-61) on rrn = |x - c_e|^2 / R_e^2: 0 QUARTIC_COMPACT
  *   ((rrn/2 - 1)^2 where rrn/2 <= 1), 1 GAUSSIAN_NONCOMPACT (exp(-rrn));
- *   weight of neighbour i (one Gauss point): mean over its vertices v of
- *   max(w(v) * measure_i, 0);
+ *   cache weight of neighbour i (one Gauss point): mean over its vertices
+ *   v of max(w(v) * measure_i, 0); getAverageField returns measure_i times
+ *   it (elements/integrable_entity.cpp:2026-2133), so
+ *   v_i = measure_i * weight_i;
  *   neighbours of another phase (behaviour source) or fractured
  *   (damage >= threshold, damagemodels/isotropiclineardamage.cpp:57-62)
- *   left out; smoothed = sum w_i eps_i / sum w_i, 0 when sum w_i <= 1e-12.
+ *   left out; smoothed = sum v_i eps_i / sum v_i (Kahan-added as
+ *   mesher/mesh.h:1453-1462), 0 when sum v_i <= 1e-12.
  * Brute force over all element pairs.  strain, out: nelem x 3|6. */
 void or_smoothed_strain(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,
                         const double *damage, double threshold, double radius, int kernel, const double *strain,
                         double *out);
+
+/* toPrincipal (elements/integrable_entity.cpp:475-560). */
+void or_principal(int dim, int doubled, const double *v, double *out);
+
+/* NonLocalMazars::grade, tension branch (physics/fracturecriteria/
+   mazars.cpp:73-151), per element from its smoothed strain. */
+void or_mazars_score(int dim, int64_t nelem, const int32_t *phase, const int32_t *crit, const double *prm,
+                     const double *cmat, const double *sbar, const double *damage, double fractured_at, double *score);
+
+/* One pass of FiberBasedIsotropicLinearDamage::step over the mesh
+   (physics/damagemodels/fiberbasedisotropiclineardamage.cpp:58-92). */
+int64_t or_fiber_pass(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,
+                      const int32_t *crit, const double *score, double *damage, double radius, double fraction,
+                      double fractured_at, double tol, int64_t first_deferred, int32_t *changed, double *nbr_max);
 
 /* FeatureTree::renumber (features/features.cpp:603-729, one time plane):
  * the elements are walked in order and each node takes the next id the first

@@ -53,6 +53,11 @@ def lib():
         L.or_damage_update.restype = i64
         L.or_average_stress.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp]
         L.or_smoothed_strain.argtypes = [i32, i64, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
+        L.or_assemble_fractured.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, dbl, i64, vp, vp, vp]
+        L.or_principal.argtypes = [i32, i32, vp, vp]
+        L.or_mazars_score.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, vp]
+        L.or_fiber_pass.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, i64, vp, vp]
+        L.or_fiber_pass.restype = i64
         L.or_renumber.argtypes = [i64, i64, i32, vp, vp]
         L.or_renumber.restype = i64
         L.or_add_midpoints.argtypes = [i64, vp, i64, vp, vp, vp]
@@ -355,3 +360,88 @@ def voxel_owner(xyz, conn, box, n):
     lib().or_voxel_owner(xyz.shape[0], _p(xyz), conn.shape[0], _p(conn), _p(box), n, n, n, _p(owner))
     return owner
 
+
+
+# ------------------------------------------------------- event-driven damage
+# FeatureTree's damage iteration with StiffnessAndFracture + NonLocalMazars +
+# FiberBasedIsotropicLinearDamage (its default damage model): see
+# or_mazars_score and or_fiber_pass in amie_oracle.cpp.
+FIBRE_FRACTION, FRACTURED_AT, RESIDUAL, SCORE_TOLERANCE, FIRST_DEFERRED = 0.1, 0.6, 1e-4, 2e-3, 65
+
+
+def assemble_fractured(etype, conn, coords, cmats, phase, damage, fractured_at=FRACTURED_AT, residual=RESIDUAL):
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    cmats = _c(cmats, np.float64)
+    phase = _c(phase, np.int32)
+    damage = _c(damage, np.float64)
+    rs, ci = pattern(etype, conn, coords.shape[0])
+    A = BlockCSR(DIM[etype], rs, ci)
+    lib().or_assemble_fractured(etype, conn.shape[0], _p(conn), None, _p(coords), _p(cmats), _p(phase), _p(damage),
+                                fractured_at, residual, A.nrows, _p(A.row_size), _p(A.column_index), _p(A.array))
+    return A
+
+
+def principal(v, doubled):
+    """toPrincipal of one stress (doubled False) or engineering strain."""
+    v = _c(np.asarray(v, dtype=np.float64), np.float64)
+    dim = 2 if v.shape[0] == 3 else 3
+    out = np.zeros(dim)
+    lib().or_principal(dim, int(bool(doubled)), _p(v), _p(out))
+    return out
+
+
+def mazars_score(dim, phase, crit, prm, cmats, sbar, damage, fractured_at=FRACTURED_AT):
+    """NonLocalMazars fracture score of every element from its smoothed
+    strain; prm[phase] = (threshold, E, nu, Gf)."""
+    phase = _c(phase, np.int32)
+    crit = _c(np.asarray(crit), np.int32)
+    prm = _c(np.asarray(prm, dtype=np.float64).reshape(-1, 4), np.float64)
+    cmats = _c(cmats, np.float64)
+    sbar = _c(sbar, np.float64)
+    damage = _c(damage, np.float64)
+    out = np.zeros(sbar.shape[0])
+    lib().or_mazars_score(dim, sbar.shape[0], _p(phase), _p(crit), _p(prm), _p(cmats), _p(sbar), _p(damage),
+                          fractured_at, _p(out))
+    return out
+
+
+def fiber_pass(etype, conn, coords, phase, crit, score, damage, radius, fraction=FIBRE_FRACTION,
+               fractured_at=FRACTURED_AT, tol=SCORE_TOLERANCE, first_deferred=FIRST_DEFERRED, nbr_max=None):
+    """One FiberBasedIsotropicLinearDamage pass; updates damage in place and
+    returns the mask of the elements it incremented (nbr_max, if given,
+    receives each criterion element's neighbourhood maximum)."""
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    phase = _c(phase, np.int32)
+    crit = _c(np.asarray(crit), np.int32)
+    score = _c(score, np.float64)
+    assert damage.dtype == np.float64 and damage.flags.c_contiguous
+    changed = np.zeros(conn.shape[0], dtype=np.int32)
+    lib().or_fiber_pass(etype, conn.shape[0], _p(conn), _p(coords), _p(phase), _p(crit), _p(score), _p(damage),
+                        radius, fraction, fractured_at, tol, first_deferred, _p(changed), _p(nbr_max))
+    return changed.astype(bool)
+
+
+def event_damage(etype, conn, coords, cmats, phase, crit, prm, bc_dofs, bc_vals, damage, radius,
+                 kernel=QUARTIC_COMPACT, rtol=1e-12, max_passes=100000, first_deferred=FIRST_DEFERRED):
+    """One FeatureTree::step of the event-driven damage (features/features.cpp:
+    6150-6373): solve with the current damage (fractured elements at the
+    residual stiffness), score every element on its smoothed strain, run one
+    damage pass; repeat until a pass changes nothing.  Returns (damage,
+    displacements, scores of the last pass, passes)."""
+    d = np.array(damage, dtype=np.float64)
+    dim = DIM[etype]
+    passes = 0
+    while True:
+        A = assemble_fractured(etype, conn, coords, cmats, phase, d)
+        b = np.zeros(A.n)
+        apply_dirichlet(A, b, bc_dofs, bc_vals)
+        x = pcg(A, b, rtol=rtol, maxit=100000)[0]
+        eps = element_strain(etype, conn, coords, x)
+        sbar = smoothed_strain(etype, conn, coords, eps, radius, kernel, phase, d, FRACTURED_AT)
+        score = mazars_score(dim, phase, crit, prm, cmats, sbar, d)
+        changed = fiber_pass(etype, conn, coords, phase, crit, score, d, radius, first_deferred=first_deferred)
+        passes += 1
+        if not changed.any() or passes >= max_passes:
+            return d, x, score, passes

@@ -31,6 +31,9 @@
 #include "physics/fracturecriteria/fracturecriterion.h"
 #include "polynomial/vm_function_extra.h"
 #include "physics/stiffness.h"
+#include "physics/stiffness_and_fracture.h"
+#include "physics/fracturecriteria/mazars.h"
+#include "physics/damagemodels/fiberbasedisotropiclineardamage.h"
 #include "solvers/assembly.h"
 #include "solvers/conjugategradient.h"
 #include "solvers/inversediagonal.h"
@@ -662,6 +665,125 @@ static void golden_writer(const std::string &tag, double sampling)
     npy_f64(tag + "_stress", sig, {sig.size() / 9, 3, 3});
 }
 
+// The reference's event-driven damage through FeatureTree::step
+// (features/features.cpp:6150-6373, stepElements :4875-5290): a unit square
+// of StiffnessAndFracture(E, nu, NonLocalMazars(threshold, E, nu, Gf,
+// radius)) with the default FiberBasedIsotropicLinearDamage (+0.1 per event,
+// fractured at 0.6, residual stiffness 1e-4; physics/damagemodels/
+// fiberbasedisotropiclineardamage.cpp:14-100) around a stiffer elastic
+// inclusion, u_x = 0 left, u_y = 0 bottom, u_x = load[k] right, one
+// FeatureTree::step per load.  After each step: every element's damage and
+// fracture score (mesh order), the displacements and the number of damage
+// model step() calls (passes x damaging elements); once, the order the
+// first pass visits the elements in (element positions; every pass visits
+// them in the same order) and, per call of that pass, the element's score,
+// its neighbourhood's maximum score and its smoothed principal strains and
+// stresses.
+struct LoggedFiberDamage : public FiberBasedIsotropicLinearDamage {
+    static std::vector<int64_t> *calls;
+    static std::map<const IntegrableEntity *, int64_t> *position;
+    // the first pass in full: per call the element's score, the maximum
+    // score of its neighbourhood and its smoothed principal strains and
+    // stresses, as the criterion sees them
+    static std::vector<double> *first;
+    static size_t first_calls;
+    static FILE *trace;  // AMIE_EVENT_TRACE: one line per call (debugging only, not a fixture)
+    LoggedFiberDamage(double f, double c) : FiberBasedIsotropicLinearDamage(f, c) { }
+    virtual void step(ElementState &s, double maxscore)
+    {
+        FractureCriterion *fc = s.getParent()->getBehaviour()->getFractureCriterion();
+        const bool record = fc && ((first && first->size() < first_calls * 6) || trace);
+        const double before = getState()[0];
+        const double nbr = record ? fc->getMaxScoreInNeighbourhood(s) : 0.;
+        std::pair<Vector, Vector> sf;
+        if (record) sf = fc->getSmoothedFields(PRINCIPAL_REAL_STRESS_FIELD, PRINCIPAL_TOTAL_STRAIN_FIELD, s, 0.);
+        FiberBasedIsotropicLinearDamage::step(s, maxscore);
+        const int64_t pos = position && position->count(s.getParent()) ? (*position)[s.getParent()] : -1;
+        if (calls) calls->push_back(pos);
+        if (record && first && first->size() < first_calls * 6)
+            first->insert(first->end(), {fc->getScoreAtState(), nbr, sf.second[0], sf.second[1], sf.first[0], sf.first[1]});
+        if (record && trace)
+            fprintf(trace, "%lld %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", (long long)pos, fc->getScoreAtState(),
+                    nbr, before, getState()[0], sf.first[0], sf.first[1], sf.second[0], sf.second[1]);
+    }
+    virtual DamageModel *getCopy() const
+    {
+        LoggedFiberDamage *ret = new LoggedFiberDamage(fibreFraction, getThresholdDamageDensity());
+        ret->copyEssentialParameters(this);
+        return ret;
+    }
+};
+std::vector<int64_t> *LoggedFiberDamage::calls = nullptr;
+std::map<const IntegrableEntity *, int64_t> *LoggedFiberDamage::position = nullptr;
+std::vector<double> *LoggedFiberDamage::first = nullptr;
+size_t LoggedFiberDamage::first_calls = 0;
+FILE *LoggedFiberDamage::trace = nullptr;
+
+static void golden_events(const std::string &tag, double sampling, const std::vector<double> &loads, double E,
+                          double nu, double threshold, double Gf, double radius, double E_inc)
+{
+    RectangularFeature sample(nullptr, 1., 1., 0.5, 0.5);
+    sample.setBehaviour(new StiffnessAndFracture(E, nu, new NonLocalMazars(threshold, E, nu, Gf, radius, PLANE_STRESS),
+                                                 new LoggedFiberDamage(0.1, 0.6)));
+    FeatureTree F(&sample);
+    Inclusion *inc = new Inclusion(&sample, 0.18, 0.5, 0.5);
+    inc->setBehaviour(new Stiffness(E_inc, nu));
+    F.addFeature(&sample, inc);
+    F.setSamplingNumber(sampling);
+    F.setOrder(LINEAR);
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_XI, LEFT));
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_ETA, BOTTOM));
+    BoundingBoxDefinedBoundaryCondition *right = new BoundingBoxDefinedBoundaryCondition(SET_ALONG_XI, RIGHT, 0.);
+    F.addBoundaryCondition(right);
+    renumber_tree(F);
+    auto *m = F.get2DMesh();
+    std::map<const IntegrableEntity *, int64_t> pos;
+    std::vector<int64_t> phase;
+    for (auto i = m->begin(); i != m->end(); i++) {
+        pos[&*i] = i.getPosition();
+        phase.push_back(i->getBehaviour()->getFractureCriterion() ? 0 : 1);
+    }
+    std::vector<int64_t> calls, ncalls;
+    std::vector<double> dmg, score, u, first;
+    int64_t ncrit = 0;
+    for (int64_t v : phase) ncrit += v == 0;
+    LoggedFiberDamage::position = &pos;
+    LoggedFiberDamage::calls = &calls;
+    LoggedFiberDamage::first = &first;
+    LoggedFiberDamage::first_calls = (size_t)ncrit;
+    if (getenv("AMIE_EVENT_TRACE")) LoggedFiberDamage::trace = fopen(getenv("AMIE_EVENT_TRACE"), "w");
+    for (double v : loads) {
+        right->setData(v);
+        const size_t before = calls.size();
+        F.step();
+        ncalls.push_back((int64_t)(calls.size() - before));
+        for (auto i = m->begin(); i != m->end(); i++) {
+            DamageModel *dm = i->getBehaviour()->getDamageModel();
+            FractureCriterion *fc = i->getBehaviour()->getFractureCriterion();
+            dmg.push_back(dm ? dm->getState()[0] : 0.);
+            score.push_back(fc ? fc->getScoreAtState() : 0.);
+        }
+        Vector x = F.getDisplacements();
+        u.insert(u.end(), std::begin(x), std::end(x));
+    }
+    LoggedFiberDamage::calls = nullptr;
+    LoggedFiberDamage::position = nullptr;
+    LoggedFiberDamage::first = nullptr;
+    if (LoggedFiberDamage::trace) fclose(LoggedFiberDamage::trace);
+    LoggedFiberDamage::trace = nullptr;
+    const size_t ne = phase.size(), ns = loads.size();
+    dump_mesh(tag, m, 2);
+    npy_i64(tag + "_phase", phase);
+    npy_f64(tag + "_loads", loads);
+    npy_f64(tag + "_params", {E, nu, threshold, Gf, radius, E_inc});
+    npy_f64(tag + "_damage", dmg, {ns, ne});
+    npy_f64(tag + "_score", score, {ns, ne});
+    npy_f64(tag + "_u", u, {ns, u.size() / ns});
+    npy_i64(tag + "_ncalls", ncalls);
+    npy_i64(tag + "_order", std::vector<int64_t>(calls.begin(), calls.begin() + ncrit));
+    npy_f64(tag + "_first", first, {first.size() / 6, 6});
+}
+
 // The reference's own ConjugateGradient::solve (solvers/conjugategradient.cpp:
 // 69-318, default InverseDiagonal preconditioner, nssor 32 as
 // Assembly::cgsolve sets it, solvers/assembly.h:262 and assembly.cpp:1842)
@@ -771,7 +893,17 @@ int main(int argc, char **argv)
 {
     if (argc == 3 && std::string(argv[1]) == "--cg") return run_cg(argv[2]);
     if (argc == 4 && std::string(argv[1]) == "--cg-rate") return run_cg_rate(argv[2], atoi(argv[3]));
-    if (argc < 2) { fprintf(stderr, "usage: %s outdir | --cg system.bin\n", argv[0]); return 2; }
+    if (argc == 3 && std::string(argv[1]) == "--events") {
+        g_out = argv[2];
+        omp_set_num_threads(1);
+        golden_events("events_p1", 10, {0.8e-4, 1.2e-4, 1.6e-4, 2.0e-4}, 1e10, 0.2, 1e-4, 100., 0.05, 4e10);
+        // a smaller radius: elements fractured within a pass change which
+        // later-visited elements are at their neighbourhood's maximum
+        golden_events("events_r01", 10, {1.1e-4, 1.375e-4, 1.65e-4}, 1e10, 0.2, 1e-4, 30., 0.01, 4e10);
+        printf("event-damage golden vectors written to %s\n", g_out.c_str());
+        return 0;
+    }
+    if (argc < 2) { fprintf(stderr, "usage: %s outdir | --events outdir | --cg system.bin\n", argv[0]); return 2; }
     g_out = argv[1];
     omp_set_num_threads(1);
     golden_cauchy_green();

@@ -1,0 +1,157 @@
+"""Pins the oracle's event-driven damage (or_principal, or_mazars_score,
+or_fiber_pass and the FeatureTree loop of oracle.event_damage) against two
+runs of the compiled reference's FeatureTree::step with NonLocalMazars and
+FiberBasedIsotropicLinearDamage (tests/events_case.py), and the pieces
+against what the mathematics fixes: principal values against
+numpy.linalg.eigvalsh, the Mazars score of a uniaxial strain in closed form,
+and a fibre pass on a hand-made case."""
+import numpy as np
+import pytest
+
+import events_case as ev
+import oracle as orc
+
+
+@pytest.fixture(autouse=True)
+def _one_thread():
+    with orc.one_thread():
+        yield
+
+
+def run_case(case, first_deferred=orc.FIRST_DEFERRED):
+    d = np.zeros(case["conn"].shape[0])
+    out = []
+    for v in case["loads"]:
+        d, x, sc, passes = orc.event_damage(orc.P1, case["conn"], case["coords"], case["cmats"], case["phase"],
+                                            case["crit"], case["prm"], case["bc_dofs"], ev.bc_values(case, v), d,
+                                            case["radius"], first_deferred=first_deferred)
+        out.append((d.copy(), x, sc, passes))
+    return out
+
+
+@pytest.mark.parametrize("tag", ev.TAGS)
+def test_event_damage_trajectory_vs_reference(tag):
+    """Every load step: the damage of every element equal to the
+    reference's (multiples of the 0.1 fibre fraction, fractured at 0.6),
+    the number of passes (solves) equal, the scores of the last pass to
+    1e-8 and the displacements to 1e-9."""
+    case = ev.load(tag)
+    m = case["phase"] == 0
+    for k, (d, x, sc, passes) in enumerate(run_case(case)):
+        np.testing.assert_allclose(d, case["damage"][k], rtol=0, atol=1e-12, err_msg=f"step {k}")
+        assert passes == case["passes"][k], (k, passes, case["passes"][k])
+        np.testing.assert_allclose(sc[m], case["score"][k][m], rtol=0, atol=1e-8)
+        assert np.linalg.norm(x - case["u"][k]) <= 1e-9 * np.linalg.norm(case["u"][k])
+    assert (case["damage"][-1] >= orc.FRACTURED_AT).sum() > 10     # the runs do fracture elements
+
+
+def test_visiting_order_is_the_references():
+    """On the small-radius run an element fractured within a pass changes
+    which later-visited elements are at their neighbourhood's maximum: the
+    reference's order (positions >= 65 first, then 0..64) reproduces it,
+    mesh order and an order-free pass (every element sees the damage from
+    before the pass) do not."""
+    case = ev.load("events_r01")
+    nc = case["ncrit"]
+    pos = np.nonzero(case["phase"] == 0)[0]
+    np.testing.assert_array_equal(case["order"], np.concatenate([pos[pos >= 65], pos[pos < 65]]))
+    for fd in (0, -1):
+        d = run_case(case, first_deferred=fd)[0][0]
+        assert np.abs(d - case["damage"][0]).max() > 0.05, fd
+    assert nc == len(case["order"])
+
+
+@pytest.mark.parametrize("tag", ev.TAGS)
+def test_first_pass_scores_vs_reference(tag):
+    """The first solve of the first step: each element's smoothed principal
+    strains and stresses (Mesh::getSmoothedFields with the measure-weighted
+    cache), its NonLocalMazars score and its neighbourhood's maximum score
+    as the reference's criterion saw them."""
+    case = ev.load(tag)
+    conn, coords, phase = case["conn"], case["coords"], case["phase"]
+    d = np.zeros(conn.shape[0])
+    A = orc.assemble_fractured(orc.P1, conn, coords, case["cmats"], phase, d)
+    b = np.zeros(A.n)
+    orc.apply_dirichlet(A, b, case["bc_dofs"], ev.bc_values(case, case["loads"][0]))
+    x = orc.pcg(A, b, rtol=1e-12, maxit=100000)[0]
+    eps = orc.element_strain(orc.P1, conn, coords, x)
+    sbar = orc.smoothed_strain(orc.P1, conn, coords, eps, case["radius"], orc.QUARTIC_COMPACT, phase, d,
+                               orc.FRACTURED_AT)
+    sc = orc.mazars_score(2, phase, case["crit"], case["prm"], case["cmats"], sbar, d)
+    order, first = case["order"], case["first"]
+    pe = np.array([orc.principal(sbar[e], True) for e in order])
+    ps = np.array([orc.principal(sbar[e] @ (case["cmats"][0]), False) for e in order])
+    np.testing.assert_allclose(pe, first[:, 2:4], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(ps, first[:, 4:6], rtol=0, atol=1e-9 * np.abs(first[:, 4:6]).max())
+    np.testing.assert_allclose(sc[order], first[:, 0], rtol=0, atol=1e-9)
+    # the neighbourhood maxima over the reference's own scores (the first
+    # pass increments nothing before the elements it fractures)
+    ref_sc = np.zeros(conn.shape[0])
+    ref_sc[order] = first[:, 0]
+    nbr = np.zeros(conn.shape[0])
+    changed = orc.fiber_pass(orc.P1, conn, coords, phase, case["crit"], ref_sc, d.copy(), case["radius"],
+                             nbr_max=nbr)
+    np.testing.assert_array_equal(nbr[order], first[:, 1])
+    assert changed.sum() >= 1
+
+
+@pytest.mark.parametrize("doubled", [False, True])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_principal_values_vs_eigvalsh(dim, doubled):
+    """toPrincipal: the eigenvalues of the symmetric tensor (strain shear
+    entries halved), 2D in ascending order."""
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        v = rng.normal(size=3 if dim == 2 else 6)
+        h = 0.5 if doubled else 1.
+        if dim == 2:
+            T = np.array([[v[0], v[2] * h], [v[2] * h, v[1]]])
+        else:
+            T = np.array([[v[0], v[5] * h, v[4] * h], [v[5] * h, v[1], v[3] * h], [v[4] * h, v[3] * h, v[2]]])
+        ref = np.linalg.eigvalsh(T)
+        out = orc.principal(v, doubled)
+        if dim == 2:
+            np.testing.assert_allclose(out, ref, rtol=0, atol=1e-14 * np.abs(v).max() * 10)
+        else:
+            np.testing.assert_allclose(np.sort(out), ref, rtol=0, atol=1e-12 * np.abs(v).max())
+
+
+def test_mazars_score_uniaxial_closed_form():
+    """Plane stress, strain (e, -nu e, 0) in the element frame: principal
+    strains e and -nu e, e_zz = -nu (1 - nu) e / (1 - nu) < 0, so
+    kappa = e, and score = 1 - (k0/e) exp(-B_t (e - k0)) - d with
+    B_t = 0.1 E k0 / (Gf - 0.05 E k0^2); below k0 the score is -d; a
+    fractured element scores -1; a phase without a criterion scores 0."""
+    E, nu, k0, Gf = 1e10, 0.2, 1e-4, 100.
+    C = orc.cauchy_green(E, nu)
+    Bt = 0.1 * E * k0 / (Gf - 0.05 * E * k0 * k0)
+    for e, d in ((1.5e-4, 0.), (2.5e-4, 0.3), (0.5e-4, 0.2), (1.2e-4, 0.65)):
+        sbar = np.array([[e, -nu * e, 0.], [e, -nu * e, 0.]])
+        sc = orc.mazars_score(2, np.array([0, 1], dtype=np.int32), [1, 0], [[k0, E, nu, Gf], [0, 0, 0, 1]],
+                              np.stack([C, C]), sbar, np.array([d, d]))
+        if d >= orc.FRACTURED_AT:
+            expect = -1.
+        elif e <= k0:
+            expect = -d
+        else:
+            expect = 1. - (k0 / e) * np.exp(-Bt * (e - k0)) - d
+        assert abs(sc[0] - expect) <= 1e-13, (e, d, sc[0], expect)
+        assert sc[1] == 0.
+
+
+def test_fiber_pass_hand_case():
+    """Lone triangles far apart (each its own neighbourhood) and two close
+    pairs, one phase: a lone one with a positive score takes +0.1 (also at
+    0.55, which it leaves fractured at 0.65), one with a negative score does
+    not; of a pair 0.1 apart only the higher score moves, of a pair 1e-3
+    apart (within the 2e-3 tolerance) both move."""
+    tri = np.array([[0., 0.], [0.01, 0.], [0., 0.01]])
+    offs = [(0, 0), (10, 0), (20, 0), (30, 0), (30.002, 0), (40, 0), (40.002, 0)]
+    coords = np.vstack([tri + np.array(o) for o in offs])
+    conn = np.arange(len(offs) * 3, dtype=np.int32).reshape(-1, 3)
+    phase = np.zeros(len(offs), dtype=np.int32)
+    score = np.array([0.2, -0.1, 0.05, 0.3, 0.2, 0.1, 0.099])
+    d = np.array([0., 0., 0.55, 0., 0., 0., 0.])
+    ch = orc.fiber_pass(orc.P1, conn, coords, phase, [1], score, d, 0.001, first_deferred=0)
+    np.testing.assert_array_equal(ch, [True, False, True, True, False, True, True])
+    np.testing.assert_allclose(d, [0.1, 0., 0.65, 0.1, 0., 0.1, 0.1], rtol=0, atol=1e-15)

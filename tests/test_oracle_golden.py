@@ -186,16 +186,23 @@ def test_smoothed_strain_vs_reference_caches(tag, kernel):
     """Non-local strains against the neighbourhoods and weights the compiled
     reference's Mesh::generateCache builds (mesher/mesh.h:546-640, with the
     locus and kernel of FractureCriterion::initialiseCache) on a Delaunay
-    mesh of 150 seeded points: sum w_i eps_i / sum w_i over the reference's
-    cache equals the oracle's smoothing for a random strain field."""
+    mesh of 150 seeded points, composed as Mesh::getSmoothedFields composes
+    them (mesher/mesh.h:1440-1466): each neighbour i counts with
+    v_i = area_i * w_i, the value ElementState::getAverageField returns
+    (elements/integrable_entity.cpp:2026-2133), and the smoothed strain is
+    sum v_i eps_i / sum v_i, for a random strain field."""
     xy, conn = g(tag + "_xy"), g(tag + "_conn").astype(np.int32)
     ptr, idx, w = g(tag + "_ptr"), g(tag + "_idx"), g(tag + "_w")
     radius = float(g(tag + "_radius")[0])
     eps = np.random.default_rng(5).normal(size=(conn.shape[0], 3))
+    t = xy[conn]
+    area = 0.5 * np.abs((t[:, 1, 0] - t[:, 0, 0]) * (t[:, 2, 1] - t[:, 0, 1]) -
+                        (t[:, 1, 1] - t[:, 0, 1]) * (t[:, 2, 0] - t[:, 0, 0]))
     ref = np.zeros_like(eps)
     for e in range(conn.shape[0]):
         sl = slice(ptr[e], ptr[e + 1])
-        ref[e] = (w[sl, None] * eps[idx[sl]]).sum(0) / w[sl].sum()
+        v = area[idx[sl]] * w[sl]
+        ref[e] = (v[:, None] * eps[idx[sl]]).sum(0) / v.sum()
     out = orc.smoothed_strain(orc.P1, conn, xy, eps, radius, kernel=kernel)
     np.testing.assert_allclose(out, ref, rtol=1e-11, atol=1e-12)
 

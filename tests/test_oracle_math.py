@@ -397,7 +397,9 @@ def test_smoothed_strain_constant_per_phase(etype, n):
 @pytest.mark.parametrize("etype,n", [(P1, 10), (orc.TET4, 3)])
 def test_smoothed_strain_wide_kernel_is_the_phase_average(etype, n, kernel):
     """A radius far above the mesh takes every same-phase element with a
-    weight that tends to its measure: the measure-weighted phase average."""
+    cache weight that tends to its measure, counted times its measure again
+    (getAverageField returns measure * weight): the average weighted by the
+    squared measures."""
     coords, conn = _mesh(etype, n)
     ne = conn.shape[0]
     s = 3 if etype == P1 else 6
@@ -408,7 +410,7 @@ def test_smoothed_strain_wide_kernel_is_the_phase_average(etype, n, kernel):
     meas = _measures(etype, conn, coords)
     for ph in (0, 1):
         m = phase == ph
-        avg = (meas[m, None] * eps[m]).sum(0) / meas[m].sum()
+        avg = (meas[m, None] ** 2 * eps[m]).sum(0) / (meas[m] ** 2).sum()
         np.testing.assert_allclose(out[m], np.tile(avg, (m.sum(), 1)), rtol=0, atol=1e-5 * np.abs(eps).max())
 
 

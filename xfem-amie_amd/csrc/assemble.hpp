@@ -787,8 +787,8 @@ __global__ __launch_bounds__(256) void k_smooth(NonlocalArgs a, const double *st
         lo[d] = max(0, (int)floor((c[d] - rho - g[d]) / g[3]));
         hi[d] = min(gd[d] - 1, (int)floor((c[d] + rho - g[d]) / g[3]));
     }
-    double acc[S], sum = 0.;
-    for (int q = 0; q < S; q++) acc[q] = 0.;
+    double acc[S], cor[S], sum = 0.;
+    for (int q = 0; q < S; q++) acc[q] = cor[q] = 0.;
     for (int z = lo[2]; z <= hi[2]; z++)
         for (int y = lo[1]; y <= hi[1]; y++)
             for (int x = lo[0]; x <= hi[0]; x++) {
@@ -815,8 +815,17 @@ __global__ __launch_bounds__(256) void k_smooth(NonlocalArgs a, const double *st
                         coef += fmax(smoothing_weight(a.kernel, r2 / (R * R)) * gi[5], 0.);
                     }
                     coef /= NP;
-                    for (int q = 0; q < S; q++) acc[q] += coef * strain[(int64_t)i * S + q];
-                    sum += coef;
+                    // ElementState::getAverageField returns measure * weight
+                    // (elements/integrable_entity.cpp:2026-2133), which
+                    // Mesh::getSmoothedFields Kahan-adds (mesher/mesh.h:1453-1462)
+                    const double v = gi[5] * coef;
+                    for (int q = 0; q < S; q++) {
+                        const double y = strain[(int64_t)i * S + q] * v - cor[q];
+                        const double t = acc[q] + y;
+                        cor[q] = (t - acc[q]) - y;
+                        acc[q] = t;
+                    }
+                    sum += v;
                 }
             }
     for (int q = 0; q < S; q++) out[(int64_t)e * S + q] = sum > 1e-12 ? acc[q] / sum : 0.;
