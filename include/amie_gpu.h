@@ -425,13 +425,62 @@ int amie_gpu_batch_damage_step(amie_gpu_batch *b, const amie_damage_params *prm,
  * default) or 1 GAUSSIAN_NONCOMPACT, elements with damage >= threshold count
  * as fractured and are left out (the reference default threshold is 1).
  * Once set, amie_gpu_batch_damage_step drives the damage law with the
- * smoothed strains.  The reference's event-driven damage iteration
- * (physics/damagemodels/damagemodel.cpp:23-320) is not restated. */
+ * smoothed strains; amie_gpu_batch_event_step runs the reference's
+ * event-driven iteration on them.  Each neighbour counts with its measure
+ * times its cache weight (ElementState::getAverageField returns both,
+ * elements/integrable_entity.cpp:2026-2133). */
 int amie_gpu_batch_set_nonlocal(amie_gpu_batch *b, double radius, int32_t kernel, double threshold);
 /* The smoothed strains of problem p for right-hand side rhs (nelem x 3|6). */
 int amie_gpu_batch_smoothed_strain(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain);
 int amie_gpu_batch_get_damage(amie_gpu_batch *b, int32_t p, double *d, double *kappa);
 int amie_gpu_batch_set_damage(amie_gpu_batch *b, int32_t p, const double *d);
+/* Event-driven damage: one FeatureTree::step (features/features.cpp:
+ * 6150-6373, stepElements :4875-5290) of elements with the behaviour
+ * StiffnessAndFracture(C, NonLocalMazars(threshold, E, nu, Gf, radius)) and
+ * its default damage model FiberBasedIsotropicLinearDamage (physics/
+ * stiffness_and_fracture.cpp:24-51, physics/damagemodels/
+ * fiberbasedisotropiclineardamage.cpp:14-100), at the current Dirichlet
+ * values: repeat { assemble with the current damage (C (1 - d), C * residual
+ * once fractured); solve; smooth the strains (amie_gpu_batch_set_nonlocal's
+ * restatement, fractured elements left out); score every element
+ * (NonLocalMazars::grade, physics/fracturecriteria/mazars.cpp:73-151); one
+ * pass in the reference's visiting order in which an unfractured element
+ * scoring above 0 and within score_tolerance of the largest score of its
+ * neighbourhood (FractureCriterion::getMaxScoreInNeighbourhood,
+ * fracturecriterion.cpp:251-280) takes d += fibre_fraction } until a pass
+ * changes nothing.  Linear elements (P1, Tet4), right-hand side 0. */
+typedef struct {
+    int32_t nphase;              /* materials the two arrays below cover (>= every problem's nphase) */
+    const int32_t *criterion;    /* per material: 1 = StiffnessAndFracture + NonLocalMazars, 0 = elastic */
+    const double *mazars;        /* per material: threshold, E, nu, Gf (NonLocalMazars constructor) */
+    double radius;               /* material characteristic radius of the criterion */
+    int32_t kernel;              /* smoothing kernel: 0 QUARTIC_COMPACT (default), 1 GAUSSIAN_NONCOMPACT */
+    double fibre_fraction;       /* damage per event (0.1, the model's default) */
+    double fractured_at;         /* thresholdDamageDensity (0.6) */
+    double residual;             /* residualStiffnessFraction (1e-4) */
+    double score_tolerance;      /* FractureCriterion::scoreTolerance (2e-3) */
+    int64_t first_deferred;      /* the pass visits element positions >= first_deferred, then the others:
+                                    65 is the order of the reference's OpenMP task pass on one thread
+                                    (libgomp runs the first 65 tasks of the region last) */
+    int64_t max_passes;          /* stop after this many solves (FeatureTree's own cap only counts
+                                    passes without a change) */
+} amie_event_params;
+
+typedef struct {
+    int64_t passes;              /* solves, each followed by one damage pass */
+    int64_t cg_iterations;       /* CG iterations over the passes (largest problem per pass) */
+    int64_t events;              /* damage increments */
+    int64_t fractured;           /* elements with damage >= fractured_at after the step */
+    int32_t status;              /* AMIE_OK, or the first failing solve's status */
+    int32_t converged;           /* 1: the last pass changed nothing; 0: max_passes reached */
+    double solve_ms, damage_ms;  /* wall time in the solves and in the rest of the loop */
+} amie_event_stats;
+
+void amie_gpu_default_event_params(amie_event_params *p);
+int amie_gpu_batch_event_step(amie_gpu_batch *b, const amie_event_params *prm, const amie_solve_opts *opts,
+                              amie_event_stats *st);
+/* The fracture scores of problem p's elements after the last pass. */
+int amie_gpu_batch_get_scores(amie_gpu_batch *b, int32_t p, double *score);
 /* Scale the Dirichlet values of every problem (incremental loading). */
 int amie_gpu_batch_scale_bc(amie_gpu_batch *b, double factor);
 /* New values for Dirichlet conditions of problem p, each DOF one that the

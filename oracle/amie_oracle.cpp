@@ -1170,16 +1170,24 @@ void or_principal(int dim, int doubled, const double *v, double *out)
     out[2] = -som - trmat / 3. + dif * std::sqrt(3.);
 }
 
-// NonLocalMazars::grade, tension branch (physics/fracturecriteria/
-// mazars.cpp:24-31,73-151, the constructor without compressive parameters:
-// tensionOnly), with FractureCriterion::step (fracturecriterion.cpp:534-560:
-// a fractured element scores -1).  Per element of a phase with a criterion
+// NonLocalMazars::grade (physics/fracturecriteria/mazars.cpp:24-31,
+// 73-151) of the criterion each element holds -- a getCopy of
+// NonLocalMazars(k0, E, nu, Gf, radius), made with the compressive
+// parameters -1 (:200-205), hence tensionOnly false -- with
+// FractureCriterion::step (fracturecriterion.cpp:534-560: a fractured
+// element scores -1).  Per element of a phase with a criterion
 // (crit[phase] != 0): the smoothed strain sbar (non-local, see
 // or_smoothed_strain) gives the stress sbar.(C (1 - d)) and both principal
-// sets; with e_zz = -nu (e1 + e2) / (1 - nu) in plane stress,
-//   kappa = gamma * max(0, sqrt(sum <e_i>+^2)),  gamma != 1 only when every
-//   principal stress is compressive,
-//   pseudo = min(1 - (k0 / max(k0, kappa)) exp(-B_t (max(k0, kappa) - k0)), 1),
+// sets;
+//   kappa = gamma * max(0, sqrt(sum <e_i>+^2)), with e_3 = -nu (e1 + e2) /
+//   (1 - nu) in plane stress; gamma != 1 only when every principal stress
+//   is compressive;
+//   plane stress: tension only (the 2D branch sets it, :94); 3D: tension
+//   share t = sum <s_i - nu (s_j + s_k)>+ .. / (E (1 - d)) <e_i>+ / kappa^2
+//   (0 below 1e-3), compression share 1 - t with A_c, B_c of cstress =
+//   cstrain = -1;
+//   pseudo = min(t (1 - (k0/K) exp(-B_t (K - k0))) + (1 - t) (1 - k0 (1 - A_c)/K
+//            - A_c exp(-B_c (K - k0))), 1),  K = max(k0, kappa),
 //   B_t = 0.1 E k0 / (Gf - 0.1 E k0^2 / 2),
 //   score = pseudo - d.
 // prm per phase: {k0, E, nu, Gf}.  Elements of a phase without a criterion
@@ -1213,19 +1221,32 @@ void or_mazars_score(int dim, int64_t nelem, const int32_t *phase, const int32_t
         }
         double gamma = 1.0;
         if (trc < -1e-12 && std::abs(trt / trc) < 1e-8) gamma = std::sqrt(sc2) / (-trc);
-        double kappa;
+        double kappa, talpha;
         auto pos = [](double x) { return 0.5 * (std::abs(x) + x); };
         if (dim == 2) {
             const double ezz = -nu * (pe[0] + pe[1]) / (1. - nu);
             kappa = gamma * std::max(0.0, std::sqrt(std::pow(pos(pe[0]), 2.0) + std::pow(pos(pe[1]), 2.0) +
                                                     std::pow(pos(ezz), 2.0)));
+            talpha = 1.0;
         } else {
+            const double ed = E * (1 - d);
+            const double q0 = (pos(ps[0]) - nu * (pos(ps[1]) + pos(ps[2]))) / ed;
+            const double q1 = (pos(ps[1]) - nu * (pos(ps[0]) + pos(ps[2]))) / ed;
+            const double q2 = (pos(ps[2]) - nu * (pos(ps[0]) + pos(ps[1]))) / ed;
             kappa = gamma * std::max(0., std::sqrt(std::pow(pos(pe[0]), 2.0) + std::pow(pos(pe[1]), 2.0) +
                                                    std::pow(pos(pe[2]), 2.0)));
+            talpha = (q0 * pos(pe[0]) + q1 * pos(pe[1]) + q2 * pos(pe[2])) / (kappa * kappa);
+            if (talpha < 1.0e-3) talpha = 0.;
         }
+        const double calpha = 1.0 - talpha;
+        const double cstress = -1., cstrain = -1.;
         const double Bt = (0.1 * E * k0) / (Gf - E * 0.1 * k0 * k0 * 0.5);
+        const double Bc = -1. / (std::sqrt(2.) * nu * cstrain);
+        const double Ac = -(E * k0 + std::sqrt(2.) * cstress * nu) / ((E * std::exp(Bc * k0 - 1.) / Bc) - E * k0);
         const double kt = std::max(k0, kappa);
-        const double pseudo = std::min(1. * (1. - (k0 / kt) * (std::exp(-Bt * (kt - k0)))), 1.0);
+        const double pseudo = std::min(talpha * (1. - (k0 / kt) * (std::exp(-Bt * (kt - k0)))) +
+                                           calpha * (1. - (k0 * (1 - Ac) / kt) - Ac * (std::exp(-Bc * (kt - k0)))),
+                                       1.0);
         score[e] = pseudo - d;
     }
 }

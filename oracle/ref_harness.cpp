@@ -702,9 +702,12 @@ struct LoggedFiberDamage : public FiberBasedIsotropicLinearDamage {
         if (calls) calls->push_back(pos);
         if (record && first && first->size() < first_calls * 6)
             first->insert(first->end(), {fc->getScoreAtState(), nbr, sf.second[0], sf.second[1], sf.first[0], sf.first[1]});
-        if (record && trace)
-            fprintf(trace, "%lld %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", (long long)pos, fc->getScoreAtState(),
-                    nbr, before, getState()[0], sf.first[0], sf.first[1], sf.second[0], sf.second[1]);
+        if (record && trace) {
+            fprintf(trace, "%lld %.17g %.17g %.17g %.17g", (long long)pos, fc->getScoreAtState(), nbr, before, getState()[0]);
+            for (size_t q = 0; q < sf.first.size(); q++) fprintf(trace, " %.17g", sf.first[q]);
+            for (size_t q = 0; q < sf.second.size(); q++) fprintf(trace, " %.17g", sf.second[q]);
+            fprintf(trace, "\n");
+        }
     }
     virtual DamageModel *getCopy() const
     {

@@ -54,6 +54,20 @@ class DamageParams(ctypes.Structure):
     _fields_ = [("kappa0", c_dbl), ("kappaf", c_dbl), ("dmax", c_dbl)]
 
 
+class EventParams(ctypes.Structure):
+    _fields_ = [("nphase", c_i32), ("criterion", vp), ("mazars", vp), ("radius", c_dbl), ("kernel", c_i32),
+                ("fibre_fraction", c_dbl), ("fractured_at", c_dbl), ("residual", c_dbl), ("score_tolerance", c_dbl),
+                ("first_deferred", c_i64), ("max_passes", c_i64)]
+
+
+class EventStats(ctypes.Structure):
+    _fields_ = [("passes", c_i64), ("cg_iterations", c_i64), ("events", c_i64), ("fractured", c_i64),
+                ("status", c_i32), ("converged", c_i32), ("solve_ms", c_dbl), ("damage_ms", c_dbl)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class RequestResult(ctypes.Structure):
     _fields_ = [("ticket", c_u64), ("status", c_i32), ("iterations", c_i32), ("relres", c_dbl),
                 ("submit_ms", c_dbl), ("start_ms", c_dbl), ("done_ms", c_dbl)]
@@ -112,6 +126,9 @@ SIGNATURES = {
     "amie_gpu_batch_element_strain": (c_i32, [vp, c_i32, c_i32, vp]),
     "amie_gpu_batch_average_stress": (c_i32, [vp, vp]),
     "amie_gpu_batch_damage_step": (c_i32, [vp, vp, vp]),
+    "amie_gpu_default_event_params": (None, [vp]),
+    "amie_gpu_batch_event_step": (c_i32, [vp, vp, vp, vp]),
+    "amie_gpu_batch_get_scores": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_get_damage": (c_i32, [vp, c_i32, vp, vp]),
     "amie_gpu_batch_set_damage": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_scale_bc": (c_i32, [vp, c_dbl]),
@@ -582,6 +599,37 @@ class Batch:
         g = c_i64()
         check(lib().amie_gpu_batch_damage_step(self.h, ctypes.byref(prm), ctypes.byref(g)))
         return g.value
+
+    def event_step(self, criterion, mazars, radius, kernel=0, **kw):
+        """One FeatureTree::step of the event-driven damage (NonLocalMazars
+        + FiberBasedIsotropicLinearDamage) at the current Dirichlet values:
+        criterion[m] (1/0) and mazars[m] = (threshold, E, nu, Gf) per
+        material; kw: fibre_fraction, fractured_at, residual,
+        score_tolerance, first_deferred, max_passes, and the solve options
+        (precision, rtol, ref_eps, ...).  Returns the step's statistics."""
+        prm = EventParams()
+        lib().amie_gpu_default_event_params(ctypes.byref(prm))
+        crit = np.ascontiguousarray(criterion, dtype=np.int32)
+        maz = np.ascontiguousarray(np.asarray(mazars, dtype=np.float64).reshape(-1, 4))
+        prm.nphase = len(crit)
+        prm.criterion = crit.ctypes.data
+        prm.mazars = maz.ctypes.data
+        prm.radius = radius
+        prm.kernel = kernel
+        for k in ("fibre_fraction", "fractured_at", "residual", "score_tolerance", "first_deferred", "max_passes"):
+            if k in kw:
+                setattr(prm, k, kw.pop(k))
+        o = default_opts(**kw)
+        st = EventStats()
+        rc = lib().amie_gpu_batch_event_step(self.h, ctypes.byref(prm), ctypes.byref(o), ctypes.byref(st))
+        check(rc, allow=(0, 1, -4))
+        return st.as_dict()
+
+    def scores(self, p=0):
+        """Fracture scores of problem p's elements after the last pass."""
+        out = np.zeros(self.problems[p].conn.shape[0])
+        check(lib().amie_gpu_batch_get_scores(self.h, p, _p(out)))
+        return out
 
     def damage(self, p=0):
         ne = self.problems[p].conn.shape[0]

@@ -40,7 +40,17 @@ struct AsmArgs {
     int64_t nslots;
     double *b64;              // right-hand-side planes (vplane entries each)
     int64_t vplane;
+    double frac_at = HUGE_VAL;   // damage at which an element counts as fractured (HUGE_VAL: never)
+    double residual = 0.;        // stiffness fraction a fractured element keeps
 };
+
+// Stiffness factor of element e: C (1 - d) (physics/damagemodels/
+// isotropiclineardamage.cpp:39-46), or C * residual once d >= frac_at
+// (fiberbasedisotropiclineardamage.cpp:37-46).
+AMIE_HD inline double stiffness_factor(const double *ed, int64_t e, double frac_at, double residual)
+{
+    return ed ? (ed[e] >= frac_at ? residual : 1. - ed[e]) : 1.;
+}
 
 template <int ET> struct Elem;
 template <> struct Elem<1> { static constexpr int NPER = 3, DIM = 2, B = 2, S = 3, NGP = 1; };
@@ -295,7 +305,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a, const int32_t *elem
     }
     double C[S * S];
     const double *Cp = a.cmat + (int64_t)(a.cmat_off[p] + a.ephase[e]) * S * S;
-    const double f = a.edamage ? 1. - a.edamage[e] : 1.;
+    const double f = stiffness_factor(a.edamage, e, a.frac_at, a.residual);
     for (int k = 0; k < S * S; k++) C[k] = a.edamage ? Cp[k] * f : Cp[k];
     double G[E::NGP][6][3], w[E::NGP];
     element_grads<ET>(X, G, w);
@@ -370,7 +380,7 @@ template <int ET> __device__ inline void load_element(const AsmArgs &a, int e, d
         if (D == 2) X[i][2] = 0.;
     }
     const double *Cp = a.cmat + (int64_t)(a.cmat_off[p] + a.ephase[e]) * S * S;
-    const double f = a.edamage ? 1. - a.edamage[e] : 1.;
+    const double f = stiffness_factor(a.edamage, e, a.frac_at, a.residual);
     for (int k = 0; k < S * S; k++) C[k] = a.edamage ? Cp[k] * f : Cp[k];
 }
 
@@ -739,13 +749,14 @@ __global__ __launch_bounds__(256) void k_damage(StrainArgs a, int n, const doubl
 // (physics/fracturecriteria/fracturecriterion.cpp:171-190): element e
 // averages the strains of the same-phase, unfractured elements whose
 // centroid lies within rho_e = 2.5 R_e + 1.5 r_e of its centroid (R_e =
-// max(radius, 1.1 r_e), r_e its circumradius), each weighted by the mean over
-// its vertices of max(w(|v - c_e|^2 / R_e^2) * measure, 0) with the
-// smoothing kernel w (fracturecriterion.cpp:43-61).  The candidates come
+// max(radius, 1.1 r_e), r_e its circumradius), each counted with its measure
+// times its cache weight, the mean over its vertices of
+// max(w(|v - c_e|^2 / R_e^2) * measure, 0) with the smoothing kernel w
+// (fracturecriterion.cpp:43-61).  The candidates come
 // from a uniform grid of element centroids per problem (cell = the largest
 // rho of the problem).
 struct NonlocalArgs {
-    const double *geo;         // per element: cx, cy, cz, R, rho, measure, -, -
+    const double *geo;         // per element: cx, cy, cz, R, rho, measure, r, rho2
     const int32_t *econn;
     const double *coords;
     const int32_t *ephase;
@@ -831,12 +842,226 @@ __global__ __launch_bounds__(256) void k_smooth(NonlocalArgs a, const double *st
     for (int q = 0; q < S; q++) out[(int64_t)e * S + q] = sum > 1e-12 ? acc[q] / sum : 0.;
 }
 
+// ------------------------------------------------- event-driven damage
+// FeatureTree's damage iteration for StiffnessAndFracture + NonLocalMazars +
+// FiberBasedIsotropicLinearDamage (features/features.cpp:4875-5290,
+// 6150-6373): after every solve each element of a criterion phase is scored
+// on its smoothed strain, then one pass visits the elements and increments
+// the damage of those at their neighbourhood's maximum score.
+struct EventArgs {
+    const int32_t *crit;      // per material: 1 = the criterion and damage model above
+    const double *prm;        // per material: k0, E, nu, Gf
+    const int32_t *cmat_off;  // per problem: first C matrix
+    const double *cmat;
+    const int32_t *prob_e0, *prob_ne;   // per problem: first element, elements
+    double frac_at;           // fractured at d >= frac_at (0.6)
+    double fraction;          // damage added per event (0.1)
+    double tol;               // FractureCriterion::scoreTolerance (2e-3)
+    int64_t first_deferred;   // the pass visits positions >= first_deferred first
+};
+
+// toPrincipal (elements/integrable_entity.cpp:475-560): doubled = 1 for an
+// engineering strain (shear entries 2 e_ij).
+template <int D> __device__ inline void principal_values(const double *v, bool doubled, double *out)
+{
+    AMIE_NO_CONTRACT
+    if constexpr (D == 2) {
+        const double tr = v[0] + v[1];
+        const double det = doubled ? v[0] * v[1] - 0.25 * v[2] * v[2] : v[0] * v[1] - v[2] * v[2];
+        const double delta = sqrt(tr * tr - 4. * det);
+        const double angle = doubled ? 0.5 * atan2(0.5 * v[2], v[0] - v[1]) : 0.5 * atan2(v[2], v[0] - v[1]);
+        if (cos(angle) < 0) {
+            out[0] = (tr + delta) * .5;
+            out[1] = (tr - delta) * .5;
+        } else {
+            out[0] = (tr - delta) * .5;
+            out[1] = (tr + delta) * .5;
+        }
+    } else {
+        const double h = doubled ? 0.5 : 1.;
+        const double m00 = v[0], m11 = v[1], m22 = v[2], m12 = v[3] * h, m02 = v[4] * h, m01 = v[5] * h;
+        const double trmat = -(m00 + m11 + m22);
+        const double detmat = -2.0 * m01 * m02 * m12 + m00 * m12 * m12 + m11 * m02 * m02 + m22 * m01 * m01 - m00 * m11 * m22;
+        const double m2mat = (m00 * m11 + m11 * m22 + m22 * m00) - m02 * m02 - m01 * m01 - m12 * m12;
+        const double q = m2mat / 3. - trmat * trmat / 9.;
+        const double r = (trmat * m2mat - 3. * detmat) / 6. - trmat * trmat * trmat / 27.;
+        const double d = q * q * q + r * r;
+        const double r0 = pow(r * r - d, 1. / 6.);
+        double phi = atan2(sqrt(-1. * d), r) / 3.;
+        if (fabs(phi) < 1e-12) phi = 0.;
+        if (phi < 0.) phi += M_PI;
+        const double som = r0 * cos(phi), dif = r0 * sin(phi);
+        out[0] = 2. * som - trmat / 3.;
+        out[1] = -som - trmat / 3. - dif * sqrt(3.);
+        out[2] = -som - trmat / 3. + dif * sqrt(3.);
+    }
+}
+
+__device__ inline double pos_part(double x) { return 0.5 * (fabs(x) + x); }
+
+// NonLocalMazars::grade (physics/fracturecriteria/mazars.cpp:73-151) of the
+// criterion copy every element holds (compressive parameters -1, so in 3D
+// not tension-only; the plane-stress branch is), FractureCriterion::step
+// (fracturecriterion.cpp:534-560): score = pseudo-damage - d, -1 when
+// fractured.  sbar: the smoothed strains; elements of a phase without a
+// criterion get 0.
+template <int ET>
+__global__ __launch_bounds__(256) void k_mazars_score(EventArgs ev, const int32_t *ephase, const int32_t *eprob,
+                                                      const double *edamage, const double *sbar, int n, double *score)
+{
+    AMIE_NO_CONTRACT
+    using E = Elem<ET>;
+    constexpr int S = E::S, D = E::DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int ph = ephase[e];
+    if (!ev.crit[ph]) { score[e] = 0.; return; }
+    const double d = edamage[e];
+    if (d >= ev.frac_at) { score[e] = -1.; return; }
+    const double k0 = fabs(ev.prm[4 * ph]), Ey = ev.prm[4 * ph + 1], nu = ev.prm[4 * ph + 2], Gf = ev.prm[4 * ph + 3];
+    double eps[S];
+    for (int q = 0; q < S; q++) eps[q] = sbar[(int64_t)e * S + q];
+    const double *Cp = ev.cmat + (int64_t)(ev.cmat_off[eprob[e]] + ph) * S * S;
+    double sig[S], ps[3] = {0., 0., 0.}, pe[3] = {0., 0., 0.};
+    for (int j = 0; j < S; j++) {
+        double t = 0.;
+        for (int i = 0; i < S; i++) t += eps[i] * (Cp[i * S + j] * (1. - d));
+        sig[j] = t;
+    }
+    principal_values<D>(sig, false, ps);
+    principal_values<D>(eps, true, pe);
+    double trt = 0., trc = 0., sc2 = 0.;
+    for (int i = 0; i < D; i++) {
+        trt += 0.5 * (fabs(ps[i]) + ps[i]);
+        trc += fmin(0.0, ps[i]);
+        sc2 += fmin(0.0, ps[i]) * fmin(0.0, ps[i]);
+    }
+    double gamma = 1.0;
+    if (trc < -1e-12 && fabs(trt / trc) < 1e-8) gamma = sqrt(sc2) / (-trc);
+    double kappa, talpha;
+    if constexpr (D == 2) {
+        const double ezz = -nu * (pe[0] + pe[1]) / (1. - nu);
+        const double a0 = pos_part(pe[0]), a1 = pos_part(pe[1]), a2 = pos_part(ezz);
+        const double m = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+        kappa = gamma * (0.0 < m ? m : 0.0);
+        talpha = 1.0;
+    } else {
+        const double ed = Ey * (1 - d);
+        const double q0 = (pos_part(ps[0]) - nu * (pos_part(ps[1]) + pos_part(ps[2]))) / ed;
+        const double q1 = (pos_part(ps[1]) - nu * (pos_part(ps[0]) + pos_part(ps[2]))) / ed;
+        const double q2 = (pos_part(ps[2]) - nu * (pos_part(ps[0]) + pos_part(ps[1]))) / ed;
+        const double a0 = pos_part(pe[0]), a1 = pos_part(pe[1]), a2 = pos_part(pe[2]);
+        const double m = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+        kappa = gamma * (0. < m ? m : 0.);
+        talpha = (q0 * a0 + q1 * a1 + q2 * a2) / (kappa * kappa);
+        if (talpha < 1.0e-3) talpha = 0.;
+    }
+    const double calpha = 1.0 - talpha;
+    const double cstress = -1., cstrain = -1.;
+    const double Bt = (0.1 * Ey * k0) / (Gf - Ey * 0.1 * k0 * k0 * 0.5);
+    const double Bc = -1. / (sqrt(2.) * nu * cstrain);
+    const double Ac = -(Ey * k0 + sqrt(2.) * cstress * nu) / ((Ey * exp(Bc * k0 - 1.) / Bc) - Ey * k0);
+    const double kt = k0 < kappa ? kappa : k0;
+    const double ps_ = talpha * (1. - (k0 / kt) * (exp(-Bt * (kt - k0)))) +
+                       calpha * (1. - (k0 * (1 - Ac) / kt) - Ac * (exp(-Bc * (kt - k0))));
+    const double pseudo = 1.0 < ps_ ? 1.0 : ps_;
+    score[e] = pseudo - d;
+}
+
+// position of element e in the order the reference's pass visits its
+// problem's elements: positions >= fd first, then 0 .. fd-1
+__device__ inline int64_t visit_rank(int64_t local, int64_t ne, int64_t fd)
+{
+    const int64_t f = fd < 0 ? 0 : (fd > ne ? ne : fd);
+    return local >= f ? local - f : local + (ne - f);
+}
+
+// One round of the pass's decisions (FiberBasedIsotropicLinearDamage::step,
+// physics/damagemodels/fiberbasedisotropiclineardamage.cpp:58-92, with
+// FractureCriterion::getMaxScoreInNeighbourhood, fracturecriterion.cpp:
+// 251-280): element e takes an event when it is in a criterion phase, not
+// fractured, scores above 0 and lies within tol of the largest score among
+// itself and the same-phase neighbours within rho2_e (the second cache of
+// FractureCriterion::initialiseCache, :171-190) that are not fractured when
+// the pass reaches e: fractured before the pass (dprev >= frac_at), or made
+// fractured by this pass at an element visited earlier (newfrac, the
+// previous round's guess).  Rounds repeat until newfrac stops growing; the
+// decisions then equal those of the reference's serial pass.
+template <int ET>
+__global__ __launch_bounds__(256) void k_fiber_decide(NonlocalArgs a, EventArgs ev, const double *score,
+                                                      const double *dprev, const uint8_t *newfrac, int n, uint8_t *inc)
+{
+    using E = Elem<ET>;
+    constexpr int D = E::DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int ph = a.ephase[e];
+    const double s = score[e];
+    if (!ev.crit[ph] || !(s > 0.) || dprev[e] >= ev.frac_at) { inc[e] = 0; return; }
+    const int p = a.eprob[e];
+    const int64_t e0 = ev.prob_e0[p], ne = ev.prob_ne[p];
+    const int64_t re = visit_rank(e - e0, ne, ev.first_deferred);
+    const double *ge = a.geo + (int64_t)e * GEO_STRIDE;
+    const double c[3] = {ge[0], ge[1], ge[2]};
+    const double rho = ge[7];
+    const double *g = a.grid + (int64_t)p * 4;
+    const int *gd = a.gdim + (int64_t)p * 4;
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+        lo[d] = max(0, (int)floor((c[d] - rho - g[d]) / g[3]));
+        hi[d] = min(gd[d] - 1, (int)floor((c[d] + rho - g[d]) / g[3]));
+    }
+    double mx = s;
+    for (int z = lo[2]; z <= hi[2]; z++)
+        for (int y = lo[1]; y <= hi[1]; y++)
+            for (int x = lo[0]; x <= hi[0]; x++) {
+                const int cell = gd[3] + (z * gd[1] + y) * gd[0] + x;
+                for (int j = a.cell_start[cell]; j < a.cell_start[cell + 1]; j++) {
+                    const int i = a.cell_elem[j];
+                    if (a.ephase[i] != ph) continue;
+                    const double *gi = a.geo + (int64_t)i * GEO_STRIDE;
+                    double d2 = 0.;
+                    for (int d = 0; d < D; d++) d2 += (gi[d] - c[d]) * (gi[d] - c[d]);
+                    const bool in = D == 2 ? d2 < rho * rho : d2 < rho * rho + 2. * rho * 1e-12 + 1e-24;
+                    if (!in || dprev[i] >= ev.frac_at) continue;
+                    if (newfrac[i] && visit_rank(i - e0, ne, ev.first_deferred) < re) continue;
+                    const double si = score[i];
+                    mx = mx < si ? si : mx;   // std::max(mx, si)
+                }
+            }
+    inc[e] = fabs(s - mx) < ev.tol ? 1 : 0;
+}
+
+// The elements an event fractures (d + fraction, capped at 1, reaching
+// frac_at); count = their number.
+__global__ __launch_bounds__(256) void k_fiber_mark(EventArgs ev, const double *dprev, const uint8_t *inc, int n,
+                                                    uint8_t *newfrac, unsigned long long *count)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const bool f = inc[e] && fmin(dprev[e] + ev.fraction, 1.) >= ev.frac_at;
+    newfrac[e] = f ? 1 : 0;
+    if (f) atomicAdd(count, 1ull);
+}
+
+// The pass's events: d += fraction, capped at 1; count = their number.
+__global__ __launch_bounds__(256) void k_fiber_commit(EventArgs ev, const uint8_t *inc, int n, double *d,
+                                                      unsigned long long *count)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    if (!inc[e]) return;
+    d[e] = fmin(d[e] + ev.fraction, 1.);
+    atomicAdd(count, 1ull);
+}
+
 // Volume average of C_e (1 - d_e) eps_e per problem and right-hand side:
 // one workgroup per problem, fixed summation order.
 template <int ET>
 __global__ __launch_bounds__(256) void k_average_stress(StrainArgs a, const int32_t *prob_e0, const int32_t *prob_ne,
                                                         const int32_t *ephase, const double *edamage,
-                                                        const int32_t *cmat_off, const double *cmat, double *out)
+                                                        const int32_t *cmat_off, const double *cmat, double *out,
+                                                        double frac_at, double residual)
 {
     using E = Elem<ET>;
     constexpr int S = E::S, NP = E::NPER, D = E::DIM;
@@ -865,7 +1090,7 @@ __global__ __launch_bounds__(256) void k_average_stress(StrainArgs a, const int3
                 meas = fabs(u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) + u[2] * (v[0] * w[1] - v[1] * w[0])) / 6.;
             }
             const double *C = cmat + (int64_t)(cmat_off[p] + ephase[e]) * S * S;
-            const double f = edamage ? 1. - edamage[e] : 1.;
+            const double f = stiffness_factor(edamage, e, frac_at, residual);
             for (int r = 0; r < S; r++) {
                 double t = 0.;
                 for (int c = 0; c < S; c++) t += C[r * S + c] * eps[c];

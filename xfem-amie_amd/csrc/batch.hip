@@ -217,6 +217,12 @@ static int validate(const amie_problem_desc &d, const EType &et)
     return AMIE_OK;
 }
 
+// buf holds at least bytes (kept when it already does)
+static bool ensure(DevBuf &buf, size_t bytes)
+{
+    return (buf.p && buf.n >= bytes) || buf.alloc(bytes);
+}
+
 static int alloc_and_upload(DevBuf &buf, const void *src, size_t bytes)
 {
     if (!buf.alloc(bytes)) return AMIE_ERR_NOMEM;
@@ -719,6 +725,8 @@ int Batch::assemble()
         a.nslots = nslots;
         a.b64 = b64.as<double>();
         a.vplane = vplane();
+        a.frac_at = frac_at;
+        a.residual = frac_residual;
         if (gather) {
             if (et.et == 1) { if (nrhs == 1) launch_assemble_gather<1, 1>(*this, a); else launch_assemble_gather<1, 3>(*this, a); }
             else if (et.et == 2) { if (nrhs == 1) launch_assemble_gather<2, 1>(*this, a); else launch_assemble_gather<2, 3>(*this, a); }
@@ -1832,7 +1840,7 @@ int amie_gpu_batch_average_stress(amie_gpu_batch *bb, double *outp)
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(B.nprob), dim3(256), 0, st, s, B.prob_e0.as<int32_t>(), B.prob_ne.as<int32_t>(),
                            B.ephase.as<int32_t>(), B.edamage.as<double>(), B.cmat_off.as<int32_t>(), B.cmat.as<double>(),
-                           out.as<double>());
+                           out.as<double>(), B.frac_at, B.frac_residual);
     };
     if (B.et.et == 1) go(k_average_stress<1>);
     else if (B.et.et == 2) go(k_average_stress<2>);
@@ -1866,7 +1874,7 @@ static NonlocalArgs nonlocal_args(Batch &B)
 static int smoothed_all(Batch &B, int rhs, DevBuf &tmp, DevBuf &out)
 {
     const int n = (int)B.nelem, S = B.et.s;
-    if (!tmp.alloc((size_t)n * S * 8) || !out.alloc((size_t)n * S * 8)) return AMIE_ERR_NOMEM;
+    if (!ensure(tmp, (size_t)n * S * 8) || !ensure(out, (size_t)n * S * 8)) return AMIE_ERR_NOMEM;
     StrainArgs s = strain_args(B);
     NonlocalArgs a = nonlocal_args(B);
     dim3 gr((n + 255) / 256), bl(256);
@@ -1962,10 +1970,21 @@ static void nonlocal_geometry(int dim, int nper, const double *X /*nper x dim*/,
     }
 }
 
+static int set_nonlocal(Batch &B, double radius, int32_t kernel, double threshold);
+
 int amie_gpu_batch_set_nonlocal(amie_gpu_batch *bb, double radius, int32_t kernel, double threshold)
 {
     if (!bb || !bb->b || (kernel != 0 && kernel != 1)) { set_error("bad arguments"); return AMIE_ERR_ARG; }
-    Batch &B = *bb->b;
+    return set_nonlocal(*bb->b, radius, kernel, threshold);
+}
+
+// The non-local search structure: per element its centroid, R, the
+// smoothing radius rho, its measure, its circumradius r and the radius rho2
+// of the neighbourhood whose largest score the event-driven damage compares
+// with (max(6 R, 4 r) 2.5 + r, FractureCriterion::initialiseCache,
+// fracturecriterion.cpp:183-186); per problem a grid of the centroids.
+static int set_nonlocal(Batch &B, double radius, int32_t kernel, double threshold)
+{
     if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
     if (!(radius > 0.)) {
         B.nonlocal_on = false;
@@ -1991,6 +2010,8 @@ int amie_gpu_batch_set_nonlocal(amie_gpu_batch *bb, double radius, int32_t kerne
         g[3] = std::max(radius, r * 1.1);   // R_e
         g[4] = g[3] * 2.5 + r * 1.5;        // rho_e
         g[5] = meas;
+        g[6] = r;
+        g[7] = std::max(g[3] * 6., r * 4.) * 2.5 + r;   // rho2_e
     }
     // per problem: a grid of cells of the largest rho over the centroids
     std::vector<double> grid((size_t)B.nprob * 4);
@@ -2129,6 +2150,42 @@ int amie_gpu_batch_set_bc_values(amie_gpu_batch *bb, int32_t p, int64_t nbc, con
     return AMIE_OK;
 }
 
+void amie_gpu_default_event_params(amie_event_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->radius = 0.008;            // FractureCriterion's default characteristic radius
+    p->kernel = 0;
+    p->fibre_fraction = 0.1;      // FiberBasedIsotropicLinearDamage(0.1, 0.6)
+    p->fractured_at = 0.6;
+    p->residual = 1e-4;
+    p->score_tolerance = 2e-3;
+    p->first_deferred = 65;
+    p->max_passes = 100000;
+}
+
+int amie_gpu_batch_event_step(amie_gpu_batch *bb, const amie_event_params *prm, const amie_solve_opts *opts,
+                              amie_event_stats *st)
+{
+    if (!bb || !bb->b || !prm) { set_error("bad arguments"); return AMIE_ERR_ARG; }
+    amie_solve_opts o;
+    if (opts) o = *opts;
+    else amie_gpu_default_opts(&o);
+    amie_event_stats tmp{};
+    return bb->b->event_step(*prm, o, st ? st : &tmp);
+}
+
+int amie_gpu_batch_get_scores(amie_gpu_batch *bb, int32_t p, double *score)
+{
+    if (!bb || check_p(bb->b, p) || !score) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    if (!B.ev_score.p) { set_error("no event step has run"); return AMIE_ERR_STATE; }
+    const ProbHost &P = B.ph[p];
+    AMIE_HIP(hipStreamSynchronize(B.stream));
+    AMIE_HIP(hipMemcpy(score, B.ev_score.as<double>() + P.elem0, P.nelem * 8, hipMemcpyDeviceToHost));
+    return AMIE_OK;
+}
+
 int amie_gpu_batch_scale_bc(amie_gpu_batch *bb, double factor)
 {
     if (!bb || !bb->b) return AMIE_ERR_ARG;
@@ -2141,3 +2198,130 @@ int amie_gpu_batch_scale_bc(amie_gpu_batch *bb, double factor)
 }
 
 } // extern "C"
+
+namespace amie {
+
+// One FeatureTree::step of the event-driven damage (the C ABI comment of
+// amie_gpu_batch_event_step says what it restates).
+int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, amie_event_stats *st)
+{
+    Batch &B = *this;
+    *st = amie_event_stats{};
+    if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
+    if (B.et.et == 2) { set_error("event-driven damage: linear elements (P1, Tet4) only"); return AMIE_ERR_ARG; }
+    if (prm.nphase <= 0 || !prm.criterion || !prm.mazars || !(prm.radius > 0.) || (prm.kernel != 0 && prm.kernel != 1) ||
+        !(prm.fibre_fraction > 0.) || !(prm.fractured_at > 0.) || !(prm.score_tolerance > 0.) || prm.max_passes <= 0) {
+        set_error("bad event-damage parameters");
+        return AMIE_ERR_ARG;
+    }
+    const int S = B.et.s;
+    const int64_t ncmat = (int64_t)(B.cmat.n / ((size_t)S * S * 8));
+    for (int p = 0; p < B.nprob; p++) {
+        const int64_t np = (p + 1 < B.nprob ? B.ph[p + 1].cmat0 : ncmat) - B.ph[p].cmat0;
+        if (np > prm.nphase) {
+            set_error("problem %d has %lld materials, the event parameters cover %d", p, (long long)np, prm.nphase);
+            return AMIE_ERR_ARG;
+        }
+    }
+    for (int k = 0; k < prm.nphase; k++)
+        if (prm.criterion[k] && !(prm.mazars[4 * k + 1] > 0. && prm.mazars[4 * k + 2] < 1.)) {
+            set_error("material %d: bad NonLocalMazars parameters", k);
+            return AMIE_ERR_ARG;
+        }
+    if (B.nelem >= INT32_MAX) { set_error("too many elements"); return AMIE_ERR_ARG; }
+    if (!B.nonlocal_on || B.nl_radius != prm.radius || B.nl_kernel != prm.kernel)
+        if (int rc = set_nonlocal(B, prm.radius, prm.kernel, prm.fractured_at)) return rc;
+    B.nl_threshold = prm.fractured_at;
+    const int n = (int)B.nelem;
+    int rc;
+    if ((rc = alloc_and_upload(B.ev_crit, prm.criterion, (size_t)prm.nphase * 4))) return rc;
+    if ((rc = alloc_and_upload(B.ev_prm, prm.mazars, (size_t)prm.nphase * 4 * 8))) return rc;
+    if (!ensure(B.ev_score, (size_t)n * 8) || !ensure(B.ev_dprev, (size_t)n * 8) || !ensure(B.ev_flags, (size_t)n * 2) ||
+        !ensure(B.ev_count, 16) || !ensure(B.ev_tmp, (size_t)n * S * 8) || !ensure(B.ev_sbar, (size_t)n * S * 8))
+        return AMIE_ERR_NOMEM;
+    B.frac_at = prm.fractured_at;
+    B.frac_residual = prm.residual;
+
+    EventArgs ev{};
+    ev.crit = B.ev_crit.as<int32_t>();
+    ev.prm = B.ev_prm.as<double>();
+    ev.cmat_off = B.cmat_off.as<int32_t>();
+    ev.cmat = B.cmat.as<double>();
+    ev.prob_e0 = B.prob_e0.as<int32_t>();
+    ev.prob_ne = B.prob_ne.as<int32_t>();
+    ev.frac_at = prm.fractured_at;
+    ev.fraction = prm.fibre_fraction;
+    ev.tol = prm.score_tolerance;
+    ev.first_deferred = prm.first_deferred;
+    NonlocalArgs na = nonlocal_args(B);
+    uint8_t *inc = B.ev_flags.as<uint8_t>(), *newfrac = inc + n;
+    unsigned long long *cnt = B.ev_count.as<unsigned long long>();
+    const dim3 gr((n + 255) / 256), bl(256);
+    hipStream_t sm = B.stream;
+    amie_solve_opts o = o_;
+    o.use_x0 = 1;
+    using clk = std::chrono::steady_clock;
+    for (;;) {
+        const auto t0 = clk::now();
+        if ((rc = B.assemble())) return rc;
+        amie_solve_stats ss{};
+        rc = B.solve(o, &ss);
+        if (rc < 0) return rc;
+        st->cg_iterations += ss.iterations_max;
+        const auto t1 = clk::now();
+        st->solve_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (rc != AMIE_OK) {   // a solve that did not converge ends the step
+            st->status = rc;
+            st->passes++;
+            return rc;
+        }
+        if ((rc = smoothed_all(B, 0, B.ev_tmp, B.ev_sbar))) return rc;
+        if (B.et.et == 1)
+            hipLaunchKernelGGL(k_mazars_score<1>, gr, bl, 0, sm, ev, B.ephase.as<int32_t>(), B.eprob.as<int32_t>(),
+                               B.edamage.as<double>(), B.ev_sbar.as<double>(), n, B.ev_score.as<double>());
+        else
+            hipLaunchKernelGGL(k_mazars_score<3>, gr, bl, 0, sm, ev, B.ephase.as<int32_t>(), B.eprob.as<int32_t>(),
+                               B.edamage.as<double>(), B.ev_sbar.as<double>(), n, B.ev_score.as<double>());
+        AMIE_HIP(hipGetLastError());
+        AMIE_HIP(hipMemcpyAsync(B.ev_dprev.p, B.edamage.p, (size_t)n * 8, hipMemcpyDeviceToDevice, sm));
+        AMIE_HIP(hipMemsetAsync(newfrac, 0, (size_t)n, sm));
+        // rounds until the set of elements fractured by this pass stops
+        // growing; every round grows it or ends the loop, so at most n + 1
+        unsigned long long nf_prev = 0;
+        for (int round = 0; round <= n; round++) {
+            if (B.et.et == 1)
+                hipLaunchKernelGGL(k_fiber_decide<1>, gr, bl, 0, sm, na, ev, B.ev_score.as<double>(),
+                                   B.ev_dprev.as<double>(), newfrac, n, inc);
+            else
+                hipLaunchKernelGGL(k_fiber_decide<3>, gr, bl, 0, sm, na, ev, B.ev_score.as<double>(),
+                                   B.ev_dprev.as<double>(), newfrac, n, inc);
+            AMIE_HIP(hipMemsetAsync(cnt, 0, 8, sm));
+            hipLaunchKernelGGL(k_fiber_mark, gr, bl, 0, sm, ev, B.ev_dprev.as<double>(), inc, n, newfrac, cnt);
+            AMIE_HIP(hipGetLastError());
+            unsigned long long nf = 0;
+            AMIE_HIP(hipMemcpyAsync(&nf, cnt, 8, hipMemcpyDeviceToHost, sm));
+            AMIE_HIP(hipStreamSynchronize(sm));
+            if (nf == nf_prev) break;   // the decisions were made with this very set
+            nf_prev = nf;
+        }
+        AMIE_HIP(hipMemsetAsync(cnt + 1, 0, 8, sm));
+        hipLaunchKernelGGL(k_fiber_commit, gr, bl, 0, sm, ev, inc, n, B.edamage.as<double>(), cnt + 1);
+        AMIE_HIP(hipGetLastError());
+        unsigned long long events = 0;
+        AMIE_HIP(hipMemcpyAsync(&events, cnt + 1, 8, hipMemcpyDeviceToHost, sm));
+        AMIE_HIP(hipStreamSynchronize(sm));
+        st->damage_ms += std::chrono::duration<double, std::milli>(clk::now() - t1).count();
+        st->passes++;
+        st->events += (int64_t)events;
+        if (events == 0) { st->converged = 1; break; }
+        if (st->passes >= prm.max_passes) break;
+    }
+    // elements fractured after the step
+    std::vector<double> d((size_t)n);
+    AMIE_HIP(hipMemcpy(d.data(), B.edamage.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (double v : d) st->fractured += v >= prm.fractured_at;
+    B.has_damage_state = true;
+    return AMIE_OK;
+}
+
+} // namespace amie

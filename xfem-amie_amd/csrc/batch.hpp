@@ -158,6 +158,10 @@ struct Batch {
     bool nonlocal_on = false;
     int nl_kernel = 0;
     double nl_radius = 0., nl_threshold = 1.;
+    // event-driven damage (amie_gpu_batch_event_step): an element with
+    // damage >= frac_at keeps frac_residual of its stiffness
+    double frac_at = HUGE_VAL, frac_residual = 0.;
+    DevBuf ev_tmp, ev_sbar, ev_score, ev_dprev, ev_flags, ev_crit, ev_prm, ev_count;
     std::vector<int32_t> h_active, h_active_prob;
     std::vector<ProbDev> h_prob;
     ProbDev *h_prob_pinned = nullptr;
@@ -187,6 +191,7 @@ struct Batch {
     int set_active(const std::vector<char> &probs);
     int autotune(int mode, bool bj, int iters);
     int damage_update(const amie_damage_params &prm, int64_t *n_grown);
+    int event_step(const amie_event_params &prm, const amie_solve_opts &o, amie_event_stats *st);
     void apply_tuned(int mode, bool bj, Kern &K, SolveArgs &a) const;
     int k1_format(int mode, bool bj) const;
     int64_t vplane() const { return nrows_pad * et.b; }
