@@ -515,6 +515,62 @@ def single_problem_legs(ag, configs, ctx, rtol, maxit=0, no_cpu=False, ref_iters
     return out
 
 
+def event_leg(ag, configs, ctx, n, n_cpu, no_cpu, loads=(0.8e-4, 1.2e-4)):
+    """Event-driven damage (FeatureTree's solve / score / damage-pass loop
+    with NonLocalMazars and fibre-based damage) on the event square: the
+    device at n x n, and at n_cpu x n_cpu beside the oracle's loop on the
+    same case (the oracle's damage must come out equal)."""
+    import oracle as orc
+
+    def device(m):
+        prob, dofs, nfix, crit, maz, radius = configs.event_square(m)
+        B = ag.Batch(ctx, [prob])
+        rows, t0 = [], time.perf_counter()
+        for v in loads:
+            vals = np.zeros(len(dofs))
+            vals[nfix:] = v
+            B.set_bc_values(dofs, vals)
+            st = B.event_step(crit, maz, radius, precision=ag.MIXED_FIX48, rtol=1e-12)
+            rows.append(st)
+        wall = time.perf_counter() - t0
+        d = B.damage()[0]
+        B.close()
+        passes = sum(r["passes"] for r in rows)
+        return {"elements": int(prob.conn.shape[0]), "radius": radius, "passes": passes,
+                "events": sum(r["events"] for r in rows), "fractured": int(rows[-1]["fractured"]),
+                "cg_iterations": sum(r["cg_iterations"] for r in rows), "wall_s": wall,
+                "ms_per_pass": 1e3 * wall / max(1, passes),
+                "solve_ms": sum(r["solve_ms"] for r in rows), "damage_ms": sum(r["damage_ms"] for r in rows),
+                "all_converged": all(r["converged"] == 1 for r in rows)}, d
+
+    big, _ = device(n)
+    log(f"event damage {n}^2: {big}")
+    out = {"workload": f"P1 unit square n x n, elastic 40 GPa disc r = 0.18 at the centre in a 10 GPa matrix "
+                       f"(StiffnessAndFracture + NonLocalMazars(1e-4, Gf 100), radius 3/n, fibre-based damage), "
+                       f"u_x = {loads} on the right edge, one FeatureTree step each; FIX48 solves to rel. res 1e-12",
+           f"n{n}": big}
+    small, d_dev = device(n_cpu)
+    out[f"n{n_cpu}"] = small
+    if not no_cpu:
+        prob, dofs, nfix, crit, maz, radius = configs.event_square(n_cpu)
+        d = np.zeros(prob.conn.shape[0])
+        passes, t0 = 0, time.perf_counter()
+        for v in loads:
+            vals = np.zeros(len(dofs))
+            vals[nfix:] = v
+            d, _, _, pp = orc.event_damage(orc.P1, prob.conn, prob.coords, prob.cmats, prob.phase, crit, maz, dofs,
+                                           vals, d, radius)
+            passes += pp
+        wall = time.perf_counter() - t0
+        out["cpu_oracle"] = {"n": n_cpu, "passes": passes, "wall_s": wall, "ms_per_pass": 1e3 * wall / max(1, passes),
+                             "cores": orc.get_threads(), "kind": "port",
+                             "damage_equal_to_device": bool(np.abs(d - d_dev).max() <= 1e-12),
+                             "note": "the oracle's loop: OpenMP PCG solves, brute-force neighbourhoods"}
+        out["device_vs_oracle_same_case"] = wall / max(1e-9, small["wall_s"])
+        log(f"event damage, oracle {n_cpu}^2: {out['cpu_oracle']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -535,6 +591,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the P1 128^2, P2 400^2 and Tet4 64^3 legs")
     ap.add_argument("--ingest-n", type=int, default=2048, help="mesh-input leg: structured P1 n x n (0: skip)")
+    ap.add_argument("--event-n", type=int, default=32, help="event-driven damage leg: P1 n x n (0: skip)")
     ap.add_argument("--voxel-n", type=int, default=64, help="voxel-output leg: structured Tet4 n^3 on a (2n)^3 grid "
                                                             "(0: skip)")
     ap.add_argument("--spmv-kernel", default="", help="STEPS,ORDER[,FORMAT]: set the batch's FIX48 SpMV candidate "
@@ -681,6 +738,9 @@ def main():
     if args.ingest_n > 0:
         ingest = ingest_leg(ag, ctx, args.ingest_n)
         log(f"mesh input: {ingest}")
+    events = None
+    if args.event_n > 0:
+        events = event_leg(ag, configs, ctx, args.event_n, 24, args.no_cpu)
     voxels = None
     if args.voxel_n > 0:
         voxels = voxel_leg(ag, ctx, args.voxel_n, 2 * args.voxel_n)
@@ -763,6 +823,7 @@ def main():
         "serving": serve,
         "mesh_input": ingest,
         "voxel_output": voxels,
+        "event_damage": events,
     }
     print(json.dumps(out), flush=True)
 

@@ -9,6 +9,11 @@
    [0, 0.3], bottom clamped, top u_z = steps * 2e-5.
 4. Periodic P1 RVEs with inclusions (radius U[0.02, 0.08], area fraction
    U[0.20, 0.45]), 12/60 GPa, three macro strains (exx, eyy, gxy) = 1e-3.
+
+event_square: the event-driven damage case at mesh size n (P1 unit square,
+matrix StiffnessAndFracture(10 GPa, 0.2, NonLocalMazars(1e-4, 10 GPa, 0.2,
+Gf 100, radius)) around an elastic 40 GPa disc of radius 0.18 at the
+centre, u_x = 0 left, u_y = 0 bottom, u_x = load right).
 """
 import numpy as np
 
@@ -121,3 +126,22 @@ def config5(rate=20.0, duration=10.0, seed=5, n2d=(64, 512), n3d=(16, 48), ndama
                 damage[key] = config3(ndamage, seed=300 + key)
             out.append((t, "damage", damage[key], key))
     return out
+
+
+def event_square(n, radius=None):
+    """The event-driven damage case on a structured P1 n x n mesh: (problem,
+    right-edge DOFs, the criterion and NonLocalMazars tables, radius).  The
+    characteristic radius defaults to 3 cells (3/n)."""
+    coords, conn = ag.structured_mesh(ag.P1, n)
+    phase = ag.phase_by_centroid(ag.P1, conn, coords, np.array([[0.5, 0.5, 0.18]]))
+    cm = np.stack([ag.cauchy_green(1e10, 0.2), ag.cauchy_green(4e10, 0.2)])
+    x, y = coords[:, 0], coords[:, 1]
+    left = np.nonzero(np.isclose(x, 0.))[0]
+    bottom = np.nonzero(np.isclose(y, 0.))[0]
+    right = np.nonzero(np.isclose(x, 1.))[0]
+    dofs = np.concatenate([2 * left, 2 * bottom + 1, 2 * right]).astype(np.int64)
+    vals = np.zeros(len(dofs))
+    prob = ag.Problem(ag.P1, coords, conn, cm, phase=phase, bc_dofs=dofs, bc_vals=vals)
+    crit = np.array([1, 0], dtype=np.int32)
+    mazars = np.array([[1e-4, 1e10, 0.2, 100.], [0., 0., 0., 1.]])
+    return prob, dofs, len(left) + len(bottom), crit, mazars, (3. / n if radius is None else radius)
