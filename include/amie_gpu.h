@@ -115,7 +115,7 @@ typedef struct {
     int64_t maxit;           /* per right-hand side, <= 0: max(2 ndof, 1000) (a mixed-precision solve of a
                                 tiny system needs a few refinement cycles beyond ndof iterations) */
     int32_t use_x0;          /* 1: start from the current device solution */
-    int32_t use_graph;       /* 1: replay captured hipGraphs of iteration chunks */
+    int32_t use_graph;       /* 1: replay hipGraphs of iteration chunks (default 0: plain launches) */
     int32_t chunk;           /* iterations per chunk between host status checks */
     int32_t time_kernels;    /* 1: time every SpMV-kernel launch with HIP events */
     int32_t escalate;        /* AMIE_MIXED: continue stagnating problems in AMIE_FP64 (default 1) */

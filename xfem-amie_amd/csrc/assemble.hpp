@@ -778,15 +778,21 @@ __device__ inline double smoothing_weight(int kernel, double rrn)
     return 1. - t >= 0. ? (t - 1.) * (t - 1.) : 0.;
 }
 
-template <int ET>
+// LANES = 1: a thread per element; LANES = 64: a wavefront per element, its
+// lanes taking every 64th candidate of each cell and the partial sums
+// added across the wavefront at the end (for neighbourhoods of hundreds of
+// elements, where a thread per element leaves the GPU idle).
+template <int ET, int LANES>
 __global__ __launch_bounds__(256) void k_smooth(NonlocalArgs a, const double *strain, int n, double *out)
 {
     AMIE_NO_CONTRACT
     using E = Elem<ET>;
     constexpr int S = E::S, NP = E::NPER, D = E::DIM;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const double *ge = a.geo + (int64_t)e * GEO_STRIDE;
+    const int64_t e = LANES == 1 ? (int64_t)blockIdx.x * blockDim.x + threadIdx.x
+                                 : (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = LANES == 1 ? 0 : (int)(threadIdx.x % WAVE);
+    if (e >= n) return;   // uniform over a wavefront when LANES = 64
+    const double *ge = a.geo + e * GEO_STRIDE;
     const double c[3] = {ge[0], ge[1], ge[2]};
     const double R = ge[3], rho = ge[4];
     const int p = a.eprob[e];
@@ -804,7 +810,7 @@ __global__ __launch_bounds__(256) void k_smooth(NonlocalArgs a, const double *st
         for (int y = lo[1]; y <= hi[1]; y++)
             for (int x = lo[0]; x <= hi[0]; x++) {
                 const int cell = gd[3] + (z * gd[1] + y) * gd[0] + x;
-                for (int j = a.cell_start[cell]; j < a.cell_start[cell + 1]; j++) {
+                for (int j = a.cell_start[cell] + lane; j < a.cell_start[cell + 1]; j += LANES) {
                     const int i = a.cell_elem[j];
                     if (a.ephase && a.ephase[i] != ph) continue;
                     if (a.edamage && a.edamage[i] >= a.threshold) continue;
@@ -831,15 +837,20 @@ __global__ __launch_bounds__(256) void k_smooth(NonlocalArgs a, const double *st
                     // Mesh::getSmoothedFields Kahan-adds (mesher/mesh.h:1453-1462)
                     const double v = gi[5] * coef;
                     for (int q = 0; q < S; q++) {
-                        const double y = strain[(int64_t)i * S + q] * v - cor[q];
-                        const double t = acc[q] + y;
-                        cor[q] = (t - acc[q]) - y;
+                        const double yk = strain[(int64_t)i * S + q] * v - cor[q];
+                        const double t = acc[q] + yk;
+                        cor[q] = (t - acc[q]) - yk;
                         acc[q] = t;
                     }
                     sum += v;
                 }
             }
-    for (int q = 0; q < S; q++) out[(int64_t)e * S + q] = sum > 1e-12 ? acc[q] / sum : 0.;
+    if constexpr (LANES > 1) {
+        for (int q = 0; q < S; q++) acc[q] = wave_sum(acc[q] - cor[q]);
+        sum = wave_sum(sum);
+        if (lane) return;
+    }
+    for (int q = 0; q < S; q++) out[e * S + q] = sum > 1e-12 ? acc[q] / sum : 0.;
 }
 
 // ------------------------------------------------- event-driven damage
@@ -987,21 +998,26 @@ __device__ inline int64_t visit_rank(int64_t local, int64_t ne, int64_t fd)
 // fractured by this pass at an element visited earlier (newfrac, the
 // previous round's guess).  Rounds repeat until newfrac stops growing; the
 // decisions then equal those of the reference's serial pass.
-template <int ET>
+template <int ET, int LANES>
 __global__ __launch_bounds__(256) void k_fiber_decide(NonlocalArgs a, EventArgs ev, const double *score,
                                                       const double *dprev, const uint8_t *newfrac, int n, uint8_t *inc)
 {
     using E = Elem<ET>;
     constexpr int D = E::DIM;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    const int64_t e = LANES == 1 ? (int64_t)blockIdx.x * blockDim.x + threadIdx.x
+                                 : (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = LANES == 1 ? 0 : (int)(threadIdx.x % WAVE);
+    if (e >= n) return;   // uniform over a wavefront when LANES = 64
     const int ph = a.ephase[e];
     const double s = score[e];
-    if (!ev.crit[ph] || !(s > 0.) || dprev[e] >= ev.frac_at) { inc[e] = 0; return; }
+    if (!ev.crit[ph] || !(s > 0.) || dprev[e] >= ev.frac_at) {
+        if (!lane) inc[e] = 0;
+        return;
+    }
     const int p = a.eprob[e];
     const int64_t e0 = ev.prob_e0[p], ne = ev.prob_ne[p];
     const int64_t re = visit_rank(e - e0, ne, ev.first_deferred);
-    const double *ge = a.geo + (int64_t)e * GEO_STRIDE;
+    const double *ge = a.geo + e * GEO_STRIDE;
     const double c[3] = {ge[0], ge[1], ge[2]};
     const double rho = ge[7];
     const double *g = a.grid + (int64_t)p * 4;
@@ -1016,7 +1032,7 @@ __global__ __launch_bounds__(256) void k_fiber_decide(NonlocalArgs a, EventArgs 
         for (int y = lo[1]; y <= hi[1]; y++)
             for (int x = lo[0]; x <= hi[0]; x++) {
                 const int cell = gd[3] + (z * gd[1] + y) * gd[0] + x;
-                for (int j = a.cell_start[cell]; j < a.cell_start[cell + 1]; j++) {
+                for (int j = a.cell_start[cell] + lane; j < a.cell_start[cell + 1]; j += LANES) {
                     const int i = a.cell_elem[j];
                     if (a.ephase[i] != ph) continue;
                     const double *gi = a.geo + (int64_t)i * GEO_STRIDE;
@@ -1026,9 +1042,18 @@ __global__ __launch_bounds__(256) void k_fiber_decide(NonlocalArgs a, EventArgs 
                     if (!in || dprev[i] >= ev.frac_at) continue;
                     if (newfrac[i] && visit_rank(i - e0, ne, ev.first_deferred) < re) continue;
                     const double si = score[i];
-                    mx = mx < si ? si : mx;   // std::max(mx, si)
+                    mx = mx < si ? si : mx;   // std::max(mx, si): a NaN score is passed over
                 }
             }
+    if constexpr (LANES > 1) {
+        // every lane started from s > 0, so no partial maximum is NaN
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double t = __shfl_xor(mx, o, WAVE);
+            mx = mx < t ? t : mx;
+        }
+        if (lane) return;
+    }
     inc[e] = fabs(s - mx) < ev.tol ? 1 : 0;
 }
 

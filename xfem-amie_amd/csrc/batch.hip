@@ -1404,7 +1404,7 @@ void amie_gpu_default_opts(amie_solve_opts *o)
     o->ckpt_interval = 128;
     o->maxit = 0;
     o->use_x0 = 0;
-    o->use_graph = 1;
+    o->use_graph = 0;   // plain launches: the graphs are no faster (the host enqueues far ahead)
     o->autotune = 0;
     o->chunk = 32;
     o->time_kernels = 0;
@@ -1879,12 +1879,16 @@ static int smoothed_all(Batch &B, int rhs, DevBuf &tmp, DevBuf &out)
     NonlocalArgs a = nonlocal_args(B);
     dim3 gr((n + 255) / 256), bl(256);
     hipStream_t st = B.stream;
+    // a wavefront per element where the neighbourhoods are large
+    const dim3 grw((unsigned)((n + 3) / 4));
     if (B.et.et == 1) {
         hipLaunchKernelGGL(k_strain<1>, gr, bl, 0, st, s, 0, n, rhs, tmp.as<double>());
-        hipLaunchKernelGGL(k_smooth<1>, gr, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
+        if (B.nl_wave) hipLaunchKernelGGL((k_smooth<1, 64>), grw, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
+        else hipLaunchKernelGGL((k_smooth<1, 1>), gr, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
     } else {
         hipLaunchKernelGGL(k_strain<3>, gr, bl, 0, st, s, 0, n, rhs, tmp.as<double>());
-        hipLaunchKernelGGL(k_smooth<3>, gr, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
+        if (B.nl_wave) hipLaunchKernelGGL((k_smooth<3, 64>), grw, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
+        else hipLaunchKernelGGL((k_smooth<3, 1>), gr, bl, 0, st, a, tmp.as<double>(), n, out.as<double>());
     }
     AMIE_HIP(hipGetLastError());
     return AMIE_OK;
@@ -2059,6 +2063,35 @@ static int set_nonlocal(Batch &B, double radius, int32_t kernel, double threshol
     {
         std::vector<int32_t> fill(start.begin(), start.end() - 1);
         for (int64_t e = 0; e < ne; e++) order[fill[ecell[e]]++] = (int32_t)e;
+    }
+    // candidates a thread would scan, averaged over up to 4096 elements: a
+    // wavefront per element pays off above a few hundred
+    {
+        const int64_t step = std::max<int64_t>(1, ne / 4096);
+        double scan1 = 0., scan2 = 0.;
+        int64_t cnt = 0;
+        for (int64_t e = 0; e < ne; e += step, cnt++) {
+            const int p = eprob[e];
+            const double *g = &geo[(size_t)e * GEO_STRIDE];
+            for (int which = 0; which < 2; which++) {
+                const double rr = which ? g[7] : g[4];
+                int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+                for (int d = 0; d < D; d++) {
+                    lo[d] = std::max(0, (int)std::floor((g[d] - rr - grid[p * 4 + d]) / grid[p * 4 + 3]));
+                    hi[d] = std::min(gdim[p * 4 + d] - 1, (int)std::floor((g[d] + rr - grid[p * 4 + d]) / grid[p * 4 + 3]));
+                }
+                double c = 0.;
+                for (int z = lo[2]; z <= hi[2]; z++)
+                    for (int y = lo[1]; y <= hi[1]; y++)
+                        for (int x = lo[0]; x <= hi[0]; x++) {
+                            const int64_t cell = gdim[p * 4 + 3] + ((int64_t)z * gdim[p * 4 + 1] + y) * gdim[p * 4 + 0] + x;
+                            c += start[cell + 1] - start[cell];
+                        }
+                (which ? scan2 : scan1) += c;
+            }
+        }
+        B.nl_wave = cnt && scan1 / cnt > 256.;
+        B.ev_wave = cnt && scan2 / cnt > 256.;
     }
     int rc;
     if ((rc = alloc_and_upload(B.egeo, geo.data(), geo.size() * 8))) return rc;
@@ -2260,6 +2293,7 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
     hipStream_t sm = B.stream;
     amie_solve_opts o = o_;
     o.use_x0 = 1;
+    o.use_graph = 0;   // every pass would rebuild the graphs for its active set
     using clk = std::chrono::steady_clock;
     for (;;) {
         const auto t0 = clk::now();
@@ -2289,12 +2323,15 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
         // growing; every round grows it or ends the loop, so at most n + 1
         unsigned long long nf_prev = 0;
         for (int round = 0; round <= n; round++) {
-            if (B.et.et == 1)
-                hipLaunchKernelGGL(k_fiber_decide<1>, gr, bl, 0, sm, na, ev, B.ev_score.as<double>(),
-                                   B.ev_dprev.as<double>(), newfrac, n, inc);
-            else
-                hipLaunchKernelGGL(k_fiber_decide<3>, gr, bl, 0, sm, na, ev, B.ev_score.as<double>(),
-                                   B.ev_dprev.as<double>(), newfrac, n, inc);
+            const dim3 grw((unsigned)((n + 3) / 4));
+            const double *scp = B.ev_score.as<double>(), *dpp = B.ev_dprev.as<double>();
+            if (B.et.et == 1) {
+                if (B.ev_wave) hipLaunchKernelGGL((k_fiber_decide<1, 64>), grw, bl, 0, sm, na, ev, scp, dpp, newfrac, n, inc);
+                else hipLaunchKernelGGL((k_fiber_decide<1, 1>), gr, bl, 0, sm, na, ev, scp, dpp, newfrac, n, inc);
+            } else {
+                if (B.ev_wave) hipLaunchKernelGGL((k_fiber_decide<3, 64>), grw, bl, 0, sm, na, ev, scp, dpp, newfrac, n, inc);
+                else hipLaunchKernelGGL((k_fiber_decide<3, 1>), gr, bl, 0, sm, na, ev, scp, dpp, newfrac, n, inc);
+            }
             AMIE_HIP(hipMemsetAsync(cnt, 0, 8, sm));
             hipLaunchKernelGGL(k_fiber_mark, gr, bl, 0, sm, ev, B.ev_dprev.as<double>(), inc, n, newfrac, cnt);
             AMIE_HIP(hipGetLastError());

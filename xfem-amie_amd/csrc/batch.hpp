@@ -158,6 +158,7 @@ struct Batch {
     bool nonlocal_on = false;
     int nl_kernel = 0;
     double nl_radius = 0., nl_threshold = 1.;
+    bool nl_wave = false, ev_wave = false;   // a wavefront per element in k_smooth / k_fiber_decide
     // event-driven damage (amie_gpu_batch_event_step): an element with
     // damage >= frac_at keeps frac_residual of its stiffness
     double frac_at = HUGE_VAL, frac_residual = 0.;
