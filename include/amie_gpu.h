@@ -407,6 +407,14 @@ int amie_gpu_batch_set_spmv_kernel(amie_gpu_batch *b, int32_t precision, int32_t
 /* Element strains (nelem x 3|6, Voigt, engineering shear) of right-hand side
  * rhs, including the macro strain. */
 int amie_gpu_batch_element_strain(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain);
+/* Strain and stress (nullable) at every Gauss point of problem p's elements
+ * for right-hand side rhs: nelem x ngp x 3|6 each, ngp = 4 for P2 (the
+ * 4-point rule of elements/elements.cpp:684-689), 1 for P1 and Tet4.  Restates
+ * ElementState::getField(TOTAL_STRAIN_FIELD / REAL_STRESS_FIELD) at the
+ * Gauss points (elements/integrable_entity.cpp:977-1097,1379-1394): the
+ * shape-function derivatives at the point with the inverse Jacobian cached
+ * at the centroid; stress = strain . C (1 - d). */
+int amie_gpu_batch_gauss_fields(amie_gpu_batch *b, int32_t p, int32_t rhs, double *strain, double *stress);
 /* Volume-averaged stress per right-hand side: nprob x nrhs x 3|6.  For unit
  * macro strains this is the effective stiffness. */
 int amie_gpu_batch_average_stress(amie_gpu_batch *b, double *out);

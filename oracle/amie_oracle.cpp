@@ -27,7 +27,14 @@
 //   or_average_stress    homogeneous fields; Voigt/Reuss bounds
 //   or_smoothed_strain   constant fields reproduced; kernel values at known
 //                        distances; a lone element returns its own strain;
-//                        phase and fracture exclusions; neighbourhood radius
+//                        phase and fracture exclusions; neighbourhood radius;
+//                        the reference's own smoothed principal strains
+//   or_gauss_fields      quadratic fields exact at the P2 Gauss points;
+//                        uniaxial stress
+//   or_mazars_score,     two FeatureTree runs of the compiled reference
+//   or_fiber_pass        (damage, passes, scores, neighbourhood maxima);
+//                        closed-form uniaxial score; a hand-made pass
+//   or_principal         numpy.linalg.eigvalsh
 
 #include "amie_oracle.h"
 
@@ -890,6 +897,63 @@ void or_element_strain(int etype, int64_t nelem, const int32_t *conn, const doub
                        double *strain)
 {
     or_element_strain_dofmap(etype, nelem, conn, nullptr, coords, u, nullptr, strain);
+}
+
+// Strain and stress at every Gauss point of the element's own rule
+// (gauss_points): ElementState::getField(TOTAL_STRAIN_FIELD, ...) at a
+// local point (elements/integrable_entity.cpp:977-1097; the inverse
+// Jacobian is the cache made at the centroid, :1011-1012) and
+// REAL_STRESS_FIELD (:1379-1394: strain . C(1 - d)).  strain, stress:
+// nelem x ngp x 3|6; stress may be null.
+void or_gauss_fields(int etype, int64_t nelem, const int32_t *conn, const double *coords, const double *cmat,
+                     const int32_t *phase, const double *damage, const double *u, double *strain, double *stress)
+{
+    Shape sh = shape_of(etype);
+    const int s = sh.dim == 2 ? 3 : 6;
+    double gp[12], gw[4];
+    const int ngp = gauss_points(etype, gp, gw);
+    for (int64_t e = 0; e < nelem; e++) {
+        double xy[18], Jinv[9], dxi[6], deta[6], dzeta[6];
+        const int32_t *ids = conn + e * sh.nper;
+        for (int i = 0; i < sh.nper; i++)
+            for (int d = 0; d < sh.dim; d++) xy[i * sh.dim + d] = coords[(size_t)ids[i] * sh.dim + d];
+        const double cx = sh.dim == 2 ? 1. / 3. : .25, cz = sh.dim == 2 ? 0. : .25;
+        inverse_jacobian(etype, xy, cx, cx, cz, Jinv);
+        for (int g = 0; g < ngp; g++) {
+            dshape(etype, gp[3 * g], gp[3 * g + 1], gp[3 * g + 2], dxi, deta, dzeta);
+            double *out = strain + (e * ngp + g) * s;
+            if (sh.dim == 2) {
+                double x_xi = 0, x_eta = 0, y_xi = 0, y_eta = 0;
+                for (int j = 0; j < sh.nper; j++) {
+                    const double ux = u[(size_t)ids[j] * 2], uy = u[(size_t)ids[j] * 2 + 1];
+                    x_xi += dxi[j] * ux; x_eta += deta[j] * ux;
+                    y_xi += dxi[j] * uy; y_eta += deta[j] * uy;
+                }
+                out[0] = x_xi * Jinv[0] + x_eta * Jinv[1];
+                out[1] = y_xi * Jinv[2] + y_eta * Jinv[3];
+                out[2] = x_xi * Jinv[2] + x_eta * Jinv[3] + y_xi * Jinv[0] + y_eta * Jinv[1];
+            } else {
+                double gg[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (int j = 0; j < sh.nper; j++)
+                    for (int c = 0; c < 3; c++) {
+                        const double v = u[(size_t)ids[j] * 3 + c];
+                        gg[c][0] += dxi[j] * v; gg[c][1] += deta[j] * v; gg[c][2] += dzeta[j] * v;
+                    }
+                auto d = [&](int c, int row) { return gg[c][0] * Jinv[row * 3] + gg[c][1] * Jinv[row * 3 + 1] + gg[c][2] * Jinv[row * 3 + 2]; };
+                out[0] = d(0, 0); out[1] = d(1, 1); out[2] = d(2, 2);
+                out[3] = d(1, 2) + d(2, 1); out[4] = d(0, 2) + d(2, 0); out[5] = d(1, 0) + d(0, 1);
+            }
+            if (stress) {
+                const double *Cp = cmat + (size_t)(phase ? phase[e] : 0) * s * s;
+                const double f = damage ? 1. - damage[e] : 1.;
+                for (int j = 0; j < s; j++) {
+                    double t = 0.;
+                    for (int i = 0; i < s; i++) t += out[i] * (Cp[i * s + j] * f);
+                    stress[(e * ngp + g) * s + j] = t;
+                }
+            }
+        }
+    }
 }
 
 void or_element_strain_dofmap(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap,

@@ -75,6 +75,12 @@ void or_macro_rhs(int etype, int64_t nelem, const int32_t *conn, const int32_t *
 void or_element_strain_dofmap(int etype, int64_t nelem, const int32_t *conn, const int32_t *dofmap,
                               const double *coords, const double *u, const double *macro, double *strain);
 
+/* Strain (and stress, nullable) at every Gauss point of each element:
+   ElementState::getField at the element's Gauss points
+   (elements/integrable_entity.cpp:977-1097,1379-1394).  nelem x ngp x 3|6. */
+void or_gauss_fields(int etype, int64_t nelem, const int32_t *conn, const double *coords, const double *cmat,
+                     const int32_t *phase, const double *damage, const double *u, double *strain, double *stress);
+
 /* solvers/assembly.cpp:125-322: symmetric elimination, unit diagonal,
  * RHS lift, b[id] = g.  dofs need not be sorted. */
 void or_apply_dirichlet(int stride, int64_t nrows, const uint32_t *row_size,

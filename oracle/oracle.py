@@ -55,6 +55,7 @@ def lib():
         L.or_smoothed_strain.argtypes = [i32, i64, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
         L.or_assemble_fractured.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, dbl, i64, vp, vp, vp]
         L.or_principal.argtypes = [i32, i32, vp, vp]
+        L.or_gauss_fields.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, vp, vp]
         L.or_mazars_score.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, vp]
         L.or_fiber_pass.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, i64, vp, vp]
         L.or_fiber_pass.restype = i64
@@ -275,6 +276,25 @@ def pcg(A, b, x0=None, rtol=1e-10, maxit=100000, precond=0):
     rr = np.zeros(1)
     ok = lib().or_pcg(*A.args(), _p(b), _p(x0), rtol, maxit, precond, _p(x), _p(nit), _p(rr))
     return x, int(nit[0]), float(rr[0]), ok
+
+
+NGP = {P1: 1, P2: 4, TET4: 1}
+
+
+def gauss_fields(etype, conn, coords, cmats, u, phase=None, damage=None):
+    """Strain and stress at every Gauss point: (nelem, ngp, 3|6) each."""
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    cmats = _c(cmats, np.float64)
+    u = _c(u, np.float64)
+    phase = _c(phase, np.int32)
+    damage = _c(damage, np.float64)
+    s = 3 if DIM[etype] == 2 else 6
+    eps = np.zeros((conn.shape[0], NGP[etype], s))
+    sig = np.zeros_like(eps)
+    lib().or_gauss_fields(etype, conn.shape[0], _p(conn), _p(coords), _p(cmats), _p(phase), _p(damage), _p(u),
+                          _p(eps), _p(sig))
+    return eps, sig
 
 
 def element_strain(etype, conn, coords, u):

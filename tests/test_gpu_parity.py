@@ -1024,3 +1024,41 @@ def test_triangle_writer_from_device_solution(ctx, tmp_path):
         scale = np.abs(b[:, lo:hi]).max()
         assert np.abs(a[:, lo:hi] - b[:, lo:hi]).max() <= 1e-5 * scale, (lo, hi)
 
+
+
+@pytest.mark.parametrize("case", ["p2", "tet4_damaged"])
+def test_gauss_fields_vs_oracle(ctx, case):
+    """Strain and stress at every Gauss point (4 for P2, 1 for Tet4) of the
+    device's solution against the oracle's restatement of
+    ElementState::getField at the Gauss points, to 1e-12 of the largest
+    value (the stress with each element's C (1 - d))."""
+    print("case", case, flush=True)
+    p = configs.config2(20) if case == "p2" else configs.config3(6, seed=3, step=1)
+    B = ag.Batch(ctx, [p])
+    assert B.solve(precision=ag.FP64, rtol=1e-10)["status"] == 0
+    x = B.solution()
+    eps, sig = B.gauss_fields()
+    oe, os_ = orc.gauss_fields(p.etype, p.conn, p.coords, p.cmats, x, phase=p.phase, damage=p.damage)
+    assert eps.shape == oe.shape and eps.shape[1] == (4 if case == "p2" else 1)
+    np.testing.assert_allclose(eps, oe, rtol=0, atol=1e-12 * np.abs(oe).max())
+    np.testing.assert_allclose(sig, os_, rtol=0, atol=1e-12 * np.abs(os_).max())
+    if case != "p2":
+        np.testing.assert_allclose(eps[:, 0, :], B.element_strain(), rtol=0, atol=1e-14 * np.abs(oe).max())
+    B.close()
+
+
+def test_gauss_fields_p1_vs_reference_writer(ctx):
+    """The P1 element strain and stress of the reference's own FeatureTree
+    solve (REAL_STRESS_FIELD / TOTAL_STRAIN_FIELD at the vertices, constant
+    over a P1 element; tests/golden/ref/writer_p1_*) from its displacements
+    handed to the device."""
+    conn, coords, u = g("writer_p1_conn").astype(np.int32), g("writer_p1_coords"), g("writer_p1_u")
+    strain, stress = g("writer_p1_strain"), g("writer_p1_stress")
+    p = ag.Problem(ag.P1, coords, conn, ag.cauchy_green(1e10, 0.2)[None], bc_dofs=np.array([0], dtype=np.int64),
+                   bc_vals=np.zeros(1))
+    B = ag.Batch(ctx, [p])
+    B.set_solution(u)
+    eps, sig = B.gauss_fields()
+    np.testing.assert_allclose(eps[:, 0, :], strain[:, 0, :], rtol=0, atol=1e-12 * np.abs(strain).max())
+    np.testing.assert_allclose(sig[:, 0, :], stress[:, 0, :], rtol=0, atol=1e-9 * np.abs(stress).max())
+    B.close()

@@ -461,3 +461,53 @@ def test_smoothed_strain_locus_radius():
         out = orc.smoothed_strain(P1, conn, coords, eps, radius, kernel=orc.GAUSSIAN_NONCOMPACT)
         assert (out[0, 0] > 0) == inside, (dist, rho, out[0])
         assert np.allclose(np.linalg.norm((tri + shift).mean(0) - cen), dist)
+
+
+def test_gauss_strain_of_a_quadratic_field_is_exact():
+    """P2 reproduces a quadratic displacement field exactly, so the strain at
+    each of the 4 Gauss points (0.2, 0.2), (0.6, 0.2), (0.2, 0.6), centroid,
+    mapped to the physical triangle (x = eta X0 + (1 - xi - eta) X2 + xi X4,
+    straight sides) is the analytic gradient there: a swapped or shifted
+    Gauss point, a transposed Jacobian or a lost shear term fails."""
+    coords, conn = ag.structured_mesh(ag.P2, 4)
+    x, y = coords[:, 0], coords[:, 1]
+    a, b, c, d, e, f = 0.7, -0.4, 0.3, 0.2, 0.5, -0.6
+    u = np.zeros(2 * len(x))
+    u[0::2] = 1e-3 * (a * x * x + b * x * y + c * y * y)
+    u[1::2] = 1e-3 * (d * x * x + e * x * y + f * y * y)
+    eps, _ = orc.gauss_fields(P2, conn, coords, ag.cauchy_green(1e10, 0.2)[None], u)
+    G = np.array([[0.2, 0.2], [0.6, 0.2], [0.2, 0.6], [1 / 3, 1 / 3]])
+    X = coords[conn]
+    for g, (xi, eta) in enumerate(G):
+        P = eta * X[:, 0] + (1 - xi - eta) * X[:, 2] + xi * X[:, 4]
+        px, py = P[:, 0], P[:, 1]
+        exx = 1e-3 * (2 * a * px + b * py)
+        eyy = 1e-3 * (e * px + 2 * f * py)
+        gxy = 1e-3 * (b * px + 2 * c * py + 2 * d * px + e * py)
+        np.testing.assert_allclose(eps[:, g, 0], exx, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(eps[:, g, 1], eyy, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(eps[:, g, 2], gxy, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("etype", [P1, P2, TET4])
+def test_gauss_stress_of_uniaxial_stress_state(etype):
+    """A displacement field whose strain is that of uniaxial stress sigma_xx
+    = s (plane stress: e_yy = -nu e_xx; 3D: e_yy = e_zz = -nu e_xx) gives
+    stress (s, 0, 0 ...) at every Gauss point, scaled by (1 - d) of the
+    element."""
+    E, nu, s = 1e10, 0.25, 2e6
+    coords, conn = _mesh(etype, 3)
+    exx = s / E
+    u = np.zeros(coords.size)
+    dim = coords.shape[1]
+    u[0::dim] = exx * coords[:, 0]
+    u[1::dim] = -nu * exx * coords[:, 1]
+    if dim == 3:
+        u[2::3] = -nu * exx * coords[:, 2]
+    C = ag.cauchy_green(E, nu, dim=dim)[None]
+    d = np.linspace(0., 0.5, conn.shape[0])
+    _, sig = orc.gauss_fields(etype, conn, coords, C, u, damage=d)
+    expect = np.zeros(sig.shape[2])
+    expect[0] = s
+    want = np.broadcast_to((1 - d)[:, None, None] * expect[None, None, :], sig.shape)
+    np.testing.assert_allclose(sig, want, rtol=0, atol=1e-6 * s)

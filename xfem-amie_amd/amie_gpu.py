@@ -127,6 +127,7 @@ SIGNATURES = {
     "amie_gpu_batch_average_stress": (c_i32, [vp, vp]),
     "amie_gpu_batch_damage_step": (c_i32, [vp, vp, vp]),
     "amie_gpu_default_event_params": (None, [vp]),
+    "amie_gpu_batch_gauss_fields": (c_i32, [vp, c_i32, c_i32, vp, vp]),
     "amie_gpu_batch_event_step": (c_i32, [vp, vp, vp, vp]),
     "amie_gpu_batch_get_scores": (c_i32, [vp, c_i32, vp]),
     "amie_gpu_batch_get_damage": (c_i32, [vp, c_i32, vp, vp]),
@@ -577,6 +578,15 @@ class Batch:
         out = np.zeros((pr.conn.shape[0], VOIGT[self.etype]))
         check(lib().amie_gpu_batch_element_strain(self.h, p, rhs, _p(out)))
         return out
+
+    def gauss_fields(self, p=0, rhs=0):
+        """Strain and stress at every Gauss point: (nelem, ngp, 3|6) each."""
+        pr = self.problems[p]
+        ngp = 4 if self.etype == P2 else 1
+        eps = np.zeros((pr.conn.shape[0], ngp, VOIGT[self.etype]))
+        sig = np.zeros_like(eps)
+        check(lib().amie_gpu_batch_gauss_fields(self.h, p, rhs, _p(eps), _p(sig)))
+        return eps, sig
 
     def set_nonlocal(self, radius, kernel=0, threshold=1.0):
         """Non-local damage driving strain (linear elements); radius <= 0

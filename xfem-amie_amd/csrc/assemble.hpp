@@ -639,7 +639,7 @@ struct StrainArgs {
 // Centroid strain, elements/integrable_entity.cpp:977-1097 (Voigt,
 // engineering shear), displacement = macro part + solved part.
 template <int ET>
-__device__ inline void element_strain(const StrainArgs &a, int e, int k, double *eps)
+__device__ inline void element_strain_at(const StrainArgs &a, int e, int k, double px, double py, double *eps)
 {
     AMIE_NO_CONTRACT
     using E = Elem<ET>;
@@ -651,10 +651,9 @@ __device__ inline void element_strain(const StrainArgs &a, int e, int k, double 
         if (D == 2) X[i][2] = 0.;
     }
     double m[9];
-    centroid_jinv<ET>(X, m);
+    centroid_jinv<ET>(X, m);   // the element's inverse Jacobian cache is made at the centroid
     double dxi[6], deta[6], dzeta[6];
-    if constexpr (D == 2) dshape<ET>(1. / 3., 1. / 3., dxi, deta, dzeta);
-    else dshape<ET>(.25, .25, dxi, deta, dzeta);
+    dshape<ET>(px, py, dxi, deta, dzeta);
     double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}; // [component][reference direction]
     for (int j = 0; j < NP; j++) {
         const int64_t row = a.erow[(int64_t)e * NP + j];
@@ -675,6 +674,48 @@ __device__ inline void element_strain(const StrainArgs &a, int e, int k, double 
         eps[0] = d(0, 0); eps[1] = d(1, 1); eps[2] = d(2, 2);
         eps[3] = d(1, 2) + d(2, 1); eps[4] = d(0, 2) + d(2, 0); eps[5] = d(1, 0) + d(0, 1);
         if (mac) for (int q = 0; q < 6; q++) eps[q] += mac[q];
+    }
+}
+
+template <int ET> __device__ inline void element_strain(const StrainArgs &a, int e, int k, double *eps)
+{
+    if constexpr (Elem<ET>::DIM == 2) element_strain_at<ET>(a, e, k, 1. / 3., 1. / 3., eps);
+    else element_strain_at<ET>(a, e, k, .25, .25, eps);
+}
+
+// Strain and stress at every Gauss point of each element's rule
+// (ElementState::getField TOTAL_STRAIN_FIELD / REAL_STRESS_FIELD at the
+// Gauss points, elements/integrable_entity.cpp:977-1097,1379-1394; the
+// stress is strain . C (1 - d), or C * residual once fractured).
+template <int ET>
+__global__ __launch_bounds__(256) void k_gauss_fields(StrainArgs a, int e0, int n, int k, const int32_t *ephase,
+                                                      const double *edamage, const int32_t *cmat_off,
+                                                      const double *cmat, double frac_at, double residual,
+                                                      double *strain, double *stress)
+{
+    AMIE_NO_CONTRACT
+    using E = Elem<ET>;
+    constexpr int S = E::S, NG = E::NGP;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int e = e0 + i;
+    const double *Cp = cmat + (int64_t)(cmat_off[a.eprob[e]] + ephase[e]) * S * S;
+    const double f = stiffness_factor(edamage, e, frac_at, residual);
+    for (int g = 0; g < NG; g++) {
+        double gx, gy, gw;
+        gauss<ET>(g, gx, gy, gw);
+        double eps[S];
+        element_strain_at<ET>(a, e, k, gx, gy, eps);
+        double *o = strain + ((int64_t)i * NG + g) * S;
+        for (int q = 0; q < S; q++) o[q] = eps[q];
+        if (stress) {
+            double *t = stress + ((int64_t)i * NG + g) * S;
+            for (int j = 0; j < S; j++) {
+                double acc = 0.;
+                for (int q = 0; q < S; q++) acc += eps[q] * (Cp[q * S + j] * f);
+                t[j] = acc;
+            }
+        }
     }
 }
 

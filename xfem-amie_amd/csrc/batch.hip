@@ -1827,6 +1827,35 @@ int amie_gpu_batch_element_strain(amie_gpu_batch *bb, int32_t p, int32_t rhs, do
     return AMIE_OK;
 }
 
+int amie_gpu_batch_gauss_fields(amie_gpu_batch *bb, int32_t p, int32_t rhs, double *strain, double *stress)
+{
+    if (!bb || check_p(bb->b, p) || !strain) return AMIE_ERR_ARG;
+    Batch &B = *bb->b;
+    if (!B.has_elements || rhs < 0 || rhs >= B.nrhs) { set_error("no elements or bad rhs"); return AMIE_ERR_ARG; }
+    const ProbHost &P = B.ph[p];
+    const int ngp = B.et.et == 2 ? 4 : 1;
+    const size_t bytes = (size_t)P.nelem * ngp * B.et.s * 8;
+    DevBuf eo, so;
+    if (!eo.alloc(bytes) || (stress && !so.alloc(bytes))) return AMIE_ERR_NOMEM;
+    StrainArgs s = strain_args(B);
+    const int n = (int)P.nelem;
+    dim3 gr((n + 255) / 256), bl(256);
+    hipStream_t st = B.stream;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, gr, bl, 0, st, s, (int)P.elem0, n, rhs, B.ephase.as<int32_t>(), B.edamage.as<double>(),
+                           B.cmat_off.as<int32_t>(), B.cmat.as<double>(), B.frac_at, B.frac_residual,
+                           eo.as<double>(), stress ? so.as<double>() : nullptr);
+    };
+    if (B.et.et == 1) go(k_gauss_fields<1>);
+    else if (B.et.et == 2) go(k_gauss_fields<2>);
+    else go(k_gauss_fields<3>);
+    AMIE_HIP(hipGetLastError());
+    AMIE_HIP(hipStreamSynchronize(st));
+    AMIE_HIP(hipMemcpy(strain, eo.p, bytes, hipMemcpyDeviceToHost));
+    if (stress) AMIE_HIP(hipMemcpy(stress, so.p, bytes, hipMemcpyDeviceToHost));
+    return AMIE_OK;
+}
+
 int amie_gpu_batch_average_stress(amie_gpu_batch *bb, double *outp)
 {
     if (!bb || !bb->b || !outp) return AMIE_ERR_ARG;
