@@ -690,6 +690,14 @@ def main():
         row, _ = other(headline, f"mixed at rel. res {r:g}", rtol=r)
         rate_at[r] = row["solves_per_s"]
         x_at[r] = {k: B.solution(k).copy() for k in sample}
+    planes_step = None
+    if tuned.get("fix48", {}).get("format") == 1:
+        # the same step without the value index (what a mesh with more than
+        # 65 536 distinct blocks, any damaged or unstructured one, runs)
+        tf = tuned["fix48"]
+        B.set_spmv_kernel(headline, ag.JACOBI, tf["steps"], tf["tile_order"], 0)
+        planes_step, _ = other(headline, "mixed, FIX48 value planes (no value index)")
+        B.set_spmv_kernel(headline, ag.JACOBI, tf["steps"], tf["tile_order"], 1)
     persistent = None
     if not args.no_persistent:
         # the same step as one persistent launch per solve pass (every
@@ -818,6 +826,7 @@ def main():
         "mixed_vs_fp64": (value / fp64["solves_per_s"]) if fp64 else None,
         "accuracy": accuracy,
         "persistent_batched": persistent,
+        "fix48_planes_batched": planes_step,
         "p1_128_patch": cfg1,
         "single_problems": single,
         "serving": serve,
