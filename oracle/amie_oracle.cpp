@@ -34,6 +34,10 @@
 //   or_mazars_score,     two FeatureTree runs of the compiled reference
 //   or_fiber_pass        (damage, passes, scores, neighbourhood maxima);
 //                        closed-form uniaxial score; a hand-made pass
+//   or_newton_pass       twenty passes of a compiled-reference FeatureTree
+//                        run with IsotropicLinearDamage replayed bit for bit
+//                        from its damage and scores; its first load step
+//                        (201 passes); a hand-made pass
 //   or_principal         numpy.linalg.eigvalsh
 
 #include "amie_oracle.h"
@@ -1313,6 +1317,73 @@ void or_mazars_score(int dim, int64_t nelem, const int32_t *phase, const int32_t
                                        1.0);
         score[e] = pseudo - d;
     }
+}
+
+// One pass of IsotropicLinearDamage::step, Newton branch (physics/
+// damagemodels/isotropiclineardamage.cpp:64-92), with
+// FractureCriterion::setChange at a checkpoint (fracturecriterion.cpp:
+// 282-369; the Newton branch never clears the checkpoint) and postProcess
+// (isotropiclineardamage.cpp:150-162).  T = the largest score of the pass
+// over all criterion elements (FeatureTree's maxScoreInit).  Element e of a
+// criterion phase is in its damaging set when T > 0, its score s > 0 and
+// T - s <= 4 tol; its neighbourhood (the second cache, see or_fiber_pass)
+// splits into that window and the rest, whose largest score m (max(0, T)
+// when the rest is empty) gives delta = max(s - m, T - s).  Unless
+// |T| < tol/2 and (e is out of the set or |s| < tol/20), an element in the
+// set with s > tol/20 takes d += clamp(min(delta, s (1 - d)), 0, 1), capped
+// at 1.  No decision reads damage written in the pass (postProcess adds it
+// after every element has stepped), and the model never counts as
+// fractured (its fraction is only set by DamageModel::step).  Returns the
+// number of elements changed.
+int64_t or_newton_pass(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,
+                       const int32_t *crit, const double *score, double *damage, double radius, double tol,
+                       int32_t *changed)
+{
+    Shape sh = shape_of(etype);
+    std::vector<ElemGeo> geo(nelem);
+    for (int64_t e = 0; e < nelem; e++) geo[e] = elem_geo(sh.dim, sh.nper, conn + e * sh.nper, coords);
+    double T = -1.;
+    for (int64_t e = 0; e < nelem; e++)
+        if (crit[phase ? phase[e] : 0]) T = std::max(score[e], T);
+    std::vector<double> inc(nelem, 0.);
+    int64_t nch = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : nch)
+    for (int64_t e = 0; e < nelem; e++) {
+        changed[e] = 0;
+        const int ph = phase ? phase[e] : 0;
+        if (!crit[ph]) continue;
+        const double s = score[e];
+        const bool inset = T > 0 && T - s <= 4. * tol && s > 0.;
+        double delta = 0.;
+        if (inset) {
+            const ElemGeo &ge = geo[e];
+            const double R = std::max(radius, ge.r * 1.1);
+            const double rho = std::max(R * 6., ge.r * 4.) * 2.5 + ge.r;
+            double m = std::max(0., T);
+            bool found = false;
+            for (int64_t i = 0; i < nelem; i++) {
+                if ((phase ? phase[i] : 0) != ph) continue;
+                const ElemGeo &gi = geo[i];
+                double d2 = 0.;
+                for (int d = 0; d < sh.dim; d++) d2 += (gi.c[d] - ge.c[d]) * (gi.c[d] - ge.c[d]);
+                const bool in = sh.dim == 2 ? d2 < rho * rho : d2 < rho * rho + 2. * rho * 1e-12 + 1e-24;
+                if (!in) continue;
+                if (T - score[i] <= 4. * tol && score[i] > 0.) continue;   // in the damaging window
+                m = found ? std::max(score[i], m) : score[i];
+                found = true;
+            }
+            delta = std::max(s - m, T - s);
+        }
+        if (std::abs(T) < .5 * tol && ((inset && std::abs(s) < .05 * tol) || !inset)) continue;
+        if (s > .05 * tol && inset) {
+            inc[e] = std::min(std::max(0. + std::min(delta, s * (1. - damage[e] - 0.)), 0.), 1.);
+            changed[e] = 1;
+            nch++;
+        }
+    }
+    for (int64_t e = 0; e < nelem; e++)
+        if (changed[e]) damage[e] = std::min(damage[e] + inc[e], 1.);
+    return nch;
 }
 
 // One pass of FiberBasedIsotropicLinearDamage::step over the elements

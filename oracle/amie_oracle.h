@@ -170,6 +170,12 @@ void or_principal(int dim, int doubled, const double *v, double *out);
 void or_mazars_score(int dim, int64_t nelem, const int32_t *phase, const int32_t *crit, const double *prm,
                      const double *cmat, const double *sbar, const double *damage, double fractured_at, double *score);
 
+/* One pass of IsotropicLinearDamage::step, Newton branch
+   (physics/damagemodels/isotropiclineardamage.cpp:64-92,150-162). */
+int64_t or_newton_pass(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,
+                       const int32_t *crit, const double *score, double *damage, double radius, double tol,
+                       int32_t *changed);
+
 /* One pass of FiberBasedIsotropicLinearDamage::step over the mesh
    (physics/damagemodels/fiberbasedisotropiclineardamage.cpp:58-92). */
 int64_t or_fiber_pass(int etype, int64_t nelem, const int32_t *conn, const double *coords, const int32_t *phase,

@@ -59,6 +59,8 @@ def lib():
         L.or_mazars_score.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, vp]
         L.or_fiber_pass.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, i64, vp, vp]
         L.or_fiber_pass.restype = i64
+        L.or_newton_pass.argtypes = [i32, i64, vp, vp, vp, vp, vp, vp, dbl, dbl, vp]
+        L.or_newton_pass.restype = i64
         L.or_renumber.argtypes = [i64, i64, i32, vp, vp]
         L.or_renumber.restype = i64
         L.or_add_midpoints.argtypes = [i64, vp, i64, vp, vp, vp]
@@ -443,25 +445,55 @@ def fiber_pass(etype, conn, coords, phase, crit, score, damage, radius, fraction
     return changed.astype(bool)
 
 
+def newton_pass(etype, conn, coords, phase, crit, score, damage, radius, tol=SCORE_TOLERANCE):
+    """One IsotropicLinearDamage (Newton) pass; updates damage in place and
+    returns the mask of the elements it changed."""
+    conn = _c(conn, np.int32)
+    coords = _c(coords, np.float64)
+    phase = _c(phase, np.int32)
+    crit = _c(np.asarray(crit), np.int32)
+    score = _c(score, np.float64)
+    assert damage.dtype == np.float64 and damage.flags.c_contiguous
+    changed = np.zeros(conn.shape[0], dtype=np.int32)
+    lib().or_newton_pass(etype, conn.shape[0], _p(conn), _p(coords), _p(phase), _p(crit), _p(score), _p(damage),
+                         radius, tol, _p(changed))
+    return changed.astype(bool)
+
+
+MAX_ITERATIONS_PER_STEP = 200   # FeatureTree::maxitPerStep (features/features.cpp:202)
+
+
 def event_damage(etype, conn, coords, cmats, phase, crit, prm, bc_dofs, bc_vals, damage, radius,
-                 kernel=QUARTIC_COMPACT, rtol=1e-12, max_passes=100000, first_deferred=FIRST_DEFERRED):
+                 kernel=QUARTIC_COMPACT, rtol=1e-12, max_passes=100000, first_deferred=FIRST_DEFERRED,
+                 model="fibre"):
     """One FeatureTree::step of the event-driven damage (features/features.cpp:
     6150-6373): solve with the current damage (fractured elements at the
     residual stiffness), score every element on its smoothed strain, run one
-    damage pass; repeat until a pass changes nothing.  Returns (damage,
-    displacements, scores of the last pass, passes)."""
+    damage pass; model "fibre" (FiberBasedIsotropicLinearDamage, the default
+    of StiffnessAndFracture) or "newton" (IsotropicLinearDamage).  The step
+    ends at a pass that changes nothing and leaves no element scoring above
+    0, or at such a pass once more than maxitPerStep passes have run.
+    Returns (damage, displacements, scores of the last pass, passes)."""
     d = np.array(damage, dtype=np.float64)
     dim = DIM[etype]
+    frac_at = FRACTURED_AT if model == "fibre" else np.inf
     passes = 0
     while True:
-        A = assemble_fractured(etype, conn, coords, cmats, phase, d)
+        A = assemble_fractured(etype, conn, coords, cmats, phase, d, fractured_at=frac_at)
         b = np.zeros(A.n)
         apply_dirichlet(A, b, bc_dofs, bc_vals)
         x = pcg(A, b, rtol=rtol, maxit=100000)[0]
         eps = element_strain(etype, conn, coords, x)
-        sbar = smoothed_strain(etype, conn, coords, eps, radius, kernel, phase, d, FRACTURED_AT)
-        score = mazars_score(dim, phase, crit, prm, cmats, sbar, d)
-        changed = fiber_pass(etype, conn, coords, phase, crit, score, d, radius, first_deferred=first_deferred)
+        sbar = smoothed_strain(etype, conn, coords, eps, radius, kernel, phase, d, frac_at)
+        score = mazars_score(dim, phase, crit, prm, cmats, sbar, d, fractured_at=frac_at)
+        if model == "fibre":
+            changed = fiber_pass(etype, conn, coords, phase, crit, score, d, radius, first_deferred=first_deferred)
+        else:
+            changed = newton_pass(etype, conn, coords, phase, crit, score, d, radius)
         passes += 1
-        if not changed.any() or passes >= max_passes:
+        crit_e = np.asarray(crit)[np.asarray(phase)]
+        met = bool((score[crit_e != 0] > 0).any())
+        if passes >= max_passes:
+            return d, x, score, passes
+        if not changed.any() and (not met or passes > MAX_ITERATIONS_PER_STEP):
             return d, x, score, passes

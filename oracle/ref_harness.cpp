@@ -34,6 +34,7 @@
 #include "physics/stiffness_and_fracture.h"
 #include "physics/fracturecriteria/mazars.h"
 #include "physics/damagemodels/fiberbasedisotropiclineardamage.h"
+#include "physics/damagemodels/isotropiclineardamage.h"
 #include "solvers/assembly.h"
 #include "solvers/conjugategradient.h"
 #include "solvers/inversediagonal.h"
@@ -787,6 +788,126 @@ static void golden_events(const std::string &tag, double sampling, const std::ve
     npy_f64(tag + "_first", first, {first.size() / 6, 6});
 }
 
+// The same loop with IsotropicLinearDamage as the damage model (its Newton
+// branch, physics/damagemodels/isotropiclineardamage.cpp:64-92 and
+// postProcess :150-162).  The model is final, so the passes are observed at
+// the criterion: a NonLocalMazars whose grade() (one call per element and
+// pass, FractureCriterion::step, fracturecriterion.cpp:534-560) records the
+// element's position, its damage before the pass and its score.
+struct CountingMazars : public NonLocalMazars {
+    struct Call { int64_t pos; double state, score; };
+    static std::vector<Call> *calls;
+    static std::map<const IntegrableEntity *, int64_t> *position;
+    CountingMazars(double thr, double E_, double nu_, double Gf_, double cs, double ce, double r, planeType p)
+        : NonLocalMazars(thr, E_, nu_, Gf_, cs, ce, r, p) { }
+    CountingMazars(double thr, double E_, double nu_, double Gf_, double r, planeType p)
+        : NonLocalMazars(thr, E_, nu_, Gf_, r, p) { }
+    virtual double grade(ElementState &s)
+    {
+        const double g = NonLocalMazars::grade(s);
+        if (calls && position)
+            calls->push_back({position->count(s.getParent()) ? position->at(s.getParent()) : -1,
+                              s.getParent()->getBehaviour()->getDamageModel()->getState()[0], g});
+        return g;
+    }
+    virtual FractureCriterion *getCopy() const
+    {
+        CountingMazars *ret = new CountingMazars(threshold, E, nu, Gf, cstress, cstrain, getMaterialCharacteristicRadius(), pt);
+        ret->copyEssentialParameters(this);
+        return ret;
+    }
+};
+std::vector<CountingMazars::Call> *CountingMazars::calls = nullptr;
+std::map<const IntegrableEntity *, int64_t> *CountingMazars::position = nullptr;
+
+static void golden_events_newton(const std::string &tag, double sampling, const std::vector<double> &loads, double E,
+                                 double nu, double threshold, double Gf, double radius, double E_inc)
+{
+    RectangularFeature sample(nullptr, 1., 1., 0.5, 0.5);
+    sample.setBehaviour(new StiffnessAndFracture(E, nu, new CountingMazars(threshold, E, nu, Gf, radius, PLANE_STRESS),
+                                                 new IsotropicLinearDamage()));
+    FeatureTree F(&sample);
+    Inclusion *inc = new Inclusion(&sample, 0.18, 0.5, 0.5);
+    inc->setBehaviour(new Stiffness(E_inc, nu));
+    F.addFeature(&sample, inc);
+    F.setSamplingNumber(sampling);
+    F.setOrder(LINEAR);
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_XI, LEFT));
+    F.addBoundaryCondition(new BoundingBoxDefinedBoundaryCondition(FIX_ALONG_ETA, BOTTOM));
+    BoundingBoxDefinedBoundaryCondition *right = new BoundingBoxDefinedBoundaryCondition(SET_ALONG_XI, RIGHT, 0.);
+    F.addBoundaryCondition(right);
+    renumber_tree(F);
+    auto *m = F.get2DMesh();
+    std::vector<int64_t> phase, ncalls;
+    std::map<const IntegrableEntity *, int64_t> pos;
+    for (auto i = m->begin(); i != m->end(); i++) {
+        phase.push_back(i->getBehaviour()->getFractureCriterion() ? 0 : 1);
+        pos[&*i] = i.getPosition();
+    }
+    std::vector<double> dmg, score, u;
+    std::vector<CountingMazars::Call> calls;
+    CountingMazars::calls = &calls;
+    CountingMazars::position = &pos;
+    for (double v : loads) {
+        right->setData(v);
+        const size_t before = calls.size();
+        F.step();
+        ncalls.push_back(int64_t(calls.size() - before));
+        for (auto i = m->begin(); i != m->end(); i++) {
+            DamageModel *dm = i->getBehaviour()->getDamageModel();
+            FractureCriterion *fc = i->getBehaviour()->getFractureCriterion();
+            dmg.push_back(dm ? dm->getState()[0] : 0.);
+            score.push_back(fc ? fc->getScoreAtState() : 0.);
+        }
+        Vector x = F.getDisplacements();
+        u.insert(u.end(), std::begin(x), std::end(x));
+    }
+    CountingMazars::calls = nullptr;
+    CountingMazars::position = nullptr;
+    // selected passes: the damage before pass k and k + 1 and the scores of
+    // pass k, for every element (0 outside the criterion phase); a pass is
+    // one grade() call per criterion element
+    const size_t ne = phase.size(), ns = loads.size();
+    size_t ncrit = 0;
+    for (int64_t p : phase) ncrit += p == 0;
+    const size_t npass = calls.size() / ncrit;
+    std::vector<int64_t> sel, sel_load;
+    std::vector<double> st0, sc0, st1;
+    size_t first = 0;
+    for (size_t k = 0; k < ns; k++) {
+        const size_t np = size_t(ncalls[k]) / ncrit;
+        for (size_t j : {size_t(0), size_t(1), size_t(2), np / 4, np / 2, np - 2, np - 1}) {
+            const size_t g = first + j;
+            if (j >= np || g + 1 >= npass || (!sel.empty() && sel.back() >= int64_t(g))) continue;
+            sel.push_back(int64_t(g));
+            sel_load.push_back(int64_t(k));
+            for (size_t w = 0; w < 2; w++) {
+                std::vector<double> a(ne, 0.), b(ne, 0.);
+                for (size_t c = (g + w) * ncrit; c < (g + w + 1) * ncrit; c++) {
+                    a[size_t(calls[c].pos)] = calls[c].state;
+                    b[size_t(calls[c].pos)] = calls[c].score;
+                }
+                if (w == 0) { st0.insert(st0.end(), a.begin(), a.end()); sc0.insert(sc0.end(), b.begin(), b.end()); }
+                else st1.insert(st1.end(), a.begin(), a.end());
+            }
+        }
+        first += np;
+    }
+    dump_mesh(tag, m, 2);
+    npy_i64(tag + "_phase", phase);
+    npy_f64(tag + "_loads", loads);
+    npy_f64(tag + "_params", {E, nu, threshold, Gf, radius, E_inc});
+    npy_f64(tag + "_damage", dmg, {ns, ne});
+    npy_f64(tag + "_score", score, {ns, ne});
+    npy_f64(tag + "_u", u, {ns, u.size() / ns});
+    npy_i64(tag + "_pass", sel);
+    npy_i64(tag + "_pass_load", sel_load);
+    npy_f64(tag + "_pass_state", st0, {sel.size(), ne});
+    npy_f64(tag + "_pass_score", sc0, {sel.size(), ne});
+    npy_f64(tag + "_pass_next", st1, {sel.size(), ne});
+    npy_i64(tag + "_ncalls", ncalls);
+}
+
 // The reference's own ConjugateGradient::solve (solvers/conjugategradient.cpp:
 // 69-318, default InverseDiagonal preconditioner, nssor 32 as
 // Assembly::cgsolve sets it, solvers/assembly.h:262 and assembly.cpp:1842)
@@ -903,6 +1024,7 @@ int main(int argc, char **argv)
         // a smaller radius: elements fractured within a pass change which
         // later-visited elements are at their neighbourhood's maximum
         golden_events("events_r01", 10, {1.1e-4, 1.375e-4, 1.65e-4}, 1e10, 0.2, 1e-4, 30., 0.01, 4e10);
+        golden_events_newton("events_newton", 10, {0.8e-4, 1.2e-4, 1.6e-4}, 1e10, 0.2, 1e-4, 100., 0.05, 4e10);
         printf("event-damage golden vectors written to %s\n", g_out.c_str());
         return 0;
     }
