@@ -472,7 +472,27 @@ typedef struct {
                                     (libgomp runs the first 65 tasks of the region last) */
     int64_t max_passes;          /* stop after this many solves (FeatureTree's own cap only counts
                                     passes without a change) */
+    int32_t damage_model;        /* AMIE_DAMAGE_FIBRE (default) or AMIE_DAMAGE_ISOTROPIC_LINEAR */
+    int64_t max_passes_per_step; /* FeatureTree::maxitPerStep (200): with AMIE_DAMAGE_ISOTROPIC_LINEAR a
+                                    pass that changes nothing ends the step only when no element scores
+                                    above 0 or more than this many passes have run */
 } amie_event_params;
+
+/* The damage model of the criterion's materials.  AMIE_DAMAGE_FIBRE:
+ * FiberBasedIsotropicLinearDamage as above.  AMIE_DAMAGE_ISOTROPIC_LINEAR:
+ * IsotropicLinearDamage's Newton branch (physics/damagemodels/
+ * isotropiclineardamage.cpp:64-92, postProcess :150-162): T = the largest
+ * score of the pass over the problem's criterion elements; an element with
+ * score s > 0 and T - s <= 4 score_tolerance is in the damaging window; m =
+ * the largest score of its neighbourhood's elements outside the window
+ * (max(0, T) when there are none), delta = max(s - m, T - s); unless |T| <
+ * score_tolerance / 2, a window element with s > score_tolerance / 20 takes
+ * d += clamp(min(delta, s (1 - d)), 0, 1), capped at 1, after the pass.  The
+ * model never fractures: the stiffness is C (1 - d), fibre_fraction,
+ * fractured_at, residual and first_deferred are not used.  In a batch the
+ * passes run until every problem's step has ended. */
+#define AMIE_DAMAGE_FIBRE 0
+#define AMIE_DAMAGE_ISOTROPIC_LINEAR 1
 
 typedef struct {
     int64_t passes;              /* solves, each followed by one damage pass */

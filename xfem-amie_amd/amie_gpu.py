@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libamie_gpu.so")
 
 P1, P2, TET4 = 1, 2, 3
 FP64, MIXED, MIXED64, MIXED_K64, MIXED_SPLIT, MIXED_FIX48 = 0, 1, 2, 3, 4, 5
+DAMAGE_FIBRE, DAMAGE_ISOTROPIC_LINEAR = 0, 1   # event-damage models (amie_event_params.damage_model)
 JACOBI, BLOCK_JACOBI = 0, 1
 REFINE_REPLACE, REFINE_RESTART = 0, 1
 ASSEMBLE_GATHER, ASSEMBLE_COLOURED = 0, 1
@@ -57,7 +58,8 @@ class DamageParams(ctypes.Structure):
 class EventParams(ctypes.Structure):
     _fields_ = [("nphase", c_i32), ("criterion", vp), ("mazars", vp), ("radius", c_dbl), ("kernel", c_i32),
                 ("fibre_fraction", c_dbl), ("fractured_at", c_dbl), ("residual", c_dbl), ("score_tolerance", c_dbl),
-                ("first_deferred", c_i64), ("max_passes", c_i64)]
+                ("first_deferred", c_i64), ("max_passes", c_i64), ("damage_model", c_i32),
+                ("max_passes_per_step", c_i64)]
 
 
 class EventStats(ctypes.Structure):
@@ -612,11 +614,13 @@ class Batch:
 
     def event_step(self, criterion, mazars, radius, kernel=0, **kw):
         """One FeatureTree::step of the event-driven damage (NonLocalMazars
-        + FiberBasedIsotropicLinearDamage) at the current Dirichlet values:
-        criterion[m] (1/0) and mazars[m] = (threshold, E, nu, Gf) per
+        + FiberBasedIsotropicLinearDamage, or IsotropicLinearDamage with
+        damage_model=DAMAGE_ISOTROPIC_LINEAR) at the current Dirichlet
+        values: criterion[m] (1/0) and mazars[m] = (threshold, E, nu, Gf) per
         material; kw: fibre_fraction, fractured_at, residual,
-        score_tolerance, first_deferred, max_passes, and the solve options
-        (precision, rtol, ref_eps, ...).  Returns the step's statistics."""
+        score_tolerance, first_deferred, max_passes, damage_model,
+        max_passes_per_step, and the solve options (precision, rtol,
+        ref_eps, ...).  Returns the step's statistics."""
         prm = EventParams()
         lib().amie_gpu_default_event_params(ctypes.byref(prm))
         crit = np.ascontiguousarray(criterion, dtype=np.int32)
@@ -626,7 +630,8 @@ class Batch:
         prm.mazars = maz.ctypes.data
         prm.radius = radius
         prm.kernel = kernel
-        for k in ("fibre_fraction", "fractured_at", "residual", "score_tolerance", "first_deferred", "max_passes"):
+        for k in ("fibre_fraction", "fractured_at", "residual", "score_tolerance", "first_deferred", "max_passes",
+                  "damage_model", "max_passes_per_step"):
             if k in kw:
                 setattr(prm, k, kw.pop(k))
         o = default_opts(**kw)
@@ -640,6 +645,14 @@ class Batch:
         out = np.zeros(self.problems[p].conn.shape[0])
         check(lib().amie_gpu_batch_get_scores(self.h, p, _p(out)))
         return out
+
+    def set_damage(self, d, p=0):
+        """Replace problem p's damage (one value per element); the next
+        assembly uses it."""
+        d = _c(d, np.float64)
+        if d.shape != (self.problems[p].conn.shape[0],):
+            raise ValueError("one damage value per element")
+        check(lib().amie_gpu_batch_set_damage(self.h, p, _p(d)))
 
     def damage(self, p=0):
         ne = self.problems[p].conn.shape[0]

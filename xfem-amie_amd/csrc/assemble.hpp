@@ -1121,6 +1121,126 @@ __global__ __launch_bounds__(256) void k_fiber_commit(EventArgs ev, const uint8_
     atomicAdd(count, 1ull);
 }
 
+// --- IsotropicLinearDamage, Newton branch (physics/damagemodels/
+// isotropiclineardamage.cpp:64-92, postProcess :150-162) -----------------
+//
+// Scores are ordered as unsigned keys (sign bit flipped for positives, all
+// bits for negatives) so that an integer atomicMax gives the double maximum.
+__device__ inline unsigned long long score_key(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : u | (1ull << 63);
+}
+__device__ inline double key_score(unsigned long long k)
+{
+    return __longlong_as_double((long long)((k >> 63) ? k & ~(1ull << 63) : ~k));
+}
+
+// tmax[p] = the largest score over problem p's criterion elements, at least
+// -1 (FeatureTree's maxScoreInit before the pass); tmax starts at key(-1).
+__global__ __launch_bounds__(256) void k_score_max(EventArgs ev, const int32_t *ephase, const int32_t *eprob,
+                                                   const double *score, int n, unsigned long long *tmax)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n || !ev.crit[ephase[e]]) return;
+    atomicMax(tmax + eprob[e], score_key(score[e]));
+}
+
+// One element of the pass.  T = tmax of its problem; e is in the damaging
+// window when T > 0, s > 0 and T - s <= 4 tol.  Its neighbourhood (the
+// second cache, as in k_fiber_decide) splits into the window and the rest;
+// m = the largest score of the rest (max(0, T) when the rest is empty) and
+// delta = max(s - m, T - s).  Unless |T| < tol/2 and (e is out of the window
+// or |s| < tol/20), an element of the window with s > tol/20 is changed by
+// clamp(min(delta, s (1 - d)), 0, 1) (inc[e], flag[e] = 1).  No decision
+// reads damage written in the pass: the increments are added afterwards.
+template <int ET, int LANES>
+__global__ __launch_bounds__(256) void k_newton_decide(NonlocalArgs a, EventArgs ev, const double *score,
+                                                       const double *d, const unsigned long long *tmax, int n,
+                                                       double *inc, uint8_t *flag, unsigned long long *count)
+{
+    AMIE_NO_CONTRACT
+    using E = Elem<ET>;
+    constexpr int D = E::DIM;
+    const int64_t e = LANES == 1 ? (int64_t)blockIdx.x * blockDim.x + threadIdx.x
+                                 : (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = LANES == 1 ? 0 : (int)(threadIdx.x % WAVE);
+    if (e >= n) return;   // uniform over a wavefront when LANES = 64
+    const int ph = a.ephase[e];
+    if (!ev.crit[ph]) {
+        if (!lane) flag[e] = 0;
+        return;
+    }
+    const int p = a.eprob[e];
+    const double T = key_score(tmax[p]);
+    const double s = score[e];
+    const double win = 4. * ev.tol;
+    const bool inset = T > 0. && T - s <= win && s > 0.;
+    double delta = 0.;
+    if (inset) {
+        const double *ge = a.geo + e * GEO_STRIDE;
+        const double c[3] = {ge[0], ge[1], ge[2]};
+        const double rho = ge[7];
+        const double *g = a.grid + (int64_t)p * 4;
+        const int *gd = a.gdim + (int64_t)p * 4;
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int k = 0; k < D; k++) {
+            lo[k] = max(0, (int)floor((c[k] - rho - g[k]) / g[3]));
+            hi[k] = min(gd[k] - 1, (int)floor((c[k] + rho - g[k]) / g[3]));
+        }
+        int found = 0;
+        double m = 0.;
+        for (int z = lo[2]; z <= hi[2]; z++)
+            for (int y = lo[1]; y <= hi[1]; y++)
+                for (int x = lo[0]; x <= hi[0]; x++) {
+                    const int cell = gd[3] + (z * gd[1] + y) * gd[0] + x;
+                    for (int j = a.cell_start[cell] + lane; j < a.cell_start[cell + 1]; j += LANES) {
+                        const int i = a.cell_elem[j];
+                        if (a.ephase[i] != ph) continue;
+                        const double *gi = a.geo + (int64_t)i * GEO_STRIDE;
+                        double d2 = 0.;
+                        for (int k = 0; k < D; k++) d2 += (gi[k] - c[k]) * (gi[k] - c[k]);
+                        const bool in = D == 2 ? d2 < rho * rho : d2 < rho * rho + 2. * rho * 1e-12 + 1e-24;
+                        if (!in) continue;
+                        const double si = score[i];
+                        if (T - si <= win && si > 0.) continue;   // in the damaging window
+                        m = found ? (si < m ? m : si) : si;
+                        found = 1;
+                    }
+                }
+        if constexpr (LANES > 1) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const double tm = __shfl_xor(m, o, WAVE);
+                const int tf = __shfl_xor(found, o, WAVE);
+                if (tf) m = found ? (tm < m ? m : tm) : tm;
+                found |= tf;
+            }
+        }
+        if (!found) m = 0. < T ? T : 0.;
+        delta = s - m < T - s ? T - s : s - m;
+    }
+    if (lane) return;
+    bool ch = false;
+    if (!(fabs(T) < .5 * ev.tol && ((inset && fabs(s) < .05 * ev.tol) || !inset)) && s > .05 * ev.tol && inset) {
+        const double cap = s * (1. - d[e]);
+        const double v = cap < delta ? cap : delta;
+        inc[e] = v < 0. ? 0. : (1. < v ? 1. : v);
+        ch = true;
+        atomicAdd(count, 1ull);
+    }
+    flag[e] = ch ? 1 : 0;
+}
+
+// postProcess: d += inc, capped at 1, for the elements the pass changed.
+__global__ __launch_bounds__(256) void k_newton_commit(const uint8_t *flag, const double *inc, int n, double *d)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n || !flag[e]) return;
+    const double v = d[e] + inc[e];
+    d[e] = 1. < v ? 1. : v;
+}
+
 // Volume average of C_e (1 - d_e) eps_e per problem and right-hand side:
 // one workgroup per problem, fixed summation order.
 template <int ET>

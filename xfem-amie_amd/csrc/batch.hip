@@ -2164,9 +2164,11 @@ int amie_gpu_batch_set_damage(amie_gpu_batch *bb, int32_t p, const double *d)
 {
     if (!bb || check_p(bb->b, p) || !d) return AMIE_ERR_ARG;
     Batch &B = *bb->b;
+    if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
     const ProbHost &P = B.ph[p];
     AMIE_HIP(hipMemcpy(B.edamage.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
     AMIE_HIP(hipMemcpy(B.ed0.as<double>() + P.elem0, d, P.nelem * 8, hipMemcpyHostToDevice));
+    B.has_damage_state = true;
     return AMIE_OK;
 }
 
@@ -2224,6 +2226,8 @@ void amie_gpu_default_event_params(amie_event_params *p)
     p->score_tolerance = 2e-3;
     p->first_deferred = 65;
     p->max_passes = 100000;
+    p->damage_model = AMIE_DAMAGE_FIBRE;
+    p->max_passes_per_step = 200;    // FeatureTree::maxitPerStep (features/features.cpp:202)
 }
 
 int amie_gpu_batch_event_step(amie_gpu_batch *bb, const amie_event_params *prm, const amie_solve_opts *opts,
@@ -2263,6 +2267,21 @@ int amie_gpu_batch_scale_bc(amie_gpu_batch *bb, double factor)
 
 namespace amie {
 
+// the ordered keys of k_score_max, on the host
+static unsigned long long score_key_host(double v)
+{
+    unsigned long long u;
+    std::memcpy(&u, &v, 8);
+    return (u >> 63) ? ~u : u | (1ull << 63);
+}
+static double key_score_host(unsigned long long k)
+{
+    const unsigned long long u = (k >> 63) ? k & ~(1ull << 63) : ~k;
+    double v;
+    std::memcpy(&v, &u, 8);
+    return v;
+}
+
 // One FeatureTree::step of the event-driven damage (the C ABI comment of
 // amie_gpu_batch_event_step says what it restates).
 int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, amie_event_stats *st)
@@ -2272,7 +2291,9 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
     if (!B.has_elements) { set_error("batch has no elements"); return AMIE_ERR_STATE; }
     if (B.et.et == 2) { set_error("event-driven damage: linear elements (P1, Tet4) only"); return AMIE_ERR_ARG; }
     if (prm.nphase <= 0 || !prm.criterion || !prm.mazars || !(prm.radius > 0.) || (prm.kernel != 0 && prm.kernel != 1) ||
-        !(prm.fibre_fraction > 0.) || !(prm.fractured_at > 0.) || !(prm.score_tolerance > 0.) || prm.max_passes <= 0) {
+        !(prm.fibre_fraction > 0.) || !(prm.fractured_at > 0.) || !(prm.score_tolerance > 0.) || prm.max_passes <= 0 ||
+        (prm.damage_model != AMIE_DAMAGE_FIBRE && prm.damage_model != AMIE_DAMAGE_ISOTROPIC_LINEAR) ||
+        prm.max_passes_per_step < 0) {
         set_error("bad event-damage parameters");
         return AMIE_ERR_ARG;
     }
@@ -2291,9 +2312,11 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
             return AMIE_ERR_ARG;
         }
     if (B.nelem >= INT32_MAX) { set_error("too many elements"); return AMIE_ERR_ARG; }
+    const bool newton = prm.damage_model == AMIE_DAMAGE_ISOTROPIC_LINEAR;
+    const double frac_at = newton ? HUGE_VAL : prm.fractured_at;   // IsotropicLinearDamage never fractures
     if (!B.nonlocal_on || B.nl_radius != prm.radius || B.nl_kernel != prm.kernel)
-        if (int rc = set_nonlocal(B, prm.radius, prm.kernel, prm.fractured_at)) return rc;
-    B.nl_threshold = prm.fractured_at;
+        if (int rc = set_nonlocal(B, prm.radius, prm.kernel, frac_at)) return rc;
+    B.nl_threshold = frac_at;
     const int n = (int)B.nelem;
     int rc;
     if ((rc = alloc_and_upload(B.ev_crit, prm.criterion, (size_t)prm.nphase * 4))) return rc;
@@ -2301,8 +2324,10 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
     if (!ensure(B.ev_score, (size_t)n * 8) || !ensure(B.ev_dprev, (size_t)n * 8) || !ensure(B.ev_flags, (size_t)n * 2) ||
         !ensure(B.ev_count, 16) || !ensure(B.ev_tmp, (size_t)n * S * 8) || !ensure(B.ev_sbar, (size_t)n * S * 8))
         return AMIE_ERR_NOMEM;
-    B.frac_at = prm.fractured_at;
-    B.frac_residual = prm.residual;
+    if (newton && (!ensure(B.ev_inc, (size_t)n * 8) || !ensure(B.ev_tmax, (size_t)B.nprob * 8)))
+        return AMIE_ERR_NOMEM;
+    B.frac_at = frac_at;
+    B.frac_residual = newton ? 0. : prm.residual;
 
     EventArgs ev{};
     ev.crit = B.ev_crit.as<int32_t>();
@@ -2311,7 +2336,7 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
     ev.cmat = B.cmat.as<double>();
     ev.prob_e0 = B.prob_e0.as<int32_t>();
     ev.prob_ne = B.prob_ne.as<int32_t>();
-    ev.frac_at = prm.fractured_at;
+    ev.frac_at = frac_at;
     ev.fraction = prm.fibre_fraction;
     ev.tol = prm.score_tolerance;
     ev.first_deferred = prm.first_deferred;
@@ -2346,6 +2371,44 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
             hipLaunchKernelGGL(k_mazars_score<3>, gr, bl, 0, sm, ev, B.ephase.as<int32_t>(), B.eprob.as<int32_t>(),
                                B.edamage.as<double>(), B.ev_sbar.as<double>(), n, B.ev_score.as<double>());
         AMIE_HIP(hipGetLastError());
+        if (newton) {
+            // T per problem, the pass's decisions, then the increments
+            const int np = B.nprob;
+            std::vector<unsigned long long> tkey((size_t)np, score_key_host(-1.));
+            AMIE_HIP(hipMemcpyAsync(B.ev_tmax.p, tkey.data(), (size_t)np * 8, hipMemcpyHostToDevice, sm));
+            hipLaunchKernelGGL(k_score_max, gr, bl, 0, sm, ev, B.ephase.as<int32_t>(), B.eprob.as<int32_t>(),
+                               B.ev_score.as<double>(), n, B.ev_tmax.as<unsigned long long>());
+            AMIE_HIP(hipMemsetAsync(cnt + 1, 0, 8, sm));
+            const dim3 grw((unsigned)((n + 3) / 4));
+            const double *scp = B.ev_score.as<double>();
+            const unsigned long long *tm = B.ev_tmax.as<unsigned long long>();
+            double *incp = B.ev_inc.as<double>();
+            if (B.et.et == 1) {
+                if (B.ev_wave) hipLaunchKernelGGL((k_newton_decide<1, 64>), grw, bl, 0, sm, na, ev, scp,
+                                                  B.edamage.as<double>(), tm, n, incp, inc, cnt + 1);
+                else hipLaunchKernelGGL((k_newton_decide<1, 1>), gr, bl, 0, sm, na, ev, scp, B.edamage.as<double>(),
+                                        tm, n, incp, inc, cnt + 1);
+            } else {
+                if (B.ev_wave) hipLaunchKernelGGL((k_newton_decide<3, 64>), grw, bl, 0, sm, na, ev, scp,
+                                                  B.edamage.as<double>(), tm, n, incp, inc, cnt + 1);
+                else hipLaunchKernelGGL((k_newton_decide<3, 1>), gr, bl, 0, sm, na, ev, scp, B.edamage.as<double>(),
+                                        tm, n, incp, inc, cnt + 1);
+            }
+            hipLaunchKernelGGL(k_newton_commit, gr, bl, 0, sm, inc, B.ev_inc.as<double>(), n, B.edamage.as<double>());
+            AMIE_HIP(hipGetLastError());
+            unsigned long long events = 0;
+            AMIE_HIP(hipMemcpyAsync(&events, cnt + 1, 8, hipMemcpyDeviceToHost, sm));
+            AMIE_HIP(hipMemcpyAsync(tkey.data(), B.ev_tmax.p, (size_t)np * 8, hipMemcpyDeviceToHost, sm));
+            AMIE_HIP(hipStreamSynchronize(sm));
+            bool met = false;   // an element scoring above 0 (FeatureTree's foundCheckPoint)
+            for (unsigned long long k : tkey) met = met || key_score_host(k) > 0.;
+            st->damage_ms += std::chrono::duration<double, std::milli>(clk::now() - t1).count();
+            st->passes++;
+            st->events += (int64_t)events;
+            if (events == 0 && (!met || st->passes > prm.max_passes_per_step)) { st->converged = 1; break; }
+            if (st->passes >= prm.max_passes) break;
+            continue;
+        }
         AMIE_HIP(hipMemcpyAsync(B.ev_dprev.p, B.edamage.p, (size_t)n * 8, hipMemcpyDeviceToDevice, sm));
         AMIE_HIP(hipMemsetAsync(newfrac, 0, (size_t)n, sm));
         // rounds until the set of elements fractured by this pass stops

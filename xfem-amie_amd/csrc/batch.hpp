@@ -163,6 +163,7 @@ struct Batch {
     // damage >= frac_at keeps frac_residual of its stiffness
     double frac_at = HUGE_VAL, frac_residual = 0.;
     DevBuf ev_tmp, ev_sbar, ev_score, ev_dprev, ev_flags, ev_crit, ev_prm, ev_count;
+    DevBuf ev_inc, ev_tmax;   // IsotropicLinearDamage: increments of the pass, largest score per problem
     std::vector<int32_t> h_active, h_active_prob;
     std::vector<ProbDev> h_prob;
     ProbDev *h_prob_pinned = nullptr;
