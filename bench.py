@@ -579,7 +579,7 @@ def main():
     ap.add_argument("--nrve", type=int, default=512)
     ap.add_argument("--nmax", type=int, default=256)
     ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--rtol-sweep", default="1e-9,1e-10,1e-11", help="tighter stopping residuals timed and checked "
+    ap.add_argument("--rtol-sweep", default="5e-9,1e-9,1e-10", help="tighter stopping residuals timed and checked "
                                                                        "for the 1e-7 displacement target ('' to skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

@@ -1070,7 +1070,8 @@ __device__ inline void k2_body(const SolveArgs &a, int t, int slot)
     if (P.status != ST_RUNNING) return;
     const bool ck = slot && P.ckpt_pending;
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
-    W *p = vw<W>(a.p), *r = vw<W>(a.r), *y = vw<W>(a.y);
+    W *p = vw<W>(a.p), *r = vw<W>(a.r);
+    double *y = vw<double>(a.y);   // the iterate since the last refinement: FP64 in every mode
     const W *q = cvw<W>(a.q);
     double part[2 * NR];
 #pragma unroll
@@ -1085,13 +1086,15 @@ __device__ inline void k2_body(const SolveArgs &a, int t, int slot)
             for (int k = 0; k < NR; k++) {
                 if (P.done[k]) continue;
                 const W al = (W)P.alpha[k], be = (W)P.beta[k];
+                const double al64 = P.alpha[k];
                 const size_t idx = k * a.vplane + (size_t)row * B;
-                W pv[B], yv[B];
+                W pv[B];
+                double yv[B];
                 Node<W, B, true>::load(p + idx, pv);
-                Node<W, B, true>::load(y + idx, yv);
+                Node<double, B, true>::load(y + idx, yv);
 #pragma unroll
-                for (int i = 0; i < B; i++) yv[i] += al * pv[i];
-                Node<W, B, true>::store(y + idx, yv);
+                for (int i = 0; i < B; i++) yv[i] += al64 * (double)pv[i];
+                Node<double, B, true>::store(y + idx, yv);
                 if (ck) continue;
                 W qv[B], rv[B], zv[B];
                 Node<W, B, true>::load(q + idx, qv);
@@ -1134,23 +1137,22 @@ __device__ inline void cx_body(const SolveArgs &a, int t)
     const int s = a.tile_slice0[t] + wave;
     if (lane >= a.slice_rows[s]) return;
     const int64_t row = (int64_t)s * WAVE + lane;
-    W *y = vw<W>(a.y);
+    double *y = vw<double>(a.y);
     double sr[B];
     Node<double, B>::load(a.S + (size_t)row * B, sr);
 #pragma unroll
     for (int k = 0; k < NR; k++) {
         const size_t idx = k * a.vplane + (size_t)row * B;
-        W yv[B];
-        double xv[B];
-        Node<W, B>::load(y + idx, yv);
+        double yv[B], xv[B];
+        Node<double, B>::load(y + idx, yv);
         Node<double, B>::load(a.x64 + idx, xv);
 #pragma unroll
         for (int i = 0; i < B; i++) {
-            xv[i] += sr[i] * (double)yv[i];
+            xv[i] += sr[i] * yv[i];
             yv[i] = 0;
         }
         Node<double, B>::store(a.x64 + idx, xv);
-        Node<W, B>::store(y + idx, yv);
+        Node<double, B>::store(y + idx, yv);
     }
 }
 
