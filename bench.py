@@ -10,11 +10,14 @@ matrix in 48-bit fixed point with FP64 accumulation, FP64 refinement) of all
 volume-averaged stress (effective stiffness) of every RVE.
 
 value = RVE load-case solves per second (1536 per step / step time).
-roofline = the SpMV kernel (k_spmv), timed with HIP events on the library
-stream during the timed steps; achieved = the algorithmic bytes of the FP32
-block SpMV (nnzb (b^2 4 + 4) + 4 (nb + 1) + 2 n 4 per right-hand side) of
-every launch over the kernel time, the physical bytes of the stored format
-beside it.
+roofline = the kernel with the larger share of the timed steps, the CG
+update (k_update) or the SpMV (k_spmv), both timed with HIP events on the
+library stream during the timed steps and both reported
+(roofline_update, roofline_spmv).  k_update: the algorithmic bytes of every
+launch (36 B per DOF and right-hand side) over the kernel time.  k_spmv: the
+algorithmic bytes of the FP32 block SpMV (nnzb (b^2 4 + 4) + 4 (nb + 1) +
+2 n 4 per right-hand side) of every launch over the kernel time, the
+physical bytes of the stored format beside it.
 Beside it: the all-FP32 inner iteration, the plain FP64 CG (same kernels),
 the displacement error against the CPU FP64 oracle on a sample of RVEs at
 several stopping residuals, and the same step as one persistent launch per
@@ -663,6 +666,9 @@ def main():
     spmv_ms = sum(s["spmv_kernel_ms"] for s in sts)
     spmv_bytes = sum(s["spmv_bytes"] for s in sts)
     launches = sum(s["spmv_launches"] for s in sts)
+    upd_ms = sum(s["update_kernel_ms"] for s in sts)
+    upd_bytes = sum(s["update_bytes"] for s in sts)
+    upd_launches = sum(s["update_launches"] for s in sts)
     achieved = b8 / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     achieved_phys = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     ok = all(s["status"] == 0 for s in sts)
@@ -724,6 +730,9 @@ def main():
     tr = measured_traffic()
     avg_bytes = spmv_bytes / max(1, launches)
     traffic = tr["ratio"] * avg_bytes if tr else None
+    tr_u = measured_traffic("k_update")
+    avg_upd = upd_bytes / max(1, upd_launches)
+    upd_gbs = upd_bytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
     us_full, by_full = B.time_spmv(headline, ag.JACOBI, 20)
@@ -798,7 +807,8 @@ def main():
                                f"assembly + solve + effective stiffness per step",
                    "rves": len(problems), "load_cases": problems[0].nrhs, "dofs": ndof, "blocks": nnzb,
                    "hbm_bytes": hbm_bytes, "converged": ok},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": None,   # the dominant kernel's, set below
+        "roofline_spmv": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes": "algorithmic bytes of the FP32 block SpMV per launch: nnzb (b^2 4 + 4) + 4 (nb + 1) + "
                               "2 n 4 per right-hand side, over the problems each launch works on",
@@ -834,6 +844,22 @@ def main():
         "voxel_output": voxels,
         "event_damage": events,
     }
+    update = {"bound": "hbm", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": upd_gbs / HBM_PEAK_GBS,
+              "traffic": tr_u["ratio"] * avg_upd if tr_u else None,
+              "bytes": "algorithmic bytes per launch: per DOF and right-hand side, p, q, r read and r, p written "
+                       "(FP32), y read and written (FP64), 36 B; a refinement's slot update 20 B (p read, y read "
+                       "and written); over the problems each launch works on",
+              "traffic_source": (f"FETCH_SIZE+WRITE_SIZE / algorithmic bytes = {tr_u['ratio']} at full occupancy "
+                                 f"({os.path.relpath(TRAFFIC_PROFILE, ROOT)}), times the average algorithmic bytes "
+                                 f"per launch" if tr_u else None),
+              "stream_copy_gbs": copy_gbs,
+              "kernel": "k_update (CG update: y += alpha p, r -= alpha q, p = r + beta p, fused r.r partials)",
+              "launches": upd_launches, "avg_launch_us": 1e3 * upd_ms / max(1, upd_launches),
+              "avg_bytes_per_launch": avg_upd, "kernel_ms": upd_ms, "spmv_kernel_ms": spmv_ms}
+    out["roofline_update"] = update
+    # the roofline line is the kernel with the larger share of the timed steps
+    out["roofline"] = dict(update) if upd_ms > spmv_ms else dict(out["roofline_spmv"])
     print(json.dumps(out), flush=True)
 
 

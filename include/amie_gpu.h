@@ -152,6 +152,9 @@ typedef struct {
     int32_t nproblems_handover;  /* AMIE_MIXED_SPLIT: problems whose SpMV moved to the FP64
                                     matrix at the split matrix's accuracy floor */
     int64_t iterations_handover; /* iterations run after such a hand-over, summed */
+    double update_kernel_ms; /* summed HIP-event time of the CG update kernel (time_kernels) */
+    int64_t update_launches;
+    double update_bytes;     /* algorithmic bytes of the update launches, summed */
 } amie_solve_stats;
 
 /* Damage law of the incremental-loading case (Mazars tension branch with

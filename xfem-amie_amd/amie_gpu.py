@@ -45,7 +45,8 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("status", c_i32), ("nproblems_converged", c_i32), ("iterations_max", c_i64),
                 ("iterations_sum", c_i64), ("refinements", c_i64), ("relres_max", c_dbl), ("solve_ms", c_dbl),
                 ("spmv_kernel_ms", c_dbl), ("spmv_launches", c_i64), ("spmv_bytes", c_dbl),
-                ("nproblems_escalated", c_i32), ("nproblems_handover", c_i32), ("iterations_handover", c_i64)]
+                ("nproblems_escalated", c_i32), ("nproblems_handover", c_i32), ("iterations_handover", c_i64),
+                ("update_kernel_ms", c_dbl), ("update_launches", c_i64), ("update_bytes", c_dbl)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

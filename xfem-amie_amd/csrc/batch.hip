@@ -909,22 +909,26 @@ int Batch::solve_begin(const amie_solve_opts &o)
 static void queue_period(Batch &B, SolveRun &R, hipStream_t s)
 {
     for (int i = 0; i < R.ce; i++) {
-        const bool timed = R.o.time_kernels && !R.exec;
+        // time_kernels: events around the SpMV and around the update
+        bool timed = R.o.time_kernels && !R.exec;
         if (timed) {
-            if ((size_t)(2 * R.nt + 1) >= R.tev.size()) {
+            while ((size_t)(4 * R.nt + 3) >= R.tev.size()) {
                 hipEvent_t ev;
-                if (hipEventCreate(&ev) == hipSuccess) R.tev.push_back(ev);
-                if (hipEventCreate(&ev) == hipSuccess) R.tev.push_back(ev);
+                if (hipEventCreate(&ev) != hipSuccess) break;
+                R.tev.push_back(ev);
             }
-            (void)hipEventRecord(R.tev[2 * R.nt], s);
+            timed = (size_t)(4 * R.nt + 3) < R.tev.size();
         }
+        if (timed) (void)hipEventRecord(R.tev[4 * R.nt], s);
         launch0(R.K.k1, R.gt, s, R.a);
+        if (timed) (void)hipEventRecord(R.tev[4 * R.nt + 1], s);
+        launch0(R.K.red, R.gp, s, R.a);
+        if (timed) (void)hipEventRecord(R.tev[4 * R.nt + 2], s);
+        hipLaunchKernelGGL(R.K.k2, dim3(std::max(1, R.gt)), dim3(WG), 0, s, R.a, i == R.ce - 1 ? 1 : 0);
         if (timed) {
-            (void)hipEventRecord(R.tev[2 * R.nt + 1], s);
+            (void)hipEventRecord(R.tev[4 * R.nt + 3], s);
             R.nt++;
         }
-        launch0(R.K.red, R.gp, s, R.a);
-        hipLaunchKernelGGL(R.K.k2, dim3(std::max(1, R.gt)), dim3(WG), 0, s, R.a, i == R.ce - 1 ? 1 : 0);
     }
     launch0(R.K.cx, R.gt, s, R.a);
     launch0(R.K.cr, R.gt, s, R.a);
@@ -1135,10 +1139,13 @@ int Batch::solve_poll(bool wait, bool *finished)
         return AMIE_ERR_HIP;
     }
     for (int i = 0; i < R.nt; i++) {
-        float ms = 0.f;
-        AMIE_HIP(hipEventElapsedTime(&ms, R.tev[2 * i], R.tev[2 * i + 1]));
+        float ms = 0.f, ms2 = 0.f;
+        AMIE_HIP(hipEventElapsedTime(&ms, R.tev[4 * i], R.tev[4 * i + 1]));
+        AMIE_HIP(hipEventElapsedTime(&ms2, R.tev[4 * i + 2], R.tev[4 * i + 3]));
         R.stats.spmv_kernel_ms += ms;
         R.stats.spmv_launches += 1;
+        R.stats.update_kernel_ms += ms2;
+        R.stats.update_launches += 1;
     }
     bool any_running = false, changed = false;
     for (int p = 0; p < nprob; p++) {
@@ -1167,6 +1174,17 @@ int Batch::solve_poll(bool wait, bool *finished)
                     const double n = (double)(P.iters + (P.status != ST_RUNNING ? 1 : 0));
                     const double nk = std::min(n, (double)P.iters_k64);
                     R.stats.spmv_bytes += R.pbytes[p] * (n - nk) + R.pbytes_k64[p] * nk;
+                    // the update: a refinement's slot update moves y += alpha p
+                    // only (read p, y; write y), every other one reads p, y, q,
+                    // r (and the block-Jacobi inverses) and writes y, r, p; y
+                    // is FP64, the others in the working precision w; every
+                    // right-hand side counted until the problem finishes
+                    const double w = (R.mode == AMIE_FP64 || R.mode == AMIE_MIXED64) ? 8. : 4.;
+                    const double dofs = (double)ph[p].nrows * et.b, nr = (double)nrhs;
+                    const double ck = (double)std::max<int64_t>(0, P.nckpt - 1);
+                    const double full = dofs * nr * (5. * w + 16.) + (R.bj ? dofs * et.b * w : 0.);
+                    const double slot = dofs * nr * (w + 16.);
+                    R.stats.update_bytes += full * std::max(0., (double)P.iters - ck) + slot * std::min(ck, (double)P.iters);
                 }
         // problems whose FP32 iteration stagnated continue from their current
         // iterate in FP64, in a pass of their own (an FP32 copy of a badly
