@@ -522,18 +522,21 @@ def event_leg(ag, configs, ctx, n, n_cpu, no_cpu, loads=(0.8e-4, 1.2e-4)):
     """Event-driven damage (FeatureTree's solve / score / damage-pass loop
     with NonLocalMazars and fibre-based damage) on the event square: the
     device at n x n, and at n_cpu x n_cpu beside the oracle's loop on the
-    same case (the oracle's damage must come out equal)."""
+    same case (the oracle's damage must come out equal).  The same square
+    with IsotropicLinearDamage (its Newton branch) at n x n, and its first
+    load step at n_cpu x n_cpu beside the oracle's (equal to 1e-8; later
+    steps amplify solver round-off and are not compared)."""
     import oracle as orc
 
-    def device(m):
+    def device(m, model=ag.DAMAGE_FIBRE, steps=loads):
         prob, dofs, nfix, crit, maz, radius = configs.event_square(m)
         B = ag.Batch(ctx, [prob])
         rows, t0 = [], time.perf_counter()
-        for v in loads:
+        for v in steps:
             vals = np.zeros(len(dofs))
             vals[nfix:] = v
             B.set_bc_values(dofs, vals)
-            st = B.event_step(crit, maz, radius, precision=ag.MIXED_FIX48, rtol=1e-12)
+            st = B.event_step(crit, maz, radius, precision=ag.MIXED_FIX48, rtol=1e-12, damage_model=model)
             rows.append(st)
         wall = time.perf_counter() - t0
         d = B.damage()[0]
@@ -571,6 +574,24 @@ def event_leg(ag, configs, ctx, n, n_cpu, no_cpu, loads=(0.8e-4, 1.2e-4)):
                              "note": "the oracle's loop: OpenMP PCG solves, brute-force neighbourhoods"}
         out["device_vs_oracle_same_case"] = wall / max(1e-9, small["wall_s"])
         log(f"event damage, oracle {n_cpu}^2: {out['cpu_oracle']}")
+    newton, _ = device(n, ag.DAMAGE_ISOTROPIC_LINEAR)
+    out[f"isotropic_linear_n{n}"] = newton
+    log(f"event damage, IsotropicLinearDamage {n}^2: {newton}")
+    if not no_cpu:
+        first, d_dev = device(n_cpu, ag.DAMAGE_ISOTROPIC_LINEAR, loads[:1])
+        prob, dofs, nfix, crit, maz, radius = configs.event_square(n_cpu)
+        vals = np.zeros(len(dofs))
+        vals[nfix:] = loads[0]
+        t0 = time.perf_counter()
+        d, _, _, pp = orc.event_damage(orc.P1, prob.conn, prob.coords, prob.cmats, prob.phase, crit, maz, dofs,
+                                       vals, np.zeros(prob.conn.shape[0]), radius, model="newton")
+        wall = time.perf_counter() - t0
+        out[f"isotropic_linear_first_step_n{n_cpu}"] = {
+            "device": first, "cpu_oracle": {"passes": pp, "wall_s": wall, "ms_per_pass": 1e3 * wall / max(1, pp),
+                                            "cores": orc.get_threads(), "kind": "port"},
+            "damage_max_diff": float(np.abs(d - d_dev).max()),
+            "device_vs_oracle_same_case": wall / max(1e-9, first["wall_s"])}
+        log(f"event damage, IsotropicLinearDamage first step {n_cpu}^2: {out[f'isotropic_linear_first_step_n{n_cpu}']}")
     return out
 
 
