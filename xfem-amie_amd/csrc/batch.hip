@@ -2366,6 +2366,10 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
     amie_solve_opts o = o_;
     o.use_x0 = 1;
     o.use_graph = 0;   // every pass would rebuild the graphs for its active set
+    // the solves of small meshes are latency-bound: one persistent launch per
+    // solve pass saves 10-25 % of a pass up to a few dozen tiles (n = 24..48
+    // squares) and is bitwise the launch path; larger ones keep launches
+    if (o.persistent == 0 && B.ntiles <= 32) o.persistent = 1 << 20;
     using clk = std::chrono::steady_clock;
     for (;;) {
         const auto t0 = clk::now();
