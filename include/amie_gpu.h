@@ -460,8 +460,13 @@ int amie_gpu_batch_set_damage(amie_gpu_batch *b, int32_t p, const double *d);
  * neighbourhood (FractureCriterion::getMaxScoreInNeighbourhood,
  * fracturecriterion.cpp:251-280) takes d += fibre_fraction } until a pass
  * changes nothing.  Linear elements (P1, Tet4), right-hand side 0.  With
- * opts.persistent 0, a batch of at most 32 tiles (8 192 block rows) solves
- * with persistent launches (the same results bit for bit, less latency). */
+ * opts.persistent 0 the solves of small meshes take the shorter-latency
+ * paths: a batch with one right-hand side whose every problem has at most
+ * 4 096 block rows runs each problem's whole CG in one workgroup, in FP64
+ * whatever opts.precision asks (same stopping rule: FP64 relative residual
+ * <= opts.rtol); otherwise a batch of at most 32 tiles (8 192 block rows)
+ * solves with persistent launches (the launch path's results bit for
+ * bit). */
 typedef struct {
     int32_t nphase;              /* materials the two arrays below cover (>= every problem's nphase) */
     const int32_t *criterion;    /* per material: 1 = StiffnessAndFracture + NonLocalMazars, 0 = elastic */

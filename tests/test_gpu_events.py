@@ -223,7 +223,8 @@ def test_newton_later_steps_and_batch(ctx):
     """All three load steps, two copies of the case in one batch: the copies
     bitwise equal; the first step as the reference's; later steps, whose
     element-by-element trajectory amplifies 1e-10 score differences to 1e-2,
-    within the reference's totals (damage sum to 0.5 %, passes to 2 %) and
+    within the reference's totals (damage sum to 0.5 %, passes to 5 %, as
+    far as the oracle's own runs at solver rtol 1e-12 and 1e-10 differ) and
     within 1.5 times the distance between the device's own runs at solver
     rtol 1e-12 and 1e-10."""
     case = ev.load(ev.NEWTON)
@@ -244,7 +245,7 @@ def test_newton_later_steps_and_batch(ctx):
         gap_self = np.abs(d[0] - coarse[k][1][0]).mean()
         assert gap_ref < 1.5 * gap_self, (k, gap_ref, gap_self)
         assert abs(d[0].sum() - ref.sum()) <= 5e-3 * ref.sum(), k
-        assert abs(st["passes"] - case["passes"][k]) <= 0.02 * case["passes"][k], (k, st["passes"])
+        assert abs(st["passes"] - case["passes"][k]) <= 0.05 * case["passes"][k], (k, st["passes"])
 
 
 def test_newton_rejects_bad_model(ctx):
@@ -255,3 +256,24 @@ def test_newton_rejects_bad_model(ctx):
     with pytest.raises(ag.AmieError):
         B.event_step(case["crit"], case["prm"], case["radius"], max_passes_per_step=-1, **NEWTON)
     B.close()
+
+
+@pytest.mark.parametrize("tag", ev.TAGS)
+def test_event_step_small_and_batch_solvers_agree(ctx, tag):
+    """The reference cases are small enough for the one-workgroup FP64 CG
+    (opts.persistent 0); with persistent launches requested the batch solver
+    runs instead.  Both give the reference's damage and pass counts, and
+    displacements within 1e-9 of each other."""
+    case = ev.load(tag)
+    out = {}
+    for name, kw in (("small", {}), ("batch", dict(persistent=64))):
+        B = ag.Batch(ctx, [problem(case)])
+        out[name] = run(B, case, precision=ag.FP64, rtol=1e-12, **kw)
+        B.close()
+    for k in range(len(case["loads"])):
+        for name in out:
+            st, d, sc, x = out[name][k]
+            assert st["passes"] == case["passes"][k], (name, k, st)
+            np.testing.assert_allclose(d[0], case["damage"][k], rtol=0, atol=1e-12, err_msg=f"{name} step {k}")
+        xs, xb = out["small"][k][3][0], out["batch"][k][3][0]
+        assert np.linalg.norm(xs - xb) <= 1e-9 * np.linalg.norm(xb), k

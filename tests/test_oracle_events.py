@@ -232,8 +232,9 @@ def test_newton_later_steps_within_solver_sensitivity(newton):
     differ reaches 1e-2 within a step, so the trajectory is not reproducible
     element by element, by the reference either.  What is pinned: every
     pass (the replay above), the step's totals (damage sum to 0.5 %, passes
-    to 2 %) and, element by element, a distance to the reference's damage of
-    the size the solver tolerance alone makes: within 1.5 times the
+    to 5 %, as far as the oracle's own runs at solver rtol 1e-12 and 1e-10
+    differ) and, element by element, a distance to the reference's damage
+    of the size the solver tolerance alone makes: within 1.5 times the
     distance between the oracle's own runs at solver rtol 1e-12 and 1e-10."""
     c = newton
     fine = newton_steps(c)
@@ -244,7 +245,7 @@ def test_newton_later_steps_within_solver_sensitivity(newton):
         gap_self = np.abs(fine[k][0] - coarse[k][0]).mean()
         assert gap_ref < 1.5 * gap_self, (k, gap_ref, gap_self)
         assert abs(fine[k][0].sum() - ref.sum()) <= 5e-3 * ref.sum(), k
-        assert abs(fine[k][3] - c["passes"][k]) <= 0.02 * c["passes"][k], (k, fine[k][3])
+        assert abs(fine[k][3] - c["passes"][k]) <= 0.05 * c["passes"][k], (k, fine[k][3])
     np.testing.assert_allclose(fine[0][0], c["damage"][0], rtol=0, atol=1e-8)
 
 

@@ -2300,6 +2300,64 @@ static double key_score_host(unsigned long long k)
     return v;
 }
 
+// The one-workgroup FP64 CG (k_cg_small) takes batches whose every problem
+// has at most SMALL_SLICES slices (4 096 block rows) and one right-hand side:
+// its solves are bound by the latency of an iteration's phases, which a
+// workgroup barrier separates far more cheaply than a kernel boundary.
+constexpr int SMALL_SLICES = 64;
+
+static bool small_solvable(const Batch &B, const amie_solve_opts &o)
+{
+    if (!B.has_elements || B.nrhs != 1 || o.ref_eps > 0.) return false;
+    for (int p = 0; p < B.nprob; p++)
+        if (B.ph[p].nslices > SMALL_SLICES) return false;
+    return true;
+}
+
+// Solves every problem with k_cg_small (one workgroup each) to o.rtol from
+// x64 (use_x0) or zero; returns AMIE_OK, AMIE_NOT_CONVERGED or AMIE_ERR_NAN
+// and the largest iteration count.
+static int solve_small(Batch &B, const amie_solve_opts &o, int64_t *iters_max)
+{
+    const int np = B.nprob;
+    std::vector<int2> d((size_t)np);
+    int64_t maxit = 0;
+    for (int p = 0; p < np; p++) {
+        d[p] = make_int2(B.ph[p].slice0, B.ph[p].nslices);
+        maxit = std::max<int64_t>(maxit, o.maxit > 0 ? o.maxit : std::max<int64_t>(2 * B.ph[p].nrows * B.et.b, 1000));
+    }
+    if (!ensure(B.small_desc, (size_t)np * sizeof(int2)) || !ensure(B.small_out, (size_t)np * sizeof(SmallOut)))
+        return AMIE_ERR_NOMEM;
+    hipStream_t s = B.stream;
+    AMIE_HIP(hipMemcpyAsync(B.small_desc.p, d.data(), (size_t)np * sizeof(int2), hipMemcpyHostToDevice, s));
+    const SolveArgs a = B.args(AMIE_FP64);
+    const double rtol = o.rtol > 0 ? o.rtol : 1e-8;
+    if (B.et.b == 2)
+        hipLaunchKernelGGL(k_cg_small<2>, dim3(np), dim3(WG1), 0, s, a, B.small_desc.as<int2>(), rtol, maxit,
+                           o.use_x0 ? 1 : 0, B.small_out.as<SmallOut>());
+    else
+        hipLaunchKernelGGL(k_cg_small<3>, dim3(np), dim3(WG1), 0, s, a, B.small_desc.as<int2>(), rtol, maxit,
+                           o.use_x0 ? 1 : 0, B.small_out.as<SmallOut>());
+    AMIE_HIP(hipGetLastError());
+    std::vector<SmallOut> out((size_t)np);
+    AMIE_HIP(hipMemcpyAsync(out.data(), B.small_out.p, (size_t)np * sizeof(SmallOut), hipMemcpyDeviceToHost, s));
+    AMIE_HIP(hipStreamSynchronize(s));
+    int rc = AMIE_OK;
+    *iters_max = 0;
+    for (int p = 0; p < np; p++) {
+        // the problem's record as a batch solve leaves it (amie_gpu_batch_problem_info)
+        ProbDev &P = B.h_prob[p];
+        P.status = out[p].status;
+        P.iters = out[p].iters;
+        P.nckpt = 0;
+        P.relres_out[0] = out[p].relres;
+        *iters_max = std::max(*iters_max, out[p].iters);
+        if (out[p].status == ST_NAN) rc = AMIE_ERR_NAN;
+        else if (out[p].status != ST_CONVERGED && rc == AMIE_OK) rc = AMIE_NOT_CONVERGED;
+    }
+    return rc;
+}
+
 // One FeatureTree::step of the event-driven damage (the C ABI comment of
 // amie_gpu_batch_event_step says what it restates).
 int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, amie_event_stats *st)
@@ -2366,18 +2424,28 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
     amie_solve_opts o = o_;
     o.use_x0 = 1;
     o.use_graph = 0;   // every pass would rebuild the graphs for its active set
-    // the solves of small meshes are latency-bound: one persistent launch per
-    // solve pass saves 10-25 % of a pass up to a few dozen tiles (n = 24..48
-    // squares) and is bitwise the launch path; larger ones keep launches
+    // the solves of small meshes are latency-bound: a problem of at most
+    // 4 096 block rows runs its whole CG in one workgroup (FP64, whatever
+    // the precision asked for); otherwise one persistent launch per solve
+    // pass saves 10-25 % of a pass up to a few dozen tiles and is bitwise
+    // the launch path; larger batches keep a launch per kernel
+    const bool small = o.persistent == 0 && small_solvable(B, o);
     if (o.persistent == 0 && B.ntiles <= 32) o.persistent = 1 << 20;
     using clk = std::chrono::steady_clock;
     for (;;) {
         const auto t0 = clk::now();
         if ((rc = B.assemble())) return rc;
-        amie_solve_stats ss{};
-        rc = B.solve(o, &ss);
-        if (rc < 0) return rc;
-        st->cg_iterations += ss.iterations_max;
+        if (small) {
+            int64_t itmax = 0;
+            rc = solve_small(B, o, &itmax);
+            if (rc < 0 && rc != AMIE_ERR_NAN) return rc;
+            st->cg_iterations += itmax;
+        } else {
+            amie_solve_stats ss{};
+            rc = B.solve(o, &ss);
+            if (rc < 0) return rc;
+            st->cg_iterations += ss.iterations_max;
+        }
         const auto t1 = clk::now();
         st->solve_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
         if (rc != AMIE_OK) {   // a solve that did not converge ends the step

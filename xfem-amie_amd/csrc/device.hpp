@@ -1234,6 +1234,207 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
     cr_body<W, B, NR, BJ>(a, t);
 }
 
+// ------------------------------------------- one-workgroup CG (small meshes)
+// The whole FP64 CG of one problem in one workgroup of WG1 threads: the
+// phases of an iteration are separated by workgroup barriers instead of
+// kernel boundaries or device-wide barriers, which is what the solves of a
+// few thousand block rows are bound by (the passes of the event-driven
+// damage).  The scaled system S K S (symmetric Jacobi, point Jacobi after
+// scaling) on the unscaled FP64 planes, right-hand side 0; x64 in place
+// (warm start when use_x0).  The recurrence residual decides when to check,
+// the FP64 relative residual ||b - K x|| / ||b|| <= rtol decides
+// convergence, and the checked residual replaces the recurrence's, as the
+// batch solve does.  Every sum runs in a fixed order.
+constexpr int WG1 = 1024;
+struct SmallOut {
+    int64_t iters;
+    double relres;
+    int32_t status;   // ST_CONVERGED, ST_MAXIT or ST_NAN
+    int32_t pad;
+};
+
+// One block row of the FP64 planes times a gathered vector, with ordinary
+// (cached) loads: the one-workgroup solve reads the same few hundred KB of
+// matrix every iteration, which stay in L2.  Same operation order as
+// spmv_row (blocks in slot order, column-major within a block).
+template <int B, typename G>
+__device__ inline void row_product64(const SolveArgs &a, int s, int lane, double *acc, G gather)
+{
+    // four blocks in flight (all of a row's 7-9 at once spills registers
+    // at 1 024 threads and is slower)
+    constexpr int U = 4;
+    const int w = a.slice_width[s];
+    const int64_t sp = a.slice_ptr[s] + lane, ns = a.nslots;
+    const d2v *pl = reinterpret_cast<const d2v *>(a.v64);
+    for (int k = 0; k < w; k += U) {
+        uint32_t c[U];
+        double m[U][B * B];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) c[u] = a.col[sp + (int64_t)(k + u) * WAVE];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) {
+                const int64_t sl = sp + (int64_t)(k + u) * WAVE;
+                const d2v v0 = pl[sl], v1 = pl[ns + sl];
+                m[u][0] = v0.x; m[u][1] = v0.y; m[u][2] = v1.x; m[u][3] = v1.y;
+                if constexpr (B == 3) {
+                    const d2v v2 = pl[2 * ns + sl], v3 = pl[3 * ns + sl];
+                    m[u][4] = v2.x; m[u][5] = v2.y; m[u][6] = v3.x; m[u][7] = v3.y;
+                    m[u][8] = reinterpret_cast<const double *>(pl + 4 * ns)[sl];
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) {
+                double xv[B];
+                gather(c[u], xv);
+#pragma unroll
+                for (int j = 0; j < B; j++)
+#pragma unroll
+                    for (int i = 0; i < B; i++) acc[i] += m[u][j * B + i] * xv[j];
+            }
+    }
+}
+
+template <int B>
+__global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc, double rtol, int64_t maxit,
+                                                   int use_x0, SmallOut *out)
+{
+    __shared__ double red[2][WG1 / WAVE];
+    int rb = 0;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // the workgroup's sum, the same bits in every thread (butterfly within
+    // the wavefront, then the wavefront sums in order; two buffers, so a
+    // sum's reads finish before the buffer is written again two sums later)
+    auto wg_sum = [&](double v) -> double {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+        if (lane == 0) red[rb][wave] = v;
+        __syncthreads();
+        double t = 0.;
+        for (int i = 0; i < WG1 / WAVE; i++) t += red[rb][i];
+        rb ^= 1;
+        return t;
+    };
+    const int2 d = desc[blockIdx.x];   // first slice, slices
+    double *x = a.x64, *r = vw<double>(a.r), *p = vw<double>(a.p), *q = vw<double>(a.q);
+    const double *bv = a.b64, *S = a.S;
+    // own rows: slices wave, wave + WG1 / WAVE, ... of the problem
+    auto rows = [&](auto f) {
+        for (int s = d.x + wave; s < d.x + d.y; s += WG1 / WAVE)
+            if (lane < a.slice_rows[s]) f(s, (int64_t)s * WAVE + lane);
+    };
+    // r = S (b - K x) for own rows; returns the partials |b - K x|^2, |r|^2
+    auto residual = [&](double &rr64, double &rho) {
+        rr64 = rho = 0.;
+        rows([&](int s, int64_t row) {
+            double acc[B];
+#pragma unroll
+            for (int i = 0; i < B; i++) acc[i] = 0.;
+            row_product64<B>(a, s, lane, acc, [&](uint32_t c, double *xv) {
+#pragma unroll
+                for (int j = 0; j < B; j++) xv[j] = x[(size_t)c * B + j];
+            });
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                const size_t k = (size_t)row * B + i;
+                const double r64 = bv[k] - acc[i];
+                const double rs = S[k] * r64;
+                rr64 += r64 * r64;
+                rho += rs * rs;
+                r[k] = rs;
+            }
+        });
+    };
+    double bb = 0.;
+    rows([&](int, int64_t row) {
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            const size_t k = (size_t)row * B + i;
+            bb += bv[k] * bv[k];
+            if (!use_x0) x[k] = 0.;
+        }
+    });
+    bb = wg_sum(bb);   // its barrier also orders the zeroed x before the residual's gathers
+    double rr64, rho;
+    residual(rr64, rho);
+    rows([&](int, int64_t row) {
+#pragma unroll
+        for (int i = 0; i < B; i++) p[(size_t)row * B + i] = r[(size_t)row * B + i];
+    });
+    rr64 = wg_sum(rr64);
+    rho = wg_sum(rho);
+    double rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
+    int64_t it = 0;
+    int status = rel <= rtol ? ST_CONVERGED : ST_RUNNING;
+    double target = rel > 0. ? rho * (rtol / rel) * (rtol / rel) : 0.;
+    while (status == ST_RUNNING) {
+        // q = S K S p, partial p.q
+        double pq = 0.;
+        rows([&](int s, int64_t row) {
+            double acc[B];
+#pragma unroll
+            for (int i = 0; i < B; i++) acc[i] = 0.;
+            row_product64<B>(a, s, lane, acc, [&](uint32_t c, double *xv) {
+#pragma unroll
+                for (int j = 0; j < B; j++) xv[j] = S[(size_t)c * B + j] * p[(size_t)c * B + j];
+            });
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                const size_t k = (size_t)row * B + i;
+                const double qv = S[k] * acc[i];
+                q[k] = qv;
+                pq += p[k] * qv;
+            }
+        });
+        pq = wg_sum(pq);   // every gather of p is done past this barrier
+        const double alpha = pq != 0. ? rho / pq : 0.;
+        double rho1 = 0.;
+        rows([&](int, int64_t row) {
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                const size_t k = (size_t)row * B + i;
+                x[k] += S[k] * (alpha * p[k]);
+                const double rv = r[k] - alpha * q[k];
+                r[k] = rv;
+                rho1 += rv * rv;
+            }
+        });
+        rho1 = wg_sum(rho1);   // x complete past this barrier
+        it++;
+        if (!(rho1 == rho1) || !(pq == pq)) {
+            status = ST_NAN;
+            break;
+        }
+        if (rho1 <= target || it >= maxit) {
+            residual(rr64, rho1);   // the checked residual replaces the recurrence's
+            rr64 = wg_sum(rr64);
+            rho1 = wg_sum(rho1);
+            rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
+            if (!(rel == rel)) { status = ST_NAN; break; }
+            if (rel <= rtol) { status = ST_CONVERGED; break; }
+            if (it >= maxit) { status = ST_MAXIT; break; }
+            target = rho1 * (rtol / rel) * (rtol / rel);
+        }
+        const double beta = rho != 0. ? rho1 / rho : 0.;
+        rows([&](int, int64_t row) {
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                const size_t k = (size_t)row * B + i;
+                p[k] = r[k] + beta * p[k];
+            }
+        });
+        rho = rho1;
+        __syncthreads();   // p complete before the next product's gathers
+    }
+    if (threadIdx.x == 0) {
+        out[blockIdx.x].iters = it;
+        out[blockIdx.x].relres = rel;
+        out[blockIdx.x].status = status;
+    }
+}
+
 // ------------------------------------------------------ persistent solve
 // One launch runs whole checkpoint periods: the phases of queue_period (K1,
 // R, K2 per iteration, then C1 and C2) as loops over the same per-tile and
