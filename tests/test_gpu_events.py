@@ -277,3 +277,35 @@ def test_event_step_small_and_batch_solvers_agree(ctx, tag):
             np.testing.assert_allclose(d[0], case["damage"][k], rtol=0, atol=1e-12, err_msg=f"{name} step {k}")
         xs, xb = out["small"][k][3][0], out["batch"][k][3][0]
         assert np.linalg.norm(xs - xb) <= 1e-9 * np.linalg.norm(xb), k
+
+
+def test_event_step_tet4_vs_oracle(ctx):
+    """A Tet4 cube, left face clamped, right face stretched and sheared (the
+    3D scores with the compressive branch; the one-workgroup solver with
+    3 x 3 blocks): two load steps of the device's fibre-model event step
+    equal the oracle's loop on the same mesh, damage exactly and passes
+    equal.  (The 3D neighbourhoods are the oracle's centroid balls, not the
+    reference's, so this pins device against oracle only.)"""
+    import oracle as orc
+    coords, conn = ag.structured_mesh(ag.TET4, 4)
+    x = coords[:, 0]
+    left, right = np.nonzero(np.isclose(x, 0.))[0], np.nonzero(np.isclose(x, 1.))[0]
+    dofs = np.concatenate([3 * left, 3 * left + 1, 3 * left + 2, 3 * right, 3 * right + 1]).astype(np.int64)
+    C = ag.cauchy_green(1e10, 0.2, dim=3)
+    phase = np.zeros(conn.shape[0], dtype=np.int32)
+    crit, prm, radius = np.array([1], dtype=np.int32), np.array([[1e-4, 1e10, 0.2, 100.]]), 0.15
+    prob = ag.Problem(ag.TET4, coords, conn, C[None], phase=phase, bc_dofs=dofs, bc_vals=np.zeros(len(dofs)))
+    B = ag.Batch(ctx, [prob])
+    d_or = np.zeros(conn.shape[0])
+    for load in (1.4e-4, 2e-4):
+        vals = np.concatenate([np.zeros(3 * len(left)), np.full(len(right), load), np.full(len(right), 0.5 * load)])
+        print("tet4", load, flush=True)
+        B.set_bc_values(dofs, vals)
+        st = B.event_step(crit, prm, radius, precision=ag.FP64, rtol=1e-12)
+        d_or, _, sc, passes = orc.event_damage(orc.TET4, conn, coords, C[None], phase, crit, prm, dofs, vals, d_or,
+                                               radius)
+        assert not np.isnan(sc).any()
+        assert st["status"] == 0 and st["passes"] == passes, (st, passes)
+        np.testing.assert_allclose(B.damage()[0], d_or, rtol=0, atol=1e-12)
+    assert d_or.max() > 0
+    B.close()
