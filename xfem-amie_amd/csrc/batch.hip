@@ -2301,16 +2301,18 @@ static double key_score_host(unsigned long long k)
 }
 
 // The one-workgroup FP64 CG (k_cg_small) takes batches whose every problem
-// has at most SMALL_SLICES slices (4 096 block rows) and one right-hand side:
-// its solves are bound by the latency of an iteration's phases, which a
-// workgroup barrier separates far more cheaply than a kernel boundary.
-constexpr int SMALL_SLICES = 64;
+// has at most small_slices() slices and one right-hand side: its solves are
+// bound by the latency of an iteration's phases, which a workgroup barrier
+// separates far more cheaply than a kernel boundary.  A thread keeps up to
+// MAXJ rows in registers, as many as fit without spilling: 2 (2 048 block
+// rows) for 2 x 2 blocks, 1 (1 024) for 3 x 3.
+static int small_slices(const Batch &B) { return (B.et.b == 2 ? 2 : 1) * (WG1 / WAVE); }
 
 static bool small_solvable(const Batch &B, const amie_solve_opts &o)
 {
     if (!B.has_elements || B.nrhs != 1 || o.ref_eps > 0.) return false;
     for (int p = 0; p < B.nprob; p++)
-        if (B.ph[p].nslices > SMALL_SLICES) return false;
+        if (B.ph[p].nslices > small_slices(B)) return false;
     return true;
 }
 
@@ -2332,12 +2334,17 @@ static int solve_small(Batch &B, const amie_solve_opts &o, int64_t *iters_max)
     AMIE_HIP(hipMemcpyAsync(B.small_desc.p, d.data(), (size_t)np * sizeof(int2), hipMemcpyHostToDevice, s));
     const SolveArgs a = B.args(AMIE_FP64);
     const double rtol = o.rtol > 0 ? o.rtol : 1e-8;
-    if (B.et.b == 2)
-        hipLaunchKernelGGL(k_cg_small<2>, dim3(np), dim3(WG1), 0, s, a, B.small_desc.as<int2>(), rtol, maxit,
-                           o.use_x0 ? 1 : 0, B.small_out.as<SmallOut>());
+    int maxs = 0;
+    for (int p = 0; p < np; p++) maxs = std::max(maxs, B.ph[p].nslices);
+    const int2 *dd = B.small_desc.as<int2>();
+    SmallOut *so = B.small_out.as<SmallOut>();
+    const int x0 = o.use_x0 ? 1 : 0;
+    if (B.et.b == 3)
+        hipLaunchKernelGGL((k_cg_small<3, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
+    else if (maxs <= WG1 / WAVE)
+        hipLaunchKernelGGL((k_cg_small<2, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
     else
-        hipLaunchKernelGGL(k_cg_small<3>, dim3(np), dim3(WG1), 0, s, a, B.small_desc.as<int2>(), rtol, maxit,
-                           o.use_x0 ? 1 : 0, B.small_out.as<SmallOut>());
+        hipLaunchKernelGGL((k_cg_small<2, 2>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
     AMIE_HIP(hipGetLastError());
     std::vector<SmallOut> out((size_t)np);
     AMIE_HIP(hipMemcpyAsync(out.data(), B.small_out.p, (size_t)np * sizeof(SmallOut), hipMemcpyDeviceToHost, s));

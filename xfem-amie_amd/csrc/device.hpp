@@ -1245,6 +1245,12 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
 // the FP64 relative residual ||b - K x|| / ||b|| <= rtol decides
 // convergence, and the checked residual replaces the recurrence's, as the
 // batch solve does.  Every sum runs in a fixed order.
+//
+// A thread keeps its rows' x, r, q, p and S in registers (slices wave,
+// wave + 16, ... up to MAXJ of them): the only global stores of an
+// iteration are p's, which the next product gathers, so the barriers of the
+// two sums wait for no store.  x goes to memory for a residual check and at
+// the end.
 constexpr int WG1 = 1024;
 struct SmallOut {
     int64_t iters;
@@ -1253,16 +1259,11 @@ struct SmallOut {
     int32_t pad;
 };
 
-// One block row of the FP64 planes times a gathered vector, with ordinary
-// (cached) loads: the one-workgroup solve reads the same few hundred KB of
-// matrix every iteration, which stay in L2.  Same operation order as
-// spmv_row (blocks in slot order, column-major within a block).
-template <int B, typename G>
+template <int B, int U, typename G>
 __device__ inline void row_product64(const SolveArgs &a, int s, int lane, double *acc, G gather)
 {
-    // four blocks in flight (all of a row's 7-9 at once spills registers
-    // at 1 024 threads and is slower)
-    constexpr int U = 4;
+    // U blocks in flight (all of a row's 7-9 at once spills registers at
+    // 1 024 threads and is slower)
     const int w = a.slice_width[s];
     const int64_t sp = a.slice_ptr[s] + lane, ns = a.nslots;
     const d2v *pl = reinterpret_cast<const d2v *>(a.v64);
@@ -1297,11 +1298,16 @@ __device__ inline void row_product64(const SolveArgs &a, int s, int lane, double
     }
 }
 
-template <int B>
+template <int B, int MAXJ>
 __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc, double rtol, int64_t maxit,
                                                    int use_x0, SmallOut *out)
 {
     __shared__ double red[2][WG1 / WAVE];
+    constexpr int NW = WG1 / WAVE;
+    // blocks in flight per row: as many as the 128 registers a lane has at
+    // 1 024 threads allow beside the rows kept
+    constexpr int U = (B == 2 && MAXJ == 1) ? 4 : 2;
+    static_assert(B * MAXJ <= 4, "the rows kept and the blocks in flight must fit 128 registers");
     int rb = 0;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // the workgroup's sum, the same bits in every thread (butterfly within
@@ -1313,57 +1319,72 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
         if (lane == 0) red[rb][wave] = v;
         __syncthreads();
         double t = 0.;
-        for (int i = 0; i < WG1 / WAVE; i++) t += red[rb][i];
+        for (int i = 0; i < NW; i++) t += red[rb][i];
         rb ^= 1;
         return t;
     };
-    const int2 d = desc[blockIdx.x];   // first slice, slices
-    double *x = a.x64, *r = vw<double>(a.r), *p = vw<double>(a.p), *q = vw<double>(a.q);
+    const int2 d = desc[blockIdx.x];   // first slice, slices (at most MAXJ * NW)
+    double *xg = a.x64, *pg = vw<double>(a.p);
     const double *bv = a.b64, *S = a.S;
-    // own rows: slices wave, wave + WG1 / WAVE, ... of the problem
-    auto rows = [&](auto f) {
-        for (int s = d.x + wave; s < d.x + d.y; s += WG1 / WAVE)
-            if (lane < a.slice_rows[s]) f(s, (int64_t)s * WAVE + lane);
-    };
-    // r = S (b - K x) for own rows; returns the partials |b - K x|^2, |r|^2
+    bool own[MAXJ];
+    int sl[MAXJ];
+    size_t rw[MAXJ];   // first vector entry of the row
+    double x[MAXJ][B], r[MAXJ][B], p[MAXJ][B], q[MAXJ][B], sc[MAXJ][B];
+#pragma unroll
+    for (int j = 0; j < MAXJ; j++) {
+        sl[j] = d.x + wave + j * NW;
+        own[j] = sl[j] < d.x + d.y && lane < a.slice_rows[sl[j]];
+        rw[j] = ((size_t)sl[j] * WAVE + lane) * B;
+#pragma unroll
+        for (int i = 0; i < B; i++) x[j][i] = r[j][i] = p[j][i] = q[j][i] = sc[j][i] = 0.;
+    }
+    // r = S (b - K x) from x in memory; partials |b - K x|^2 and |r|^2
     auto residual = [&](double &rr64, double &rho) {
         rr64 = rho = 0.;
-        rows([&](int s, int64_t row) {
-            double acc[B];
 #pragma unroll
-            for (int i = 0; i < B; i++) acc[i] = 0.;
-            row_product64<B>(a, s, lane, acc, [&](uint32_t c, double *xv) {
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j]) {
+                double acc[B];
 #pragma unroll
-                for (int j = 0; j < B; j++) xv[j] = x[(size_t)c * B + j];
-            });
+                for (int i = 0; i < B; i++) acc[i] = 0.;
+                row_product64<B, U>(a, sl[j], lane, acc, [&](uint32_t c, double *xv) {
 #pragma unroll
-            for (int i = 0; i < B; i++) {
-                const size_t k = (size_t)row * B + i;
-                const double r64 = bv[k] - acc[i];
-                const double rs = S[k] * r64;
-                rr64 += r64 * r64;
-                rho += rs * rs;
-                r[k] = rs;
+                    for (int i = 0; i < B; i++) xv[i] = xg[(size_t)c * B + i];
+                });
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    const double r64 = bv[rw[j] + i] - acc[i];
+                    const double rs = sc[j][i] * r64;
+                    rr64 += r64 * r64;
+                    rho += rs * rs;
+                    r[j][i] = rs;
+                }
             }
-        });
     };
     double bb = 0.;
-    rows([&](int, int64_t row) {
 #pragma unroll
-        for (int i = 0; i < B; i++) {
-            const size_t k = (size_t)row * B + i;
-            bb += bv[k] * bv[k];
-            if (!use_x0) x[k] = 0.;
-        }
-    });
+    for (int j = 0; j < MAXJ; j++)
+        if (own[j])
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                const double b = bv[rw[j] + i];
+                bb += b * b;
+                sc[j][i] = S[rw[j] + i];
+                if (use_x0) x[j][i] = xg[rw[j] + i];
+                else xg[rw[j] + i] = 0.;
+            }
     bb = wg_sum(bb);   // its barrier also orders the zeroed x before the residual's gathers
     double rr64, rho;
     residual(rr64, rho);
-    rows([&](int, int64_t row) {
 #pragma unroll
-        for (int i = 0; i < B; i++) p[(size_t)row * B + i] = r[(size_t)row * B + i];
-    });
-    rr64 = wg_sum(rr64);
+    for (int j = 0; j < MAXJ; j++)
+        if (own[j])
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                p[j][i] = r[j][i];
+                pg[rw[j] + i] = p[j][i];
+            }
+    rr64 = wg_sum(rr64);   // p in memory past this barrier
     rho = wg_sum(rho);
     double rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
     int64_t it = 0;
@@ -1372,43 +1393,49 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
     while (status == ST_RUNNING) {
         // q = S K S p, partial p.q
         double pq = 0.;
-        rows([&](int s, int64_t row) {
-            double acc[B];
 #pragma unroll
-            for (int i = 0; i < B; i++) acc[i] = 0.;
-            row_product64<B>(a, s, lane, acc, [&](uint32_t c, double *xv) {
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j]) {
+                double acc[B];
 #pragma unroll
-                for (int j = 0; j < B; j++) xv[j] = S[(size_t)c * B + j] * p[(size_t)c * B + j];
-            });
+                for (int i = 0; i < B; i++) acc[i] = 0.;
+                row_product64<B, U>(a, sl[j], lane, acc, [&](uint32_t c, double *xv) {
 #pragma unroll
-            for (int i = 0; i < B; i++) {
-                const size_t k = (size_t)row * B + i;
-                const double qv = S[k] * acc[i];
-                q[k] = qv;
-                pq += p[k] * qv;
+                    for (int i = 0; i < B; i++) xv[i] = S[(size_t)c * B + i] * pg[(size_t)c * B + i];
+                });
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    q[j][i] = sc[j][i] * acc[i];
+                    pq += p[j][i] * q[j][i];
+                }
             }
-        });
         pq = wg_sum(pq);   // every gather of p is done past this barrier
         const double alpha = pq != 0. ? rho / pq : 0.;
         double rho1 = 0.;
-        rows([&](int, int64_t row) {
 #pragma unroll
-            for (int i = 0; i < B; i++) {
-                const size_t k = (size_t)row * B + i;
-                x[k] += S[k] * (alpha * p[k]);
-                const double rv = r[k] - alpha * q[k];
-                r[k] = rv;
-                rho1 += rv * rv;
-            }
-        });
-        rho1 = wg_sum(rho1);   // x complete past this barrier
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j])
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    x[j][i] += sc[j][i] * (alpha * p[j][i]);
+                    r[j][i] -= alpha * q[j][i];
+                    rho1 += r[j][i] * r[j][i];
+                }
+        rho1 = wg_sum(rho1);
         it++;
         if (!(rho1 == rho1) || !(pq == pq)) {
             status = ST_NAN;
             break;
         }
         if (rho1 <= target || it >= maxit) {
-            residual(rr64, rho1);   // the checked residual replaces the recurrence's
+            // the checked residual replaces the recurrence's
+#pragma unroll
+            for (int j = 0; j < MAXJ; j++)
+                if (own[j])
+#pragma unroll
+                    for (int i = 0; i < B; i++) xg[rw[j] + i] = x[j][i];
+            __syncthreads();   // x in memory before the residual's gathers
+            residual(rr64, rho1);
             rr64 = wg_sum(rr64);
             rho1 = wg_sum(rho1);
             rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
@@ -1418,16 +1445,22 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
             target = rho1 * (rtol / rel) * (rtol / rel);
         }
         const double beta = rho != 0. ? rho1 / rho : 0.;
-        rows([&](int, int64_t row) {
 #pragma unroll
-            for (int i = 0; i < B; i++) {
-                const size_t k = (size_t)row * B + i;
-                p[k] = r[k] + beta * p[k];
-            }
-        });
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j])
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    p[j][i] = r[j][i] + beta * p[j][i];
+                    pg[rw[j] + i] = p[j][i];
+                }
         rho = rho1;
         __syncthreads();   // p complete before the next product's gathers
     }
+#pragma unroll
+    for (int j = 0; j < MAXJ; j++)
+        if (own[j])
+#pragma unroll
+            for (int i = 0; i < B; i++) xg[rw[j] + i] = x[j][i];
     if (threadIdx.x == 0) {
         out[blockIdx.x].iters = it;
         out[blockIdx.x].relres = rel;
