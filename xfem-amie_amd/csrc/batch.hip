@@ -2432,8 +2432,8 @@ int Batch::event_step(const amie_event_params &prm, const amie_solve_opts &o_, a
     o.use_x0 = 1;
     o.use_graph = 0;   // every pass would rebuild the graphs for its active set
     // the solves of small meshes are latency-bound: a problem of at most
-    // 4 096 block rows runs its whole CG in one workgroup (FP64, whatever
-    // the precision asked for); otherwise one persistent launch per solve
+    // small_slices() slices runs its whole CG in one workgroup (FP64,
+    // whatever the precision asked for); otherwise one persistent launch per solve
     // pass saves 10-25 % of a pass up to a few dozen tiles and is bitwise
     // the launch path; larger batches keep a launch per kernel
     const bool small = o.persistent == 0 && small_solvable(B, o);
