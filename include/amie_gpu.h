@@ -462,7 +462,7 @@ int amie_gpu_batch_set_damage(amie_gpu_batch *b, int32_t p, const double *d);
  * changes nothing.  Linear elements (P1, Tet4), right-hand side 0.  With
  * opts.persistent 0 the solves of small meshes take the shorter-latency
  * paths: a batch with one right-hand side whose every problem has at most
- * 2 048 block rows (1 024 for Tet4) runs each problem's whole CG in one
+ * 3 072 block rows (2 048 for Tet4) runs each problem's whole CG in one
  * workgroup, in FP64
  * whatever opts.precision asks (same stopping rule: FP64 relative residual
  * <= opts.rtol); otherwise a batch of at most 32 tiles (8 192 block rows)

@@ -2304,9 +2304,9 @@ static double key_score_host(unsigned long long k)
 // has at most small_slices() slices and one right-hand side: its solves are
 // bound by the latency of an iteration's phases, which a workgroup barrier
 // separates far more cheaply than a kernel boundary.  A thread keeps up to
-// MAXJ rows in registers, as many as fit without spilling: 2 (2 048 block
-// rows) for 2 x 2 blocks, 1 (1 024) for 3 x 3.
-static int small_slices(const Batch &B) { return (B.et.b == 2 ? 2 : 1) * (WG1 / WAVE); }
+// MAXJ rows in registers: 3 (3 072 block rows) for 2 x 2 blocks, 2 (2 048)
+// for 3 x 3; beyond that the registers spill.
+static int small_slices(const Batch &B) { return (B.et.b == 2 ? 3 : 2) * (WG1 / WAVE); }
 
 static bool small_solvable(const Batch &B, const amie_solve_opts &o)
 {
@@ -2339,12 +2339,16 @@ static int solve_small(Batch &B, const amie_solve_opts &o, int64_t *iters_max)
     const int2 *dd = B.small_desc.as<int2>();
     SmallOut *so = B.small_out.as<SmallOut>();
     const int x0 = o.use_x0 ? 1 : 0;
-    if (B.et.b == 3)
+    if (B.et.b == 3 && maxs <= WG1 / WAVE)
         hipLaunchKernelGGL((k_cg_small<3, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
+    else if (B.et.b == 3)
+        hipLaunchKernelGGL((k_cg_small<3, 2>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
     else if (maxs <= WG1 / WAVE)
         hipLaunchKernelGGL((k_cg_small<2, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
-    else
+    else if (maxs <= 2 * (WG1 / WAVE))
         hipLaunchKernelGGL((k_cg_small<2, 2>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
+    else
+        hipLaunchKernelGGL((k_cg_small<2, 3>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
     AMIE_HIP(hipGetLastError());
     std::vector<SmallOut> out((size_t)np);
     AMIE_HIP(hipMemcpyAsync(out.data(), B.small_out.p, (size_t)np * sizeof(SmallOut), hipMemcpyDeviceToHost, s));

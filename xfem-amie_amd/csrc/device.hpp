@@ -1246,7 +1246,7 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
 // convergence, and the checked residual replaces the recurrence's, as the
 // batch solve does.  Every sum runs in a fixed order.
 //
-// A thread keeps its rows' x, r, q, p and S in registers (slices wave,
+// A thread keeps its rows' x, r, q and p in registers (slices wave,
 // wave + 16, ... up to MAXJ of them): the only global stores of an
 // iteration are p's, which the next product gathers, so the barriers of the
 // two sums wait for no store.  x goes to memory for a residual check and at
@@ -1306,8 +1306,8 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
     constexpr int NW = WG1 / WAVE;
     // blocks in flight per row: as many as the 128 registers a lane has at
     // 1 024 threads allow beside the rows kept
-    constexpr int U = (B == 2 && MAXJ == 1) ? 4 : 2;
-    static_assert(B * MAXJ <= 4, "the rows kept and the blocks in flight must fit 128 registers");
+    constexpr int U = (B == 2 && MAXJ == 1) ? 4 : (B * MAXJ <= 4 ? 2 : 1);
+    static_assert(B * MAXJ <= 8, "the rows kept and the blocks in flight must fit 128 registers");
     int rb = 0;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // the workgroup's sum, the same bits in every thread (butterfly within
@@ -1329,14 +1329,14 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
     bool own[MAXJ];
     int sl[MAXJ];
     size_t rw[MAXJ];   // first vector entry of the row
-    double x[MAXJ][B], r[MAXJ][B], p[MAXJ][B], q[MAXJ][B], sc[MAXJ][B];
+    double x[MAXJ][B], r[MAXJ][B], p[MAXJ][B], q[MAXJ][B];
 #pragma unroll
     for (int j = 0; j < MAXJ; j++) {
         sl[j] = d.x + wave + j * NW;
         own[j] = sl[j] < d.x + d.y && lane < a.slice_rows[sl[j]];
         rw[j] = ((size_t)sl[j] * WAVE + lane) * B;
 #pragma unroll
-        for (int i = 0; i < B; i++) x[j][i] = r[j][i] = p[j][i] = q[j][i] = sc[j][i] = 0.;
+        for (int i = 0; i < B; i++) x[j][i] = r[j][i] = p[j][i] = q[j][i] = 0.;
     }
     // r = S (b - K x) from x in memory; partials |b - K x|^2 and |r|^2
     auto residual = [&](double &rr64, double &rho) {
@@ -1354,7 +1354,7 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
 #pragma unroll
                 for (int i = 0; i < B; i++) {
                     const double r64 = bv[rw[j] + i] - acc[i];
-                    const double rs = sc[j][i] * r64;
+                    const double rs = S[rw[j] + i] * r64;
                     rr64 += r64 * r64;
                     rho += rs * rs;
                     r[j][i] = rs;
@@ -1369,7 +1369,6 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
             for (int i = 0; i < B; i++) {
                 const double b = bv[rw[j] + i];
                 bb += b * b;
-                sc[j][i] = S[rw[j] + i];
                 if (use_x0) x[j][i] = xg[rw[j] + i];
                 else xg[rw[j] + i] = 0.;
             }
@@ -1405,7 +1404,7 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
                 });
 #pragma unroll
                 for (int i = 0; i < B; i++) {
-                    q[j][i] = sc[j][i] * acc[i];
+                    q[j][i] = S[rw[j] + i] * acc[i];
                     pq += p[j][i] * q[j][i];
                 }
             }
@@ -1417,7 +1416,7 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
             if (own[j])
 #pragma unroll
                 for (int i = 0; i < B; i++) {
-                    x[j][i] += sc[j][i] * (alpha * p[j][i]);
+                    x[j][i] += S[rw[j] + i] * (alpha * p[j][i]);
                     r[j][i] -= alpha * q[j][i];
                     rho1 += r[j][i] * r[j][i];
                 }
