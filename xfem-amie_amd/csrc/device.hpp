@@ -1304,10 +1304,11 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
 {
     __shared__ double red[2][WG1 / WAVE];
     constexpr int NW = WG1 / WAVE;
-    // blocks in flight per row: as many as the 128 registers a lane has at
-    // 1 024 threads allow beside the rows kept
+    // blocks in flight per row: fewer the more rows a lane keeps (128
+    // registers a lane at 1 024 threads; the compiler spills 11 of them at
+    // B = 2, MAXJ = 3 and 2 at B = 3, MAXJ = 2, many more beyond)
     constexpr int U = (B == 2 && MAXJ == 1) ? 4 : (B * MAXJ <= 4 ? 2 : 1);
-    static_assert(B * MAXJ <= 8, "the rows kept and the blocks in flight must fit 128 registers");
+    static_assert(B * MAXJ <= 6, "more rows a lane spill most of them");
     int rb = 0;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // the workgroup's sum, the same bits in every thread (butterfly within
