@@ -460,14 +460,14 @@ int amie_gpu_batch_set_damage(amie_gpu_batch *b, int32_t p, const double *d);
  * neighbourhood (FractureCriterion::getMaxScoreInNeighbourhood,
  * fracturecriterion.cpp:251-280) takes d += fibre_fraction } until a pass
  * changes nothing.  Linear elements (P1, Tet4), right-hand side 0.  With
- * opts.persistent 0 the solves of small meshes take the shorter-latency
- * paths: a batch with one right-hand side whose every problem has at most
- * 3 072 block rows (2 048 for Tet4) runs each problem's whole CG in one
- * workgroup, in FP64
+ * opts.persistent 0 the solves take shorter-latency paths, in FP64
  * whatever opts.precision asks (same stopping rule: FP64 relative residual
- * <= opts.rtol); otherwise a batch of at most 32 tiles (8 192 block rows)
- * solves with persistent launches (the launch path's results bit for
- * bit). */
+ * <= opts.rtol), for one right-hand side: each problem's whole CG in one
+ * workgroup when every problem has at most 3 072 block rows (2 048 for
+ * Tet4), or a batch of one problem of at most 131 072 block rows over a
+ * cooperative grid of up to 64 workgroups; otherwise a batch of at most 32
+ * tiles (8 192 block rows) solves with persistent launches (the launch
+ * path's results bit for bit). */
 typedef struct {
     int32_t nphase;              /* materials the two arrays below cover (>= every problem's nphase) */
     const int32_t *criterion;    /* per material: 1 = StiffnessAndFracture + NonLocalMazars, 0 = elastic */

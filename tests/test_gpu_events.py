@@ -309,3 +309,40 @@ def test_event_step_tet4_vs_oracle(ctx):
         np.testing.assert_allclose(B.damage()[0], d_or, rtol=0, atol=1e-12)
     assert d_or.max() > 0
     B.close()
+
+
+@pytest.mark.parametrize("etype", ["p1", "tet4"])
+def test_event_step_cooperative_grid_agrees(ctx, etype):
+    """One problem too large for one workgroup (P1 56 x 56: 51 slices; a
+    Tet4 12^3 cube: 35 slices) runs its solves over a cooperative grid of
+    workgroups (device-wide barriers between the phases); with persistent
+    launches requested the batch solver runs instead.  One load step each:
+    the same damage, the same passes, displacements within 1e-9."""
+    if etype == "p1":
+        import configs
+        prob, dofs, nfix, crit, maz, radius = configs.event_square(56)
+        vals = np.concatenate([np.zeros(nfix), np.full(len(dofs) - nfix, 0.8e-4)])
+    else:
+        coords, conn = ag.structured_mesh(ag.TET4, 12)
+        x = coords[:, 0]
+        left, right = np.nonzero(np.isclose(x, 0.))[0], np.nonzero(np.isclose(x, 1.))[0]
+        dofs = np.concatenate([3 * left, 3 * left + 1, 3 * left + 2, 3 * right, 3 * right + 1]).astype(np.int64)
+        vals = np.concatenate([np.zeros(3 * len(left)), np.full(len(right), 1.4e-4), np.full(len(right), 0.7e-4)])
+        C = ag.cauchy_green(1e10, 0.2, dim=3)
+        prob = ag.Problem(ag.TET4, coords, conn, C[None], phase=np.zeros(conn.shape[0], dtype=np.int32),
+                          bc_dofs=dofs, bc_vals=np.zeros(len(dofs)))
+        crit, maz, radius = np.array([1], dtype=np.int32), np.array([[1e-4, 1e10, 0.2, 100.]]), 0.15
+    out = {}
+    for name, kw in (("grid", {}), ("batch", dict(persistent=64))):
+        print("cooperative", etype, name, flush=True)
+        B = ag.Batch(ctx, [prob])
+        B.set_bc_values(dofs, vals)
+        st = B.event_step(crit, maz, radius, precision=ag.FP64, rtol=1e-12, **kw)
+        assert st["status"] == 0 and st["converged"] == 1, (name, st)
+        out[name] = (st, B.damage()[0], B.solution())
+        B.close()
+    (sg, dg, xg), (sb, db, xb) = out["grid"], out["batch"]
+    assert sg["passes"] == sb["passes"] and sg["events"] == sb["events"], (sg, sb)
+    np.testing.assert_array_equal(dg, db)
+    assert np.linalg.norm(xg - xb) <= 1e-9 * np.linalg.norm(xb)
+    assert dg.max() > 0

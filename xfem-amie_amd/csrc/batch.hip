@@ -2300,59 +2300,89 @@ static double key_score_host(unsigned long long k)
     return v;
 }
 
-// The one-workgroup FP64 CG (k_cg_small) takes batches whose every problem
-// has at most small_slices() slices and one right-hand side: its solves are
-// bound by the latency of an iteration's phases, which a workgroup barrier
-// separates far more cheaply than a kernel boundary.  A thread keeps up to
-// MAXJ rows in registers: 3 (3 072 block rows) for 2 x 2 blocks, 2 (2 048)
-// for 3 x 3; beyond that the registers spill.
+// The FP64 CG of k_cg_small: one workgroup per problem when every problem
+// has at most small_slices() slices, or, for a batch of one larger problem
+// (at most COOP_MAX_WG workgroups of COOP_MAXJ slices a wavefront), one
+// cooperative grid; one right-hand side.  Its solves are bound by the
+// latency of an iteration's phases, which a workgroup (or a small grid's)
+// barrier separates far more cheaply than kernel boundaries.  A thread keeps
+// up to MAXJ rows in registers: 3 for 2 x 2 blocks, 2 for 3 x 3 in one
+// workgroup, 2 in a grid; beyond that the registers spill.
 static int small_slices(const Batch &B) { return (B.et.b == 2 ? 3 : 2) * (WG1 / WAVE); }
+constexpr int COOP_MAXJ = 2, COOP_MAX_WG = 64;
 
 static bool small_solvable(const Batch &B, const amie_solve_opts &o)
 {
     if (!B.has_elements || B.nrhs != 1 || o.ref_eps > 0.) return false;
+    if (B.nprob == 1) return B.ph[0].nslices <= COOP_MAX_WG * COOP_MAXJ * (WG1 / WAVE);
     for (int p = 0; p < B.nprob; p++)
         if (B.ph[p].nslices > small_slices(B)) return false;
     return true;
 }
 
-// Solves every problem with k_cg_small (one workgroup each) to o.rtol from
-// x64 (use_x0) or zero; returns AMIE_OK, AMIE_NOT_CONVERGED or AMIE_ERR_NAN
-// and the largest iteration count.
+// Solves every problem with k_cg_small to o.rtol from x64 (use_x0) or zero;
+// returns AMIE_OK, AMIE_NOT_CONVERGED, AMIE_ERR_NAN (or AMIE_ERR_HIP when a
+// grid barrier's wait gave up) and the largest iteration count.
 static int solve_small(Batch &B, const amie_solve_opts &o, int64_t *iters_max)
 {
-    const int np = B.nprob;
+    const int np = B.nprob, nw = WG1 / WAVE;
     std::vector<int2> d((size_t)np);
     int64_t maxit = 0;
+    int maxs = 0;
     for (int p = 0; p < np; p++) {
         d[p] = make_int2(B.ph[p].slice0, B.ph[p].nslices);
         maxit = std::max<int64_t>(maxit, o.maxit > 0 ? o.maxit : std::max<int64_t>(2 * B.ph[p].nrows * B.et.b, 1000));
+        maxs = std::max(maxs, B.ph[p].nslices);
     }
-    if (!ensure(B.small_desc, (size_t)np * sizeof(int2)) || !ensure(B.small_out, (size_t)np * sizeof(SmallOut)))
+    const bool coop = np == 1 && maxs > small_slices(B);
+    const int G = coop ? (maxs + COOP_MAXJ * nw - 1) / (COOP_MAXJ * nw) : np;
+    if (!ensure(B.small_desc, (size_t)np * sizeof(int2)) || !ensure(B.small_out, (size_t)np * sizeof(SmallOut)) ||
+        !ensure(B.small_part, (size_t)2 * G * sizeof(double)))
         return AMIE_ERR_NOMEM;
     hipStream_t s = B.stream;
     AMIE_HIP(hipMemcpyAsync(B.small_desc.p, d.data(), (size_t)np * sizeof(int2), hipMemcpyHostToDevice, s));
-    const SolveArgs a = B.args(AMIE_FP64);
-    const double rtol = o.rtol > 0 ? o.rtol : 1e-8;
-    int maxs = 0;
-    for (int p = 0; p < np; p++) maxs = std::max(maxs, B.ph[p].nslices);
+    SolveArgs a = B.args(AMIE_FP64);
+    double rtol = o.rtol > 0 ? o.rtol : 1e-8;
     const int2 *dd = B.small_desc.as<int2>();
     SmallOut *so = B.small_out.as<SmallOut>();
-    const int x0 = o.use_x0 ? 1 : 0;
-    if (B.et.b == 3 && maxs <= WG1 / WAVE)
-        hipLaunchKernelGGL((k_cg_small<3, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
-    else if (B.et.b == 3)
-        hipLaunchKernelGGL((k_cg_small<3, 2>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
-    else if (maxs <= WG1 / WAVE)
-        hipLaunchKernelGGL((k_cg_small<2, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
-    else if (maxs <= 2 * (WG1 / WAVE))
-        hipLaunchKernelGGL((k_cg_small<2, 2>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
-    else
-        hipLaunchKernelGGL((k_cg_small<2, 3>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so);
+    int x0 = o.use_x0 ? 1 : 0, cp = coop ? 1 : 0;
+    PersistArgs pa{};
+    double *part = B.small_part.as<double>();
+    if (coop) {
+        const void *fn = B.et.b == 2 ? (const void *)k_cg_small<2, COOP_MAXJ> : (const void *)k_cg_small<3, COOP_MAXJ>;
+        int per_cu = 0;
+        AMIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG1, 0));
+        if ((int64_t)per_cu * B.ctx->num_cu < G) {
+            set_error("one-workgroup solve: %d workgroups cannot all be resident", G);
+            return AMIE_ERR_HIP;
+        }
+        AMIE_HIP(hipMemsetAsync(B.persist.p, 0, B.persist.n, s));
+        pa.bar = B.persist.as<unsigned long long>();
+        pa.err = reinterpret_cast<int *>(B.persist.as<char>() + 8);
+        pa.periods = reinterpret_cast<int *>(B.persist.as<char>() + 12);
+        void *kargs[] = {&a, &dd, &rtol, &maxit, &x0, &so, &cp, &pa, &part};
+        AMIE_HIP(hipLaunchCooperativeKernel(fn, dim3(G), dim3(WG1), kargs, 0, s));
+    } else if (B.et.b == 3 && maxs <= nw) {
+        hipLaunchKernelGGL((k_cg_small<3, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so, 0, pa, part);
+    } else if (B.et.b == 3) {
+        hipLaunchKernelGGL((k_cg_small<3, 2>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so, 0, pa, part);
+    } else if (maxs <= nw) {
+        hipLaunchKernelGGL((k_cg_small<2, 1>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so, 0, pa, part);
+    } else if (maxs <= 2 * nw) {
+        hipLaunchKernelGGL((k_cg_small<2, 2>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so, 0, pa, part);
+    } else {
+        hipLaunchKernelGGL((k_cg_small<2, 3>), dim3(np), dim3(WG1), 0, s, a, dd, rtol, maxit, x0, so, 0, pa, part);
+    }
     AMIE_HIP(hipGetLastError());
     std::vector<SmallOut> out((size_t)np);
+    int err = 0;
     AMIE_HIP(hipMemcpyAsync(out.data(), B.small_out.p, (size_t)np * sizeof(SmallOut), hipMemcpyDeviceToHost, s));
+    if (coop) AMIE_HIP(hipMemcpyAsync(&err, B.persist.as<char>() + 8, 4, hipMemcpyDeviceToHost, s));
     AMIE_HIP(hipStreamSynchronize(s));
+    if (err) {
+        set_error("one-workgroup solve: a grid barrier's wait gave up (workgroups not all resident?)");
+        return AMIE_ERR_HIP;
+    }
     int rc = AMIE_OK;
     *iters_max = 0;
     for (int p = 0; p < np; p++) {

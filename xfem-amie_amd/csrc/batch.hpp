@@ -164,7 +164,7 @@ struct Batch {
     double frac_at = HUGE_VAL, frac_residual = 0.;
     DevBuf ev_tmp, ev_sbar, ev_score, ev_dprev, ev_flags, ev_crit, ev_prm, ev_count;
     DevBuf ev_inc, ev_tmax;   // IsotropicLinearDamage: increments of the pass, largest score per problem
-    DevBuf small_desc, small_out;   // one-workgroup solves (k_cg_small): per problem {slice0, nslices}, results
+    DevBuf small_desc, small_out, small_part;   // k_cg_small: per problem {slice0, nslices}, results; grid sums
     std::vector<int32_t> h_active, h_active_prob;
     std::vector<ProbDev> h_prob;
     ProbDev *h_prob_pinned = nullptr;

@@ -1234,240 +1234,6 @@ __global__ __launch_bounds__(WG) void k_ckpt_r(SolveArgs a)
     cr_body<W, B, NR, BJ>(a, t);
 }
 
-// ------------------------------------------- one-workgroup CG (small meshes)
-// The whole FP64 CG of one problem in one workgroup of WG1 threads: the
-// phases of an iteration are separated by workgroup barriers instead of
-// kernel boundaries or device-wide barriers, which is what the solves of a
-// few thousand block rows are bound by (the passes of the event-driven
-// damage).  The scaled system S K S (symmetric Jacobi, point Jacobi after
-// scaling) on the unscaled FP64 planes, right-hand side 0; x64 in place
-// (warm start when use_x0).  The recurrence residual decides when to check,
-// the FP64 relative residual ||b - K x|| / ||b|| <= rtol decides
-// convergence, and the checked residual replaces the recurrence's, as the
-// batch solve does.  Every sum runs in a fixed order.
-//
-// A thread keeps its rows' x, r, q and p in registers (slices wave,
-// wave + 16, ... up to MAXJ of them): the only global stores of an
-// iteration are p's, which the next product gathers, so the barriers of the
-// two sums wait for no store.  x goes to memory for a residual check and at
-// the end.
-constexpr int WG1 = 1024;
-struct SmallOut {
-    int64_t iters;
-    double relres;
-    int32_t status;   // ST_CONVERGED, ST_MAXIT or ST_NAN
-    int32_t pad;
-};
-
-template <int B, int U, typename G>
-__device__ inline void row_product64(const SolveArgs &a, int s, int lane, double *acc, G gather)
-{
-    // U blocks in flight (all of a row's 7-9 at once spills registers at
-    // 1 024 threads and is slower)
-    const int w = a.slice_width[s];
-    const int64_t sp = a.slice_ptr[s] + lane, ns = a.nslots;
-    const d2v *pl = reinterpret_cast<const d2v *>(a.v64);
-    for (int k = 0; k < w; k += U) {
-        uint32_t c[U];
-        double m[U][B * B];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (k + u < w) c[u] = a.col[sp + (int64_t)(k + u) * WAVE];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (k + u < w) {
-                const int64_t sl = sp + (int64_t)(k + u) * WAVE;
-                const d2v v0 = pl[sl], v1 = pl[ns + sl];
-                m[u][0] = v0.x; m[u][1] = v0.y; m[u][2] = v1.x; m[u][3] = v1.y;
-                if constexpr (B == 3) {
-                    const d2v v2 = pl[2 * ns + sl], v3 = pl[3 * ns + sl];
-                    m[u][4] = v2.x; m[u][5] = v2.y; m[u][6] = v3.x; m[u][7] = v3.y;
-                    m[u][8] = reinterpret_cast<const double *>(pl + 4 * ns)[sl];
-                }
-            }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (k + u < w) {
-                double xv[B];
-                gather(c[u], xv);
-#pragma unroll
-                for (int j = 0; j < B; j++)
-#pragma unroll
-                    for (int i = 0; i < B; i++) acc[i] += m[u][j * B + i] * xv[j];
-            }
-    }
-}
-
-template <int B, int MAXJ>
-__global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc, double rtol, int64_t maxit,
-                                                   int use_x0, SmallOut *out)
-{
-    __shared__ double red[2][WG1 / WAVE];
-    constexpr int NW = WG1 / WAVE;
-    // blocks in flight per row: fewer the more rows a lane keeps (128
-    // registers a lane at 1 024 threads; the compiler spills 11 of them at
-    // B = 2, MAXJ = 3 and 2 at B = 3, MAXJ = 2, many more beyond)
-    constexpr int U = (B == 2 && MAXJ == 1) ? 4 : (B * MAXJ <= 4 ? 2 : 1);
-    static_assert(B * MAXJ <= 6, "more rows a lane spill most of them");
-    int rb = 0;
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    // the workgroup's sum, the same bits in every thread (butterfly within
-    // the wavefront, then the wavefront sums in order; two buffers, so a
-    // sum's reads finish before the buffer is written again two sums later)
-    auto wg_sum = [&](double v) -> double {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-        if (lane == 0) red[rb][wave] = v;
-        __syncthreads();
-        double t = 0.;
-        for (int i = 0; i < NW; i++) t += red[rb][i];
-        rb ^= 1;
-        return t;
-    };
-    const int2 d = desc[blockIdx.x];   // first slice, slices (at most MAXJ * NW)
-    double *xg = a.x64, *pg = vw<double>(a.p);
-    const double *bv = a.b64, *S = a.S;
-    bool own[MAXJ];
-    int sl[MAXJ];
-    size_t rw[MAXJ];   // first vector entry of the row
-    double x[MAXJ][B], r[MAXJ][B], p[MAXJ][B], q[MAXJ][B];
-#pragma unroll
-    for (int j = 0; j < MAXJ; j++) {
-        sl[j] = d.x + wave + j * NW;
-        own[j] = sl[j] < d.x + d.y && lane < a.slice_rows[sl[j]];
-        rw[j] = ((size_t)sl[j] * WAVE + lane) * B;
-#pragma unroll
-        for (int i = 0; i < B; i++) x[j][i] = r[j][i] = p[j][i] = q[j][i] = 0.;
-    }
-    // r = S (b - K x) from x in memory; partials |b - K x|^2 and |r|^2
-    auto residual = [&](double &rr64, double &rho) {
-        rr64 = rho = 0.;
-#pragma unroll
-        for (int j = 0; j < MAXJ; j++)
-            if (own[j]) {
-                double acc[B];
-#pragma unroll
-                for (int i = 0; i < B; i++) acc[i] = 0.;
-                row_product64<B, U>(a, sl[j], lane, acc, [&](uint32_t c, double *xv) {
-#pragma unroll
-                    for (int i = 0; i < B; i++) xv[i] = xg[(size_t)c * B + i];
-                });
-#pragma unroll
-                for (int i = 0; i < B; i++) {
-                    const double r64 = bv[rw[j] + i] - acc[i];
-                    const double rs = S[rw[j] + i] * r64;
-                    rr64 += r64 * r64;
-                    rho += rs * rs;
-                    r[j][i] = rs;
-                }
-            }
-    };
-    double bb = 0.;
-#pragma unroll
-    for (int j = 0; j < MAXJ; j++)
-        if (own[j])
-#pragma unroll
-            for (int i = 0; i < B; i++) {
-                const double b = bv[rw[j] + i];
-                bb += b * b;
-                if (use_x0) x[j][i] = xg[rw[j] + i];
-                else xg[rw[j] + i] = 0.;
-            }
-    bb = wg_sum(bb);   // its barrier also orders the zeroed x before the residual's gathers
-    double rr64, rho;
-    residual(rr64, rho);
-#pragma unroll
-    for (int j = 0; j < MAXJ; j++)
-        if (own[j])
-#pragma unroll
-            for (int i = 0; i < B; i++) {
-                p[j][i] = r[j][i];
-                pg[rw[j] + i] = p[j][i];
-            }
-    rr64 = wg_sum(rr64);   // p in memory past this barrier
-    rho = wg_sum(rho);
-    double rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
-    int64_t it = 0;
-    int status = rel <= rtol ? ST_CONVERGED : ST_RUNNING;
-    double target = rel > 0. ? rho * (rtol / rel) * (rtol / rel) : 0.;
-    while (status == ST_RUNNING) {
-        // q = S K S p, partial p.q
-        double pq = 0.;
-#pragma unroll
-        for (int j = 0; j < MAXJ; j++)
-            if (own[j]) {
-                double acc[B];
-#pragma unroll
-                for (int i = 0; i < B; i++) acc[i] = 0.;
-                row_product64<B, U>(a, sl[j], lane, acc, [&](uint32_t c, double *xv) {
-#pragma unroll
-                    for (int i = 0; i < B; i++) xv[i] = S[(size_t)c * B + i] * pg[(size_t)c * B + i];
-                });
-#pragma unroll
-                for (int i = 0; i < B; i++) {
-                    q[j][i] = S[rw[j] + i] * acc[i];
-                    pq += p[j][i] * q[j][i];
-                }
-            }
-        pq = wg_sum(pq);   // every gather of p is done past this barrier
-        const double alpha = pq != 0. ? rho / pq : 0.;
-        double rho1 = 0.;
-#pragma unroll
-        for (int j = 0; j < MAXJ; j++)
-            if (own[j])
-#pragma unroll
-                for (int i = 0; i < B; i++) {
-                    x[j][i] += S[rw[j] + i] * (alpha * p[j][i]);
-                    r[j][i] -= alpha * q[j][i];
-                    rho1 += r[j][i] * r[j][i];
-                }
-        rho1 = wg_sum(rho1);
-        it++;
-        if (!(rho1 == rho1) || !(pq == pq)) {
-            status = ST_NAN;
-            break;
-        }
-        if (rho1 <= target || it >= maxit) {
-            // the checked residual replaces the recurrence's
-#pragma unroll
-            for (int j = 0; j < MAXJ; j++)
-                if (own[j])
-#pragma unroll
-                    for (int i = 0; i < B; i++) xg[rw[j] + i] = x[j][i];
-            __syncthreads();   // x in memory before the residual's gathers
-            residual(rr64, rho1);
-            rr64 = wg_sum(rr64);
-            rho1 = wg_sum(rho1);
-            rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
-            if (!(rel == rel)) { status = ST_NAN; break; }
-            if (rel <= rtol) { status = ST_CONVERGED; break; }
-            if (it >= maxit) { status = ST_MAXIT; break; }
-            target = rho1 * (rtol / rel) * (rtol / rel);
-        }
-        const double beta = rho != 0. ? rho1 / rho : 0.;
-#pragma unroll
-        for (int j = 0; j < MAXJ; j++)
-            if (own[j])
-#pragma unroll
-                for (int i = 0; i < B; i++) {
-                    p[j][i] = r[j][i] + beta * p[j][i];
-                    pg[rw[j] + i] = p[j][i];
-                }
-        rho = rho1;
-        __syncthreads();   // p complete before the next product's gathers
-    }
-#pragma unroll
-    for (int j = 0; j < MAXJ; j++)
-        if (own[j])
-#pragma unroll
-            for (int i = 0; i < B; i++) xg[rw[j] + i] = x[j][i];
-    if (threadIdx.x == 0) {
-        out[blockIdx.x].iters = it;
-        out[blockIdx.x].relres = rel;
-        out[blockIdx.x].status = status;
-    }
-}
-
 // ------------------------------------------------------ persistent solve
 // One launch runs whole checkpoint periods: the phases of queue_period (K1,
 // R, K2 per iteration, then C1 and C2) as loops over the same per-tile and
@@ -1519,6 +1285,274 @@ __device__ inline bool persist_barrier(const PersistArgs &pa, unsigned long long
     }
     __syncthreads();
     return ok_s != 0;
+}
+
+// ------------------------------------------- one-workgroup CG (small meshes)
+// The whole FP64 CG of one problem in one workgroup of WG1 threads: the
+// phases of an iteration are separated by workgroup barriers instead of
+// kernel boundaries or device-wide barriers, which is what the solves of a
+// few thousand block rows are bound by (the passes of the event-driven
+// damage).  The scaled system S K S (symmetric Jacobi, point Jacobi after
+// scaling) on the unscaled FP64 planes, right-hand side 0; x64 in place
+// (warm start when use_x0).  The recurrence residual decides when to check,
+// the FP64 relative residual ||b - K x|| / ||b|| <= rtol decides
+// convergence, and the checked residual replaces the recurrence's, as the
+// batch solve does.  Every sum runs in a fixed order.
+//
+// A thread keeps its rows' x, r, q and p in registers (slices wave,
+// wave + 16, ... up to MAXJ of them): the only global stores of an
+// iteration are p's, which the next product gathers, so the barriers of the
+// two sums wait for no store.  x goes to memory for a residual check and at
+// the end.
+//
+// coop: one problem over the whole grid, launched cooperatively (every
+// workgroup resident): global wavefront g = block * 16 + wave takes slices
+// g, g + 16 G, ...; a sum adds the workgroups' sums (written to part, read
+// by all in workgroup order after a device-wide barrier, persist_barrier),
+// and the barriers that order p's and x's stores are device-wide.  A wait
+// that gives up (pa.err) ends every workgroup with ST_NAN; the host reports
+// it as an error.  Otherwise one workgroup per problem.
+constexpr int WG1 = 1024;
+struct SmallOut {
+    int64_t iters;
+    double relres;
+    int32_t status;   // ST_CONVERGED, ST_MAXIT or ST_NAN
+    int32_t pad;
+};
+
+template <int B, int U, typename G>
+__device__ inline void row_product64(const SolveArgs &a, int s, int lane, double *acc, G gather)
+{
+    // U blocks in flight (all of a row's 7-9 at once spills registers at
+    // 1 024 threads and is slower)
+    const int w = a.slice_width[s];
+    const int64_t sp = a.slice_ptr[s] + lane, ns = a.nslots;
+    const d2v *pl = reinterpret_cast<const d2v *>(a.v64);
+    for (int k = 0; k < w; k += U) {
+        uint32_t c[U];
+        double m[U][B * B];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) c[u] = a.col[sp + (int64_t)(k + u) * WAVE];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) {
+                const int64_t sl = sp + (int64_t)(k + u) * WAVE;
+                const d2v v0 = pl[sl], v1 = pl[ns + sl];
+                m[u][0] = v0.x; m[u][1] = v0.y; m[u][2] = v1.x; m[u][3] = v1.y;
+                if constexpr (B == 3) {
+                    const d2v v2 = pl[2 * ns + sl], v3 = pl[3 * ns + sl];
+                    m[u][4] = v2.x; m[u][5] = v2.y; m[u][6] = v3.x; m[u][7] = v3.y;
+                    m[u][8] = reinterpret_cast<const double *>(pl + 4 * ns)[sl];
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) {
+                double xv[B];
+                gather(c[u], xv);
+#pragma unroll
+                for (int j = 0; j < B; j++)
+#pragma unroll
+                    for (int i = 0; i < B; i++) acc[i] += m[u][j * B + i] * xv[j];
+            }
+    }
+}
+
+template <int B, int MAXJ>
+__global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc, double rtol, int64_t maxit,
+                                                   int use_x0, SmallOut *out, int coop, PersistArgs pa, double *part)
+{
+    __shared__ double red[2][WG1 / WAVE];
+    constexpr int NW = WG1 / WAVE;
+    // blocks in flight per row: fewer the more rows a lane keeps (128
+    // registers a lane at 1 024 threads; the compiler spills 11 of them at
+    // B = 2, MAXJ = 3 and 2 at B = 3, MAXJ = 2, many more beyond)
+    constexpr int U = (B == 2 && MAXJ == 1) ? 4 : (B * MAXJ <= 4 ? 2 : 1);
+    static_assert(B * MAXJ <= 6, "more rows a lane spill most of them");
+    int rb = 0;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // the workgroup's sum, the same bits in every thread (butterfly within
+    // the wavefront, then the wavefront sums in order; two buffers, so a
+    // sum's reads finish before the buffer is written again two sums later)
+    auto wg_sum = [&](double v) -> double {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+        if (lane == 0) red[rb][wave] = v;
+        __syncthreads();
+        double t = 0.;
+        for (int i = 0; i < NW; i++) t += red[rb][i];
+        rb ^= 1;
+        return t;
+    };
+    // coop: the grid's barrier and sum (part: two buffers of G sums, as the
+    // LDS ones); ok turns false for every workgroup once a wait gave up
+    const int G = coop ? (int)gridDim.x : 1;
+    unsigned long long round = 0;
+    int pb = 0;
+    bool ok = true;
+    auto barrier = [&]() {
+        if (coop) ok = persist_barrier(pa, round) && ok;
+        else __syncthreads();
+    };
+    auto sum = [&](double v) -> double {
+        double t = wg_sum(v);
+        if (!coop) return t;
+        if (threadIdx.x == 0) part[pb * G + blockIdx.x] = t;
+        barrier();
+        t = 0.;
+        for (int i = 0; i < G; i++) t += part[pb * G + i];
+        pb ^= 1;
+        return t;
+    };
+    const int2 d = desc[coop ? 0 : blockIdx.x];   // first slice, slices (at most MAXJ * NW * G)
+    const int g0 = coop ? (int)blockIdx.x * NW : 0, gstride = G * NW;
+    double *xg = a.x64, *pg = vw<double>(a.p);
+    const double *bv = a.b64, *S = a.S;
+    bool own[MAXJ];
+    int sl[MAXJ];
+    size_t rw[MAXJ];   // first vector entry of the row
+    double x[MAXJ][B], r[MAXJ][B], p[MAXJ][B], q[MAXJ][B];
+#pragma unroll
+    for (int j = 0; j < MAXJ; j++) {
+        sl[j] = d.x + g0 + wave + j * gstride;
+        own[j] = sl[j] < d.x + d.y && lane < a.slice_rows[sl[j]];
+        rw[j] = ((size_t)sl[j] * WAVE + lane) * B;
+#pragma unroll
+        for (int i = 0; i < B; i++) x[j][i] = r[j][i] = p[j][i] = q[j][i] = 0.;
+    }
+    // r = S (b - K x) from x in memory; partials |b - K x|^2 and |r|^2
+    auto residual = [&](double &rr64, double &rho) {
+        rr64 = rho = 0.;
+#pragma unroll
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j]) {
+                double acc[B];
+#pragma unroll
+                for (int i = 0; i < B; i++) acc[i] = 0.;
+                row_product64<B, U>(a, sl[j], lane, acc, [&](uint32_t c, double *xv) {
+#pragma unroll
+                    for (int i = 0; i < B; i++) xv[i] = xg[(size_t)c * B + i];
+                });
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    const double r64 = bv[rw[j] + i] - acc[i];
+                    const double rs = S[rw[j] + i] * r64;
+                    rr64 += r64 * r64;
+                    rho += rs * rs;
+                    r[j][i] = rs;
+                }
+            }
+    };
+    double bb = 0.;
+#pragma unroll
+    for (int j = 0; j < MAXJ; j++)
+        if (own[j])
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                const double b = bv[rw[j] + i];
+                bb += b * b;
+                if (use_x0) x[j][i] = xg[rw[j] + i];
+                else xg[rw[j] + i] = 0.;
+            }
+    bb = sum(bb);   // its barrier also orders the zeroed x before the residual's gathers
+    double rr64, rho;
+    residual(rr64, rho);
+#pragma unroll
+    for (int j = 0; j < MAXJ; j++)
+        if (own[j])
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                p[j][i] = r[j][i];
+                pg[rw[j] + i] = p[j][i];
+            }
+    rr64 = sum(rr64);   // p in memory past this barrier
+    rho = sum(rho);
+    double rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
+    int64_t it = 0;
+    int status = rel <= rtol ? ST_CONVERGED : ST_RUNNING;
+    double target = rel > 0. ? rho * (rtol / rel) * (rtol / rel) : 0.;
+    while (status == ST_RUNNING) {
+        // q = S K S p, partial p.q
+        double pq = 0.;
+#pragma unroll
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j]) {
+                double acc[B];
+#pragma unroll
+                for (int i = 0; i < B; i++) acc[i] = 0.;
+                row_product64<B, U>(a, sl[j], lane, acc, [&](uint32_t c, double *xv) {
+#pragma unroll
+                    for (int i = 0; i < B; i++) xv[i] = S[(size_t)c * B + i] * pg[(size_t)c * B + i];
+                });
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    q[j][i] = S[rw[j] + i] * acc[i];
+                    pq += p[j][i] * q[j][i];
+                }
+            }
+        pq = sum(pq);   // every gather of p is done past this barrier
+        const double alpha = pq != 0. ? rho / pq : 0.;
+        double rho1 = 0.;
+#pragma unroll
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j])
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    x[j][i] += S[rw[j] + i] * (alpha * p[j][i]);
+                    r[j][i] -= alpha * q[j][i];
+                    rho1 += r[j][i] * r[j][i];
+                }
+        rho1 = sum(rho1);
+        it++;
+        if (!ok || !(rho1 == rho1) || !(pq == pq)) {
+            status = ST_NAN;
+            break;
+        }
+        if (rho1 <= target || it >= maxit) {
+            // the checked residual replaces the recurrence's
+#pragma unroll
+            for (int j = 0; j < MAXJ; j++)
+                if (own[j])
+#pragma unroll
+                    for (int i = 0; i < B; i++) xg[rw[j] + i] = x[j][i];
+            barrier();   // x in memory before the residual's gathers
+            residual(rr64, rho1);
+            rr64 = sum(rr64);
+            rho1 = sum(rho1);
+            rel = bb > 0. ? sqrt(rr64 / bb) : sqrt(rr64);
+            if (!(rel == rel)) { status = ST_NAN; break; }
+            if (rel <= rtol) { status = ST_CONVERGED; break; }
+            if (it >= maxit) { status = ST_MAXIT; break; }
+            target = rho1 * (rtol / rel) * (rtol / rel);
+        }
+        const double beta = rho != 0. ? rho1 / rho : 0.;
+#pragma unroll
+        for (int j = 0; j < MAXJ; j++)
+            if (own[j])
+#pragma unroll
+                for (int i = 0; i < B; i++) {
+                    p[j][i] = r[j][i] + beta * p[j][i];
+                    pg[rw[j] + i] = p[j][i];
+                }
+        rho = rho1;
+        barrier();   // p complete before the next product's gathers
+        if (!ok) {
+            status = ST_NAN;
+            break;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXJ; j++)
+        if (own[j])
+#pragma unroll
+            for (int i = 0; i < B; i++) xg[rw[j] + i] = x[j][i];
+    if (threadIdx.x == 0 && (!coop || blockIdx.x == 0)) {
+        SmallOut &o = out[coop ? 0 : blockIdx.x];
+        o.iters = it;
+        o.relres = rel;
+        o.status = ok ? status : ST_NAN;
+    }
 }
 
 // Workgroup b takes active tiles b, b + G, ... and active problems b, b + G,
