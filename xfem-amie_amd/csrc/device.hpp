@@ -1366,8 +1366,8 @@ __global__ __launch_bounds__(WG1) void k_cg_small(SolveArgs a, const int2 *desc,
     __shared__ double red[2][WG1 / WAVE];
     constexpr int NW = WG1 / WAVE;
     // blocks in flight per row: fewer the more rows a lane keeps (128
-    // registers a lane at 1 024 threads; the compiler spills 11 of them at
-    // B = 2, MAXJ = 3 and 2 at B = 3, MAXJ = 2, many more beyond)
+    // registers a lane at 1 024 threads; the compiler spills 2 of them at
+    // B = 3, MAXJ = 2, none below, dozens beyond)
     constexpr int U = (B == 2 && MAXJ == 1) ? 4 : (B * MAXJ <= 4 ? 2 : 1);
     static_assert(B * MAXJ <= 6, "more rows a lane spill most of them");
     int rb = 0;
