@@ -83,7 +83,7 @@ Batch::~Batch()
 size_t Batch::bytes() const
 {
     const DevBuf *all[] = {&slice_ptr, &slice_width, &slice_rows, &slice_desc, &tile_desc, &slot_off, &dict_ids, &dict_table,
-                           &egeo, &ngrid, &ngdim, &cell_start, &cell_elem, &col, &dslot, &row_len, &v64,
+                           &egeo, &ngrid, &ngdim, &cell_start, &cell_elem, &col, &dslot, &row_len, &v64, &vs64,
                            &v32, &vsp, &vfx, &fix48_bad, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
                            &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &q, &minv32, &minv64, &bcmask,
@@ -122,7 +122,8 @@ SolveArgs Batch::args(int mode) const
     a.col = col.as<uint32_t>();
     a.nslots = nslots;
     a.v64 = v64.as<double>();
-    a.vw = m32 ? v32.p : (mode == AMIE_MIXED_SPLIT ? vsp.p : (mode == AMIE_MIXED_FIX48 ? vfx.p : v64.p));
+    // FP64 and FP64-matrix modes iterate on the scaled FP64 copy (ensure_pre64)
+    a.vw = m32 ? v32.p : (mode == AMIE_MIXED_SPLIT ? vsp.p : (mode == AMIE_MIXED_FIX48 ? vfx.p : vs64.p));
     a.fix48_bad = fix48_bad.as<int32_t>();
     a.tile_slice0 = tile_slice0.as<int32_t>();
     a.tile_desc = tile_desc.as<int4>();
@@ -703,6 +704,27 @@ template <int Bk> static void launch_post(Batch &B)
                        B.minv64.as<double>(), (int)B.ntiles);
 }
 
+// The scaled FP64 matrix Ks = S K S of the FP64 and FP64-matrix modes: one
+// pass over the FP64 planes after the assembly that last changed K or S.
+int Batch::ensure_pre64(int mode)
+{
+    if ((mode != AMIE_FP64 && mode != AMIE_MIXED_K64) || pre64_ok) return AMIE_OK;
+    if (!vs64.n && !vs64.alloc(nslots * slot_bytes(true, et.b))) {
+        set_error("out of device memory for the scaled FP64 matrix");
+        return AMIE_ERR_NOMEM;
+    }
+    SolveArgs a = args(AMIE_FP64);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid()), dim3(WG), 0, stream, a, vs64.p, dslot.as<int64_t>(), (double *)nullptr,
+                           (int)ntiles);
+    };
+    if (et.b == 2) go(k_scale_cast<Pre64, double, 2>);
+    else go(k_scale_cast<Pre64, double, 3>);
+    AMIE_HIP(hipGetLastError());
+    pre64_ok = true;
+    return AMIE_OK;
+}
+
 int Batch::assemble()
 {
     hipStream_t st = stream;
@@ -738,6 +760,7 @@ int Batch::assemble()
         }
     }
     AMIE_HIP(hipMemsetAsync(fix48_bad.p, 0, 4, st));
+    pre64_ok = false;
     if (et.b == 2) launch_post<2>(*this);
     else launch_post<3>(*this);
     AMIE_HIP(hipGetLastError());
@@ -755,7 +778,7 @@ int Batch::assemble()
 // run the launch path.
 template <typename W, typename M, int Bk, int NR, bool BJ> static void (*persist_for())(SolveArgs, PersistArgs)
 {
-    if constexpr ((std::is_same<W, double>::value && std::is_same<M, double>::value) ||
+    if constexpr ((std::is_same<W, double>::value && std::is_same<M, Pre64>::value) ||
                   (std::is_same<W, float>::value && (std::is_same<M, Fix48>::value || std::is_same<M, Dict>::value)))
         return k_persistent<W, M, Bk, NR, BJ>;
     else
@@ -791,10 +814,10 @@ static Kern select_kern(int mode, int b, int nr, bool bj)
 {
     if (mode == AMIE_MIXED) return b == 2 ? kern_nr<float, float, 2>(nr, bj) : kern_nr<float, float, 3>(nr, bj);
     if (mode == AMIE_MIXED64) return b == 2 ? kern_nr<double, float, 2>(nr, bj) : kern_nr<double, float, 3>(nr, bj);
-    if (mode == AMIE_MIXED_K64) return b == 2 ? kern_nr<float, double, 2>(nr, bj) : kern_nr<float, double, 3>(nr, bj);
+    if (mode == AMIE_MIXED_K64) return b == 2 ? kern_nr<float, Pre64, 2>(nr, bj) : kern_nr<float, Pre64, 3>(nr, bj);
     if (mode == AMIE_MIXED_SPLIT) return b == 2 ? kern_nr<float, Split, 2>(nr, bj) : kern_nr<float, Split, 3>(nr, bj);
     if (mode == AMIE_MIXED_FIX48) return b == 2 ? kern_nr<float, Fix48, 2>(nr, bj) : kern_nr<float, Fix48, 3>(nr, bj);
-    return b == 2 ? kern_nr<double, double, 2>(nr, bj) : kern_nr<double, double, 3>(nr, bj);
+    return b == 2 ? kern_nr<double, Pre64, 2>(nr, bj) : kern_nr<double, Pre64, 3>(nr, bj);
 }
 
 // ------------------------------------------------------------ autotuner
@@ -831,7 +854,7 @@ static K1Var k1_variant(int mode, int b, int nr, bool bj, int which, int format 
     if (mode == AMIE_MIXED) return b == 2 ? k1_nr<float, float, 2>(nr, bj, which) : k1_nr<float, float, 3>(nr, bj, which);
     if (mode == AMIE_MIXED_FIX48)
         return b == 2 ? k1_nr<float, Fix48, 2>(nr, bj, which) : k1_nr<float, Fix48, 3>(nr, bj, which);
-    if (mode == AMIE_FP64) return b == 2 ? k1_nr<double, double, 2>(nr, bj, which) : k1_nr<double, double, 3>(nr, bj, which);
+    if (mode == AMIE_FP64) return b == 2 ? k1_nr<double, Pre64, 2>(nr, bj, which) : k1_nr<double, Pre64, 3>(nr, bj, which);
     return {nullptr, 0};
 }
 
@@ -985,6 +1008,7 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         set_error("AMIE_MIXED_FIX48: a scaled matrix entry is outside (-2, 2) (K not symmetric positive definite?)");
         return AMIE_ERR_ARG;
     }
+    if (int rc = ensure_pre64(mode)) return rc;
     if (o.autotune && !k1_tuned.count(tune_key(mode, R.bj)) && k1_variant(mode, et.b, nrhs, R.bj, 1).k1) {
         const int rc = autotune(mode, R.bj, 5);
         if (rc) return rc;
@@ -1704,6 +1728,7 @@ int Batch::autotune(int mode, bool bj, int iters)
         set_error("AMIE_MIXED_FIX48: a scaled matrix entry is outside (-2, 2) (K not symmetric positive definite?)");
         return AMIE_ERR_ARG;
     }
+    if (int rc = ensure_pre64(mode)) return rc;
     if (mode == AMIE_MIXED_FIX48 && !dict_wanted) {
         // from now on every assembly also builds the value-indexed matrix
         dict_wanted = true;
@@ -1741,6 +1766,7 @@ int amie_gpu_batch_time_spmv(amie_gpu_batch *bb, int32_t precision, int32_t prec
     Batch &B = *bb->b;
     if (B.run.started) { set_error("a solve is in progress"); return AMIE_ERR_BUSY; }
     const bool bj = precond == AMIE_BLOCK_JACOBI;
+    if (int rc = B.ensure_pre64(precision)) return rc;
     Kern K = select_kern(precision, B.et.b, B.nrhs, bj);
     SolveArgs a = B.args(precision);
     B.apply_tuned(precision, bj, K, a);

@@ -138,6 +138,8 @@ struct Batch {
     bool dict_ok = false;             // the dictionary matches the current FIX48 planes
     int64_t dict_n = 0;               // its distinct blocks
     DevBuf v64, v32, vsp, vfx;   // FP64 unscaled K; FP32, split FP32+bf16 and 48-bit fixed-point scaled Ks
+    DevBuf vs64;                 // FP64 scaled Ks, built on the first FP64 solve after an assembly
+    bool pre64_ok = false;       // vs64 matches the current assembly
     DevBuf fix48_bad;            // int flag: a scaled value outside the fixed-point range
     DevBuf tile_slice0, tile_nslices, tile_prob, active, n_active, active_prob, n_active_prob;
     DevBuf prob, part1, part2, partc, partn;
@@ -184,6 +186,7 @@ struct Batch {
     SolveArgs args(int mode) const;
     int grid(int64_t tiles = -1) const;
     int assemble();
+    int ensure_pre64(int mode);
     int solve(const amie_solve_opts &o, amie_solve_stats *st);
     int solve_begin(const amie_solve_opts &o);
     int start_pass(int mode, const std::vector<char> &subset);

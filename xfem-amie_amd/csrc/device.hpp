@@ -509,6 +509,17 @@ template <typename M> struct MatKind {
 template <> struct MatKind<double> {
     static constexpr bool onfly = true, acc64 = true;
 };
+
+// FP64 planes of the scaled Ks = S K S, the FP64 CG's matrix.  Scaling on
+// the fly gathers S beside p for every block column (32 bytes a 2x2 column
+// instead of 16) and that second gather is what the P2 meshes, whose
+// columns spread widely, re-read from HBM; the copy costs one more FP64
+// matrix in HBM, built once per assembly when an FP64 solve asks for it.
+struct Pre64 {};
+template <int B> struct Blk<Pre64, B> : Blk<double, B> {};
+template <> struct MatKind<Pre64> {
+    static constexpr bool onfly = false, acc64 = true;
+};
 template <> struct MatKind<Split> {
     static constexpr bool onfly = false, acc64 = true;
 };
