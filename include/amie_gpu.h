@@ -54,7 +54,10 @@ enum { AMIE_P1 = 1, AMIE_P2 = 2, AMIE_TET4 = 3 };
 /* solve precision.  Every mixed mode keeps the residual r = b - K x, the
  * solution and the CG scalars in FP64 (refinement) and iterates on FP32 or
  * FP64 vectors inside:
- *   AMIE_FP64         FP64 matrix and vectors
+ *   AMIE_FP64         FP64 matrix and vectors (the iteration reads a scaled
+ *                     FP64 copy S K S, built on the first AMIE_FP64 or
+ *                     AMIE_MIXED_K64 solve after an assembly: one more FP64
+ *                     matrix of device memory)
  *   AMIE_MIXED        FP32 matrix, vectors and SpMV accumulation
  *   AMIE_MIXED64      FP32 matrix, FP64 vectors
  *   AMIE_MIXED_K64    FP64 matrix, FP32 vectors, FP64 accumulation
