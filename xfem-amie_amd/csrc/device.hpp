@@ -90,6 +90,12 @@ struct ProbDev {
     int64_t iters_k64;       // iterations run with kmode = 1
     double rr[MAXRHS], rr_ckpt[MAXRHS], rr_target[MAXRHS], bb[MAXRHS], bb64[MAXRHS], rel_best[MAXRHS];
     int64_t best_it[MAXRHS]; // iteration of the last halving of rel_best
+    // The update adds alpha p to y every second iteration, two terms at
+    // once: p_k lives in plane set (k & 1) (k = iters when the SpMV runs),
+    // so p_{k-1} is still there when the next update pairs it with p_k.
+    int32_t ypend;           // the previous update left its alpha p out of y
+    int32_t pad_y;
+    double alpha_prev[MAXRHS]; // that update's alpha
     // results for the host
     double relres_out[MAXRHS];
     int32_t done_out[MAXRHS];
@@ -126,6 +132,7 @@ struct SolveArgs {
     const double *b64;
     const double *S;         // symmetric Jacobi scaling, per dof
     void *y, *r, *p, *q;     // iteration vectors (float or double)
+    void *p2;                // the second plane set of p (p_k in p if k is even, else in p2)
     const void *minv;        // block-Jacobi inverses of the scaled diagonal blocks, per row (b*b, row-major)
     int32_t *fix48_bad;      // set when a scaled value does not fit the fixed-point range
 };
@@ -696,6 +703,8 @@ template <typename W, int B, bool NT = false> struct Node {
 
 template <typename W> __device__ inline W *vw(void *p) { return reinterpret_cast<W *>(p); }
 template <typename W> __device__ inline const W *cvw(const void *p) { return reinterpret_cast<const W *>(p); }
+// the plane set of p_k
+__device__ inline void *p_planes(const SolveArgs &a, int64_t k) { return (k & 1) ? a.p2 : a.p; }
 
 // z = M v with M the row's inverse diagonal block (BJ) or the identity.
 template <typename W, int B, bool BJ> __device__ inline void precond(const W *mi, const W *v, W *z)
@@ -889,7 +898,7 @@ __device__ inline void spmv_tile(const SolveArgs &a, const void *vals, int t, co
     typedef typename std::conditional<MatKind<M>::acc64, double, W>::type A;
     const bool has = wave < td.y;
     const int4 sd = a.slice_desc[has ? td.x + wave : td.x];
-    const W *p = cvw<W>(a.p), *r = cvw<W>(a.r);
+    const W *p = cvw<W>(p_planes(a, a.prob[td.z].iters)), *r = cvw<W>(a.r);
     W *q = vw<W>(a.q);
     double part[3 * NR];
 #pragma unroll
@@ -1038,6 +1047,7 @@ __device__ inline void reduce_body(const SolveArgs &a, int pi)
         if (!(pq == pq) || !(rho == rho) || !(qz == qz) || !(qmq == qmq)) bad = true;
         const double alpha = pq != 0. ? rho / pq : 0.;
         const double rho1 = rho - 2. * alpha * qz + alpha * alpha * qmq;
+        P.alpha_prev[k] = P.alpha[k];
         P.alpha[k] = alpha;
         P.beta[k] = rho != 0. ? rho1 / rho : 0.;
         P.rho_pred[k] = rho1;
@@ -1058,6 +1068,9 @@ __device__ inline void reduce_body(const SolveArgs &a, int pi)
     if (all) { P.status = ST_CONVERGED; return; }
     if (ck && P.iters >= P.maxit) { P.status = ST_MAXIT; return; }
     if (stalled) { P.status = ST_STALLED; return; }
+    // the previous update deferred its y term (no pending term then) or
+    // added both; a refinement added everything
+    P.ypend = ck ? 0 : 1 - P.ypend;
     P.iters += 1;
     if (P.iters >= P.maxit || P.iters - P.last_ckpt >= P.ckpt_interval) need = true;
     if (need) P.ckpt_pending = 1;
@@ -1080,8 +1093,18 @@ __device__ inline void k2_body(const SolveArgs &a, int t, int slot)
     const ProbDev &P = a.prob[a.tile_prob[t]];
     if (P.status != ST_RUNNING) return;
     const bool ck = slot && P.ckpt_pending;
+    // k, the iteration being completed: the reduce kernel has counted it.
+    // p_k is read from its plane set and p_{k+1} written to the other one,
+    // which holds p_{k-1}.  y += alpha p is paired: an update with no
+    // pending term leaves y alone, the next one (or a refinement's slot
+    // update) adds alpha_{k-1} p_{k-1} and then alpha_k p_k, the same FP64
+    // operations in the same order as one add per iteration.
+    const int64_t k_it = P.iters - 1;
+    const bool pend = P.ypend != 0, flush = pend || ck;
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
-    W *p = vw<W>(a.p), *r = vw<W>(a.r);
+    const W *pk = cvw<W>(p_planes(a, k_it));
+    W *pn = vw<W>(p_planes(a, k_it + 1));
+    W *r = vw<W>(a.r);
     double *y = vw<double>(a.y);   // the iterate since the last refinement: FP64 in every mode
     const W *q = cvw<W>(a.q);
     double part[2 * NR];
@@ -1100,12 +1123,21 @@ __device__ inline void k2_body(const SolveArgs &a, int t, int slot)
                 const double al64 = P.alpha[k];
                 const size_t idx = k * a.vplane + (size_t)row * B;
                 W pv[B];
-                double yv[B];
-                Node<W, B, true>::load(p + idx, pv);
-                Node<double, B, true>::load(y + idx, yv);
+                Node<W, B, true>::load(pk + idx, pv);
+                if (flush) {
+                    double yv[B];
+                    Node<double, B, true>::load(y + idx, yv);
+                    if (pend) {
+                        W pp[B];
+                        Node<W, B, true>::load(pn + idx, pp);
+                        const double alp = P.alpha_prev[k];
 #pragma unroll
-                for (int i = 0; i < B; i++) yv[i] += al64 * (double)pv[i];
-                Node<double, B, true>::store(y + idx, yv);
+                        for (int i = 0; i < B; i++) yv[i] += alp * (double)pp[i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < B; i++) yv[i] += al64 * (double)pv[i];
+                    Node<double, B, true>::store(y + idx, yv);
+                }
                 if (ck) continue;
                 W qv[B], rv[B], zv[B];
                 Node<W, B, true>::load(q + idx, qv);
@@ -1120,7 +1152,7 @@ __device__ inline void k2_body(const SolveArgs &a, int t, int slot)
                     part[k] += (double)rv[i] * (double)zv[i];
                     part[NR + k] += (double)rv[i] * (double)rv[i];
                 }
-                Node<W, B, true>::store(p + idx, pv);
+                Node<W, B, true>::store(pn + idx, pv);
             }
         }
     }
@@ -1183,7 +1215,12 @@ __device__ inline void cr_body(const SolveArgs &a, int t)
     const ProbDev &P = a.prob[a.tile_prob[t]];
     if (P.status != ST_RUNNING || !P.ckpt_pending) return;
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
-    W *r = vw<W>(a.r), *p = vw<W>(a.p);
+    // p_{k+1} = M r + beta p_k: from p_k's plane set to the next one (k + 1 =
+    // iters, counted by the reduce kernel; at a pass's opening refinement
+    // iters = 0 and both sets hold zeros)
+    W *r = vw<W>(a.r);
+    const W *pk = cvw<W>(p_planes(a, P.iters - 1));
+    W *pn = vw<W>(p_planes(a, P.iters));
     double part[4 * NR];
 #pragma unroll
     for (int k = 0; k < 4 * NR; k++) part[k] = 0.;
@@ -1223,14 +1260,14 @@ __device__ inline void cr_body(const SolveArgs &a, int t)
                 }
                 Node<W, B>::store(r + idx, rv);
                 precond<W, B, BJ>(mi, rv, zv);
-                Node<W, B>::load(p + idx, pv);
+                Node<W, B>::load(pk + idx, pv);
 #pragma unroll
                 for (int i = 0; i < B; i++) {
                     pv[i] = zv[i] + be * pv[i];
                     part[k] += (double)rv[i] * (double)zv[i];
                     part[NR + k] += (double)rv[i] * (double)rv[i];
                 }
-                Node<W, B>::store(p + idx, pv);
+                Node<W, B>::store(pn + idx, pv);
             }
         }
     }
