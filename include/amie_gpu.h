@@ -396,8 +396,10 @@ int amie_gpu_batch_time_spmv(amie_gpu_batch *b, int32_t precision, int32_t preco
  * with AMIE_MIXED_FIX48 also the value-indexed matrix (format 1: a 16-bit
  * index per block into a table of the distinct FIX48 blocks, built on the
  * device when there are at most 65536 of them, then rebuilt at every
- * assembly).  Results do not depend on the choice (the same values, the
- * sums in the same order).  Precisions with candidates: AMIE_FP64,
+ * assembly).  Each candidate is timed twice (iters launches each time, in
+ * two rounds over the list) and keeps the faster time.  Results do not
+ * depend on the choice (the same values, the sums in the same order).
+ * Precisions with candidates: AMIE_FP64,
  * AMIE_MIXED and AMIE_MIXED_FIX48.  Outputs (each nullable): the chosen
  * blocks per step, tile order and format (0 value planes, 1 value-indexed)
  * and its time per launch. */
