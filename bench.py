@@ -14,8 +14,8 @@ roofline = the kernel with the larger share of the timed steps, the CG
 update (k_update) or the SpMV (k_spmv), both timed with HIP events on the
 library stream during the timed steps and both reported
 (roofline_update, roofline_spmv).  k_update: the algorithmic bytes of every
-launch (30 B per DOF and right-hand side on average: y += alpha p is added
-every second iteration, two terms at once) over the kernel time.  k_spmv: the
+launch (27 B per DOF and right-hand side on average: y += alpha p is added
+every fourth iteration, four terms at once) over the kernel time.  k_spmv: the
 algorithmic bytes of the FP32 block SpMV (nnzb (b^2 4 + 4) + 4 (nb + 1) +
 2 n 4 per right-hand side) of every launch over the kernel time, the
 physical bytes of the stored format beside it.
@@ -870,9 +870,10 @@ def main():
               "frac": upd_gbs / HBM_PEAK_GBS,
               "traffic": tr_u["ratio"] * avg_upd if tr_u else None,
               "bytes": "algorithmic bytes per launch: per DOF and right-hand side, p, q, r read and r, p written "
-                       "(FP32), and every second launch p_{k-1} read and y read and written (FP64, y += alpha p "
-                       "paired): 20 and 40 B, 30 B on average; a refinement's slot update 20 or 24 B (p and "
-                       "p_{k-1} read, y read and written), 22 B on average; over the problems each launch works on",
+                       "(FP32), and every fourth launch p_{k-3} .. p_{k-1} read and y read and written (FP64, "
+                       "y += alpha p grouped): 20 and 48 B, 27 B on average; a refinement's slot update, which "
+                       "closes a group, 32 B (p_{k-3} .. p_k read, y read and written); over the problems each "
+                       "launch works on",
               "traffic_source": (f"FETCH_SIZE+WRITE_SIZE / algorithmic bytes = {tr_u['ratio']} at full occupancy "
                                  f"({os.path.relpath(TRAFFIC_PROFILE, ROOT)}), times the average algorithmic bytes "
                                  f"per launch" if tr_u else None),

@@ -86,7 +86,7 @@ size_t Batch::bytes() const
                            &egeo, &ngrid, &ngdim, &cell_start, &cell_elem, &col, &dslot, &row_len, &v64, &vs64,
                            &v32, &vsp, &vfx, &fix48_bad, &tile_slice0,
                            &tile_nslices, &tile_prob, &active, &n_active, &active_prob, &n_active_prob, &prob, &part1,
-                           &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &p2, &q, &minv32, &minv64, &bcmask,
+                           &part2, &partc, &partn, &x64, &b64, &fext, &S, &y, &r, &p, &p2, &p3, &p4, &q, &minv32, &minv64, &bcmask,
                            &bcval, &coords,
                            &econn, &erow, &eslot, &ephase, &edamage, &ekappa, &ed0, &eprob, &cmat, &cmat_off,
                            &macro, &colour_elems, &prob_e0, &prob_ne, &grown, &slot_cptr, &slot_contrib,
@@ -149,7 +149,11 @@ SolveArgs Batch::args(int mode) const
     a.y = y.p;
     a.r = r.p;
     a.p = p.p;
-    a.p2 = p2.p;
+    static_assert(YGROUP == 4, "one plane set of p per iteration of a y group");
+    a.pset[0] = p.p;
+    a.pset[1] = p2.p;
+    a.pset[2] = p3.p;
+    a.pset[3] = p4.p;
     a.q = q.p;
     a.minv = w32 ? minv32.p : minv64.p;
     return a;
@@ -235,7 +239,7 @@ static int alloc_and_upload(DevBuf &buf, const void *src, size_t bytes)
 static int alloc_vectors(Batch &B)
 {
     const size_t nv = (size_t)B.nrows_pad * B.et.b * B.nrhs;
-    DevBuf *vecs[] = {&B.x64, &B.b64, &B.fext, &B.y, &B.r, &B.p, &B.p2, &B.q};
+    DevBuf *vecs[] = {&B.x64, &B.b64, &B.fext, &B.y, &B.r, &B.p, &B.p2, &B.p3, &B.p4, &B.q};
     for (auto v : vecs) {
         if (!v->alloc(nv * 8)) return AMIE_ERR_NOMEM;
         AMIE_HIP(hipMemset(v->p, 0, v->n));
@@ -1040,7 +1044,8 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
         P.last_ckpt = 0;
         P.ypend = 0;
         for (int k = 0; k < MAXRHS; k++) {
-            P.alpha[k] = P.beta[k] = P.alpha_prev[k] = 0.;
+            P.alpha[k] = P.beta[k] = 0.;
+            for (int j = 0; j < YGROUP; j++) P.alpha_ring[k][j] = 0.;
             P.rho_pred[k] = 0.;
             P.rr[k] = 0.;
             P.rr_ckpt[k] = DBL_MAX;
@@ -1056,6 +1061,8 @@ int Batch::start_pass(int mode, const std::vector<char> &subset)
     AMIE_HIP(hipMemsetAsync(y.p, 0, y.n, s));
     AMIE_HIP(hipMemsetAsync(p.p, 0, p.n, s));
     AMIE_HIP(hipMemsetAsync(p2.p, 0, p2.n, s));
+    AMIE_HIP(hipMemsetAsync(p3.p, 0, p3.n, s));
+    AMIE_HIP(hipMemsetAsync(p4.p, 0, p4.n, s));
     int rc = set_active(subset);
     if (rc) return rc;
     R.gt = (int)h_active.size();
@@ -1202,18 +1209,18 @@ int Batch::solve_poll(bool wait, bool *finished)
                     const double nk = std::min(n, (double)P.iters_k64);
                     R.stats.spmv_bytes += R.pbytes[p] * (n - nk) + R.pbytes_k64[p] * nk;
                     // the update: every update reads p, q, r (and the
-                    // block-Jacobi inverses) and writes r, p; every second one
-                    // also reads p_{k-1} and reads and writes y (y += alpha p
-                    // paired), so 2.5 w + 8 more on average; a refinement's
-                    // slot update reads p (and p_{k-1} every second time) and
-                    // reads and writes y.  y is FP64, the others in the
-                    // working precision w; every right-hand side counted
-                    // until the problem finishes
+                    // block-Jacobi inverses) and writes r, p; every fourth one
+                    // also reads p_{k-3} .. p_{k-1} and reads and writes y
+                    // (y += alpha p grouped), so (3 w + 16) / 4 more on
+                    // average; a refinement's slot update closes a group:
+                    // it reads p_{k-3} .. p_k and reads and writes y.  y is
+                    // FP64, the others in the working precision w; every
+                    // right-hand side counted until the problem finishes
                     const double w = (R.mode == AMIE_FP64 || R.mode == AMIE_MIXED64) ? 8. : 4.;
                     const double dofs = (double)ph[p].nrows * et.b, nr = (double)nrhs;
                     const double ck = (double)std::max<int64_t>(0, P.nckpt - 1);
-                    const double full = dofs * nr * (5.5 * w + 8.) + (R.bj ? dofs * et.b * w : 0.);
-                    const double slot = dofs * nr * (1.5 * w + 16.);
+                    const double full = dofs * nr * (5.75 * w + 4.) + (R.bj ? dofs * et.b * w : 0.);
+                    const double slot = dofs * nr * (4. * w + 16.);
                     R.stats.update_bytes += full * std::max(0., (double)P.iters - ck) + slot * std::min(ck, (double)P.iters);
                 }
         // problems whose FP32 iteration stagnated continue from their current

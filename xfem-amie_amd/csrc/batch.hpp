@@ -146,7 +146,7 @@ struct Batch {
     // vectors: one plane of nrows_pad * b entries per right-hand side; 8 bytes
     // per entry (the iteration vectors y, r, p, q hold floats in mixed mode)
     DevBuf x64, b64, fext, S, y, r, p, q, minv32, minv64;
-    DevBuf p2;                   // p's second plane set (p_k in p for even k, in p2 for odd k)
+    DevBuf p2, p3, p4;           // p's other plane sets (p_k in set k % YGROUP; set 0 is p)
     DevBuf bcmask, bcval;
     // elements
     DevBuf coords, econn, erow, eslot, ephase, edamage, ekappa, ed0, eprob, cmat, cmat_off, macro, colour_elems,

@@ -58,6 +58,7 @@ constexpr int WAVE = 64;
 constexpr int TILE_SLICES = 4;
 constexpr int WG = WAVE * TILE_SLICES;
 constexpr int MAXRHS = 4;
+constexpr int YGROUP = 4;   // CG iterations per y += alpha p group (a power of 2, at most 4)
 constexpr int MAXQ = 3 * MAXRHS;
 constexpr int NXCD = 8;
 
@@ -90,12 +91,13 @@ struct ProbDev {
     int64_t iters_k64;       // iterations run with kmode = 1
     double rr[MAXRHS], rr_ckpt[MAXRHS], rr_target[MAXRHS], bb[MAXRHS], bb64[MAXRHS], rel_best[MAXRHS];
     int64_t best_it[MAXRHS]; // iteration of the last halving of rel_best
-    // The update adds alpha p to y every second iteration, two terms at
-    // once: p_k lives in plane set (k & 1) (k = iters when the SpMV runs),
-    // so p_{k-1} is still there when the next update pairs it with p_k.
-    int32_t ypend;           // the previous update left its alpha p out of y
+    // The update adds alpha p to y every YGROUP-th iteration, YGROUP terms
+    // at once: p_k lives in plane set k % YGROUP (k = iters when the SpMV
+    // runs), so p_{k-3} .. p_{k-1} are still there when the update of p_k
+    // adds them.
+    int32_t ypend;           // terms the previous updates left out of y (0 .. YGROUP - 1)
     int32_t pad_y;
-    double alpha_prev[MAXRHS]; // that update's alpha
+    double alpha_ring[MAXRHS][YGROUP]; // alpha_k at [k % YGROUP]
     // results for the host
     double relres_out[MAXRHS];
     int32_t done_out[MAXRHS];
@@ -132,7 +134,7 @@ struct SolveArgs {
     const double *b64;
     const double *S;         // symmetric Jacobi scaling, per dof
     void *y, *r, *p, *q;     // iteration vectors (float or double)
-    void *p2;                // the second plane set of p (p_k in p if k is even, else in p2)
+    void *pset[YGROUP];      // p's plane sets: p_k in pset[k % YGROUP] (pset[0] = p)
     const void *minv;        // block-Jacobi inverses of the scaled diagonal blocks, per row (b*b, row-major)
     int32_t *fix48_bad;      // set when a scaled value does not fit the fixed-point range
 };
@@ -704,7 +706,7 @@ template <typename W, int B, bool NT = false> struct Node {
 template <typename W> __device__ inline W *vw(void *p) { return reinterpret_cast<W *>(p); }
 template <typename W> __device__ inline const W *cvw(const void *p) { return reinterpret_cast<const W *>(p); }
 // the plane set of p_k
-__device__ inline void *p_planes(const SolveArgs &a, int64_t k) { return (k & 1) ? a.p2 : a.p; }
+__device__ inline void *p_planes(const SolveArgs &a, int64_t k) { return a.pset[k & (YGROUP - 1)]; }
 
 // z = M v with M the row's inverse diagonal block (BJ) or the identity.
 template <typename W, int B, bool BJ> __device__ inline void precond(const W *mi, const W *v, W *z)
@@ -1047,8 +1049,8 @@ __device__ inline void reduce_body(const SolveArgs &a, int pi)
         if (!(pq == pq) || !(rho == rho) || !(qz == qz) || !(qmq == qmq)) bad = true;
         const double alpha = pq != 0. ? rho / pq : 0.;
         const double rho1 = rho - 2. * alpha * qz + alpha * alpha * qmq;
-        P.alpha_prev[k] = P.alpha[k];
         P.alpha[k] = alpha;
+        P.alpha_ring[k][P.iters & (YGROUP - 1)] = alpha;
         P.beta[k] = rho != 0. ? rho1 / rho : 0.;
         P.rho_pred[k] = rho1;
         P.rr[k] = rr;
@@ -1068,9 +1070,9 @@ __device__ inline void reduce_body(const SolveArgs &a, int pi)
     if (all) { P.status = ST_CONVERGED; return; }
     if (ck && P.iters >= P.maxit) { P.status = ST_MAXIT; return; }
     if (stalled) { P.status = ST_STALLED; return; }
-    // the previous update deferred its y term (no pending term then) or
-    // added both; a refinement added everything
-    P.ypend = ck ? 0 : 1 - P.ypend;
+    // the previous update deferred its y term (one more pending) or added
+    // every pending one; a refinement added everything
+    P.ypend = (ck || P.ypend == YGROUP - 1) ? 0 : P.ypend + 1;
     P.iters += 1;
     if (P.iters >= P.maxit || P.iters - P.last_ckpt >= P.ckpt_interval) need = true;
     if (need) P.ckpt_pending = 1;
@@ -1094,13 +1096,15 @@ __device__ inline void k2_body(const SolveArgs &a, int t, int slot)
     if (P.status != ST_RUNNING) return;
     const bool ck = slot && P.ckpt_pending;
     // k, the iteration being completed: the reduce kernel has counted it.
-    // p_k is read from its plane set and p_{k+1} written to the other one,
-    // which holds p_{k-1}.  y += alpha p is paired: an update with no
-    // pending term leaves y alone, the next one (or a refinement's slot
-    // update) adds alpha_{k-1} p_{k-1} and then alpha_k p_k, the same FP64
-    // operations in the same order as one add per iteration.
+    // p_k is read from its plane set and p_{k+1} written to the set of
+    // p_{k+1-YGROUP}.  y += alpha p is grouped: an update with fewer than
+    // YGROUP - 1 pending terms leaves y alone; the next one (or a
+    // refinement's slot update) adds alpha_j p_j for the pending j in order
+    // and then alpha_k p_k: the same FP64 operations in the same order as one
+    // add per iteration.
     const int64_t k_it = P.iters - 1;
-    const bool pend = P.ypend != 0, flush = pend || ck;
+    const int pend = P.ypend;
+    const bool flush = pend == YGROUP - 1 || ck;
     const int wave = wave_id(), lane = threadIdx.x % WAVE;
     const W *pk = cvw<W>(p_planes(a, k_it));
     W *pn = vw<W>(p_planes(a, k_it + 1));
@@ -1127,10 +1131,10 @@ __device__ inline void k2_body(const SolveArgs &a, int t, int slot)
                 if (flush) {
                     double yv[B];
                     Node<double, B, true>::load(y + idx, yv);
-                    if (pend) {
+                    for (int j = pend; j >= 1; j--) {
                         W pp[B];
-                        Node<W, B, true>::load(pn + idx, pp);
-                        const double alp = P.alpha_prev[k];
+                        Node<W, B, true>::load(cvw<W>(p_planes(a, k_it - j)) + idx, pp);
+                        const double alp = P.alpha_ring[k][(k_it - j) & (YGROUP - 1)];
 #pragma unroll
                         for (int i = 0; i < B; i++) yv[i] += alp * (double)pp[i];
                     }
